@@ -1,0 +1,200 @@
+"""A uniform driver over a pfn* primitive library, used by the query plans.
+
+``Backend`` is written against the reference ABI only: it builds value nodes, calls primitives
+by name and reads results back as numpy arrays.  ``GpuBackend`` drives libhorsepower_amd (HIP,
+payloads resident in HBM).  Any other library exporting the same ABI can be driven by
+subclassing ``Backend`` and supplying node allocation and payload reads.
+"""
+import ctypes as C
+
+import numpy as np
+
+from .runtime import Host, cell, load
+from .vnode import H_B, H_E, H_G, H_I, H_L, H_N, NP_OF, V, V0, Expr
+
+FUNC_ADDR_NAMES = ("pfnSum", "pfnAvg", "pfnMin", "pfnMax", "pfnLen", "pfnIndex")
+
+
+class Backend:
+    name = "abstract"
+
+    def __init__(self, lib):
+        self.lib = lib
+        self.host = Host(lib, self.node)
+        self.fa = {n: lib.addr(n) for n in FUNC_ADDR_NAMES}
+
+    # -- to be provided -----------------------------------------------------------------------
+    def node(self):
+        raise NotImplementedError
+
+    def read_raw(self, z, nbytes, dst):
+        raise NotImplementedError
+
+    def symbol(self, name):
+        raise NotImplementedError
+
+    def resident(self, z):
+        return z
+
+    # -- construction ----------------------------------------------------------------------------
+    def vec(self, arr, typ=None):
+        return self.resident(self.host.vec(arr, typ))
+
+    def scalar(self, v, typ):
+        return self.host.scalar(v, typ)
+
+    def sym(self, name):
+        return self.host.sym(self.symbol(name))
+
+    def list(self, items):
+        return self.host.list(items)
+
+    def bools(self, flags):
+        return self.host.vec(np.array(flags, dtype=np.uint8), H_B)
+
+    def call(self, name, *args):
+        return self.lib.call(name, *args)
+
+    def new(self, name, *args):
+        z = self.node()
+        self.lib.call(name, z, *args)
+        return z
+
+    # -- primitives by name ----------------------------------------------------------------------
+    def __getattr__(self, name):
+        if name.startswith("pfn"):
+            return lambda *a: self.new(name, *a)
+        raise AttributeError(name)
+
+    def each(self, x, fname):
+        z = self.node()
+        self.lib.call("pfnEach", z, x, self.fa[fname])
+        return z
+
+    def each_right(self, x, y, fname):
+        z = self.node()
+        self.lib.call("pfnEachRight", z, x, y, self.fa[fname])
+        return z
+
+    # -- reading -------------------------------------------------------------------------------
+    def read(self, z):
+        """Vector/scalar node -> numpy array."""
+        v = z.contents
+        if v.typ == H_G:
+            raise TypeError("list node: use read_list / read_groups")
+        dt = np.dtype(NP_OF[v.typ])
+        n = v.len
+        out = np.empty(n, dtype=dt)
+        if n == 0:
+            return out
+        if n == 1 and v.g == C.addressof(v) + 24:
+            C.memmove(out.ctypes.data, v.g, dt.itemsize)
+            return out
+        self.read_raw(z, n * dt.itemsize, out.ctypes.data)
+        return out
+
+    def read_scalar(self, z):
+        return self.read(z)[0]
+
+    def read_list(self, z):
+        v = z.contents
+        if v.typ not in (H_G, H_N):
+            raise TypeError("not a list")
+        if v.typ == H_G and v.u.p:                          # flat list
+            return self.read_flat(z)
+        n = v.len if v.typ == H_G else 2
+        return [self.read_any(cell(z, k)) for k in range(n)]
+
+    def read_any(self, z):
+        return self.read_list(z) if z.contents.typ in (H_G, H_N) else self.read(z)
+
+    def read_flat(self, z):
+        """Flat list -> list of arrays split by the group sizes."""
+        v = z.contents
+        counts = np.empty(v.len, dtype=np.int64)
+        if v.len:
+            self.read_raw(z, 8 * v.len, counts.ctypes.data)
+        inner = C.cast(C.c_void_p(v.u.p), V)
+        flat = self.read(inner)
+        return np.split(flat, np.cumsum(counts)[:-1]) if v.len else []
+
+    def read_groups(self, z):
+        """Group dict -> (keys, counts, flat) with flat concatenating the group index lists."""
+        keys = self.read(cell(z, 0))
+        vals = cell(z, 1)
+        v = vals.contents
+        if v.u.p:
+            counts = np.empty(v.len, dtype=np.int64)
+            if v.len:
+                self.read_raw(vals, 8 * v.len, counts.ctypes.data)
+            flat = self.read(C.cast(C.c_void_p(v.u.p), V))
+        else:
+            parts = [self.read(cell(vals, k)) for k in range(v.len)]
+            counts = np.array([len(p) for p in parts], dtype=np.int64)
+            flat = np.concatenate(parts) if parts else np.zeros(0, np.int64)
+        return keys, counts, flat
+
+
+class GpuBackend(Backend):
+    """libhorsepower_amd on the current HIP device."""
+    name = "gpu"
+
+    def __init__(self, lib=None):
+        super().__init__(lib or load())
+        self.dev = self.lib
+
+    def node(self):
+        return self.lib.node()
+
+    def resident(self, z):
+        self.lib.rc("hpaUpload", z)
+        return z
+
+    def read_raw(self, z, nbytes, dst):
+        self.lib.rc("hpaReadPayload", z, C.c_void_p(dst), nbytes)
+
+    def symbol(self, name):
+        return self.lib.symbol(name)
+
+    def mark(self):
+        return self.lib.mark()
+
+    def release(self, m):
+        self.lib.release(m)
+
+    def sync(self):
+        self.lib.sync()
+
+    # fused entry points (GPU only)
+    def fused_compress(self, ys, xs, cond):
+        n = len(ys)
+        zs = [self.node() for _ in range(n)]
+        za = (V * n)(*zs)
+        ya = (V * n)(*ys)
+        xa = (V * len(xs))(*xs)
+        self.lib.rc("hpaFusedCompress", n, za, ya, len(xs), xa, C.byref(cond))
+        return zs
+
+    def compress_many(self, mask, ys):
+        n = len(ys)
+        zs = [self.node() for _ in range(n)]
+        self.lib.rc("hpaCompressMany", n, (V * n)(*zs), mask, (V * n)(*ys))
+        return zs
+
+    def fused_reduce(self, kind, xs, cond, body):
+        z = self.node()
+        xa = (V * len(xs))(*xs)
+        self.lib.rc("hpaFusedReduce", z, kind, len(xs), xa,
+                    C.byref(cond) if cond is not None else None, C.byref(body))
+        return z
+
+    def fused_map(self, typ, xs, e):
+        z = self.node()
+        xa = (V * len(xs))(*xs)
+        self.lib.rc("hpaFusedMap", z, typ, len(xs), xa, C.byref(e))
+        return z
+
+    def group_reduce(self, kind, x, groups):
+        z = self.node()
+        self.lib.rc("hpaGroupReduce", z, kind, x, groups)
+        return z

@@ -1,0 +1,425 @@
+// Hash primitives: join_index, index_of, member.
+//
+// join_index (h_primitive.cc:1944-2155, backend/h_hash_v3.cc:230-277): the shorter side is the
+// build side (ties: x), the longer side probes.  Pairs come out probe-row ascending and, within
+// one probe row, build rows ascending -- the reference's insertion-order duplicate arrays.  The
+// build side is radix-sorted into runs of equal keys (rows ascending inside a run); an
+// open-addressing table maps each distinct key to its run.  Probe pass 1 stores each probe row's
+// run and match count, an exclusive scan places the rows, pass 2 writes the pairs.
+// index_of (h_primitive.cc:1525-1579, h_libs.cc:299-373): first occurrence via atomicMin on the
+// slot of each key; an ordered (monotone) i32/i64 x is searched with the reference's binary
+// search (backend/h_system.cc:715-745), which returns whichever equal element it meets first.
+// member (h_primitive.cc:2288-2336, h_libs.cc:897-1018): set membership.
+// Float keys compare with == (so -0 matches +0 and NaN matches nothing).
+#include "hpa_internal.h"
+
+#define EMPTY_KEY 0xffffffffffffffffull
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z){
+    z ^= z >> 33; z *= 0xff51afd7ed558ccdull; z ^= z >> 33; z *= 0xc4ceb9fe1a85ec53ull; z ^= z >> 33; return z;
+}
+
+// key bits of element i of a typed column (after promotion)
+template<typename T> __device__ __forceinline__ uint64_t kbits(T v){ return (uint64_t)(int64_t)v; }
+template<> __device__ __forceinline__ uint64_t kbits<float>(float v){ if(v == 0.0f) v = 0.0f; return (uint64_t)__float_as_uint(v); }
+template<> __device__ __forceinline__ uint64_t kbits<double>(double v){ if(v == 0.0) v = 0.0; return (uint64_t)__double_as_longlong(v); }
+template<typename T> __device__ __forceinline__ bool knan(T v){ return false; }
+template<> __device__ __forceinline__ bool knan<float>(float v){ return v != v; }
+template<> __device__ __forceinline__ bool knan<double>(double v){ return v != v; }
+
+struct HTab { uint64_t *keys; unsigned long long *vals; uint64_t mask; unsigned long long *ones_val; };
+
+__device__ __forceinline__ unsigned long long *ht_slot(const HTab &t, uint64_t k, bool insert){
+    if(k == EMPTY_KEY) return t.ones_val;
+    uint64_t h = mix64(k) & t.mask;
+    while(true){
+        uint64_t cur = t.keys[h];
+        if(cur == k) return &t.vals[h];
+        if(cur == EMPTY_KEY){
+            if(!insert) return nullptr;
+            unsigned long long prev = atomicCAS((unsigned long long*)&t.keys[h], (unsigned long long)EMPTY_KEY, (unsigned long long)k);
+            if(prev == EMPTY_KEY || prev == k) return &t.vals[h];
+            continue;    // lost the race to another key: re-examine this slot
+        }
+        h = (h + 1) & t.mask;
+    }
+}
+__device__ __forceinline__ const unsigned long long *ht_find(const HTab &t, uint64_t k){
+    if(k == EMPTY_KEY) return (*t.ones_val != ~0ull) ? t.ones_val : nullptr;
+    uint64_t h = mix64(k) & t.mask;
+    while(true){
+        uint64_t cur = t.keys[h];
+        if(cur == k) return &t.vals[h];
+        if(cur == EMPTY_KEY) return nullptr;
+        h = (h + 1) & t.mask;
+    }
+}
+
+__global__ void k_ht_clear(uint64_t *keys, unsigned long long *vals, L cap, unsigned long long *ones){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += (L)gridDim.x * blockDim.x){ keys[i] = EMPTY_KEY; vals[i] = ~0ull; }
+    if(blockIdx.x == 0 && threadIdx.x == 0) *ones = ~0ull;
+}
+static int32_t ht_create(hpa_ctx *c, L n, HTab *t){
+    L cap = 16; while(cap < 2 * n) cap <<= 1;
+    t->keys = (uint64_t*)dalloc(c, 8 * cap); t->vals = (unsigned long long*)dalloc(c, 8 * cap);
+    t->ones_val = (unsigned long long*)dalloc(c, 16);
+    if(!t->keys || !t->vals || !t->ones_val) R HPA_E_MEMORY;
+    t->mask = (uint64_t)cap - 1;
+    hipLaunchKernelGGL(k_ht_clear, dim3(grid_for(cap, 256, 4)), dim3(256), 0, c->stream, t->keys, t->vals, cap, t->ones_val);
+    R check_launch("hash_clear");
+}
+
+// ---------------------------------------------------------------- index_of / member build+probe
+template<typename T>
+__global__ void k_ht_insert_min(const T *x, L n, HTab t){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
+        T v = x[i]; if(knan(v)) continue;
+        unsigned long long *s = ht_slot(t, kbits(v), true);
+        atomicMin(s, (unsigned long long)i);
+    }
+}
+template<typename T>
+__global__ void k_ht_probe_index(const T *y, L n, HTab t, int64_t *z, int64_t miss){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
+        T v = y[i];
+        const unsigned long long *s = knan(v) ? nullptr : ht_find(t, kbits(v));
+        z[i] = (s && *s != ~0ull) ? (int64_t)*s : miss;
+    }
+}
+template<typename T>
+__global__ void k_ht_probe_member(const T *y, L n, HTab t, uint8_t *z){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
+        T v = y[i];
+        const unsigned long long *s = knan(v) ? nullptr : ht_find(t, kbits(v));
+        z[i] = (s && *s != ~0ull) ? 1 : 0;
+    }
+}
+// small sets (<= 2 items): the reference compares with == directly (h_libs.cc:899-944)
+template<typename T>
+__global__ void k_member_small(const T *y, L n, const T *x, int nx, uint8_t *z){
+    T a = nx > 0 ? x[0] : T(), b = nx > 1 ? x[1] : T();
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
+        T v = y[i]; z[i] = (nx > 0 && v == a) || (nx > 1 && v == b);
+    }
+}
+// bool / char lookup tables (lib_index_of_B/C, lib_member_B/C)
+__global__ void k_table_probe_index(const uint8_t *y, L n, const int64_t *tab, int64_t *z, int64_t miss, int nt){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){ unsigned v = y[i]; if(nt == 2) v = v != 0; int64_t t = tab[v]; z[i] = t < 0 ? miss : t; }
+}
+__global__ void k_table_build_first(const uint8_t *x, L n, unsigned long long *tab, int nt){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){ unsigned v = x[i]; if(nt == 2) v = v != 0; atomicMin(&tab[v], (unsigned long long)i); }
+}
+__global__ void k_table_fix(unsigned long long *tab, int64_t *out, int nt, int bool_bug){
+    for(int i = threadIdx.x; i < nt; i += blockDim.x){
+        int64_t v = tab[i] == ~0ull ? -1 : (int64_t)tab[i];
+        // lib_index_of_B initialises flag[2]={-1} so flag[1] starts at 0 and is never set
+        if(bool_bug && i == 1) v = 0;
+        out[i] = v;
+    }
+}
+
+// binarySearch_I/L (backend/h_system.cc:715-729) restated iteratively
+template<typename T>
+__global__ void k_search_ordered(const T *x, L nx, const T *y, L n, int64_t *z){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
+        T val = y[i]; L low = 0, high = nx, r = -1;
+        if(nx > 0){
+            while(true){
+                L mid = (low + high) >> 1; T xm = x[mid];
+                if(xm == val){ r = mid; break; }
+                else if(low == mid){ r = -1; break; }
+                else if(xm > val) high = mid;
+                else low = mid;
+            }
+        }
+        z[i] = r < 0 ? nx : r;
+    }
+}
+
+int32_t is_ordered_dev(hpa_ctx *c, V x, bool *ord);
+
+template<typename T>
+static int32_t index_of_hash(hpa_ctx *c, const T *px, L nx, const T *py, L ny, int64_t *z){
+    HTab t; int32_t e;
+    if((e = ht_create(c, nx, &t))) R e;
+    ProfScope ps("index_of", c->stream);
+    if(nx > 0) hipLaunchKernelGGL(k_ht_insert_min<T>, dim3(grid_for(nx, 256, 4)), dim3(256), 0, c->stream, px, nx, t);
+    if(ny > 0) hipLaunchKernelGGL(k_ht_probe_index<T>, dim3(grid_for(ny, 256, 4)), dim3(256), 0, c->stream, py, ny, t, z, (int64_t)nx);
+    R check_launch("index_of");
+}
+
+static int32_t index_of_table(hpa_ctx *c, const uint8_t *px, L nx, const uint8_t *py, L ny, int64_t *z, int nt, bool boolbug){
+    unsigned long long *tab = (unsigned long long*)dalloc(c, 8 * nt); int64_t *tt = (int64_t*)dalloc(c, 8 * nt);
+    if(!tab || !tt) R HPA_E_MEMORY;
+    hipMemsetAsync(tab, 0xff, 8 * nt, c->stream);
+    if(nx > 0) hipLaunchKernelGGL(k_table_build_first, dim3(grid_for(nx, 256, 4)), dim3(256), 0, c->stream, px, nx, tab, nt);
+    hipLaunchKernelGGL(k_table_fix, dim3(1), dim3(256), 0, c->stream, tab, tt, nt, boolbug ? 1 : 0);
+    if(ny > 0) hipLaunchKernelGGL(k_table_probe_index, dim3(grid_for(ny, 256, 4)), dim3(256), 0, c->stream, py, ny, tt, z, (int64_t)nx, nt);
+    R check_launch("index_of");
+}
+
+extern "C" int32_t pfnIndexOf(hpa_V z, hpa_V x, hpa_V y){
+    hpa_ctx *c = ctx_cur();
+    int32_t e;
+    L nx = x->len, ny = y->len;
+    if(is_real(x->typ) && is_real(y->typ)){
+        int32_t t = x->typ > y->typ ? x->typ : y->typ;
+        bool ord = false;
+        if((e = is_ordered_dev(c, x, &ord))) R e;
+        const void *px = nullptr, *py = nullptr;
+        if(ord){
+            // searchOrdered: y compared in x's integer type, widened when y is wider
+            int32_t st = (y->typ == HPA_L || x->typ == HPA_L) ? HPA_L : HPA_I;
+            if(!is_int(y->typ) || y->typ == HPA_B) st = HPA_L;
+            if(!is_int(y->typ)) R HPA_E_NOT_IMPL;
+            if(nx > 0 && (e = promote_dev(c, x, st, &px))) R e;
+            if(ny > 0 && (e = promote_dev(c, y, st, &py))) R e;
+            void *dz; L mark = c->dcur;
+            if((e = out_init(c, z, HPA_L, ny, &dz))) R e;
+            if(ny > 0){
+                ProfScope ps("index_of", c->stream);
+                if(st == HPA_I) hipLaunchKernelGGL(k_search_ordered<int32_t>, dim3(grid_for(ny, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)px, nx, (const int32_t*)py, ny, (int64_t*)dz);
+                else hipLaunchKernelGGL(k_search_ordered<int64_t>, dim3(grid_for(ny, 256, 4)), dim3(256), 0, c->stream, (const int64_t*)px, nx, (const int64_t*)py, ny, (int64_t*)dz);
+                if((e = check_launch("index_of"))) R e;
+            }
+            if(is_inline(z)){ e = commit_inline(c, z, dz); c->dcur = mark; }
+            R e;
+        }
+        if(nx > 0 && (e = promote_dev(c, x, t, &px))) R e;
+        if(ny > 0 && (e = promote_dev(c, y, t, &py))) R e;
+        void *dz; L mark = c->dcur;
+        if((e = out_init(c, z, HPA_L, ny, &dz))) R e;
+        int64_t *zz = (int64_t*)dz;
+        switch(t){
+            case HPA_B: e = index_of_table(c, (const uint8_t*)px, nx, (const uint8_t*)py, ny, zz, 2, true); break;
+            case HPA_J: e = index_of_hash<int8_t>(c, (const int8_t*)px, nx, (const int8_t*)py, ny, zz); break;
+            case HPA_H: e = index_of_hash<int16_t>(c, (const int16_t*)px, nx, (const int16_t*)py, ny, zz); break;
+            case HPA_I: e = index_of_hash<int32_t>(c, (const int32_t*)px, nx, (const int32_t*)py, ny, zz); break;
+            case HPA_L: e = index_of_hash<int64_t>(c, (const int64_t*)px, nx, (const int64_t*)py, ny, zz); break;
+            case HPA_F: e = index_of_hash<float>(c, (const float*)px, nx, (const float*)py, ny, zz); break;
+            case HPA_E: e = index_of_hash<double>(c, (const double*)px, nx, (const double*)py, ny, zz); break;
+        }
+        if(e) R e;
+        if(is_inline(z)){ e = commit_inline(c, z, dz); c->dcur = mark; }
+        R e;
+    }
+    if(x->typ == y->typ && is_basic(x->typ)){
+        const void *px = nullptr, *py = nullptr;
+        if(nx > 0 && (e = dev_view(c, x, &px))) R e;
+        if(ny > 0 && (e = dev_view(c, y, &py))) R e;
+        void *dz; L mark = c->dcur;
+        if((e = out_init(c, z, HPA_L, ny, &dz))) R e;
+        int64_t *zz = (int64_t*)dz;
+        switch(x->typ){
+            case HPA_C: e = index_of_table(c, (const uint8_t*)px, nx, (const uint8_t*)py, ny, zz, 256, false); break;
+            case HPA_M: case HPA_D: case HPA_U: case HPA_W: case HPA_T:
+                e = index_of_hash<int32_t>(c, (const int32_t*)px, nx, (const int32_t*)py, ny, zz); break;
+            case HPA_Z: case HPA_Q:
+                e = index_of_hash<int64_t>(c, (const int64_t*)px, nx, (const int64_t*)py, ny, zz); break;
+            default: e = HPA_E_NOT_IMPL;
+        }
+        if(e) R e;
+        if(is_inline(z)){ e = commit_inline(c, z, dz); c->dcur = mark; }
+        R e;
+    }
+    R HPA_E_TYPE;
+}
+
+template<typename T>
+static int32_t member_t(hpa_ctx *c, const T *py, L ny, const T *px, L nx, uint8_t *z){
+    ProfScope ps("member", c->stream);
+    if(nx <= 2){
+        if(ny > 0) hipLaunchKernelGGL(k_member_small<T>, dim3(grid_for(ny, 256, 4)), dim3(256), 0, c->stream, py, ny, px, (int)nx, z);
+        R check_launch("member");
+    }
+    HTab t; int32_t e;
+    if((e = ht_create(c, nx, &t))) R e;
+    hipLaunchKernelGGL(k_ht_insert_min<T>, dim3(grid_for(nx, 256, 4)), dim3(256), 0, c->stream, px, nx, t);
+    if(ny > 0) hipLaunchKernelGGL(k_ht_probe_member<T>, dim3(grid_for(ny, 256, 4)), dim3(256), 0, c->stream, py, ny, t, z);
+    R check_launch("member");
+}
+__global__ void k_member_table(const uint8_t *y, L n, const uint8_t *x, L nx, uint8_t *z, int nt){
+    __shared__ uint8_t tab[256];
+    for(int i = threadIdx.x; i < 256; i += blockDim.x) tab[i] = 0;
+    __syncthreads();
+    for(L i = threadIdx.x; i < nx; i += blockDim.x){ unsigned v = x[i]; if(nt == 2) v = v != 0; tab[v] = 1; }
+    __syncthreads();
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){ unsigned v = y[i]; if(nt == 2) v = v != 0; z[i] = tab[v]; }
+}
+
+// pfnMember(z, y, x): z[i] = y[i] in x, length of y (h_primitive.cc:2288)
+extern "C" int32_t pfnMember(hpa_V z, hpa_V y, hpa_V x){
+    hpa_ctx *c = ctx_cur();
+    int32_t e;
+    L nx = x->len, ny = y->len;
+    const void *px = nullptr, *py = nullptr;
+    if(is_real(x->typ) && is_real(y->typ)){
+        int32_t t = x->typ > y->typ ? x->typ : y->typ;
+        if(t == HPA_J) R HPA_E_NOT_IMPL;                 // no MEMBER(J) case in the reference
+        if(nx > 0 && (e = promote_dev(c, x, t, &px))) R e;
+        if(ny > 0 && (e = promote_dev(c, y, t, &py))) R e;
+        void *dz; L mark = c->dcur;
+        if((e = out_init(c, z, HPA_B, ny, &dz))) R e;
+        uint8_t *zz = (uint8_t*)dz;
+        switch(t){
+            case HPA_B:
+                if(ny > 0) hipLaunchKernelGGL(k_member_table, dim3(grid_for(ny, 256, 4)), dim3(256), 0, c->stream, (const uint8_t*)py, ny, (const uint8_t*)px, nx, zz, 2);
+                e = check_launch("member"); break;
+            case HPA_H: e = member_t<int16_t>(c, (const int16_t*)py, ny, (const int16_t*)px, nx, zz); break;
+            case HPA_I: e = member_t<int32_t>(c, (const int32_t*)py, ny, (const int32_t*)px, nx, zz); break;
+            case HPA_L: e = member_t<int64_t>(c, (const int64_t*)py, ny, (const int64_t*)px, nx, zz); break;
+            case HPA_F: e = member_t<float>(c, (const float*)py, ny, (const float*)px, nx, zz); break;
+            case HPA_E: e = member_t<double>(c, (const double*)py, ny, (const double*)px, nx, zz); break;
+        }
+        if(e) R e;
+        if(is_inline(z)){ e = commit_inline(c, z, dz); c->dcur = mark; }
+        R e;
+    }
+    if(x->typ == y->typ && is_basic(x->typ)){
+        if(nx > 0 && (e = dev_view(c, x, &px))) R e;
+        if(ny > 0 && (e = dev_view(c, y, &py))) R e;
+        void *dz; L mark = c->dcur;
+        if((e = out_init(c, z, HPA_B, ny, &dz))) R e;
+        uint8_t *zz = (uint8_t*)dz;
+        switch(x->typ){
+            case HPA_C:
+                if(ny > 0) hipLaunchKernelGGL(k_member_table, dim3(grid_for(ny, 256, 4)), dim3(256), 0, c->stream, (const uint8_t*)py, ny, (const uint8_t*)px, nx, zz, 256);
+                e = check_launch("member"); break;
+            case HPA_M: case HPA_D: case HPA_U: case HPA_W: case HPA_T:
+                e = member_t<int32_t>(c, (const int32_t*)py, ny, (const int32_t*)px, nx, zz); break;
+            case HPA_Z: case HPA_Q:
+                e = member_t<int64_t>(c, (const int64_t*)py, ny, (const int64_t*)px, nx, zz); break;
+            default: e = HPA_E_NOT_IMPL;
+        }
+        if(e) R e;
+        if(is_inline(z)){ e = commit_inline(c, z, dz); c->dcur = mark; }
+        R e;
+    }
+    R HPA_E_DOMAIN;
+}
+
+// ---------------------------------------------------------------- join
+__global__ void k_join_build_heads(const uint32_t *sk, L n, const int64_t *pos, const uint32_t *flag, HTab t, int64_t *start){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
+        if(flag[i]){
+            int64_t r = pos[i]; start[r] = i;
+            unsigned long long *s = ht_slot(t, (uint64_t)sk[i], true);
+            *s = (unsigned long long)r;
+        }
+    }
+}
+__global__ void k_join_flags(const uint32_t *sk, L n, uint32_t *flag){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x)
+        flag[i] = (i == 0 || sk[i] != sk[i - 1]) ? 1u : 0u;
+}
+__global__ void k_join_probe(const int32_t *y, L ny, HTab t, const int64_t *start, L g, L nb, uint32_t *cnt, uint32_t *rid){
+    for(L j = (L)blockIdx.x * blockDim.x + threadIdx.x; j < ny; j += (L)gridDim.x * blockDim.x){
+        const unsigned long long *s = ht_find(t, (uint64_t)(uint32_t)y[j]);
+        if(s){ L r = (L)*s; rid[j] = (uint32_t)r; cnt[j] = (uint32_t)((r + 1 < g ? start[r + 1] : nb) - start[r]); }
+        else { rid[j] = 0; cnt[j] = 0; }
+    }
+}
+__global__ void k_join_write(L ny, const uint32_t *cnt, const uint32_t *rid, const int64_t *off, const int64_t *start,
+                             const uint32_t *sidx, int64_t *z0, int64_t *z1, int swap){
+    for(L j = (L)blockIdx.x * blockDim.x + threadIdx.x; j < ny; j += (L)gridDim.x * blockDim.x){
+        uint32_t c0 = cnt[j]; if(!c0) continue;
+        int64_t o = off[j]; L s = start[rid[j]];
+        for(uint32_t q = 0; q < c0; q++){
+            int64_t b = sidx[s + q];
+            if(swap){ z0[o + q] = j; z1[o + q] = b; } else { z0[o + q] = b; z1[o + q] = j; }
+        }
+    }
+}
+// lib_join_index_linear (h_libs.cc:1542-1575): the first matching build row only
+template<typename T>
+__global__ void k_join_linear(const T *x, int nx, const T *y, L ny, int eq, uint32_t *cnt, uint32_t *rid){
+    for(L j = (L)blockIdx.x * blockDim.x + threadIdx.x; j < ny; j += (L)gridDim.x * blockDim.x){
+        T v = y[j]; int hit = -1;
+        for(int i = 0; i < nx; i++){ bool m = eq ? (v == x[i]) : (v != x[i]); if(m){ hit = i; break; } }
+        cnt[j] = hit >= 0 ? 1u : 0u; rid[j] = hit >= 0 ? (uint32_t)hit : 0u;
+    }
+}
+
+static int32_t set_l_vec(hpa_ctx *c, V z, int64_t *p, L n){
+    z->typ = HPA_L; z->len = n;
+    if(n <= 1){ z->g = (char*)&z->b; z->i64 = 0; if(n == 1) R d2h(c, &z->b, p, 8); R 0; }
+    z->g = (char*)p; R 0;
+}
+
+// build on bx (shorter), probe py; `swap` writes (probe, build) into (z0, z1)
+static int32_t join_run(hpa_ctx *c, V z, V bx, V py, int32_t t, int op, bool swap){
+    L nb = bx->len, np = py->len; int32_t e;
+    const void *pb = nullptr, *pp = nullptr;
+    if(nb > 0 && (e = promote_dev(c, bx, t, &pb))) R e;
+    if(np > 0 && (e = promote_dev(c, py, t, &pp))) R e;
+    uint32_t *cnt = (uint32_t*)dalloc(c, 4 * (np > 0 ? np : 1)), *rid = (uint32_t*)dalloc(c, 4 * (np > 0 ? np : 1));
+    int64_t *off = (int64_t*)dalloc(c, 8 * (np + 1));
+    if(!cnt || !rid || !off) R HPA_E_MEMORY;
+    int64_t *start = nullptr; uint32_t *sidx = nullptr;
+    ProfScope ps("join", c->stream);
+    if(nb >= 10){                                    // NUM_JOIN_LINEAR (common.h:152): hash path
+        if(op != 4 || t != HPA_I) R HPA_E_NOT_IMPL;  // v3 radix hash join: i32 equality only (h_hash_v3.cc:233,270-277)
+        uint32_t *k = (uint32_t*)dalloc(c, 4 * nb); if(!k) R HPA_E_MEMORY;
+        hipMemcpyAsync(k, pb, 4 * nb, hipMemcpyDeviceToDevice, c->stream);
+        uint32_t *sk;
+        if((e = radix_sort_pairs_u32(c, k, nullptr, nb, 0, 32, &sk, &sidx))) R e;
+        uint32_t *flag = (uint32_t*)dalloc(c, 4 * nb); int64_t *pos = (int64_t*)dalloc(c, 8 * (nb + 1));
+        if(!flag || !pos) R HPA_E_MEMORY;
+        hipLaunchKernelGGL(k_join_flags, dim3(grid_for(nb, 256, 4)), dim3(256), 0, c->stream, sk, nb, flag);
+        if((e = scan_u32_to_i64(c, flag, pos, nb, pos + nb))) R e;
+        int64_t g = 0; if((e = d2h(c, &g, pos + nb, 8))) R e;
+        HTab tb; if((e = ht_create(c, g, &tb))) R e;
+        start = (int64_t*)dalloc(c, 8 * (g > 0 ? g : 1)); if(!start) R HPA_E_MEMORY;
+        hipLaunchKernelGGL(k_join_build_heads, dim3(grid_for(nb, 256, 4)), dim3(256), 0, c->stream, sk, nb, pos, flag, tb, start);
+        if(np > 0) hipLaunchKernelGGL(k_join_probe, dim3(grid_for(np, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)pp, np, tb, start, g, nb, cnt, rid);
+    } else {
+        if(!(t == HPA_I || t == HPA_L || t == HPA_E)) R HPA_E_NOT_IMPL;
+        // the linear path writes the build row itself: start = identity, one match per probe row
+        start = (int64_t*)dalloc(c, 8 * 16); sidx = (uint32_t*)dalloc(c, 4 * 16);
+        if(!start || !sidx) R HPA_E_MEMORY;
+        int64_t hs[16]; uint32_t hi[16]; for(int i = 0; i < 16; i++){ hs[i] = i; hi[i] = (uint32_t)i; }
+        hipMemcpyAsync(start, hs, sizeof(hs), hipMemcpyHostToDevice, c->stream);
+        hipMemcpyAsync(sidx, hi, sizeof(hi), hipMemcpyHostToDevice, c->stream);
+        if(np > 0){
+            dim3 gg(grid_for(np, 256, 4)), bb(256);
+            int eq = op == 4;
+            if(t == HPA_I) hipLaunchKernelGGL(k_join_linear<int32_t>, gg, bb, 0, c->stream, (const int32_t*)pb, (int)nb, (const int32_t*)pp, np, eq, cnt, rid);
+            else if(t == HPA_L) hipLaunchKernelGGL(k_join_linear<int64_t>, gg, bb, 0, c->stream, (const int64_t*)pb, (int)nb, (const int64_t*)pp, np, eq, cnt, rid);
+            else hipLaunchKernelGGL(k_join_linear<double>, gg, bb, 0, c->stream, (const double*)pb, (int)nb, (const double*)pp, np, eq, cnt, rid);
+        }
+        hipStreamSynchronize(c->stream);
+    }
+    if((e = scan_u32_to_i64(c, cnt, off, np, off + np))) R e;
+    int64_t P = 0; if((e = d2h(c, &P, off + np, 8))) R e;
+    int64_t *z0 = (int64_t*)dalloc(c, 8 * (P > 0 ? P : 1)), *z1 = (int64_t*)dalloc(c, 8 * (P > 0 ? P : 1));
+    if(!z0 || !z1) R HPA_E_MEMORY;
+    if(np > 0) hipLaunchKernelGGL(k_join_write, dim3(grid_for(np, 256, 4)), dim3(256), 0, c->stream, np, cnt, rid, off, start, sidx, z0, z1, swap ? 1 : 0);
+    if((e = check_launch("join"))) R e;
+    V cells = (V)halloc(c, 2 * sizeof(V0)); if(!cells) R HPA_E_MEMORY;
+    z->typ = HPA_G; z->len = 2; z->g = (char*)cells; z->g2 = nullptr;
+    if((e = set_l_vec(c, cells, z0, P))) R e;
+    R set_l_vec(c, cells + 1, z1, P);
+}
+
+static int op_from_name(const char *s){            // backend/h_system.cc:779-789
+    if(!s) R -1;
+    if(!strcmp(s, "lt")) R 0; if(!strcmp(s, "leq")) R 1; if(!strcmp(s, "gt")) R 2;
+    if(!strcmp(s, "geq")) R 3; if(!strcmp(s, "eq")) R 4; if(!strcmp(s, "neq")) R 5;
+    R -1;
+}
+
+// pfnJoinIndex (h_primitive.cc:2122-2155): z = list(i64 x-rows, i64 y-rows)
+extern "C" int32_t pfnJoinIndex(hpa_V z, hpa_V x, hpa_V y, hpa_V f){
+    hpa_ctx *c = ctx_cur();
+    if(f->typ != HPA_Q || f->len < 1) R HPA_E_DOMAIN;
+    if(f->len > 1) R HPA_E_NOT_IMPL;                 // multi-column joins (joinOtherColumns) are not on this path
+    int64_t q; int32_t e = read_scalar(c, f, &q, 8); if(e) R e;
+    int op = op_from_name(hpaSymbolName(q));
+    if(op < 0) R HPA_E_DOMAIN;
+    if(!(is_real(x->typ) && is_real(y->typ))) R HPA_E_NOT_IMPL;
+    if(op != 4 && op != 5) R HPA_E_NOT_IMPL;         // the reference's brute-force path is a debug stop
+    int32_t t = x->typ > y->typ ? x->typ : y->typ;
+    // build on the shorter side; after the swap z0 still indexes x (swap2, h_system.cc:47-56)
+    if(x->len > y->len) R join_run(c, z, y, x, t, op, true);
+    R join_run(c, z, x, y, t, op, false);
+}

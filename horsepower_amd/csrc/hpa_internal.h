@@ -1,0 +1,109 @@
+// Internal declarations shared by the HIP translation units of libhorsepower_amd.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+#include <vector>
+#include <type_traits>
+#include <map>
+#include <string>
+#include "../../include/horsepower_amd.h"
+
+typedef hpa_V  V;
+typedef hpa_V0 V0;
+typedef int64_t L;
+
+#define R return
+
+// ---------------------------------------------------------------- context and arenas
+struct hpa_ctx {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int  device = 0;
+    // device bump arena (the reference's process heap, backend/h_memory.cc:93-112)
+    char *dbase = nullptr; L dcur = 0, dtop = 0, dpeak = 0;
+    // host arena for V0 nodes and list cells
+    char *hbase = nullptr; L hcur = 0, htop = 0;
+    std::map<L, L> marks;         // device watermark -> host watermark when it was taken
+    // pinned host staging for small device->host reads
+    char *pinned = nullptr; L pinned_bytes = 0;
+    // host buffers handed out by hpaToHost (released with the context)
+    std::vector<void*> host_bufs;
+};
+
+hpa_ctx *ctx_cur();
+void    *dalloc(hpa_ctx *c, L bytes);               // nullptr when the arena is full
+void    *halloc(hpa_ctx *c, L bytes);
+V        node_alloc(hpa_ctx *c);
+void     set_error_detail(const char *fmt, ...);
+
+// ---------------------------------------------------------------- value model helpers
+L     type_width(int32_t typ);                      // payload bytes of one element (C: 1)
+L     type_bytes(int32_t typ, L len);               // getTypeSize, backend/h_memory.cc:124-153
+bool  is_basic(int32_t t);
+bool  is_int(int32_t t);
+bool  is_real(int32_t t);
+bool  is_comparable(int32_t t);
+bool  is_inline(V x);                               // vg(x) == &x->b
+bool  is_device(const void *p);
+void  register_device_range(void *p, L bytes);
+void  unregister_device_range(void *p);
+
+// Output allocation following initV's inline-scalar rule.  Returns the device buffer kernels
+// write; for inline outputs this is arena scratch and `commit_inline` copies it back.
+int32_t out_init(hpa_ctx *c, V z, int32_t typ, L len, void **dev);
+int32_t commit_inline(hpa_ctx *c, V z, const void *dev);
+// Device view of x's payload (staging host payloads into the arena).  Inline scalars are
+// staged too, so every operand has a device address.
+int32_t dev_view(hpa_ctx *c, V x, const void **p);
+// Scalar read of element 0 of x as int64 / double bits (host or device payload).
+int32_t read_scalar(hpa_ctx *c, V x, void *dst, L width);
+// Copy small device data to host (synchronous on the context stream).
+int32_t d2h(hpa_ctx *c, void *dst, const void *src, L bytes);
+int32_t check_launch(const char *what);
+
+// ---------------------------------------------------------------- timing
+struct ProfScope {
+    const char *name; hipStream_t s; hipEvent_t a = nullptr, b = nullptr; bool on;
+    ProfScope(const char *n, hipStream_t st);
+    ~ProfScope();
+};
+bool prof_enabled();
+
+// ---------------------------------------------------------------- launch geometry
+int  num_cus();
+inline L ceil_div(L a, L b){ R (a + b - 1) / b; }
+inline int grid_for(L n, int block, int per_thread, int max_blocks = 0){
+    L g = ceil_div(n, (L)block * per_thread);
+    L cap = max_blocks > 0 ? max_blocks : (L)num_cus() * 16;
+    if(g > cap) g = cap;
+    if(g < 1) g = 1;
+    R (int)g;
+}
+
+// ---------------------------------------------------------------- shared device algorithms
+// exclusive scan of n int64 values (in place allowed); total returned through *total_dev
+// (device pointer, may be null).
+int32_t scan_i64(hpa_ctx *c, const int64_t *in, int64_t *out, L n, int64_t *total_dev);
+int32_t scan_u32_to_i64(hpa_ctx *c, const uint32_t *in, int64_t *out, L n, int64_t *total_dev);
+// promote a device vector to type `to` (promoteValue, backend/h_system.cc:168-252)
+int32_t promote_dev(hpa_ctx *c, V x, int32_t to, const void **p);
+// gather: dst[i] = src[idx[i]] for element width w
+int32_t gather_dev(hpa_ctx *c, void *dst, const void *src, const int64_t *idx, L n, L w);
+// stable LSD radix sort of (key, value) pairs by unsigned key bits [lo_bit, hi_bit)
+int32_t radix_sort_pairs_u64(hpa_ctx *c, uint64_t *keys, uint32_t *vals, L n, int lo_bit, int hi_bit,
+                             uint64_t **keys_out, uint32_t **vals_out);
+int32_t radix_sort_pairs_u32(hpa_ctx *c, uint32_t *keys, uint32_t *vals, L n, int lo_bit, int hi_bit,
+                             uint32_t **keys_out, uint32_t **vals_out);
+int32_t iota_u32(hpa_ctx *c, uint32_t *p, L n);
+// Group a sorted key sequence into runs; emits the reference's dict layout with a flat list:
+// keys = first original index of each run, values = flat list (counts, i64 indices).
+int32_t runs_to_group_u64(hpa_ctx *c, V z, const uint64_t *skeys, const uint32_t *sidx, L n,
+                          const uint64_t *eqkeys);
+int32_t runs_to_group_u32(hpa_ctx *c, V z, const uint32_t *skeys, const uint32_t *sidx, L n);
+// Build the dict(keys, flat list) node structure from device arrays.
+int32_t make_group_dict(hpa_ctx *c, V z, int64_t *keys, int64_t *counts, int64_t *flat, L g, L n);
+
+// flat list helpers (backend/h_memory.cc:210-215)
+bool   is_flat(V x);
+int32_t flat_offsets(hpa_ctx *c, V y, const int64_t **offs, const int64_t **idx, L *g, L *n);

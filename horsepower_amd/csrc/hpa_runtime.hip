@@ -1,0 +1,531 @@
+// Runtime of libhorsepower_amd: device/host arenas, contexts, residency, staging, symbols,
+// kernel timing and the device-wide scan used by every order-preserving primitive.
+#include "hpa_internal.h"
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <mutex>
+#include <algorithm>
+
+static std::mutex g_mu;
+static bool g_init = false;
+static int  g_device = 0;
+static int  g_cus = 256;
+static hpa_ctx *g_default = nullptr;
+static thread_local hpa_ctx *t_cur = nullptr;
+static thread_local char t_err[512];
+
+struct Range { uintptr_t lo, hi; };
+static std::vector<Range> g_ranges;            // device ranges: arenas and resident payloads
+static std::map<uintptr_t, L> g_resident;       // resident payload -> bytes
+
+int num_cus(){ R g_cus; }
+
+void set_error_detail(const char *fmt, ...){
+    va_list ap; va_start(ap, fmt); vsnprintf(t_err, sizeof(t_err), fmt, ap); va_end(ap);
+}
+extern "C" const char *hpaLastErrorDetail(void){ R t_err; }
+
+extern "C" const char *hpaErrorString(int32_t e){
+    switch(e){
+        case 0: R "ok";
+        case HPA_E_DOMAIN: R "domain error";
+        case HPA_E_GENERAL: R "general error";
+        case HPA_E_INDEX: R "index error";
+        case HPA_E_DIV_ZERO: R "divide by zero";
+        case HPA_E_LENGTH: R "length error";
+        case HPA_E_TYPE: R "type error";
+        case HPA_E_NULL_VALUE: R "null value";
+        case HPA_E_MATCH: R "match error";
+        case HPA_E_RAZE_LIST: R "raze list error";
+        case HPA_E_UNKNOWN: R "unknown error";
+        case HPA_E_NOT_IMPL: R "not implemented";
+        case HPA_E_MEMORY: R "device arena exhausted";
+        case HPA_E_DEVICE: R "HIP runtime error";
+    }
+    R "error";
+}
+
+int32_t check_launch(const char *what){
+    hipError_t e = hipGetLastError();
+    if(e != hipSuccess){ set_error_detail("%s: %s", what, hipGetErrorString(e)); R HPA_E_DEVICE; }
+    R 0;
+}
+
+// ---------------------------------------------------------------- device ranges
+void register_device_range(void *p, L bytes){
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_ranges.push_back({(uintptr_t)p, (uintptr_t)p + (uintptr_t)bytes});
+}
+void unregister_device_range(void *p){
+    std::lock_guard<std::mutex> lk(g_mu);
+    for(size_t i=0;i<g_ranges.size();i++) if(g_ranges[i].lo == (uintptr_t)p){ g_ranges.erase(g_ranges.begin()+i); break; }
+}
+bool is_device(const void *p){
+    uintptr_t u = (uintptr_t)p;
+    std::lock_guard<std::mutex> lk(g_mu);
+    for(auto &r : g_ranges) if(u >= r.lo && u < r.hi) R true;
+    R false;
+}
+extern "C" int32_t hpaIsDevicePtr(const void *p){ R is_device(p) ? 1 : 0; }
+
+// ---------------------------------------------------------------- contexts
+static hpa_ctx *ctx_new(L arena_bytes, L host_bytes){
+    hpa_ctx *c = new hpa_ctx();
+    c->device = g_device;
+    if(hipSetDevice(g_device) != hipSuccess){ delete c; R nullptr; }
+    if(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess){ delete c; R nullptr; }
+    c->own_stream = true;
+    if(arena_bytes <= 0) arena_bytes = 17179869184LL;          // 16 GiB (backend/h_memory.cc:11)
+    size_t freeb = 0, totalb = 0;
+    if(hipMemGetInfo(&freeb, &totalb) == hipSuccess && (size_t)arena_bytes > freeb - (512ull<<20))
+        arena_bytes = (L)(freeb > (1ull<<30) ? freeb - (512ull<<20) : freeb/2);
+    if(hipMalloc((void**)&c->dbase, (size_t)arena_bytes) != hipSuccess){
+        hipStreamDestroy(c->stream); delete c; R nullptr;
+    }
+    c->dtop = arena_bytes;
+    register_device_range(c->dbase, arena_bytes);
+    c->hbase = (char*)malloc((size_t)host_bytes); c->htop = host_bytes;
+    c->pinned_bytes = 1 << 20;
+    if(hipHostMalloc((void**)&c->pinned, (size_t)c->pinned_bytes, hipHostMallocDefault) != hipSuccess) c->pinned = nullptr;
+    R c;
+}
+
+extern "C" int32_t hpaInit(int32_t device, int64_t arena_bytes){
+    if(g_init) R 0;
+    int n = 0;
+    if(hipGetDeviceCount(&n) != hipSuccess || n <= device){ set_error_detail("no HIP device %d", device); R HPA_E_DEVICE; }
+    g_device = device;
+    if(hipSetDevice(device) != hipSuccess) R HPA_E_DEVICE;
+    hipDeviceProp_t pr;
+    if(hipGetDeviceProperties(&pr, device) == hipSuccess) g_cus = pr.multiProcessorCount;
+    const char *env = getenv("HPA_ARENA_BYTES");
+    if(env && arena_bytes <= 0) arena_bytes = atoll(env);
+    g_default = ctx_new(arena_bytes, 256LL << 20);
+    if(!g_default){ set_error_detail("default context allocation failed"); R HPA_E_MEMORY; }
+    g_init = true;
+    R 0;
+}
+
+extern "C" void hpaCtxDestroy(hpa_ctx *c){
+    if(!c) R;
+    hipStreamSynchronize(c->stream);
+    if(c->dbase){ unregister_device_range(c->dbase); hipFree(c->dbase); }
+    if(c->own_stream) hipStreamDestroy(c->stream);
+    if(c->pinned) hipHostFree(c->pinned);
+    for(void *p : c->host_bufs) free(p);
+    free(c->hbase);
+    if(t_cur == c) t_cur = nullptr;
+    delete c;
+}
+
+extern "C" void hpaShutdown(void){
+    if(!g_init) R;
+    hipDeviceSynchronize();
+    hpaCtxDestroy(g_default); g_default = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        for(auto &kv : g_resident) hipFree((void*)kv.first);
+        g_resident.clear(); g_ranges.clear();
+    }
+    g_init = false;
+}
+
+extern "C" hpa_ctx *hpaCtxCreate(int64_t arena_bytes){
+    if(!g_init && hpaInit(0, 0)) R nullptr;
+    R ctx_new(arena_bytes > 0 ? arena_bytes : (1LL<<30), 64LL << 20);
+}
+extern "C" hpa_ctx *hpaCtxDefault(void){ R g_default; }
+extern "C" void hpaCtxSetCurrent(hpa_ctx *c){ t_cur = c; }
+extern "C" hpa_ctx *hpaCtxCurrent(void){ R ctx_cur(); }
+extern "C" void *hpaCtxStream(hpa_ctx *c){ R c ? (void*)c->stream : nullptr; }
+extern "C" int64_t hpaCtxArenaBytes(hpa_ctx *c){ R c ? c->dtop : 0; }
+extern "C" int64_t hpaCtxArenaPeak(hpa_ctx *c){ R c ? c->dpeak : 0; }
+extern "C" void hpaCtxResetPeak(hpa_ctx *c){ if(c) c->dpeak = c->dcur; }
+
+hpa_ctx *ctx_cur(){
+    if(t_cur) R t_cur;
+    if(!g_init) hpaInit(0, 0);
+    R g_default;
+}
+
+void *dalloc(hpa_ctx *c, L bytes){
+    L a = (bytes + 255) & ~(L)255;
+    if(a == 0) a = 256;
+    if(c->dcur + a > c->dtop){
+        set_error_detail("device arena full: %lld + %lld > %lld", (long long)c->dcur, (long long)a, (long long)c->dtop);
+        R nullptr;
+    }
+    void *p = c->dbase + c->dcur; c->dcur += a;
+    if(c->dcur > c->dpeak) c->dpeak = c->dcur;
+    R p;
+}
+
+void *halloc(hpa_ctx *c, L bytes){
+    L a = (bytes + 15) & ~(L)15;
+    if(c->hcur + a > c->htop){ set_error_detail("host node arena full"); R nullptr; }
+    void *p = c->hbase + c->hcur; c->hcur += a;
+    memset(p, 0, (size_t)a);
+    R p;
+}
+
+V node_alloc(hpa_ctx *c){ R (V)halloc(c, sizeof(V0)); }
+
+extern "C" hpa_V hpaAllocNode(void){ R node_alloc(ctx_cur()); }
+
+extern "C" int64_t hpaGetHeapOffset(void){
+    hpa_ctx *c = ctx_cur();
+    c->marks[c->dcur] = c->hcur;
+    R c->dcur;
+}
+extern "C" int64_t hpaSetHeapOffset(int64_t cur){
+    hpa_ctx *c = ctx_cur();
+    hipStreamSynchronize(c->stream);          // outstanding kernels may still use the payloads
+    auto it = c->marks.find(cur);
+    if(it != c->marks.end()) c->hcur = it->second;
+    c->marks.erase(c->marks.upper_bound(cur), c->marks.end());
+    for(void *p : c->host_bufs) free(p);
+    c->host_bufs.clear();
+    R c->dcur = cur;
+}
+
+// ---------------------------------------------------------------- value model
+L type_width(int32_t typ){
+    switch(typ){
+        case HPA_B: case HPA_J: case HPA_C: R 1;
+        case HPA_H: R 2;
+        case HPA_I: case HPA_F: case HPA_M: case HPA_D: case HPA_U: case HPA_W: case HPA_T: R 4;
+        case HPA_L: case HPA_E: case HPA_X: case HPA_Q: case HPA_S: case HPA_Z: case HPA_Y: R 8;
+        case HPA_G: case HPA_N: case HPA_A: case HPA_K: case HPA_V: R (L)sizeof(V0);
+    }
+    R -1;
+}
+L type_bytes(int32_t typ, L len){ R type_width(typ) * len + (typ == HPA_C ? 1 : 0); }
+bool is_basic(int32_t t){ R t >= HPA_B && t <= HPA_T; }
+bool is_int(int32_t t){ R t==HPA_B||t==HPA_J||t==HPA_H||t==HPA_I||t==HPA_L; }
+bool is_real(int32_t t){ R is_int(t) || t==HPA_F || t==HPA_E; }
+bool is_comparable(int32_t t){ R is_real(t) || t==HPA_C || t==HPA_Q || (t>=HPA_M && t<=HPA_T); }
+bool is_inline(V x){ R x->g == (char*)&x->b; }
+bool is_flat(V x){ R x->typ == HPA_G && x->g2 != nullptr; }
+
+int32_t d2h(hpa_ctx *c, void *dst, const void *src, L bytes){
+    if(bytes <= 0) R 0;
+    if(c->pinned && bytes <= c->pinned_bytes){
+        if(hipMemcpyAsync(c->pinned, src, (size_t)bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess) R HPA_E_DEVICE;
+        if(hipStreamSynchronize(c->stream) != hipSuccess) R HPA_E_DEVICE;
+        memcpy(dst, c->pinned, (size_t)bytes);
+        R 0;
+    }
+    if(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess) R HPA_E_DEVICE;
+    if(hipStreamSynchronize(c->stream) != hipSuccess) R HPA_E_DEVICE;
+    R 0;
+}
+
+int32_t out_init(hpa_ctx *c, V z, int32_t typ, L len, void **dev){
+    z->typ = typ; z->len = len;
+    L bytes = type_bytes(typ, len);
+    if(is_basic(typ) && len <= 1){
+        z->g = (char*)&z->b;
+        z->i64 = 0;
+        *dev = dalloc(c, 16);
+        R *dev ? 0 : HPA_E_MEMORY;
+    }
+    void *p = dalloc(c, bytes > 0 ? bytes : 1);
+    if(!p) R HPA_E_MEMORY;
+    z->g = (char*)p;
+    if(typ == HPA_G) z->g2 = nullptr;
+    *dev = p;
+    R 0;
+}
+
+int32_t commit_inline(hpa_ctx *c, V z, const void *dev){
+    if(!is_inline(z) || z->len < 1) R 0;
+    L w = type_width(z->typ);
+    R d2h(c, &z->b, dev, w);
+}
+
+extern "C" int32_t hpaInitV(hpa_V x, int32_t typ, int64_t len){
+    void *d; R out_init(ctx_cur(), x, typ, len, &d);
+}
+
+int32_t dev_view(hpa_ctx *c, V x, const void **p){
+    L bytes = type_bytes(x->typ, x->len);
+    if(x->len == 0){ *p = dalloc(c, 16); R *p ? 0 : HPA_E_MEMORY; }
+    if(is_inline(x)){
+        void *d = dalloc(c, 16); if(!d) R HPA_E_MEMORY;
+        if(hipMemcpyAsync(d, &x->b, 8, hipMemcpyHostToDevice, c->stream) != hipSuccess) R HPA_E_DEVICE;
+        if(hipStreamSynchronize(c->stream) != hipSuccess) R HPA_E_DEVICE;
+        *p = d; R 0;
+    }
+    if(is_device(x->g)){ *p = x->g; R 0; }
+    void *d = dalloc(c, bytes); if(!d) R HPA_E_MEMORY;
+    if(hipMemcpyAsync(d, x->g, (size_t)bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess) R HPA_E_DEVICE;
+    if(hipStreamSynchronize(c->stream) != hipSuccess) R HPA_E_DEVICE;
+    *p = d;
+    R 0;
+}
+
+int32_t read_scalar(hpa_ctx *c, V x, void *dst, L width){
+    if(is_inline(x) || !is_device(x->g)){ memcpy(dst, x->g, (size_t)width); R 0; }
+    R d2h(c, dst, x->g, width);
+}
+
+// ---------------------------------------------------------------- residency
+static int32_t upload_rec(hpa_ctx *c, V x){
+    if(is_basic(x->typ)){
+        if(is_inline(x) || x->len <= 1 || is_device(x->g)) R 0;
+        L bytes = type_bytes(x->typ, x->len);
+        void *d = nullptr;
+        if(hipMalloc(&d, (size_t)bytes) != hipSuccess){ set_error_detail("hipMalloc(%lld) failed", (long long)bytes); R HPA_E_MEMORY; }
+        if(hipMemcpy(d, x->g, (size_t)bytes, hipMemcpyHostToDevice) != hipSuccess){ hipFree(d); R HPA_E_DEVICE; }
+        register_device_range(d, bytes);
+        { std::lock_guard<std::mutex> lk(g_mu); g_resident[(uintptr_t)d] = bytes; }
+        x->g = (char*)d;
+        R 0;
+    }
+    if(x->typ == HPA_G || x->typ == HPA_N || x->typ == HPA_A){
+        if(is_flat(x)) R upload_rec(c, x->g2);
+        L n = x->typ == HPA_G ? x->len : 2;
+        for(L i=0;i<n;i++){ int32_t e = upload_rec(c, ((V)x->g) + i); if(e) R e; }
+        R 0;
+    }
+    R HPA_E_NOT_IMPL;
+}
+extern "C" int32_t hpaUpload(hpa_V x){ R upload_rec(ctx_cur(), x); }
+
+extern "C" int32_t hpaFreeResident(hpa_V x){
+    void *p = x->g;
+    L bytes = 0;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_resident.find((uintptr_t)p);
+        if(it == g_resident.end()) R HPA_E_DOMAIN;
+        bytes = it->second; g_resident.erase(it);
+    }
+    (void)bytes;
+    unregister_device_range(p);
+    hipDeviceSynchronize();
+    hipFree(p);
+    x->g = nullptr;
+    R 0;
+}
+
+static int32_t tohost_rec(hpa_ctx *c, V x){
+    if(is_basic(x->typ)){
+        if(is_inline(x) || !is_device(x->g)) R 0;
+        L bytes = type_bytes(x->typ, x->len);
+        void *h = malloc((size_t)(bytes > 0 ? bytes : 1));
+        if(!h) R HPA_E_MEMORY;
+        if(hipMemcpyAsync(h, x->g, (size_t)bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess){ free(h); R HPA_E_DEVICE; }
+        if(hipStreamSynchronize(c->stream) != hipSuccess){ free(h); R HPA_E_DEVICE; }
+        c->host_bufs.push_back(h);
+        x->g = (char*)h;
+        R 0;
+    }
+    if(x->typ == HPA_G && is_flat(x)){
+        // counts vector lives on the device too
+        if(!is_inline(x) && is_device(x->g)){
+            L bytes = 8 * x->len;
+            void *h = malloc((size_t)(bytes > 0 ? bytes : 1));
+            if(hipMemcpyAsync(h, x->g, (size_t)bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess){ free(h); R HPA_E_DEVICE; }
+            if(hipStreamSynchronize(c->stream) != hipSuccess){ free(h); R HPA_E_DEVICE; }
+            c->host_bufs.push_back(h);
+            x->g = (char*)h;
+        }
+        R tohost_rec(c, x->g2);
+    }
+    if(x->typ == HPA_G || x->typ == HPA_N || x->typ == HPA_A){
+        L n = x->typ == HPA_G ? x->len : 2;
+        for(L i=0;i<n;i++){ int32_t e = tohost_rec(c, ((V)x->g) + i); if(e) R e; }
+        R 0;
+    }
+    R 0;
+}
+extern "C" int32_t hpaToHost(hpa_V x){ R tohost_rec(ctx_cur(), x); }
+
+extern "C" int32_t hpaReadPayload(hpa_V x, void *dst, int64_t bytes){
+    hpa_ctx *c = ctx_cur();
+    if(bytes <= 0) R 0;
+    if(is_inline(x) || !is_device(x->g)){ memcpy(dst, x->g, (size_t)bytes); R 0; }
+    if(hipMemcpyAsync(dst, x->g, (size_t)bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess) R HPA_E_DEVICE;
+    R hipStreamSynchronize(c->stream) == hipSuccess ? 0 : HPA_E_DEVICE;
+}
+
+extern "C" int32_t hpaSync(void){
+    hpa_ctx *c = ctx_cur();
+    R hipStreamSynchronize(c->stream) == hipSuccess ? 0 : HPA_E_DEVICE;
+}
+
+// ---------------------------------------------------------------- symbols
+static std::vector<std::string> g_syms;
+extern "C" int64_t hpaSymbol(const char *name){
+    std::lock_guard<std::mutex> lk(g_mu);
+    for(size_t i=0;i<g_syms.size();i++) if(g_syms[i] == name) R (int64_t)i;
+    g_syms.push_back(name);
+    R (int64_t)g_syms.size() - 1;
+}
+extern "C" const char *hpaSymbolName(int64_t id){
+    std::lock_guard<std::mutex> lk(g_mu);
+    if(id < 0 || id >= (int64_t)g_syms.size()) R nullptr;
+    R g_syms[id].c_str();
+}
+
+// ---------------------------------------------------------------- kernel timing
+static bool g_prof = false;
+struct PendingEv { std::string name; hipEvent_t a, b; };
+static std::vector<PendingEv> g_pending;
+static std::map<std::string, std::pair<int64_t,double>> g_stats;
+
+bool prof_enabled(){ R g_prof; }
+ProfScope::ProfScope(const char *n, hipStream_t st) : name(n), s(st), on(g_prof){
+    if(!on) R;
+    hipEventCreate(&a); hipEventCreate(&b);
+    hipEventRecord(a, s);
+}
+ProfScope::~ProfScope(){
+    if(!on) R;
+    hipEventRecord(b, s);
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_pending.push_back({name, a, b});
+}
+static void prof_resolve(){
+    std::vector<PendingEv> p;
+    { std::lock_guard<std::mutex> lk(g_mu); p.swap(g_pending); }
+    for(auto &e : p){
+        hipEventSynchronize(e.b);
+        float ms = 0; hipEventElapsedTime(&ms, e.a, e.b);
+        hipEventDestroy(e.a); hipEventDestroy(e.b);
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto &s = g_stats[e.name]; s.first++; s.second += ms;
+    }
+}
+extern "C" void hpaProfileEnable(int32_t on){ g_prof = on != 0; }
+extern "C" void hpaProfileReset(void){ prof_resolve(); std::lock_guard<std::mutex> lk(g_mu); g_stats.clear(); }
+extern "C" int32_t hpaProfileStats(const char *kernel, int64_t *count, double *total_ms){
+    prof_resolve();
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_stats.find(kernel);
+    if(it == g_stats.end()){ *count = 0; *total_ms = 0; R 1; }
+    *count = it->second.first; *total_ms = it->second.second; R 0;
+}
+extern "C" int32_t hpaProfileList(char *buf, int64_t cap){
+    prof_resolve();
+    std::lock_guard<std::mutex> lk(g_mu);
+    std::string s;
+    for(auto &kv : g_stats){ char line[256]; snprintf(line, sizeof(line), "%s %lld %.6f\n", kv.first.c_str(), (long long)kv.second.first, kv.second.second); s += line; }
+    if((int64_t)s.size() + 1 > cap) R (int32_t)(s.size() + 1);
+    memcpy(buf, s.c_str(), s.size() + 1);
+    R 0;
+}
+
+// ---------------------------------------------------------------- device-wide exclusive scan
+// Three-phase reduce-then-scan over int64: per-block sums, a single-block scan of the block
+// sums, then per-block scans seeded with the block offset.  Deterministic and order-exact.
+#define SCAN_BLOCK 256
+#define SCAN_ITEMS 16
+template<typename TI>
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_reduce(const TI *in, L n, int64_t *bsum){
+    __shared__ int64_t sh[SCAN_BLOCK / 64];
+    L base = (L)blockIdx.x * SCAN_BLOCK * SCAN_ITEMS;
+    int64_t s = 0;
+    for(int k=0;k<SCAN_ITEMS;k++){ L i = base + (L)k*SCAN_BLOCK + threadIdx.x; if(i < n) s += (int64_t)in[i]; }
+    for(int o=32;o>0;o>>=1) s += __shfl_down(s, o, 64);
+    if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if(threadIdx.x == 0){ int64_t t = 0; for(int w=0;w<SCAN_BLOCK/64;w++) t += sh[w]; bsum[blockIdx.x] = t; }
+}
+__global__ void __launch_bounds__(1024) k_scan_blocks(int64_t *bsum, L nb, int64_t *total){
+    __shared__ int64_t sh[1024];
+    __shared__ int64_t carry;
+    if(threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for(L base = 0; base < nb; base += 1024){
+        L i = base + threadIdx.x;
+        int64_t v = i < nb ? bsum[i] : 0;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for(int o=1;o<1024;o<<=1){
+            int64_t t = threadIdx.x >= (unsigned)o ? sh[threadIdx.x - o] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += t;
+            __syncthreads();
+        }
+        int64_t incl = sh[threadIdx.x];
+        if(i < nb) bsum[i] = carry + incl - v;
+        __syncthreads();
+        if(threadIdx.x == 1023) carry += incl;
+        __syncthreads();
+    }
+    if(threadIdx.x == 0 && total) *total = carry;
+}
+template<typename TI>
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_down(const TI *in, int64_t *out, L n, const int64_t *boff){
+    __shared__ int64_t wsum[SCAN_BLOCK / 64];
+    __shared__ int64_t run;
+    L base = (L)blockIdx.x * SCAN_BLOCK * SCAN_ITEMS;
+    if(threadIdx.x == 0) run = boff[blockIdx.x];
+    __syncthreads();
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for(int k=0;k<SCAN_ITEMS;k++){
+        L i = base + (L)k*SCAN_BLOCK + threadIdx.x;
+        int64_t v = i < n ? (int64_t)in[i] : 0;
+        int64_t x = v;
+        for(int o=1;o<64;o<<=1){ int64_t t = __shfl_up(x, o, 64); if(lane >= o) x += t; }
+        if(lane == 63) wsum[w] = x;
+        __syncthreads();
+        int64_t pre = run;
+        for(int q=0;q<w;q++) pre += wsum[q];
+        if(i < n) out[i] = pre + x - v;
+        __syncthreads();
+        if(threadIdx.x == SCAN_BLOCK - 1) run = pre + x;
+        __syncthreads();
+    }
+}
+template<typename TI>
+static int32_t scan_impl(hpa_ctx *c, const TI *in, int64_t *out, L n, int64_t *total_dev){
+    L per = (L)SCAN_BLOCK * SCAN_ITEMS;
+    L nb = ceil_div(n, per); if(nb < 1) nb = 1;
+    int64_t *bsum = (int64_t*)dalloc(c, 8 * nb);
+    if(!bsum) R HPA_E_MEMORY;
+    if(n > 0) hipLaunchKernelGGL(k_scan_reduce<TI>, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, c->stream, in, n, bsum);
+    else hipMemsetAsync(bsum, 0, 8, c->stream);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, c->stream, bsum, nb, total_dev);
+    if(n > 0) hipLaunchKernelGGL(k_scan_down<TI>, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, c->stream, in, out, n, bsum);
+    R check_launch("scan");
+}
+int32_t scan_i64(hpa_ctx *c, const int64_t *in, int64_t *out, L n, int64_t *total_dev){ R scan_impl<int64_t>(c, in, out, n, total_dev); }
+int32_t scan_u32_to_i64(hpa_ctx *c, const uint32_t *in, int64_t *out, L n, int64_t *total_dev){ R scan_impl<uint32_t>(c, in, out, n, total_dev); }
+
+__global__ void k_iota_u32(uint32_t *p, L n){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x) p[i] = (uint32_t)i;
+}
+int32_t iota_u32(hpa_ctx *c, uint32_t *p, L n){
+    if(n <= 0) R 0;
+    hipLaunchKernelGGL(k_iota_u32, dim3(grid_for(n, 256, 4)), dim3(256), 0, c->stream, p, n);
+    R check_launch("iota");
+}
+
+// flat list (CSR) accessors: counts in vg(y) (i64, device), indices in vg2(y) (i64, device)
+__global__ void k_dummy(){}
+int32_t flat_offsets(hpa_ctx *c, V y, const int64_t **offs, const int64_t **idx, L *g, L *n){
+    if(!is_flat(y)) R HPA_E_DOMAIN;
+    *g = y->len; *n = y->g2->len;
+    const void *pc = nullptr, *pi = nullptr;
+    int32_t e;
+    if(is_device(y->g)) pc = y->g;
+    else {
+        // host counts (a flat list built by the caller): stage 8*g bytes
+        void *d = dalloc(c, 8 * (*g > 0 ? *g : 1)); if(!d) R HPA_E_MEMORY;
+        if(*g > 0 && hipMemcpyAsync(d, y->g, (size_t)(8 * *g), hipMemcpyHostToDevice, c->stream) != hipSuccess) R HPA_E_DEVICE;
+        if(hipStreamSynchronize(c->stream) != hipSuccess) R HPA_E_DEVICE;
+        pc = d;
+    }
+    V t = y->g2;
+    if(t->typ != HPA_L) R HPA_E_TYPE;
+    if((e = dev_view(c, t, &pi))) R e;
+    int64_t *o = (int64_t*)dalloc(c, 8 * (*g + 1));
+    if(!o) R HPA_E_MEMORY;
+    if((e = scan_i64(c, (const int64_t*)pc, o, *g, o + *g))) R e;
+    *offs = o; *idx = (const int64_t*)pi;
+    R 0;
+}

@@ -1,0 +1,143 @@
+"""HorseIR query plans for the benchmark workloads, written against the pfn* ABI.
+
+``*_plain`` functions are the interpreter's statement sequences of the reference programs
+(tests/hyper/online-hir/q6.hir:1-27, q1.hir:1-72, q3.hir:1-85), restated for the synthetic
+schemas in datagen.py (dates as day numbers, quantity i32, i32 keys).  ``*_fused`` functions
+are what the optimising compiler emits for the same programs -- FE/FA fused loops and FP
+grouped reductions (optimizer/autofusion.cc:770-935, optimizer/pattern.cc:180-219;
+src/horseir/v1/compile-static/opt/q6.h:31-48, cur/q1.h) -- on the HIP fused entry points.
+Both take a backend (backend.Backend) and a dict of column nodes, and return numpy results.
+"""
+import numpy as np
+
+from . import datagen as D
+from .vnode import H_B, H_E, H_I, H_L, expr
+
+
+def load_table(B, cols):
+    return {k: B.vec(v) for k, v in cols.items()}
+
+
+# ------------------------------------------------------------------------------------- Q6
+def q6_plain(B, T, lo=D.Q6_LO, hi=D.Q6_HI, dlo=0.05, dhi=0.07, qmax=24):
+    t18 = B.pfnGeq(T["ship_day"], B.scalar(lo, H_I))
+    t19 = B.pfnLt(T["ship_day"], B.scalar(hi, H_I))
+    t20 = B.pfnAnd(t18, t19)
+    t21 = B.pfnLt(T["quantity"], B.scalar(qmax, H_I))
+    t22 = B.pfnGeq(T["discount"], B.scalar(dlo, H_E))
+    t23 = B.pfnLeq(T["discount"], B.scalar(dhi, H_E))
+    t24 = B.pfnAnd(t22, t23)
+    t25 = B.pfnAnd(t20, t21)
+    t26 = B.pfnAnd(t25, t24)
+    t32 = B.pfnCompress(t26, T["price"])
+    t33 = B.pfnCompress(t26, T["discount"])
+    t43 = B.pfnMul(t32, t33)
+    t44 = B.pfnSum(t43)
+    return {"revenue": B.read_scalar(t44)}
+
+
+def q6_cond(lo=D.Q6_LO, hi=D.Q6_HI, dlo=0.05, dhi=0.07, qmax=24):
+    """FA condition over columns (ship_day, quantity, discount, price)."""
+    return expr([("COL", 0), ("I", lo), "GEQ", ("COL", 0), ("I", hi), "LT", "AND",
+                 ("COL", 1), ("I", qmax), "LT", "AND",
+                 ("COL", 2), ("F", dlo), "GEQ", ("COL", 2), ("F", dhi), "LEQ", "AND", "AND"])
+
+
+def q6_fused(B, T, **kw):
+    """q6_chf_01: filter -> compress -> mul -> sum in one pass (opt/q6.h:31-48)."""
+    cols = [T["ship_day"], T["quantity"], T["discount"], T["price"]]
+    body = expr([("COL", 3), ("COL", 2), "MUL"])
+    z = B.fused_reduce(0, cols, q6_cond(**kw), body)
+    return {"revenue": B.read_scalar(z)}
+
+
+# ------------------------------------------------------------------------------------- Q1
+Q1_NAMES = ("return_flag", "line_status", "sum_qty", "sum_base_price", "sum_disc_price",
+            "sum_charge", "avg_qty", "avg_price", "avg_disc", "count_order")
+
+
+def q1_plain(B, T, ship_max=D.Q1_SHIP_MAX):
+    one = B.scalar(1, H_I)
+    t18 = B.pfnLeq(T["ship_day"], B.scalar(ship_max, H_I))
+    t23 = B.pfnCompress(t18, T["quantity"])
+    t24 = B.pfnCompress(t18, T["price"])
+    t25 = B.pfnCompress(t18, T["discount"])
+    t26 = B.pfnCompress(t18, T["tax"])
+    t27 = B.pfnCompress(t18, T["return_flag"])
+    t28 = B.pfnCompress(t18, T["line_status"])
+    t35 = B.pfnMinus(one, t25)
+    t36 = B.pfnMul(t24, t35)
+    t37 = B.pfnPlus(one, t26)
+    t38 = B.pfnMinus(one, t25)
+    t39 = B.pfnMul(t24, t38)
+    t40 = B.pfnMul(t39, t37)
+    t41 = B.list([t27, t28])
+    t42 = B.pfnGroup(t41)
+    t43 = B.pfnKeys(t42)
+    t44 = B.pfnValues(t42)
+    t45 = B.pfnIndex(t27, t43)
+    t46 = B.pfnIndex(t28, t43)
+
+    def agg(col, fn):
+        return B.pfnRaze(B.each(B.each_right(col, t44, "pfnIndex"), fn))
+
+    t49 = agg(t23, "pfnSum")
+    t52 = agg(t23, "pfnAvg")
+    t55 = agg(t24, "pfnSum")
+    t58 = agg(t24, "pfnAvg")
+    t61 = agg(t36, "pfnSum")
+    t64 = agg(t40, "pfnSum")
+    t70 = agg(t25, "pfnAvg")
+    t73 = agg(t27, "pfnLen")
+    t74 = B.list([t45, t46])
+    t75 = B.pfnOrderBy(t74, B.bools([1, 1]))
+    out = [B.pfnIndex(v, t75) for v in (t45, t46, t49, t55, t61, t64, t52, t58, t70, t73)]
+    return dict(zip(Q1_NAMES, (B.read(v) for v in out)))
+
+
+def q1_fused(B, T, ship_max=D.Q1_SHIP_MAX):
+    """FA scan (one mask, six columns), FE arithmetic, group, FP1/FP2/FP7 grouped reductions."""
+    mask = B.pfnLeq(T["ship_day"], B.scalar(ship_max, H_I))
+    qty, price, disc, tax, flag, status = B.compress_many(
+        mask, [T["quantity"], T["price"], T["discount"], T["tax"], T["return_flag"], T["line_status"]])
+    disc_price = B.fused_map(H_E, [price, disc], expr([("COL", 0), ("I", 1), ("COL", 1), "SUB", "MUL"]))
+    charge = B.fused_map(H_E, [price, disc, tax], expr(
+        [("COL", 0), ("I", 1), ("COL", 1), "SUB", "MUL", ("I", 1), ("COL", 2), "ADD", "MUL"]))
+    g = B.pfnGroup(B.list([flag, status]))
+    keys, groups = B.pfnKeys(g), B.pfnValues(g)
+    kf, ks = B.pfnIndex(flag, keys), B.pfnIndex(status, keys)
+    aggs = [B.group_reduce(0, qty, groups), B.group_reduce(0, price, groups),
+            B.group_reduce(0, disc_price, groups), B.group_reduce(0, charge, groups),
+            B.group_reduce(1, qty, groups), B.group_reduce(1, price, groups),
+            B.group_reduce(1, disc, groups), B.group_reduce(4, flag, groups)]
+    perm = B.pfnOrderBy(B.list([kf, ks]), B.bools([1, 1]))
+    out = [B.pfnIndex(v, perm) for v in [kf, ks] + aggs]
+    return dict(zip(Q1_NAMES, (B.read(v) for v in out)))
+
+
+# ------------------------------------------------------------------------------------- Q3
+Q3_NAMES = ("orderkey", "revenue", "order_day")
+
+
+def q3_plain(B, Dm, F, day=D.Q3_DAY, limit=10):
+    d_mask = B.pfnAnd(B.pfnEq(Dm["segment"], B.scalar(1, H_I)), B.pfnLt(Dm["order_day"], B.scalar(day, H_I)))
+    lk = B.pfnCompress(d_mask, Dm["key"])
+    lday = B.pfnCompress(d_mask, Dm["order_day"])
+    f_mask = B.pfnGt(F["ship_day"], B.scalar(day, H_I))
+    rk = B.pfnCompress(f_mask, F["fk"])
+    rp = B.pfnCompress(f_mask, F["price"])
+    rd = B.pfnCompress(f_mask, F["discount"])
+    grp = B.pfnGroup(rk)
+    gkeys, gvals = B.pfnKeys(grp), B.pfnValues(grp)
+    gk = B.pfnIndex(rk, gkeys)
+    rev = B.pfnMul(B.pfnMinus(B.scalar(1, H_L), rd), rp)
+    gsum = B.pfnRaze(B.each(B.each_right(rev, gvals, "pfnIndex"), "pfnSum"))
+    j = B.pfnJoinIndex(lk, gk, B.sym("eq"))
+    j0, j1 = B.pfnIndex(j, B.scalar(0, H_L)), B.pfnIndex(j, B.scalar(1, H_L))
+    revenue = B.pfnIndex(gsum, j1)
+    oday = B.pfnIndex(lday, j0)
+    okey = B.pfnIndex(lk, j0)
+    perm = B.pfnOrderBy(B.list([revenue, oday]), B.bools([0, 1]))
+    res = {k: B.read(B.pfnIndex(v, perm)) for k, v in zip(Q3_NAMES, (okey, revenue, oday))}
+    res["rows"] = int(B.read(perm).shape[0])
+    return {k: (v[:limit] if isinstance(v, np.ndarray) else v) for k, v in res.items()}

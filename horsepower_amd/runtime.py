@@ -1,0 +1,173 @@
+"""Loader of libhorsepower_amd.so and the device-side value-node helpers.
+
+The HIP library is the only compute path: ``load()`` raises when the built library is
+missing, and nothing here falls back to host computation.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .vnode import (V0, V, H_B, H_C, H_G, H_L, H_N, H_Q, NP_OF, TYP_OF, Expr, HorseError,
+                    PrimitiveLib)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhorsepower_amd.so")
+
+_lib = None
+
+
+def load(init=True, device=0, arena_bytes=0):
+    """Load (once) and initialise the HIP primitive library."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python -m horsepower_amd.build`")
+        _lib = Device(C.CDLL(LIB_PATH))
+    if init:
+        _lib.init(device, arena_bytes)
+    return _lib
+
+
+def _bind(dll, name, res, args):
+    f = getattr(dll, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+class Device(PrimitiveLib):
+    """The HIP library: pfn* primitives plus runtime, residency and fused entry points."""
+
+    def __init__(self, dll):
+        super().__init__(dll, "")
+        i32, i64, vp = C.c_int32, C.c_int64, C.c_void_p
+        self._init = _bind(dll, "hpaInit", i32, [i32, i64])
+        _bind(dll, "hpaShutdown", None, [])
+        _bind(dll, "hpaErrorString", C.c_char_p, [i32])
+        _bind(dll, "hpaLastErrorDetail", C.c_char_p, [])
+        _bind(dll, "hpaAllocNode", V, [])
+        _bind(dll, "hpaGetHeapOffset", i64, [])
+        _bind(dll, "hpaSetHeapOffset", i64, [i64])
+        _bind(dll, "hpaInitV", i32, [V, i32, i64])
+        _bind(dll, "hpaUpload", i32, [V])
+        _bind(dll, "hpaFreeResident", i32, [V])
+        _bind(dll, "hpaToHost", i32, [V])
+        _bind(dll, "hpaReadPayload", i32, [V, vp, i64])
+        _bind(dll, "hpaIsDevicePtr", i32, [vp])
+        _bind(dll, "hpaSync", i32, [])
+        _bind(dll, "hpaCtxCreate", vp, [i64])
+        _bind(dll, "hpaCtxDestroy", None, [vp])
+        _bind(dll, "hpaCtxDefault", vp, [])
+        _bind(dll, "hpaCtxSetCurrent", None, [vp])
+        _bind(dll, "hpaCtxCurrent", vp, [])
+        _bind(dll, "hpaCtxStream", vp, [vp])
+        _bind(dll, "hpaCtxArenaBytes", i64, [vp])
+        _bind(dll, "hpaCtxArenaPeak", i64, [vp])
+        _bind(dll, "hpaCtxResetPeak", None, [vp])
+        _bind(dll, "hpaProfileEnable", None, [i32])
+        _bind(dll, "hpaProfileReset", None, [])
+        _bind(dll, "hpaProfileStats", i32, [C.c_char_p, C.POINTER(i64), C.POINTER(C.c_double)])
+        _bind(dll, "hpaProfileList", i32, [C.c_char_p, i64])
+        _bind(dll, "hpaSymbol", i64, [C.c_char_p])
+        _bind(dll, "hpaSymbolName", C.c_char_p, [i64])
+        _bind(dll, "hpaFusedMap", i32, [V, i32, i32, C.POINTER(V), C.POINTER(Expr)])
+        _bind(dll, "hpaFusedReduce", i32, [V, i32, i32, C.POINTER(V), C.POINTER(Expr), C.POINTER(Expr)])
+        _bind(dll, "hpaFusedCompress", i32, [i32, C.POINTER(V), C.POINTER(V), i32, C.POINTER(V), C.POINTER(Expr)])
+        _bind(dll, "hpaCompressMany", i32, [i32, C.POINTER(V), V, C.POINTER(V)])
+        _bind(dll, "hpaGroupReduce", i32, [V, i32, V, V])
+        self.initialised = False
+        self._keep = []
+
+    def init(self, device=0, arena_bytes=0):
+        if not self.initialised:
+            rc = self._init(device, arena_bytes)
+            if rc != 0:
+                raise HorseError(rc, "hpaInit", self.detail())
+            self.initialised = True
+
+    def detail(self):
+        d = self.dll.hpaLastErrorDetail()
+        return d.decode() if d else ""
+
+    def call(self, name, *args):
+        rc = self.fn(name)(*args)
+        if rc != 0:
+            raise HorseError(rc, name, self.detail())
+        return rc
+
+    def rc(self, name, *args):
+        rc = getattr(self.dll, name)(*args)
+        if rc != 0:
+            raise HorseError(rc, name, self.detail())
+        return rc
+
+    # -- nodes -------------------------------------------------------------------------------
+    def node(self):
+        return self.dll.hpaAllocNode()
+
+    def mark(self):
+        return self.dll.hpaGetHeapOffset()
+
+    def release(self, mark):
+        self.dll.hpaSetHeapOffset(mark)
+
+    def symbol(self, name):
+        return self.dll.hpaSymbol(name.encode())
+
+    def sync(self):
+        self.rc("hpaSync")
+
+
+class Host:
+    """Host-side construction and reading of value nodes for a library binding (`lib`)."""
+
+    def __init__(self, lib, node_fn):
+        self.lib = lib
+        self._node = node_fn
+        self._keep = []
+
+    def node(self):
+        return self._node()
+
+    def vec(self, arr, typ=None):
+        """A node whose payload is the numpy array (kept alive by this object)."""
+        a = np.ascontiguousarray(arr)
+        if typ is None:
+            typ = TYP_OF[a.dtype]
+        a = a.astype(NP_OF[typ], copy=False)
+        z = self.node()
+        z.contents.typ = typ
+        z.contents.len = a.shape[0]
+        if a.shape[0] <= 1 and typ not in (H_G, H_N):
+            raw = np.zeros(8, dtype=np.uint8)
+            if a.shape[0] == 1:
+                raw[: a.itemsize] = a.view(np.uint8)
+            z.contents.u.i64 = int(raw.view(np.int64)[0])
+            z.contents.g = C.addressof(z.contents) + 24
+        else:
+            self._keep.append(a)
+            z.contents.g = a.ctypes.data
+        return z
+
+    def scalar(self, v, typ):
+        return self.vec(np.array([v], dtype=NP_OF[typ]), typ)
+
+    def sym(self, symid):
+        z = self.node()
+        z.contents.typ = H_Q
+        z.contents.len = 1
+        z.contents.u.i64 = symid
+        z.contents.g = C.addressof(z.contents) + 24
+        return z
+
+    def list(self, items):
+        z = self.node()
+        arr = (V * len(items))(*items)
+        self.lib.call("pfnList", z, len(items), arr)
+        return z
+
+
+def cell(z, k):
+    """k-th cell of a list/dict node (cells are host V0 arrays)."""
+    return C.cast(C.c_void_p(z.contents.g + 32 * k), V)

@@ -1,0 +1,244 @@
+/*
+ * horsepower_amd.h -- C-ABI of the MI355X (gfx950) HorseIR primitive execution layer.
+ *
+ * The library replaces HorsePower's primitive library (src/horseir/v3/backend/h_primitive.cc,
+ * h_libs.cc, h_hash_v3.cc, h_system.cc) behind the same calling convention:
+ *
+ *     I pfnX(V z, V x [, V y [, V m]])      -- 0 on success, an E_* code otherwise
+ *
+ * over value nodes whose layout is byte-identical to the reference's V0
+ * (src/horseir/v3/backend/common.h:31-55; 32 bytes, `typ` at offset 16).  A caller that was
+ * linked against the reference primitives relinks against libhorsepower_amd.so; its V0 headers,
+ * type codes and error codes are unchanged.
+ *
+ * Memory model.  Vector payloads (vg(x) when len > 1) produced by this library live in HBM of
+ * the current device, carved from a per-context bump arena (the reference's process heap,
+ * backend/h_memory.cc:93-112).  Inputs may be device payloads (produced here, or made resident
+ * with hpaUpload) or host payloads (staged to HBM on use).  Basic values with len <= 1 keep the
+ * reference's inline-scalar rule (vg(x) == &x->b, backend/h_memory.cc:182-196) and are readable
+ * on the host.  List/dict cells (the V0 arrays behind H_G/H_N nodes) stay in host memory, as in
+ * the reference.  Group results use the reference's flat-list form (typ H_G, vg = i64 group
+ * sizes, vg2 -> one i64 vector of row indices; backend/h_memory.cc:210-215).  hpaToHost copies a
+ * device payload back before host code dereferences it.
+ *
+ * Errors are returned, never turned into process exits (the reference prints and exits through
+ * printErrMsg/EP, backend/h_system.cc:1009-1028, global.h:65-66).
+ *
+ * Threading.  Each host thread issues work on its current context (a HIP stream plus arenas).
+ * The plain ABI uses the process default context, so single-threaded callers see the
+ * reference's one-primitive-at-a-time behaviour.  Concurrent requests (serving) each use their
+ * own context; calls that share one context from several threads race.
+ */
+#ifndef HORSEPOWER_AMD_H
+#define HORSEPOWER_AMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- value node: layout of V0 in src/horseir/v3/backend/common.h:31-55 ---------------- */
+typedef struct hpa_V0 {
+    int64_t len;          /* vn(x) */
+    char   *g;            /* vg(x): payload, or &b for inline scalars */
+    int32_t typ;          /* vp(x): H_* type code */
+    int32_t _pad;
+    union {               /* inline scalar storage / list flat pointer (vg2) */
+        bool     b;  int8_t j; int16_t h; int32_t i32; int64_t i64;
+        float    f32; double f64; char c; int64_t q; char *s;
+        int32_t  d;  int64_t z;
+        struct hpa_V0 *g2;
+        void    *p;
+    };
+} hpa_V0, *hpa_V;
+
+/* type codes: src/horseir/v3/backend/h_type.h:1-23 */
+#define HPA_B 0
+#define HPA_J 1
+#define HPA_H 2
+#define HPA_I 3
+#define HPA_L 4
+#define HPA_F 5
+#define HPA_E 6
+#define HPA_X 7
+#define HPA_C 8
+#define HPA_Q 9
+#define HPA_S 10
+#define HPA_M 11
+#define HPA_D 12
+#define HPA_Z 13
+#define HPA_U 14
+#define HPA_W 15
+#define HPA_T 16
+#define HPA_G 17
+#define HPA_N 18
+#define HPA_Y 19
+#define HPA_A 90
+#define HPA_K 91
+#define HPA_V 93
+
+/* error codes: src/horseir/v3/backend/h_system.h:128-142 (plus HPA_E_MEMORY / HPA_E_DEVICE,
+ * which replace the reference's "Heap full!!" exits and report HIP runtime failures) */
+#define HPA_E_DOMAIN          1
+#define HPA_E_GENERAL         2
+#define HPA_E_INDEX           3
+#define HPA_E_DIV_ZERO        4
+#define HPA_E_LENGTH          5
+#define HPA_E_TYPE            6
+#define HPA_E_NULL_VALUE      7
+#define HPA_E_LIKE_PATTERN    8
+#define HPA_E_MATCH           9
+#define HPA_E_ENUM_INDEX      10
+#define HPA_E_RAZE_LIST       11
+#define HPA_E_UNKNOWN         12
+#define HPA_E_TABLE_NOT_FOUND 50
+#define HPA_E_COL_NOT_FOUND   51
+#define HPA_E_NOT_IMPL        99
+#define HPA_E_MEMORY          100
+#define HPA_E_DEVICE          101
+
+typedef int32_t (*hpa_fn1)(hpa_V, hpa_V);
+typedef int32_t (*hpa_fn2)(hpa_V, hpa_V, hpa_V);
+
+/* ---- runtime ----------------------------------------------------------------------------- */
+typedef struct hpa_ctx hpa_ctx;
+
+/* Initialise on `device` with a default context whose device arena holds `arena_bytes`
+ * (0 = 16 GiB, the reference's INIT_HEAP_SIZE, backend/h_memory.cc:11).  Idempotent. */
+int32_t  hpaInit(int32_t device, int64_t arena_bytes);
+void     hpaShutdown(void);
+const char *hpaErrorString(int32_t code);
+const char *hpaLastErrorDetail(void);
+
+/* allocNode (backend/h_memory.cc:155-157): a zeroed host V0 from the current context. */
+hpa_V    hpaAllocNode(void);
+/* getHeapOffset / setHeapOffset (backend/h_memory.cc:147-153): arena watermark of the current
+ * context; setting it releases every payload and node allocated after the mark. */
+int64_t  hpaGetHeapOffset(void);
+int64_t  hpaSetHeapOffset(int64_t cur);
+/* initV (backend/h_memory.cc:182-196) with the payload in HBM. */
+int32_t  hpaInitV(hpa_V x, int32_t typ, int64_t len);
+
+/* Residency: copy a host payload into a long-lived HBM allocation (kept across heap resets)
+ * and repoint vg(x) at it.  Lists/tables are walked recursively. */
+int32_t  hpaUpload(hpa_V x);
+/* Release an hpaUpload allocation (the node is left pointing at nothing). */
+int32_t  hpaFreeResident(hpa_V x);
+/* Copy a device payload to a host buffer owned by the context and repoint vg(x) at it
+ * (recursively for lists and dicts; flat lists keep their form). */
+int32_t  hpaToHost(hpa_V x);
+/* Copy `bytes` of x's payload (device or host) into caller memory. */
+int32_t  hpaReadPayload(hpa_V x, void *dst, int64_t bytes);
+int32_t  hpaIsDevicePtr(const void *p);
+int32_t  hpaSync(void);
+
+/* Contexts: a stream plus device/host arenas.  The calling thread's current context is the
+ * target of every pfn* call. */
+hpa_ctx *hpaCtxCreate(int64_t arena_bytes);
+void     hpaCtxDestroy(hpa_ctx *c);
+hpa_ctx *hpaCtxDefault(void);
+void     hpaCtxSetCurrent(hpa_ctx *c);
+hpa_ctx *hpaCtxCurrent(void);
+void    *hpaCtxStream(hpa_ctx *c);          /* hipStream_t */
+int64_t  hpaCtxArenaBytes(hpa_ctx *c);
+int64_t  hpaCtxArenaPeak(hpa_ctx *c);
+void     hpaCtxResetPeak(hpa_ctx *c);
+
+/* Kernel timing: while enabled, launches named by the library are bracketed with HIP events
+ * on the launching stream; stats are (launch count, total ms). */
+void     hpaProfileEnable(int32_t on);
+void     hpaProfileReset(void);
+int32_t  hpaProfileStats(const char *kernel, int64_t *count, double *total_ms);
+int32_t  hpaProfileList(char *buf, int64_t cap);
+
+/* ---- primitives: src/horseir/v3/backend/h_primitive.h:72-211 --------------------------- */
+int32_t pfnList       (hpa_V z, int64_t n, hpa_V x[]);     /* h_primitive.cc:32-36 */
+int32_t pfnIndex      (hpa_V z, hpa_V x, hpa_V y);         /* h_primitive.cc:43-94 */
+int32_t pfnLen        (hpa_V z, hpa_V x);                  /* h_primitive.cc:656-662 */
+int32_t pfnWhere      (hpa_V z, hpa_V x);                  /* h_primitive.cc:728-762 */
+int32_t pfnSum        (hpa_V z, hpa_V x);                  /* h_primitive.cc:764-786 */
+int32_t pfnAvg        (hpa_V z, hpa_V x);                  /* h_primitive.cc:788-800 */
+int32_t pfnReduce     (hpa_V z, hpa_V x, int32_t op);      /* h_primitive.cc:808-840 */
+int32_t pfnMin        (hpa_V z, hpa_V x);                  /* h_primitive.cc:842-844 */
+int32_t pfnMax        (hpa_V z, hpa_V x);                  /* h_primitive.cc:846-848 */
+int32_t pfnRaze       (hpa_V z, hpa_V x);                  /* h_primitive.cc:935-955 */
+int32_t pfnGroup      (hpa_V z, hpa_V x);                  /* h_primitive.cc:993-999 */
+int32_t pfnGroupTrie  (hpa_V z, hpa_V x);                  /* h_primitive.cc:2656-2681 */
+int32_t pfnGroupBasic (hpa_V z, hpa_V x);                  /* h_primitive.cc:1002-1057 */
+int32_t pfnCompare    (hpa_V z, hpa_V x, hpa_V y, int32_t op); /* h_primitive.cc:1073-1169 */
+int32_t pfnLt         (hpa_V z, hpa_V x, hpa_V y);
+int32_t pfnLeq        (hpa_V z, hpa_V x, hpa_V y);
+int32_t pfnGt         (hpa_V z, hpa_V x, hpa_V y);
+int32_t pfnGeq        (hpa_V z, hpa_V x, hpa_V y);
+int32_t pfnEq         (hpa_V z, hpa_V x, hpa_V y);
+int32_t pfnNeq        (hpa_V z, hpa_V x, hpa_V y);
+int32_t pfnArith      (hpa_V z, hpa_V x, hpa_V y, int32_t op); /* h_primitive.cc:1229-1291 */
+int32_t pfnPlus       (hpa_V z, hpa_V x, hpa_V y);
+int32_t pfnMinus      (hpa_V z, hpa_V x, hpa_V y);
+int32_t pfnMul        (hpa_V z, hpa_V x, hpa_V y);
+int32_t pfnDiv        (hpa_V z, hpa_V x, hpa_V y);
+int32_t pfnAnd        (hpa_V z, hpa_V x, hpa_V y);         /* h_primitive.cc:1315-1357 */
+int32_t pfnNand       (hpa_V z, hpa_V x, hpa_V y);
+int32_t pfnOr         (hpa_V z, hpa_V x, hpa_V y);
+int32_t pfnNor        (hpa_V z, hpa_V x, hpa_V y);
+int32_t pfnXor        (hpa_V z, hpa_V x, hpa_V y);
+int32_t pfnCompress   (hpa_V z, hpa_V x, hpa_V y);         /* h_primitive.cc:1479-1521 */
+int32_t pfnIndexOf    (hpa_V z, hpa_V x, hpa_V y);         /* h_primitive.cc:1525-1579 */
+int32_t pfnOrderBy    (hpa_V z, hpa_V x, hpa_V y);         /* h_primitive.cc:1769-1816 */
+int32_t pfnEach       (hpa_V z, hpa_V x, hpa_fn1 foo);     /* h_primitive.cc:1818-1832 */
+int32_t pfnEachRight  (hpa_V z, hpa_V x, hpa_V y, hpa_fn2 foo); /* h_primitive.cc:1882-1904 */
+int32_t pfnJoinIndex  (hpa_V z, hpa_V x, hpa_V y, hpa_V f);/* h_primitive.cc:2122-2155 */
+int32_t pfnMember     (hpa_V z, hpa_V y, hpa_V x);         /* h_primitive.cc:2288-2336 */
+int32_t pfnKeys       (hpa_V z, hpa_V x);                  /* h_primitive.cc:190-203 */
+int32_t pfnValues     (hpa_V z, hpa_V x);                  /* h_primitive.cc:205-218 */
+
+/* Join operator names.  pfnJoinIndex's `f` is a symbol vector of operator names
+ * ("eq", "neq", ...; backend/h_system.cc:779-789).  Symbols are interned ids in the reference
+ * (backend/h_symbol.cc); this library interns the names it understands here. */
+int64_t hpaSymbol(const char *name);
+const char *hpaSymbolName(int64_t id);
+
+/* ---- fused loops emitted by the HorseIR optimizer ---------------------------------------
+ * An expression is postfix code over the input columns x[0..nx).  Each instruction yields an
+ * int64 or a float64 value, following C's usual arithmetic conversions as in the emitted
+ * loops (optimizer/elementwise.cc:245-276, src/horseir/v1/compile-static/fuse.h:13-26: float
+ * compares are exact, arithmetic is per-operation IEEE without contraction). */
+enum hpa_op {
+    HPA_OP_COL = 1,   /* push x[arg][i] */
+    HPA_OP_CONST_I,   /* push imm_i */
+    HPA_OP_CONST_F,   /* push imm_f */
+    HPA_OP_ADD, HPA_OP_SUB, HPA_OP_MUL, HPA_OP_DIV,
+    HPA_OP_LT, HPA_OP_LEQ, HPA_OP_GT, HPA_OP_GEQ, HPA_OP_EQ, HPA_OP_NEQ,
+    HPA_OP_AND, HPA_OP_OR, HPA_OP_NOT, HPA_OP_NEG
+};
+typedef struct hpa_ins { int32_t op; int32_t arg; int64_t imm_i; double imm_f; } hpa_ins;
+typedef struct hpa_expr { int32_t n; int32_t _pad; const hpa_ins *code; } hpa_expr;
+
+/* FE (optimizer/elementwise.cc:239-281): z[i] = e(x..)[i] as type `typ` (B, I, L or E). */
+int32_t hpaFusedMap(hpa_V z, int32_t typ, int32_t nx, hpa_V *x, const hpa_expr *e);
+/* FA reduction with filter (optimizer/autofusion.cc:795-817,886-917):
+ * c = reduce over {i : cond(i)} of body(i); kind 0 sum, 1 avg (c/count), 2 min, 3 max.
+ * cond may be NULL (all rows).  z is E for an E body, else L (sum) / same as body. */
+int32_t hpaFusedReduce(hpa_V z, int32_t kind, int32_t nx, hpa_V *x,
+                       const hpa_expr *cond, const hpa_expr *body);
+/* FA compress scan (optimizer/autofusion.cc:886-899) and the multi-column peephole
+ * (src/horseir/v1/compile-static/cur/q6.h:12-26): z[k] = compress(cond, y[k]) for k < ny,
+ * one mask evaluation, order preserving. */
+int32_t hpaFusedCompress(int32_t ny, hpa_V *z, hpa_V *y, int32_t nx, hpa_V *x,
+                         const hpa_expr *cond);
+/* Peephole compress with a ready mask: z[k] = compress(mask, y[k]). */
+int32_t hpaCompressMany(int32_t ny, hpa_V *z, hpa_V mask, hpa_V *y);
+/* FP1/FP2/FP5/FP6/FP7 grouped reductions (optimizer/pattern.cc:180-219): for each group list
+ * t of y (flat or normal list of i64 row indices), z[g] = reduce_{j in t}(x[j]);
+ * kind 0 sum, 1 avg, 2 min, 3 max, 4 len.  z is a vector over groups. */
+int32_t hpaGroupReduce(hpa_V z, int32_t kind, hpa_V x, hpa_V y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HORSEPOWER_AMD_H */

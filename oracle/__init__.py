@@ -1,0 +1,88 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes driver of the CPU oracle (oracle/horse_oracle.c).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this package, as the
+checker; the product never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+
+_dll = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+def load(threads=0, heap_bytes=1 << 32):
+    global _dll
+    if _dll is None:
+        if not os.path.exists(LIB):
+            build()
+        _dll = C.CDLL(LIB)
+        _dll.orc_init.argtypes = [C.c_int64, C.c_int32]
+        _dll.orc_init.restype = C.c_int32
+        if _dll.orc_init(heap_bytes, threads) != 0:
+            raise RuntimeError("oracle heap allocation failed")
+    elif threads:
+        _dll.orc_init(0, threads)
+    return _dll
+
+
+def backend(threads=0):
+    """A horsepower_amd.backend.Backend over the oracle's orc_pfn* functions."""
+    from horsepower_amd.backend import Backend
+    from horsepower_amd.vnode import V, PrimitiveLib
+
+    dll = load(threads)
+    dll.orc_alloc_node.restype = V
+    dll.orc_get_heap.restype = C.c_int64
+    dll.orc_set_heap.argtypes = [C.c_int64]
+    dll.orc_set_heap.restype = C.c_int64
+    dll.orc_symbol.argtypes = [C.c_char_p]
+    dll.orc_symbol.restype = C.c_int64
+    dll.orc_threads.restype = C.c_int32
+    dll.orc_sum_neumaier.argtypes = [C.c_void_p, C.c_int64]
+    dll.orc_sum_neumaier.restype = C.c_double
+    dll.orc_sum_neumaier_idx.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+    dll.orc_sum_neumaier_idx.restype = C.c_double
+    dll.orc_q6_fused.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32]
+    dll.orc_q6_fused.restype = C.c_double
+
+    class OracleBackend(Backend):
+        name = "oracle"
+
+        def node(self):
+            return dll.orc_alloc_node()
+
+        def read_raw(self, z, nbytes, dst):
+            C.memmove(dst, z.contents.g, nbytes)
+
+        def symbol(self, name):
+            return dll.orc_symbol(name.encode())
+
+        def mark(self):
+            return dll.orc_get_heap()
+
+        def release(self, m):
+            dll.orc_set_heap(m)
+
+        def sync(self):
+            pass
+
+        def threads(self):
+            return dll.orc_threads()
+
+    return OracleBackend(PrimitiveLib(dll, "orc_"))
+
+
+def neumaier(x):
+    import numpy as np
+    dll = load()
+    a = np.ascontiguousarray(x, dtype=np.float64)
+    return dll.orc_sum_neumaier(C.c_void_p(a.ctypes.data), a.shape[0])

@@ -1,0 +1,1075 @@
+/*
+ * horse_oracle.c -- TEST INFRASTRUCTURE ONLY (see horse_oracle.h).
+ *
+ * Clean-room C restatement of the HorsePower v3 primitive library.  Citations are to
+ * src/horseir/v3/backend/ unless another path is given.  Parallel parts use OpenMP with the
+ * reference's schedules: DOP = `omp parallel for simd` (common.h:93-96) and DOT = static
+ * contiguous per-thread ranges (common.h:97-104), so order-dependent outputs are identical for
+ * any thread count.  The single-threaded reference phases (join, radix group, group trie,
+ * sorts, index_of build/probe) are single-threaded here too.
+ *
+ * Deliberate, documented departures (reference behaviour undefined):
+ *  - pfnIndex gathers through the promoted i64 index (the reference reads y as i64 whatever its
+ *    type, h_primitive.cc:56; identical for the i64 indices every call site passes) and rejects
+ *    negative indices with E_INDEX (unchecked in the reference, h_primitive.cc:50).
+ *  - char descending order reverses [low,high) (the reference's swap indexes rtn[high-i+1],
+ *    one and two past the range, h_libs.cc:404-406).
+ *  - searchOrdered reads y with x's element type only when the types match; mismatched
+ *    integer widths are promoted first (the reference reinterprets memory, h_system.cc:731-744).
+ */
+#include "horse_oracle.h"
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <float.h>
+#include <limits.h>
+#include <omp.h>
+
+#define R return
+#define H_EPS 1e-13                   /* h_system.cc:9 */
+#define MINV(a,b) (((a)<(b))?(a):(b))  /* global.h:44 */
+#define MAXV(a,b) (((a)>(b))?(a):(b))  /* global.h:45 */
+
+typedef int64_t L;
+typedef double  E;
+
+static char   *g_heap = NULL;
+static L       g_cur = 0, g_top = 0;
+static int     g_threads = 1;
+
+/* ------------------------------------------------------------------ memory (h_memory.cc) */
+static L type_size(int32_t typ, L len){           /* getTypeSize, h_memory.cc:124-153 */
+    L r = -1, k = 0;
+    switch(typ){
+        case HPA_B: case HPA_J: r = 1; break;
+        case HPA_C: r = 1; k = 1; break;
+        case HPA_H: r = 2; break;
+        case HPA_I: case HPA_F: case HPA_M: case HPA_D: case HPA_U: case HPA_W: case HPA_T: r = 4; break;
+        case HPA_L: case HPA_E: case HPA_X: case HPA_Q: case HPA_S: case HPA_Z: case HPA_Y: r = 8; break;
+        case HPA_G: case HPA_N: case HPA_A: case HPA_K: case HPA_V: r = (L)sizeof(O0); break;
+        default: R -1;
+    }
+    R r * len + k;
+}
+
+static int is_basic(int32_t t){                    /* isTypeGroupBasic, h_system.cc:967 */
+    R (t >= HPA_B && t <= HPA_T);
+}
+static int is_int(int32_t t){ R t==HPA_B||t==HPA_J||t==HPA_H||t==HPA_I||t==HPA_L; }
+static int is_real(int32_t t){ R is_int(t) || t==HPA_F || t==HPA_E; }
+static int is_comparable(int32_t t){                /* h_system.cc:963 */
+    R is_real(t) || t==HPA_C || t==HPA_Q || (t>=HPA_M && t<=HPA_T);
+}
+
+int32_t orc_init(int64_t heap_bytes, int32_t threads){
+    if(!g_heap){
+        if(heap_bytes <= 0) heap_bytes = 1LL<<32;
+        g_heap = (char*)malloc((size_t)heap_bytes);
+        if(!g_heap) R HPA_E_MEMORY;
+        g_top = heap_bytes; g_cur = 0;
+    }
+    if(threads > 0) g_threads = threads;
+    else g_threads = omp_get_max_threads();
+    omp_set_num_threads(g_threads);
+    R 0;
+}
+int32_t orc_threads(void){ R g_threads; }
+int64_t orc_get_heap(void){ R g_cur; }
+int64_t orc_set_heap(int64_t cur){ R g_cur = cur; }
+
+static char *heap_alloc(L size){                   /* allocMem, h_memory.cc:93-112 */
+    if(size & 0xF) size = ((size>>4)+1)<<4;
+    if(g_cur + size >= g_top){ R NULL; }
+    char *g = g_heap + g_cur; g_cur += size; R g;
+}
+
+OV orc_alloc_node(void){
+    OV x = (OV)heap_alloc(sizeof(O0));
+    if(x) memset(x, 0, sizeof(O0));
+    R x;
+}
+
+OV orc_initV(OV x, int32_t typ, int64_t len){      /* initV, h_memory.cc:182-196 */
+    x->typ = typ; x->len = len;
+    if(is_basic(typ)) x->g = (len > 1) ? heap_alloc(type_size(typ,len)) : (char*)&x->b;
+    else x->g = heap_alloc(type_size(typ,len));
+    if(typ==HPA_G) x->g2 = NULL;
+    R x;
+}
+static void copyV(OV z, OV x){ *z = *x; }         /* h_system.cc:70-73 */
+#define vV(x,k) (((OV)(x)->g)+(k))
+
+/* ------------------------------------------------------------------ symbols (h_symbol.cc) */
+static const char *g_syms[64]; static int g_nsym = 0;
+int64_t orc_symbol(const char *name){
+    for(int i=0;i<g_nsym;i++) if(!strcmp(g_syms[i],name)) R i;
+    if(g_nsym >= 64) R -1;
+    g_syms[g_nsym] = strdup(name); R g_nsym++;
+}
+static int op_from_symbol(int64_t q){             /* getOpFromName, h_system.cc:779-789 */
+    if(q < 0 || q >= g_nsym) R -1;
+    const char *s = g_syms[q];
+    if(!strcmp(s,"lt")) R 0; if(!strcmp(s,"leq")) R 1; if(!strcmp(s,"gt")) R 2;
+    if(!strcmp(s,"geq")) R 3; if(!strcmp(s,"eq")) R 4; if(!strcmp(s,"neq")) R 5;
+    R -1;
+}
+
+/* ------------------------------------------------------------------ element access */
+#define sB(x) ((uint8_t*)(x)->g)
+#define sJ(x) ((int8_t*)(x)->g)
+#define sH(x) ((int16_t*)(x)->g)
+#define sI(x) ((int32_t*)(x)->g)
+#define sL(x) ((int64_t*)(x)->g)
+#define sF(x) ((float*)(x)->g)
+#define sE(x) ((double*)(x)->g)
+#define sC(x) ((char*)(x)->g)
+
+/* promoteValue, h_system.cc:168-252 (without index) */
+static int32_t promote(OV z, OV x, int32_t typMax){
+    if(x->typ == typMax){ copyV(z,x); R 0; }
+    L n = x->len;
+    orc_initV(z, typMax, n);
+    if(n > 1 && !z->g) R HPA_E_MEMORY;
+#define PROM(T, ctype) { ctype *d=(ctype*)z->g; switch(x->typ){ \
+        case HPA_B: for(L i=0;i<n;i++) d[i]=(ctype)sB(x)[i]; break; \
+        case HPA_J: for(L i=0;i<n;i++) d[i]=(ctype)sJ(x)[i]; break; \
+        case HPA_H: for(L i=0;i<n;i++) d[i]=(ctype)sH(x)[i]; break; \
+        case HPA_I: for(L i=0;i<n;i++) d[i]=(ctype)sI(x)[i]; break; \
+        case HPA_L: for(L i=0;i<n;i++) d[i]=(ctype)sL(x)[i]; break; \
+        case HPA_F: for(L i=0;i<n;i++) d[i]=(ctype)sF(x)[i]; break; \
+        default: R HPA_E_TYPE; } }
+    switch(typMax){
+        case HPA_J: PROM(J, int8_t); break;
+        case HPA_H: PROM(H, int16_t); break;
+        case HPA_I: PROM(I, int32_t); break;
+        case HPA_L: PROM(L, int64_t); break;
+        case HPA_F: PROM(F, float); break;
+        case HPA_E: PROM(E, double); break;
+        default: R HPA_E_TYPE;
+    }
+#undef PROM
+    R 0;
+}
+
+/* ------------------------------------------------------------------ compare (h_primitive.cc:1061-1200) */
+static inline int cmp_float(double x, double y){   /* compareFloat, h_primitive.cc:1061 */
+    R x<y ? (y-x<H_EPS?0:-1) : (x-y<H_EPS?0:1);
+}
+static inline int comp_op(int op, int c){          /* COMPFN on a three-way result */
+    switch(op){ case 0: R c<0; case 1: R c<=0; case 2: R c>0; case 3: R c>=0; case 4: R c==0; case 5: R c!=0; }
+    R 0;
+}
+#define COMP(op,a,b) (op==0?(a)<(b):op==1?(a)<=(b):op==2?(a)>(b):op==3?(a)>=(b):op==4?(a)==(b):op==5?(a)!=(b):0)
+
+int32_t orc_pfnCompare(OV z, OV x, OV y, int32_t op){
+    if(!(is_comparable(x->typ) && is_comparable(y->typ))) R HPA_E_DOMAIN;
+    if(!(x->len==1 || y->len==1 || x->len==y->len)) R HPA_E_LENGTH;
+    L n = (x->len==1) ? y->len : x->len;
+    L sx = (x->len==1)?0:1, sy = (y->len==1)?0:1;
+    if(is_real(x->typ) && is_real(y->typ)){
+        int32_t t = MAXV(x->typ, y->typ);
+        O0 tx, ty; int32_t e;
+        if((e=promote(&tx,x,t))) R e;
+        if((e=promote(&ty,y,t))) R e;
+        orc_initV(z, HPA_B, n); uint8_t *zz = sB(z);
+        switch(t){
+#define CMPI(ct) { ct *a=(ct*)tx.g, *b=(ct*)ty.g; _Pragma("omp parallel for simd") for(L i=0;i<n;i++) zz[i]=COMP(op,a[i*sx],b[i*sy]); } break
+            case HPA_B: CMPI(uint8_t);
+            case HPA_J: CMPI(int8_t);
+            case HPA_H: CMPI(int16_t);
+            case HPA_I: CMPI(int32_t);
+            case HPA_L: CMPI(int64_t);
+#undef CMPI
+            case HPA_F: { float *a=(float*)tx.g, *b=(float*)ty.g;
+                #pragma omp parallel for simd
+                for(L i=0;i<n;i++){ float p=a[i*sx], q=b[i*sy];
+                    int c = p<q ? ((double)(q-p)<H_EPS?0:-1) : ((double)(p-q)<H_EPS?0:1);
+                    zz[i]=comp_op(op,c);} } break;
+            case HPA_E: { double *a=(double*)tx.g, *b=(double*)ty.g;
+                #pragma omp parallel for simd
+                for(L i=0;i<n;i++) zz[i]=comp_op(op,cmp_float(a[i*sx],b[i*sy])); } break;
+        }
+        R 0;
+    }
+    else if(x->typ == y->typ){
+        orc_initV(z, HPA_B, n); uint8_t *zz = sB(z);
+        switch(x->typ){
+#define CMPS(ct) { ct *a=(ct*)x->g, *b=(ct*)y->g; for(L i=0;i<n;i++) zz[i]=COMP(op,a[i*sx],b[i*sy]); } break
+            case HPA_C: CMPS(char);
+            case HPA_M: case HPA_D: case HPA_U: case HPA_W: case HPA_T: CMPS(int32_t);
+            case HPA_Z: CMPS(int64_t);
+            case HPA_Q: if(op==4||op==5) CMPS(int64_t); R HPA_E_NOT_IMPL;
+#undef CMPS
+            default: R HPA_E_NOT_IMPL;
+        }
+        R 0;
+    }
+    R HPA_E_TYPE;
+}
+int32_t orc_pfnLt (OV z, OV x, OV y){ R orc_pfnCompare(z,x,y,0); }
+int32_t orc_pfnLeq(OV z, OV x, OV y){ R orc_pfnCompare(z,x,y,1); }
+int32_t orc_pfnGt (OV z, OV x, OV y){ R orc_pfnCompare(z,x,y,2); }
+int32_t orc_pfnGeq(OV z, OV x, OV y){ R orc_pfnCompare(z,x,y,3); }
+int32_t orc_pfnEq (OV z, OV x, OV y){ R orc_pfnCompare(z,x,y,4); }
+int32_t orc_pfnNeq(OV z, OV x, OV y){ R orc_pfnCompare(z,x,y,5); }
+
+/* ------------------------------------------------------------------ arith (h_primitive.cc:1227-1309) */
+static int check_zero(OV x){                       /* checkZero, h_system.cc:254-267 */
+    L n = x->len;
+    switch(x->typ){
+        case HPA_B: for(L i=0;i<n;i++) if(!sB(x)[i]) R 1; break;
+        case HPA_J: for(L i=0;i<n;i++) if(!sJ(x)[i]) R 1; break;
+        case HPA_H: for(L i=0;i<n;i++) if(!sH(x)[i]) R 1; break;
+        case HPA_I: for(L i=0;i<n;i++) if(!sI(x)[i]) R 1; break;
+        case HPA_L: for(L i=0;i<n;i++) if(!sL(x)[i]) R 1; break;
+        case HPA_F: for(L i=0;i<n;i++) if(sF(x)[i]==0) R 1; break;
+        case HPA_E: for(L i=0;i<n;i++) if(sE(x)[i]==0) R 1; break;
+    }
+    R 0;
+}
+#define ARITH2(op,a,b) (op==0?(a)+(b):op==1?(a)-(b):op==2?(a)*(b):(a)/(b))
+
+int32_t orc_pfnArith(OV z, OV x, OV y, int32_t op){
+    if(!(is_real(x->typ) && is_real(y->typ))) R HPA_E_DOMAIN;
+    if(!(x->len==1 || y->len==1 || x->len==y->len)) R HPA_E_LENGTH;
+    if(op==3 && check_zero(y)) R HPA_E_DIV_ZERO;
+    L n = (x->len==1) ? y->len : x->len;
+    L sx = (x->len==1)?0:1, sy = (y->len==1)?0:1;
+    int32_t t = MAXV(x->typ, y->typ);
+    O0 tx, ty; int32_t e;
+    if((e=promote(&tx,x,t))) R e;
+    if((e=promote(&ty,y,t))) R e;
+    orc_initV(z, t, n);
+    switch(t){
+        case HPA_B: { uint8_t *a=sB(&tx),*b=sB(&ty),*d=sB(z);
+            #pragma omp parallel for simd
+            for(L i=0;i<n;i++){ int v = ARITH2(op,(int)a[i*sx],(int)b[i*sy]); d[i]=(v!=0); } } break;
+#define AR(ct) { ct *a=(ct*)tx.g,*b=(ct*)ty.g,*d=(ct*)z->g; _Pragma("omp parallel for simd") for(L i=0;i<n;i++) d[i]=(ct)ARITH2(op,a[i*sx],b[i*sy]); } break
+        case HPA_J: AR(int8_t);
+        case HPA_H: AR(int16_t);
+        case HPA_I: AR(int32_t);
+        case HPA_L: AR(int64_t);
+        case HPA_F: AR(float);
+        case HPA_E: AR(double);
+#undef AR
+    }
+    R 0;
+}
+int32_t orc_pfnPlus (OV z, OV x, OV y){ R orc_pfnArith(z,x,y,0); }
+int32_t orc_pfnMinus(OV z, OV x, OV y){ R orc_pfnArith(z,x,y,1); }
+int32_t orc_pfnMul  (OV z, OV x, OV y){ R orc_pfnArith(z,x,y,2); }
+int32_t orc_pfnDiv  (OV z, OV x, OV y){ R orc_pfnArith(z,x,y,3); }
+
+/* ------------------------------------------------------------------ logic (h_primitive.cc:1315-1357, h_system.cc:898-916)
+ * B is C++ bool: and = x&y, or = x|y, xor = x^y; nand = ~(x&y) and nor = ~(x|y) are ints
+ * -1/-2 converted to bool, i.e. always true. */
+static int32_t logic(OV z, OV x, OV y, int which){
+    if(!(x->typ==HPA_B && y->typ==HPA_B)) R HPA_E_DOMAIN;
+    if(!(x->len==1 || y->len==1 || x->len==y->len)) R HPA_E_LENGTH;
+    L n = (x->len==1) ? y->len : x->len;
+    L sx = (x->len==1)?0:1, sy = (y->len==1)?0:1;
+    uint8_t *a=sB(x), *b=sB(y);
+    orc_initV(z, HPA_B, n); uint8_t *d = sB(z);
+    #pragma omp parallel for simd
+    for(L i=0;i<n;i++){
+        int p=a[i*sx], q=b[i*sy], r;
+        switch(which){ case 0: r=p&q; break; case 1: r=~(p&q); break; case 2: r=p|q; break;
+                       case 3: r=~(p|q); break; default: r=p^q; }
+        d[i] = (r!=0);
+    }
+    R 0;
+}
+int32_t orc_pfnAnd (OV z, OV x, OV y){ R logic(z,x,y,0); }
+int32_t orc_pfnNand(OV z, OV x, OV y){ R logic(z,x,y,1); }
+int32_t orc_pfnOr  (OV z, OV x, OV y){ R logic(z,x,y,2); }
+int32_t orc_pfnNor (OV z, OV x, OV y){ R logic(z,x,y,3); }
+int32_t orc_pfnXor (OV z, OV x, OV y){ R logic(z,x,y,4); }
+
+/* ------------------------------------------------------------------ compress / where */
+static void dot_counts(const uint8_t *m, L n, L *par, int T){   /* getNumOfNonZero, h_system.cc:304-310 */
+    L seg = n / T;
+    #pragma omp parallel num_threads(T)
+    {
+        int tid = omp_get_thread_num(); L c = 0;
+        L lo = tid*seg, hi = (tid != T-1) ? lo+seg : n;
+        for(L i=lo;i<hi;i++) c += m[i] ? 1 : 0;
+        par[tid] = c;
+    }
+}
+
+int32_t orc_pfnCompress(OV z, OV x, OV y){        /* h_primitive.cc:1479-1521 */
+    if(x->typ != HPA_B) R HPA_E_DOMAIN;
+    if(x->len != y->len) R HPA_E_LENGTH;
+    L n = x->len; int T = g_threads;
+    L *par = (L*)calloc(T, sizeof(L)), *off = (L*)calloc(T, sizeof(L));
+    dot_counts(sB(x), n, par, T);
+    L k = 0; for(int t=0;t<T;t++) k += par[t];
+    for(int t=1;t<T;t++) off[t] = off[t-1] + par[t-1];
+    int32_t typ = y->typ;
+    if(n != k){
+        L w;
+        switch(typ){
+            case HPA_B: case HPA_J: case HPA_C: w=1; break;
+            case HPA_H: w=2; break;
+            case HPA_I: case HPA_F: case HPA_D: w=4; break;
+            case HPA_L: case HPA_E: case HPA_Q: case HPA_S: case HPA_Y: w=8; break;
+            default: free(par); free(off); R HPA_E_NOT_IMPL;
+        }
+        orc_initV(z, typ, k);
+        char *dst = z->g; const char *src = y->g; const uint8_t *m = sB(x);
+        L seg = n / T;
+        #pragma omp parallel num_threads(T)
+        {
+            int tid = omp_get_thread_num(); L o = off[tid];
+            L lo = tid*seg, hi = (tid != T-1) ? lo+seg : n;
+            for(L i=lo;i<hi;i++) if(m[i]){ memcpy(dst + o*w, src + i*w, (size_t)w); o++; }
+        }
+    }
+    else if(k > 0) copyV(z, y);
+    else orc_initV(z, typ, 0);
+    free(par); free(off);
+    R 0;
+}
+
+int32_t orc_pfnWhere(OV z, OV x){                  /* h_primitive.cc:728-762 */
+    if(x->typ != HPA_B) R HPA_E_DOMAIN;
+    L n = x->len, k = 0; const uint8_t *m = sB(x);
+    for(L i=0;i<n;i++) k += m[i]?1:0;
+    orc_initV(z, HPA_L, k);
+    L c = 0; for(L i=0;i<n;i++) if(m[i]) sL(z)[c++] = i;
+    R 0;
+}
+
+/* ------------------------------------------------------------------ index (h_primitive.cc:43-94) */
+int32_t orc_pfnIndex(OV z, OV x, OV y){
+    if(!is_int(y->typ)) R HPA_E_DOMAIN;
+    L nz = y->len, nx = x->len;
+    O0 ty; int32_t e;
+    if((e=promote(&ty, y, HPA_L))) R e;
+    const int64_t *iy = sL(&ty);
+    for(L i=0;i<nz;i++) if(iy[i] >= nx || iy[i] < 0) R HPA_E_INDEX;
+    if(is_basic(x->typ)){
+        L w = type_size(x->typ, 1); if(x->typ==HPA_C) w = 1;
+        orc_initV(z, x->typ, nz);
+        char *d = z->g; const char *s = x->g;
+        #pragma omp parallel for
+        for(L i=0;i<nz;i++) memcpy(d + i*w, s + iy[i]*w, (size_t)w);
+        R 0;
+    }
+    else if(x->typ == HPA_G){
+        if(nz == 1){ copyV(z, vV(x, iy[0])); R 0; }
+        R HPA_E_NOT_IMPL;
+    }
+    R HPA_E_NOT_IMPL;
+}
+
+int32_t orc_pfnList(OV z, int64_t n, OV x[]){      /* h_primitive.cc:32-36 */
+    orc_initV(z, HPA_G, n); z->g2 = NULL;
+    for(L i=0;i<n;i++) *vV(z,i) = *x[i];
+    R 0;
+}
+
+int32_t orc_pfnLen(OV z, OV x){                    /* h_primitive.cc:656-662 */
+    orc_initV(z, HPA_L, 1);
+    z->i64 = x->len;
+    R 0;
+}
+
+/* ------------------------------------------------------------------ reductions (h_primitive.cc:764-848) */
+int32_t orc_pfnSum(OV z, OV x){
+    if(!is_real(x->typ)) R HPA_E_DOMAIN;
+    int32_t tz = x->typ==HPA_F?HPA_F:x->typ==HPA_E?HPA_E:HPA_L;
+    L n = x->len;
+    orc_initV(z, tz, 1);
+    switch(x->typ){
+#define SUMI(ct) { L t=0; ct *a=(ct*)x->g; _Pragma("omp parallel for simd reduction(+:t)") for(L i=0;i<n;i++) t+=a[i]; z->i64=t; } break
+        case HPA_B: SUMI(uint8_t);
+        case HPA_J: SUMI(int8_t);
+        case HPA_H: SUMI(int16_t);
+        case HPA_I: SUMI(int32_t);
+        case HPA_L: SUMI(int64_t);
+#undef SUMI
+        case HPA_F: { float t=0; float *a=sF(x);
+            #pragma omp parallel for simd reduction(+:t)
+            for(L i=0;i<n;i++) t+=a[i];
+            z->f32=t; } break;
+        case HPA_E: { double t=0; double *a=sE(x);
+            #pragma omp parallel for simd reduction(+:t)
+            for(L i=0;i<n;i++) t+=a[i];
+            z->f64=t; } break;
+    }
+    R 0;
+}
+
+int32_t orc_pfnAvg(OV z, OV x){
+    O0 t; int32_t e;
+    if((e=orc_pfnSum(&t, x))) R e;
+    int32_t tz = t.typ==HPA_F ? HPA_F : HPA_E;
+    orc_initV(z, tz, 1);
+    switch(t.typ){
+        case HPA_L: z->f64 = t.i64*1.0/x->len; break;
+        case HPA_F: z->f32 = t.f32/x->len; break;
+        case HPA_E: z->f64 = t.f64/x->len; break;
+    }
+    R 0;
+}
+
+int32_t orc_pfnReduce(OV z, OV x, int32_t op){
+    if(!is_real(x->typ)) R HPA_E_DOMAIN;
+    L n = x->len;
+    if(n < 1){
+        int32_t tz = is_int(x->typ) ? HPA_L : x->typ;
+        orc_initV(z, tz, 1);
+        switch(tz){
+            case HPA_L: z->i64 = op==0 ? LLONG_MAX : LLONG_MIN; break;
+            case HPA_F: z->f32 = op==0 ? FLT_MAX : FLT_MIN; break;
+            case HPA_E: z->f64 = op==0 ? DBL_MAX : DBL_MIN; break;
+        }
+        R 0;
+    }
+    orc_initV(z, x->typ, 1);
+    switch(x->typ){
+#define RED(ct, fld) { ct *a=(ct*)x->g; ct t=a[0]; for(L i=0;i<n;i++) t = op==0?MINV(t,a[i]):MAXV(t,a[i]); z->fld=t; } break
+        case HPA_B: { uint8_t *a=sB(x); uint8_t t=a[0]; for(L i=0;i<n;i++) t = op==0?MINV(t,a[i]):MAXV(t,a[i]); z->b=t; } break;
+        case HPA_J: RED(int8_t, j);
+        case HPA_H: RED(int16_t, h);
+        case HPA_I: RED(int32_t, i32);
+        case HPA_L: RED(int64_t, i64);
+        case HPA_F: RED(float, f32);
+        case HPA_E: RED(double, f64);
+#undef RED
+    }
+    R 0;
+}
+int32_t orc_pfnMin(OV z, OV x){ R orc_pfnReduce(z,x,0); }
+int32_t orc_pfnMax(OV z, OV x){ R orc_pfnReduce(z,x,1); }
+
+double orc_sum_neumaier(const double *x, int64_t n){
+    double s = 0, c = 0;
+    for(L i=0;i<n;i++){ double v=x[i], t=s+v; c += (fabs(s)>=fabs(v)) ? (s-t)+v : (v-t)+s; s=t; }
+    R s + c;
+}
+double orc_sum_neumaier_idx(const double *x, const int64_t *idx, int64_t n){
+    double s = 0, c = 0;
+    for(L i=0;i<n;i++){ double v=x[idx[i]], t=s+v; c += (fabs(s)>=fabs(v)) ? (s-t)+v : (v-t)+s; s=t; }
+    R s + c;
+}
+
+/* ------------------------------------------------------------------ sorting (h_libs.cc) */
+typedef struct { int32_t x; L i; } PosI;   /* Pos_i, h_libs.cc:1182 */
+typedef struct { L x; int32_t i; } PosL;   /* Pos,   h_libs.cc:1181 */
+
+static void radix_core_int(PosI *v, L len){        /* lib_radixsort_core_int, h_libs.cc:1184-1209 */
+    L *cnt = (L*)malloc(sizeof(L)*256); PosI *tmp = (PosI*)malloc(sizeof(PosI)*(len>0?len:1));
+    for(int j=0;j<32;j+=8){
+        memset(cnt, 0, sizeof(L)*256);
+        for(L i=0;i<len;i++) cnt[(v[i].x>>j)&255]++;
+        for(int i=1;i<256;i++) cnt[i]+=cnt[i-1];
+        for(L i=len-1;i>=0;i--){ L k=(v[i].x>>j)&255; tmp[--cnt[k]] = v[i]; }
+        memcpy(v, tmp, sizeof(PosI)*len);
+    }
+    free(cnt); free(tmp);
+}
+
+static void radix_core_long(PosL *v, L len){       /* lib_radixsort_core_long, h_libs.cc:1240-1266 */
+    int32_t *cnt = (int32_t*)malloc(sizeof(int32_t)*256); PosL *tmp = (PosL*)malloc(sizeof(PosL)*(len>0?len:1));
+    int maxSize = 32;
+    for(L i=0;i<len;i++) if((v[i].x>>32) > 0){ maxSize = 64; break; }
+    for(int j=0;j<maxSize;j+=8){
+        memset(cnt, 0, sizeof(int32_t)*256);
+        for(L i=0;i<len;i++) cnt[(v[i].x>>j)&255]++;
+        for(int i=1;i<256;i++) cnt[i]+=cnt[i-1];
+        for(L i=len-1;i>=0;i--){ L k=(v[i].x>>j)&255; tmp[--cnt[k]] = v[i]; }
+        memcpy(v, tmp, sizeof(PosL)*len);
+    }
+    free(cnt); free(tmp);
+}
+
+static void radixsort_int(L *rtn, OV val, L len, int up){   /* lib_radixsort_int, h_libs.cc:1215-1238 */
+    PosI *p = (PosI*)malloc(sizeof(PosI)*(len>0?len:1));
+    for(L i=0;i<len;i++){ p[i].x = sI(val)[i]; p[i].i = i; }
+    radix_core_int(p, len);
+    for(L i=0;i<len;i++) rtn[i] = up ? p[i].i : p[len-i-1].i;
+    free(p);
+}
+static void radixsort_long_keys(L *rtn, const int64_t *keys, L len, int up){ /* lib_radixsort_long, h_libs.cc:1268-1283 */
+    PosL *p = (PosL*)malloc(sizeof(PosL)*(len>0?len:1));
+    for(L i=0;i<len;i++){ p[i].x = keys[i]; p[i].i = (int32_t)i; }
+    radix_core_long(p, len);
+    for(L i=0;i<len;i++) rtn[i] = up ? p[i].i : p[len-i-1].i;
+    free(p);
+}
+
+/* lib_quicksort_cmp_item, h_libs.cc:451-482 (SORT_MERGE off: ties compare 0) */
+static L cmp_item(OV t, L a, L b, const uint8_t *isUp){
+    int f = isUp ? *isUp : 1;
+#define SC(ct) { ct p=((ct*)t->g)[a], q=((ct*)t->g)[b]; if(p!=q) R p<q ? (f?-1:1) : (f?1:-1); } break
+    switch(t->typ){
+        case HPA_B: SC(uint8_t); case HPA_J: SC(int8_t); case HPA_H: SC(int16_t);
+        case HPA_I: case HPA_M: case HPA_D: case HPA_U: case HPA_W: case HPA_T: SC(int32_t);
+        case HPA_L: case HPA_Z: case HPA_Q: SC(int64_t);
+        case HPA_F: SC(float); case HPA_E: SC(double); case HPA_C: SC(char);
+        default: break;
+    }
+#undef SC
+    R 0;
+}
+/* lib_quicksort_cmp, h_libs.cc:445-449 */
+static L cmp_list(OV val, L a, L b, const uint8_t *isUp){
+    for(L i=0;i<val->len;i++){ uint8_t f = isUp ? isUp[i] : 1; L x = cmp_item(vV(val,i), a, b, &f); if(x) R x; }
+    R 0;
+}
+
+/* lib_partition / lib_quicksort_other, h_libs.cc:409-442 */
+static L partition(L *rtn, OV val, L low, L high, const uint8_t *isUp, int *leftSame){
+    L pivot = low, i = low, j = high, t;
+    while(1){
+        while(++i && i<high){
+            L c = cmp_item(val, rtn[i], rtn[pivot], isUp);
+            if(c != 0) *leftSame = 0;
+            if(c > 0) break;
+        }
+        do{ --j; } while(j>=0 && cmp_item(val, rtn[j], rtn[pivot], isUp) > 0);
+        if(i>=j) break;
+        t = rtn[i]; rtn[i] = rtn[j]; rtn[j] = t;
+    }
+    if(!*leftSame){ t = rtn[low]; rtn[low] = rtn[j]; rtn[j] = t; }
+    R j;
+}
+static void quicksort_other(L *rtn, OV val, L low, L high, const uint8_t *isUp){
+    if(low < high){
+        int leftSame = 1;
+        L pos = partition(rtn, val, low, high, isUp, &leftSame);
+        if(!leftSame) quicksort_other(rtn, val, low, pos, isUp);
+        if(pos < high) quicksort_other(rtn, val, pos+1, high, isUp);
+    }
+}
+/* lib_quicksort_char, h_libs.cc:386-407: <10 items use the all-pairs exchange with `>` (which
+ * leaves the range in non-increasing order), otherwise a stable counting sort */
+static void quicksort_char(L *rtn, OV val, L low, L high, const uint8_t *isUp){
+    const char *str = sC(val); L len = high-low; int f0 = isUp ? *isUp : 1;
+    if(len < 10){
+        for(L i=low;i<high;i++) for(L j=low;j<high;j++)
+            if(str[rtn[i]] > str[rtn[j]]){ L t=rtn[i]; rtn[i]=rtn[j]; rtn[j]=t; }
+    } else {
+        L cnt[256]={0}, step[256]={0}, cursor=0;
+        L *temp = (L*)malloc(sizeof(L)*len);
+        for(L i=low;i<high;i++) cnt[(unsigned char)str[rtn[i]]]++;
+        for(int i=0;i<256;i++){ if(cnt[i]!=0) step[i]=cursor; cursor+=cnt[i]; }
+        for(L i=low;i<high;i++) temp[step[(unsigned char)str[rtn[i]]]++] = rtn[i];
+        memcpy(rtn+low, temp, sizeof(L)*len); free(temp);
+    }
+    if(!f0){ for(L i=low, k=high-1; i<k; i++, k--){ L t=rtn[i]; rtn[i]=rtn[k]; rtn[k]=t; } }
+}
+static void lib_quicksort(L *rtn, OV val, L low, L high, const uint8_t *isUp){ /* h_libs.cc:375-384 */
+    switch(val->typ){
+        case HPA_C: quicksort_char(rtn, val, low, high, isUp); break;
+        case HPA_I: radixsort_int(rtn, val, high, isUp ? *isUp : 1); break;
+        case HPA_L: radixsort_long_keys(rtn, sL(val), high, isUp ? *isUp : 1); break;
+        default: quicksort_other(rtn, val, low, high, isUp); break;
+    }
+}
+
+/* lib_mergesort, h_libs.cc:490-536 */
+static L *g_buff = NULL;
+static void mergesort_r(L *rtn, OV val, L low, L high, const uint8_t *isUp){
+    if(low < high - 1){
+        if(low + 1 == high - 1){
+            if(cmp_item(val, rtn[low], rtn[high-1], isUp) > 0){ L t=rtn[low]; rtn[low]=rtn[high-1]; rtn[high-1]=t; }
+        } else {
+            L mid = (low + high) >> 1, len = high-low;
+            mergesort_r(rtn, val, low, mid, isUp);
+            mergesort_r(rtn, val, mid, high, isUp);
+            if(cmp_item(val, rtn[mid-1], rtn[mid], isUp) > 0){
+                L cl = low, cr = mid;
+                for(L i=0;i<len;i++){
+                    if(cl == mid) g_buff[i] = rtn[cr++];
+                    else if(cr == high) g_buff[i] = rtn[cl++];
+                    else if(cmp_item(val, rtn[cl], rtn[cr], isUp) > 0) g_buff[i] = rtn[cr++];
+                    else g_buff[i] = rtn[cl++];
+                }
+                memcpy(rtn+low, g_buff, sizeof(L)*len);
+            }
+        }
+    }
+}
+
+/* lib_order_by_sorted, h_libs.cc:604-617 */
+static int order_by_sorted(L *targ, OV val, const uint8_t *isUp, L low, L high){
+    L rtn = 0;
+    for(L i=low+1;i<high;i++){
+        L t = cmp_item(val, targ[i-1], targ[i], isUp);
+        rtn = rtn==0 ? t : rtn==-1 ? (t==1?-2:rtn) : (t==-1?-2:rtn);
+        if(rtn == -2) R 0;
+    }
+    int isOrder = rtn == (*isUp ? -1 : 1);
+    if(isOrder && !(*isUp)){
+        for(L i=low;i<(low+high)/2;i++){ L pos = high-(i-low)-1; L t=targ[i]; targ[i]=targ[pos]; targ[pos]=t; }
+    }
+    R isOrder;
+}
+
+static int is_integers(OV v){ R v->typ==HPA_I || v->typ==HPA_L; }
+
+/* lib_quicksort_list, h_libs.cc:637-697 */
+static void quicksort_list(L *targ, OV val, const uint8_t *isUp, L low, L high, L colId){
+    if(colId >= val->len) R;
+    OV cur = vV(val, colId); const uint8_t *cb = isUp + colId;
+    if(order_by_sorted(targ, cur, cb, low, high)) { ; }
+    else if(is_integers(cur)){
+        g_buff = (L*)malloc(sizeof(L)*(cur->len>0?cur->len:1));
+        mergesort_r(targ, cur, low, high, cb);
+        free(g_buff); g_buff = NULL;
+    }
+    else lib_quicksort(targ, cur, low, high, cb);
+    L pos = low;
+    do{
+        L start = pos, end = high;
+        L hend = start + 5, hpos = -1;
+        if(hend > high) hend = high;
+        for(L i=start+1;i<hend;i++) if(cmp_item(cur, targ[i-1], targ[i], cb) != 0){ hpos = i; break; }
+        if(hpos >= 0){ quicksort_list(targ, val, isUp, start, hpos, colId+1); pos = hpos; }
+        else if(hend == high){ quicksort_list(targ, val, isUp, start, high, colId+1); pos = high; }
+        else {
+            L pivot = start;
+            do{
+                if(0 != cmp_item(cur, targ[pivot], targ[end-1], cb)){
+                    L mid = (start + end) >> 1;
+                    if(0 != cmp_item(cur, targ[pivot], targ[mid], cb)) end = mid;
+                    else start = mid+1;
+                } else break;
+            } while(start < end);
+            quicksort_list(targ, val, isUp, pivot, end, colId+1);
+            pos = end;
+        }
+    } while(pos < high);
+}
+
+static void order_by_list(L *targ, OV val, const uint8_t *isUp, L n){ /* h_libs.cc:619-635 */
+    for(L i=0;i<n;i++) targ[i] = i;
+    quicksort_list(targ, val, isUp, 0, n, 0);
+}
+static void order_by_vector(L *targ, OV val, const uint8_t *isUp, L n){ /* h_libs.cc:730-736 */
+    if(!is_integers(val)) for(L i=0;i<n;i++) targ[i] = i;
+    lib_quicksort(targ, val, 0, n, isUp);
+}
+
+/* checkOrderCase1, h_libs.cc:1912-1940: both columns i32 and equal flags.  The packed key is
+ * ((0|x0)<<32)|x1 with x0, x1 sign-extended, so a negative x1 sets all high bits. */
+static int check_order_case1(OV z, OV x, OV f){
+    if(x->len==2 && vV(x,0)->typ==HPA_I && vV(x,1)->typ==HPA_I && sB(f)[0]==sB(f)[1]){
+        OV x0 = vV(x,0), x1 = vV(x,1);
+        L n = x0->len;
+        int64_t *k = (int64_t*)malloc(sizeof(int64_t)*(n>0?n:1));
+        for(L i=0;i<n;i++) k[i] = (int64_t)((uint64_t)(int64_t)sI(x0)[i] << 32) | (int64_t)sI(x1)[i];
+        orc_initV(z, HPA_L, n);
+        radixsort_long_keys(sL(z), k, n, sB(f)[0]);
+        free(k);
+        R 1;
+    }
+    R 0;
+}
+
+static int32_t order_basic(OV z, OV x, OV y){      /* lib_order_basic, h_primitive.cc:1760-1767 */
+    if(is_basic(x->typ) && y->typ==HPA_B && y->len==1){
+        if(x->typ==HPA_S || x->typ==HPA_X) R HPA_E_NOT_IMPL;
+        orc_initV(z, HPA_L, x->len);
+        order_by_vector(sL(z), x, sB(y), x->len);
+        R 0;
+    }
+    R 1;
+}
+
+int32_t orc_pfnOrderBy(OV z, OV x, OV y){          /* h_primitive.cc:1769-1816 */
+    if(x->typ==HPA_G && y->typ==HPA_B && x->len==y->len){
+        for(L i=0;i<x->len;i++) if(!is_basic(vV(x,i)->typ)) R HPA_E_DOMAIN;
+        if(x->len > 0){ L n=vV(x,0)->len; for(L i=0;i<x->len;i++) if(vV(x,i)->len!=n) R HPA_E_MATCH; }
+        if(check_order_case1(z, x, y)) { ; }
+        else if(x->len == 1) R order_basic(z, vV(x,0), y);
+        else {
+            L n = x->len==0 ? 0 : vV(x,0)->len;
+            orc_initV(z, HPA_L, n);
+            order_by_list(sL(z), x, sB(y), n);
+        }
+        R 0;
+    }
+    R order_basic(z, x, y);
+}
+
+/* ------------------------------------------------------------------ group */
+static void init_dict(OV z, L g){
+    orc_initV(z, HPA_N, 2);
+    orc_initV(vV(z,0), HPA_L, g);
+    orc_initV(vV(z,1), HPA_G, g);
+}
+
+/* lib_group_by_normal_int, h_libs.cc:1295-1329 */
+static int32_t group_int(OV z, OV x){
+    L n = x->len;
+    PosI *p = (PosI*)malloc(sizeof(PosI)*(n>0?n:1));
+    for(L i=0;i<n;i++){ p[i].x = sI(x)[i]; p[i].i = i; }
+    radix_core_int(p, n);
+    L g = n>0 ? 1 : 0;
+    for(L i=1;i<n;i++) if(p[i-1].x != p[i].x) g++;
+    init_dict(z, g);
+    OV keys = vV(z,0), vals = vV(z,1);
+    L k = 0, s = 0;
+    for(L i=1;i<=n;i++){
+        if(i==n || p[i-1].x != p[i].x){
+            OV d = vV(vals,k); orc_initV(d, HPA_L, i-s);
+            for(L j=s;j<i;j++) sL(d)[j-s] = p[j].i;
+            sL(keys)[k] = p[s].i; k++; s = i;
+        }
+    }
+    free(p); R 0;
+}
+/* lib_group_by_normal_long, h_libs.cc:1369-1403 */
+static int32_t group_long(OV z, OV x){
+    L n = x->len;
+    PosL *p = (PosL*)malloc(sizeof(PosL)*(n>0?n:1));
+    for(L i=0;i<n;i++){ p[i].x = sL(x)[i]; p[i].i = (int32_t)i; }
+    radix_core_long(p, n);
+    L g = n>0 ? 1 : 0;
+    for(L i=1;i<n;i++) if(p[i-1].x != p[i].x) g++;
+    init_dict(z, g);
+    OV keys = vV(z,0), vals = vV(z,1);
+    L k = 0, s = 0;
+    for(L i=1;i<=n;i++){
+        if(i==n || p[i-1].x != p[i].x){
+            OV d = vV(vals,k); orc_initV(d, HPA_L, i-s);
+            for(L j=s;j<i;j++) sL(d)[j-s] = p[j].i;
+            sL(keys)[k] = p[s].i; k++; s = i;
+        }
+    }
+    free(p); R 0;
+}
+/* lib_get_group_by_other / _order, h_libs.cc:776-846: runs of equal keys in `index` order
+ * (index NULL = identity) */
+static int32_t group_runs(OV z, OV val, const L *index, L n, int isList){
+    L g = n>0 ? 1 : 0;
+#define IX(i) (index ? index[i] : (i))
+#define CMPR(a,b) (isList ? cmp_list(val,(a),(b),NULL) : cmp_item(val,(a),(b),NULL))
+    for(L i=1;i<n;i++) if(CMPR(IX(i-1), IX(i)) != 0) g++;
+    init_dict(z, g);
+    OV keys = vV(z,0), vals = vV(z,1);
+    L k = 0, s = 0;
+    for(L i=1;i<=n;i++){
+        if(i==n || CMPR(IX(i-1), IX(i)) != 0){
+            OV d = vV(vals,k); orc_initV(d, HPA_L, i-s);
+            for(L j=s;j<i;j++) sL(d)[j-s] = IX(j);
+            sL(keys)[k] = IX(s); k++; s = i;
+        }
+    }
+#undef IX
+#undef CMPR
+    R 0;
+}
+
+/* isOrdered, h_system.cc:678-698 */
+static int is_ordered(OV x){
+    L n = x->len;
+    if(x->typ == HPA_I || x->typ == HPA_L){
+        int order = -2;
+#define GETV(i) (x->typ==HPA_I ? (int64_t)sI(x)[i] : sL(x)[i])
+        for(L i=1;i<n;i++){ int64_t a=GETV(i), b=GETV(i-1); int v = a>b?1:a<b?-1:0; if(order==-2 && v!=0){ order=v; break; } }
+        if(order == -2) R 1;
+        for(L i=1;i<n;i++){ int64_t a=GETV(i), b=GETV(i-1); int v = a>b?1:a<b?-1:0; if(v!=0 && order!=v) R 0; }
+#undef GETV
+        R 1;
+    }
+    R 0;
+}
+
+int32_t orc_pfnGroupTrie(OV z, OV x){              /* h_primitive.cc:2656-2681 */
+    OV c0 = vV(x,0), c1 = vV(x,1);
+    L n = c0->len;
+    L *cnt = (L*)calloc(65536, sizeof(L)), *id = (L*)malloc(sizeof(L)*65536);
+#define CODE(i) ((((int)(unsigned char)sC(c0)[i])<<8) + (int)(unsigned char)sC(c1)[i])
+    for(L i=0;i<n;i++) cnt[CODE(i)]++;
+    L g = 0; for(int i=0;i<65536;i++) if(cnt[i]) id[i] = g++;
+    init_dict(z, g);
+    OV keys = vV(z,0), vals = vV(z,1);
+    L c = 0; for(int i=0;i<65536;i++) if(cnt[i]) orc_initV(vV(vals,c++), HPA_L, cnt[i]);
+    L *fill = (L*)calloc(g>0?g:1, sizeof(L));
+    for(L i=0;i<n;i++){ L k = id[CODE(i)]; sL(vV(vals,k))[fill[k]++] = i; }
+    for(L k=0;k<g;k++) sL(keys)[k] = sL(vV(vals,k))[0];
+#undef CODE
+    free(cnt); free(id); free(fill);
+    R 0;
+}
+
+int32_t orc_pfnGroupBasic(OV z, OV x){             /* h_primitive.cc:1002-1057 */
+    L *order = NULL; O0 t;
+    if(is_ordered(x)) order = NULL;
+    else if(is_integers(x)) { ; }
+    else {
+        L lenZ = x->typ==HPA_G ? x->len : 1;
+        O0 y; orc_initV(&y, HPA_B, lenZ);
+        for(L i=0;i<lenZ;i++) sB(&y)[i] = 1;
+        int32_t e = orc_pfnOrderBy(&t, x, &y); if(e) R e;
+        order = sL(&t);
+    }
+    if(x->typ == HPA_G){
+        L n = x->len==0 ? 0 : vV(x,0)->len;
+        R group_runs(z, x, order, n, 1);
+    }
+    else if(x->typ == HPA_I) R group_int(z, x);
+    else if(x->typ == HPA_L) R group_long(z, x);
+    else if(is_basic(x->typ)) R group_runs(z, x, order, x->len, 0);
+    R HPA_E_DOMAIN;
+}
+
+int32_t orc_pfnGroup(OV z, OV x){                  /* h_primitive.cc:993-999 */
+    if(x->typ==HPA_G && x->len==2 && vV(x,0)->typ==HPA_C && vV(x,1)->typ==HPA_C && vV(x,0)->len==vV(x,1)->len)
+        R orc_pfnGroupTrie(z, x);
+    R orc_pfnGroupBasic(z, x);
+}
+
+int32_t orc_pfnKeys(OV z, OV x){                   /* h_primitive.cc:190-203 */
+    if(x->typ==HPA_N || x->typ==HPA_A){ copyV(z, vV(x,0)); R 0; }
+    R HPA_E_DOMAIN;
+}
+int32_t orc_pfnValues(OV z, OV x){                 /* h_primitive.cc:205-218 */
+    if(x->typ==HPA_N || x->typ==HPA_A){ copyV(z, vV(x,1)); R 0; }
+    R HPA_E_DOMAIN;
+}
+
+/* ------------------------------------------------------------------ each / raze */
+int32_t orc_pfnEach(OV z, OV x, int32_t (*foo)(OV, OV)){      /* h_primitive.cc:1818-1832 */
+    if(x->typ != HPA_G) R HPA_E_DOMAIN;
+    orc_initV(z, HPA_G, x->len);
+    for(L i=0;i<x->len;i++){ int32_t e = foo(vV(z,i), vV(x,i)); if(e) R e; }
+    R 0;
+}
+int32_t orc_pfnEachRight(OV z, OV x, OV y, int32_t (*foo)(OV, OV, OV)){ /* h_primitive.cc:1882-1904 */
+    if(y->typ == HPA_G){
+        orc_initV(z, HPA_G, y->len);
+        for(L i=0;i<y->len;i++){ int32_t e = foo(vV(z,i), x, vV(y,i)); if(e) R e; }
+        R 0;
+    }
+    R foo(z, x, y);
+}
+int32_t orc_pfnRaze(OV z, OV x){                   /* h_primitive.cc:935-955, fillRaze */
+    if(x->typ == HPA_G){
+        if(x->len == 0) R HPA_E_RAZE_LIST;
+        int32_t t = vV(x,0)->typ; L n = 0;
+        for(L i=0;i<x->len;i++){ if(vV(x,i)->typ != t) R HPA_E_TYPE; n += vV(x,i)->len; }
+        L w = type_size(t,1); if(t==HPA_C) w = 1;
+        orc_initV(z, t, n);
+        char *d = z->g; L c = 0;
+        for(L i=0;i<x->len;i++){ OV v = vV(x,i); memcpy(d + c*w, v->g, (size_t)(w*v->len)); c += v->len; }
+        R 0;
+    }
+    if(is_basic(x->typ)){ copyV(z, x); R 0; }
+    R HPA_E_DOMAIN;
+}
+
+/* ------------------------------------------------------------------ join (h_primitive.cc:1944-2155, h_hash_v3.cc:230-260) */
+static int cmp_keys_i32(const void *a, const void *b){
+    const int64_t *p = (const int64_t*)a, *q = (const int64_t*)b;
+    if(p[0] != q[0]) R p[0] < q[0] ? -1 : 1;
+    R p[1] < q[1] ? -1 : (p[1] > q[1]);
+}
+/* lib_join_radix_hash_v3: build on x (inserted in row order, duplicates appended in order),
+ * probe y in row order; pairs (x-row, y-row) are written probe-major, build rows ascending. */
+static int32_t join_hash(OV z0, OV z1, OV x, OV y){
+    L nx = x->len, ny = y->len;
+    int64_t *b = (int64_t*)malloc(sizeof(int64_t)*2*(nx>0?nx:1));
+    for(L i=0;i<nx;i++){ b[2*i] = sI(x)[i]; b[2*i+1] = i; }
+    qsort(b, (size_t)nx, 2*sizeof(int64_t), cmp_keys_i32);
+    L *lo = (L*)malloc(sizeof(L)*(ny>0?ny:1)), *cnt = (L*)malloc(sizeof(L)*(ny>0?ny:1));
+    L c = 0;
+    for(L j=0;j<ny;j++){
+        int64_t key = sI(y)[j]; L a = 0, h = nx;
+        while(a < h){ L m = (a+h)>>1; if(b[2*m] < key) a = m+1; else h = m; }
+        L s = a; while(a < nx && b[2*a] == key) a++;
+        lo[j] = s; cnt[j] = a - s; c += a - s;
+    }
+    orc_initV(z0, HPA_L, c); orc_initV(z1, HPA_L, c);
+    c = 0;
+    for(L j=0;j<ny;j++) for(L k=0;k<cnt[j];k++){ sL(z0)[c] = b[2*(lo[j]+k)+1]; sL(z1)[c] = j; c++; }
+    free(b); free(lo); free(cnt);
+    R 0;
+}
+/* lib_join_index_linear, h_libs.cc:1542-1575: for each y row, the first matching x row only */
+static int32_t join_linear(OV z0, OV z1, OV x, OV y, int isEq){
+    L nx = x->len, ny = y->len, size = 0;
+#define KV(v,i) (v->typ==HPA_I ? (double)sI(v)[i] : v->typ==HPA_L ? (double)sL(v)[i] : sE(v)[i])
+#define MATCH(i,j) (x->typ==HPA_L ? (isEq ? sL(y)[i]==sL(x)[j] : sL(y)[i]!=sL(x)[j]) : (isEq ? KV(y,i)==KV(x,j) : KV(y,i)!=KV(x,j)))
+    if(!(x->typ==HPA_I || x->typ==HPA_L || x->typ==HPA_E)) R HPA_E_NOT_IMPL;
+    for(L i=0;i<ny;i++) for(L j=0;j<nx;j++) if(MATCH(i,j)){ size++; break; }
+    orc_initV(z0, HPA_L, size); orc_initV(z1, HPA_L, size);
+    size = 0;
+    for(L i=0;i<ny;i++) for(L j=0;j<nx;j++) if(MATCH(i,j)){ sL(z0)[size] = j; sL(z1)[size++] = i; break; }
+#undef KV
+#undef MATCH
+    R 0;
+}
+static int32_t join_single(OV z, OV x, OV y, OV f){
+    int op = op_from_symbol(f->q);
+    if(op < 0) R HPA_E_DOMAIN;
+    if(!(is_real(x->typ) && is_real(y->typ))) R HPA_E_NOT_IMPL;
+    int32_t t = MAXV(x->typ, y->typ);
+    O0 tx, ty; int32_t e;
+    if((e=promote(&tx,x,t))) R e;
+    if((e=promote(&ty,y,t))) R e;
+    orc_initV(z, HPA_G, 2);
+    if(op==4 || op==5){
+        if(tx.len >= 10){                          /* NUM_JOIN_LINEAR, common.h:152 */
+            if(op != 4 || t != HPA_I) R HPA_E_NOT_IMPL;   /* v3 hash path: i32 eq only, h_hash_v3.cc:233,272-275 */
+            R join_hash(vV(z,0), vV(z,1), &tx, &ty);
+        }
+        R join_linear(vV(z,0), vV(z,1), &tx, &ty, op==4);
+    }
+    R HPA_E_NOT_IMPL;
+}
+int32_t orc_pfnJoinIndex(OV z, OV x, OV y, OV f){  /* h_primitive.cc:2122-2155 */
+    if(f->len == 1){
+        if(x->len > y->len){
+            int32_t e = join_single(z, y, x, f); if(e) R e;
+            O0 a = *vV(z,0), b = *vV(z,1);               /* swap2, h_system.cc:47-56 */
+            *vV(z,0) = b; *vV(z,1) = a;
+            R 0;
+        }
+        R join_single(z, x, y, f);
+    }
+    R HPA_E_NOT_IMPL;
+}
+
+/* ------------------------------------------------------------------ index_of / member */
+static int cmp_val_idx(const void *a, const void *b){
+    const int64_t *p = (const int64_t*)a, *q = (const int64_t*)b;
+    if(p[0] != q[0]) R p[0] < q[0] ? -1 : 1;
+    R p[1] < q[1] ? -1 : (p[1] > q[1]);
+}
+/* keys compared by value bits for floats (the reference hashes float bytes, h_hash_common.h:111-117) */
+static int64_t key_bits(OV v, L i){
+    switch(v->typ){
+        case HPA_B: R sB(v)[i]; case HPA_J: R sJ(v)[i]; case HPA_H: R sH(v)[i];
+        case HPA_I: case HPA_D: case HPA_M: case HPA_U: case HPA_W: case HPA_T: R sI(v)[i];
+        case HPA_L: case HPA_Z: case HPA_Q: R sL(v)[i];
+        case HPA_F: { int32_t b; memcpy(&b, &sF(v)[i], 4); R b; }
+        case HPA_E: { int64_t b; memcpy(&b, &sE(v)[i], 8); R b; }
+        case HPA_C: R (unsigned char)sC(v)[i];
+    }
+    R 0;
+}
+/* binarySearch_I/L + searchOrdered, h_system.cc:715-745 */
+static L bsearch_ref(OV x, L low, L high, int64_t val){
+    for(;;){
+        L mid = (low+high)>>1;
+        int64_t xm = x->typ==HPA_I ? sI(x)[mid] : sL(x)[mid];
+        if(xm == val) R mid;
+        else if(low == mid) R -1;
+        else if(xm > val) high = mid;
+        else low = mid;
+    }
+}
+int32_t orc_pfnIndexOf(OV z, OV x, OV y){          /* h_primitive.cc:1525-1579 */
+    if(is_real(x->typ) && is_real(y->typ)){
+        if(is_ordered(x)){
+            O0 ty; int32_t e;
+            if((e=promote(&ty, y, x->typ > y->typ ? x->typ : y->typ))) R e;
+            L n = y->len; orc_initV(z, HPA_L, n);
+            for(L i=0;i<n;i++){
+                int64_t v = ty.typ==HPA_I ? sI(&ty)[i] : ty.typ==HPA_L ? sL(&ty)[i] : 0;
+                if(!is_int(ty.typ)){ R HPA_E_NOT_IMPL; }
+                L k = x->len>0 ? bsearch_ref(x, 0, x->len, v) : -1;
+                sL(z)[i] = k<0 ? x->len : k;
+            }
+            R 0;
+        }
+        int32_t t = MAXV(x->typ, y->typ);
+        O0 tx, ty; int32_t e;
+        if((e=promote(&tx,x,t))) R e;
+        if((e=promote(&ty,y,t))) R e;
+        L nx = tx.len, ny = ty.len;
+        orc_initV(z, HPA_L, ny);
+        if(t == HPA_B){                            /* lib_index_of_B, h_libs.cc:309-314: flag[1] starts at 0 */
+            L flag[2] = {-1, 0}; int c = 0;
+            for(L i=0;i<nx;i++){ int s = sB(&tx)[i]; if(c<2 && flag[s]==-1){ flag[s]=i; c++; if(c==2) break; } }
+            for(L i=0;i<ny;i++){ L q = flag[sB(&ty)[i]]; sL(z)[i] = q<0 ? nx : q; }
+            R 0;
+        }
+        int64_t *b = (int64_t*)malloc(sizeof(int64_t)*2*(nx>0?nx:1));
+        for(L i=0;i<nx;i++){ b[2*i] = key_bits(&tx,i); b[2*i+1] = i; }
+        qsort(b, (size_t)nx, 2*sizeof(int64_t), cmp_val_idx);
+        for(L i=0;i<ny;i++){
+            int64_t key = key_bits(&ty,i); L a = 0, h = nx;
+            while(a < h){ L m = (a+h)>>1; if(b[2*m] < key) a = m+1; else h = m; }
+            sL(z)[i] = (a < nx && b[2*a] == key) ? b[2*a+1] : nx;
+        }
+        free(b);
+        R 0;
+    }
+    if(x->typ == y->typ && x->typ == HPA_C){       /* lib_index_of_C, h_libs.cc:352-358 */
+        L nx = x->len, ny = y->len, buf[256];
+        for(int i=0;i<256;i++) buf[i] = -1;
+        for(L i=0;i<nx;i++){ unsigned char c = sC(x)[i]; if(buf[c]==-1) buf[c]=i; }
+        orc_initV(z, HPA_L, ny);
+        for(L i=0;i<ny;i++){ L q = buf[(unsigned char)sC(y)[i]]; sL(z)[i] = q==-1 ? nx : q; }
+        R 0;
+    }
+    R HPA_E_NOT_IMPL;
+}
+
+int32_t orc_pfnMember(OV z, OV y, OV x){           /* h_primitive.cc:2288-2336: z[i] = y[i] in x */
+    if(is_real(x->typ) && is_real(y->typ)){
+        int32_t t = MAXV(x->typ, y->typ);
+        O0 tx, ty; int32_t e;
+        if((e=promote(&tx,x,t))) R e;
+        if((e=promote(&ty,y,t))) R e;
+        L sLen = tx.len, vLen = ty.len;
+        orc_initV(z, HPA_B, vLen);
+        if(t == HPA_B){                            /* lib_member_B, h_libs.cc:969-974 */
+            int flag[2] = {0,0};
+            for(L i=0;i<sLen;i++) flag[sB(&tx)[i]] = 1;
+            for(L i=0;i<vLen;i++) sB(z)[i] = flag[sB(&ty)[i]];
+            R 0;
+        }
+        if(t == HPA_J) R HPA_E_NOT_IMPL;           /* no MEMBER(J) case: EP exit, h_primitive.cc:2299-2305 */
+        if(sLen <= 2){                             /* lib_member_fast1/2, h_libs.cc:899-944: value == */
+            for(L i=0;i<vLen;i++){
+                int hit = 0;
+                for(L k=0;k<sLen;k++){
+                    switch(t){
+                        case HPA_F: hit |= sF(&ty)[i]==sF(&tx)[k]; break;
+                        case HPA_E: hit |= sE(&ty)[i]==sE(&tx)[k]; break;
+                        default: hit |= key_bits(&ty,i)==key_bits(&tx,k);
+                    }
+                }
+                sB(z)[i] = (uint8_t)hit;
+            }
+            R 0;
+        }
+        int64_t *b = (int64_t*)malloc(sizeof(int64_t)*2*(sLen>0?sLen:1));
+        for(L i=0;i<sLen;i++){ b[2*i] = key_bits(&tx,i); b[2*i+1] = i; }
+        qsort(b, (size_t)sLen, 2*sizeof(int64_t), cmp_val_idx);
+        for(L i=0;i<vLen;i++){
+            int64_t key = key_bits(&ty,i); L a = 0, h = sLen;
+            while(a < h){ L m = (a+h)>>1; if(b[2*m] < key) a = m+1; else h = m; }
+            sB(z)[i] = (a < sLen && b[2*a] == key);
+        }
+        free(b);
+        R 0;
+    }
+    if(x->typ == y->typ && x->typ == HPA_C){       /* lib_member_C, h_libs.cc:1005-1011 */
+        uint8_t buf[256] = {0};
+        for(L i=0;i<x->len;i++) buf[(unsigned char)sC(x)[i]] = 1;
+        orc_initV(z, HPA_B, y->len);
+        for(L i=0;i<y->len;i++) sB(z)[i] = buf[(unsigned char)sC(y)[i]];
+        R 0;
+    }
+    R HPA_E_NOT_IMPL;
+}
+
+/* ------------------------------------------------------------------ fused Q6 (opt/q6.h:31-48) */
+double orc_q6_fused(int64_t n, const int32_t *ship, const double *disc, const int32_t *qty,
+                    const double *price, int32_t lo, int32_t hi, double dlo, double dhi, int32_t qmax){
+    double total = 0;
+    #pragma omp parallel for simd reduction(+:total)
+    for(L i=0;i<n;i++)
+        if(ship[i]>=lo && ship[i]<hi && qty[i]<qmax && disc[i]>=dlo && disc[i]<=dhi)
+            total += price[i]*disc[i];
+    R total;
+}

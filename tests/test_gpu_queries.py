@@ -1,0 +1,55 @@
+"""End-to-end parity of the benchmark queries (plain interpreter sequence and the fused
+compiled form) on the GPU against the oracle running the same plans."""
+import math
+
+import numpy as np
+import pytest
+
+from horsepower_amd import datagen as D
+from horsepower_amd import queries as Q
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-12
+
+
+def close(a, b, key):
+    a, b = np.asarray(a), np.asarray(b)
+    if a.dtype.kind == "f":
+        assert np.allclose(a, b, rtol=RTOL, atol=0), (key, a, b)
+    else:
+        assert np.array_equal(a, b), (key, a, b)
+
+
+@pytest.mark.parametrize("n", [1000, 1_000_003])
+def test_q6(gpu, orc, n):
+    t = D.gen_q6(n)
+    Tg, To = Q.load_table(gpu, t), Q.load_table(orc, t)
+    o = Q.q6_plain(orc, To)["revenue"]
+    m = (t["ship_day"] >= D.Q6_LO) & (t["ship_day"] < D.Q6_HI) & (t["quantity"] < 24) & \
+        (t["discount"] >= 0.05) & (t["discount"] <= 0.07)
+    exact = math.fsum(t["price"][m] * t["discount"][m])
+    for fn in (Q.q6_plain, Q.q6_fused):
+        g = fn(gpu, Tg)["revenue"]
+        assert abs(g - o) <= RTOL * abs(o), fn.__name__
+        assert abs(g - exact) <= RTOL * abs(exact), fn.__name__
+
+
+@pytest.mark.parametrize("n", [5000, 600_001])
+def test_q1(gpu, orc, n):
+    t = D.gen_q1(n)
+    Tg, To = Q.load_table(gpu, t), Q.load_table(orc, t)
+    o = Q.q1_plain(orc, To)
+    for fn in (Q.q1_plain, Q.q1_fused):
+        g = fn(gpu, Tg)
+        for k in Q.Q1_NAMES:
+            close(g[k], o[k], (fn.__name__, k))
+
+
+@pytest.mark.parametrize("nd", [3000, 200_000])
+def test_q3(gpu, orc, nd):
+    dm, f = D.gen_q3(nd)
+    o = Q.q3_plain(orc, Q.load_table(orc, dm), Q.load_table(orc, f))
+    g = Q.q3_plain(gpu, Q.load_table(gpu, dm), Q.load_table(gpu, f))
+    assert g["rows"] == o["rows"] and g["rows"] > 0
+    for k in Q.Q3_NAMES:
+        close(g[k], o[k], k)
