@@ -38,7 +38,8 @@ __global__ void __launch_bounds__(256) k_map2(Src<TA> a, Src<TB> b, TZ *z, L n, 
 template<typename TA, typename TB, typename TZ, typename F>
 static int32_t launch_map2(hpa_ctx *c, const char *name, Src<TA> a, Src<TB> b, TZ *z, L n, F f){
     if(n <= 0) R 0;
-    ProfScope ps(name, c->stream);
+    double bytes = (double)n * ((a.p ? sizeof(TA) : 0) + (b.p ? sizeof(TB) : 0) + sizeof(TZ));
+    ProfScope ps(name, c->stream, bytes);
     L nq = (n + 3) >> 2;
     hipLaunchKernelGGL((k_map2<TA,TB,TZ,F>), dim3(grid_for(nq, 256, 1)), dim3(256), 0, c->stream, a, b, z, n, f);
     R check_launch(name);
@@ -371,7 +372,7 @@ __global__ void __launch_bounds__(256) k_gather(const T *src, const int64_t *idx
 }
 static int32_t gather_checked(hpa_ctx *c, void *dst, const void *src, const int64_t *idx, L n, L nsrc, L w, int *bad){
     if(n <= 0) R 0;
-    ProfScope ps("gather", c->stream);
+    ProfScope ps("gather", c->stream, (double)n * (8 + 2 * w));
     dim3 g(grid_for(n, 256, 4)), b(256);
     switch(w){
         case 1: hipLaunchKernelGGL(k_gather<uint8_t>, g, b, 0, c->stream, (const uint8_t*)src, idx, (uint8_t*)dst, n, nsrc, bad); break;

@@ -63,9 +63,13 @@ int32_t d2h(hpa_ctx *c, void *dst, const void *src, L bytes);
 int32_t check_launch(const char *what);
 
 // ---------------------------------------------------------------- timing
+// Brackets one kernel launch with HIP events on its stream while profiling is enabled;
+// `bytes` is the launch's algorithmic HBM traffic (inputs read once, outputs written once).
 struct ProfScope {
-    const char *name; hipStream_t s; hipEvent_t a = nullptr, b = nullptr; bool on;
-    ProfScope(const char *n, hipStream_t st);
+    const char *name; hipStream_t s; hipEvent_t a = nullptr, b = nullptr; bool on; double bytes = 0;
+    bool stopped = false;
+    ProfScope(const char *n, hipStream_t st, double algo_bytes = 0);
+    void stop();                      // record the end event now (before later stream work)
     ~ProfScope();
 };
 bool prof_enabled();

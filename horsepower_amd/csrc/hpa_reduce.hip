@@ -103,11 +103,11 @@ static int32_t sum_fp(hpa_ctx *c, V x, const void *p, double *res){
     KS *part = (KS*)dalloc(c, sizeof(KS) * G); double *out = (double*)dalloc(c, 16);
     if(!part || !out) R HPA_E_MEMORY;
     {
-        ProfScope ps("sum", c->stream);
+        ProfScope ps("sum", c->stream, (double)n * type_width(x->typ));
         if(x->typ == HPA_E) hipLaunchKernelGGL(k_sum_f<double>, dim3(G), dim3(RD_BLOCK), 0, c->stream, (const double*)p, n, part);
         else hipLaunchKernelGGL(k_sum_f<float>, dim3(G), dim3(RD_BLOCK), 0, c->stream, (const float*)p, n, part);
-        hipLaunchKernelGGL(k_sum_f_final, dim3(1), dim3(64), 0, c->stream, part, G, out);
     }
+    hipLaunchKernelGGL(k_sum_f_final, dim3(1), dim3(64), 0, c->stream, part, G, out);
     int32_t e = check_launch("sum"); if(e) R e;
     R d2h(c, res, out, 8);
 }
@@ -116,7 +116,7 @@ static int32_t sum_int(hpa_ctx *c, V x, const void *p, int64_t *res){
     unsigned long long *acc = (unsigned long long*)dalloc(c, 16); if(!acc) R HPA_E_MEMORY;
     hipMemsetAsync(acc, 0, 8, c->stream);
     {
-        ProfScope ps("sum", c->stream);
+        ProfScope ps("sum", c->stream, (double)n * type_width(x->typ));
         dim3 g(G), b(RD_BLOCK);
         switch(x->typ){
             case HPA_B: hipLaunchKernelGGL(k_sum_i<uint8_t>, g, b, 0, c->stream, (const uint8_t*)p, n, acc); break;
@@ -171,14 +171,13 @@ static int32_t minmax_typed(hpa_ctx *c, const void *p, L n, int op, T *res){
     int G = sum_grid(n);
     T *part = (T*)dalloc(c, sizeof(T) * G + 16); T *out = (T*)dalloc(c, 16);
     if(!part || !out) R HPA_E_MEMORY;
-    ProfScope ps("minmax", c->stream);
-    if(op == 0){
-        hipLaunchKernelGGL((k_minmax<T,0>), dim3(G), dim3(RD_BLOCK), 0, c->stream, (const T*)p, n, part);
-        hipLaunchKernelGGL((k_minmax_final<T,0>), dim3(1), dim3(64), 0, c->stream, part, G, out);
-    } else {
-        hipLaunchKernelGGL((k_minmax<T,1>), dim3(G), dim3(RD_BLOCK), 0, c->stream, (const T*)p, n, part);
-        hipLaunchKernelGGL((k_minmax_final<T,1>), dim3(1), dim3(64), 0, c->stream, part, G, out);
+    {
+        ProfScope ps("minmax", c->stream, (double)n * sizeof(T));
+        if(op == 0) hipLaunchKernelGGL((k_minmax<T,0>), dim3(G), dim3(RD_BLOCK), 0, c->stream, (const T*)p, n, part);
+        else hipLaunchKernelGGL((k_minmax<T,1>), dim3(G), dim3(RD_BLOCK), 0, c->stream, (const T*)p, n, part);
     }
+    if(op == 0) hipLaunchKernelGGL((k_minmax_final<T,0>), dim3(1), dim3(64), 0, c->stream, part, G, out);
+    else hipLaunchKernelGGL((k_minmax_final<T,1>), dim3(1), dim3(64), 0, c->stream, part, G, out);
     int32_t e = check_launch("minmax"); if(e) R e;
     R d2h(c, res, out, sizeof(T));
 }
@@ -355,15 +354,17 @@ __global__ void k_seg_maxlen(const int64_t *off, L g, unsigned long long *mx){
 template<typename T, bool FLT>
 static int32_t seg_reduce_t(hpa_ctx *c, const T *x, const int64_t *idx, const int64_t *off, L g, L n, int kind, void *out, L nx, int *bad){
     if(g <= 0) R 0;
-    ProfScope ps("group_reduce", c->stream);
+    double bytes = (double)n * ((idx ? 8 : 0) + sizeof(T)) + 8.0 * g + 8.0 * (g + 1);
     double avg = (double)n / (double)g;
     if(avg <= 16.0){
+        ProfScope ps("group_reduce", c->stream, bytes);
         hipLaunchKernelGGL((k_seg_thread<T,FLT>), dim3(grid_for(g, 256, 1)), dim3(256), 0, c->stream, x, idx, off, g, kind, out, nx, bad);
         R check_launch("group_reduce");
     }
     if(avg <= 16384.0 || g > 65535){
         L waves = g; L blocks = ceil_div(waves * 64, 256);
         if(blocks > (L)num_cus() * 32) blocks = (L)num_cus() * 32;
+        ProfScope ps("group_reduce", c->stream, bytes);
         hipLaunchKernelGGL((k_seg_wave<T,FLT>), dim3((unsigned)blocks), dim3(256), 0, c->stream, x, idx, off, g, kind, out, nx, bad);
         R check_launch("group_reduce");
     }
@@ -377,7 +378,10 @@ static int32_t seg_reduce_t(hpa_ctx *c, const T *x, const int64_t *idx, const in
     if(kind == GK_SUM || kind == GK_AVG){ if(FLT) pf = (KS*)dalloc(c, sizeof(KS) * slots); else pi = (unsigned long long*)dalloc(c, 8 * slots); }
     else pm = (T*)dalloc(c, sizeof(T) * slots + 16);
     if(!pf && !pi && !pm) R HPA_E_MEMORY;
-    hipLaunchKernelGGL((k_seg_block<T,FLT>), dim3(nch, (unsigned)g), dim3(256), 0, c->stream, x, idx, off, kind, pf, pi, pm, nch, nx, bad);
+    {
+        ProfScope ps("group_reduce", c->stream, bytes);
+        hipLaunchKernelGGL((k_seg_block<T,FLT>), dim3(nch, (unsigned)g), dim3(256), 0, c->stream, x, idx, off, kind, pf, pi, pm, nch, nx, bad);
+    }
     hipLaunchKernelGGL((k_seg_block_final<T,FLT>), dim3(grid_for(g, 64, 1)), dim3(64), 0, c->stream, off, g, kind, pf, pi, pm, nch, out);
     R check_launch("group_reduce");
 }
@@ -451,6 +455,7 @@ extern "C" int32_t hpaGroupReduce(hpa_V z, int32_t kind, hpa_V x, hpa_V y){
     int32_t e;
     L g, n; const int64_t *off, *idx;
     if(is_flat(y)){
+        if(y->g2->typ != HPA_L) R HPA_E_TYPE;
         if((e = flat_offsets(c, y, &off, &idx, &g, &n))) R e;
     } else {
         // normal list of index vectors: concatenate on the device
@@ -642,7 +647,8 @@ extern "C" int32_t hpaFusedMap(hpa_V z, int32_t typ, int32_t nx, hpa_V *x, const
     void *dz;
     if((e = out_init(c, z, typ, n, &dz))) R e;
     if(n > 0){
-        ProfScope ps("fused_map", c->stream);
+        double cb = 0; for(int k = 0; k < nx; k++) cb += (double)type_width(x[k]->typ);
+        ProfScope ps("fused_map", c->stream, (double)n * (cb + type_width(typ)));
         dim3 g(grid_for(n, 256, 4)), b(256);
         switch(typ){
             case HPA_B: hipLaunchKernelGGL(k_fused_map<HPA_B>, g, b, 0, c->stream, de, cols, n, dz); break;
@@ -721,10 +727,11 @@ extern "C" int32_t hpaFusedReduce(hpa_V z, int32_t kind, int32_t nx, hpa_V *x, c
     FaPart *part = (FaPart*)dalloc(c, sizeof(FaPart) * G), *out = (FaPart*)dalloc(c, sizeof(FaPart));
     if(!part || !out) R HPA_E_MEMORY;
     {
-        ProfScope ps("fused_reduce", c->stream);
+        double cb = 0; for(int k = 0; k < nx; k++) cb += (double)type_width(x[k]->typ);
+        ProfScope ps("fused_reduce", c->stream, (double)n * cb);
         hipLaunchKernelGGL(k_fused_reduce, dim3(G), dim3(256), 0, c->stream, ce, cond != nullptr, be, cols, n, part);
-        hipLaunchKernelGGL(k_fused_reduce_final, dim3(1), dim3(64), 0, c->stream, part, G, out);
     }
+    hipLaunchKernelGGL(k_fused_reduce_final, dim3(1), dim3(64), 0, c->stream, part, G, out);
     if((e = check_launch("fused_reduce"))) R e;
     FaPart h; if((e = d2h(c, &h, out, sizeof(FaPart)))) R e;
     c->dcur = mark;

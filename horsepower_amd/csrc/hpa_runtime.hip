@@ -372,21 +372,27 @@ extern "C" const char *hpaSymbolName(int64_t id){
 
 // ---------------------------------------------------------------- kernel timing
 static bool g_prof = false;
-struct PendingEv { std::string name; hipEvent_t a, b; };
+struct PendingEv { std::string name; hipEvent_t a, b; double bytes; };
 static std::vector<PendingEv> g_pending;
-static std::map<std::string, std::pair<int64_t,double>> g_stats;
+struct Stat { int64_t count = 0; double ms = 0, bytes = 0; };
+static std::map<std::string, Stat> g_stats;
 
 bool prof_enabled(){ R g_prof; }
-ProfScope::ProfScope(const char *n, hipStream_t st) : name(n), s(st), on(g_prof){
+ProfScope::ProfScope(const char *n, hipStream_t st, double algo_bytes) : name(n), s(st), on(g_prof), bytes(algo_bytes){
     if(!on) R;
     hipEventCreate(&a); hipEventCreate(&b);
     hipEventRecord(a, s);
 }
+void ProfScope::stop(){
+    if(!on || stopped) R;
+    hipEventRecord(b, s);
+    stopped = true;
+}
 ProfScope::~ProfScope(){
     if(!on) R;
-    hipEventRecord(b, s);
+    stop();
     std::lock_guard<std::mutex> lk(g_mu);
-    g_pending.push_back({name, a, b});
+    g_pending.push_back({name, a, b, bytes});
 }
 static void prof_resolve(){
     std::vector<PendingEv> p;
@@ -396,7 +402,7 @@ static void prof_resolve(){
         float ms = 0; hipEventElapsedTime(&ms, e.a, e.b);
         hipEventDestroy(e.a); hipEventDestroy(e.b);
         std::lock_guard<std::mutex> lk(g_mu);
-        auto &s = g_stats[e.name]; s.first++; s.second += ms;
+        auto &s = g_stats[e.name]; s.count++; s.ms += ms; s.bytes += e.bytes;
     }
 }
 extern "C" void hpaProfileEnable(int32_t on){ g_prof = on != 0; }
@@ -406,13 +412,13 @@ extern "C" int32_t hpaProfileStats(const char *kernel, int64_t *count, double *t
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_stats.find(kernel);
     if(it == g_stats.end()){ *count = 0; *total_ms = 0; R 1; }
-    *count = it->second.first; *total_ms = it->second.second; R 0;
+    *count = it->second.count; *total_ms = it->second.ms; R 0;
 }
 extern "C" int32_t hpaProfileList(char *buf, int64_t cap){
     prof_resolve();
     std::lock_guard<std::mutex> lk(g_mu);
     std::string s;
-    for(auto &kv : g_stats){ char line[256]; snprintf(line, sizeof(line), "%s %lld %.6f\n", kv.first.c_str(), (long long)kv.second.first, kv.second.second); s += line; }
+    for(auto &kv : g_stats){ char line[256]; snprintf(line, sizeof(line), "%s %lld %.6f %.0f\n", kv.first.c_str(), (long long)kv.second.count, kv.second.ms, kv.second.bytes); s += line; }
     if((int64_t)s.size() + 1 > cap) R (int32_t)(s.size() + 1);
     memcpy(buf, s.c_str(), s.size() + 1);
     R 0;
@@ -521,8 +527,9 @@ int32_t flat_offsets(hpa_ctx *c, V y, const int64_t **offs, const int64_t **idx,
         pc = d;
     }
     V t = y->g2;
-    if(t->typ != HPA_L) R HPA_E_TYPE;
-    if((e = dev_view(c, t, &pi))) R e;
+    // the inner vector is the i64 row list of a group result, or the gathered values of
+    // each_right(@index) (any element type); only the former is returned as an index
+    if(t->typ == HPA_L){ if((e = dev_view(c, t, &pi))) R e; }
     int64_t *o = (int64_t*)dalloc(c, 8 * (*g + 1));
     if(!o) R HPA_E_MEMORY;
     if((e = scan_i64(c, (const int64_t*)pc, o, *g, o + *g))) R e;

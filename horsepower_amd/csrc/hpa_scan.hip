@@ -109,8 +109,10 @@ __global__ void __launch_bounds__(CP_BLOCK) k_compress(Pred pred, L n, CpCols co
     }
 }
 
+// per_row / per_sel: algorithmic bytes read per input row and written per selected row
 template<typename Pred>
-static int32_t run_compress(hpa_ctx *c, const char *name, Pred pred, L n, CpCols cols, int64_t *idx_out, int64_t *k_host){
+static int32_t run_compress(hpa_ctx *c, const char *name, Pred pred, L n, CpCols cols, int64_t *idx_out, int64_t *k_host,
+                            double per_row, double per_sel){
     L ntiles = ceil_div(n, CP_TILE);
     if(ntiles < 1){ *k_host = 0; R 0; }
     uint64_t *status = (uint64_t*)dalloc(c, 8 * (ntiles + 2));
@@ -118,13 +120,14 @@ static int32_t run_compress(hpa_ctx *c, const char *name, Pred pred, L n, CpCols
     unsigned *ctr = (unsigned*)(status + ntiles);
     int64_t *total = (int64_t*)(status + ntiles + 1);
     int32_t e = fill_u64(c, status, ntiles + 2, 0); if(e) R e;
-    {
-        ProfScope ps(name, c->stream);
-        hipLaunchKernelGGL(k_compress<Pred>, dim3((unsigned)ntiles), dim3(CP_BLOCK), 0, c->stream,
-                           pred, n, cols, idx_out, status, ctr, ntiles, total);
-    }
+    ProfScope ps(name, c->stream);
+    hipLaunchKernelGGL(k_compress<Pred>, dim3((unsigned)ntiles), dim3(CP_BLOCK), 0, c->stream,
+                       pred, n, cols, idx_out, status, ctr, ntiles, total);
+    ps.stop();
     if((e = check_launch(name))) R e;
-    R d2h(c, k_host, total, 8);
+    e = d2h(c, k_host, total, 8);
+    ps.bytes = per_row * (double)n + per_sel * (double)*k_host;
+    R e;
 }
 
 // pfnCompress (h_primitive.cc:1479-1521)
@@ -156,8 +159,11 @@ static int32_t compress_many(hpa_ctx *c, int ny, V *z, V x, V *y, bool expr_mode
         if(!cols.dst[k]){ c->dcur = mark; R HPA_E_MEMORY; }
     }
     int64_t kk = 0;
-    if(expr_mode) e = run_compress(c, "compress", *ep, n, cols, nullptr, &kk);
-    else e = run_compress(c, "compress", MaskPred{m}, n, cols, nullptr, &kk);
+    double wsum = 0; for(int k = 0; k < ny; k++) wsum += cols.w[k];
+    double wcond = 0;
+    if(expr_mode) for(int k = 0; k < ep->x.n; k++) wcond += (double)type_width(ep->x.typ[k]);
+    if(expr_mode) e = run_compress(c, "fused_compress", *ep, n, cols, nullptr, &kk, wcond + wsum, wsum);
+    else e = run_compress(c, "compress", MaskPred{m}, n, cols, nullptr, &kk, 1.0 + wsum, wsum);
     if(e) R e;
     for(int k = 0; k < ny; k++){
         V zk = z[k];
@@ -201,7 +207,7 @@ extern "C" int32_t pfnWhere(hpa_V z, hpa_V x){
     int64_t *out = (int64_t*)dalloc(c, 8 * (n > 0 ? n : 1)); if(!out) R HPA_E_MEMORY;
     CpCols cols; cols.n = 0;
     int64_t kk = 0;
-    if((e = run_compress(c, "where", MaskPred{(const uint8_t*)p}, n, cols, out, &kk))) R e;
+    if((e = run_compress(c, "where", MaskPred{(const uint8_t*)p}, n, cols, out, &kk, 1.0, 8.0))) R e;
     z->typ = HPA_L; z->len = kk;
     if(kk <= 1){
         z->g = (char*)&z->b; z->i64 = 0;

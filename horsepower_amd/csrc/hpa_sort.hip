@@ -117,13 +117,18 @@ static int32_t radix_sort_t(hpa_ctx *c, K *keys, uint32_t *vals, L n, int lo, in
     K *ka = keys, *kb = k2; uint32_t *va = vals, *vb = v2;
     uint32_t *spare = vals ? vals : v1;
     bool any = false;
-    ProfScope ps("radix_sort", c->stream);
     for(int s = lo; s < hi; s += 8){
         if(((vary >> s) & 255ull) == 0) continue;
-        hipLaunchKernelGGL(k_rs_hist<K>, dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream, ka, n, s, hist, ntiles);
+        {
+            ProfScope ps("radix_hist", c->stream, (double)n * sizeof(K) + 4.0 * 256 * ntiles);
+            hipLaunchKernelGGL(k_rs_hist<K>, dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream, ka, n, s, hist, ntiles);
+        }
         if((e = scan_u32_to_i64(c, hist, off, 256 * ntiles, nullptr))) R e;
-        hipLaunchKernelGGL(k_rs_scatter<K>, dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream,
-                           ka, va, kb, vb, n, s, off, ntiles);
+        {
+            ProfScope ps("radix_scatter", c->stream, (double)n * (2 * sizeof(K) + (va ? 8 : 4)) + 8.0 * 256 * ntiles);
+            hipLaunchKernelGGL(k_rs_scatter<K>, dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream,
+                               ka, va, kb, vb, n, s, off, ntiles);
+        }
         if((e = check_launch("radix_sort"))) R e;
         any = true;
         K *tk = ka; ka = kb; kb = tk;

@@ -11,11 +11,15 @@ Both take a backend (backend.Backend) and a dict of column nodes, and return num
 import numpy as np
 
 from . import datagen as D
-from .vnode import H_B, H_E, H_I, H_L, expr
+from .vnode import H_B, H_C, H_E, H_I, H_L, expr
+
+
+CHAR_COLUMNS = ("return_flag", "line_status")
 
 
 def load_table(B, cols):
-    return {k: B.vec(v) for k, v in cols.items()}
+    """Columns as resident nodes; the flag/status bytes are char (H_C) columns."""
+    return {k: B.vec(v, H_C if k in CHAR_COLUMNS else None) for k, v in cols.items()}
 
 
 # ------------------------------------------------------------------------------------- Q6
