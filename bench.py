@@ -33,7 +33,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="q1", choices=["q1", "q6", "q3"])
     p.add_argument("--rows", type=int, default=0)
-    p.add_argument("--plan", default="fused", choices=["fused", "plain"])
+    p.add_argument("--plan", default="best", choices=["best", "fused", "plain"])
     p.add_argument("--cpu-rows", type=int, default=0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--profile-steps", type=int, default=3)
@@ -44,15 +44,15 @@ def workload(name, rows):
     from horsepower_amd import datagen as D, queries as Q
     if name == "q1":
         n = rows or 60_000_000
-        return dict(n=n, gen=lambda k: (D.gen_q1(k),), plans=dict(fused=Q.q1_fused, plain=Q.q1_plain),
+        return dict(n=n, gen=lambda k: (D.gen_q1(k),), plans=dict(best=Q.q1_chf, fused=Q.q1_fused, plain=Q.q1_plain),
                     desc=f"q1 hash group-by, {n} rows, filter ship_day<=2434, 8 aggregates, order by key",
                     cpu_default=6_000_000)
     if name == "q6":
         n = rows or 60_000_000
-        return dict(n=n, gen=lambda k: (D.gen_q6(k),), plans=dict(fused=Q.q6_fused, plain=Q.q6_plain),
+        return dict(n=n, gen=lambda k: (D.gen_q6(k),), plans=dict(best=Q.q6_fused, fused=Q.q6_fused, plain=Q.q6_plain),
                     desc=f"q6 filter-compress-aggregate, {n} rows", cpu_default=6_000_000)
     nd = (rows or 60_000_000) // 4
-    return dict(n=None, nd=nd, gen=lambda k: D.gen_q3(k), plans=dict(fused=Q.q3_plain, plain=Q.q3_plain),
+    return dict(n=None, nd=nd, gen=lambda k: D.gen_q3(k), plans=dict(best=Q.q3_plain, fused=Q.q3_plain, plain=Q.q3_plain),
                 desc=f"q3 join+group+top-10, {nd} orders", cpu_default=1_500_000)
 
 

@@ -194,6 +194,24 @@ class GpuBackend(Backend):
         self.lib.rc("hpaFusedMap", z, typ, len(xs), xa, C.byref(e))
         return z
 
+    def fused_group_agg(self, keys, xs, cond, spec):
+        """One-pass filter + group + aggregates.  Returns (first, [key vectors], [aggregates]) or
+        None when the key cardinality per workgroup exceeds the fused kernel's slots."""
+        from .vnode import HorseError, aggs
+        nk, na = len(keys), len(spec)
+        zk = [self.node() for _ in range(nk)]
+        za = [self.node() for _ in range(na)]
+        zf = self.node()
+        ag = aggs(spec)
+        rc = self.lib.dll.hpaFusedGroupAgg(nk, (V * nk)(*keys), len(xs), (V * len(xs))(*xs),
+                                           C.byref(cond) if cond is not None else None, na,
+                                           C.cast(ag, C.c_void_p), zf, (V * nk)(*zk), (V * na)(*za))
+        if rc == 99:
+            return None
+        if rc != 0:
+            raise HorseError(rc, "hpaFusedGroupAgg", self.lib.detail())
+        return zf, zk, za
+
     def group_reduce(self, kind, x, groups):
         z = self.node()
         self.lib.rc("hpaGroupReduce", z, kind, x, groups)

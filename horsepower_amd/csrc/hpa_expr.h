@@ -78,3 +78,8 @@ __device__ __forceinline__ bool expr_true(const DevExpr &e, const Slot &s){
 int32_t expr_compile(const hpa_expr *e, int nx, const int32_t *coltyp, DevExpr *out);
 // Host: device views of the input columns (all of equal length n).
 int32_t cols_view(hpa_ctx *c, int nx, V *x, DevCols *cols, L *n);
+// Generated-code versions of the FA reduction and FE map (nonzero: not available, the caller
+// interprets the expressions instead).
+int32_t jit_fused_reduce(hpa_ctx *c, int kind, int nx, V *x, const hpa_expr *cond, const hpa_expr *body,
+                         const DevCols &cols, L n, void *part, int G);
+int32_t jit_fused_map(hpa_ctx *c, int otyp, int nx, V *x, const hpa_expr *ex, const DevCols &cols, L n, void *dz);

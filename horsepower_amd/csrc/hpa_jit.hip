@@ -1,0 +1,636 @@
+// Run-time code generation of fused loops.
+//
+// The reference's optimiser turns fused sections into C text (optimizer/elementwise.cc:239-281,
+// autofusion.cc:770-935, pattern.cc:180-219) that a C compiler builds offline.  Here each fused
+// section is emitted as HIP C++ for the running gfx950 device, compiled once with hipRTC and
+// cached by its source text; the postfix expressions of the public API become straight-line
+// typed C expressions, so the kernels stream their columns at HBM speed instead of interpreting
+// an instruction list per row.
+//
+// Kernels generated here:
+//   fa_reduce  FA reduction with filter (sum / avg / min / max of a body over rows where cond)
+//   fe_map     FE element-wise map into a typed output
+//   fga        filter + group on one or two byte-wide key columns + N aggregates, one pass:
+//              a 64K-entry LDS table maps each key code to a block-local slot (first come,
+//              first served), each thread keeps register accumulators per slot, slots are
+//              reduced per block into partials; fga_merge combines the partials per key code
+//              in block order (deterministic) and emits groups in ascending code order, the
+//              order of pfnGroupTrie (h_primitive.cc:2656-2681).  More than NS distinct codes
+//              in one block is reported as an overflow and the caller takes the unfused path.
+#include "hpa_internal.h"
+#include "hpa_expr.h"
+#include <hip/hiprtc.h>
+#include <mutex>
+#include <sstream>
+#include <stdio.h>
+
+#define FGA_NS 8
+#define FGA_BLOCK 256
+
+static std::mutex g_jit_mu;
+struct JitFn { hipModule_t mod; hipFunction_t fn; };
+static std::map<std::string, JitFn> g_jit;
+static int g_jit_disabled = -1;
+static double g_jit_compile_ms = 0;
+
+static bool jit_enabled(){
+    if(g_jit_disabled < 0){ const char *e = getenv("HPA_NO_JIT"); g_jit_disabled = (e && *e && *e != '0') ? 1 : 0; }
+    R g_jit_disabled == 0;
+}
+
+static std::string arch_name(){
+    hipDeviceProp_t p; int d = 0; hipGetDevice(&d);
+    if(hipGetDeviceProperties(&p, d) == hipSuccess){
+        std::string a = p.gcnArchName;
+        size_t k = a.find(':'); if(k != std::string::npos) a = a.substr(0, k);
+        if(!a.empty()) R a;
+    }
+    R "gfx950";
+}
+
+static bool g_compile_only = false;       // diagnostic: generate + compile, never load or launch
+
+static int32_t jit_get(const std::string &src, const char *kname, hipFunction_t *fn){
+    std::lock_guard<std::mutex> lk(g_jit_mu);
+    std::string key = std::string(kname) + "\n" + src;
+    auto it = g_jit.find(key);
+    if(it != g_jit.end()){ *fn = it->second.fn; R 0; }
+    hipEvent_t t0, t1;
+    hiprtcProgram prog;
+    if(hiprtcCreateProgram(&prog, src.c_str(), "fused.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS){
+        set_error_detail("hiprtcCreateProgram failed"); R HPA_E_DEVICE;
+    }
+    std::string arch = "--offload-arch=" + arch_name();
+    const char *opts[] = {arch.c_str(), "-O3", "-std=c++17", "-ffp-contract=off",
+                          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    (void)t0; (void)t1;
+    hiprtcResult r = hiprtcCompileProgram(prog, 8, opts);
+    if(r != HIPRTC_SUCCESS){
+        size_t ls = 0; hiprtcGetProgramLogSize(prog, &ls);
+        std::string log(ls + 1, '\0'); hiprtcGetProgramLog(prog, &log[0]);
+        set_error_detail("hipRTC compile of %s failed: %.400s", kname, log.c_str());
+        hiprtcDestroyProgram(&prog);
+        R HPA_E_DEVICE;
+    }
+    size_t cs = 0; hiprtcGetCodeSize(prog, &cs);
+    std::vector<char> code(cs); hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    if(g_compile_only){ *fn = nullptr; R cs > 0 ? 0 : HPA_E_DEVICE; }
+    JitFn j;
+    if(hipModuleLoadData(&j.mod, code.data()) != hipSuccess){ set_error_detail("hipModuleLoadData failed"); R HPA_E_DEVICE; }
+    if(hipModuleGetFunction(&j.fn, j.mod, kname) != hipSuccess){ set_error_detail("hipModuleGetFunction %s failed", kname); R HPA_E_DEVICE; }
+    g_jit[key] = j;
+    *fn = j.fn;
+    R 0;
+}
+
+template<typename A>
+static int32_t jit_launch(hpa_ctx *c, hipFunction_t fn, unsigned grid, unsigned block, A &args){
+    void *params[] = {&args};
+    if(hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, 0, c->stream, params, nullptr) != hipSuccess)
+        R check_launch("jit launch") ? HPA_E_DEVICE : HPA_E_DEVICE;
+    R check_launch("jit launch");
+}
+
+// ---------------------------------------------------------------- expression -> C
+static const char *ctype(int32_t t){
+    switch(t){
+        case HPA_B: R "unsigned char"; case HPA_J: R "signed char"; case HPA_C: R "char";
+        case HPA_H: R "short";
+        case HPA_I: case HPA_M: case HPA_D: case HPA_U: case HPA_W: case HPA_T: R "int";
+        case HPA_L: case HPA_Z: case HPA_Q: R "long long";
+        case HPA_F: R "float"; case HPA_E: R "double";
+    }
+    R nullptr;
+}
+
+struct CExpr { std::string s; bool f; };
+
+static int32_t expr_to_c(const hpa_expr *e, int nx, const int32_t *typ, CExpr *out, std::vector<int> &used){
+    if(!e || e->n <= 0 || e->n > 256) R HPA_E_DOMAIN;
+    std::vector<CExpr> st;
+    char buf[128];
+    for(int k = 0; k < e->n; k++){
+        const hpa_ins &in = e->code[k];
+        switch(in.op){
+            case HPA_OP_COL: {
+                if(in.arg < 0 || in.arg >= nx || !ctype(typ[in.arg])) R HPA_E_TYPE;
+                used[in.arg] = 1;
+                bool f = typ[in.arg] == HPA_F || typ[in.arg] == HPA_E;
+                snprintf(buf, sizeof(buf), f ? "((double)v%d)" : "((i64)v%d)", in.arg);
+                st.push_back({buf, f}); break;
+            }
+            case HPA_OP_CONST_I:
+                if(in.imm_i == INT64_MIN) snprintf(buf, sizeof(buf), "(-9223372036854775807LL-1)");
+                else snprintf(buf, sizeof(buf), "((i64)%lldLL)", (long long)in.imm_i);
+                st.push_back({buf, false}); break;
+            case HPA_OP_CONST_F: snprintf(buf, sizeof(buf), "(%a)", in.imm_f); st.push_back({buf, true}); break;
+            case HPA_OP_NOT: {
+                if(st.empty()) R HPA_E_DOMAIN;
+                CExpr a = st.back(); st.pop_back();
+                st.push_back({"((" + a.s + ") == 0 ? (i64)1 : (i64)0)", false}); break;
+            }
+            case HPA_OP_NEG: {
+                if(st.empty()) R HPA_E_DOMAIN;
+                CExpr a = st.back(); st.pop_back();
+                st.push_back({a.f ? "(-(" + a.s + "))" : "((i64)(0ULL - (u64)(" + a.s + ")))", a.f}); break;
+            }
+            default: {
+                if(st.size() < 2) R HPA_E_DOMAIN;
+                CExpr b = st.back(); st.pop_back(); CExpr a = st.back(); st.pop_back();
+                bool f = a.f || b.f;
+                std::string A = f && !a.f ? "((double)" + a.s + ")" : a.s;
+                std::string B = f && !b.f ? "((double)" + b.s + ")" : b.s;
+                const char *op = nullptr;
+                switch(in.op){
+                    case HPA_OP_ADD: op = "+"; break; case HPA_OP_SUB: op = "-"; break; case HPA_OP_MUL: op = "*"; break;
+                    case HPA_OP_DIV: op = "/"; break; case HPA_OP_LT: op = "<"; break; case HPA_OP_LEQ: op = "<="; break;
+                    case HPA_OP_GT: op = ">"; break; case HPA_OP_GEQ: op = ">="; break; case HPA_OP_EQ: op = "=="; break;
+                    case HPA_OP_NEQ: op = "!="; break; case HPA_OP_AND: op = "&&"; break; case HPA_OP_OR: op = "||"; break;
+                    default: R HPA_E_DOMAIN;
+                }
+                if(in.op <= HPA_OP_MUL){
+                    if(f) st.push_back({"(" + A + " " + op + " " + B + ")", true});
+                    else st.push_back({"((i64)((u64)(" + A + ") " + op + " (u64)(" + B + ")))", false});
+                } else if(in.op == HPA_OP_DIV){
+                    if(f) st.push_back({"(" + A + " / " + B + ")", true});
+                    else st.push_back({"hdiv(" + A + ", " + B + ")", false});
+                } else if(in.op == HPA_OP_AND || in.op == HPA_OP_OR){
+                    st.push_back({"(((" + a.s + ") != 0 " + op + " (" + b.s + ") != 0) ? (i64)1 : (i64)0)", false});
+                } else {
+                    st.push_back({"((" + A + " " + op + " " + B + ") ? (i64)1 : (i64)0)", false});
+                }
+            }
+        }
+    }
+    if(st.size() != 1) R HPA_E_DOMAIN;
+    *out = st[0];
+    R 0;
+}
+
+static const char *PREAMBLE = R"(
+typedef long long i64; typedef unsigned long long u64;
+#define NCOL 12
+struct KS { double s, c; };
+__device__ __forceinline__ i64 hdiv(i64 a, i64 b){ return b == 0 ? 0 : (b == -1 ? (i64)(0ULL - (u64)a) : a / b); }
+__device__ __forceinline__ void ks_add(KS &a, double v){
+    double t = a.s + v;
+    a.c += (__builtin_fabs(a.s) >= __builtin_fabs(v)) ? (a.s - t) + v : (v - t) + a.s;
+    a.s = t;
+}
+__device__ __forceinline__ void ks_merge(KS &a, const KS &b){
+    double t = a.s + b.s; double bb = t - a.s;
+    double err = (a.s - (t - bb)) + (b.s - bb);
+    a.s = t; a.c += b.c + err;
+}
+__device__ __forceinline__ KS ks_wave(KS a){
+    for(int o = 32; o > 0; o >>= 1){ KS b; b.s = __shfl_down(a.s, o, 64); b.c = __shfl_down(a.c, o, 64); ks_merge(a, b); }
+    return a;
+}
+)";
+
+static std::string loads_code(int nx, const int32_t *typ, const std::vector<int> &used){
+    std::ostringstream o;
+    for(int k = 0; k < nx; k++) if(used[k]) o << "        const " << ctype(typ[k]) << " v" << k << " = ((const " << ctype(typ[k]) << "*)a.c[" << k << "])[i];\n";
+    R o.str();
+}
+
+
+static const char *COND_TRUE = "((i64)1)";
+
+// ---------------------------------------------------------------- FA reduce (fa_reduce)
+struct JArgs { const void *c[EXPR_MAX_COLS]; long long n; void *out; void *aux; };
+
+
+static int32_t split_used(int nx, const std::vector<int> &uc, const std::vector<int> &ub, std::vector<int> &late){
+    late.assign(nx, 0);
+    for(int k = 0; k < nx; k++) if(ub[k] && !uc[k]) late[k] = 1;
+    R 0;
+}
+
+// FA-reduce through generated code; HPA_E_NOT_IMPL when code generation is unavailable
+int32_t jit_fused_reduce(hpa_ctx *c, int kind, int nx, V *x, const hpa_expr *cond, const hpa_expr *body,
+                         const DevCols &cols, L n, void *part, int G){
+    if(!jit_enabled()) R HPA_E_NOT_IMPL;
+    int32_t typ[EXPR_MAX_COLS]; for(int k = 0; k < nx; k++) typ[k] = x[k]->typ;
+    std::vector<int> uc(nx, 0), ub(nx, 0), late;
+    CExpr ce{COND_TRUE, false}, be;
+    int32_t e;
+    if(cond && (e = expr_to_c(cond, nx, typ, &ce, uc))) R e;
+    if((e = expr_to_c(body, nx, typ, &be, ub))) R e;
+    split_used(nx, uc, ub, late);
+    std::ostringstream s;
+    s << PREAMBLE;
+    s << "struct A { const void *c[" << EXPR_MAX_COLS << "]; i64 n; void *out; void *aux; };\n"
+         "struct P { KS s; u64 i, cnt; double mn, mx; i64 imn, imx; };\n"
+         "extern \"C\" __global__ void __launch_bounds__(256) fa_reduce(A a){\n"
+         "    __shared__ P sh[4];\n"
+         "    P p; p.s.s = 0; p.s.c = 0; p.i = 0; p.cnt = 0; p.mn = 1.7976931348623157e308; p.mx = -1.7976931348623157e308;\n"
+         "    p.imn = 9223372036854775807LL; p.imx = -9223372036854775807LL - 1;\n"
+         "    for(i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < a.n; i += (i64)gridDim.x * 256){\n"
+      << loads_code(nx, typ, uc)
+      << "        if(!(" << ce.s << ")) continue;\n"
+      << loads_code(nx, typ, late);
+    if(be.f) s << "        double v = " << be.s << ";\n        p.cnt++; ks_add(p.s, v); p.mn = v < p.mn ? v : p.mn; p.mx = v > p.mx ? v : p.mx;\n";
+    else s << "        i64 v = " << be.s << ";\n        p.cnt++; p.i += (u64)v; p.imn = v < p.imn ? v : p.imn; p.imx = v > p.imx ? v : p.imx;\n";
+    s << "    }\n"
+         "    for(int o = 32; o > 0; o >>= 1){\n"
+         "        KS b; b.s = __shfl_down(p.s.s, o, 64); b.c = __shfl_down(p.s.c, o, 64); ks_merge(p.s, b);\n"
+         "        p.i += __shfl_down(p.i, o, 64); p.cnt += __shfl_down(p.cnt, o, 64);\n"
+         "        double m1 = __shfl_down(p.mn, o, 64), m2 = __shfl_down(p.mx, o, 64);\n"
+         "        i64 q1 = __shfl_down(p.imn, o, 64), q2 = __shfl_down(p.imx, o, 64);\n"
+         "        p.mn = m1 < p.mn ? m1 : p.mn; p.mx = m2 > p.mx ? m2 : p.mx; p.imn = q1 < p.imn ? q1 : p.imn; p.imx = q2 > p.imx ? q2 : p.imx;\n"
+         "    }\n"
+         "    if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = p;\n"
+         "    __syncthreads();\n"
+         "    if(threadIdx.x == 0){\n"
+         "        P r = sh[0];\n"
+         "        for(int w = 1; w < 4; w++){ P b = sh[w]; ks_merge(r.s, b.s); r.i += b.i; r.cnt += b.cnt;\n"
+         "            r.mn = b.mn < r.mn ? b.mn : r.mn; r.mx = b.mx > r.mx ? b.mx : r.mx; r.imn = b.imn < r.imn ? b.imn : r.imn; r.imx = b.imx > r.imx ? b.imx : r.imx; }\n"
+         "        ((P*)a.out)[blockIdx.x] = r;\n"
+         "    }\n"
+         "}\n";
+    hipFunction_t fn;
+    if((e = jit_get(s.str(), "fa_reduce", &fn))) R e;
+    JArgs a; memset(&a, 0, sizeof(a));
+    for(int k = 0; k < nx; k++) a.c[k] = cols.p[k];
+    a.n = n; a.out = part;
+    double cb = 0; for(int k = 0; k < nx; k++) cb += (double)type_width(typ[k]);
+    ProfScope ps("fused_reduce", c->stream, (double)n * cb);
+    R jit_launch(c, fn, (unsigned)G, 256, a);
+}
+
+// ---------------------------------------------------------------- FE map (fe_map)
+int32_t jit_fused_map(hpa_ctx *c, int otyp, int nx, V *x, const hpa_expr *ex, const DevCols &cols, L n, void *dz){
+    if(!jit_enabled()) R HPA_E_NOT_IMPL;
+    int32_t typ[EXPR_MAX_COLS]; for(int k = 0; k < nx; k++) typ[k] = x[k]->typ;
+    std::vector<int> used(nx, 0);
+    CExpr be; int32_t e;
+    if((e = expr_to_c(ex, nx, typ, &be, used))) R e;
+    const char *ot = otyp == HPA_B ? "unsigned char" : otyp == HPA_I ? "int" : otyp == HPA_L ? "long long" : "double";
+    std::ostringstream s;
+    s << PREAMBLE
+      << "struct A { const void *c[" << EXPR_MAX_COLS << "]; i64 n; void *out; void *aux; };\n"
+         "extern \"C\" __global__ void __launch_bounds__(256) fe_map(A a){\n"
+         "    for(i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < a.n; i += (i64)gridDim.x * 256){\n"
+      << loads_code(nx, typ, used);
+    if(otyp == HPA_B) s << "        ((unsigned char*)a.out)[i] = (unsigned char)((" << be.s << ") != 0);\n";
+    else s << "        ((" << ot << "*)a.out)[i] = (" << ot << ")(" << be.s << ");\n";
+    s << "    }\n}\n";
+    hipFunction_t fn;
+    if((e = jit_get(s.str(), "fe_map", &fn))) R e;
+    JArgs a; memset(&a, 0, sizeof(a));
+    for(int k = 0; k < nx; k++) a.c[k] = cols.p[k];
+    a.n = n; a.out = dz;
+    double cb = 0; for(int k = 0; k < nx; k++) if(used[k]) cb += (double)type_width(typ[k]);
+    ProfScope ps("fused_map", c->stream, (double)n * (cb + type_width(otyp)));
+    R jit_launch(c, fn, (unsigned)grid_for(n, 256, 4), 256, a);
+}
+
+// ---------------------------------------------------------------- fused group + aggregates (fga)
+// Accumulator classes: sums (and avgs, which share the sum of the same body), mins, maxes.
+struct Acc { std::string body; bool f; int cls; };   // cls 0 sum, 2 min, 3 max
+
+static std::string fga_source(int nk, const int32_t *ktyp, int nx, const int32_t *typ, const CExpr &ce,
+                              const std::vector<int> &uc, const std::vector<int> &late, const std::vector<Acc> &acc){
+    const int NA = (int)acc.size() > 0 ? (int)acc.size() : 1;
+    std::ostringstream s;
+    s << PREAMBLE;
+    s << "#define NS " << FGA_NS << "\n#define NA " << NA << "\n#define HT 64\n";
+    s << "struct A { const void *c[" << EXPR_MAX_COLS << "]; const void *k0; const void *k1; i64 n; void *part; unsigned *used; int *ovf; };\n"
+         "struct P { unsigned code, valid; u64 cnt; i64 first; double f[2 * NA]; i64 q[NA]; };\n"
+         // LDS hash of key code -> slot: entry = code << 8 | slot; 0xFFFFFFFF empty; slot 0xFE busy, 0xFD overflow
+         "__device__ __forceinline__ unsigned slot_of(unsigned *ht, unsigned *s_code, unsigned *s_used, unsigned code){\n"
+         "    unsigned h = ((code * 2654435761u) >> 26) & (HT - 1);\n"
+         "    unsigned probes = 0;\n"
+         "    while(true){\n"
+         "        unsigned e = __hip_atomic_load(&ht[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);\n"
+         "        if(e == 0xFFFFFFFFu){\n"
+         "            unsigned want = (code << 8) | 0xFEu;\n"
+         "            unsigned prev = atomicCAS(&ht[h], 0xFFFFFFFFu, want);\n"
+         "            if(prev == 0xFFFFFFFFu){\n"
+         "                unsigned n = atomicAdd(s_used, 1u);\n"
+         "                unsigned sl = n < NS ? n : 0xFDu;\n"
+         "                if(n < NS) s_code[n] = code;\n"
+         "                __threadfence_block();\n"
+         "                atomicExch(&ht[h], (code << 8) | sl);\n"
+         "                return sl;\n"
+         "            }\n"
+         "            continue;\n"
+         "        }\n"
+         "        if((e >> 8) == code){\n"
+         "            unsigned sl = e & 0xFFu;\n"
+         "            if(sl == 0xFEu){ __builtin_amdgcn_s_sleep(1); continue; }\n"
+         "            return sl;\n"
+         "        }\n"
+         "        h = (h + 1) & (HT - 1);\n"
+         "        if(++probes >= HT) return 0xFDu;      // table full: more codes than the kernel keeps\n"
+         "    }\n"
+         "}\n"
+         "extern \"C\" __global__ void __launch_bounds__(256) fga(A a){\n"
+         "    __shared__ int s_ovf;\n"
+         "    __shared__ unsigned ht[HT];\n"
+         "    __shared__ unsigned s_code[NS];\n"
+         "    __shared__ unsigned s_used;\n"
+         "    __shared__ double shf[4][2 * NA + 1];\n"
+         "    __shared__ i64 shq[4][NA + 2];\n"
+         "    if(threadIdx.x < HT) ht[threadIdx.x] = 0xFFFFFFFFu;\n"
+         "    if(threadIdx.x == 0){ s_used = 0; s_ovf = 0; }\n"
+         "    __syncthreads();\n";
+    // accumulators
+    for(int q = 0; q < FGA_NS; q++){
+        s << "    u64 cnt" << q << " = 0; i64 fst" << q << " = 9223372036854775807LL;";
+        for(int k = 0; k < (int)acc.size(); k++){
+            const Acc &A = acc[k];
+            const char *init = A.cls == 0 ? "0" : (A.f ? (A.cls == 2 ? "1.7976931348623157e308" : "-1.7976931348623157e308")
+                                                       : (A.cls == 2 ? "9223372036854775807LL" : "(-9223372036854775807LL-1)"));
+            s << (A.f ? " double" : " i64") << " a" << k << "_" << q << " = " << init << ";";
+        }
+        s << "\n";
+    }
+    s << "    int ovf = 0;\n"
+         "    for(i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < a.n; i += (i64)gridDim.x * 256){\n"
+         "        if(__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)){ ovf = 1; break; }\n"
+      << loads_code(nx, typ, uc)
+      << "        if(!(" << ce.s << ")) continue;\n";
+    if(nk == 2) s << "        unsigned code = ((unsigned)((const unsigned char*)a.k0)[i] << 8) | (unsigned)((const unsigned char*)a.k1)[i];\n";
+    else s << "        unsigned code = (unsigned)((const unsigned char*)a.k0)[i];\n";
+    s << "        unsigned sl = slot_of(ht, s_code, &s_used, code);\n"
+         "        if(sl >= NS){ ovf = 1; __hip_atomic_store(&s_ovf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); break; }\n"
+      << loads_code(nx, typ, late);
+    for(int k = 0; k < (int)acc.size(); k++) s << "        const " << (acc[k].f ? "double" : "i64") << " b" << k << " = " << acc[k].body << ";\n";
+    s << "        switch(sl){\n";
+    for(int q = 0; q < FGA_NS; q++){
+        s << "        case " << q << ": cnt" << q << "++; fst" << q << " = i < fst" << q << " ? i : fst" << q << ";";
+        for(int k = 0; k < (int)acc.size(); k++){
+            std::string v = "a" + std::to_string(k) + "_" + std::to_string(q), b = "b" + std::to_string(k);
+            if(acc[k].cls == 0) s << (acc[k].f ? " " + v + " += " + b + ";" : " " + v + " = (i64)((u64)" + v + " + (u64)" + b + ");");
+            else if(acc[k].cls == 2) s << " " << v << " = " << b << " < " << v << " ? " << b << " : " << v << ";";
+            else s << " " << v << " = " << b << " > " << v << " ? " << b << " : " << v << ";";
+        }
+        s << " break;\n";
+    }
+    s << "        }\n    }\n"
+         "    if(ovf) atomicOr(a.ovf, 1);\n"
+         "    __syncthreads();\n"
+         "    const unsigned used = s_used < NS ? s_used : NS;\n"
+         "    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;\n"
+         "    P *out = (P*)a.part + (i64)blockIdx.x * NS;\n";
+    // per-slot block reduction, fixed order
+    for(int q = 0; q < FGA_NS; q++){
+        s << "    if(" << q << " < used){\n"
+             "        u64 c = cnt" << q << "; i64 f = fst" << q << ";\n"
+             "        for(int o = 32; o > 0; o >>= 1){ c += __shfl_down(c, o, 64); i64 t = __shfl_down(f, o, 64); f = t < f ? t : f; }\n";
+        for(int k = 0; k < (int)acc.size(); k++){
+            std::string v = "a" + std::to_string(k) + "_" + std::to_string(q);
+            if(acc[k].f && acc[k].cls == 0)
+                s << "        KS k" << k << "; k" << k << ".s = " << v << "; k" << k << ".c = 0; k" << k << " = ks_wave(k" << k << ");\n";
+            else if(acc[k].f)
+                s << "        double k" << k << " = " << v << "; for(int o = 32; o > 0; o >>= 1){ double t = __shfl_down(k" << k << ", o, 64); k" << k << " = "
+                  << (acc[k].cls == 2 ? "t < k" : "t > k") << k << " ? t : k" << k << "; }\n";
+            else if(acc[k].cls == 0)
+                s << "        u64 k" << k << " = (u64)" << v << "; for(int o = 32; o > 0; o >>= 1) k" << k << " += __shfl_down(k" << k << ", o, 64);\n";
+            else
+                s << "        i64 k" << k << " = " << v << "; for(int o = 32; o > 0; o >>= 1){ i64 t = __shfl_down(k" << k << ", o, 64); k" << k << " = "
+                  << (acc[k].cls == 2 ? "t < k" : "t > k") << k << " ? t : k" << k << "; }\n";
+        }
+        s << "        if(lane == 0){ shq[w][NA] = (i64)c; shq[w][NA + 1] = f;";
+        for(int k = 0; k < (int)acc.size(); k++){
+            if(acc[k].f && acc[k].cls == 0) s << " shf[w][2*" << k << "] = k" << k << ".s; shf[w][2*" << k << "+1] = k" << k << ".c;";
+            else if(acc[k].f) s << " shf[w][2*" << k << "] = k" << k << ";";
+            else s << " shq[w][" << k << "] = (i64)k" << k << ";";
+        }
+        s << " }\n        __syncthreads();\n"
+             "        if(threadIdx.x == 0){\n"
+             "            P r; r.code = s_code[" << q << "]; r.valid = 1; r.cnt = 0; r.first = 9223372036854775807LL;\n"
+             "            for(int ww = 0; ww < 4; ww++){ r.cnt += (u64)shq[ww][NA]; r.first = shq[ww][NA + 1] < r.first ? shq[ww][NA + 1] : r.first; }\n";
+        for(int k = 0; k < (int)acc.size(); k++){
+            if(acc[k].f && acc[k].cls == 0)
+                s << "            { KS t; t.s = 0; t.c = 0; for(int ww = 0; ww < 4; ww++){ KS u; u.s = shf[ww][2*" << k << "]; u.c = shf[ww][2*" << k << "+1]; ks_merge(t, u); } r.f[2*" << k << "] = t.s; r.f[2*" << k << "+1] = t.c; }\n";
+            else if(acc[k].f)
+                s << "            { double t = shf[0][2*" << k << "]; for(int ww = 1; ww < 4; ww++){ double u = shf[ww][2*" << k << "]; t = " << (acc[k].cls == 2 ? "u < t" : "u > t") << " ? u : t; } r.f[2*" << k << "] = t; }\n";
+            else if(acc[k].cls == 0)
+                s << "            { u64 t = 0; for(int ww = 0; ww < 4; ww++) t += (u64)shq[ww][" << k << "]; r.q[" << k << "] = (i64)t; }\n";
+            else
+                s << "            { i64 t = shq[0][" << k << "]; for(int ww = 1; ww < 4; ww++){ i64 u = shq[ww][" << k << "]; t = " << (acc[k].cls == 2 ? "u < t" : "u > t") << " ? u : t; } r.q[" << k << "] = t; }\n";
+        }
+        s << "            out[" << q << "] = r;\n        }\n        __syncthreads();\n    }\n";
+    }
+    s << "    if(threadIdx.x == 0) a.used[blockIdx.x] = used;\n}\n";
+
+    // merge kernel: one block of 1024 threads
+    s << "struct M { const void *part; const unsigned *used; int G; unsigned *codes; int *g; u64 *cnt; i64 *first; double *f; i64 *q; };\n"
+         "extern \"C\" __global__ void __launch_bounds__(1024) fga_merge(M m){\n"
+         "    __shared__ unsigned bits[2048];\n"
+         "    __shared__ unsigned pre[1024];\n"
+         "    const P *part = (const P*)m.part;\n"
+         "    for(int k = threadIdx.x; k < 2048; k += 1024) bits[k] = 0;\n"
+         "    __syncthreads();\n"
+         "    for(int b = threadIdx.x; b < m.G; b += 1024){ unsigned u = m.used[b]; for(unsigned s = 0; s < u; s++){ unsigned cd = part[(i64)b * NS + s].code; atomicOr(&bits[cd >> 5], 1u << (cd & 31)); } }\n"
+         "    __syncthreads();\n"
+         "    unsigned c0 = __popc(bits[2 * threadIdx.x]), c1 = __popc(bits[2 * threadIdx.x + 1]);\n"
+         "    pre[threadIdx.x] = c0 + c1;\n"
+         "    __syncthreads();\n"
+         "    for(int o = 1; o < 1024; o <<= 1){ unsigned t = threadIdx.x >= (unsigned)o ? pre[threadIdx.x - o] : 0; __syncthreads(); pre[threadIdx.x] += t; __syncthreads(); }\n"
+         "    unsigned pos = pre[threadIdx.x] - c0 - c1;\n"
+         "    for(int h = 0; h < 2; h++){ unsigned wd = bits[2 * threadIdx.x + h]; while(wd){ int bt = __ffs(wd) - 1; wd &= wd - 1; m.codes[pos++] = (unsigned)((2 * threadIdx.x + h) * 32 + bt); } }\n"
+         "    const int g = (int)pre[1023];\n"
+         "    if(threadIdx.x == 0) *m.g = g;\n"
+         "    __syncthreads();\n"
+         "    __threadfence_block();\n"
+         "    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;\n"
+         "    for(int r = wv; r < g; r += 16){\n"
+         "        const unsigned code = m.codes[r];\n"
+         "        u64 c = 0; i64 fs = 9223372036854775807LL;\n";
+    for(int k = 0; k < (int)acc.size(); k++){
+        if(acc[k].f && acc[k].cls == 0) s << "        KS t" << k << "; t" << k << ".s = 0; t" << k << ".c = 0;\n";
+        else if(acc[k].f) s << "        double t" << k << " = " << (acc[k].cls == 2 ? "1.7976931348623157e308" : "-1.7976931348623157e308") << ";\n";
+        else if(acc[k].cls == 0) s << "        u64 t" << k << " = 0;\n";
+        else s << "        i64 t" << k << " = " << (acc[k].cls == 2 ? "9223372036854775807LL" : "(-9223372036854775807LL-1)") << ";\n";
+    }
+    s << "        for(int b = lane; b < m.G; b += 64){\n"
+         "            unsigned u = m.used[b];\n"
+         "            for(unsigned sidx = 0; sidx < u; sidx++){\n"
+         "                const P &p = part[(i64)b * NS + sidx];\n"
+         "                if(p.code != code) continue;\n"
+         "                c += p.cnt; fs = p.first < fs ? p.first : fs;\n";
+    for(int k = 0; k < (int)acc.size(); k++){
+        if(acc[k].f && acc[k].cls == 0) s << "                { KS u; u.s = p.f[2*" << k << "]; u.c = p.f[2*" << k << "+1]; ks_merge(t" << k << ", u); }\n";
+        else if(acc[k].f) s << "                t" << k << " = p.f[2*" << k << "] " << (acc[k].cls == 2 ? "<" : ">") << " t" << k << " ? p.f[2*" << k << "] : t" << k << ";\n";
+        else if(acc[k].cls == 0) s << "                t" << k << " += (u64)p.q[" << k << "];\n";
+        else s << "                t" << k << " = p.q[" << k << "] " << (acc[k].cls == 2 ? "<" : ">") << " t" << k << " ? p.q[" << k << "] : t" << k << ";\n";
+    }
+    s << "            }\n        }\n"
+         "        for(int o = 32; o > 0; o >>= 1){\n"
+         "            c += __shfl_down(c, o, 64); { i64 t = __shfl_down(fs, o, 64); fs = t < fs ? t : fs; }\n";
+    for(int k = 0; k < (int)acc.size(); k++){
+        if(acc[k].f && acc[k].cls == 0) s << "            { KS u; u.s = __shfl_down(t" << k << ".s, o, 64); u.c = __shfl_down(t" << k << ".c, o, 64); ks_merge(t" << k << ", u); }\n";
+        else if(acc[k].f) s << "            { double u = __shfl_down(t" << k << ", o, 64); t" << k << " = u " << (acc[k].cls == 2 ? "<" : ">") << " t" << k << " ? u : t" << k << "; }\n";
+        else if(acc[k].cls == 0) s << "            t" << k << " += __shfl_down(t" << k << ", o, 64);\n";
+        else s << "            { i64 u = __shfl_down(t" << k << ", o, 64); t" << k << " = u " << (acc[k].cls == 2 ? "<" : ">") << " t" << k << " ? u : t" << k << "; }\n";
+    }
+    s << "        }\n"
+         "        if(lane == 0){\n"
+         "            m.cnt[r] = c; m.first[r] = fs;\n";
+    for(int k = 0; k < (int)acc.size(); k++){
+        if(acc[k].f && acc[k].cls == 0) s << "            m.f[(i64)r * NA + " << k << "] = t" << k << ".s + t" << k << ".c;\n";
+        else if(acc[k].f) s << "            m.f[(i64)r * NA + " << k << "] = t" << k << ";\n";
+        else s << "            m.q[(i64)r * NA + " << k << "] = (i64)t" << k << ";\n";
+    }
+    s << "        }\n    }\n}\n";
+    (void)ktyp;
+    R s.str();
+}
+
+// outputs of the merge are laid out [group][accumulator]; this kernel makes typed vectors
+struct FgaOut { int na, stride, nk; int kind[16]; int accid[16]; int accf[16]; void *out[16]; uint8_t *key0, *key1; int64_t *first; };
+__global__ void k_fga_emit(const int *gp, const unsigned *codes, const unsigned long long *cnt, const long long *first,
+                           const double *f, const long long *q, FgaOut o){
+    const int g = *gp;
+    for(int r = blockIdx.x * blockDim.x + threadIdx.x; r < g; r += gridDim.x * blockDim.x){
+        unsigned cd = codes[r];
+        if(o.nk == 2){ o.key0[r] = (uint8_t)(cd >> 8); o.key1[r] = (uint8_t)(cd & 255); }
+        else o.key0[r] = (uint8_t)cd;
+        o.first[r] = first[r];
+        for(int a = 0; a < o.na; a++){
+            long long at = (long long)r * o.stride + o.accid[a];
+            switch(o.kind[a]){
+                case 4: ((int64_t*)o.out[a])[r] = (int64_t)cnt[r]; break;
+                case 1:      // avg = sum / len (pfnAvg, h_primitive.cc:788-800)
+                    if(o.accf[a]) ((double*)o.out[a])[r] = f[at] / (double)cnt[r];
+                    else ((double*)o.out[a])[r] = (double)q[at] * 1.0 / (double)cnt[r];
+                    break;
+                default:
+                    if(o.accf[a]) ((double*)o.out[a])[r] = f[at];
+                    else ((int64_t*)o.out[a])[r] = q[at];
+            }
+        }
+    }
+}
+
+static void set_dev_vec(hpa_ctx *c, V z, int32_t typ, L len, void *p){
+    z->typ = typ; z->len = len;
+    if(len <= 1){ z->g = (char*)&z->b; z->i64 = 0; if(len == 1) d2h(c, &z->b, p, type_width(typ)); }
+    else z->g = (char*)p;
+}
+
+// hpaFusedGroupAgg: see the public header.  Returns HPA_E_NOT_IMPL (nothing written) when a
+// block meets more distinct key codes than it has slots; the caller then runs the unfused plan.
+extern "C" int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_expr *cond,
+                                    int32_t na, const hpa_agg *aggs, hpa_V zfirst, hpa_V *zkeys, hpa_V *zaggs){
+    hpa_ctx *c = ctx_cur();
+    if(nk < 1 || nk > 2 || na < 1 || na > 16 || nx < 0 || nx > EXPR_MAX_COLS) R HPA_E_DOMAIN;
+    if(!jit_enabled()) R HPA_E_NOT_IMPL;
+    L n = keys[0]->len;
+    for(int k = 0; k < nk; k++){
+        int32_t t = keys[k]->typ;
+        if(!(t == HPA_C || t == HPA_B || t == HPA_J)) R HPA_E_TYPE;
+        if(keys[k]->len != n) R HPA_E_LENGTH;
+    }
+    DevCols cols; L nn; int32_t e;
+    L mark = c->dcur;
+    if((e = cols_view(c, nx, x, &cols, &nn))) R e;
+    if(nx > 0 && nn != n) R HPA_E_LENGTH;
+    int32_t typ[EXPR_MAX_COLS]; for(int k = 0; k < nx; k++) typ[k] = x[k]->typ;
+    std::vector<int> uc(nx, 0), ub(nx, 0), late;
+    CExpr ce{COND_TRUE, false};
+    if(cond && (e = expr_to_c(cond, nx, typ, &ce, uc))) R e;
+    std::vector<Acc> acc;
+    FgaOut o; memset(&o, 0, sizeof(o));
+    o.na = na; o.nk = nk;
+    for(int a = 0; a < na; a++){
+        int kd = aggs[a].kind;
+        if(kd < 0 || kd > 4) R HPA_E_DOMAIN;
+        o.kind[a] = kd; o.accid[a] = 0; o.accf[a] = 0;
+        if(kd == 4) continue;
+        CExpr be; if((e = expr_to_c(aggs[a].body, nx, typ, &be, ub))) R e;
+        int cls = (kd == 0 || kd == 1) ? 0 : kd;
+        int id = -1;
+        for(int k = 0; k < (int)acc.size(); k++) if(acc[k].body == be.s && acc[k].cls == cls){ id = k; break; }
+        if(id < 0){ id = (int)acc.size(); acc.push_back({be.s, be.f, cls}); }
+        o.accid[a] = id; o.accf[a] = be.f ? 1 : 0;
+    }
+    split_used(nx, uc, ub, late);
+    std::string src = fga_source(nk, nullptr, nx, typ, ce, uc, late, acc);
+    hipFunction_t f1, f2;
+    if((e = jit_get(src, "fga", &f1)) || (e = jit_get(src, "fga_merge", &f2))) R e;
+    const int NA = acc.empty() ? 1 : (int)acc.size();
+    o.stride = NA;
+    const size_t psize = 8 + 8 + 8 + 16 * NA + 8 * NA;     // struct P
+    int G = (int)ceil_div(n, 256 * 64);
+    if(G > num_cus() * 4) G = num_cus() * 4;
+    if(G < 1) G = 1;
+    const void *pk0 = nullptr, *pk1 = nullptr;
+    if(n > 0){ if((e = dev_view(c, keys[0], &pk0))) R e; if(nk == 2 && (e = dev_view(c, keys[1], &pk1))) R e; }
+    void *part = dalloc(c, (L)psize * FGA_NS * G);
+    unsigned *used = (unsigned*)dalloc(c, 4 * G);
+    int *ovf = (int*)dalloc(c, 16);
+    const L GM = (L)G * FGA_NS;                              // groups cannot exceed the partial slots
+    unsigned *codes = (unsigned*)dalloc(c, 4 * GM);
+    int *gdev = (int*)dalloc(c, 16);
+    unsigned long long *cnt = (unsigned long long*)dalloc(c, 8 * GM);
+    long long *first = (long long*)dalloc(c, 8 * GM);
+    double *fo = (double*)dalloc(c, 8 * GM * NA);
+    long long *qo = (long long*)dalloc(c, 8 * GM * NA);
+    if(!part || !used || !ovf || !codes || !gdev || !cnt || !first || !fo || !qo){ c->dcur = mark; R HPA_E_MEMORY; }
+    hipMemsetAsync(ovf, 0, 4, c->stream);
+    struct { const void *c[EXPR_MAX_COLS]; const void *k0; const void *k1; long long n; void *part; unsigned *used; int *ovf; } ga;
+    memset(&ga, 0, sizeof(ga));
+    for(int k = 0; k < nx; k++) ga.c[k] = cols.p[k];
+    ga.k0 = pk0; ga.k1 = pk1; ga.n = n; ga.part = part; ga.used = used; ga.ovf = ovf;
+    {
+        double cb = nk; for(int k = 0; k < nx; k++) if(uc[k] || ub[k]) cb += (double)type_width(typ[k]);
+        ProfScope ps("fused_group_agg", c->stream, (double)n * cb);
+        if((e = jit_launch(c, f1, (unsigned)G, 256, ga))) R e;
+    }
+    struct { const void *part; const unsigned *used; int G; unsigned *codes; int *g; unsigned long long *cnt; long long *first; double *f; long long *q; } ma;
+    ma.part = part; ma.used = used; ma.G = G; ma.codes = codes; ma.g = gdev; ma.cnt = cnt; ma.first = first; ma.f = fo; ma.q = qo;
+    if((e = jit_launch(c, f2, 1, 1024, ma))) R e;
+    int hv[2] = {0, 0};
+    if((e = d2h(c, &hv[0], ovf, 4)) || (e = d2h(c, &hv[1], gdev, 4))) R e;
+    if(hv[0]){ c->dcur = mark; R HPA_E_NOT_IMPL; }
+    const L g = hv[1];
+    uint8_t *k0 = (uint8_t*)dalloc(c, g > 0 ? g : 1), *k1 = (uint8_t*)dalloc(c, g > 0 ? g : 1);
+    int64_t *zf = (int64_t*)dalloc(c, 8 * (g > 0 ? g : 1));
+    if(!k0 || !k1 || !zf) R HPA_E_MEMORY;
+    o.key0 = k0; o.key1 = k1; o.first = zf;
+    for(int a = 0; a < na; a++){ o.out[a] = dalloc(c, 8 * (g > 0 ? g : 1)); if(!o.out[a]) R HPA_E_MEMORY; }
+    if(g > 0) hipLaunchKernelGGL(k_fga_emit, dim3(grid_for(g, 64, 1)), dim3(64), 0, c->stream, gdev, codes, cnt, first, fo, qo, o);
+    if((e = check_launch("fga_emit"))) R e;
+    set_dev_vec(c, zkeys[0], keys[0]->typ, g, k0);
+    if(nk == 2) set_dev_vec(c, zkeys[1], keys[1]->typ, g, k1);
+    if(zfirst) set_dev_vec(c, zfirst, HPA_L, g, zf);
+    for(int a = 0; a < na; a++){
+        int kd = aggs[a].kind;
+        int32_t t = kd == 4 ? HPA_L : kd == 1 ? HPA_E : (o.accf[a] ? HPA_E : HPA_L);
+        set_dev_vec(c, zaggs[a], t, g, o.out[a]);
+    }
+    R 0;
+}
+
+// Diagnostic for hosts without a GPU: emit the generated sources of one fused group-aggregate,
+// one FA reduction and one FE map for the given expressions and compile them with hipRTC.
+extern "C" int32_t hpaJitCompileCheck(int32_t nx, const int32_t *coltyp, const hpa_expr *cond,
+                                      int32_t na, const hpa_agg *aggs){
+    if(nx < 0 || nx > EXPR_MAX_COLS || na < 1 || na > 16) R HPA_E_DOMAIN;
+    int32_t typ[EXPR_MAX_COLS]; for(int k = 0; k < nx; k++) typ[k] = coltyp[k];
+    std::vector<int> uc(nx, 0), ub(nx, 0), late;
+    CExpr ce{COND_TRUE, false};
+    int32_t e;
+    if(cond && (e = expr_to_c(cond, nx, typ, &ce, uc))) R e;
+    std::vector<Acc> acc;
+    for(int a = 0; a < na; a++){
+        if(aggs[a].kind == 4) continue;
+        CExpr be; if((e = expr_to_c(aggs[a].body, nx, typ, &be, ub))) R e;
+        int cls = aggs[a].kind <= 1 ? 0 : aggs[a].kind;
+        bool dup = false; for(auto &q : acc) if(q.body == be.s && q.cls == cls) dup = true;
+        if(!dup) acc.push_back({be.s, be.f, cls});
+    }
+    split_used(nx, uc, ub, late);
+    std::string src = fga_source(2, nullptr, nx, typ, ce, uc, late, acc);
+    g_compile_only = true;
+    hipFunction_t f;
+    e = jit_get(src, "fga", &f);
+    g_compile_only = false;
+    R e;
+}

@@ -646,9 +646,9 @@ extern "C" int32_t hpaFusedMap(hpa_V z, int32_t typ, int32_t nx, hpa_V *x, const
     if((e = expr_compile(ex, nx, ct, &de))) R e;
     void *dz;
     if((e = out_init(c, z, typ, n, &dz))) R e;
-    if(n > 0){
+    if(n > 0 && jit_fused_map(c, typ, nx, x, ex, cols, n, dz) != 0){
         double cb = 0; for(int k = 0; k < nx; k++) cb += (double)type_width(x[k]->typ);
-        ProfScope ps("fused_map", c->stream, (double)n * (cb + type_width(typ)));
+        ProfScope ps("fused_map_interp", c->stream, (double)n * (cb + type_width(typ)));
         dim3 g(grid_for(n, 256, 4)), b(256);
         switch(typ){
             case HPA_B: hipLaunchKernelGGL(k_fused_map<HPA_B>, g, b, 0, c->stream, de, cols, n, dz); break;
@@ -726,9 +726,10 @@ extern "C" int32_t hpaFusedReduce(hpa_V z, int32_t kind, int32_t nx, hpa_V *x, c
     int G = sum_grid(n);
     FaPart *part = (FaPart*)dalloc(c, sizeof(FaPart) * G), *out = (FaPart*)dalloc(c, sizeof(FaPart));
     if(!part || !out) R HPA_E_MEMORY;
-    {
+    if(jit_fused_reduce(c, kind, nx, x, cond, body, cols, n, part, G) != 0){
+        // generated code unavailable: interpret the expressions on the device
         double cb = 0; for(int k = 0; k < nx; k++) cb += (double)type_width(x[k]->typ);
-        ProfScope ps("fused_reduce", c->stream, (double)n * cb);
+        ProfScope ps("fused_reduce_interp", c->stream, (double)n * cb);
         hipLaunchKernelGGL(k_fused_reduce, dim3(G), dim3(256), 0, c->stream, ce, cond != nullptr, be, cols, n, part);
     }
     hipLaunchKernelGGL(k_fused_reduce_final, dim3(1), dim3(64), 0, c->stream, part, G, out);

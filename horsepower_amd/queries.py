@@ -119,6 +119,36 @@ def q1_fused(B, T, ship_max=D.Q1_SHIP_MAX):
     return dict(zip(Q1_NAMES, (B.read(v) for v in out)))
 
 
+def q1_chf_spec(ship_max=D.Q1_SHIP_MAX):
+    """Columns, condition and aggregates of the one-pass Q1 (x = ship_day, quantity, price,
+    discount, tax).  disc_price = price*(1-discount), charge = (price*(1-discount))*(1+tax),
+    the association of q1.hir's t36 and t40."""
+    cond = expr([("COL", 0), ("I", ship_max), "LEQ"])
+    qty = expr([("COL", 1)])
+    price = expr([("COL", 2)])
+    disc = expr([("COL", 3)])
+    dprice = expr([("COL", 2), ("I", 1), ("COL", 3), "SUB", "MUL"])
+    charge = expr([("COL", 2), ("I", 1), ("COL", 3), "SUB", "MUL", ("I", 1), ("COL", 4), "ADD", "MUL"])
+    spec = [("sum", qty), ("sum", price), ("sum", dprice), ("sum", charge),
+            ("avg", qty), ("avg", price), ("avg", disc), ("len", None)]
+    return cond, spec
+
+
+def q1_chf(B, T, ship_max=D.Q1_SHIP_MAX):
+    """Q1 in one pass over the table (filter, two-char group, eight aggregates), then the
+    order by (flag, status) of the small result.  Falls back to q1_fused when the key
+    cardinality exceeds the fused kernel's per-workgroup slots."""
+    cond, spec = q1_chf_spec(ship_max)
+    xs = [T["ship_day"], T["quantity"], T["price"], T["discount"], T["tax"]]
+    r = B.fused_group_agg([T["return_flag"], T["line_status"]], xs, cond, spec)
+    if r is None:
+        return q1_fused(B, T, ship_max)
+    _, (kf, ks), aggs = r
+    perm = B.pfnOrderBy(B.list([kf, ks]), B.bools([1, 1]))
+    out = [B.pfnIndex(v, perm) for v in [kf, ks] + aggs]
+    return dict(zip(Q1_NAMES, (B.read(v) for v in out)))
+
+
 # ------------------------------------------------------------------------------------- Q3
 Q3_NAMES = ("orderkey", "revenue", "order_day")
 

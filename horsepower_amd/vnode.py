@@ -55,6 +55,26 @@ class Expr(C.Structure):
     _fields_ = [("n", C.c_int32), ("_pad", C.c_int32), ("code", C.POINTER(Ins))]
 
 
+class Agg(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("_pad", C.c_int32), ("body", C.POINTER(Expr))]
+
+
+AGG_KIND = dict(sum=0, avg=1, min=2, max=3, len=4)
+
+
+def aggs(spec):
+    """[(kind, expr-or-None), ...] -> ctypes array of Agg (expressions kept alive)."""
+    arr = (Agg * len(spec))()
+    keep = []
+    for k, (kind, e) in enumerate(spec):
+        arr[k].kind = AGG_KIND[kind]
+        if e is not None:
+            keep.append(e)
+            arr[k].body = C.pointer(e)
+    arr._keep = keep
+    return arr
+
+
 def expr(code):
     """Build a postfix expression: items are ('COL', k) / ('I', v) / ('F', v) / 'ADD' ..."""
     ins = (Ins * len(code))()

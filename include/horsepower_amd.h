@@ -231,6 +231,20 @@ int32_t hpaFusedReduce(hpa_V z, int32_t kind, int32_t nx, hpa_V *x,
  * one mask evaluation, order preserving. */
 int32_t hpaFusedCompress(int32_t ny, hpa_V *z, hpa_V *y, int32_t nx, hpa_V *x,
                          const hpa_expr *cond);
+/* Fused filter + group + aggregates in one pass over the table: the FA filter, pfnGroupTrie
+ * on one or two byte-wide key columns (h_primitive.cc:2656-2681) and the FP grouped reductions
+ * (optimizer/pattern.cc:180-219) of the reference fused into a single scan.  Groups come out in
+ * ascending key-code order as pfnGroupTrie orders them; zkeys[k] holds key column k per group,
+ * zfirst the first row of each group, zaggs[a] aggregate a (kind 0 sum, 1 avg, 2 min, 3 max,
+ * 4 len; body unused for len).  Returns HPA_E_NOT_IMPL without writing outputs when one
+ * workgroup meets more than 8 distinct key codes (the caller then runs the unfused primitives). */
+typedef struct hpa_agg { int32_t kind; int32_t _pad; const hpa_expr *body; } hpa_agg;
+int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_expr *cond,
+                         int32_t na, const hpa_agg *aggs, hpa_V zfirst, hpa_V *zkeys, hpa_V *zaggs);
+/* Diagnostic (no device needed): generate the fused group-aggregate kernel for these column
+ * types and expressions and compile it with hipRTC; 0 when the generated code compiles. */
+int32_t hpaJitCompileCheck(int32_t nx, const int32_t *coltyp, const hpa_expr *cond,
+                           int32_t na, const hpa_agg *aggs);
 /* Peephole compress with a ready mask: z[k] = compress(mask, y[k]). */
 int32_t hpaCompressMany(int32_t ny, hpa_V *z, hpa_V mask, hpa_V *y);
 /* FP1/FP2/FP5/FP6/FP7 grouped reductions (optimizer/pattern.cc:180-219): for each group list

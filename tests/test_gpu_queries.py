@@ -53,3 +53,31 @@ def test_q3(gpu, orc, nd):
     assert g["rows"] == o["rows"] and g["rows"] > 0
     for k in Q.Q3_NAMES:
         close(g[k], o[k], k)
+
+
+@pytest.mark.parametrize("n", [3000, 1_000_003])
+def test_q1_one_pass(gpu, orc, n):
+    t = D.gen_q1(n)
+    o = Q.q1_plain(orc, Q.load_table(orc, t))
+    g = Q.q1_chf(gpu, Q.load_table(gpu, t))
+    for k in Q.Q1_NAMES:
+        close(g[k], o[k], k)
+
+
+def test_fused_group_agg_overflow_and_single_key(gpu, orc):
+    from horsepower_amd.vnode import H_C, expr
+    r = np.random.default_rng(5)
+    n = 200_000
+    k0 = r.integers(0, 200, n).astype(np.uint8)           # > 8 codes per workgroup: overflow
+    v = np.round(r.uniform(0, 100, n), 2)
+    res = gpu.fused_group_agg([gpu.vec(k0, H_C)], [gpu.vec(v)], None, [("sum", expr([("COL", 0)]))])
+    assert res is None
+    k1 = r.choice(np.frombuffer(b"xyz", np.uint8), n)
+    zf, (zk,), (zs, zc) = gpu.fused_group_agg([gpu.vec(k1, H_C)], [gpu.vec(v)], None,
+                                              [("sum", expr([("COL", 0)])), ("len", None)])
+    keys, counts, flat = orc.read_groups(orc.pfnGroup(orc.vec(k1, H_C)))
+    assert gpu.read(zk).tolist() == sorted(set(k1.tolist()))
+    assert gpu.read(zf).tolist() == keys.tolist()
+    assert gpu.read(zc).tolist() == counts.tolist()
+    sums = [math.fsum(v[g]) for g in np.split(flat, np.cumsum(counts)[:-1])]
+    assert np.allclose(gpu.read(zs), sums, rtol=RTOL, atol=0)
