@@ -399,21 +399,26 @@ extern "C" int32_t pfnIndex(hpa_V z, hpa_V x, hpa_V y){
         if((e = promote_dev(c, y, HPA_L, &py))) R e;
         const void *px = nullptr;
         if(nx > 0 && (e = dev_view(c, x, &px))) R e;
-        int *bad = (int*)dalloc(c, 16); if(!bad) R HPA_E_MEMORY;
-        hipMemsetAsync(bad, 0, 4, c->stream);
+        // index vectors made here (order, group, join, where) are known to be in range
+        const bool known = y->typ == HPA_L && within_bound(c, py, nx);
+        int *bad = nullptr;
+        if(!known){ bad = (int*)dalloc(c, 16); if(!bad) R HPA_E_MEMORY; hipMemsetAsync(bad, 0, 4, c->stream); }
         void *dst = dalloc(c, type_bytes(x->typ, nz > 0 ? nz : 1)); if(!dst) R HPA_E_MEMORY;
         if(nz > 0 && nx == 0){ c->dcur = mark; R HPA_E_INDEX; }
         if((e = gather_checked(c, dst, px, (const int64_t*)py, nz, nx, w, bad))) R e;
-        int h = 0; if((e = d2h(c, &h, bad, 4))) R e;
-        if(h){ c->dcur = mark; R HPA_E_INDEX; }
+        if(!known){
+            int h = 0; if((e = d2h(c, &h, bad, 4))) R e;
+            if(h){ c->dcur = mark; R HPA_E_INDEX; }
+        }
         z->typ = x->typ; z->len = nz;
         if(nz <= 1){
             z->g = (char*)&z->b; z->i64 = 0;
             if(nz == 1 && (e = d2h(c, &z->b, dst, w))) R e;
             c->dcur = mark;
         } else {
-            // keep dst: move it down over the scratch when possible
             z->g = (char*)dst;
+            // gathering a bounded index vector keeps its bound
+            if(x->typ == HPA_L){ auto it = c->bounds.find(px); if(it != c->bounds.end()) note_bound(c, dst, it->second); }
         }
         R 0;
     }

@@ -398,7 +398,9 @@ static int32_t join_run(hpa_ctx *c, V z, V bx, V py, int32_t t, int op, bool swa
     V cells = (V)halloc(c, 2 * sizeof(V0)); if(!cells) R HPA_E_MEMORY;
     z->typ = HPA_G; z->len = 2; z->g = (char*)cells; z->g2 = nullptr;
     if((e = set_l_vec(c, cells, z0, P))) R e;
-    R set_l_vec(c, cells + 1, z1, P);
+    if((e = set_l_vec(c, cells + 1, z1, P))) R e;
+    note_bound(c, z0, swap ? np : nb); note_bound(c, z1, swap ? nb : np);   // rows of x and of y
+    R 0;
 }
 
 static int op_from_name(const char *s){            // backend/h_system.cc:779-789

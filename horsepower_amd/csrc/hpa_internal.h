@@ -29,7 +29,12 @@ struct hpa_ctx {
     char *pinned = nullptr; L pinned_bytes = 0;
     // host buffers handed out by hpaToHost (released with the context)
     std::vector<void*> host_bufs;
+    // index vectors produced here, with an exclusive upper bound of their (non-negative) values;
+    // pfnIndex skips its device bounds check for them (entries die with their arena range)
+    std::map<const void*, L> bounds;
 };
+void note_bound(hpa_ctx *c, const void *p, L bound);
+bool within_bound(hpa_ctx *c, const void *p, L limit);
 
 hpa_ctx *ctx_cur();
 void    *dalloc(hpa_ctx *c, L bytes);               // nullptr when the arena is full

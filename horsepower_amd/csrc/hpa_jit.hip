@@ -24,7 +24,8 @@
 #include <sstream>
 #include <stdio.h>
 
-#define FGA_NS 8
+#define FGA_NS 16          // largest slot count tried before the unfused fallback
+#define FGA_ROWS 4         // consecutive rows per thread and iteration (one vector load per column)
 #define FGA_BLOCK 256
 
 static std::mutex g_jit_mu;
@@ -291,13 +292,15 @@ int32_t jit_fused_map(hpa_ctx *c, int otyp, int nx, V *x, const hpa_expr *ex, co
 // Accumulator classes: sums (and avgs, which share the sum of the same body), mins, maxes.
 struct Acc { std::string body; bool f; int cls; };   // cls 0 sum, 2 min, 3 max
 
-static std::string fga_source(int nk, const int32_t *ktyp, int nx, const int32_t *typ, const CExpr &ce,
-                              const std::vector<int> &uc, const std::vector<int> &late, const std::vector<Acc> &acc){
+static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const CExpr &ce,
+                              const std::vector<int> &used, const std::vector<Acc> &acc){
     const int NA = (int)acc.size() > 0 ? (int)acc.size() : 1;
+    const int U = FGA_ROWS;
     std::ostringstream s;
     s << PREAMBLE;
-    s << "#define NS " << FGA_NS << "\n#define NA " << NA << "\n#define HT 64\n";
-    s << "struct A { const void *c[" << EXPR_MAX_COLS << "]; const void *k0; const void *k1; i64 n; void *part; unsigned *used; int *ovf; };\n"
+    s << "#define NS " << ns << "\n#define NA " << NA << "\n#define HT 64\n#define U " << U << "\n";
+    s << "template<typename T> struct __attribute__((aligned(U * sizeof(T)))) VU { T v[U]; };\n"
+         "struct A { const void *c[" << EXPR_MAX_COLS << "]; const void *k0; const void *k1; i64 n; void *part; unsigned *used; int *ovf; };\n"
          "struct P { unsigned code, valid; u64 cnt; i64 first; double f[2 * NA]; i64 q[NA]; };\n"
          // LDS hash of key code -> slot: entry = code << 8 | slot; 0xFFFFFFFF empty; slot 0xFE busy, 0xFD overflow
          "__device__ __forceinline__ unsigned slot_of(unsigned *ht, unsigned *s_code, unsigned *s_used, unsigned code){\n"
@@ -337,9 +340,9 @@ static std::string fga_source(int nk, const int32_t *ktyp, int nx, const int32_t
          "    if(threadIdx.x < HT) ht[threadIdx.x] = 0xFFFFFFFFu;\n"
          "    if(threadIdx.x == 0){ s_used = 0; s_ovf = 0; }\n"
          "    __syncthreads();\n";
-    // accumulators
-    for(int q = 0; q < FGA_NS; q++){
-        s << "    u64 cnt" << q << " = 0; i64 fst" << q << " = 9223372036854775807LL;";
+    // accumulators: count and first row (as the iteration-relative offset it*U+u) per slot
+    for(int q = 0; q < ns; q++){
+        s << "    unsigned cnt" << q << " = 0, fst" << q << " = 0xFFFFFFFFu;";
         for(int k = 0; k < (int)acc.size(); k++){
             const Acc &A = acc[k];
             const char *init = A.cls == 0 ? "0" : (A.f ? (A.cls == 2 ? "1.7976931348623157e308" : "-1.7976931348623157e308")
@@ -348,38 +351,61 @@ static std::string fga_source(int nk, const int32_t *ktyp, int nx, const int32_t
         }
         s << "\n";
     }
-    s << "    int ovf = 0;\n"
-         "    for(i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < a.n; i += (i64)gridDim.x * 256){\n"
-         "        if(__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)){ ovf = 1; break; }\n"
-      << loads_code(nx, typ, uc)
-      << "        if(!(" << ce.s << ")) continue;\n";
-    if(nk == 2) s << "        unsigned code = ((unsigned)((const unsigned char*)a.k0)[i] << 8) | (unsigned)((const unsigned char*)a.k1)[i];\n";
-    else s << "        unsigned code = (unsigned)((const unsigned char*)a.k0)[i];\n";
-    s << "        unsigned sl = slot_of(ht, s_code, &s_used, code);\n"
-         "        if(sl >= NS){ ovf = 1; __hip_atomic_store(&s_ovf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); break; }\n"
-      << loads_code(nx, typ, late);
-    for(int k = 0; k < (int)acc.size(); k++) s << "        const " << (acc[k].f ? "double" : "i64") << " b" << k << " = " << acc[k].body << ";\n";
-    s << "        switch(sl){\n";
-    for(int q = 0; q < FGA_NS; q++){
-        s << "        case " << q << ": cnt" << q << "++; fst" << q << " = i < fst" << q << " ? i : fst" << q << ";";
+    // one row: values v<k> of the used columns, key bytes kb0/kb1, offset `off` = it*U+u
+    std::ostringstream row;
+    row << "            if(" << ce.s << "){\n";
+    if(nk == 2) row << "                const unsigned code = ((unsigned)kb0 << 8) | (unsigned)kb1;\n";
+    else row << "                const unsigned code = (unsigned)kb0;\n";
+    row << "                const unsigned sl = slot_of(ht, s_code, &s_used, code);\n"
+           "                if(sl >= NS){ __hip_atomic_store(&s_ovf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); goto fga_done; }\n";
+    for(int k = 0; k < (int)acc.size(); k++) row << "                const " << (acc[k].f ? "double" : "i64") << " b" << k << " = " << acc[k].body << ";\n";
+    row << "                switch(sl){\n";
+    for(int q = 0; q < ns; q++){
+        row << "                case " << q << ": if(cnt" << q << "++ == 0) fst" << q << " = off;";
         for(int k = 0; k < (int)acc.size(); k++){
             std::string v = "a" + std::to_string(k) + "_" + std::to_string(q), b = "b" + std::to_string(k);
-            if(acc[k].cls == 0) s << (acc[k].f ? " " + v + " += " + b + ";" : " " + v + " = (i64)((u64)" + v + " + (u64)" + b + ");");
-            else if(acc[k].cls == 2) s << " " << v << " = " << b << " < " << v << " ? " << b << " : " << v << ";";
-            else s << " " << v << " = " << b << " > " << v << " ? " << b << " : " << v << ";";
+            if(acc[k].cls == 0) row << (acc[k].f ? " " + v + " += " + b + ";" : " " + v + " = (i64)((u64)" + v + " + (u64)" + b + ");");
+            else if(acc[k].cls == 2) row << " " << v << " = " << b << " < " << v << " ? " << b << " : " << v << ";";
+            else row << " " << v << " = " << b << " > " << v << " ? " << b << " : " << v << ";";
         }
-        s << " break;\n";
+        row << " break;\n";
     }
-    s << "        }\n    }\n"
-         "    if(ovf) atomicOr(a.ovf, 1);\n"
+    row << "                }\n            }\n";
+    s << "    const i64 tid = (i64)blockIdx.x * 256 + threadIdx.x;\n"
+         "    const i64 stride = (i64)gridDim.x * 256 * U;\n"
+         "    unsigned it = 0;\n"
+         "    for(i64 base = tid * U; base < a.n; base += stride, it++){\n"
+         "        if(__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;\n"
+         "        if(base + U <= a.n){\n";
+    // vector loads of U consecutive rows
+    for(int k = 0; k < nx; k++) if(used[k])
+        s << "            const VU<" << ctype(typ[k]) << "> w" << k << " = *(const VU<" << ctype(typ[k]) << ">*)((const " << ctype(typ[k]) << "*)a.c[" << k << "] + base);\n";
+    s << "            const VU<unsigned char> wk0 = *(const VU<unsigned char>*)((const unsigned char*)a.k0 + base);\n";
+    if(nk == 2) s << "            const VU<unsigned char> wk1 = *(const VU<unsigned char>*)((const unsigned char*)a.k1 + base);\n";
+    for(int u = 0; u < U; u++){
+        s << "          {\n            const unsigned off = it * U + " << u << ";\n";
+        for(int k = 0; k < nx; k++) if(used[k]) s << "            const " << ctype(typ[k]) << " v" << k << " = w" << k << ".v[" << u << "];\n";
+        s << "            const unsigned char kb0 = wk0.v[" << u << "];" << (nk == 2 ? " const unsigned char kb1 = wk1.v[" + std::to_string(u) + "];" : "") << "\n";
+        s << row.str() << "          }\n";
+    }
+    s << "        } else {\n"
+         "            for(int u = 0; u < U && base + u < a.n; u++){\n"
+         "                const i64 i = base + u; const unsigned off = it * U + u;\n"
+      << loads_code(nx, typ, used)
+      << "                const unsigned char kb0 = ((const unsigned char*)a.k0)[i];"
+      << (nk == 2 ? " const unsigned char kb1 = ((const unsigned char*)a.k1)[i];" : "") << "\n"
+      << row.str()
+      << "            }\n        }\n    }\n"
+         "fga_done:\n"
+         "    if(__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) && threadIdx.x == 0) atomicOr(a.ovf, 1);\n"
          "    __syncthreads();\n"
          "    const unsigned used = s_used < NS ? s_used : NS;\n"
          "    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;\n"
          "    P *out = (P*)a.part + (i64)blockIdx.x * NS;\n";
     // per-slot block reduction, fixed order
-    for(int q = 0; q < FGA_NS; q++){
+    for(int q = 0; q < ns; q++){
         s << "    if(" << q << " < used){\n"
-             "        u64 c = cnt" << q << "; i64 f = fst" << q << ";\n"
+             "        u64 c = cnt" << q << "; i64 f = fst" << q << " == 0xFFFFFFFFu ? 9223372036854775807LL : tid * U + (i64)(fst" << q << " / U) * stride + (fst" << q << " % U);\n"
              "        for(int o = 32; o > 0; o >>= 1){ c += __shfl_down(c, o, 64); i64 t = __shfl_down(f, o, 64); f = t < f ? t : f; }\n";
         for(int k = 0; k < (int)acc.size(); k++){
             std::string v = "a" + std::to_string(k) + "_" + std::to_string(q);
@@ -478,7 +504,6 @@ static std::string fga_source(int nk, const int32_t *ktyp, int nx, const int32_t
         else s << "            m.q[(i64)r * NA + " << k << "] = (i64)t" << k << ";\n";
     }
     s << "        }\n    }\n}\n";
-    (void)ktyp;
     R s.str();
 }
 
@@ -550,20 +575,18 @@ extern "C" int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *
         if(id < 0){ id = (int)acc.size(); acc.push_back({be.s, be.f, cls}); }
         o.accid[a] = id; o.accf[a] = be.f ? 1 : 0;
     }
-    split_used(nx, uc, ub, late);
-    std::string src = fga_source(nk, nullptr, nx, typ, ce, uc, late, acc);
-    hipFunction_t f1, f2;
-    if((e = jit_get(src, "fga", &f1)) || (e = jit_get(src, "fga_merge", &f2))) R e;
+    std::vector<int> used(nx, 0);
+    for(int k = 0; k < nx; k++) used[k] = uc[k] | ub[k];
     const int NA = acc.empty() ? 1 : (int)acc.size();
     o.stride = NA;
     const size_t psize = 8 + 8 + 8 + 16 * NA + 8 * NA;     // struct P
-    int G = (int)ceil_div(n, 256 * 64);
+    int G = (int)ceil_div(n, 256 * FGA_ROWS * 8);
     if(G > num_cus() * 4) G = num_cus() * 4;
     if(G < 1) G = 1;
     const void *pk0 = nullptr, *pk1 = nullptr;
     if(n > 0){ if((e = dev_view(c, keys[0], &pk0))) R e; if(nk == 2 && (e = dev_view(c, keys[1], &pk1))) R e; }
     void *part = dalloc(c, (L)psize * FGA_NS * G);
-    unsigned *used = (unsigned*)dalloc(c, 4 * G);
+    unsigned *usedb = (unsigned*)dalloc(c, 4 * G);
     int *ovf = (int*)dalloc(c, 16);
     const L GM = (L)G * FGA_NS;                              // groups cannot exceed the partial slots
     unsigned *codes = (unsigned*)dalloc(c, 4 * GM);
@@ -572,22 +595,30 @@ extern "C" int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *
     long long *first = (long long*)dalloc(c, 8 * GM);
     double *fo = (double*)dalloc(c, 8 * GM * NA);
     long long *qo = (long long*)dalloc(c, 8 * GM * NA);
-    if(!part || !used || !ovf || !codes || !gdev || !cnt || !first || !fo || !qo){ c->dcur = mark; R HPA_E_MEMORY; }
-    hipMemsetAsync(ovf, 0, 4, c->stream);
+    if(!part || !usedb || !ovf || !codes || !gdev || !cnt || !first || !fo || !qo){ c->dcur = mark; R HPA_E_MEMORY; }
     struct { const void *c[EXPR_MAX_COLS]; const void *k0; const void *k1; long long n; void *part; unsigned *used; int *ovf; } ga;
     memset(&ga, 0, sizeof(ga));
     for(int k = 0; k < nx; k++) ga.c[k] = cols.p[k];
-    ga.k0 = pk0; ga.k1 = pk1; ga.n = n; ga.part = part; ga.used = used; ga.ovf = ovf;
-    {
-        double cb = nk; for(int k = 0; k < nx; k++) if(uc[k] || ub[k]) cb += (double)type_width(typ[k]);
-        ProfScope ps("fused_group_agg", c->stream, (double)n * cb);
-        if((e = jit_launch(c, f1, (unsigned)G, 256, ga))) R e;
-    }
-    struct { const void *part; const unsigned *used; int G; unsigned *codes; int *g; unsigned long long *cnt; long long *first; double *f; long long *q; } ma;
-    ma.part = part; ma.used = used; ma.G = G; ma.codes = codes; ma.g = gdev; ma.cnt = cnt; ma.first = first; ma.f = fo; ma.q = qo;
-    if((e = jit_launch(c, f2, 1, 1024, ma))) R e;
+    ga.k0 = pk0; ga.k1 = pk1; ga.n = n; ga.part = part; ga.used = usedb; ga.ovf = ovf;
+    double cb = nk; for(int k = 0; k < nx; k++) if(used[k]) cb += (double)type_width(typ[k]);
+    // slot counts tried in turn: fewer slots keep the accumulators in fewer registers; a block
+    // that meets more distinct codes than slots reports overflow and the next size reruns
     int hv[2] = {0, 0};
-    if((e = d2h(c, &hv[0], ovf, 4)) || (e = d2h(c, &hv[1], gdev, 4))) R e;
+    for(int ns = 4; ns <= FGA_NS; ns *= 2){
+        std::string src = fga_source(nk, ns, nx, typ, ce, used, acc);
+        hipFunction_t f1, f2;
+        if((e = jit_get(src, "fga", &f1)) || (e = jit_get(src, "fga_merge", &f2))) R e;
+        hipMemsetAsync(ovf, 0, 4, c->stream);
+        {
+            ProfScope ps("fused_group_agg", c->stream, (double)n * cb);
+            if((e = jit_launch(c, f1, (unsigned)G, 256, ga))) R e;
+        }
+        struct { const void *part; const unsigned *used; int G; unsigned *codes; int *g; unsigned long long *cnt; long long *first; double *f; long long *q; } ma;
+        ma.part = part; ma.used = usedb; ma.G = G; ma.codes = codes; ma.g = gdev; ma.cnt = cnt; ma.first = first; ma.f = fo; ma.q = qo;
+        if((e = jit_launch(c, f2, 1, 1024, ma))) R e;
+        if((e = d2h(c, &hv[0], ovf, 4)) || (e = d2h(c, &hv[1], gdev, 4))) R e;
+        if(!hv[0]) break;
+    }
     if(hv[0]){ c->dcur = mark; R HPA_E_NOT_IMPL; }
     const L g = hv[1];
     uint8_t *k0 = (uint8_t*)dalloc(c, g > 0 ? g : 1), *k1 = (uint8_t*)dalloc(c, g > 0 ? g : 1);
@@ -599,7 +630,7 @@ extern "C" int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *
     if((e = check_launch("fga_emit"))) R e;
     set_dev_vec(c, zkeys[0], keys[0]->typ, g, k0);
     if(nk == 2) set_dev_vec(c, zkeys[1], keys[1]->typ, g, k1);
-    if(zfirst) set_dev_vec(c, zfirst, HPA_L, g, zf);
+    if(zfirst){ set_dev_vec(c, zfirst, HPA_L, g, zf); note_bound(c, zf, n); }
     for(int a = 0; a < na; a++){
         int kd = aggs[a].kind;
         int32_t t = kd == 4 ? HPA_L : kd == 1 ? HPA_E : (o.accf[a] ? HPA_E : HPA_L);
@@ -626,8 +657,9 @@ extern "C" int32_t hpaJitCompileCheck(int32_t nx, const int32_t *coltyp, const h
         bool dup = false; for(auto &q : acc) if(q.body == be.s && q.cls == cls) dup = true;
         if(!dup) acc.push_back({be.s, be.f, cls});
     }
-    split_used(nx, uc, ub, late);
-    std::string src = fga_source(2, nullptr, nx, typ, ce, uc, late, acc);
+    std::vector<int> used(nx, 0);
+    for(int k = 0; k < nx; k++) used[k] = uc[k] | ub[k];
+    std::string src = fga_source(2, 4, nx, typ, ce, used, acc);
     g_compile_only = true;
     hipFunction_t f;
     e = jit_get(src, "fga", &f);

@@ -186,7 +186,16 @@ extern "C" int64_t hpaSetHeapOffset(int64_t cur){
     c->marks.erase(c->marks.upper_bound(cur), c->marks.end());
     for(void *p : c->host_bufs) free(p);
     c->host_bufs.clear();
+    c->bounds.erase(c->bounds.lower_bound((const void*)(c->dbase + cur)), c->bounds.end());
     R c->dcur = cur;
+}
+
+void note_bound(hpa_ctx *c, const void *p, L bound){
+    if(p && (const char*)p >= c->dbase && (const char*)p < c->dbase + c->dtop) c->bounds[p] = bound;
+}
+bool within_bound(hpa_ctx *c, const void *p, L limit){
+    auto it = c->bounds.find(p);
+    R it != c->bounds.end() && it->second <= limit;
 }
 
 // ---------------------------------------------------------------- value model

@@ -209,6 +209,7 @@ extern "C" int32_t pfnWhere(hpa_V z, hpa_V x){
     int64_t kk = 0;
     if((e = run_compress(c, "where", MaskPred{(const uint8_t*)p}, n, cols, out, &kk, 1.0, 8.0))) R e;
     z->typ = HPA_L; z->len = kk;
+    note_bound(c, out, n);
     if(kk <= 1){
         z->g = (char*)&z->b; z->i64 = 0;
         if(kk == 1 && (e = d2h(c, &z->b, out, 8))) R e;

@@ -186,6 +186,7 @@ int32_t make_group_dict(hpa_ctx *c, V z, int64_t *keys, int64_t *counts, int64_t
     if(n <= 1){ inner->g = (char*)&inner->b; inner->i64 = 0; if(n == 1){ int32_t e = d2h(c, &inner->b, flat, 8); if(e) R e; } }
     else inner->g = (char*)flat;
     vz->typ = HPA_G; vz->len = g; vz->g = (char*)counts; vz->g2 = inner;
+    note_bound(c, keys, n); note_bound(c, flat, n);   // row numbers of the grouped vector
     R 0;
 }
 
@@ -515,6 +516,7 @@ static int32_t order_emit(hpa_ctx *c, V z, const uint32_t *perm, L n, bool rever
     if(n > 0) hipLaunchKernelGGL(k_perm_to_i64, dim3(grid_for(n, 256, 4)), dim3(256), 0, c->stream, perm, (int64_t*)dz, n, reverse ? 1 : 0);
     if((e = check_launch("order"))) R e;
     if(is_inline(z)){ e = commit_inline(c, z, dz); c->dcur = mark; }
+    else note_bound(c, dz, n);                   // a permutation of 0..n-1
     R e;
 }
 
