@@ -12,7 +12,12 @@ The JSON line carries:
                 interpreter statement sequence) on a bounded sample of the same workload
   plain         the same query through the interpreter's per-primitive sequence on the GPU
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload q1|q6|q3] [--rows N]
+Workload "batch": 64 independent filter -> group -> sum queries (row counts log-uniform 1e4..1e7,
+key ranges log-uniform 4..1e6) in one persistent launch replayed from a captured hipGraph, with
+the autotuner's configuration; `value` is queries per second, compared with the same 64 queries
+run one by one through the per-query primitive plans on the GPU.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload q1|q6|q3|batch] [--rows N]
 """
 import argparse
 import json
@@ -31,7 +36,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="q1", choices=["q1", "q6", "q3"])
+    p.add_argument("--workload", default="q1", choices=["q1", "q6", "q3", "batch"])
     p.add_argument("--rows", type=int, default=0)
     p.add_argument("--plan", default="best", choices=["best", "fused", "plain"])
     p.add_argument("--cpu-rows", type=int, default=0)
@@ -73,6 +78,8 @@ def main():
     from horsepower_amd import queries as Q
     from horsepower_amd.backend import GpuBackend
 
+    if a.workload == "batch":
+        return bench_batch(a)
     W = workload(a.workload, a.rows)
     t0 = time.time()
     cols = W["gen"](W["n"] if W["n"] else W["nd"])
@@ -148,6 +155,136 @@ def main():
         "roofline": roof, "cpu_baseline": cpu, "plain_gpu": plain, "kernels": per_kernel,
     }
     print(json.dumps(out), flush=True)
+
+
+def profile_kernels(lib, fn, steps):
+    """Per-scope device time and algorithmic bytes from the library's HIP-event profiler."""
+    import ctypes as C
+    lib.dll.hpaProfileReset()
+    lib.dll.hpaProfileEnable(1)
+    for _ in range(steps):
+        fn()
+    lib.sync()
+    lib.dll.hpaProfileEnable(0)
+    buf = C.create_string_buffer(1 << 16)
+    lib.dll.hpaProfileList(buf, 1 << 16)
+    kern = {}
+    for line in buf.value.decode().splitlines():
+        name, cnt, tms, byt = line.split()
+        kern[name] = dict(count=int(cnt), ms=float(tms), bytes=float(byt))
+    return kern
+
+
+def roofline_of(kern, name):
+    k = kern[name]
+    avg_ms = k["ms"] / k["count"]
+    per_launch = k["bytes"] / k["count"]
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+            "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+            "bytes_per_launch": per_launch, "avg_launch_us": round(avg_ms * 1e3, 2)}
+
+
+def pct(xs, p):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs), p))
+
+
+def bench_batch(a):
+    import numpy as np
+    from horsepower_amd import datagen as D, queries as Q
+    from horsepower_amd.backend import Batch, GpuBackend
+    nq = 64
+    ns, gs, thr = D.batch_shapes(nq)
+    t0 = time.time()
+    cols = [D.gen_batch_query(q, int(ns[q]), int(gs[q])) for q in range(nq)]
+    rows = int(ns.sum())
+    print(f"[bench] generated {nq} queries, {rows} rows in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    B = GpuBackend()
+    tables = [Q.load_table(B, c) for c in cols]
+    B.sync()
+    b = Batch(B, tables, thr)
+    tune = b.autotune(2)
+    cfg = b.config()
+    print(f"[bench] autotuned {len(tune)} candidates -> {cfg}", file=sys.stderr, flush=True)
+    mark = B.mark()
+
+    def step():
+        b.run(True)
+        return b.results()
+
+    for _ in range(a.warmup):
+        step()
+    lat = []
+    B.sync()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        t1 = time.perf_counter()
+        res = step()
+        lat.append(time.perf_counter() - t1)
+    dt = time.perf_counter() - t0
+    ms = dt * 1e3 / a.steps
+    print(f"[bench] batched: {ms:.3f} ms per batch of {nq}", file=sys.stderr, flush=True)
+    kern = profile_kernels(B.lib, step, a.profile_steps)
+    roof = roofline_of(kern, "batch_graph") if "batch_graph" in kern else None
+
+    # the same queries one by one through the per-query primitive plans
+    def per_query(plan):
+        out, t = [], []
+        for q in range(nq):
+            t1 = time.perf_counter()
+            out.append(plan(B, tables[q], thr[q]))
+            t.append(time.perf_counter() - t1)
+            B.release(mark)
+        return out, t
+    per_query(Q.batch_fused)
+    plain_res, plain_t = per_query(Q.batch_plain)
+    fused_res, fused_t = per_query(Q.batch_fused)
+    equal = True
+    for q in range(nq):
+        for other in (plain_res[q], fused_res[q]):
+            equal &= bool(np.array_equal(res[q]["key"], other["key"]) and
+                          np.array_equal(res[q]["count"], other["count"]) and
+                          np.allclose(res[q]["sum"], other["sum"], rtol=1e-12, atol=0))
+    print(f"[bench] per-query plain {sum(plain_t) * 1e3:.1f} ms, fused {sum(fused_t) * 1e3:.1f} ms, "
+          f"equal={equal}", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not a.no_cpu:
+        import oracle
+        ob = oracle.backend(os.cpu_count() or 1)
+        om = ob.mark()
+        k = 0
+        t0 = time.perf_counter()
+        while k < nq and time.perf_counter() - t0 < 25.0:
+            Q.batch_plain(ob, Q.load_table(ob, cols[k]), thr[k])
+            ob.release(om)
+            k += 1
+        cdt = time.perf_counter() - t0
+        crow = int(ns[:k].sum())
+        cpu = {"value": k / cdt, "unit": "queries/s", "cores": ob.threads(), "kind": "port",
+               "sample": f"the first {k} of the {nq} queries ({crow} rows), one run each, in order",
+               "rows_per_s": crow / cdt}
+    out = {
+        "metric": "rows/s + % HBM roofline per query; batched queries/s & p99 latency vs plain",
+        "value": nq / (dt / a.steps), "unit": "queries/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded counter-based generator)",
+        "config": {"workload": f"batched group-by: {nq} queries in one persistent launch, {rows} rows, "
+                               "row counts log-uniform 1e4..1e7, key ranges log-uniform 4..1e6",
+                   "rows": rows, "queries": nq, "tuned": cfg},
+        "rows_per_s": rows / (dt / a.steps),
+        "latency_ms": {"p50": round(pct(lat, 50) * 1e3, 3), "p99": round(pct(lat, 99) * 1e3, 3)},
+        "plain_gpu": {"queries_per_s": nq / sum(plain_t), "ms_per_batch": round(sum(plain_t) * 1e3, 3),
+                      "query_p50_ms": round(pct(plain_t, 50) * 1e3, 3), "query_p99_ms": round(pct(plain_t, 99) * 1e3, 3)},
+        "fused_gpu": {"queries_per_s": nq / sum(fused_t), "ms_per_batch": round(sum(fused_t) * 1e3, 3)},
+        "equal_results": equal, "roofline": roof, "cpu_baseline": cpu,
+        "kernels": {k: {"launches_per_step": v["count"] / a.profile_steps,
+                        "us_per_step": round(v["ms"] * 1e3 / a.profile_steps, 2)} for k, v in kern.items()},
+        "autotune_ms": [round(x, 4) for x in tune],
+    }
+    print(json.dumps(out), flush=True)
+    b.close()
 
 
 def cpu_baseline(a, W):

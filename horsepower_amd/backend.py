@@ -96,6 +96,9 @@ class Backend:
     def read_scalar(self, z):
         return self.read(z)[0]
 
+    def read_many(self, zs):
+        return [self.read(z) for z in zs]
+
     def read_list(self, z):
         v = z.contents
         if v.typ not in (H_G, H_N):
@@ -152,6 +155,15 @@ class GpuBackend(Backend):
 
     def read_raw(self, z, nbytes, dst):
         self.lib.rc("hpaReadPayload", z, C.c_void_p(dst), nbytes)
+
+    def read_many(self, zs):
+        """Vector nodes -> numpy arrays with one device synchronisation."""
+        outs = [np.empty(z.contents.len, dtype=np.dtype(NP_OF[z.contents.typ])) for z in zs]
+        n = len(zs)
+        dst = (C.c_void_p * n)(*[o.ctypes.data for o in outs])
+        nb = (C.c_int64 * n)(*[o.nbytes for o in outs])
+        self.lib.rc("hpaReadMany", n, (V * n)(*zs), dst, nb)
+        return outs
 
     def symbol(self, name):
         return self.lib.symbol(name)
@@ -216,3 +228,52 @@ class GpuBackend(Backend):
         z = self.node()
         self.lib.rc("hpaGroupReduce", z, kind, x, groups)
         return z
+
+
+class Batch:
+    """A resident plan over many filter -> group -> sum queries (hpaBatch*), run in one
+    persistent launch.  tables: dicts with "key" (I), "val" (E), "flt" (I) column nodes;
+    rows with flt < thr[q] take part in query q."""
+
+    def __init__(self, B, tables, thr):
+        from .vnode import OP, HorseError
+        self.B, self.lib, self.nq = B, B.lib, len(tables)
+        nq = self.nq
+        cols = [(V * nq)(*[t[k] for t in tables]) for k in ("key", "val", "flt")]
+        ops = (C.c_int32 * nq)(*([OP["LT"]] * nq))
+        consts = (C.c_int32 * nq)(*[int(x) for x in thr])
+        self.h = self.lib.dll.hpaBatchCreate(nq, cols[0], cols[1], cols[2], ops, consts)
+        if not self.h:
+            raise HorseError(-1, "hpaBatchCreate", self.lib.detail())
+
+    def run(self, graph=True):
+        self.lib.rc("hpaBatchRun", self.h, 1 if graph else 0)
+
+    def results(self):
+        """[{key, count, sum}] per query, read back with one synchronisation."""
+        zs = []
+        for q in range(self.nq):
+            z = [self.B.node() for _ in range(3)]
+            self.lib.rc("hpaBatchResult", self.h, q, *z)
+            zs += z
+        arrs = self.B.read_many(zs)
+        return [dict(zip(("key", "count", "sum"), arrs[3 * q:3 * q + 3])) for q in range(self.nq)]
+
+    def config(self):
+        v = [C.c_int32() for _ in range(3)]
+        self.lib.rc("hpaBatchGetConfig", self.h, *[C.byref(x) for x in v])
+        return dict(block=v[0].value, tile_rows=v[1].value, lds_keys=v[2].value)
+
+    def set_config(self, block, tile_rows, lds_keys):
+        self.lib.rc("hpaBatchSetConfig", self.h, block, tile_rows, lds_keys)
+
+    def autotune(self, iters=3):
+        ms = (C.c_float * 64)()
+        n = C.c_int32()
+        self.lib.rc("hpaBatchAutotune", self.h, iters, ms, 64, C.byref(n))
+        return list(ms[:n.value])
+
+    def close(self):
+        if self.h:
+            self.lib.dll.hpaBatchDestroy(self.h)
+            self.h = None

@@ -590,7 +590,7 @@ extern "C" int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *
     int *ovf = (int*)dalloc(c, 16);
     const L GM = (L)G * FGA_NS;                              // groups cannot exceed the partial slots
     unsigned *codes = (unsigned*)dalloc(c, 4 * GM);
-    int *gdev = (int*)dalloc(c, 16);
+    int *gdev = ovf + 1;                                     // overflow flag and group count, one read
     unsigned long long *cnt = (unsigned long long*)dalloc(c, 8 * GM);
     long long *first = (long long*)dalloc(c, 8 * GM);
     double *fo = (double*)dalloc(c, 8 * GM * NA);
@@ -616,7 +616,7 @@ extern "C" int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *
         struct { const void *part; const unsigned *used; int G; unsigned *codes; int *g; unsigned long long *cnt; long long *first; double *f; long long *q; } ma;
         ma.part = part; ma.used = usedb; ma.G = G; ma.codes = codes; ma.g = gdev; ma.cnt = cnt; ma.first = first; ma.f = fo; ma.q = qo;
         if((e = jit_launch(c, f2, 1, 1024, ma))) R e;
-        if((e = d2h(c, &hv[0], ovf, 4)) || (e = d2h(c, &hv[1], gdev, 4))) R e;
+        if((e = d2h(c, hv, ovf, 8))) R e;
         if(!hv[0]) break;
     }
     if(hv[0]){ c->dcur = mark; R HPA_E_NOT_IMPL; }

@@ -360,6 +360,20 @@ extern "C" int32_t hpaReadPayload(hpa_V x, void *dst, int64_t bytes){
     R hipStreamSynchronize(c->stream) == hipSuccess ? 0 : HPA_E_DEVICE;
 }
 
+// Copy the payloads of n vectors into caller memory with one stream synchronisation.
+extern "C" int32_t hpaReadMany(int32_t n, hpa_V *x, void **dst, const int64_t *bytes){
+    hpa_ctx *c = ctx_cur();
+    bool any = false;
+    for(int k = 0; k < n; k++){
+        if(bytes[k] <= 0) continue;
+        if(is_inline(x[k]) || !is_device(x[k]->g)){ memcpy(dst[k], x[k]->g, (size_t)bytes[k]); continue; }
+        if(hipMemcpyAsync(dst[k], x[k]->g, (size_t)bytes[k], hipMemcpyDeviceToHost, c->stream) != hipSuccess) R HPA_E_DEVICE;
+        any = true;
+    }
+    if(any && hipStreamSynchronize(c->stream) != hipSuccess) R HPA_E_DEVICE;
+    R 0;
+}
+
 extern "C" int32_t hpaSync(void){
     hpa_ctx *c = ctx_cur();
     R hipStreamSynchronize(c->stream) == hipSuccess ? 0 : HPA_E_DEVICE;

@@ -487,10 +487,58 @@ __global__ void k_count_ties(const uint64_t *k, L n, unsigned long long *cnt){
 
 // Stable permutation of x (list of columns, flags per column) in lexicographic order, ties by
 // ascending row.  LSD over columns: sort by the last column, then stably by each earlier one.
+// Small inputs (n <= 1024, <= 8 columns): one workgroup ranks every row against every other on
+// the same order keys -- ties by row -- in a single launch with no host round trip.
+#define SMALL_ORDER 1024
+#define SMALL_COLS 8
+struct SmallCols { int nc; int typ[SMALL_COLS]; int desc[SMALL_COLS]; const void *p[SMALL_COLS]; };
+__device__ __forceinline__ uint64_t okey_any(int t, const void *p, L i){
+    switch(t){
+        case HPA_B: return okey<uint8_t>(((const uint8_t*)p)[i]);
+        case HPA_J: return okey<int8_t>(((const int8_t*)p)[i]);
+        case HPA_C: return okey<char>(((const char*)p)[i]);
+        case HPA_H: return okey<int16_t>(((const int16_t*)p)[i]);
+        case HPA_L: case HPA_Z: return okey<int64_t>(((const int64_t*)p)[i]);
+        case HPA_F: return okey<float>(((const float*)p)[i]);
+        case HPA_E: return okey<double>(((const double*)p)[i]);
+        default: return okey<int32_t>(((const int32_t*)p)[i]);
+    }
+}
+__global__ void __launch_bounds__(1024) k_order_small(SmallCols s, int n, uint32_t *perm){
+    __shared__ uint64_t k[SMALL_COLS][SMALL_ORDER];
+    for(int q = 0; q < s.nc; q++)
+        for(int i = threadIdx.x; i < n; i += blockDim.x){ uint64_t v = okey_any(s.typ[q], s.p[q], i); k[q][i] = s.desc[q] ? ~v : v; }
+    __syncthreads();
+    for(int i = threadIdx.x; i < n; i += blockDim.x){
+        int rank = 0;
+        for(int j = 0; j < n; j++){
+            int less = j < i, q = 0;
+            for(; q < s.nc; q++){ uint64_t a = k[q][j], b = k[q][i]; if(a != b){ less = a < b; break; } }
+            rank += less;
+        }
+        perm[rank] = (uint32_t)i;
+    }
+}
+
 static int32_t order_perm(hpa_ctx *c, V x, const uint8_t *flags, uint32_t **perm_out, bool *ties){
     L nc = x->len; L n = ((V)x->g)->len; int32_t e;
     *ties = false;
     uint32_t *perm = nullptr;
+    if(n > 0 && n <= SMALL_ORDER && nc <= SMALL_COLS){
+        SmallCols s; memset(&s, 0, sizeof(s)); s.nc = (int)nc;
+        for(L q = 0; q < nc; q++){
+            V col = ((V)x->g) + q;
+            if(col->typ == HPA_S || col->typ == HPA_X || col->typ == HPA_Q) R HPA_E_NOT_IMPL;
+            const void *p; if((e = dev_view(c, col, &p))) R e;
+            s.typ[q] = col->typ; s.p[q] = p; s.desc[q] = flags[q] ? 0 : 1;
+        }
+        perm = (uint32_t*)dalloc(c, 4 * n); if(!perm) R HPA_E_MEMORY;
+        ProfScope ps("order_small", c->stream);
+        hipLaunchKernelGGL(k_order_small, dim3(1), dim3(1024), 0, c->stream, s, (int)n, perm);
+        if((e = check_launch("order_small"))) R e;
+        *perm_out = perm;
+        R 0;
+    }
     uint64_t *k = (uint64_t*)dalloc(c, 8 * (n > 0 ? n : 1)); if(!k) R HPA_E_MEMORY;
     for(L q = nc - 1; q >= 0; q--){
         V col = ((V)x->g) + q;

@@ -96,7 +96,7 @@ def q1_plain(B, T, ship_max=D.Q1_SHIP_MAX):
     t74 = B.list([t45, t46])
     t75 = B.pfnOrderBy(t74, B.bools([1, 1]))
     out = [B.pfnIndex(v, t75) for v in (t45, t46, t49, t55, t61, t64, t52, t58, t70, t73)]
-    return dict(zip(Q1_NAMES, (B.read(v) for v in out)))
+    return dict(zip(Q1_NAMES, B.read_many(out)))
 
 
 def q1_fused(B, T, ship_max=D.Q1_SHIP_MAX):
@@ -116,7 +116,7 @@ def q1_fused(B, T, ship_max=D.Q1_SHIP_MAX):
             B.group_reduce(1, disc, groups), B.group_reduce(4, flag, groups)]
     perm = B.pfnOrderBy(B.list([kf, ks]), B.bools([1, 1]))
     out = [B.pfnIndex(v, perm) for v in [kf, ks] + aggs]
-    return dict(zip(Q1_NAMES, (B.read(v) for v in out)))
+    return dict(zip(Q1_NAMES, B.read_many(out)))
 
 
 def q1_chf_spec(ship_max=D.Q1_SHIP_MAX):
@@ -146,7 +146,7 @@ def q1_chf(B, T, ship_max=D.Q1_SHIP_MAX):
     _, (kf, ks), aggs = r
     perm = B.pfnOrderBy(B.list([kf, ks]), B.bools([1, 1]))
     out = [B.pfnIndex(v, perm) for v in [kf, ks] + aggs]
-    return dict(zip(Q1_NAMES, (B.read(v) for v in out)))
+    return dict(zip(Q1_NAMES, B.read_many(out)))
 
 
 # ------------------------------------------------------------------------------------- Q3
@@ -175,3 +175,32 @@ def q3_plain(B, Dm, F, day=D.Q3_DAY, limit=10):
     res = {k: B.read(B.pfnIndex(v, perm)) for k, v in zip(Q3_NAMES, (okey, revenue, oday))}
     res["rows"] = int(B.read(perm).shape[0])
     return {k: (v[:limit] if isinstance(v, np.ndarray) else v) for k, v in res.items()}
+
+
+# ---------------------------------------------------------------------- batched group-by
+BATCH_NAMES = ("key", "count", "sum")
+
+
+def batch_plain(B, T, thr):
+    """One query of the batched workload as the interpreter runs it: rows with flt < thr,
+    grouped on key, FP1 sum of val and FP7 len per group (optimizer/pattern.cc:180-219);
+    (key, count, sum) over the groups in pfnGroup order."""
+    m = B.pfnLt(T["flt"], B.scalar(int(thr), H_I))
+    k, v = B.pfnCompress(m, T["key"]), B.pfnCompress(m, T["val"])
+    g = B.pfnGroup(k)
+    keys, groups = B.pfnKeys(g), B.pfnValues(g)
+    gk = B.pfnIndex(k, keys)
+    s = B.pfnRaze(B.each(B.each_right(v, groups, "pfnIndex"), "pfnSum"))
+    n = B.pfnRaze(B.each(B.each_right(k, groups, "pfnIndex"), "pfnLen"))
+    return dict(zip(BATCH_NAMES, B.read_many([gk, n, s])))
+
+
+def batch_fused(B, T, thr):
+    """The same query with the compiler's fused forms: one mask, peephole compress of both
+    columns, group, FP1/FP7 grouped reductions over the group lists."""
+    m = B.pfnLt(T["flt"], B.scalar(int(thr), H_I))
+    k, v = B.compress_many(m, [T["key"], T["val"]])
+    g = B.pfnGroup(k)
+    keys, groups = B.pfnKeys(g), B.pfnValues(g)
+    gk = B.pfnIndex(k, keys)
+    return dict(zip(BATCH_NAMES, B.read_many([gk, B.group_reduce(4, k, groups), B.group_reduce(0, v, groups)])))

@@ -55,6 +55,7 @@ class Device(PrimitiveLib):
         _bind(dll, "hpaToHost", i32, [V])
         _bind(dll, "hpaReadPayload", i32, [V, vp, i64])
         _bind(dll, "hpaIsDevicePtr", i32, [vp])
+        _bind(dll, "hpaReadMany", i32, [i32, C.POINTER(V), C.POINTER(vp), C.POINTER(i64)])
         _bind(dll, "hpaSync", i32, [])
         _bind(dll, "hpaCtxCreate", vp, [i64])
         _bind(dll, "hpaCtxDestroy", None, [vp])
@@ -79,6 +80,14 @@ class Device(PrimitiveLib):
         _bind(dll, "hpaFusedGroupAgg", i32, [i32, C.POINTER(V), i32, C.POINTER(V), C.POINTER(Expr), i32,
                                              vp, V, C.POINTER(V), C.POINTER(V)])
         _bind(dll, "hpaJitCompileCheck", i32, [i32, C.POINTER(i32), C.POINTER(Expr), i32, vp])
+        _bind(dll, "hpaBatchCreate", vp, [i32, C.POINTER(V), C.POINTER(V), C.POINTER(V),
+                                          C.POINTER(i32), C.POINTER(i32)])
+        _bind(dll, "hpaBatchDestroy", None, [vp])
+        _bind(dll, "hpaBatchRun", i32, [vp, i32])
+        _bind(dll, "hpaBatchResult", i32, [vp, i32, V, V, V])
+        _bind(dll, "hpaBatchSetConfig", i32, [vp, i32, i32, i32])
+        _bind(dll, "hpaBatchGetConfig", i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)])
+        _bind(dll, "hpaBatchAutotune", i32, [vp, i32, C.POINTER(C.c_float), i32, C.POINTER(i32)])
         self.initialised = False
         self._keep = []
 

@@ -134,6 +134,8 @@ int32_t  hpaFreeResident(hpa_V x);
 int32_t  hpaToHost(hpa_V x);
 /* Copy `bytes` of x's payload (device or host) into caller memory. */
 int32_t  hpaReadPayload(hpa_V x, void *dst, int64_t bytes);
+/* Copy the payloads of n vectors into caller buffers with a single synchronisation. */
+int32_t  hpaReadMany(int32_t n, hpa_V *x, void **dst, const int64_t *bytes);
 int32_t  hpaIsDevicePtr(const void *p);
 int32_t  hpaSync(void);
 
@@ -251,6 +253,27 @@ int32_t hpaCompressMany(int32_t ny, hpa_V *z, hpa_V mask, hpa_V *y);
  * t of y (flat or normal list of i64 row indices), z[g] = reduce_{j in t}(x[j]);
  * kind 0 sum, 1 avg, 2 min, 3 max, 4 len.  z is a vector over groups. */
 int32_t hpaGroupReduce(hpa_V z, int32_t kind, hpa_V x, hpa_V y);
+
+/* ---- batched execution (no single reference counterpart: it runs, for many queries at once,
+ * the reference's per-query sequence compare -> compress -> group -> FP1/FP7 grouped sum/len,
+ * h_primitive.cc:1073-1106,1479-1511,993-999 and optimizer/pattern.cc:180-219).
+ * Query q: rows with flt[q] <ops[q]> consts[q] (ops HPA_OP_LT..HPA_OP_NEQ), grouped on the i32
+ * column keys[q], with the f64 sum of vals[q] and the row count per group, groups in pfnGroup's
+ * order for i32 keys (h_libs.cc:1295-1329).  The plan lives in the current context's arena
+ * (free it with hpaSetHeapOffset after hpaBatchDestroy); NULL on error (hpaLastErrorDetail). */
+typedef struct hpa_batch hpa_batch;
+hpa_batch *hpaBatchCreate(int32_t nq, hpa_V *keys, hpa_V *vals, hpa_V *flts,
+                          const int32_t *ops, const int32_t *consts);
+void     hpaBatchDestroy(hpa_batch *b);
+/* One persistent launch over all queries; use_graph != 0 replays a captured hipGraph. */
+int32_t  hpaBatchRun(hpa_batch *b, int32_t use_graph);
+/* Query qi of the last run: zkey (I), zcnt (L), zsum (E), one element per group. */
+int32_t  hpaBatchResult(hpa_batch *b, int32_t qi, hpa_V zkey, hpa_V zcnt, hpa_V zsum);
+/* Tunables: block size, rows per work item, LDS key budget (0 = global tables only). */
+int32_t  hpaBatchSetConfig(hpa_batch *b, int32_t block, int32_t tile_rows, int32_t lds_keys);
+int32_t  hpaBatchGetConfig(hpa_batch *b, int32_t *block, int32_t *tile_rows, int32_t *lds_keys);
+/* Autotuner: times every candidate configuration on this plan and keeps the fastest. */
+int32_t  hpaBatchAutotune(hpa_batch *b, int32_t iters, float *ms, int32_t cap, int32_t *ncand);
 
 #ifdef __cplusplus
 }
