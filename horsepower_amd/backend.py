@@ -200,6 +200,22 @@ class GpuBackend(Backend):
                     C.byref(cond) if cond is not None else None, C.byref(body))
         return z
 
+    def fused_reduce_many(self, kind, xs, conds, body):
+        """Shared scan: one reduction of `body` per condition, one pass over the columns."""
+        n = len(conds)
+        zs = [self.node() for _ in range(n)]
+        self.lib.rc("hpaFusedReduceMany", n, (V * n)(*zs), kind, len(xs), (V * len(xs))(*xs),
+                    (Expr * n)(*conds), C.byref(body))
+        return zs
+
+    def gen_serve_tables(self, n_dim, seed):
+        """Serving tables generated in HBM: (dim, fact) dicts of resident column nodes."""
+        from .datagen import SERVE_DIM, SERVE_FACT
+        dim = [C.pointer(V0()) for _ in SERVE_DIM]         # Python-owned nodes: outlive arena resets
+        fact = [C.pointer(V0()) for _ in SERVE_FACT]
+        self.lib.rc("hpaGenServeTables", n_dim, seed, (V * 3)(*dim), (V * 8)(*fact))
+        return dict(zip(SERVE_DIM, dim)), dict(zip(SERVE_FACT, fact))
+
     def fused_map(self, typ, xs, e):
         z = self.node()
         xa = (V * len(xs))(*xs)

@@ -19,6 +19,7 @@ typedef int64_t L;
 struct hpa_ctx {
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    bool own_arena = true;         // false: the arena is a slice of a pool (hpaCtxSetArena)
     int  device = 0;
     // device bump arena (the reference's process heap, backend/h_memory.cc:93-112)
     char *dbase = nullptr; L dcur = 0, dtop = 0, dpeak = 0;
@@ -35,6 +36,7 @@ struct hpa_ctx {
 };
 void note_bound(hpa_ctx *c, const void *p, L bound);
 bool within_bound(hpa_ctx *c, const void *p, L limit);
+int32_t resident_alloc(V x, int32_t typ, L len);   // long-lived HBM payload (hpaFreeResident)
 
 hpa_ctx *ctx_cur();
 void    *dalloc(hpa_ctx *c, L bytes);               // nullptr when the arena is full

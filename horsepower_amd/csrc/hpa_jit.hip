@@ -107,7 +107,16 @@ static const char *ctype(int32_t t){
 
 struct CExpr { std::string s; bool f; };
 
-static int32_t expr_to_c(const hpa_expr *e, int nx, const int32_t *typ, CExpr *out, std::vector<int> &used){
+// Constants of the expressions are kernel arguments (a.ki / a.kf), not literals, so one compiled
+// kernel serves every value of a query's constants (a serving loop sees new predicate constants
+// with each request).  With kp == nullptr the constants are written as literals (used as a key to
+// recognise equal aggregate bodies).
+#define JIT_MAX_CONST 32
+#define KONST_FIELDS "i64 ki[32]; double kf[32];"
+struct Konst { int ni = 0, nf = 0; long long i[JIT_MAX_CONST]; double f[JIT_MAX_CONST]; };
+
+static int32_t expr_to_c(const hpa_expr *e, int nx, const int32_t *typ, CExpr *out, std::vector<int> &used,
+                         Konst *kp = nullptr){
     if(!e || e->n <= 0 || e->n > 256) R HPA_E_DOMAIN;
     std::vector<CExpr> st;
     char buf[128];
@@ -122,10 +131,20 @@ static int32_t expr_to_c(const hpa_expr *e, int nx, const int32_t *typ, CExpr *o
                 st.push_back({buf, f}); break;
             }
             case HPA_OP_CONST_I:
-                if(in.imm_i == INT64_MIN) snprintf(buf, sizeof(buf), "(-9223372036854775807LL-1)");
+                if(kp){
+                    if(kp->ni >= JIT_MAX_CONST) R HPA_E_DOMAIN;
+                    kp->i[kp->ni] = in.imm_i;
+                    snprintf(buf, sizeof(buf), "((i64)a.ki[%d])", kp->ni++);
+                } else if(in.imm_i == INT64_MIN) snprintf(buf, sizeof(buf), "(-9223372036854775807LL-1)");
                 else snprintf(buf, sizeof(buf), "((i64)%lldLL)", (long long)in.imm_i);
                 st.push_back({buf, false}); break;
-            case HPA_OP_CONST_F: snprintf(buf, sizeof(buf), "(%a)", in.imm_f); st.push_back({buf, true}); break;
+            case HPA_OP_CONST_F:
+                if(kp){
+                    if(kp->nf >= JIT_MAX_CONST) R HPA_E_DOMAIN;
+                    kp->f[kp->nf] = in.imm_f;
+                    snprintf(buf, sizeof(buf), "(a.kf[%d])", kp->nf++);
+                } else snprintf(buf, sizeof(buf), "(%a)", in.imm_f);
+                st.push_back({buf, true}); break;
             case HPA_OP_NOT: {
                 if(st.empty()) R HPA_E_DOMAIN;
                 CExpr a = st.back(); st.pop_back();
@@ -200,13 +219,13 @@ static std::string loads_code(int nx, const int32_t *typ, const std::vector<int>
 static const char *COND_TRUE = "((i64)1)";
 
 // ---------------------------------------------------------------- FA reduce (fa_reduce)
-struct JArgs { const void *c[EXPR_MAX_COLS]; long long n; void *out; void *aux; };
+struct JArgs { const void *c[EXPR_MAX_COLS]; long long n; void *out; void *aux; long long ki[JIT_MAX_CONST]; double kf[JIT_MAX_CONST]; };
 
 
-static int32_t split_used(int nx, const std::vector<int> &uc, const std::vector<int> &ub, std::vector<int> &late){
+// columns read only for selected rows (used by the body, not by the condition)
+static void split_used(int nx, const std::vector<int> &uc, const std::vector<int> &ub, std::vector<int> &late){
     late.assign(nx, 0);
     for(int k = 0; k < nx; k++) if(ub[k] && !uc[k]) late[k] = 1;
-    R 0;
 }
 
 // FA-reduce through generated code; HPA_E_NOT_IMPL when code generation is unavailable
@@ -216,13 +235,14 @@ int32_t jit_fused_reduce(hpa_ctx *c, int kind, int nx, V *x, const hpa_expr *con
     int32_t typ[EXPR_MAX_COLS]; for(int k = 0; k < nx; k++) typ[k] = x[k]->typ;
     std::vector<int> uc(nx, 0), ub(nx, 0), late;
     CExpr ce{COND_TRUE, false}, be;
+    Konst kc;
     int32_t e;
-    if(cond && (e = expr_to_c(cond, nx, typ, &ce, uc))) R e;
-    if((e = expr_to_c(body, nx, typ, &be, ub))) R e;
+    if(cond && (e = expr_to_c(cond, nx, typ, &ce, uc, &kc))) R e;
+    if((e = expr_to_c(body, nx, typ, &be, ub, &kc))) R e;
     split_used(nx, uc, ub, late);
     std::ostringstream s;
     s << PREAMBLE;
-    s << "struct A { const void *c[" << EXPR_MAX_COLS << "]; i64 n; void *out; void *aux; };\n"
+    s << "struct A { const void *c[" << EXPR_MAX_COLS << "]; i64 n; void *out; void *aux; " KONST_FIELDS " };\n"
          "struct P { KS s; u64 i, cnt; double mn, mx; i64 imn, imx; };\n"
          "extern \"C\" __global__ void __launch_bounds__(256) fa_reduce(A a){\n"
          "    __shared__ P sh[4];\n"
@@ -255,9 +275,65 @@ int32_t jit_fused_reduce(hpa_ctx *c, int kind, int nx, V *x, const hpa_expr *con
     if((e = jit_get(s.str(), "fa_reduce", &fn))) R e;
     JArgs a; memset(&a, 0, sizeof(a));
     for(int k = 0; k < nx; k++) a.c[k] = cols.p[k];
+    memcpy(a.ki, kc.i, sizeof(a.ki)); memcpy(a.kf, kc.f, sizeof(a.kf));
     a.n = n; a.out = part;
     double cb = 0; for(int k = 0; k < nx; k++) cb += (double)type_width(typ[k]);
     ProfScope ps("fused_reduce", c->stream, (double)n * cb);
+    R jit_launch(c, fn, (unsigned)G, 256, a);
+}
+
+// ---------------------------------------------------------------- shared-scan FA reduce (fa_many)
+// nq reductions of one body under nq conditions in one pass: each row's columns are loaded once
+// and every condition adds the body to its own Neumaier accumulator (sum / avg only).  Partials
+// are laid out [query][block] in the FaPart layout of fa_reduce.
+int32_t jit_fused_reduce_many(hpa_ctx *c, int nq, int nx, V *x, const hpa_expr *conds, const hpa_expr *body,
+                              const DevCols &cols, L n, void *part, int G){
+    if(!jit_enabled()) R HPA_E_NOT_IMPL;
+    int32_t typ[EXPR_MAX_COLS]; for(int k = 0; k < nx; k++) typ[k] = x[k]->typ;
+    std::vector<int> used(nx, 0);
+    Konst kc;
+    std::vector<CExpr> ce(nq);
+    CExpr be; int32_t e;
+    for(int q = 0; q < nq; q++) if((e = expr_to_c(&conds[q], nx, typ, &ce[q], used, &kc))) R e;
+    if((e = expr_to_c(body, nx, typ, &be, used, &kc))) R e;
+    std::ostringstream s;
+    s << PREAMBLE << "#define NQ " << nq << "\n"
+      << "struct A { const void *c[" << EXPR_MAX_COLS << "]; i64 n; void *out; void *aux; " KONST_FIELDS " };\n"
+         "struct P { KS s; u64 i, cnt; double mn, mx; i64 imn, imx; };\n"
+         "extern \"C\" __global__ void __launch_bounds__(256) fa_many(A a){\n"
+         "    __shared__ KS shs[4][NQ]; __shared__ u64 shi[4][NQ], shc[4][NQ];\n";
+    for(int q = 0; q < nq; q++) s << "    KS s" << q << "; s" << q << ".s = 0; s" << q << ".c = 0; u64 i" << q << " = 0, c" << q << " = 0;\n";
+    s << "    for(i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < a.n; i += (i64)gridDim.x * 256){\n"
+      << loads_code(nx, typ, used);
+    if(be.f) s << "        const double v = " << be.s << ";\n";
+    else s << "        const i64 v = " << be.s << ";\n";
+    for(int q = 0; q < nq; q++){
+        s << "        if(" << ce[q].s << "){ c" << q << "++; ";
+        if(be.f) s << "ks_add(s" << q << ", v); }\n";
+        else s << "i" << q << " += (u64)v; }\n";
+    }
+    s << "    }\n    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;\n";
+    for(int q = 0; q < nq; q++){
+        s << "    s" << q << " = ks_wave(s" << q << ");\n"
+             "    for(int o = 32; o > 0; o >>= 1){ i" << q << " += __shfl_down(i" << q << ", o, 64); c" << q << " += __shfl_down(c" << q << ", o, 64); }\n"
+             "    if(lane == 0){ shs[w][" << q << "] = s" << q << "; shi[w][" << q << "] = i" << q << "; shc[w][" << q << "] = c" << q << "; }\n";
+    }
+    s << "    __syncthreads();\n"
+         "    if(threadIdx.x < NQ){\n"
+         "        const int q = threadIdx.x;\n"
+         "        P r; r.s = shs[0][q]; r.i = shi[0][q]; r.cnt = shc[0][q];\n"
+         "        for(int ww = 1; ww < 4; ww++){ ks_merge(r.s, shs[ww][q]); r.i += shi[ww][q]; r.cnt += shc[ww][q]; }\n"
+         "        r.mn = 1.7976931348623157e308; r.mx = -1.7976931348623157e308; r.imn = 9223372036854775807LL; r.imx = -9223372036854775807LL - 1;\n"
+         "        ((P*)a.out)[(i64)q * gridDim.x + blockIdx.x] = r;\n"
+         "    }\n}\n";
+    hipFunction_t fn;
+    if((e = jit_get(s.str(), "fa_many", &fn))) R e;
+    JArgs a; memset(&a, 0, sizeof(a));
+    for(int k = 0; k < nx; k++) a.c[k] = cols.p[k];
+    memcpy(a.ki, kc.i, sizeof(a.ki)); memcpy(a.kf, kc.f, sizeof(a.kf));
+    a.n = n; a.out = part;
+    double cb = 0; for(int k = 0; k < nx; k++) if(used[k]) cb += (double)type_width(typ[k]);
+    ProfScope ps("fused_reduce_many", c->stream, (double)n * cb);
     R jit_launch(c, fn, (unsigned)G, 256, a);
 }
 
@@ -267,11 +343,12 @@ int32_t jit_fused_map(hpa_ctx *c, int otyp, int nx, V *x, const hpa_expr *ex, co
     int32_t typ[EXPR_MAX_COLS]; for(int k = 0; k < nx; k++) typ[k] = x[k]->typ;
     std::vector<int> used(nx, 0);
     CExpr be; int32_t e;
-    if((e = expr_to_c(ex, nx, typ, &be, used))) R e;
+    Konst kc;
+    if((e = expr_to_c(ex, nx, typ, &be, used, &kc))) R e;
     const char *ot = otyp == HPA_B ? "unsigned char" : otyp == HPA_I ? "int" : otyp == HPA_L ? "long long" : "double";
     std::ostringstream s;
     s << PREAMBLE
-      << "struct A { const void *c[" << EXPR_MAX_COLS << "]; i64 n; void *out; void *aux; };\n"
+      << "struct A { const void *c[" << EXPR_MAX_COLS << "]; i64 n; void *out; void *aux; " KONST_FIELDS " };\n"
          "extern \"C\" __global__ void __launch_bounds__(256) fe_map(A a){\n"
          "    for(i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < a.n; i += (i64)gridDim.x * 256){\n"
       << loads_code(nx, typ, used);
@@ -282,6 +359,7 @@ int32_t jit_fused_map(hpa_ctx *c, int otyp, int nx, V *x, const hpa_expr *ex, co
     if((e = jit_get(s.str(), "fe_map", &fn))) R e;
     JArgs a; memset(&a, 0, sizeof(a));
     for(int k = 0; k < nx; k++) a.c[k] = cols.p[k];
+    memcpy(a.ki, kc.i, sizeof(a.ki)); memcpy(a.kf, kc.f, sizeof(a.kf));
     a.n = n; a.out = dz;
     double cb = 0; for(int k = 0; k < nx; k++) if(used[k]) cb += (double)type_width(typ[k]);
     ProfScope ps("fused_map", c->stream, (double)n * (cb + type_width(otyp)));
@@ -300,7 +378,7 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
     s << PREAMBLE;
     s << "#define NS " << ns << "\n#define NA " << NA << "\n#define HT 64\n#define U " << U << "\n";
     s << "template<typename T> struct __attribute__((aligned(U * sizeof(T)))) VU { T v[U]; };\n"
-         "struct A { const void *c[" << EXPR_MAX_COLS << "]; const void *k0; const void *k1; i64 n; void *part; unsigned *used; int *ovf; };\n"
+         "struct A { const void *c[" << EXPR_MAX_COLS << "]; const void *k0; const void *k1; i64 n; void *part; unsigned *used; int *ovf; " KONST_FIELDS " };\n"
          "struct P { unsigned code, valid; u64 cnt; i64 first; double f[2 * NA]; i64 q[NA]; };\n"
          // LDS hash of key code -> slot: entry = code << 8 | slot; 0xFFFFFFFF empty; slot 0xFE busy, 0xFD overflow
          "__device__ __forceinline__ unsigned slot_of(unsigned *ht, unsigned *s_code, unsigned *s_used, unsigned code){\n"
@@ -557,10 +635,12 @@ extern "C" int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *
     if((e = cols_view(c, nx, x, &cols, &nn))) R e;
     if(nx > 0 && nn != n) R HPA_E_LENGTH;
     int32_t typ[EXPR_MAX_COLS]; for(int k = 0; k < nx; k++) typ[k] = x[k]->typ;
-    std::vector<int> uc(nx, 0), ub(nx, 0), late;
+    std::vector<int> uc(nx, 0), ub(nx, 0);
     CExpr ce{COND_TRUE, false};
-    if(cond && (e = expr_to_c(cond, nx, typ, &ce, uc))) R e;
+    Konst kc;
+    if(cond && (e = expr_to_c(cond, nx, typ, &ce, uc, &kc))) R e;
     std::vector<Acc> acc;
+    std::vector<std::string> acc_key;                        // bodies with literal constants
     FgaOut o; memset(&o, 0, sizeof(o));
     o.na = na; o.nk = nk;
     for(int a = 0; a < na; a++){
@@ -568,12 +648,15 @@ extern "C" int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *
         if(kd < 0 || kd > 4) R HPA_E_DOMAIN;
         o.kind[a] = kd; o.accid[a] = 0; o.accf[a] = 0;
         if(kd == 4) continue;
-        CExpr be; if((e = expr_to_c(aggs[a].body, nx, typ, &be, ub))) R e;
+        CExpr lit; if((e = expr_to_c(aggs[a].body, nx, typ, &lit, ub))) R e;
         int cls = (kd == 0 || kd == 1) ? 0 : kd;
         int id = -1;
-        for(int k = 0; k < (int)acc.size(); k++) if(acc[k].body == be.s && acc[k].cls == cls){ id = k; break; }
-        if(id < 0){ id = (int)acc.size(); acc.push_back({be.s, be.f, cls}); }
-        o.accid[a] = id; o.accf[a] = be.f ? 1 : 0;
+        for(int k = 0; k < (int)acc.size(); k++) if(acc_key[k] == lit.s && acc[k].cls == cls){ id = k; break; }
+        if(id < 0){
+            CExpr be; if((e = expr_to_c(aggs[a].body, nx, typ, &be, ub, &kc))) R e;
+            id = (int)acc.size(); acc.push_back({be.s, be.f, cls}); acc_key.push_back(lit.s);
+        }
+        o.accid[a] = id; o.accf[a] = lit.f ? 1 : 0;
     }
     std::vector<int> used(nx, 0);
     for(int k = 0; k < nx; k++) used[k] = uc[k] | ub[k];
@@ -596,9 +679,11 @@ extern "C" int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *
     double *fo = (double*)dalloc(c, 8 * GM * NA);
     long long *qo = (long long*)dalloc(c, 8 * GM * NA);
     if(!part || !usedb || !ovf || !codes || !gdev || !cnt || !first || !fo || !qo){ c->dcur = mark; R HPA_E_MEMORY; }
-    struct { const void *c[EXPR_MAX_COLS]; const void *k0; const void *k1; long long n; void *part; unsigned *used; int *ovf; } ga;
+    struct { const void *c[EXPR_MAX_COLS]; const void *k0; const void *k1; long long n; void *part; unsigned *used; int *ovf;
+             long long ki[JIT_MAX_CONST]; double kf[JIT_MAX_CONST]; } ga;
     memset(&ga, 0, sizeof(ga));
     for(int k = 0; k < nx; k++) ga.c[k] = cols.p[k];
+    memcpy(ga.ki, kc.i, sizeof(ga.ki)); memcpy(ga.kf, kc.f, sizeof(ga.kf));
     ga.k0 = pk0; ga.k1 = pk1; ga.n = n; ga.part = part; ga.used = usedb; ga.ovf = ovf;
     double cb = nk; for(int k = 0; k < nx; k++) if(used[k]) cb += (double)type_width(typ[k]);
     // slot counts tried in turn: fewer slots keep the accumulators in fewer registers; a block
@@ -645,17 +730,16 @@ extern "C" int32_t hpaJitCompileCheck(int32_t nx, const int32_t *coltyp, const h
                                       int32_t na, const hpa_agg *aggs){
     if(nx < 0 || nx > EXPR_MAX_COLS || na < 1 || na > 16) R HPA_E_DOMAIN;
     int32_t typ[EXPR_MAX_COLS]; for(int k = 0; k < nx; k++) typ[k] = coltyp[k];
-    std::vector<int> uc(nx, 0), ub(nx, 0), late;
+    std::vector<int> uc(nx, 0), ub(nx, 0);
     CExpr ce{COND_TRUE, false};
+    Konst kc;
     int32_t e;
-    if(cond && (e = expr_to_c(cond, nx, typ, &ce, uc))) R e;
+    if(cond && (e = expr_to_c(cond, nx, typ, &ce, uc, &kc))) R e;
     std::vector<Acc> acc;
     for(int a = 0; a < na; a++){
         if(aggs[a].kind == 4) continue;
-        CExpr be; if((e = expr_to_c(aggs[a].body, nx, typ, &be, ub))) R e;
-        int cls = aggs[a].kind <= 1 ? 0 : aggs[a].kind;
-        bool dup = false; for(auto &q : acc) if(q.body == be.s && q.cls == cls) dup = true;
-        if(!dup) acc.push_back({be.s, be.f, cls});
+        CExpr be; if((e = expr_to_c(aggs[a].body, nx, typ, &be, ub, &kc))) R e;
+        acc.push_back({be.s, be.f, aggs[a].kind <= 1 ? 0 : aggs[a].kind});
     }
     std::vector<int> used(nx, 0);
     for(int k = 0; k < nx; k++) used[k] = uc[k] | ub[k];

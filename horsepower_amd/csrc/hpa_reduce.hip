@@ -709,6 +709,51 @@ __global__ void k_fused_reduce_final(const FaPart *part, int np, FaPart *out){
     }
 }
 
+__global__ void k_fused_reduce_final_many(const FaPart *part, int np, FaPart *out){
+    const FaPart *p = part + (L)blockIdx.x * np;
+    if(threadIdx.x == 0){
+        FaPart r = p[0];
+        for(int q = 1; q < np; q++){ ks_merge(r.s, p[q].s); r.i += p[q].i; r.cnt += p[q].cnt; }
+        out[blockIdx.x] = r;
+    }
+}
+
+// Shared-scan FA reductions: z[q] = sum (kind 0) or avg (kind 1) of body over the rows where
+// conds[q] holds, for nq <= 8 conditions over the same columns, computed in one pass (each
+// result as hpaFusedReduce would give it alone).
+extern "C" int32_t hpaFusedReduceMany(int32_t nq, hpa_V *z, int32_t kind, int32_t nx, hpa_V *x,
+                                      const hpa_expr *conds, const hpa_expr *body){
+    hpa_ctx *c = ctx_cur();
+    if(kind < 0 || kind > 1 || nq < 1 || nq > 8) R HPA_E_DOMAIN;
+    DevCols cols; L n; int32_t e;
+    L mark = c->dcur;
+    if((e = cols_view(c, nx, x, &cols, &n))) R e;
+    int32_t ct[EXPR_MAX_COLS]; for(int k = 0; k < nx; k++) ct[k] = x[k]->typ;
+    DevExpr be;
+    if((e = expr_compile(body, nx, ct, &be))) R e;
+    bool bf = be.ins[be.n - 1].kf;
+    int G = sum_grid(n);
+    FaPart *part = (FaPart*)dalloc(c, sizeof(FaPart) * G * nq), *out = (FaPart*)dalloc(c, sizeof(FaPart) * nq);
+    if(!part || !out){ c->dcur = mark; R HPA_E_MEMORY; }
+    if(jit_fused_reduce_many(c, nq, nx, x, conds, body, cols, n, part, G) != 0){
+        c->dcur = mark;
+        for(int q = 0; q < nq; q++) if((e = hpaFusedReduce(z[q], kind, nx, x, &conds[q], body))) R e;
+        R 0;
+    }
+    hipLaunchKernelGGL(k_fused_reduce_final_many, dim3(nq), dim3(64), 0, c->stream, part, G, out);
+    if((e = check_launch("fused_reduce_many"))) R e;
+    FaPart h[8]; if((e = d2h(c, h, out, sizeof(FaPart) * nq))) R e;
+    c->dcur = mark;
+    for(int q = 0; q < nq; q++){
+        V zq = z[q];
+        zq->len = 1; zq->g = (char*)&zq->b; zq->i64 = 0;
+        if(bf){ double t = h[q].s.s + h[q].s.c; zq->typ = HPA_E; zq->f64 = kind == 0 ? t : t / (double)h[q].cnt; }
+        else if(kind == 1){ zq->typ = HPA_E; zq->f64 = (double)(int64_t)h[q].i / (double)h[q].cnt; }
+        else { zq->typ = HPA_L; zq->i64 = (int64_t)h[q].i; }
+    }
+    R 0;
+}
+
 // FA-reduce (optimizer/autofusion.cc:795-817,886-917): sum c += body over rows where cond
 // holds; avg c/tot; min/max seed with the type's extreme as the emitted code does.
 extern "C" int32_t hpaFusedReduce(hpa_V z, int32_t kind, int32_t nx, hpa_V *x, const hpa_expr *cond, const hpa_expr *body){

@@ -76,7 +76,10 @@ static hpa_ctx *ctx_new(L arena_bytes, L host_bytes){
     if(hipSetDevice(g_device) != hipSuccess){ delete c; R nullptr; }
     if(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess){ delete c; R nullptr; }
     c->own_stream = true;
-    if(arena_bytes <= 0) arena_bytes = 17179869184LL;          // 16 GiB (backend/h_memory.cc:11)
+    if(arena_bytes < 0){
+        c->own_arena = false;                                  // the caller attaches a pool slice
+    } else {
+    if(arena_bytes == 0) arena_bytes = 17179869184LL;          // 16 GiB (backend/h_memory.cc:11)
     size_t freeb = 0, totalb = 0;
     if(hipMemGetInfo(&freeb, &totalb) == hipSuccess && (size_t)arena_bytes > freeb - (512ull<<20))
         arena_bytes = (L)(freeb > (1ull<<30) ? freeb - (512ull<<20) : freeb/2);
@@ -85,6 +88,7 @@ static hpa_ctx *ctx_new(L arena_bytes, L host_bytes){
     }
     c->dtop = arena_bytes;
     register_device_range(c->dbase, arena_bytes);
+    }
     c->hbase = (char*)malloc((size_t)host_bytes); c->htop = host_bytes;
     c->pinned_bytes = 1 << 20;
     if(hipHostMalloc((void**)&c->pinned, (size_t)c->pinned_bytes, hipHostMallocDefault) != hipSuccess) c->pinned = nullptr;
@@ -110,7 +114,7 @@ extern "C" int32_t hpaInit(int32_t device, int64_t arena_bytes){
 extern "C" void hpaCtxDestroy(hpa_ctx *c){
     if(!c) R;
     hipStreamSynchronize(c->stream);
-    if(c->dbase){ unregister_device_range(c->dbase); hipFree(c->dbase); }
+    if(c->dbase && c->own_arena){ unregister_device_range(c->dbase); hipFree(c->dbase); }
     if(c->own_stream) hipStreamDestroy(c->stream);
     if(c->pinned) hipHostFree(c->pinned);
     for(void *p : c->host_bufs) free(p);
@@ -133,7 +137,37 @@ extern "C" void hpaShutdown(void){
 
 extern "C" hpa_ctx *hpaCtxCreate(int64_t arena_bytes){
     if(!g_init && hpaInit(0, 0)) R nullptr;
-    R ctx_new(arena_bytes > 0 ? arena_bytes : (1LL<<30), 64LL << 20);
+    R ctx_new(arena_bytes > 0 ? arena_bytes : arena_bytes < 0 ? -1 : (1LL<<30), 64LL << 20);
+}
+
+// A long-lived device allocation registered as device memory (a pool that contexts' arenas are
+// carved from, hpaCtxSetArena).
+extern "C" void *hpaDeviceAlloc(int64_t bytes){
+    if(!g_init && hpaInit(0, 0)) R nullptr;
+    void *p = nullptr;
+    if(bytes <= 0 || hipMalloc(&p, (size_t)bytes) != hipSuccess){ set_error_detail("hipMalloc of %lld bytes failed", (long long)bytes); R nullptr; }
+    register_device_range(p, bytes);
+    R p;
+}
+extern "C" int32_t hpaDeviceFree(void *p){
+    if(!p) R 0;
+    hipDeviceSynchronize();
+    unregister_device_range(p);
+    R hipFree(p) == hipSuccess ? 0 : HPA_E_DEVICE;
+}
+
+// Point a context created with a negative arena size at [base, base+bytes) of a registered
+// device allocation and rewind it (device arena, host nodes, marks): the next request starts on
+// an empty arena of exactly that budget.  The context's stream must be idle.
+extern "C" int32_t hpaCtxSetArena(hpa_ctx *c, void *base, int64_t bytes){
+    if(!c || c->own_arena || bytes < 0){ set_error_detail("hpaCtxSetArena: context owns its arena"); R HPA_E_DOMAIN; }
+    if(base && !(is_device(base) && is_device((char*)base + bytes - 1))){ set_error_detail("hpaCtxSetArena: not a device pool range"); R HPA_E_DOMAIN; }
+    if(hipStreamSynchronize(c->stream) != hipSuccess) R HPA_E_DEVICE;
+    c->dbase = (char*)base; c->dtop = base ? bytes : 0; c->dcur = 0; c->dpeak = 0;
+    c->hcur = 0; c->marks.clear(); c->bounds.clear();
+    for(void *p : c->host_bufs) free(p);
+    c->host_bufs.clear();
+    R 0;
 }
 extern "C" hpa_ctx *hpaCtxDefault(void){ R g_default; }
 extern "C" void hpaCtxSetCurrent(hpa_ctx *c){ t_cur = c; }
@@ -280,6 +314,18 @@ int32_t read_scalar(hpa_ctx *c, V x, void *dst, L width){
 }
 
 // ---------------------------------------------------------------- residency
+// A long-lived HBM payload for x (typ, len): kept across heap resets, released by
+// hpaFreeResident.  Used by device-side table generation.
+int32_t resident_alloc(V x, int32_t typ, L len){
+    L bytes = type_bytes(typ, len);
+    void *d = nullptr;
+    if(hipMalloc(&d, (size_t)(bytes > 0 ? bytes : 1)) != hipSuccess){ set_error_detail("hipMalloc(%lld) failed", (long long)bytes); R HPA_E_MEMORY; }
+    register_device_range(d, bytes > 0 ? bytes : 1);
+    { std::lock_guard<std::mutex> lk(g_mu); g_resident[(uintptr_t)d] = bytes; }
+    x->typ = typ; x->len = len; x->g = (char*)d;
+    R 0;
+}
+
 static int32_t upload_rec(hpa_ctx *c, V x){
     if(is_basic(x->typ)){
         if(is_inline(x) || x->len <= 1 || is_device(x->g)) R 0;

@@ -107,6 +107,33 @@ def gen_q3(n_dim, seed=1002):
     return dim, fact
 
 
+SERVE_FACT = ("fk", "ship_day", "price", "discount", "quantity", "tax", "return_flag", "line_status")
+SERVE_DIM = ("key", "order_day", "segment")
+
+
+def gen_serve(n_dim, seed=1004):
+    """Orders and one lineitem table carrying the columns of q1, q3 and q6 (the serving
+    tables); the host twin of the device generator hpaGenServeTables."""
+    perm = _affine_perm(n_dim, seed)
+    key = (4 * perm + 1 + uniform(seed, 11, n_dim, 0, 3, np.int64)).astype(np.int32)
+    order_day = uniform(seed, 12, n_dim, 0, 2405)
+    segment = uniform(seed, 13, n_dim, 0, 4)
+    per = uniform(seed, 14, n_dim, 1, 7, np.int64)
+    owner = np.repeat(np.arange(n_dim, dtype=np.int64), per)
+    nf = owner.shape[0]
+    ship = (order_day[owner] + uniform(seed, 15, nf, 1, 121)).astype(np.int32)
+    receipt = ship + uniform(seed, 20, nf, 1, 30)
+    ra = np.where(uniform(seed, 21, nf, 0, 1) == 1, ord("R"), ord("A"))
+    fact = dict(
+        fk=key[owner], ship_day=ship,
+        price=cents(seed, 16, nf, 90000, 10500000), discount=cents(seed, 17, nf, 0, 10),
+        quantity=uniform(seed, 18, nf, 1, 50), tax=cents(seed, 19, nf, 0, 8),
+        return_flag=np.where(receipt > Q1_CUT, ord("N"), ra).astype(np.uint8),
+        line_status=np.where(ship > Q1_CUT, ord("O"), ord("F")).astype(np.uint8),
+    )
+    return dict(key=key, order_day=order_day, segment=segment), fact
+
+
 def gen_batch_query(q, n, card, seed=1003):
     """One batched filter->compress->group->sum query: keys uniform over `card` values."""
     s = seed * 1000 + q

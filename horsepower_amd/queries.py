@@ -55,6 +55,15 @@ def q6_fused(B, T, **kw):
     return {"revenue": B.read_scalar(z)}
 
 
+def q6_many(B, T, params):
+    """Several Q6 requests (one dict of q6 constants each) in one shared scan: one pass over
+    the four columns computes every request's revenue."""
+    cols = [T["ship_day"], T["quantity"], T["discount"], T["price"]]
+    body = expr([("COL", 3), ("COL", 2), "MUL"])
+    zs = B.fused_reduce_many(0, cols, [q6_cond(**p) for p in params], body)
+    return [{"revenue": B.read_scalar(z)} for z in zs]
+
+
 # ------------------------------------------------------------------------------------- Q1
 Q1_NAMES = ("return_flag", "line_status", "sum_qty", "sum_base_price", "sum_disc_price",
             "sum_charge", "avg_qty", "avg_price", "avg_disc", "count_order")
@@ -153,8 +162,8 @@ def q1_chf(B, T, ship_max=D.Q1_SHIP_MAX):
 Q3_NAMES = ("orderkey", "revenue", "order_day")
 
 
-def q3_plain(B, Dm, F, day=D.Q3_DAY, limit=10):
-    d_mask = B.pfnAnd(B.pfnEq(Dm["segment"], B.scalar(1, H_I)), B.pfnLt(Dm["order_day"], B.scalar(day, H_I)))
+def q3_plain(B, Dm, F, day=D.Q3_DAY, limit=10, segment=1):
+    d_mask = B.pfnAnd(B.pfnEq(Dm["segment"], B.scalar(segment, H_I)), B.pfnLt(Dm["order_day"], B.scalar(day, H_I)))
     lk = B.pfnCompress(d_mask, Dm["key"])
     lday = B.pfnCompress(d_mask, Dm["order_day"])
     f_mask = B.pfnGt(F["ship_day"], B.scalar(day, H_I))
@@ -172,9 +181,11 @@ def q3_plain(B, Dm, F, day=D.Q3_DAY, limit=10):
     oday = B.pfnIndex(lday, j0)
     okey = B.pfnIndex(lk, j0)
     perm = B.pfnOrderBy(B.list([revenue, oday]), B.bools([0, 1]))
-    res = {k: B.read(B.pfnIndex(v, perm)) for k, v in zip(Q3_NAMES, (okey, revenue, oday))}
-    res["rows"] = int(B.read(perm).shape[0])
-    return {k: (v[:limit] if isinstance(v, np.ndarray) else v) for k, v in res.items()}
+    rows = int(perm.contents.len)
+    head = B.pfnIndex(perm, B.host.vec(np.arange(min(limit, rows), dtype=np.int64), H_L))   # limit
+    res = dict(zip(Q3_NAMES, B.read_many([B.pfnIndex(v, head) for v in (okey, revenue, oday)])))
+    res["rows"] = rows
+    return res
 
 
 # ---------------------------------------------------------------------- batched group-by

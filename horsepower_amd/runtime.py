@@ -59,6 +59,9 @@ class Device(PrimitiveLib):
         _bind(dll, "hpaSync", i32, [])
         _bind(dll, "hpaCtxCreate", vp, [i64])
         _bind(dll, "hpaCtxDestroy", None, [vp])
+        _bind(dll, "hpaDeviceAlloc", vp, [i64])
+        _bind(dll, "hpaDeviceFree", i32, [vp])
+        _bind(dll, "hpaCtxSetArena", i32, [vp, vp, i64])
         _bind(dll, "hpaCtxDefault", vp, [])
         _bind(dll, "hpaCtxSetCurrent", None, [vp])
         _bind(dll, "hpaCtxCurrent", vp, [])
@@ -74,6 +77,7 @@ class Device(PrimitiveLib):
         _bind(dll, "hpaSymbolName", C.c_char_p, [i64])
         _bind(dll, "hpaFusedMap", i32, [V, i32, i32, C.POINTER(V), C.POINTER(Expr)])
         _bind(dll, "hpaFusedReduce", i32, [V, i32, i32, C.POINTER(V), C.POINTER(Expr), C.POINTER(Expr)])
+        _bind(dll, "hpaFusedReduceMany", i32, [i32, C.POINTER(V), i32, i32, C.POINTER(V), C.POINTER(Expr), C.POINTER(Expr)])
         _bind(dll, "hpaFusedCompress", i32, [i32, C.POINTER(V), C.POINTER(V), i32, C.POINTER(V), C.POINTER(Expr)])
         _bind(dll, "hpaCompressMany", i32, [i32, C.POINTER(V), V, C.POINTER(V)])
         _bind(dll, "hpaGroupReduce", i32, [V, i32, V, V])
@@ -83,6 +87,8 @@ class Device(PrimitiveLib):
         _bind(dll, "hpaBatchCreate", vp, [i32, C.POINTER(V), C.POINTER(V), C.POINTER(V),
                                           C.POINTER(i32), C.POINTER(i32)])
         _bind(dll, "hpaBatchDestroy", None, [vp])
+        _bind(dll, "hpaGenUniform", i32, [V, i32, i64, i64, C.c_uint64, C.c_uint64, i64, i64])
+        _bind(dll, "hpaGenServeTables", i32, [i64, C.c_uint64, C.POINTER(V), C.POINTER(V)])
         _bind(dll, "hpaBatchRun", i32, [vp, i32])
         _bind(dll, "hpaBatchResult", i32, [vp, i32, V, V, V])
         _bind(dll, "hpaBatchSetConfig", i32, [vp, i32, i32, i32])

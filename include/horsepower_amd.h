@@ -141,7 +141,11 @@ int32_t  hpaSync(void);
 
 /* Contexts: a stream plus device/host arenas.  The calling thread's current context is the
  * target of every pfn* call. */
-hpa_ctx *hpaCtxCreate(int64_t arena_bytes);
+hpa_ctx *hpaCtxCreate(int64_t arena_bytes);          /* < 0: no arena until hpaCtxSetArena */
+/* Registered device pool, and per-request arena slices of it (memory-budget admission). */
+void    *hpaDeviceAlloc(int64_t bytes);
+int32_t  hpaDeviceFree(void *p);
+int32_t  hpaCtxSetArena(hpa_ctx *c, void *base, int64_t bytes);
 void     hpaCtxDestroy(hpa_ctx *c);
 hpa_ctx *hpaCtxDefault(void);
 void     hpaCtxSetCurrent(hpa_ctx *c);
@@ -228,6 +232,11 @@ int32_t hpaFusedMap(hpa_V z, int32_t typ, int32_t nx, hpa_V *x, const hpa_expr *
  * cond may be NULL (all rows).  z is E for an E body, else L (sum) / same as body. */
 int32_t hpaFusedReduce(hpa_V z, int32_t kind, int32_t nx, hpa_V *x,
                        const hpa_expr *cond, const hpa_expr *body);
+/* Shared scan of nq <= 8 FA reductions (sum kind 0, avg kind 1) of one body under nq
+ * conditions (conds[q]) over the same columns: z[q] equals hpaFusedReduce(z[q], kind, nx, x,
+ * &conds[q], body), computed in one pass over the columns. */
+int32_t hpaFusedReduceMany(int32_t nq, hpa_V *z, int32_t kind, int32_t nx, hpa_V *x,
+                           const hpa_expr *conds, const hpa_expr *body);
 /* FA compress scan (optimizer/autofusion.cc:886-899) and the multi-column peephole
  * (src/horseir/v1/compile-static/cur/q6.h:12-26): z[k] = compress(cond, y[k]) for k < ny,
  * one mask evaluation, order preserving. */
@@ -253,6 +262,14 @@ int32_t hpaCompressMany(int32_t ny, hpa_V *z, hpa_V mask, hpa_V *y);
  * t of y (flat or normal list of i64 row indices), z[g] = reduce_{j in t}(x[j]);
  * kind 0 sum, 1 avg, 2 min, 3 max, 4 len.  z is a vector over groups. */
 int32_t hpaGroupReduce(hpa_V z, int32_t kind, hpa_V x, hpa_V y);
+
+/* ---- synthetic tables generated in HBM (benchmarks, serving): counter-based splitmix64
+ * columns, the values of horsepower_amd/datagen.py.  Payloads are resident (hpaFreeResident). */
+int32_t hpaGenUniform(hpa_V z, int32_t typ, int64_t n, int64_t start, uint64_t seed, uint64_t stream,
+                      int64_t lo, int64_t hi);
+/* dim[3] = orders(key, order_day, segment), fact[8] = lineitem(fk, ship_day, price, discount,
+ * quantity, tax, return_flag, line_status), 1..7 lineitem rows per order. */
+int32_t hpaGenServeTables(int64_t n_dim, uint64_t seed, hpa_V *dim, hpa_V *fact);
 
 /* ---- batched execution (no single reference counterpart: it runs, for many queries at once,
  * the reference's per-query sequence compare -> compress -> group -> FP1/FP7 grouped sum/len,
