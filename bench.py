@@ -17,7 +17,15 @@ key ranges log-uniform 4..1e6) in one persistent launch replayed from a captured
 the autotuner's configuration; `value` is queries per second, compared with the same 64 queries
 run one by one through the per-query primitive plans on the GPU.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload q1|q6|q3|batch] [--rows N]
+Workload "prims": every hot-path primitive alone at the configs' sizes (60M-row columns of the
+group-by table; 15M-row sorts and groups; 1.5M x 6M joins), device time per call from HIP events
+around the whole call, algorithmic bytes per call, fraction of the 8 TB/s HBM peak.
+
+Workload "serve": a Poisson stream of mixed Q6/Q1/Q3 requests with random constants over a
+600M-row lineitem and 150M-row orders table generated in HBM; the server (continuous batching,
+memory-budget admission, one stream per slot) against the same stream run one request at a time.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload q1|q6|q3|batch|prims|serve]
 """
 import argparse
 import json
@@ -36,12 +44,15 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="q1", choices=["q1", "q6", "q3", "batch"])
+    p.add_argument("--workload", default="q1", choices=["q1", "q6", "q3", "batch", "prims", "serve"])
     p.add_argument("--rows", type=int, default=0)
     p.add_argument("--plan", default="best", choices=["best", "fused", "plain"])
     p.add_argument("--cpu-rows", type=int, default=0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--profile-steps", type=int, default=3)
+    p.add_argument("--requests", type=int, default=120)
+    p.add_argument("--rate", type=float, default=0.0)
+    p.add_argument("--slots", type=int, default=4)
     return p.parse_args()
 
 
@@ -80,6 +91,10 @@ def main():
 
     if a.workload == "batch":
         return bench_batch(a)
+    if a.workload == "prims":
+        return bench_prims(a)
+    if a.workload == "serve":
+        return bench_serve(a)
     W = workload(a.workload, a.rows)
     t0 = time.time()
     cols = W["gen"](W["n"] if W["n"] else W["nd"])
@@ -159,13 +174,17 @@ def main():
 
 def profile_kernels(lib, fn, steps):
     """Per-scope device time and algorithmic bytes from the library's HIP-event profiler."""
-    import ctypes as C
     lib.dll.hpaProfileReset()
     lib.dll.hpaProfileEnable(1)
     for _ in range(steps):
         fn()
     lib.sync()
     lib.dll.hpaProfileEnable(0)
+    return read_profile(lib)
+
+
+def read_profile(lib):
+    import ctypes as C
     buf = C.create_string_buffer(1 << 16)
     lib.dll.hpaProfileList(buf, 1 << 16)
     kern = {}
@@ -285,6 +304,142 @@ def bench_batch(a):
     }
     print(json.dumps(out), flush=True)
     b.close()
+
+
+def bench_prims(a):
+    import ctypes as C
+    import numpy as np
+    from horsepower_amd import datagen as D, queries as Q
+    from horsepower_amd.backend import GpuBackend
+    from horsepower_amd.vnode import H_E, H_I, H_L
+    n = a.rows or 60_000_000
+    ns = max(n // 4, 1)                       # sorts, groups and hash builds: config 2's 15M rows
+    t0 = time.time()
+    t = D.gen_q1(n)
+    B = GpuBackend()
+    T = Q.load_table(B, t)
+    key = B.vec(D.uniform(7, 1, ns, 0, 999_999))                   # 1e6-key i32 group column
+    dim_keys = D.gen_q3(ns // 10)[0]["key"]
+    build = B.vec(dim_keys)                                        # unique sparse keys
+    probe = B.vec(dim_keys[D.uniform(7, 2, 4 * len(dim_keys), 0, len(dim_keys) - 1, np.int64)])
+    idx = B.vec(D.uniform(7, 3, n // 2, 0, n - 1, np.int64))
+    print(f"[bench] prims inputs ready in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    lib = B.lib
+    m1 = B.pfnLeq(T["ship_day"], B.scalar(2434, H_I))
+    m6 = B.pfnAnd(B.pfnGeq(T["ship_day"], B.scalar(730, H_I)), B.pfnLt(T["ship_day"], B.scalar(1095, H_I)))
+    k1 = int(B.read_scalar(B.pfnSum(m1)))
+    k6 = int(B.read_scalar(B.pfnSum(m6)))
+    B.sync()
+    mark = B.mark()
+    nb, npb = len(dim_keys), 4 * len(dim_keys)
+    cond6 = Q.q6_cond()
+    from horsepower_amd.vnode import expr
+    cases = [
+        ("compare_i32", 5 * n, lambda: B.pfnLeq(T["ship_day"], B.scalar(2434, H_I))),
+        ("compare_f64", 9 * n, lambda: B.pfnGeq(T["discount"], B.scalar(0.05, H_E))),
+        ("and", 3 * n, lambda: B.pfnAnd(m1, m6)),
+        ("arith_EE", 24 * n, lambda: B.pfnMul(T["price"], T["discount"])),
+        ("arith_sE", 16 * n, lambda: B.pfnMinus(B.scalar(1, H_L), T["discount"])),
+        ("compress_E_96pct", n * 9 + 8 * k1, lambda: B.pfnCompress(m1, T["price"])),
+        ("compress_E_14pct", n * 9 + 8 * k6, lambda: B.pfnCompress(m6, T["price"])),
+        ("where_14pct", n + 8 * k6, lambda: B.pfnWhere(m6)),
+        ("index_E", (n // 2) * (8 + 16), lambda: B.pfnIndex(T["price"], idx)),
+        ("sum_E", 8 * n, lambda: B.pfnSum(T["price"])),
+        ("group_trie", 2 * n + 8 * n + 8 * 4, lambda: B.pfnGroup(B.list([T["return_flag"], T["line_status"]]))),
+        ("group_i32_1e6keys", 12 * ns + 8 * 1_000_000, lambda: B.pfnGroup(key)),
+        ("order_i32", 12 * ns, lambda: B.pfnOrderBy(B.list([key]), B.bools([1]))),
+        ("join_i32", 4 * (nb + npb) + 16 * npb, lambda: B.pfnJoinIndex(build, probe, B.sym("eq"))),
+        ("index_of_i32", 4 * (nb + npb) + 8 * npb, lambda: B.pfnIndexOf(build, probe)),
+        ("member_i32", 4 * (nb + npb) + npb, lambda: B.pfnMember(probe, build)),
+        ("fused_q6_reduce", 24 * n, lambda: B.fused_reduce(0, [T["ship_day"], T["quantity"], T["discount"], T["price"]],
+                                                          cond6, expr([("COL", 3), ("COL", 2), "MUL"]))),
+    ]
+    reps = max(a.steps // 4, 3)
+    res = {}
+    for name, nbytes, fn in cases:
+        fn()
+        B.release(mark)
+        B.sync()
+        lib.dll.hpaProfileReset()
+        lib.dll.hpaProfileEnable(1)
+        for _ in range(reps):
+            lib.dll.hpaProfileBegin(name.encode(), float(nbytes))
+            fn()
+            lib.dll.hpaProfileEnd()
+            B.release(mark)
+        B.sync()
+        lib.dll.hpaProfileEnable(0)
+        kern = read_profile(B.lib)
+        ms = kern[name]["ms"] / max(kern[name]["count"], 1)
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        inner = {k: round(v["ms"] * 1e3 / v["count"], 2) for k, v in kern.items() if k != name}
+        res[name] = {"bytes": nbytes, "us": round(ms * 1e3, 2), "GBs": round(gbs, 1),
+                     "frac": round(gbs / PEAK_HBM_GBS, 4), "launch_scopes_us": inner}
+        print(f"[bench] {name:20s} {ms * 1e3:9.1f} us {gbs:8.1f} GB/s {gbs / PEAK_HBM_GBS:6.1%}  {inner}", file=sys.stderr, flush=True)
+    import math
+    gm = math.exp(sum(math.log(v["GBs"]) for v in res.values()) / len(res))
+    out = {"metric": "rows/s + % HBM roofline per query; batched queries/s & p99 latency vs plain",
+           "value": round(gm, 1), "unit": "GB/s (geometric mean over primitives)", "n_gpus": 1,
+           "steps": reps, "warmup": 1, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "mixed (i32/f64/u8 as each primitive)",
+           "data": "synthetic (seeded counter-based generator)",
+           "config": {"workload": f"each primitive alone: {n}-row columns, {ns}-row sorts/groups, "
+                                  f"join/index_of/member {nb} x {npb}", "rows": n},
+           "primitives": res}
+    print(json.dumps(out), flush=True)
+
+
+def bench_serve(a):
+    import numpy as np
+    from horsepower_amd import serve as S
+    from horsepower_amd.backend import GpuBackend
+    n_dim = a.rows or 150_000_000
+    # the one-at-a-time path runs on the default context: its arena must hold a Q3 over 600M rows
+    os.environ.setdefault("HPA_ARENA_BYTES", str(120 << 30))
+    B = GpuBackend()
+    t0 = time.time()
+    dim, fact = B.gen_serve_tables(n_dim, 1004)
+    T = {"dim": dim, "fact": fact}
+    nf = fact["fk"].contents.len
+    print(f"[bench] serve tables: {n_dim} orders, {nf} lineitems in HBM ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    srv = S.Server(B, T, slots=a.slots, pool_bytes=80 << 30, merge=True)
+    est = srv.calibrate()
+    print(f"[bench] calibrated arena estimates (MiB): { {k: v >> 20 for k, v in est.items()} }", file=sys.stderr, flush=True)
+    # service time of the per-query fused path sets the offered load
+    probe = S.make_requests(12, rate=None, seed=99)
+    S.serve_serial(B, T, probe, plans="fused", realtime=False)
+    mean_s = float(np.mean([r.done - r.admit for r in probe]))
+    rate = a.rate or 0.9 / mean_s
+    print(f"[bench] serial fused mean service {mean_s * 1e3:.2f} ms -> rate {rate:.1f}/s", file=sys.stderr, flush=True)
+    N = a.requests
+    runs = {}
+    runs["server"] = S.summarize(srv.serve(S.make_requests(N, rate, seed=2024)))
+    print(f"[bench] server: {runs['server']}", file=sys.stderr, flush=True)
+    ser = S.serve_serial(B, T, S.make_requests(N, rate, seed=2024), plans="fused")
+    runs["serial_fused"] = S.summarize(ser)
+    print(f"[bench] serial fused: {runs['serial_fused']}", file=sys.stderr, flush=True)
+    closed_srv = srv.serve(S.make_requests(N, None, seed=2025), realtime=False)
+    runs["server_closed"] = S.summarize(closed_srv)
+    closed_ser = S.serve_serial(B, T, S.make_requests(N, None, seed=2025), plans="fused", realtime=False)
+    runs["serial_fused_closed"] = S.summarize(closed_ser)
+    plain = S.serve_serial(B, T, S.make_requests(N, None, seed=2025), plans="plain", realtime=False)
+    runs["serial_plain_closed"] = S.summarize(plain)
+    print(f"[bench] closed loop: server {runs['server_closed']['queries_per_s']:.1f} q/s, serial fused "
+          f"{runs['serial_fused_closed']['queries_per_s']:.1f} q/s, serial plain {runs['serial_plain_closed']['queries_per_s']:.1f} q/s",
+          file=sys.stderr, flush=True)
+    equal = S.same_results(closed_srv, closed_ser) == [] and S.same_results(closed_srv, plain) == []
+    srv.close()
+    out = {"metric": "rows/s + % HBM roofline per query; batched queries/s & p99 latency vs plain",
+           "value": runs["server"]["queries_per_s"], "unit": "queries/s", "n_gpus": 1, "steps": N, "warmup": 12,
+           "ms_per_step": runs["server"]["p50_ms"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded counter-based generator, in HBM)",
+           "config": {"workload": f"serving: Poisson mix of q6/q1/q3 requests (45/35/20%) with random constants, "
+                                  f"{nf} lineitem + {n_dim} order rows resident", "rate_per_s": rate,
+                      "slots": a.slots, "requests": N},
+           "latency_ms": {"p50": runs["server"]["p50_ms"], "p99": runs["server"]["p99_ms"]},
+           "runs": runs, "equal_results": equal, "arena_estimates_mib": {k: v >> 20 for k, v in est.items()},
+           "peak_pool_in_use_mib": srv.peak_in_use >> 20}
+    print(json.dumps(out), flush=True)
 
 
 def cpu_baseline(a, W):

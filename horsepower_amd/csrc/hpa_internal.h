@@ -84,6 +84,14 @@ bool prof_enabled();
 // ---------------------------------------------------------------- launch geometry
 int  num_cus();
 inline L ceil_div(L a, L b){ R (a + b - 1) / b; }
+// Grid of a whole-array reduction that ends in one global atomic per workgroup: few enough
+// workgroups that the atomics to one address do not serialise the kernel.
+inline int reduce_grid(L n, int block){
+    L g = ceil_div(n, (L)block * 16), cap = (L)num_cus() * 2;
+    if(g > cap) g = cap;
+    if(g < 1) g = 1;
+    R (int)g;
+}
 inline int grid_for(L n, int block, int per_thread, int max_blocks = 0){
     L g = ceil_div(n, (L)block * per_thread);
     L cap = max_blocks > 0 ? max_blocks : (L)num_cus() * 16;

@@ -191,6 +191,7 @@ static int32_t expr_to_c(const hpa_expr *e, int nx, const int32_t *typ, CExpr *o
 static const char *PREAMBLE = R"(
 typedef long long i64; typedef unsigned long long u64;
 #define NCOL 12
+template<typename T> struct __attribute__((aligned(4 * sizeof(T)))) V4 { T v[4]; };
 struct KS { double s, c; };
 __device__ __forceinline__ i64 hdiv(i64 a, i64 b){ return b == 0 ? 0 : (b == -1 ? (i64)(0ULL - (u64)a) : a / b); }
 __device__ __forceinline__ void ks_add(KS &a, double v){
@@ -228,6 +229,67 @@ static void split_used(int nx, const std::vector<int> &uc, const std::vector<int
     for(int k = 0; k < nx; k++) if(ub[k] && !uc[k]) late[k] = 1;
 }
 
+// The generated loops read 4 consecutive rows with one vector load: every column they read must
+// start on a 4-element boundary (allocation starts always do; anything else takes the
+// interpreted path).
+static bool cols_aligned(int nx, const int32_t *typ, const std::vector<int> &used, const DevCols &cols){
+    for(int k = 0; k < nx; k++)
+        if(used[k] && ((uintptr_t)cols.p[k] % (uintptr_t)(4 * type_width(typ[k]))) != 0) R false;
+    R true;
+}
+
+// Row loop shared by the generated FA reductions: 4 consecutive rows per thread and iteration,
+// visited in row order.  Condition columns are read with one vector load per column; columns that
+// only the body reads are loaded when some of the four rows passes some condition (at low
+// selectivity most of their cache lines are then never fetched).  update(q) is the code run for
+// a row that passes condition q, with the row's values bound to v<k> and the body value to `v`.
+template<typename F>
+static std::string fa_row_loop(int nx, const int32_t *typ, const std::vector<int> &uc, const std::vector<int> &late,
+                               const std::vector<CExpr> &conds, const CExpr &body, F update){
+    const int nq = (int)conds.size();
+    std::ostringstream s;
+    auto bind = [&](const std::vector<int> &cols, const char *w, int u){
+        std::ostringstream b;
+        for(int k = 0; k < nx; k++) if(cols[k]) b << " const " << ctype(typ[k]) << " v" << k << " = " << w << k << ".v[" << u << "];";
+        return b.str();
+    };
+    const char *bt = body.f ? "double" : "i64";
+    s << "    const i64 stride = (i64)gridDim.x * 1024;\n"
+         "    for(i64 base = ((i64)blockIdx.x * 256 + threadIdx.x) * 4; base < a.n; base += stride){\n"
+         "        if(base + 4 <= a.n){\n";
+    for(int k = 0; k < nx; k++) if(uc[k])
+        s << "            const V4<" << ctype(typ[k]) << "> w" << k << " = *(const V4<" << ctype(typ[k]) << ">*)((const " << ctype(typ[k]) << "*)a.c[" << k << "] + base);\n";
+    for(int u = 0; u < 4; u++) for(int q = 0; q < nq; q++) s << "            bool m" << u << "_" << q << ";\n";
+    for(int u = 0; u < 4; u++){
+        s << "            {" << bind(uc, "w", u);
+        for(int q = 0; q < nq; q++) s << " m" << u << "_" << q << " = (" << conds[q].s << ") != 0;";
+        s << " }\n";
+    }
+    s << "            if(";
+    for(int u = 0; u < 4; u++) for(int q = 0; q < nq; q++) s << (u || q ? " | " : "") << "m" << u << "_" << q;
+    s << "){\n";
+    for(int k = 0; k < nx; k++) if(late[k])
+        s << "                const V4<" << ctype(typ[k]) << "> l" << k << " = *(const V4<" << ctype(typ[k]) << ">*)((const " << ctype(typ[k]) << "*)a.c[" << k << "] + base);\n";
+    for(int u = 0; u < 4; u++){
+        s << "                if(";
+        for(int q = 0; q < nq; q++) s << (q ? " | " : "") << "m" << u << "_" << q;
+        s << "){" << bind(uc, "w", u) << bind(late, "l", u) << "\n"
+          << "                    const " << bt << " v = " << body.s << ";\n";
+        for(int q = 0; q < nq; q++) s << "                    if(m" << u << "_" << q << "){ " << update(q) << " }\n";
+        s << "                }\n";
+    }
+    s << "            }\n"
+         "        } else {\n"
+         "            for(i64 i = base; i < a.n; i++){\n";
+    std::vector<int> all(nx, 0);
+    for(int k = 0; k < nx; k++) all[k] = uc[k] | late[k];
+    s << loads_code(nx, typ, all);
+    for(int q = 0; q < nq; q++)
+        s << "                if((" << conds[q].s << ") != 0){ const " << bt << " v = " << body.s << "; " << update(q) << " }\n";
+    s << "            }\n        }\n    }\n";
+    R s.str();
+}
+
 // FA-reduce through generated code; HPA_E_NOT_IMPL when code generation is unavailable
 int32_t jit_fused_reduce(hpa_ctx *c, int kind, int nx, V *x, const hpa_expr *cond, const hpa_expr *body,
                          const DevCols &cols, L n, void *part, int G){
@@ -240,6 +302,11 @@ int32_t jit_fused_reduce(hpa_ctx *c, int kind, int nx, V *x, const hpa_expr *con
     if(cond && (e = expr_to_c(cond, nx, typ, &ce, uc, &kc))) R e;
     if((e = expr_to_c(body, nx, typ, &be, ub, &kc))) R e;
     split_used(nx, uc, ub, late);
+    {
+        std::vector<int> used(nx, 0);
+        for(int k = 0; k < nx; k++) used[k] = uc[k] | ub[k];
+        if(!cols_aligned(nx, typ, used, cols)) R HPA_E_NOT_IMPL;
+    }
     std::ostringstream s;
     s << PREAMBLE;
     s << "struct A { const void *c[" << EXPR_MAX_COLS << "]; i64 n; void *out; void *aux; " KONST_FIELDS " };\n"
@@ -248,14 +315,10 @@ int32_t jit_fused_reduce(hpa_ctx *c, int kind, int nx, V *x, const hpa_expr *con
          "    __shared__ P sh[4];\n"
          "    P p; p.s.s = 0; p.s.c = 0; p.i = 0; p.cnt = 0; p.mn = 1.7976931348623157e308; p.mx = -1.7976931348623157e308;\n"
          "    p.imn = 9223372036854775807LL; p.imx = -9223372036854775807LL - 1;\n"
-         "    for(i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < a.n; i += (i64)gridDim.x * 256){\n"
-      << loads_code(nx, typ, uc)
-      << "        if(!(" << ce.s << ")) continue;\n"
-      << loads_code(nx, typ, late);
-    if(be.f) s << "        double v = " << be.s << ";\n        p.cnt++; ks_add(p.s, v); p.mn = v < p.mn ? v : p.mn; p.mx = v > p.mx ? v : p.mx;\n";
-    else s << "        i64 v = " << be.s << ";\n        p.cnt++; p.i += (u64)v; p.imn = v < p.imn ? v : p.imn; p.imx = v > p.imx ? v : p.imx;\n";
-    s << "    }\n"
-         "    for(int o = 32; o > 0; o >>= 1){\n"
+      << fa_row_loop(nx, typ, uc, late, std::vector<CExpr>{ce}, be, [&](int){
+             return be.f ? std::string("p.cnt++; ks_add(p.s, v); p.mn = v < p.mn ? v : p.mn; p.mx = v > p.mx ? v : p.mx;")
+                         : std::string("p.cnt++; p.i += (u64)v; p.imn = v < p.imn ? v : p.imn; p.imx = v > p.imx ? v : p.imx;"); });
+    s << "    for(int o = 32; o > 0; o >>= 1){\n"
          "        KS b; b.s = __shfl_down(p.s.s, o, 64); b.c = __shfl_down(p.s.c, o, 64); ks_merge(p.s, b);\n"
          "        p.i += __shfl_down(p.i, o, 64); p.cnt += __shfl_down(p.cnt, o, 64);\n"
          "        double m1 = __shfl_down(p.mn, o, 64), m2 = __shfl_down(p.mx, o, 64);\n"
@@ -290,12 +353,15 @@ int32_t jit_fused_reduce_many(hpa_ctx *c, int nq, int nx, V *x, const hpa_expr *
                               const DevCols &cols, L n, void *part, int G){
     if(!jit_enabled()) R HPA_E_NOT_IMPL;
     int32_t typ[EXPR_MAX_COLS]; for(int k = 0; k < nx; k++) typ[k] = x[k]->typ;
-    std::vector<int> used(nx, 0);
+    std::vector<int> uc(nx, 0), ub(nx, 0), late, used(nx, 0);
     Konst kc;
     std::vector<CExpr> ce(nq);
     CExpr be; int32_t e;
-    for(int q = 0; q < nq; q++) if((e = expr_to_c(&conds[q], nx, typ, &ce[q], used, &kc))) R e;
-    if((e = expr_to_c(body, nx, typ, &be, used, &kc))) R e;
+    for(int q = 0; q < nq; q++) if((e = expr_to_c(&conds[q], nx, typ, &ce[q], uc, &kc))) R e;
+    if((e = expr_to_c(body, nx, typ, &be, ub, &kc))) R e;
+    split_used(nx, uc, ub, late);
+    for(int k = 0; k < nx; k++) used[k] = uc[k] | ub[k];
+    if(!cols_aligned(nx, typ, used, cols)) R HPA_E_NOT_IMPL;
     std::ostringstream s;
     s << PREAMBLE << "#define NQ " << nq << "\n"
       << "struct A { const void *c[" << EXPR_MAX_COLS << "]; i64 n; void *out; void *aux; " KONST_FIELDS " };\n"
@@ -303,16 +369,10 @@ int32_t jit_fused_reduce_many(hpa_ctx *c, int nq, int nx, V *x, const hpa_expr *
          "extern \"C\" __global__ void __launch_bounds__(256) fa_many(A a){\n"
          "    __shared__ KS shs[4][NQ]; __shared__ u64 shi[4][NQ], shc[4][NQ];\n";
     for(int q = 0; q < nq; q++) s << "    KS s" << q << "; s" << q << ".s = 0; s" << q << ".c = 0; u64 i" << q << " = 0, c" << q << " = 0;\n";
-    s << "    for(i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < a.n; i += (i64)gridDim.x * 256){\n"
-      << loads_code(nx, typ, used);
-    if(be.f) s << "        const double v = " << be.s << ";\n";
-    else s << "        const i64 v = " << be.s << ";\n";
-    for(int q = 0; q < nq; q++){
-        s << "        if(" << ce[q].s << "){ c" << q << "++; ";
-        if(be.f) s << "ks_add(s" << q << ", v); }\n";
-        else s << "i" << q << " += (u64)v; }\n";
-    }
-    s << "    }\n    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;\n";
+    s << fa_row_loop(nx, typ, uc, late, ce, be, [&](int q){
+        std::string qs = std::to_string(q);
+        return be.f ? "c" + qs + "++; ks_add(s" + qs + ", v);" : "c" + qs + "++; i" + qs + " += (u64)v;"; });
+    s << "    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;\n";
     for(int q = 0; q < nq; q++){
         s << "    s" << q << " = ks_wave(s" << q << ");\n"
              "    for(int o = 32; o > 0; o >>= 1){ i" << q << " += __shfl_down(i" << q << ", o, 64); c" << q << " += __shfl_down(c" << q << ", o, 64); }\n"
@@ -660,6 +720,7 @@ extern "C" int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *
     }
     std::vector<int> used(nx, 0);
     for(int k = 0; k < nx; k++) used[k] = uc[k] | ub[k];
+    if(!cols_aligned(nx, typ, used, cols)) R HPA_E_NOT_IMPL;
     const int NA = acc.empty() ? 1 : (int)acc.size();
     o.stride = NA;
     const size_t psize = 8 + 8 + 8 + 16 * NA + 8 * NA;     // struct P
@@ -668,6 +729,7 @@ extern "C" int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *
     if(G < 1) G = 1;
     const void *pk0 = nullptr, *pk1 = nullptr;
     if(n > 0){ if((e = dev_view(c, keys[0], &pk0))) R e; if(nk == 2 && (e = dev_view(c, keys[1], &pk1))) R e; }
+    if(((uintptr_t)pk0 | (uintptr_t)pk1) % 4){ c->dcur = mark; R HPA_E_NOT_IMPL; }     // 4-byte key vector loads
     void *part = dalloc(c, (L)psize * FGA_NS * G);
     unsigned *usedb = (unsigned*)dalloc(c, 4 * G);
     int *ovf = (int*)dalloc(c, 16);

@@ -49,27 +49,56 @@ template<typename T>
 __device__ __forceinline__ double as_d(T v){ return (double)v; }
 
 // kind: 0 sum, 2 min, 3 max;  F=float results accumulate in double then round (see note)
+template<typename T> struct __attribute__((aligned(4 * sizeof(T)))) V4 { T v[4]; };
+
+// Four consecutive elements per thread and iteration (one vector load), four independent
+// Neumaier accumulators (no dependency between the adds of one iteration), merged in a fixed
+// order: the result depends only on the input and the grid size.
 template<typename T>
 __global__ void __launch_bounds__(RD_BLOCK) k_sum_f(const T *x, L n, KS *part){
     __shared__ KS sh[RD_BLOCK / 64];
-    KS a = {0, 0};
-    for(L i = (L)blockIdx.x * RD_BLOCK + threadIdx.x; i < n; i += (L)gridDim.x * RD_BLOCK) ks_add(a, as_d(x[i]));
-    KS r = ks_block(a, sh);
+    KS a0 = {0, 0}, a1 = {0, 0}, a2 = {0, 0}, a3 = {0, 0};
+    const L stride = (L)gridDim.x * RD_BLOCK * 4;
+    L i = ((L)blockIdx.x * RD_BLOCK + threadIdx.x) * 4;
+    for(; i + 4 <= n; i += stride){
+        const V4<T> w = *(const V4<T>*)(x + i);
+        ks_add(a0, as_d(w.v[0])); ks_add(a1, as_d(w.v[1])); ks_add(a2, as_d(w.v[2])); ks_add(a3, as_d(w.v[3]));
+    }
+    for(L j = i; j < n && j < i + 4; j++) ks_add(a0, as_d(x[j]));
+    ks_merge(a0, a1); ks_merge(a2, a3); ks_merge(a0, a2);
+    KS r = ks_block(a0, sh);
     if(threadIdx.x == 0) part[blockIdx.x] = r;
 }
-__global__ void k_sum_f_final(const KS *part, int np, double *out){
-    if(threadIdx.x == 0 && blockIdx.x == 0){
-        KS a = {0, 0};
-        for(int q = 0; q < np; q++) ks_merge(a, part[q]);
-        *out = a.s + a.c;
-    }
+// Partials merged by 256 threads over fixed contiguous slices, then in thread order.
+__global__ void __launch_bounds__(RD_BLOCK) k_sum_f_final(const KS *part, int np, double *out){
+    __shared__ KS sh[RD_BLOCK / 64];
+    const int per = (np + RD_BLOCK - 1) / RD_BLOCK;
+    KS a = {0, 0};
+    for(int q = threadIdx.x * per; q < np && q < (int)(threadIdx.x + 1) * per; q++) ks_merge(a, part[q]);
+    KS r = ks_block(a, sh);
+    if(threadIdx.x == 0) *out = r.s + r.c;
 }
+// Exact int64 sum (wrapping, as the reference's i64 accumulator): 4 elements per thread and
+// iteration, one global atomic per workgroup.
 template<typename T>
 __global__ void __launch_bounds__(RD_BLOCK) k_sum_i(const T *x, L n, unsigned long long *acc){
+    __shared__ unsigned long long sh[RD_BLOCK / 64];
     unsigned long long a = 0;
-    for(L i = (L)blockIdx.x * RD_BLOCK + threadIdx.x; i < n; i += (L)gridDim.x * RD_BLOCK) a += (unsigned long long)(int64_t)x[i];
+    const L stride = (L)gridDim.x * RD_BLOCK * 4;
+    L i = ((L)blockIdx.x * RD_BLOCK + threadIdx.x) * 4;
+    for(; i + 4 <= n; i += stride){
+        const V4<T> w = *(const V4<T>*)(x + i);
+        a += (unsigned long long)(int64_t)w.v[0] + (unsigned long long)(int64_t)w.v[1]
+           + (unsigned long long)(int64_t)w.v[2] + (unsigned long long)(int64_t)w.v[3];
+    }
+    for(L j = i; j < n && j < i + 4; j++) a += (unsigned long long)(int64_t)x[j];
     for(int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
-    if((threadIdx.x & 63) == 0) atomicAdd(acc, a);
+    if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if(threadIdx.x == 0){
+        for(int w = 1; w < RD_BLOCK / 64; w++) a += sh[w];
+        atomicAdd(acc, a);
+    }
 }
 template<typename T, int MAXK>
 __global__ void __launch_bounds__(RD_BLOCK) k_minmax(const T *x, L n, T *part){
@@ -96,7 +125,12 @@ __global__ void k_minmax_final(const T *part, int np, T *out){
     }
 }
 
-static int sum_grid(L n){ L g = ceil_div(n, RD_BLOCK * 8); if(g > 1024) g = 1024; if(g < 1) g = 1; R (int)g; }
+static int sum_grid(L n){
+    L g = ceil_div(n, RD_BLOCK * 16), cap = (L)num_cus() * 8;
+    if(g > cap) g = cap;
+    if(g < 1) g = 1;
+    R (int)g;
+}
 
 static int32_t sum_fp(hpa_ctx *c, V x, const void *p, double *res){
     L n = x->len; int G = sum_grid(n);
@@ -104,10 +138,11 @@ static int32_t sum_fp(hpa_ctx *c, V x, const void *p, double *res){
     if(!part || !out) R HPA_E_MEMORY;
     {
         ProfScope ps("sum", c->stream, (double)n * type_width(x->typ));
+        if(((uintptr_t)p & 31) != 0){ set_error_detail("sum: payload not 32-byte aligned"); R HPA_E_DOMAIN; }
         if(x->typ == HPA_E) hipLaunchKernelGGL(k_sum_f<double>, dim3(G), dim3(RD_BLOCK), 0, c->stream, (const double*)p, n, part);
         else hipLaunchKernelGGL(k_sum_f<float>, dim3(G), dim3(RD_BLOCK), 0, c->stream, (const float*)p, n, part);
     }
-    hipLaunchKernelGGL(k_sum_f_final, dim3(1), dim3(64), 0, c->stream, part, G, out);
+    hipLaunchKernelGGL(k_sum_f_final, dim3(1), dim3(RD_BLOCK), 0, c->stream, part, G, out);
     int32_t e = check_launch("sum"); if(e) R e;
     R d2h(c, res, out, 8);
 }
@@ -117,7 +152,8 @@ static int32_t sum_int(hpa_ctx *c, V x, const void *p, int64_t *res){
     hipMemsetAsync(acc, 0, 8, c->stream);
     {
         ProfScope ps("sum", c->stream, (double)n * type_width(x->typ));
-        dim3 g(G), b(RD_BLOCK);
+        if(((uintptr_t)p & 31) != 0){ set_error_detail("sum: payload not 32-byte aligned"); R HPA_E_DOMAIN; }
+        dim3 g(reduce_grid(n, RD_BLOCK)), b(RD_BLOCK);
         switch(x->typ){
             case HPA_B: hipLaunchKernelGGL(k_sum_i<uint8_t>, g, b, 0, c->stream, (const uint8_t*)p, n, acc); break;
             case HPA_J: hipLaunchKernelGGL(k_sum_i<int8_t>, g, b, 0, c->stream, (const int8_t*)p, n, acc); break;
@@ -348,7 +384,13 @@ __global__ void k_seg_maxlen(const int64_t *off, L g, unsigned long long *mx){
         unsigned long long v = (unsigned long long)(off[s + 1] - off[s]); m = v > m ? v : m;
     }
     for(int o = 32; o > 0; o >>= 1){ unsigned long long v = __shfl_down(m, o, 64); m = v > m ? v : m; }
-    if((threadIdx.x & 63) == 0) atomicMax(mx, m);
+    __shared__ unsigned long long sh[4];
+    if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if(threadIdx.x == 0){
+        for(int w = 1; w < (int)(blockDim.x >> 6); w++) m = sh[w] > m ? sh[w] : m;
+        atomicMax(mx, m);
+    }
 }
 
 template<typename T, bool FLT>
@@ -370,7 +412,7 @@ static int32_t seg_reduce_t(hpa_ctx *c, const T *x, const int64_t *idx, const in
     }
     unsigned long long *mx = (unsigned long long*)dalloc(c, 16); if(!mx) R HPA_E_MEMORY;
     hipMemsetAsync(mx, 0, 8, c->stream);
-    hipLaunchKernelGGL(k_seg_maxlen, dim3(grid_for(g, 256, 1)), dim3(256), 0, c->stream, off, g, mx);
+    hipLaunchKernelGGL(k_seg_maxlen, dim3(reduce_grid(g, 256)), dim3(256), 0, c->stream, off, g, mx);
     unsigned long long maxlen = 0; int32_t e = d2h(c, &maxlen, mx, 8); if(e) R e;
     int nch = (int)((maxlen + SEG_CHUNK - 1) / SEG_CHUNK); if(nch < 1) nch = 1;
     KS *pf = nullptr; unsigned long long *pi = nullptr; T *pm = nullptr;
@@ -696,24 +738,24 @@ __global__ void __launch_bounds__(256) k_fused_reduce(DevExpr ce, bool has_cond,
         part[blockIdx.x] = r;
     }
 }
-__global__ void k_fused_reduce_final(const FaPart *part, int np, FaPart *out){
-    if(threadIdx.x == 0 && blockIdx.x == 0){
-        FaPart r = part[0];
-        for(int q = 1; q < np; q++){
-            const FaPart &b = part[q];
-            ks_merge(r.s, b.s); r.i += b.i; r.cnt += b.cnt;
-            r.mn = b.mn < r.mn ? b.mn : r.mn; r.mx = b.mx > r.mx ? b.mx : r.mx;
-            r.imn = b.imn < r.imn ? b.imn : r.imn; r.imx = b.imx > r.imx ? b.imx : r.imx;
-        }
-        *out = r;
-    }
+__device__ __forceinline__ void fa_merge(FaPart &r, const FaPart &b){
+    ks_merge(r.s, b.s); r.i += b.i; r.cnt += b.cnt;
+    r.mn = b.mn < r.mn ? b.mn : r.mn; r.mx = b.mx > r.mx ? b.mx : r.mx;
+    r.imn = b.imn < r.imn ? b.imn : r.imn; r.imx = b.imx > r.imx ? b.imx : r.imx;
 }
-
-__global__ void k_fused_reduce_final_many(const FaPart *part, int np, FaPart *out){
+// np partials merged by 64 lanes over fixed contiguous slices, then in lane order (one wave
+// per result: block q merges part[q * np .. q * np + np)).
+__global__ void __launch_bounds__(64) k_fused_reduce_final(const FaPart *part, int np, FaPart *out){
+    __shared__ FaPart sh[64];
     const FaPart *p = part + (L)blockIdx.x * np;
+    const int per = (np + 63) / 64;
+    FaPart a; a.s = {0, 0}; a.i = 0; a.cnt = 0; a.mn = DBL_MAX; a.mx = -DBL_MAX; a.imn = LLONG_MAX; a.imx = LLONG_MIN;
+    for(int q = threadIdx.x * per; q < np && q < (int)(threadIdx.x + 1) * per; q++) fa_merge(a, p[q]);
+    sh[threadIdx.x] = a;
+    __syncthreads();
     if(threadIdx.x == 0){
-        FaPart r = p[0];
-        for(int q = 1; q < np; q++){ ks_merge(r.s, p[q].s); r.i += p[q].i; r.cnt += p[q].cnt; }
+        FaPart r = sh[0];
+        for(int t = 1; t < 64; t++) fa_merge(r, sh[t]);
         out[blockIdx.x] = r;
     }
 }
@@ -740,7 +782,7 @@ extern "C" int32_t hpaFusedReduceMany(int32_t nq, hpa_V *z, int32_t kind, int32_
         for(int q = 0; q < nq; q++) if((e = hpaFusedReduce(z[q], kind, nx, x, &conds[q], body))) R e;
         R 0;
     }
-    hipLaunchKernelGGL(k_fused_reduce_final_many, dim3(nq), dim3(64), 0, c->stream, part, G, out);
+    hipLaunchKernelGGL(k_fused_reduce_final, dim3(nq), dim3(64), 0, c->stream, part, G, out);
     if((e = check_launch("fused_reduce_many"))) R e;
     FaPart h[8]; if((e = d2h(c, h, out, sizeof(FaPart) * nq))) R e;
     c->dcur = mark;

@@ -1,6 +1,7 @@
 // Runtime of libhorsepower_amd: device/host arenas, contexts, residency, staging, symbols,
 // kernel timing and the device-wide scan used by every order-preserving primitive.
 #include "hpa_internal.h"
+#include <deque>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -475,6 +476,21 @@ static void prof_resolve(){
     }
 }
 extern "C" void hpaProfileEnable(int32_t on){ g_prof = on != 0; }
+
+// Caller-named scopes: HIP events on the current context's stream around any sequence of calls
+// (a whole primitive, a query), reported beside the library's own launch scopes.
+struct UserScope { std::string name; ProfScope *ps; };
+static thread_local std::deque<UserScope> t_scopes;     // deque: names keep their address
+extern "C" void hpaProfileBegin(const char *name, double algo_bytes){
+    if(!g_prof) R;
+    t_scopes.push_back({name, nullptr});
+    t_scopes.back().ps = new ProfScope(t_scopes.back().name.c_str(), ctx_cur()->stream, algo_bytes);
+}
+extern "C" void hpaProfileEnd(void){
+    if(t_scopes.empty()) R;
+    delete t_scopes.back().ps;          // records the end event and queues the pair
+    t_scopes.pop_back();
+}
 extern "C" void hpaProfileReset(void){ prof_resolve(); std::lock_guard<std::mutex> lk(g_mu); g_stats.clear(); }
 extern "C" int32_t hpaProfileStats(const char *kernel, int64_t *count, double *total_ms){
     prof_resolve();

@@ -11,6 +11,7 @@
 #define CP_ITEMS 16
 #define CP_TILE  (CP_BLOCK * CP_ITEMS)
 #define CP_MAXC  8
+static_assert(CP_BLOCK == 256 && CP_ITEMS * 4 == 64, "the tile scan gives one (item, wave) count to each lane of one wave");
 
 #define ST_AGG  (1ull << 62)
 #define ST_PRE  (2ull << 62)
@@ -36,26 +37,35 @@ struct ExprPred {
     __device__ __forceinline__ bool operator()(L i) const { return expr_true(e, eval_expr(e, x, i)); }
 };
 
-// Tile-order look-back: returns the exclusive prefix of `tile` and publishes its inclusive one.
-__device__ __forceinline__ int64_t lookback(uint64_t *status, L tile, uint64_t agg){
+// Tile-order decoupled look-back, run by one whole wave: publishes the tile's aggregate, then
+// reads the statuses of 64 predecessors at once (lane l reads tile - 1 - l - 64k), sums the
+// aggregates up to the nearest published inclusive prefix and publishes its own inclusive
+// prefix.  Returns the tile's exclusive prefix in every lane.  Statuses before tile 0 read as a
+// prefix of 0.
+__device__ __forceinline__ int64_t lookback_wave(uint64_t *status, L tile, uint64_t agg){
+    const int lane = threadIdx.x & 63;
     if(tile == 0){
-        __hip_atomic_store(&status[0], ST_PRE | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if(lane == 0) __hip_atomic_store(&status[0], ST_PRE | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return 0;
     }
-    __hip_atomic_store(&status[tile], ST_AGG | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if(lane == 0) __hip_atomic_store(&status[tile], ST_AGG | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint64_t pre = 0;
     L j = tile - 1;
     while(true){
-        uint64_t s = __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        while((s >> 62) == 0){
-            __builtin_amdgcn_s_sleep(1);
-            s = __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        pre += s & ST_MASK;
-        if((s >> 62) == 2) break;
-        j--;
+        const L idx = j - lane;
+        const uint64_t s = idx >= 0 ? __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : ST_PRE;
+        const uint64_t is_pre = __ballot((s >> 62) == 2);
+        const uint64_t not_ready = __ballot((s >> 62) == 0);
+        const int p = is_pre ? __ffsll((long long)is_pre) - 1 : 63;
+        const uint64_t upto = p == 63 ? ~0ull : ((2ull << p) - 1);        // lanes 0..p
+        if(not_ready & upto){ __builtin_amdgcn_s_sleep(1); continue; }
+        uint64_t v = lane <= p ? (s & ST_MASK) : 0;
+        for(int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        pre += v;
+        if(is_pre) break;
+        j -= 64;
     }
-    __hip_atomic_store(&status[tile], ST_PRE | (pre + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if(lane == 0) __hip_atomic_store(&status[tile], ST_PRE | (pre + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return (int64_t)pre;
 }
 
@@ -88,12 +98,18 @@ __global__ void __launch_bounds__(CP_BLOCK) k_compress(Pred pred, L n, CpCols co
         if(lane == 0) s_cnt[k * 4 + w] = (uint32_t)__popcll(bal[k]);
     }
     __syncthreads();
-    if(threadIdx.x == 0){
-        uint32_t run = 0;
-        for(int q = 0; q < CP_ITEMS * 4; q++){ uint32_t t = s_cnt[q]; s_cnt[q] = run; run += t; }
-        int64_t pre = lookback(status, tile, run);
-        s_pre = pre;
-        if(tile == ntiles - 1 && total) *total = pre + run;
+    if(w == 0){
+        // exclusive scan of the 64 (item, wave) counts in row order, one per lane
+        const uint32_t mine = s_cnt[lane];
+        uint32_t inc = mine;
+        for(int o = 1; o < 64; o <<= 1){ uint32_t t = __shfl_up(inc, o, 64); if(lane >= o) inc += t; }
+        s_cnt[lane] = inc - mine;
+        const uint32_t run = __shfl(inc, 63, 64);
+        const int64_t pre = lookback_wave(status, tile, run);
+        if(lane == 0){
+            s_pre = pre;
+            if(tile == ntiles - 1 && total) *total = pre + run;
+        }
     }
     __syncthreads();
     const int64_t pre = s_pre;

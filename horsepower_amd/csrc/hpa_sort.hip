@@ -83,12 +83,20 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K *kin, const uin
     }
 }
 
+// OR and AND of all keys (which bits vary): wave and workgroup reductions, one pair of global
+// atomics per workgroup.
 template<typename K>
-__global__ void k_rs_bits(const K *keys, L n, unsigned long long *orv, unsigned long long *andv){
+__global__ void __launch_bounds__(256) k_rs_bits(const K *keys, L n, unsigned long long *orv, unsigned long long *andv){
+    __shared__ unsigned long long so[4], sa[4];
     unsigned long long o = 0, a = ~0ull;
     for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){ unsigned long long k = (unsigned long long)keys[i]; o |= k; a &= k; }
     for(int s = 32; s > 0; s >>= 1){ o |= __shfl_down(o, s, 64); a &= __shfl_down(a, s, 64); }
-    if((threadIdx.x & 63) == 0){ atomicOr(orv, o); atomicAnd(andv, a); }
+    if((threadIdx.x & 63) == 0){ so[threadIdx.x >> 6] = o; sa[threadIdx.x >> 6] = a; }
+    __syncthreads();
+    if(threadIdx.x == 0){
+        for(int w = 1; w < 4; w++){ o |= so[w]; a &= sa[w]; }
+        atomicOr(orv, o); atomicAnd(andv, a);
+    }
 }
 
 // Stable sort of (keys, vals) by key bits [lo, hi).  vals may be null (row ids 0..n-1).
@@ -102,7 +110,7 @@ static int32_t radix_sort_t(hpa_ctx *c, K *keys, uint32_t *vals, L n, int lo, in
     if(!bits) R HPA_E_MEMORY;
     unsigned long long init[2] = {0ull, ~0ull};
     hipMemcpyAsync(bits, init, 16, hipMemcpyHostToDevice, c->stream);
-    hipLaunchKernelGGL(k_rs_bits<K>, dim3(grid_for(n, 256, 8)), dim3(256), 0, c->stream, keys, n, bits, bits + 1);
+    hipLaunchKernelGGL(k_rs_bits<K>, dim3(reduce_grid(n, 256)), dim3(256), 0, c->stream, keys, n, bits, bits + 1);
     unsigned long long hb[2]; if((e = d2h(c, hb, bits, 16))) R e;
     unsigned long long vary = hb[0] ^ hb[1];
     L ntiles = ceil_div(n, RS_TILE);
@@ -151,21 +159,64 @@ int32_t radix_sort_pairs_u32(hpa_ctx *c, uint32_t *keys, uint32_t *vals, L n, in
 }
 
 // ---------------------------------------------------------------- runs -> dict(keys, flat list)
+// Runs of equal keys in a sorted sequence, in two light passes over 4096-row tiles: count the
+// run heads of each tile (a head is a row whose key differs from the previous row's), scan the
+// tile counts, then write every row's original index into the flat i64 list and, for each head,
+// the run's original first index (its key) and its start position.
+#define RUN_ITEMS 16
+#define RUN_TILE (256 * RUN_ITEMS)
 template<typename K>
-__global__ void k_run_flags(const K *sk, const uint32_t *sidx, const uint64_t *eq, L n, uint32_t *flag){
-    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
-        bool f;
-        if(i == 0) f = true;
-        else if(eq) f = eq[sidx[i]] != eq[sidx[i - 1]];
-        else f = sk[i] != sk[i - 1];
-        flag[i] = f ? 1u : 0u;
-    }
+__device__ __forceinline__ bool run_head(const K *sk, const uint32_t *sidx, const uint64_t *eq, L i){
+    if(i == 0) return true;
+    return eq ? eq[sidx[i]] != eq[sidx[i - 1]] : sk[i] != sk[i - 1];
 }
-__global__ void k_run_emit(const uint32_t *flag, const int64_t *pos, const uint32_t *sidx, L n,
-                           int64_t *keys, int64_t *start, int64_t *flat){
-    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
-        flat[i] = sidx[i];
-        if(flag[i]){ int64_t r = pos[i]; keys[r] = sidx[i]; start[r] = i; }
+template<typename K>
+__global__ void __launch_bounds__(256) k_run_tilecount(const K *sk, const uint32_t *sidx, const uint64_t *eq, L n, uint32_t *tcount){
+    const L base = (L)blockIdx.x * RUN_TILE;
+    uint32_t c = 0;
+    #pragma unroll
+    for(int k = 0; k < RUN_ITEMS; k++){
+        L i = base + (L)k * 256 + threadIdx.x;
+        c += (i < n && run_head(sk, sidx, eq, i)) ? 1u : 0u;
+    }
+    __shared__ uint32_t sh[4];
+    for(int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+    if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if(threadIdx.x == 0) tcount[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+template<typename K>
+__global__ void __launch_bounds__(256) k_run_write(const K *sk, const uint32_t *sidx, const uint64_t *eq, L n,
+                                                   const int64_t *toff, int64_t *keys, int64_t *start, int64_t *flat){
+    __shared__ uint32_t s_cnt[RUN_ITEMS * 4];
+    const L base = (L)blockIdx.x * RUN_TILE;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t bal[RUN_ITEMS];
+    #pragma unroll
+    for(int k = 0; k < RUN_ITEMS; k++){
+        L i = base + (L)k * 256 + threadIdx.x;
+        bool h = false;
+        if(i < n){ flat[i] = (int64_t)sidx[i]; h = run_head(sk, sidx, eq, i); }
+        bal[k] = __ballot(h);
+        if(lane == 0) s_cnt[k * 4 + w] = (uint32_t)__popcll(bal[k]);
+    }
+    __syncthreads();
+    if(w == 0){
+        const uint32_t mine = s_cnt[lane];
+        uint32_t inc = mine;
+        for(int o = 1; o < 64; o <<= 1){ uint32_t t = __shfl_up(inc, o, 64); if(lane >= o) inc += t; }
+        s_cnt[lane] = inc - mine;
+    }
+    __syncthreads();
+    const int64_t pre = toff[blockIdx.x];
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    #pragma unroll
+    for(int k = 0; k < RUN_ITEMS; k++){
+        if((bal[k] >> lane) & 1ull){
+            L i = base + (L)k * 256 + threadIdx.x;
+            int64_t r = pre + s_cnt[k * 4 + w] + __popcll(bal[k] & lt);
+            keys[r] = (int64_t)sidx[i]; start[r] = i;
+        }
     }
 }
 __global__ void k_run_counts(const int64_t *start, L g, L n, int64_t *cnt){
@@ -193,19 +244,23 @@ int32_t make_group_dict(hpa_ctx *c, V z, int64_t *keys, int64_t *counts, int64_t
 template<typename K>
 static int32_t runs_to_group_t(hpa_ctx *c, V z, const K *sk, const uint32_t *sidx, L n, const uint64_t *eq){
     int32_t e;
-    uint32_t *flag = (uint32_t*)dalloc(c, 4 * (n > 0 ? n : 1));
-    int64_t *pos = (int64_t*)dalloc(c, 8 * (n + 1));
-    if(!flag || !pos) R HPA_E_MEMORY;
-    if(n > 0) hipLaunchKernelGGL(k_run_flags<K>, dim3(grid_for(n, 256, 4)), dim3(256), 0, c->stream, sk, sidx, eq, n, flag);
-    if((e = scan_u32_to_i64(c, flag, pos, n, pos + n))) R e;
-    int64_t g = 0; if((e = d2h(c, &g, pos + n, 8))) R e;
+    const L nt = ceil_div(n, RUN_TILE) > 0 ? ceil_div(n, RUN_TILE) : 1;
+    uint32_t *tcount = (uint32_t*)dalloc(c, 4 * nt);
+    int64_t *toff = (int64_t*)dalloc(c, 8 * (nt + 1));
+    if(!tcount || !toff) R HPA_E_MEMORY;
+    int64_t g = 0;
+    if(n > 0){
+        hipLaunchKernelGGL(k_run_tilecount<K>, dim3((unsigned)nt), dim3(256), 0, c->stream, sk, sidx, eq, n, tcount);
+        if((e = scan_u32_to_i64(c, tcount, toff, nt, toff + nt))) R e;
+        if((e = d2h(c, &g, toff + nt, 8))) R e;
+    }
     int64_t *keys = (int64_t*)dalloc(c, 8 * (g > 0 ? g : 1));
     int64_t *start = (int64_t*)dalloc(c, 8 * (g > 0 ? g : 1));
     int64_t *cnt = (int64_t*)dalloc(c, 8 * (g > 0 ? g : 1));
     int64_t *flat = (int64_t*)dalloc(c, 8 * (n > 0 ? n : 1));
     if(!keys || !start || !cnt || !flat) R HPA_E_MEMORY;
     if(n > 0){
-        hipLaunchKernelGGL(k_run_emit, dim3(grid_for(n, 256, 4)), dim3(256), 0, c->stream, flag, pos, sidx, n, keys, start, flat);
+        hipLaunchKernelGGL(k_run_write<K>, dim3((unsigned)nt), dim3(256), 0, c->stream, sk, sidx, eq, n, toff, keys, start, flat);
         hipLaunchKernelGGL(k_run_counts, dim3(grid_for(g, 256, 1)), dim3(256), 0, c->stream, start, g, n, cnt);
     }
     if((e = check_launch("group_runs"))) R e;
@@ -383,6 +438,14 @@ __global__ void k_perm_flags_f(int nc, const void *const *cols, const int *typ, 
 }
 
 static int32_t order_perm(hpa_ctx *c, V x, const uint8_t *flags, uint32_t **perm_out, bool *ties);
+// run heads flagged per row (list keys): scatter each head's original index and position
+__global__ void k_run_emit(const uint32_t *flag, const int64_t *pos, const uint32_t *sidx, L n,
+                           int64_t *keys, int64_t *start, int64_t *flat){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
+        flat[i] = sidx[i];
+        if(flag[i]){ int64_t r = pos[i]; keys[r] = sidx[i]; start[r] = i; }
+    }
+}
 
 static int32_t runs_along_perm(hpa_ctx *c, V z, int nc, V cols, const uint32_t *perm, L n){
     int32_t e;
@@ -415,13 +478,13 @@ static int32_t runs_along_perm(hpa_ctx *c, V z, int nc, V cols, const uint32_t *
 
 // isOrdered (backend/h_system.cc:678-698) on the device: monotone (either direction, non-strict)
 template<typename T>
-__global__ void k_monotone(const T *x, L n, int *flags){
+__global__ void __launch_bounds__(256) k_monotone(const T *x, L n, int *flags){
     int up = 0, down = 0;
     for(L i = 1 + (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
         T a = x[i - 1], b = x[i]; up |= b > a; down |= b < a;
     }
-    if(__any(up) && (threadIdx.x & 63) == 0) atomicOr(&flags[0], 1);
-    if(__any(down) && (threadIdx.x & 63) == 0) atomicOr(&flags[1], 1);
+    up = __syncthreads_or(up); down = __syncthreads_or(down);
+    if(threadIdx.x == 0){ if(up) atomicOr(&flags[0], 1); if(down) atomicOr(&flags[1], 1); }
 }
 int32_t is_ordered_dev(hpa_ctx *c, V x, bool *ord){
     *ord = false;
@@ -430,8 +493,8 @@ int32_t is_ordered_dev(hpa_ctx *c, V x, bool *ord){
     const void *p; int32_t e = dev_view(c, x, &p); if(e) R e;
     int *f = (int*)dalloc(c, 16); if(!f) R HPA_E_MEMORY;
     hipMemsetAsync(f, 0, 8, c->stream);
-    if(x->typ == HPA_I) hipLaunchKernelGGL(k_monotone<int32_t>, dim3(grid_for(n, 256, 8)), dim3(256), 0, c->stream, (const int32_t*)p, n, f);
-    else hipLaunchKernelGGL(k_monotone<int64_t>, dim3(grid_for(n, 256, 8)), dim3(256), 0, c->stream, (const int64_t*)p, n, f);
+    if(x->typ == HPA_I) hipLaunchKernelGGL(k_monotone<int32_t>, dim3(reduce_grid(n, 256)), dim3(256), 0, c->stream, (const int32_t*)p, n, f);
+    else hipLaunchKernelGGL(k_monotone<int64_t>, dim3(reduce_grid(n, 256)), dim3(256), 0, c->stream, (const int64_t*)p, n, f);
     int h[2]; if((e = d2h(c, h, f, 8))) R e;
     *ord = !(h[0] && h[1]);
     R 0;

@@ -73,6 +73,8 @@ class Device(PrimitiveLib):
         _bind(dll, "hpaProfileReset", None, [])
         _bind(dll, "hpaProfileStats", i32, [C.c_char_p, C.POINTER(i64), C.POINTER(C.c_double)])
         _bind(dll, "hpaProfileList", i32, [C.c_char_p, i64])
+        _bind(dll, "hpaProfileBegin", None, [C.c_char_p, C.c_double])
+        _bind(dll, "hpaProfileEnd", None, [])
         _bind(dll, "hpaSymbol", i64, [C.c_char_p])
         _bind(dll, "hpaSymbolName", C.c_char_p, [i64])
         _bind(dll, "hpaFusedMap", i32, [V, i32, i32, C.POINTER(V), C.POINTER(Expr)])

@@ -161,6 +161,10 @@ void     hpaProfileEnable(int32_t on);
 void     hpaProfileReset(void);
 int32_t  hpaProfileStats(const char *kernel, int64_t *count, double *total_ms);
 int32_t  hpaProfileList(char *buf, int64_t cap);
+/* A named scope around any calls on the current context (e.g. one whole primitive), with
+ * its algorithmic bytes; listed by hpaProfileList with the library's own scopes. */
+void     hpaProfileBegin(const char *name, double algo_bytes);
+void     hpaProfileEnd(void);
 
 /* ---- primitives: src/horseir/v3/backend/h_primitive.h:72-211 --------------------------- */
 int32_t pfnList       (hpa_V z, int64_t n, hpa_V x[]);     /* h_primitive.cc:32-36 */
