@@ -25,7 +25,11 @@ Workload "serve": a Poisson stream of mixed Q6/Q1/Q3 requests with random consta
 600M-row lineitem and 150M-row orders table generated in HBM; the server (continuous batching,
 memory-budget admission, one stream per slot) against the same stream run one request at a time.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload q1|q6|q3|batch|prims|serve]
+Workload "copy": a plain device copy of a 60M-element f64 column (the generated FE map of one
+column), 480 MB read and 480 MB written per launch: the reference for scaling the HBM traffic
+counters (FETCH_SIZE, WRITE_SIZE) of the other kernels.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload q1|q6|q3|batch|prims|serve|copy]
 """
 import argparse
 import json
@@ -38,17 +42,33 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
 
+# HBM traffic per launch measured with the FETCH_SIZE / WRITE_SIZE counters (scaled by a copy of
+# known size); the summaries hold the commands and the raw counter values.
+TRAFFIC = {("q1", 60_000_000, "fused_group_agg"): "profiles/fga_q1_60M_traffic.md"}
+
+
+def measured_traffic(workload, rows, kernel):
+    f = TRAFFIC.get((workload, rows, kernel))
+    path = os.path.join(ROOT, f) if f else None
+    if not path or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="q1", choices=["q1", "q6", "q3", "batch", "prims", "serve"])
+    p.add_argument("--workload", default="q1", choices=["q1", "q6", "q3", "batch", "prims", "serve", "copy"])
     p.add_argument("--rows", type=int, default=0)
     p.add_argument("--plan", default="best", choices=["best", "fused", "plain"])
     p.add_argument("--cpu-rows", type=int, default=0)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-plain", action="store_true")
     p.add_argument("--profile-steps", type=int, default=3)
     p.add_argument("--requests", type=int, default=120)
     p.add_argument("--rate", type=float, default=0.0)
@@ -95,6 +115,8 @@ def main():
         return bench_prims(a)
     if a.workload == "serve":
         return bench_serve(a)
+    if a.workload == "copy":
+        return bench_copy(a)
     W = workload(a.workload, a.rows)
     t0 = time.time()
     cols = W["gen"](W["n"] if W["n"] else W["nd"])
@@ -137,7 +159,8 @@ def main():
         per_launch = k["bytes"] / k["count"]
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                "traffic": measured_traffic(a.workload, rows, dom),
                 "bytes_per_launch": per_launch, "avg_launch_us": round(avg_ms * 1e3, 2)}
     per_kernel = {k: {"launches_per_step": v["count"] / a.profile_steps,
                       "us_per_step": round(v["ms"] * 1e3 / a.profile_steps, 2),
@@ -146,7 +169,7 @@ def main():
 
     # the interpreter's per-primitive sequence on the GPU, for comparison
     plain = None
-    if a.plan != "plain":
+    if a.plan != "plain" and not a.no_plain:
         pl = W["plans"]["plain"]
         run_steps(B, pl, tables, 1, mark)
         B.sync()
@@ -439,6 +462,31 @@ def bench_serve(a):
            "latency_ms": {"p50": runs["server"]["p50_ms"], "p99": runs["server"]["p99_ms"]},
            "runs": runs, "equal_results": equal, "arena_estimates_mib": {k: v >> 20 for k, v in est.items()},
            "peak_pool_in_use_mib": srv.peak_in_use >> 20}
+    print(json.dumps(out), flush=True)
+
+
+def bench_copy(a):
+    import numpy as np
+    from horsepower_amd import datagen as D
+    from horsepower_amd.backend import GpuBackend
+    from horsepower_amd.vnode import H_E, expr
+    n = a.rows or 60_000_000
+    B = GpuBackend()
+    x = B.vec(D.cents(5, 1, n, 0, 10_000_000))
+    e = expr([("COL", 0)])
+    B.sync()
+    mark = B.mark()
+    for _ in range(a.warmup):
+        B.fused_map(H_E, [x], e)
+        B.release(mark)
+    kern = profile_kernels(B.lib, lambda: (B.fused_map(H_E, [x], e), B.release(mark)), a.steps)
+    k = kern["fused_map"]
+    avg_ms = k["ms"] / k["count"]
+    out = {"metric": "device copy (traffic-counter reference)", "value": round(16.0 * n / (avg_ms * 1e-3) / 1e9, 1),
+           "unit": "GB/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(avg_ms, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic", "config": {"workload": f"copy of {n} f64 (read {8 * n} B, write {8 * n} B per launch)",
+                                            "rows": n}}
     print(json.dumps(out), flush=True)
 
 

@@ -193,6 +193,13 @@ class GpuBackend(Backend):
         self.lib.rc("hpaCompressMany", n, (V * n)(*zs), mask, (V * n)(*ys))
         return zs
 
+    def index_many(self, xs, y):
+        """Peephole index: [index(x, y) for x in xs] in one launch."""
+        n = len(xs)
+        zs = [self.node() for _ in range(n)]
+        self.lib.rc("hpaIndexMany", n, (V * n)(*zs), (V * n)(*xs), y)
+        return zs
+
     def fused_reduce(self, kind, xs, cond, body):
         z = self.node()
         xa = (V * len(xs))(*xs)

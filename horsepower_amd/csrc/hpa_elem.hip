@@ -442,6 +442,84 @@ extern "C" int32_t pfnIndex(hpa_V z, hpa_V x, hpa_V y){
     R HPA_E_NOT_IMPL;
 }
 
+// Gather of several basic columns by one index vector in one launch: z[k] = index(x[k], y).
+#define GM_MAXC 16
+struct GatherCols { int n; int w[GM_MAXC]; const char *src[GM_MAXC]; char *dst[GM_MAXC]; L nsrc[GM_MAXC]; };
+__global__ void __launch_bounds__(256) k_gather_many(GatherCols g, const int64_t *idx, L n, int *bad){
+    int f = 0;
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
+        const int64_t k = idx[i];
+        for(int c = 0; c < g.n; c++){
+            if(k < 0 || k >= g.nsrc[c]){ f = 1; continue; }
+            switch(g.w[c]){
+                case 1: ((uint8_t*)g.dst[c])[i] = ((const uint8_t*)g.src[c])[k]; break;
+                case 2: ((uint16_t*)g.dst[c])[i] = ((const uint16_t*)g.src[c])[k]; break;
+                case 4: ((uint32_t*)g.dst[c])[i] = ((const uint32_t*)g.src[c])[k]; break;
+                default: ((uint64_t*)g.dst[c])[i] = ((const uint64_t*)g.src[c])[k]; break;
+            }
+        }
+    }
+    if(bad && __any(f) && (threadIdx.x & 63) == 0) atomicOr(bad, 1);
+}
+
+// The multi-column peephole of index (the gathers after a group or an order, as in
+// src/horseir/v1/compile-static/cur/q1.h and q3.h): each result is pfnIndex(z[k], x[k], y),
+// with one kernel and one bounds check for all columns.
+extern "C" int32_t hpaIndexMany(int32_t nx, hpa_V *z, hpa_V *x, hpa_V y){
+    hpa_ctx *c = ctx_cur();
+    if(!is_int(y->typ)) R HPA_E_DOMAIN;
+    if(nx < 1 || nx > GM_MAXC) R HPA_E_DOMAIN;
+    for(int k = 0; k < nx; k++) if(!is_basic(x[k]->typ)) R HPA_E_NOT_IMPL;
+    const L nz = y->len;
+    L mark = c->dcur;
+    const void *py; int32_t e;
+    if((e = promote_dev(c, y, HPA_L, &py))) R e;
+    GatherCols g; memset(&g, 0, sizeof(g)); g.n = nx;
+    bool known = y->typ == HPA_L;
+    for(int k = 0; k < nx; k++){
+        const void *px = nullptr;
+        if(x[k]->len > 0 && (e = dev_view(c, x[k], &px))){ c->dcur = mark; R e; }
+        if(nz > 0 && x[k]->len == 0){ c->dcur = mark; R HPA_E_INDEX; }
+        g.src[k] = (const char*)px; g.w[k] = (int)type_width(x[k]->typ); g.nsrc[k] = x[k]->len;
+        known = known && within_bound(c, py, x[k]->len);
+    }
+    for(int k = 0; k < nx; k++){
+        g.dst[k] = (char*)dalloc(c, type_bytes(x[k]->typ, nz > 0 ? nz : 1));
+        if(!g.dst[k]){ c->dcur = mark; R HPA_E_MEMORY; }
+    }
+    int *bad = nullptr;
+    if(!known){ bad = (int*)dalloc(c, 16); if(!bad){ c->dcur = mark; R HPA_E_MEMORY; } hipMemsetAsync(bad, 0, 4, c->stream); }
+    if(nz > 0){
+        double by = 8.0 * nz; for(int k = 0; k < nx; k++) by += 2.0 * g.w[k] * nz;
+        ProfScope ps("gather_many", c->stream, by);
+        hipLaunchKernelGGL(k_gather_many, dim3(grid_for(nz, 256, 4)), dim3(256), 0, c->stream, g, (const int64_t*)py, nz, bad);
+        if((e = check_launch("gather_many"))) R e;
+    }
+    if(!known){
+        int h = 0; if((e = d2h(c, &h, bad, 4))) R e;
+        if(h){ c->dcur = mark; R HPA_E_INDEX; }
+    }
+    // one read back for every inline (single-row) result
+    if(nz == 1){
+        uint64_t tmp[GM_MAXC];
+        for(int k = 0; k < nx; k++)
+            if(hipMemcpyAsync(&tmp[k], g.dst[k], (size_t)g.w[k], hipMemcpyDeviceToHost, c->stream) != hipSuccess) R HPA_E_DEVICE;
+        if(hipStreamSynchronize(c->stream) != hipSuccess) R HPA_E_DEVICE;
+        for(int k = 0; k < nx; k++){
+            V zk = z[k]; zk->typ = x[k]->typ; zk->len = 1; zk->g = (char*)&zk->b; zk->i64 = 0;
+            memcpy(&zk->b, &tmp[k], (size_t)g.w[k]);
+        }
+        c->dcur = mark;
+        R 0;
+    }
+    for(int k = 0; k < nx; k++){
+        V zk = z[k]; zk->typ = x[k]->typ; zk->len = nz;
+        if(nz == 0){ zk->g = (char*)&zk->b; zk->i64 = 0; }
+        else zk->g = g.dst[k];
+    }
+    R 0;
+}
+
 extern "C" int32_t pfnLen(hpa_V z, hpa_V x){       // h_primitive.cc:656-662
     z->typ = HPA_L; z->len = 1; z->g = (char*)&z->b;
     z->i64 = x->len;

@@ -1,9 +1,11 @@
 // Order-preserving stream compaction: compress, where, the multi-column peephole compress and
-// the FA fused compress scan.  One pass: each 4096-row tile counts its selected rows with
-// wavefront ballots, publishes its count, looks back over earlier tiles for its output offset
-// (decoupled look-back), then scatters the selected rows of every column.  Rows keep their
-// input order, as the reference's DOT count + ordered scatter does
-// (h_primitive.cc:1479-1521, backend/h_system.cc:304-310, optimizer/autofusion.cc:886-899).
+// the FA fused compress scan.  Two passes over 4096-row tiles, as the reference's DOT count +
+// ordered scatter (h_primitive.cc:1479-1521, backend/h_system.cc:304-310,
+// optimizer/autofusion.cc:886-899): count the selected rows of each tile (a byte mask is read
+// 16 rows per thread with one vector load), scan the tile counts, then each tile ranks its
+// selected rows with wave ballots and scatters every column at its offset.  The total is read
+// between the passes, so outputs are allocated at their exact size, and an all-true mask
+// aliases its input without a second pass.
 #include "hpa_internal.h"
 #include "hpa_expr.h"
 
@@ -13,20 +15,8 @@
 #define CP_MAXC  8
 static_assert(CP_BLOCK == 256 && CP_ITEMS * 4 == 64, "the tile scan gives one (item, wave) count to each lane of one wave");
 
-#define ST_AGG  (1ull << 62)
-#define ST_PRE  (2ull << 62)
-#define ST_MASK ((1ull << 62) - 1)
 
 struct CpCols { int n; int w[CP_MAXC]; const char *src[CP_MAXC]; char *dst[CP_MAXC]; };
-
-__global__ void k_fill_u64(uint64_t *p, L n, uint64_t v){
-    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x) p[i] = v;
-}
-int32_t fill_u64(hpa_ctx *c, uint64_t *p, L n, uint64_t v){
-    if(n <= 0) R 0;
-    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256, 4)), dim3(256), 0, c->stream, p, n, v);
-    R check_launch("fill");
-}
 
 struct MaskPred {
     const uint8_t *m;
@@ -37,38 +27,6 @@ struct ExprPred {
     __device__ __forceinline__ bool operator()(L i) const { return expr_true(e, eval_expr(e, x, i)); }
 };
 
-// Tile-order decoupled look-back, run by one whole wave: publishes the tile's aggregate, then
-// reads the statuses of 64 predecessors at once (lane l reads tile - 1 - l - 64k), sums the
-// aggregates up to the nearest published inclusive prefix and publishes its own inclusive
-// prefix.  Returns the tile's exclusive prefix in every lane.  Statuses before tile 0 read as a
-// prefix of 0.
-__device__ __forceinline__ int64_t lookback_wave(uint64_t *status, L tile, uint64_t agg){
-    const int lane = threadIdx.x & 63;
-    if(tile == 0){
-        if(lane == 0) __hip_atomic_store(&status[0], ST_PRE | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return 0;
-    }
-    if(lane == 0) __hip_atomic_store(&status[tile], ST_AGG | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint64_t pre = 0;
-    L j = tile - 1;
-    while(true){
-        const L idx = j - lane;
-        const uint64_t s = idx >= 0 ? __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : ST_PRE;
-        const uint64_t is_pre = __ballot((s >> 62) == 2);
-        const uint64_t not_ready = __ballot((s >> 62) == 0);
-        const int p = is_pre ? __ffsll((long long)is_pre) - 1 : 63;
-        const uint64_t upto = p == 63 ? ~0ull : ((2ull << p) - 1);        // lanes 0..p
-        if(not_ready & upto){ __builtin_amdgcn_s_sleep(1); continue; }
-        uint64_t v = lane <= p ? (s & ST_MASK) : 0;
-        for(int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-        pre += v;
-        if(is_pre) break;
-        j -= 64;
-    }
-    if(lane == 0) __hip_atomic_store(&status[tile], ST_PRE | (pre + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return (int64_t)pre;
-}
-
 __device__ __forceinline__ void copy_elem(char *dst, const char *src, int w, int64_t pos, L i){
     switch(w){
         case 1: ((uint8_t*)dst)[pos] = ((const uint8_t*)src)[i]; break;
@@ -78,16 +36,48 @@ __device__ __forceinline__ void copy_elem(char *dst, const char *src, int w, int
     }
 }
 
+// Pass 1: selected rows per 4096-row tile.
 template<typename Pred>
-__global__ void __launch_bounds__(CP_BLOCK) k_compress(Pred pred, L n, CpCols cols, int64_t *idx_out,
-                                                       uint64_t *status, unsigned *tile_ctr, L ntiles, int64_t *total){
-    __shared__ unsigned s_tile;
-    __shared__ uint32_t s_cnt[CP_ITEMS * 4];
-    __shared__ int64_t s_pre;
-    if(threadIdx.x == 0) s_tile = atomicAdd(tile_ctr, 1u);
+__global__ void __launch_bounds__(CP_BLOCK) k_cp_count(Pred pred, L n, uint32_t *tcount){
+    __shared__ uint32_t sh[CP_BLOCK / 64];
+    const L base = (L)blockIdx.x * CP_TILE;
+    uint32_t c = 0;
+    #pragma unroll
+    for(int k = 0; k < CP_ITEMS; k++){
+        L i = base + (L)k * CP_BLOCK + threadIdx.x;
+        c += (i < n && pred(i)) ? 1u : 0u;
+    }
+    for(int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+    if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
     __syncthreads();
-    const L tile = s_tile;
-    const L base = tile * CP_TILE;
+    if(threadIdx.x == 0) tcount[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+// the same for a byte mask: 16 consecutive mask bytes per thread, one 16-byte load
+__global__ void __launch_bounds__(CP_BLOCK) k_cp_count_mask(const uint8_t *m, L n, uint32_t *tcount){
+    __shared__ uint32_t sh[CP_BLOCK / 64];
+    const L i0 = (L)blockIdx.x * CP_TILE + (L)threadIdx.x * 16;
+    uint32_t c = 0;
+    if(i0 + 16 <= n){
+        const uint4 v = *(const uint4*)(m + i0);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+        #pragma unroll
+        for(int q = 0; q < 4; q++)
+            c += ((w[q] & 0xFFu) != 0) + ((w[q] & 0xFF00u) != 0) + ((w[q] & 0xFF0000u) != 0) + ((w[q] & 0xFF000000u) != 0);
+    } else {
+        for(L i = i0; i < n && i < i0 + 16; i++) c += m[i] != 0;
+    }
+    for(int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+    if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if(threadIdx.x == 0) tcount[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// Pass 2: each tile takes its output offset from the scanned tile counts, ranks its selected
+// rows with wave ballots (row order) and scatters every column (and/or the row index).
+template<typename Pred>
+__global__ void __launch_bounds__(CP_BLOCK) k_cp_write(Pred pred, L n, CpCols cols, int64_t *idx_out, const int64_t *toff){
+    __shared__ uint32_t s_cnt[CP_ITEMS * 4];
+    const L base = (L)blockIdx.x * CP_TILE;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint64_t bal[CP_ITEMS];
     #pragma unroll
@@ -99,20 +89,13 @@ __global__ void __launch_bounds__(CP_BLOCK) k_compress(Pred pred, L n, CpCols co
     }
     __syncthreads();
     if(w == 0){
-        // exclusive scan of the 64 (item, wave) counts in row order, one per lane
         const uint32_t mine = s_cnt[lane];
         uint32_t inc = mine;
         for(int o = 1; o < 64; o <<= 1){ uint32_t t = __shfl_up(inc, o, 64); if(lane >= o) inc += t; }
         s_cnt[lane] = inc - mine;
-        const uint32_t run = __shfl(inc, 63, 64);
-        const int64_t pre = lookback_wave(status, tile, run);
-        if(lane == 0){
-            s_pre = pre;
-            if(tile == ntiles - 1 && total) *total = pre + run;
-        }
     }
     __syncthreads();
-    const int64_t pre = s_pre;
+    const int64_t pre = toff[blockIdx.x];
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     #pragma unroll
     for(int k = 0; k < CP_ITEMS; k++){
@@ -125,25 +108,31 @@ __global__ void __launch_bounds__(CP_BLOCK) k_compress(Pred pred, L n, CpCols co
     }
 }
 
-// per_row / per_sel: algorithmic bytes read per input row and written per selected row
+// Counting pass: tile offsets on the device and the total on the host.
 template<typename Pred>
-static int32_t run_compress(hpa_ctx *c, const char *name, Pred pred, L n, CpCols cols, int64_t *idx_out, int64_t *k_host,
-                            double per_row, double per_sel){
+static int32_t compress_count(hpa_ctx *c, Pred pred, const uint8_t *mask, L n, int64_t **toff_out, int64_t *k_host){
     L ntiles = ceil_div(n, CP_TILE);
-    if(ntiles < 1){ *k_host = 0; R 0; }
-    uint64_t *status = (uint64_t*)dalloc(c, 8 * (ntiles + 2));
-    if(!status) R HPA_E_MEMORY;
-    unsigned *ctr = (unsigned*)(status + ntiles);
-    int64_t *total = (int64_t*)(status + ntiles + 1);
-    int32_t e = fill_u64(c, status, ntiles + 2, 0); if(e) R e;
-    ProfScope ps(name, c->stream);
-    hipLaunchKernelGGL(k_compress<Pred>, dim3((unsigned)ntiles), dim3(CP_BLOCK), 0, c->stream,
-                       pred, n, cols, idx_out, status, ctr, ntiles, total);
-    ps.stop();
-    if((e = check_launch(name))) R e;
-    e = d2h(c, k_host, total, 8);
-    ps.bytes = per_row * (double)n + per_sel * (double)*k_host;
-    R e;
+    *k_host = 0;
+    if(ntiles < 1) R 0;
+    uint32_t *tcount = (uint32_t*)dalloc(c, 4 * ntiles);
+    int64_t *toff = (int64_t*)dalloc(c, 8 * (ntiles + 1));
+    if(!tcount || !toff) R HPA_E_MEMORY;
+    if(mask && ((uintptr_t)mask & 15) == 0)
+        hipLaunchKernelGGL(k_cp_count_mask, dim3((unsigned)ntiles), dim3(CP_BLOCK), 0, c->stream, mask, n, tcount);
+    else
+        hipLaunchKernelGGL(k_cp_count<Pred>, dim3((unsigned)ntiles), dim3(CP_BLOCK), 0, c->stream, pred, n, tcount);
+    int32_t e;
+    if((e = check_launch("compress_count"))) R e;
+    if((e = scan_u32_to_i64(c, tcount, toff, ntiles, toff + ntiles))) R e;
+    *toff_out = toff;
+    R d2h(c, k_host, toff + ntiles, 8);
+}
+template<typename Pred>
+static int32_t compress_write(hpa_ctx *c, Pred pred, L n, CpCols cols, int64_t *idx_out, const int64_t *toff){
+    L ntiles = ceil_div(n, CP_TILE);
+    if(ntiles < 1) R 0;
+    hipLaunchKernelGGL(k_cp_write<Pred>, dim3((unsigned)ntiles), dim3(CP_BLOCK), 0, c->stream, pred, n, cols, idx_out, toff);
+    R check_launch("compress_write");
 }
 
 // pfnCompress (h_primitive.cc:1479-1521)
@@ -168,22 +157,30 @@ static int32_t compress_many(hpa_ctx *c, int ny, V *z, V x, V *y, bool expr_mode
     }
     const uint8_t *m = nullptr;
     if(!expr_mode && n > 0){ const void *p; if((e = dev_view(c, x, &p))) R e; m = (const uint8_t*)p; }
-    // worst-case outputs; the unused tail of each stays in the arena until the next reset
-    L outmark = c->dcur;
+    double wsum = 0; for(int k = 0; k < ny; k++) wsum += cols.w[k];
+    double wcond = expr_mode ? 0 : 1.0;
+    if(expr_mode) for(int k = 0; k < ep->x.n; k++) wcond += (double)type_width(ep->x.typ[k]);
+    ProfScope ps(expr_mode ? "fused_compress" : "compress", c->stream);
+    int64_t kk = 0, *toff = nullptr;
+    e = expr_mode ? compress_count(c, *ep, nullptr, n, &toff, &kk) : compress_count(c, MaskPred{m}, m, n, &toff, &kk);
+    if(e){ c->dcur = mark; R e; }
+    if(kk == n && n > 0 && !expr_mode){                    // copyV alias (h_primitive.cc:1513-1515)
+        ps.stop(); ps.bytes = (double)n;
+        c->dcur = mark;
+        for(int k = 0; k < ny; k++) *z[k] = *y[k];
+        R 0;
+    }
+    // outputs sized exactly: the counting pass already knows how many rows are selected
     for(int k = 0; k < ny; k++){
-        cols.dst[k] = (char*)dalloc(c, type_bytes(y[k]->typ, n > 0 ? n : 1));
+        cols.dst[k] = (char*)dalloc(c, type_bytes(y[k]->typ, kk > 0 ? kk : 1));
         if(!cols.dst[k]){ c->dcur = mark; R HPA_E_MEMORY; }
     }
-    int64_t kk = 0;
-    double wsum = 0; for(int k = 0; k < ny; k++) wsum += cols.w[k];
-    double wcond = 0;
-    if(expr_mode) for(int k = 0; k < ep->x.n; k++) wcond += (double)type_width(ep->x.typ[k]);
-    if(expr_mode) e = run_compress(c, "fused_compress", *ep, n, cols, nullptr, &kk, wcond + wsum, wsum);
-    else e = run_compress(c, "compress", MaskPred{m}, n, cols, nullptr, &kk, 1.0 + wsum, wsum);
+    e = expr_mode ? compress_write(c, *ep, n, cols, nullptr, toff) : compress_write(c, MaskPred{m}, n, cols, nullptr, toff);
+    ps.stop();
+    ps.bytes = wcond * (double)n + wsum * (double)kk;
     if(e) R e;
     for(int k = 0; k < ny; k++){
         V zk = z[k];
-        if(kk == n && n > 0 && !expr_mode){ *zk = *y[k]; continue; }   // copyV alias (h_primitive.cc:1513-1515)
         zk->typ = y[k]->typ; zk->len = kk;
         if(kk <= 1){
             zk->g = (char*)&zk->b; zk->i64 = 0;
@@ -193,8 +190,6 @@ static int32_t compress_many(hpa_ctx *c, int ny, V *z, V x, V *y, bool expr_mode
             if(zk->typ == HPA_Y) zk->p = y[k]->p;            // enum keeps its target (vy(z)=vy(y))
         }
     }
-    if(kk <= 1 || (kk == n && !expr_mode)) c->dcur = outmark;  // nothing of the scratch outputs is referenced
-    else if(ny == 1) c->dcur = outmark + ((type_bytes(y[0]->typ, kk) + 255) & ~(L)255);
     R 0;
 }
 
@@ -220,10 +215,14 @@ extern "C" int32_t pfnWhere(hpa_V z, hpa_V x){
     L mark = c->dcur;
     const void *p = nullptr; int32_t e;
     if(n > 0 && (e = dev_view(c, x, &p))) R e;
-    int64_t *out = (int64_t*)dalloc(c, 8 * (n > 0 ? n : 1)); if(!out) R HPA_E_MEMORY;
+    const uint8_t *m = (const uint8_t*)p;
+    ProfScope ps("where", c->stream);
+    int64_t kk = 0, *toff = nullptr;
+    if((e = compress_count(c, MaskPred{m}, m, n, &toff, &kk))){ c->dcur = mark; R e; }
+    int64_t *out = (int64_t*)dalloc(c, 8 * (kk > 0 ? kk : 1)); if(!out){ c->dcur = mark; R HPA_E_MEMORY; }
     CpCols cols; cols.n = 0;
-    int64_t kk = 0;
-    if((e = run_compress(c, "where", MaskPred{(const uint8_t*)p}, n, cols, out, &kk, 1.0, 8.0))) R e;
+    if((e = compress_write(c, MaskPred{m}, n, cols, out, toff))) R e;
+    ps.stop(); ps.bytes = (double)n + 8.0 * (double)kk;
     z->typ = HPA_L; z->len = kk;
     note_bound(c, out, n);
     if(kk <= 1){

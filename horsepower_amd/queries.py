@@ -124,8 +124,7 @@ def q1_fused(B, T, ship_max=D.Q1_SHIP_MAX):
             B.group_reduce(1, qty, groups), B.group_reduce(1, price, groups),
             B.group_reduce(1, disc, groups), B.group_reduce(4, flag, groups)]
     perm = B.pfnOrderBy(B.list([kf, ks]), B.bools([1, 1]))
-    out = [B.pfnIndex(v, perm) for v in [kf, ks] + aggs]
-    return dict(zip(Q1_NAMES, B.read_many(out)))
+    return dict(zip(Q1_NAMES, B.read_many(B.index_many([kf, ks] + aggs, perm))))
 
 
 def q1_chf_spec(ship_max=D.Q1_SHIP_MAX):
@@ -154,8 +153,7 @@ def q1_chf(B, T, ship_max=D.Q1_SHIP_MAX):
         return q1_fused(B, T, ship_max)
     _, (kf, ks), aggs = r
     perm = B.pfnOrderBy(B.list([kf, ks]), B.bools([1, 1]))
-    out = [B.pfnIndex(v, perm) for v in [kf, ks] + aggs]
-    return dict(zip(Q1_NAMES, B.read_many(out)))
+    return dict(zip(Q1_NAMES, B.read_many(B.index_many([kf, ks] + aggs, perm))))
 
 
 # ------------------------------------------------------------------------------------- Q3

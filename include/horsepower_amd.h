@@ -260,6 +260,9 @@ int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hp
  * types and expressions and compile it with hipRTC; 0 when the generated code compiles. */
 int32_t hpaJitCompileCheck(int32_t nx, const int32_t *coltyp, const hpa_expr *cond,
                            int32_t na, const hpa_agg *aggs);
+/* Peephole index: z[k] = index(x[k], y) for nx <= 16 basic columns, one launch and one bounds
+ * check (the gathers of every aggregate by a group or order permutation, q1.hir:57-66). */
+int32_t hpaIndexMany(int32_t nx, hpa_V *z, hpa_V *x, hpa_V y);
 /* Peephole compress with a ready mask: z[k] = compress(mask, y[k]). */
 int32_t hpaCompressMany(int32_t ny, hpa_V *z, hpa_V mask, hpa_V *y);
 /* FP1/FP2/FP5/FP6/FP7 grouped reductions (optimizer/pattern.cc:180-219): for each group list

@@ -127,6 +127,24 @@ def test_index_gather(gpu, orc, n):
     assert ei.value.code == 3
 
 
+@pytest.mark.parametrize("n", [0, 1, 3, 5000])
+def test_index_many(gpu, n):
+    """The multi-column gather gives what pfnIndex gives column by column."""
+    r = rng(n + 17)
+    m = max(n, 1) + 3
+    cols = [r.uniform(0, 1, m), r.integers(-9, 9, m).astype(np.int32),
+            r.integers(0, 255, m).astype(np.uint8), r.integers(-2**40, 2**40, m).astype(np.int64)]
+    typs = [H_E, H_I, H_C, H_L]
+    xs = [gpu.vec(c, t) for c, t in zip(cols, typs)]
+    idx = gpu.vec(r.integers(0, m, n).astype(np.int64))
+    many = [gpu.read(z) for z in gpu.index_many(xs, idx)]
+    for a, x in zip(many, xs):
+        assert np.array_equal(a, gpu.read(gpu.pfnIndex(x, idx)))
+    with pytest.raises(HorseError) as ei:
+        gpu.index_many(xs, gpu.vec(np.array([0, m], np.int64)))
+    assert ei.value.code == 3
+
+
 @pytest.mark.parametrize("n", [0, 1, 5, 4096, 100003, 3_000_001])
 def test_sum_avg_minmax(gpu, orc, n):
     r = rng(n + 6)
