@@ -88,7 +88,7 @@ def workload(name, rows):
         return dict(n=n, gen=lambda k: (D.gen_q6(k),), plans=dict(best=Q.q6_fused, fused=Q.q6_fused, plain=Q.q6_plain),
                     desc=f"q6 filter-compress-aggregate, {n} rows", cpu_default=6_000_000)
     nd = (rows or 60_000_000) // 4
-    return dict(n=None, nd=nd, gen=lambda k: D.gen_q3(k), plans=dict(best=Q.q3_plain, fused=Q.q3_plain, plain=Q.q3_plain),
+    return dict(n=None, nd=nd, gen=lambda k: D.gen_q3(k), plans=dict(best=Q.q3_fused, fused=Q.q3_fused, plain=Q.q3_plain),
                 desc=f"q3 join+group+top-10, {nd} orders", cpu_default=1_500_000)
 
 

@@ -273,14 +273,25 @@ class Batch:
         self.lib.rc("hpaBatchRun", self.h, 1 if graph else 0)
 
     def results(self):
-        """[{key, count, sum}] per query, read back with one synchronisation."""
-        zs = []
-        for q in range(self.nq):
-            z = [self.B.node() for _ in range(3)]
-            self.lib.rc("hpaBatchResult", self.h, q, *z)
-            zs += z
-        arrs = self.B.read_many(zs)
-        return [dict(zip(("key", "count", "sum"), arrs[3 * q:3 * q + 3])) for q in range(self.nq)]
+        """[{key, count, sum}] per query: numpy views of the plan's pinned staging, filled with
+        one synchronisation (valid until the next call)."""
+        nq = self.nq
+        ng = (C.c_int64 * nq)()
+        k, c, s = (C.c_void_p * nq)(), (C.c_void_p * nq)(), (C.c_void_p * nq)()
+        self.lib.rc("hpaBatchFetch", self.h, ng, k, c, s)
+
+        def view(p, n, ct, dt):
+            if n == 0:
+                return np.zeros(0, dt)
+            return np.ctypeslib.as_array(C.cast(C.c_void_p(p), C.POINTER(ct)), shape=(n,))
+        return [dict(key=view(k[q], ng[q], C.c_int32, np.int32), count=view(c[q], ng[q], C.c_int64, np.int64),
+                     sum=view(s[q], ng[q], C.c_double, np.float64)) for q in range(nq)]
+
+    def result_nodes(self, q):
+        """Query q's results as value nodes (device payloads) for further primitives."""
+        z = [self.B.node() for _ in range(3)]
+        self.lib.rc("hpaBatchResult", self.h, q, *z)
+        return z
 
     def config(self):
         v = [C.c_int32() for _ in range(3)]

@@ -260,6 +260,8 @@ struct hpa_batch {
     hipGraph_t graph = nullptr; hipGraphExec_t exec = nullptr; bool captured = false;
     std::vector<int64_t> ng; bool ng_valid = false;
     double scan_bytes = 0;         // algorithmic bytes of one run: the three input columns, once
+    char *pinned = nullptr;        // host staging of every query's results (hpaBatchFetch)
+    std::vector<int64_t> pin_off;  // per query: offset of its (keys, counts, sums) block
 };
 
 static BPlan plan_of(hpa_batch *b){
@@ -303,6 +305,7 @@ static int32_t batch_enqueue(hpa_batch *b, hipStream_t s){
 extern "C" void hpaBatchDestroy(hpa_batch *b){
     if(!b) R;
     drop_graph(b);
+    if(b->pinned) hipHostFree(b->pinned);
     delete b;
 }
 
@@ -364,6 +367,41 @@ extern "C" hpa_batch *hpaBatchCreate(int32_t nq, hpa_V *keys, hpa_V *vals, hpa_V
     b->ng.assign(2 * (size_t)nq, 0);
     if(batch_layout(b, 256, 8192, 2048)){ delete b; R nullptr; }
     R b;
+}
+
+// All queries' results of the last run, copied into pinned host memory owned by the plan with
+// one read of the group counts, one copy per output vector and one synchronisation.  For query
+// q: ng[q] groups; keys[q] (i32), cnts[q] (i64) and sums[q] (f64) point into the staging, valid
+// until the next fetch or the plan's destruction.
+extern "C" int32_t hpaBatchFetch(hpa_batch *b, int64_t *ng, void **keys, void **cnts, void **sums){
+    hpa_ctx *c = b->c;
+    int32_t e;
+    if(!b->pinned){
+        int64_t tot = 0;
+        b->pin_off.resize(b->nq);
+        for(int qi = 0; qi < b->nq; qi++){ b->pin_off[qi] = tot; tot += 24 * (int64_t)b->hq[qi].range; }
+        if(hipHostMalloc((void**)&b->pinned, (size_t)(tot > 0 ? tot : 8), hipHostMallocDefault) != hipSuccess){
+            b->pinned = nullptr; set_error_detail("batch: pinned staging of %lld bytes", (long long)tot); R HPA_E_MEMORY;
+        }
+    }
+    if(!b->ng_valid){
+        if((e = d2h(c, b->ng.data(), b->ngdev, 16 * (L)b->nq))) R e;
+        b->ng_valid = true;
+    }
+    for(int qi = 0; qi < b->nq; qi++){
+        if(b->ng[2 * qi + 1]){ set_error_detail("batch query %d: key outside the planned range", qi); R HPA_E_DOMAIN; }
+        const BQuery &q = b->hq[qi];
+        const int64_t g = b->ng[2 * qi];
+        char *base = b->pinned + b->pin_off[qi];
+        char *hk = base, *hc = base + 8 * (int64_t)q.range, *hs = base + 16 * (int64_t)q.range;
+        ng[qi] = g; keys[qi] = hk; cnts[qi] = hc; sums[qi] = hs;
+        if(g > 0){
+            if(hipMemcpyAsync(hk, q.okey, 4 * g, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+               hipMemcpyAsync(hc, q.ocnt, 8 * g, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+               hipMemcpyAsync(hs, q.osum, 8 * g, hipMemcpyDeviceToHost, c->stream) != hipSuccess) R HPA_E_DEVICE;
+        }
+    }
+    R hipStreamSynchronize(c->stream) == hipSuccess ? 0 : HPA_E_DEVICE;
 }
 
 extern "C" int32_t hpaBatchSetConfig(hpa_batch *b, int32_t block, int32_t tile_rows, int32_t lds_keys){

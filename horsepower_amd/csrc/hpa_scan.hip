@@ -293,5 +293,17 @@ extern "C" int32_t hpaFusedCompress(int32_t ny, hpa_V *z, hpa_V *y, int32_t nx, 
     if((e = cols_view(c, nx, x, &ep.x, &n))) R e;
     int32_t ct[EXPR_MAX_COLS]; for(int k = 0; k < nx; k++) ct[k] = x[k]->typ;
     if((e = expr_compile(cond, nx, ct, &ep.e))) R e;
+    if(n > 1){
+        // the condition through generated code into a byte mask, then the mask compress
+        L mark = c->dcur;
+        uint8_t *mask = (uint8_t*)dalloc(c, n);
+        if(!mask) R HPA_E_MEMORY;
+        if(jit_fused_map(c, HPA_B, nx, x, cond, ep.x, n, mask) == 0){
+            V0 m; memset(&m, 0, sizeof(m));
+            m.typ = HPA_B; m.len = n; m.g = (char*)mask;
+            R compress_many(c, ny, z, &m, y, false, nullptr, 0);
+        }
+        c->dcur = mark;
+    }
     R compress_many(c, ny, z, nullptr, y, true, &ep, n);
 }

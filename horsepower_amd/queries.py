@@ -186,6 +186,31 @@ def q3_plain(B, Dm, F, day=D.Q3_DAY, limit=10, segment=1):
     return res
 
 
+def q3_fused(B, Dm, F, day=D.Q3_DAY, limit=10, segment=1):
+    """Q3 as the optimising compiler emits it (src/horseir/v1/compile-static/cur/q3.h:1-56):
+    one mask per table and the multi-column peephole compress of its filtered columns, the FE
+    revenue map, group, the grouped sum, join and the multi-column gathers."""
+    d_mask = B.pfnAnd(B.pfnEq(Dm["segment"], B.scalar(segment, H_I)), B.pfnLt(Dm["order_day"], B.scalar(day, H_I)))
+    lk, lday = B.compress_many(d_mask, [Dm["key"], Dm["order_day"]])
+    f_mask = B.pfnGt(F["ship_day"], B.scalar(day, H_I))
+    rk, rp, rd = B.compress_many(f_mask, [F["fk"], F["price"], F["discount"]])
+    rev = B.fused_map(H_E, [rd, rp], expr([("I", 1), ("COL", 0), "SUB", ("COL", 1), "MUL"]))
+    grp = B.pfnGroup(rk)
+    gkeys, gvals = B.pfnKeys(grp), B.pfnValues(grp)
+    gk = B.pfnIndex(rk, gkeys)
+    gsum = B.pfnRaze(B.each(B.each_right(rev, gvals, "pfnIndex"), "pfnSum"))
+    j = B.pfnJoinIndex(lk, gk, B.sym("eq"))
+    j0, j1 = B.pfnIndex(j, B.scalar(0, H_L)), B.pfnIndex(j, B.scalar(1, H_L))
+    revenue = B.pfnIndex(gsum, j1)
+    okey, oday = B.index_many([lk, lday], j0)
+    perm = B.pfnOrderBy(B.list([revenue, oday]), B.bools([0, 1]))
+    rows = int(perm.contents.len)
+    head = B.pfnIndex(perm, B.host.vec(np.arange(min(limit, rows), dtype=np.int64), H_L))
+    res = dict(zip(Q3_NAMES, B.read_many(B.index_many([okey, revenue, oday], head))))
+    res["rows"] = rows
+    return res
+
+
 # ---------------------------------------------------------------------- batched group-by
 BATCH_NAMES = ("key", "count", "sum")
 

@@ -82,7 +82,7 @@ def make_requests(n, rate=None, seed=2024, mix=(0.45, 0.35, 0.20)):
 PLANS = {
     "fused": {"q6": lambda B, T, p: Q.q6_fused(B, T["fact"], **p),
               "q1": lambda B, T, p: Q.q1_chf(B, T["fact"], **p),
-              "q3": lambda B, T, p: Q.q3_plain(B, T["dim"], T["fact"], **p)},
+              "q3": lambda B, T, p: Q.q3_fused(B, T["dim"], T["fact"], **p)},
     "plain": {"q6": lambda B, T, p: Q.q6_plain(B, T["fact"], **p),
               "q1": lambda B, T, p: Q.q1_plain(B, T["fact"], **p),
               "q3": lambda B, T, p: Q.q3_plain(B, T["dim"], T["fact"], **p)},

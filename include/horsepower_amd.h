@@ -293,6 +293,9 @@ void     hpaBatchDestroy(hpa_batch *b);
 int32_t  hpaBatchRun(hpa_batch *b, int32_t use_graph);
 /* Query qi of the last run: zkey (I), zcnt (L), zsum (E), one element per group. */
 int32_t  hpaBatchResult(hpa_batch *b, int32_t qi, hpa_V zkey, hpa_V zcnt, hpa_V zsum);
+/* Every query's results of the last run in pinned host memory owned by the plan (valid until
+ * the next fetch): ng[q] groups, keys[q] (i32), cnts[q] (i64), sums[q] (f64). */
+int32_t  hpaBatchFetch(hpa_batch *b, int64_t *ng, void **keys, void **cnts, void **sums);
 /* Tunables: block size, rows per work item, LDS key budget (0 = global tables only). */
 int32_t  hpaBatchSetConfig(hpa_batch *b, int32_t block, int32_t tile_rows, int32_t lds_keys);
 int32_t  hpaBatchGetConfig(hpa_batch *b, int32_t *block, int32_t *tile_rows, int32_t *lds_keys);

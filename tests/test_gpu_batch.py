@@ -57,6 +57,11 @@ def test_batch_vs_oracle(gpu, orc):
             b.set_config(*cfg)
             b.run(True)
             check(b.results(), exp, cfg)
+        for q in (0, 3, len(ts) - 1):                 # the node form of one query's results
+            zk, zc, zs = b.result_nodes(q)
+            r = b.results()[q]
+            assert np.array_equal(gpu.read(zk), r["key"]) and np.array_equal(gpu.read(zc), r["count"])
+            assert np.array_equal(gpu.read(zs), r["sum"])
         ms = b.autotune(1)
         assert len(ms) == 27 and all(t > 0 for t in ms)
         b.run(True)
@@ -75,7 +80,7 @@ def test_batch_graph_replays_with_eager_work_between(gpu, orc):
     b = Batch(gpu, T, thrs)
     try:
         b.run(False)
-        ref = b.results()
+        ref = [{k: v.copy() for k, v in d.items()} for d in b.results()]      # results() are views
         for _ in range(3):
             Q.batch_fused(gpu, T[0], thrs[0])
             b.run(True)

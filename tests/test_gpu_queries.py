@@ -49,10 +49,12 @@ def test_q1(gpu, orc, n):
 def test_q3(gpu, orc, nd):
     dm, f = D.gen_q3(nd)
     o = Q.q3_plain(orc, Q.load_table(orc, dm), Q.load_table(orc, f))
-    g = Q.q3_plain(gpu, Q.load_table(gpu, dm), Q.load_table(gpu, f))
-    assert g["rows"] == o["rows"] and g["rows"] > 0
-    for k in Q.Q3_NAMES:
-        close(g[k], o[k], k)
+    Dg, Fg = Q.load_table(gpu, dm), Q.load_table(gpu, f)
+    for fn in (Q.q3_plain, Q.q3_fused):
+        g = fn(gpu, Dg, Fg)
+        assert g["rows"] == o["rows"] and g["rows"] > 0, fn.__name__
+        for k in Q.Q3_NAMES:
+            close(g[k], o[k], (fn.__name__, k))
 
 
 @pytest.mark.parametrize("n", [3000, 1_000_003])
