@@ -294,12 +294,12 @@ class Batch:
         return z
 
     def config(self):
-        v = [C.c_int32() for _ in range(3)]
+        v = [C.c_int32() for _ in range(4)]
         self.lib.rc("hpaBatchGetConfig", self.h, *[C.byref(x) for x in v])
-        return dict(block=v[0].value, tile_rows=v[1].value, lds_keys=v[2].value)
+        return dict(block=v[0].value, tile_rows=v[1].value, lds_keys=v[2].value, part_keys=v[3].value)
 
-    def set_config(self, block, tile_rows, lds_keys):
-        self.lib.rc("hpaBatchSetConfig", self.h, block, tile_rows, lds_keys)
+    def set_config(self, block, tile_rows, lds_keys, part_keys=2048):
+        self.lib.rc("hpaBatchSetConfig", self.h, block, tile_rows, lds_keys, part_keys)
 
     def autotune(self, iters=3):
         ms = (C.c_float * 64)()

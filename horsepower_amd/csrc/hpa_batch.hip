@@ -1,5 +1,5 @@
 // Batched execution: many independent filter -> compress -> group -> sum queries in one
-// persistent launch.
+// captured launch sequence.
 //
 // Query q reads its own table (i32 key, f64 value, i32 filter column) and computes, over the
 // rows with `flt <op> c`, per distinct key: the key, the row count and the FP64 sum of the
@@ -9,18 +9,25 @@
 //   raze(each(sum, each_right(index, val, groups))), raze(each(len, ...))
 // (the FP1/FP7 patterns of optimizer/pattern.cc:180-219 over a q3.hir-style group-by).
 //
-// Plan, built once and kept resident (its kernels are captured once into a hipGraph and
-// replayed):
-//   k_batch_clear  zero the per-query dense slot arrays (a fill kernel: no memset node in the
-//                  graph)
-//   k_batch_agg    persistent workgroups take (query, row tile) items from an atomic queue.
-//                  Queries whose key range fits the LDS budget aggregate a tile in LDS (one
-//                  replica per wave when the range is small, against same-address contention)
-//                  and flush one global add per touched key; larger ranges add straight into
-//                  the query's slots in HBM.
-//   k_batch_emit   persistent workgroups take (query, slot tile) items and compact the
-//                  non-empty slots in key order (per-query decoupled look-back).
-// Slots are dense over the key range [kmin, kmax] measured when the plan is built.
+// Slots are dense over each query's key range [kmin, kmax], measured when the plan is built.
+// Three aggregation paths, chosen per query by its key range and the tunables:
+//   LDS         range <= lds_keys: a workgroup aggregates a row tile in LDS (one replica per
+//               wave when the range is small) and flushes one global add per touched key.
+//   partitioned range > lds_keys with partitioning on: each row tile counts its rows per key
+//               partition (part_keys consecutive keys), a per-query scan of the (partition,
+//               tile) counts gives every tile its place in each partition, the tile scatters
+//               its rows there with LDS cursors, and one workgroup per partition aggregates it
+//               in LDS and owns its slots outright (no global atomics anywhere on this path).
+//   global      otherwise: compensated global atomics straight into the slots.
+// Kernels, captured once into a hipGraph and replayed (all persistent over an atomic queue of
+// items of every query):
+//   k_batch_clear    zero the slots that are accumulated atomically (a fill kernel: no memset node)
+//   k_batch_agg      row tiles: LDS / global aggregation, or partition counts
+//   k_batch_pscan    (partition, tile) offsets (one workgroup per partitioned query)
+//   k_batch_scatter  row tiles of partitioned queries into their partitions
+//   k_batch_pagg     partitions (split over several workgroups when large): LDS aggregation
+//   k_batch_emit     slot tiles: non-empty slots compacted in key order into one output of all
+//                    queries (one decoupled look-back across every query's slot tiles)
 //
 // FP64 sums are compensated: every atomic add returns the previous value, so the rounding
 // error of that add is recovered exactly (Knuth's TwoSum) and accumulated beside the sum;
@@ -35,31 +42,46 @@
 #define EMIT_ITEMS 8
 #define EMIT_TILE (256 * EMIT_ITEMS)
 #define BATCH_MAX_RANGE (1ll << 26)
+#define PART_MAX 4096          // partitions per query (LDS counters of the count and scatter kernels)
+#define PART_MIN_KEYS 1024     // smallest partition width the tunables may choose
+#define PART_MIN_TILE 8192     // smallest row tile the tunables may choose (sizes the count matrix)
+#define SC_BLOCK 256
 
 struct BQuery {
     const int32_t *key; const double *val; const int32_t *flt;
     int64_t n; int32_t op; int32_t thr;
-    int32_t kmin; int32_t range; int32_t rot; int32_t _pad;  // slot = key - kmin; emit starts at rot
-    double *hi; double *lo; unsigned int *cnt;               // dense slots [range]
-    int32_t *okey; double *osum; int64_t *ocnt;              // compacted outputs (capacity range)
-    int64_t *ng;                                             // [0] groups, [1] keys out of range
-    uint64_t *status;                                        // emit look-back, per slot tile
+    int32_t kmin; int32_t range; int32_t rot; int32_t part;   // slot = key - kmin; emit starts at rot
+    double *hi; double *lo; unsigned int *cnt;                // dense slots [range]
+    int64_t *ng;                                              // [0] end of its groups in the output, [1] keys out of range
     int64_t tile0, ntiles, etile0, entiles;
+    // partitioned aggregation
+    int32_t nparts, psplit;                                   // partitions; workgroups per partition
+    unsigned *pcnt;                                           // rows per (partition, tile), partition-major
+    int64_t *poff;                                            // exclusive scan of pcnt [nparts * ntiles + 1]
+    uint16_t *skey; double *sval;                             // partition-contiguous rows (capacity n)
+    int64_t pitem0;
 };
 
 struct BPlan {
     BQuery *q; int nq;
-    int64_t total_tiles, total_etiles;
-    unsigned *ctr;
-    int tile_rows, lds_keys;
+    int64_t total_tiles, total_etiles, total_pitems;
+    unsigned *ctr;                                            // [0] agg [1] emit [2] scatter [3] pagg
+    uint64_t *estatus;                                        // emit look-back, per slot tile of all queries
+    int32_t *gkey; double *gsum; int64_t *gcnt;               // groups of every query, query after query
+    int tile_rows, lds_keys, part_keys, part_shift;
+    const int *pq; int npq;                                   // partitioned queries
 };
 
-__device__ __forceinline__ int find_query(const BQuery *q, int nq, int64_t t, bool emit){
+enum { SEL_TILE, SEL_EMIT, SEL_PART };
+__device__ __forceinline__ int64_t item0_of(const BQuery &q, int sel){
+    return sel == SEL_TILE ? q.tile0 : sel == SEL_EMIT ? q.etile0 : q.pitem0;
+}
+// the query owning queue item t: the last query whose first item is <= t
+__device__ __forceinline__ int find_query(const BQuery *q, int nq, int64_t t, int sel){
     int lo = 0, hi = nq - 1;
     while(lo < hi){
         int mid = (lo + hi + 1) >> 1;
-        int64_t s = emit ? q[mid].etile0 : q[mid].tile0;
-        if(s <= t) lo = mid; else hi = mid - 1;
+        if(item0_of(q[mid], sel) <= t) lo = mid; else hi = mid - 1;
     }
     return lo;
 }
@@ -78,6 +100,15 @@ __device__ __forceinline__ bool batch_pass(int op, int32_t f, int32_t c){
 __device__ __forceinline__ double two_sum_err(double a, double b, double s){
     double bp = s - a;
     return (a - (s - bp)) + (b - bp);
+}
+
+// next item of one queue for the whole workgroup
+__device__ __forceinline__ int64_t next_item(unsigned *ctr, int64_t *s_item){
+    if(threadIdx.x == 0) *s_item = (int64_t)atomicAdd(ctr, 1u);
+    __syncthreads();
+    const int64_t item = *s_item;
+    __syncthreads();
+    return item;
 }
 
 __global__ void k_batch_minmax(BQuery *qs, int nq, int32_t *mm){
@@ -106,13 +137,12 @@ __global__ void k_batch_clear(BPlan p){
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for(int qi = blockIdx.y; qi < p.nq; qi += gridDim.y){
         const BQuery &q = p.q[qi];
-        for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < q.range; i += stride){
-            q.hi[i] = 0.0; q.lo[i] = 0.0; q.cnt[i] = 0u;
-        }
-        for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < q.entiles; i += stride) q.status[i] = 0;
+        const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if(!q.part || q.psplit > 1) for(int64_t i = t0; i < q.range; i += stride){ q.hi[i] = 0.0; q.lo[i] = 0.0; q.cnt[i] = 0u; }
         if(blockIdx.x == 0 && threadIdx.x < 2) q.ng[threadIdx.x] = 0;
     }
-    if(blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 2) p.ctr[threadIdx.x] = 0;
+    if(blockIdx.y == 0) for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < p.total_etiles; i += stride) p.estatus[i] = 0;
+    if(blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 4) p.ctr[threadIdx.x] = 0;
 }
 
 extern __shared__ double g_batch_lds[];
@@ -121,18 +151,30 @@ __global__ void k_batch_agg(BPlan p){
     __shared__ int64_t s_item;
     const int nw = blockDim.x >> 6, w = threadIdx.x >> 6;
     while(true){
-        if(threadIdx.x == 0) s_item = (int64_t)atomicAdd(&p.ctr[0], 1u);
-        __syncthreads();
-        const int64_t item = s_item;
-        __syncthreads();
+        const int64_t item = next_item(&p.ctr[0], &s_item);
         if(item >= p.total_tiles) break;
-        const BQuery &q = p.q[find_query(p.q, p.nq, item, false)];
+        const BQuery &q = p.q[find_query(p.q, p.nq, item, SEL_TILE)];
         const int64_t r0 = (item - q.tile0) * p.tile_rows;
         const int64_t r1 = r0 + p.tile_rows < q.n ? r0 + p.tile_rows : q.n;
         const int32_t kmin = q.kmin, op = q.op, thr = q.thr;
         const uint32_t range = (uint32_t)q.range;
         bool bad = false;
-        if(q.range <= p.lds_keys){
+        if(q.part){
+            // partitioned query: rows per key partition (reads the filter and key columns only)
+            unsigned *lc = (unsigned*)g_batch_lds;
+            for(int k = threadIdx.x; k < q.nparts; k += blockDim.x) lc[k] = 0u;
+            __syncthreads();
+            for(int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x){
+                if(batch_pass(op, q.flt[i], thr)){
+                    uint32_t s = (uint32_t)q.key[i] - (uint32_t)kmin;
+                    if(s < range) atomicAdd(&lc[s >> p.part_shift], 1u); else bad = true;
+                }
+            }
+            __syncthreads();
+            const int64_t tile = item - q.tile0;
+            for(int k = threadIdx.x; k < q.nparts; k += blockDim.x) q.pcnt[(int64_t)k * q.ntiles + tile] = lc[k];
+            __syncthreads();
+        } else if(q.range <= p.lds_keys){
             int reps = p.lds_keys / q.range; reps = reps > nw ? nw : reps;
             const int tot = reps * q.range;
             double *lhi = g_batch_lds, *llo = lhi + p.lds_keys;
@@ -182,6 +224,106 @@ __global__ void k_batch_agg(BPlan p){
     }
 }
 
+// Exclusive scan of one partitioned query's (partition, tile) counts per workgroup, in rounds of
+// 4096 entries (four consecutive per thread), carrying the running total between rounds.
+__global__ void __launch_bounds__(1024) k_batch_pscan(BPlan p){
+    __shared__ unsigned long long sh[1024];
+    __shared__ unsigned long long s_carry;
+    const BQuery &q = p.q[p.pq[blockIdx.x]];
+    const int t = threadIdx.x;
+    const int64_t N = (int64_t)q.nparts * q.ntiles;
+    if(t == 0) s_carry = 0;
+    __syncthreads();
+    for(int64_t r0 = 0; r0 < N; r0 += 4096){
+        unsigned long long v[4], s = 0;
+        for(int k = 0; k < 4; k++){ int64_t i = r0 + t * 4 + k; v[k] = i < N ? q.pcnt[i] : 0ull; s += v[k]; }
+        sh[t] = s;
+        __syncthreads();
+        for(int o = 1; o < 1024; o <<= 1){
+            unsigned long long a = t >= o ? sh[t - o] : 0ull;
+            __syncthreads();
+            sh[t] += a;
+            __syncthreads();
+        }
+        const unsigned long long carry = s_carry;
+        unsigned long long run = carry + sh[t] - s;
+        for(int k = 0; k < 4; k++){ int64_t i = r0 + t * 4 + k; if(i < N) q.poff[i] = (int64_t)run; run += v[k]; }
+        __syncthreads();
+        if(t == 1023) s_carry = carry + sh[1023];
+        __syncthreads();
+    }
+    if(t == 0) q.poff[N] = (int64_t)s_carry;
+}
+
+// Row tiles of partitioned queries into their partitions: LDS cursors start at the tile's
+// offset in each partition and each selected row takes the next place of its partition.
+__global__ void __launch_bounds__(SC_BLOCK) k_batch_scatter(BPlan p){
+    __shared__ int64_t s_item;
+    unsigned long long *cur = (unsigned long long*)g_batch_lds;         // [PART_MAX]
+    while(true){
+        const int64_t item = next_item(&p.ctr[2], &s_item);
+        if(item >= p.total_tiles) break;
+        const BQuery &q = p.q[find_query(p.q, p.nq, item, SEL_TILE)];
+        if(!q.part) continue;                                            // uniform over the workgroup
+        const int64_t tile = item - q.tile0;
+        const int64_t r0 = tile * p.tile_rows;
+        const int64_t r1 = r0 + p.tile_rows < q.n ? r0 + p.tile_rows : q.n;
+        const uint32_t range = (uint32_t)q.range, mask = (uint32_t)p.part_keys - 1u;
+        for(int k = threadIdx.x; k < q.nparts; k += SC_BLOCK) cur[k] = (unsigned long long)q.poff[(int64_t)k * q.ntiles + tile];
+        __syncthreads();
+        for(int64_t i = r0 + threadIdx.x; i < r1; i += SC_BLOCK){
+            if(batch_pass(q.op, q.flt[i], q.thr)){
+                const uint32_t s = (uint32_t)q.key[i] - (uint32_t)q.kmin;
+                if(s < range){
+                    const unsigned long long pos = atomicAdd(&cur[s >> p.part_shift], 1ull);
+                    q.skey[pos] = (uint16_t)(s & mask); q.sval[pos] = q.val[i];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// One slice of a partition per item (a partition of a query with many rows per partition is
+// split over psplit workgroups): LDS aggregation of its rows, then its part_keys slots are
+// written directly when the partition is whole, or added with compensated atomics when split.
+__global__ void k_batch_pagg(BPlan p){
+    __shared__ int64_t s_item;
+    double *lhi = g_batch_lds, *llo = lhi + p.part_keys;
+    unsigned *lcnt = (unsigned*)(llo + p.part_keys);
+    while(true){
+        const int64_t item = next_item(&p.ctr[3], &s_item);
+        if(item >= p.total_pitems) break;
+        const BQuery &q = p.q[find_query(p.q, p.nq, item, SEL_PART)];
+        const int64_t pt = (item - q.pitem0) / q.psplit, sl = (item - q.pitem0) % q.psplit;
+        for(int k = threadIdx.x; k < p.part_keys; k += blockDim.x){ lhi[k] = 0.0; llo[k] = 0.0; lcnt[k] = 0u; }
+        __syncthreads();
+        const int64_t pa = q.poff[pt * q.ntiles], pb = q.poff[(pt + 1) * q.ntiles];
+        const int64_t a = pa + (pb - pa) * sl / q.psplit, b = pa + (pb - pa) * (sl + 1) / q.psplit;
+        for(int64_t i = a + threadIdx.x; i < b; i += blockDim.x){
+            const uint32_t s = q.skey[i];
+            const double v = q.sval[i];
+            double old = atomicAdd(&lhi[s], v);
+            atomicAdd(&llo[s], two_sum_err(old, v, old + v));
+            atomicAdd(&lcnt[s], 1u);
+        }
+        __syncthreads();
+        const int64_t base = pt * (int64_t)p.part_keys;
+        for(int k = threadIdx.x; k < p.part_keys; k += blockDim.x){
+            const int64_t s = base + k;
+            if(s >= q.range) continue;
+            if(q.psplit == 1){ q.hi[s] = lhi[k]; q.lo[s] = llo[k]; q.cnt[s] = lcnt[k]; }
+            else if(lcnt[k]){
+                const double h = lhi[k];
+                double old = unsafeAtomicAdd(&q.hi[s], h);
+                unsafeAtomicAdd(&q.lo[s], two_sum_err(old, h, old + h) + llo[k]);
+                atomicAdd(&q.cnt[s], lcnt[k]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void __launch_bounds__(256) k_batch_emit(BPlan p){
     __shared__ int64_t s_item;
     __shared__ uint32_t s_cnt[EMIT_ITEMS * 4];
@@ -189,12 +331,9 @@ __global__ void __launch_bounds__(256) k_batch_emit(BPlan p){
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     while(true){
-        if(threadIdx.x == 0) s_item = (int64_t)atomicAdd(&p.ctr[1], 1u);
-        __syncthreads();
-        const int64_t item = s_item;
-        __syncthreads();
+        const int64_t item = next_item(&p.ctr[1], &s_item);
         if(item >= p.total_etiles) break;
-        const BQuery &q = p.q[find_query(p.q, p.nq, item, true)];
+        const BQuery &q = p.q[find_query(p.q, p.nq, item, SEL_EMIT)];
         const int64_t tile = item - q.etile0;
         const int64_t base = tile * EMIT_TILE;
         uint64_t bal[EMIT_ITEMS];
@@ -210,27 +349,27 @@ __global__ void __launch_bounds__(256) k_batch_emit(BPlan p){
         if(threadIdx.x == 0){
             uint32_t run = 0;
             for(int k = 0; k < EMIT_ITEMS * 4; k++){ uint32_t t = s_cnt[k]; s_cnt[k] = run; run += t; }
-            // tiles of one query are consecutive queue items, so the predecessor of this tile
-            // was taken earlier by a running workgroup and its status always arrives
+            // queue items are dispensed in order, so the predecessor of this tile was taken
+            // earlier by a running workgroup and its status always arrives
             uint64_t pre = 0;
-            if(tile == 0) __hip_atomic_store(&q.status[0], ST_PRE | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if(item == 0) __hip_atomic_store(&p.estatus[0], ST_PRE | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             else {
-                __hip_atomic_store(&q.status[tile], ST_AGG | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                int64_t j = tile - 1;
+                __hip_atomic_store(&p.estatus[item], ST_AGG | run, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                int64_t j = item - 1;
                 while(true){
-                    uint64_t st = __hip_atomic_load(&q.status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    uint64_t st = __hip_atomic_load(&p.estatus[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     while((st >> 62) == 0){
                         __builtin_amdgcn_s_sleep(1);
-                        st = __hip_atomic_load(&q.status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        st = __hip_atomic_load(&p.estatus[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     }
                     pre += st & ST_MASK;
                     if((st >> 62) == 2) break;
                     j--;
                 }
-                __hip_atomic_store(&q.status[tile], ST_PRE | (pre + run), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&p.estatus[item], ST_PRE | (pre + run), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             s_pre = (int64_t)pre;
-            if(tile == q.entiles - 1) q.ng[0] = (int64_t)(pre + run);
+            if(tile == q.entiles - 1) q.ng[0] = (int64_t)(pre + run);    // end of the query's groups
         }
         __syncthreads();
         const int64_t pre = s_pre;
@@ -240,7 +379,7 @@ __global__ void __launch_bounds__(256) k_batch_emit(BPlan p){
                 int64_t e = base + (int64_t)k * 256 + threadIdx.x;
                 int64_t s = e + q.rot; if(s >= q.range) s -= q.range;
                 int64_t pos = pre + s_cnt[k * 4 + w] + __popcll(bal[k] & lt);
-                q.okey[pos] = (int32_t)(s + q.kmin); q.osum[pos] = q.hi[s] + q.lo[s]; q.ocnt[pos] = (int64_t)q.cnt[s];
+                p.gkey[pos] = (int32_t)(s + q.kmin); p.gsum[pos] = q.hi[s] + q.lo[s]; p.gcnt[pos] = (int64_t)q.cnt[s];
             }
         }
         __syncthreads();
@@ -255,39 +394,70 @@ struct hpa_batch {
     BQuery *dq = nullptr;
     unsigned *ctr = nullptr;
     int64_t *ngdev = nullptr;
-    int64_t total_tiles = 0, total_etiles = 0;
-    int block = 256, tile_rows = 8192, lds_keys = 2048, grid = 0;
+    uint64_t *estatus = nullptr;
+    int32_t *gkey = nullptr; double *gsum = nullptr; int64_t *gcnt = nullptr;
+    int64_t slots = 0;             // sum of the key ranges: capacity of the output
+    int *dpq = nullptr; int npq = 0;
+    int64_t total_tiles = 0, total_etiles = 0, total_pitems = 0;
+    int block = 256, tile_rows = 8192, lds_keys = 2730, part_keys = 2048, grid = 0, max_parts = 0;
     hipGraph_t graph = nullptr; hipGraphExec_t exec = nullptr; bool captured = false;
     std::vector<int64_t> ng; bool ng_valid = false;
     double scan_bytes = 0;         // algorithmic bytes of one run: the three input columns, once
     char *pinned = nullptr;        // host staging of every query's results (hpaBatchFetch)
-    std::vector<int64_t> pin_off;  // per query: offset of its (keys, counts, sums) block
 };
 
 static BPlan plan_of(hpa_batch *b){
-    BPlan p; p.q = b->dq; p.nq = b->nq; p.total_tiles = b->total_tiles; p.total_etiles = b->total_etiles;
+    BPlan p;
+    p.q = b->dq; p.nq = b->nq; p.total_tiles = b->total_tiles; p.total_etiles = b->total_etiles;
+    p.total_pitems = b->total_pitems;
     p.ctr = b->ctr; p.tile_rows = b->tile_rows; p.lds_keys = b->lds_keys;
+    p.estatus = b->estatus; p.gkey = b->gkey; p.gsum = b->gsum; p.gcnt = b->gcnt;
+    p.part_keys = b->part_keys; p.part_shift = 0;
+    while(b->part_keys > 0 && (1 << p.part_shift) < b->part_keys) p.part_shift++;
+    p.pq = b->dpq; p.npq = b->npq;
     return p;
 }
 
-static size_t lds_bytes(int lds_keys){ return (size_t)lds_keys * (2 * sizeof(double) + sizeof(unsigned)); }
+static size_t lds_bytes(int keys){ return (size_t)keys * (2 * sizeof(double) + sizeof(unsigned)); }
 
 static void drop_graph(hpa_batch *b){
     if(b->captured){ hipGraphExecDestroy(b->exec); hipGraphDestroy(b->graph); }
     b->captured = false; b->exec = nullptr; b->graph = nullptr;
 }
 
-// Tile bookkeeping for the tunables; uploads the descriptors.
-static int32_t batch_layout(hpa_batch *b, int block, int tile_rows, int lds_keys){
+// Tile and partition bookkeeping for the tunables; uploads the descriptors.
+static int32_t batch_layout(hpa_batch *b, int block, int tile_rows, int lds_keys, int part_keys){
     if(block != 64 && block != 128 && block != 256 && block != 512 && block != 1024){ set_error_detail("batch block %d", block); R HPA_E_DOMAIN; }
     if(tile_rows < 256 || lds_keys < 0 || lds_bytes(lds_keys) > 64 * 1024){ set_error_detail("batch tile %d lds %d", tile_rows, lds_keys); R HPA_E_DOMAIN; }
-    b->block = block; b->tile_rows = tile_rows; b->lds_keys = lds_keys;
-    int64_t t = 0;
-    for(auto &q : b->hq){ q.ntiles = ceil_div(q.n, tile_rows); q.tile0 = t; t += q.ntiles; }
-    b->total_tiles = t;
+    if(part_keys != 0 && (part_keys < PART_MIN_KEYS || part_keys > 2048 || (part_keys & (part_keys - 1)))){
+        set_error_detail("batch partition width %d (0, or a power of two in [%d, 2048])", part_keys, PART_MIN_KEYS); R HPA_E_DOMAIN;
+    }
+    if(part_keys != 0 && tile_rows < PART_MIN_TILE){
+        set_error_detail("batch tile %d: partitioning needs tiles of at least %d rows", tile_rows, PART_MIN_TILE); R HPA_E_DOMAIN;
+    }
+    b->block = block; b->tile_rows = tile_rows; b->lds_keys = lds_keys; b->part_keys = part_keys;
+    int64_t t = 0, pi = 0;
+    std::vector<int> pq;
+    b->max_parts = 0;
+    for(int qi = 0; qi < b->nq; qi++){
+        BQuery &q = b->hq[qi];
+        q.ntiles = ceil_div(q.n, tile_rows); q.tile0 = t; t += q.ntiles;
+        const int64_t P = part_keys ? ceil_div(q.range, part_keys) : 0;
+        // the count matrix was sized for the narrowest partitions and the smallest tiles
+        q.part = (part_keys && q.range > lds_keys && q.skey && P <= PART_MAX) ? 1 : 0;
+        q.nparts = q.part ? (int32_t)P : 0;
+        // about 64K input rows per aggregating workgroup, at most 64 workgroups per partition
+        int64_t sp = q.part ? ceil_div(ceil_div(q.n, P), 65536) : 1;
+        q.psplit = (int32_t)(sp < 1 ? 1 : sp > 64 ? 64 : sp);
+        q.pitem0 = pi; pi += (int64_t)q.nparts * q.psplit;
+        if(q.part){ pq.push_back(qi); b->max_parts = q.nparts > b->max_parts ? q.nparts : b->max_parts; }
+    }
+    b->total_tiles = t; b->total_pitems = pi;
+    b->npq = (int)pq.size();
     b->grid = num_cus() * (2048 / block);
     drop_graph(b);
     if(hipMemcpyAsync(b->dq, b->hq.data(), sizeof(BQuery) * b->nq, hipMemcpyHostToDevice, b->c->stream) != hipSuccess) R HPA_E_DEVICE;
+    if(b->npq && hipMemcpyAsync(b->dpq, pq.data(), sizeof(int) * b->npq, hipMemcpyHostToDevice, b->c->stream) != hipSuccess) R HPA_E_DEVICE;
     if(hipStreamSynchronize(b->c->stream) != hipSuccess) R HPA_E_DEVICE;
     R 0;
 }
@@ -297,7 +467,14 @@ static int32_t batch_enqueue(hpa_batch *b, hipStream_t s){
     int64_t maxr = 1; for(auto &q : b->hq) maxr = q.range > maxr ? q.range : maxr;
     int64_t gx = ceil_div(maxr, 256); gx = gx > 256 ? 256 : gx;
     hipLaunchKernelGGL(k_batch_clear, dim3((unsigned)gx, (unsigned)(b->nq < 64 ? b->nq : 64)), dim3(256), 0, s, p);
-    hipLaunchKernelGGL(k_batch_agg, dim3(b->grid), dim3(b->block), lds_bytes(b->lds_keys), s, p);
+    size_t agg_lds = lds_bytes(b->lds_keys);
+    if((size_t)b->max_parts * 4 > agg_lds) agg_lds = (size_t)b->max_parts * 4;
+    hipLaunchKernelGGL(k_batch_agg, dim3(b->grid), dim3(b->block), agg_lds, s, p);
+    if(b->npq){
+        hipLaunchKernelGGL(k_batch_pscan, dim3((unsigned)b->npq), dim3(1024), 0, s, p);
+        hipLaunchKernelGGL(k_batch_scatter, dim3(num_cus() * 4), dim3(SC_BLOCK), (size_t)b->max_parts * 8, s, p);
+        hipLaunchKernelGGL(k_batch_pagg, dim3(num_cus() * 4), dim3(256), lds_bytes(b->part_keys), s, p);
+    }
     hipLaunchKernelGGL(k_batch_emit, dim3(num_cus() * 8), dim3(256), 0, s, p);
     R check_launch("batch");
 }
@@ -333,7 +510,8 @@ extern "C" hpa_batch *hpaBatchCreate(int32_t nq, hpa_V *keys, hpa_V *vals, hpa_V
     // key ranges: one min/max pass over every query's keys
     b->dq = (BQuery*)dalloc(c, sizeof(BQuery) * nq);
     int32_t *mm = (int32_t*)dalloc(c, 8 * (L)nq);
-    if(!b->dq || !mm) R fail("arena full", 0);
+    b->dpq = (int*)dalloc(c, sizeof(int) * nq);
+    if(!b->dq || !mm || !b->dpq) R fail("arena full", 0);
     std::vector<int32_t> hmm(2 * nq);
     for(int qi = 0; qi < nq; qi++){ hmm[2 * qi] = INT32_MAX; hmm[2 * qi + 1] = INT32_MIN; }
     hipMemcpyAsync(b->dq, b->hq.data(), sizeof(BQuery) * nq, hipMemcpyHostToDevice, c->stream);
@@ -352,64 +530,38 @@ extern "C" hpa_batch *hpaBatchCreate(int32_t nq, hpa_V *keys, hpa_V *vals, hpa_V
         q.rot = (lo < 0 && hi >= 0) ? -lo : 0;
         q.hi = (double*)dalloc(c, 8 * range); q.lo = (double*)dalloc(c, 8 * range);
         q.cnt = (unsigned*)dalloc(c, 4 * range);
-        q.okey = (int32_t*)dalloc(c, 4 * range); q.osum = (double*)dalloc(c, 8 * range);
-        q.ocnt = (int64_t*)dalloc(c, 8 * range);
         q.entiles = ceil_div(range, EMIT_TILE);
-        q.status = (uint64_t*)dalloc(c, 8 * q.entiles);
-        if(!q.hi || !q.lo || !q.cnt || !q.okey || !q.osum || !q.ocnt || !q.status) R fail("arena full", qi);
+        b->slots += range;
+        if(!q.hi || !q.lo || !q.cnt) R fail("arena full", qi);
         q.etile0 = et; et += q.entiles;
+        // partition buffers for any partition width and tile size the tunables allow
+        if(ceil_div(range, PART_MIN_KEYS) <= PART_MAX && range > 256 && q.n > 0){
+            const int64_t M = ceil_div(range, PART_MIN_KEYS) * ceil_div(q.n, PART_MIN_TILE);
+            q.pcnt = (unsigned*)dalloc(c, 4 * M); q.poff = (int64_t*)dalloc(c, 8 * (M + 1));
+            q.skey = (uint16_t*)dalloc(c, 2 * q.n); q.sval = (double*)dalloc(c, 8 * q.n);
+            if(!q.pcnt || !q.poff || !q.skey || !q.sval) R fail("arena full", qi);
+        }
     }
     b->total_etiles = et;
+    b->estatus = (uint64_t*)dalloc(c, 8 * et);
+    b->gkey = (int32_t*)dalloc(c, 4 * b->slots); b->gsum = (double*)dalloc(c, 8 * b->slots);
+    b->gcnt = (int64_t*)dalloc(c, 8 * b->slots);
+    if(!b->estatus || !b->gkey || !b->gsum || !b->gcnt) R fail("arena full", 0);
     b->ngdev = (int64_t*)dalloc(c, 16 * (L)nq);
     b->ctr = (unsigned*)dalloc(c, 16);
     if(!b->ngdev || !b->ctr) R fail("arena full", 0);
     for(int qi = 0; qi < nq; qi++) b->hq[qi].ng = b->ngdev + 2 * qi;
     b->ng.assign(2 * (size_t)nq, 0);
-    if(batch_layout(b, 256, 8192, 2048)){ delete b; R nullptr; }
+    if(batch_layout(b, 256, 8192, 2730, 2048)){ delete b; R nullptr; }
     R b;
 }
 
-// All queries' results of the last run, copied into pinned host memory owned by the plan with
-// one read of the group counts, one copy per output vector and one synchronisation.  For query
-// q: ng[q] groups; keys[q] (i32), cnts[q] (i64) and sums[q] (f64) point into the staging, valid
-// until the next fetch or the plan's destruction.
-extern "C" int32_t hpaBatchFetch(hpa_batch *b, int64_t *ng, void **keys, void **cnts, void **sums){
-    hpa_ctx *c = b->c;
-    int32_t e;
-    if(!b->pinned){
-        int64_t tot = 0;
-        b->pin_off.resize(b->nq);
-        for(int qi = 0; qi < b->nq; qi++){ b->pin_off[qi] = tot; tot += 24 * (int64_t)b->hq[qi].range; }
-        if(hipHostMalloc((void**)&b->pinned, (size_t)(tot > 0 ? tot : 8), hipHostMallocDefault) != hipSuccess){
-            b->pinned = nullptr; set_error_detail("batch: pinned staging of %lld bytes", (long long)tot); R HPA_E_MEMORY;
-        }
-    }
-    if(!b->ng_valid){
-        if((e = d2h(c, b->ng.data(), b->ngdev, 16 * (L)b->nq))) R e;
-        b->ng_valid = true;
-    }
-    for(int qi = 0; qi < b->nq; qi++){
-        if(b->ng[2 * qi + 1]){ set_error_detail("batch query %d: key outside the planned range", qi); R HPA_E_DOMAIN; }
-        const BQuery &q = b->hq[qi];
-        const int64_t g = b->ng[2 * qi];
-        char *base = b->pinned + b->pin_off[qi];
-        char *hk = base, *hc = base + 8 * (int64_t)q.range, *hs = base + 16 * (int64_t)q.range;
-        ng[qi] = g; keys[qi] = hk; cnts[qi] = hc; sums[qi] = hs;
-        if(g > 0){
-            if(hipMemcpyAsync(hk, q.okey, 4 * g, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-               hipMemcpyAsync(hc, q.ocnt, 8 * g, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-               hipMemcpyAsync(hs, q.osum, 8 * g, hipMemcpyDeviceToHost, c->stream) != hipSuccess) R HPA_E_DEVICE;
-        }
-    }
-    R hipStreamSynchronize(c->stream) == hipSuccess ? 0 : HPA_E_DEVICE;
+extern "C" int32_t hpaBatchSetConfig(hpa_batch *b, int32_t block, int32_t tile_rows, int32_t lds_keys, int32_t part_keys){
+    R batch_layout(b, block, tile_rows, lds_keys, part_keys);
 }
 
-extern "C" int32_t hpaBatchSetConfig(hpa_batch *b, int32_t block, int32_t tile_rows, int32_t lds_keys){
-    R batch_layout(b, block, tile_rows, lds_keys);
-}
-
-extern "C" int32_t hpaBatchGetConfig(hpa_batch *b, int32_t *block, int32_t *tile_rows, int32_t *lds_keys){
-    *block = b->block; *tile_rows = b->tile_rows; *lds_keys = b->lds_keys;
+extern "C" int32_t hpaBatchGetConfig(hpa_batch *b, int32_t *block, int32_t *tile_rows, int32_t *lds_keys, int32_t *part_keys){
+    *block = b->block; *tile_rows = b->tile_rows; *lds_keys = b->lds_keys; *part_keys = b->part_keys;
     R 0;
 }
 
@@ -434,42 +586,81 @@ extern "C" int32_t hpaBatchRun(hpa_batch *b, int32_t use_graph){
     R batch_enqueue(b, c->stream);
 }
 
+static int32_t batch_ends(hpa_batch *b){
+    if(b->ng_valid) R 0;
+    int32_t e = d2h(b->c, b->ng.data(), b->ngdev, 16 * (L)b->nq);
+    if(e) R e;
+    b->ng_valid = true;
+    for(int qi = 0; qi < b->nq; qi++)
+        if(b->ng[2 * qi + 1]){ set_error_detail("batch query %d: key outside the planned range", qi); R HPA_E_DOMAIN; }
+    R 0;
+}
+// groups of query qi in the output: [start, end)
+static void batch_span(hpa_batch *b, int qi, int64_t *start, int64_t *end){
+    *start = qi > 0 ? b->ng[2 * (qi - 1)] : 0; *end = b->ng[2 * qi];
+}
+
 // Result of query qi of the last run: zkey (I), zcnt (L), zsum (E) over the groups in the
-// pfnGroup order.  The vectors point into the plan's buffers (valid until the next run).
+// pfnGroup order.  The vectors point into the plan's output (valid until the next run).
 extern "C" int32_t hpaBatchResult(hpa_batch *b, int32_t qi, hpa_V zkey, hpa_V zcnt, hpa_V zsum){
     hpa_ctx *c = b->c;
     if(qi < 0 || qi >= b->nq) R HPA_E_INDEX;
     int32_t e;
-    if(!b->ng_valid){
-        if((e = d2h(c, b->ng.data(), b->ngdev, 16 * (L)b->nq))) R e;
-        b->ng_valid = true;
-    }
-    if(b->ng[2 * qi + 1]){ set_error_detail("batch query %d: key outside the planned range", qi); R HPA_E_DOMAIN; }
-    const int64_t g = b->ng[2 * qi];
-    const BQuery &q = b->hq[qi];
-    auto setv = [&](V z, int32_t typ, void *p, L w) -> int32_t {
-        z->typ = typ; z->len = g;
-        if(g <= 1){ z->g = (char*)&z->b; z->i64 = 0; R g == 1 ? d2h(c, &z->b, p, w) : 0; }
-        z->g = (char*)p; R 0;
+    if((e = batch_ends(b))) R e;
+    int64_t a, z; batch_span(b, qi, &a, &z);
+    const int64_t g = z - a;
+    auto setv = [&](V v, int32_t typ, void *p, L w) -> int32_t {
+        v->typ = typ; v->len = g;
+        if(g <= 1){ v->g = (char*)&v->b; v->i64 = 0; R g == 1 ? d2h(c, &v->b, p, w) : 0; }
+        v->g = (char*)p; R 0;
     };
-    if((e = setv(zkey, HPA_I, q.okey, 4)) || (e = setv(zcnt, HPA_L, q.ocnt, 8)) || (e = setv(zsum, HPA_E, q.osum, 8))) R e;
+    if((e = setv(zkey, HPA_I, b->gkey + a, 4)) || (e = setv(zcnt, HPA_L, b->gcnt + a, 8)) || (e = setv(zsum, HPA_E, b->gsum + a, 8))) R e;
     R 0;
 }
 
-// Autotuner: time each candidate (block size, tile rows, LDS key budget) over `iters` runs of
-// this plan and keep the fastest.  Writes the timings of all candidates to `ms` (capacity
-// `cap`, may be null) in candidate order; returns the number of candidates through *ncand.
+// All queries' results of the last run, copied into pinned host memory owned by the plan: one
+// read of the query ends, three copies (keys, counts, sums of every query, query after query)
+// and one synchronisation.  For query q: ng[q] groups; keys[q] (i32), cnts[q] (i64) and
+// sums[q] (f64) point into the staging, valid until the next fetch or the plan's destruction.
+extern "C" int32_t hpaBatchFetch(hpa_batch *b, int64_t *ng, void **keys, void **cnts, void **sums){
+    hpa_ctx *c = b->c;
+    int32_t e;
+    if(!b->pinned){
+        if(hipHostMalloc((void**)&b->pinned, (size_t)(20 * (b->slots > 0 ? b->slots : 1)), hipHostMallocDefault) != hipSuccess){
+            b->pinned = nullptr; set_error_detail("batch: pinned staging of %lld bytes", (long long)(20 * b->slots)); R HPA_E_MEMORY;
+        }
+    }
+    if((e = batch_ends(b))) R e;
+    const int64_t G = b->ng[2 * (b->nq - 1)];
+    int32_t *hk = (int32_t*)(b->pinned + 16 * b->slots);
+    int64_t *hc = (int64_t*)b->pinned;
+    double *hs = (double*)(b->pinned + 8 * b->slots);
+    if(G > 0){
+        if(hipMemcpyAsync(hk, b->gkey, 4 * G, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+           hipMemcpyAsync(hc, b->gcnt, 8 * G, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+           hipMemcpyAsync(hs, b->gsum, 8 * G, hipMemcpyDeviceToHost, c->stream) != hipSuccess) R HPA_E_DEVICE;
+    }
+    for(int qi = 0; qi < b->nq; qi++){
+        int64_t a, z; batch_span(b, qi, &a, &z);
+        ng[qi] = z - a; keys[qi] = hk + a; cnts[qi] = hc + a; sums[qi] = hs + a;
+    }
+    R hipStreamSynchronize(c->stream) == hipSuccess ? 0 : HPA_E_DEVICE;
+}
+
+// Autotuner: time each candidate (block size, tile rows, LDS key budget, partition width) over
+// `iters` runs of this plan and keep the fastest.  Writes the timings of all candidates to `ms`
+// (capacity `cap`, may be null) in candidate order; returns their number through *ncand.
 extern "C" int32_t hpaBatchAutotune(hpa_batch *b, int32_t iters, float *ms, int32_t cap, int32_t *ncand){
     static const int blocks[] = {256, 512, 1024};
-    static const int tiles[] = {4096, 16384, 65536};
-    static const int ldsk[] = {0, 512, 2730};
+    static const int tiles[] = {8192, 32768};
+    static const int modes[][2] = {{2730, 0}, {2730, 2048}, {1024, 1024}, {1024, 2048}};
     hpa_ctx *c = b->c;
     hipEvent_t a, z;
     hipEventCreate(&a); hipEventCreate(&z);
-    float best = 1e30f; int bb = b->block, bt = b->tile_rows, bl = b->lds_keys, k = 0;
+    float best = 1e30f; int bb = b->block, bt = b->tile_rows, bl = b->lds_keys, bp = b->part_keys, k = 0;
     int32_t e = 0;
-    for(int bi : blocks) for(int ti : tiles) for(int li : ldsk){
-        if((e = batch_layout(b, bi, ti, li))) goto done;
+    for(int bi : blocks) for(int ti : tiles) for(auto &md : modes){
+        if((e = batch_layout(b, bi, ti, md[0], md[1]))) goto done;
         if((e = batch_enqueue(b, c->stream))) goto done;           // warm
         hipEventRecord(a, c->stream);
         for(int it = 0; it < iters; it++) if((e = batch_enqueue(b, c->stream))) goto done;
@@ -478,12 +669,12 @@ extern "C" int32_t hpaBatchAutotune(hpa_batch *b, int32_t iters, float *ms, int3
         float t = 0; hipEventElapsedTime(&t, a, z); t /= iters > 0 ? iters : 1;
         if(ms && k < cap) ms[k] = t;
         k++;
-        if(t < best){ best = t; bb = bi; bt = ti; bl = li; }
+        if(t < best){ best = t; bb = bi; bt = ti; bl = md[0]; bp = md[1]; }
     }
 done:
     hipEventDestroy(a); hipEventDestroy(z);
     if(ncand) *ncand = k;
-    int32_t e2 = batch_layout(b, bb, bt, bl);
+    int32_t e2 = batch_layout(b, bb, bt, bl, bp);
     b->ng_valid = false;
     R e ? e : e2;
 }

@@ -296,9 +296,11 @@ int32_t  hpaBatchResult(hpa_batch *b, int32_t qi, hpa_V zkey, hpa_V zcnt, hpa_V 
 /* Every query's results of the last run in pinned host memory owned by the plan (valid until
  * the next fetch): ng[q] groups, keys[q] (i32), cnts[q] (i64), sums[q] (f64). */
 int32_t  hpaBatchFetch(hpa_batch *b, int64_t *ng, void **keys, void **cnts, void **sums);
-/* Tunables: block size, rows per work item, LDS key budget (0 = global tables only). */
-int32_t  hpaBatchSetConfig(hpa_batch *b, int32_t block, int32_t tile_rows, int32_t lds_keys);
-int32_t  hpaBatchGetConfig(hpa_batch *b, int32_t *block, int32_t *tile_rows, int32_t *lds_keys);
+/* Tunables: block size, rows per work item, LDS key budget, and the key-partition width of
+ * queries whose key range exceeds the LDS budget (0 = compensated global atomics instead;
+ * otherwise a power of two in [1024, 2048]). */
+int32_t  hpaBatchSetConfig(hpa_batch *b, int32_t block, int32_t tile_rows, int32_t lds_keys, int32_t part_keys);
+int32_t  hpaBatchGetConfig(hpa_batch *b, int32_t *block, int32_t *tile_rows, int32_t *lds_keys, int32_t *part_keys);
 /* Autotuner: times every candidate configuration on this plan and keeps the fastest. */
 int32_t  hpaBatchAutotune(hpa_batch *b, int32_t iters, float *ms, int32_t cap, int32_t *ncand);
 

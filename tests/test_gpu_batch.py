@@ -53,7 +53,8 @@ def test_batch_vs_oracle(gpu, orc):
         for graph in (False, True, True):
             b.run(graph)
             check(b.results(), exp, ("graph", graph))
-        for cfg in [(256, 4096, 0), (1024, 65536, 2730), (512, 16384, 512), (64, 256, 100)]:
+        for cfg in [(256, 4096, 0, 0), (1024, 65536, 2730, 2048), (512, 16384, 512, 1024), (64, 256, 100, 0),
+                    (256, 8192, 0, 2048), (128, 8192, 2730, 1024)]:
             b.set_config(*cfg)
             b.run(True)
             check(b.results(), exp, cfg)
@@ -63,7 +64,7 @@ def test_batch_vs_oracle(gpu, orc):
             assert np.array_equal(gpu.read(zk), r["key"]) and np.array_equal(gpu.read(zc), r["count"])
             assert np.array_equal(gpu.read(zs), r["sum"])
         ms = b.autotune(1)
-        assert len(ms) == 27 and all(t > 0 for t in ms)
+        assert len(ms) == 24 and all(t > 0 for t in ms)
         b.run(True)
         check(b.results(), exp, ("tuned", b.config()))
     finally:

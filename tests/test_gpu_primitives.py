@@ -306,3 +306,29 @@ def test_grouped_aggregates(gpu, orc, n, card):
     for kind, fn in ((0, "pfnSum"), (1, "pfnAvg"), (2, "pfnMin"), (3, "pfnMax")):
         fp = gpu.read(gpu.group_reduce(kind, gpu.vec(v), vals))
         assert np.allclose(fp, res["g"]["v" + fn], rtol=RTOL, atol=0)
+
+
+def test_arena_rewind_and_contexts(gpu):
+    """Heap offset rewinds the arena; a context whose arena is too small returns HPA_E_MEMORY
+    (the reference exits on a full heap) and leaves the default context usable."""
+    lib = gpu.lib
+    x = gpu.vec(np.arange(200_000, dtype=np.int64))   # before the mark: rewinding frees later nodes
+    m = gpu.mark()
+    gpu.pfnPlus(x, x)
+    assert lib.dll.hpaGetHeapOffset() > m
+    gpu.release(m)
+    assert lib.dll.hpaGetHeapOffset() == m
+    ctx = lib.dll.hpaCtxCreate(1 << 20)
+    assert ctx
+    lib.dll.hpaCtxSetCurrent(ctx)
+    try:
+        with pytest.raises(HorseError) as ei:
+            gpu.pfnPlus(x, x)                      # a 1.6 MB output: more than the 1 MiB arena
+        assert ei.value.code == 100
+        small = gpu.pfnPlus(gpu.host.vec(np.arange(10, dtype=np.int64)), gpu.host.vec(np.arange(10, dtype=np.int64)))
+        assert np.array_equal(gpu.read(small), 2 * np.arange(10))
+    finally:
+        lib.dll.hpaCtxSetCurrent(None)
+        lib.dll.hpaCtxDestroy(ctx)
+    assert np.array_equal(gpu.read(gpu.pfnPlus(x, x)), 2 * np.arange(200_000))
+    gpu.release(m)
