@@ -448,9 +448,19 @@ struct Stat { int64_t count = 0; double ms = 0, bytes = 0; };
 static std::map<std::string, Stat> g_stats;
 
 bool prof_enabled(){ R g_prof; }
+// events are recycled: creating one per launch would add its own cost to what is measured
+static std::vector<hipEvent_t> g_evpool;
+static hipEvent_t ev_get(){
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if(!g_evpool.empty()){ hipEvent_t e = g_evpool.back(); g_evpool.pop_back(); R e; }
+    }
+    hipEvent_t e = nullptr; hipEventCreate(&e); R e;
+}
+static void ev_put(hipEvent_t e){ std::lock_guard<std::mutex> lk(g_mu); g_evpool.push_back(e); }
 ProfScope::ProfScope(const char *n, hipStream_t st, double algo_bytes) : name(n), s(st), on(g_prof), bytes(algo_bytes){
     if(!on) R;
-    hipEventCreate(&a); hipEventCreate(&b);
+    a = ev_get(); b = ev_get();
     hipEventRecord(a, s);
 }
 void ProfScope::stop(){
@@ -470,7 +480,7 @@ static void prof_resolve(){
     for(auto &e : p){
         hipEventSynchronize(e.b);
         float ms = 0; hipEventElapsedTime(&ms, e.a, e.b);
-        hipEventDestroy(e.a); hipEventDestroy(e.b);
+        ev_put(e.a); ev_put(e.b);
         std::lock_guard<std::mutex> lk(g_mu);
         auto &s = g_stats[e.name]; s.count++; s.ms += ms; s.bytes += e.bytes;
     }

@@ -108,6 +108,88 @@ __global__ void __launch_bounds__(CP_BLOCK) k_cp_write(Pred pred, L n, CpCols co
     }
 }
 
+// Pass 2 for a byte mask: thread t owns the 16 consecutive rows [16t, 16t + 16) of the tile and
+// reads their mask bytes with one 16-byte load and a selected run's values with vector loads;
+// the tile's selected values are compacted in LDS (at the thread's offset from a scan of the
+// threads' counts) and then written out by all threads together, consecutive threads to
+// consecutive places.  Requires the mask and every column to start on a 16-element boundary,
+// as allocations do.
+template<int W>
+__device__ __forceinline__ void cp_stage(const char *src, uint64_t *buf, L i0, uint32_t off, uint32_t m, bool full){
+    typedef typename std::conditional<W == 1, uint8_t, typename std::conditional<W == 2, uint16_t,
+            typename std::conditional<W == 4, uint32_t, uint64_t>::type>::type>::type E;
+    E *lb = (E*)buf;
+    if(full){
+        E v[16];
+        const uint4 *s4 = (const uint4*)((const E*)src + i0);
+        #pragma unroll
+        for(int q = 0; q < (16 * W) / 16; q++) ((uint4*)v)[q] = s4[q];
+        #pragma unroll
+        for(int k = 0; k < 16; k++) if((m >> k) & 1u) lb[off++] = v[k];
+    } else {
+        for(int k = 0; k < 16; k++) if((m >> k) & 1u) lb[off++] = ((const E*)src)[i0 + k];
+    }
+}
+template<int W>
+__device__ __forceinline__ void cp_flush(const uint64_t *buf, char *dst, int64_t pre, uint32_t cnt){
+    typedef typename std::conditional<W == 1, uint8_t, typename std::conditional<W == 2, uint16_t,
+            typename std::conditional<W == 4, uint32_t, uint64_t>::type>::type>::type E;
+    const E *lb = (const E*)buf;
+    for(uint32_t j = threadIdx.x; j < cnt; j += CP_BLOCK) ((E*)dst)[pre + j] = lb[j];
+}
+
+__global__ void __launch_bounds__(CP_BLOCK) k_cp_write_mask(const uint8_t *mask, L n, CpCols cols, int64_t *idx_out, const int64_t *toff){
+    __shared__ uint64_t buf[CP_TILE];
+    __shared__ uint32_t sw[CP_BLOCK / 64];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const L i0 = (L)blockIdx.x * CP_TILE + (L)t * 16;
+    uint32_t m = 0;
+    const bool full = i0 + 16 <= n;
+    if(full){
+        const uint4 v = *(const uint4*)(mask + i0);
+        const uint32_t x[4] = {v.x, v.y, v.z, v.w};
+        #pragma unroll
+        for(int q = 0; q < 4; q++)
+            #pragma unroll
+            for(int b = 0; b < 4; b++) if((x[q] >> (8 * b)) & 0xFFu) m |= 1u << (4 * q + b);
+    } else {
+        for(int k = 0; k < 16 && i0 + k < n; k++) if(mask[i0 + k]) m |= 1u << k;
+    }
+    // exclusive scan of the per-thread counts in thread (= row) order
+    const uint32_t c = (uint32_t)__popc(m);
+    uint32_t inc = c;
+    for(int o = 1; o < 64; o <<= 1){ uint32_t y = __shfl_up(inc, o, 64); if(lane >= o) inc += y; }
+    if(lane == 63) sw[w] = inc;
+    __syncthreads();
+    uint32_t wbase = 0, total = 0;
+    for(int q = 0; q < CP_BLOCK / 64; q++){ if(q < w) wbase += sw[q]; total += sw[q]; }
+    if(total == 0) return;                                   // uniform over the workgroup
+    const uint32_t off = wbase + inc - c;
+    const int64_t pre = toff[blockIdx.x];
+    for(int col = 0; col < cols.n; col++){
+        if(m) switch(cols.w[col]){
+            case 1: cp_stage<1>(cols.src[col], buf, i0, off, m, full); break;
+            case 2: cp_stage<2>(cols.src[col], buf, i0, off, m, full); break;
+            case 4: cp_stage<4>(cols.src[col], buf, i0, off, m, full); break;
+            default: cp_stage<8>(cols.src[col], buf, i0, off, m, full); break;
+        }
+        __syncthreads();
+        switch(cols.w[col]){
+            case 1: cp_flush<1>(buf, cols.dst[col], pre, total); break;
+            case 2: cp_flush<2>(buf, cols.dst[col], pre, total); break;
+            case 4: cp_flush<4>(buf, cols.dst[col], pre, total); break;
+            default: cp_flush<8>(buf, cols.dst[col], pre, total); break;
+        }
+        __syncthreads();
+    }
+    if(idx_out){
+        uint32_t o = off;
+        for(int k = 0; k < 16; k++) if((m >> k) & 1u) buf[o++] = (uint64_t)(i0 + k);
+        __syncthreads();
+        cp_flush<8>(buf, (char*)idx_out, pre, total);
+    }
+}
+
 // Counting pass: tile offsets on the device and the total on the host.
 template<typename Pred>
 static int32_t compress_count(hpa_ctx *c, Pred pred, const uint8_t *mask, L n, int64_t **toff_out, int64_t *k_host){
@@ -132,6 +214,17 @@ static int32_t compress_write(hpa_ctx *c, Pred pred, L n, CpCols cols, int64_t *
     L ntiles = ceil_div(n, CP_TILE);
     if(ntiles < 1) R 0;
     hipLaunchKernelGGL(k_cp_write<Pred>, dim3((unsigned)ntiles), dim3(CP_BLOCK), 0, c->stream, pred, n, cols, idx_out, toff);
+    R check_launch("compress_write");
+}
+// the byte-mask form: 16 consecutive rows per thread when every operand is 16-element aligned
+static int32_t compress_write(hpa_ctx *c, MaskPred pred, L n, CpCols cols, int64_t *idx_out, const int64_t *toff){
+    L ntiles = ceil_div(n, CP_TILE);
+    if(ntiles < 1) R 0;
+    bool aligned = ((uintptr_t)pred.m & 15) == 0;
+    for(int k = 0; k < cols.n; k++)
+        aligned = aligned && ((uintptr_t)cols.src[k] % (uintptr_t)(16 * cols.w[k])) == 0 && cols.w[k] <= 8;
+    if(aligned) hipLaunchKernelGGL(k_cp_write_mask, dim3((unsigned)ntiles), dim3(CP_BLOCK), 0, c->stream, pred.m, n, cols, idx_out, toff);
+    else hipLaunchKernelGGL(k_cp_write<MaskPred>, dim3((unsigned)ntiles), dim3(CP_BLOCK), 0, c->stream, pred, n, cols, idx_out, toff);
     R check_launch("compress_write");
 }
 

@@ -22,19 +22,49 @@
 #define RS_ROWS  16
 #define RS_TILE  (RS_BLOCK * RS_ROWS)
 
+// Digit histogram of one 4096-key tile.  With `bits` set (the first pass) it also writes the
+// tile's OR and AND of all keys to bits[2*tile], bits[2*tile+1]: which key bits vary.
 template<typename K>
-__global__ void __launch_bounds__(RS_BLOCK) k_rs_hist(const K *keys, L n, int shift, uint32_t *hist, L ntiles){
+__global__ void __launch_bounds__(RS_BLOCK) k_rs_hist(const K *keys, L n, int shift, uint32_t *hist, L ntiles,
+                                                      unsigned long long *bits){
     __shared__ uint32_t cnt[256];
+    __shared__ unsigned long long so[RS_BLOCK / 64], sa[RS_BLOCK / 64];
     cnt[threadIdx.x] = 0;
     __syncthreads();
     L base = (L)blockIdx.x * RS_TILE;
+    unsigned long long o = 0, a = ~0ull;
     #pragma unroll 4
     for(int r = 0; r < RS_ROWS; r++){
         L i = base + (L)r * RS_BLOCK + threadIdx.x;
-        if(i < n){ unsigned d = (unsigned)((keys[i] >> shift) & 255); atomicAdd(&cnt[d], 1u); }
+        if(i < n){
+            const K k = keys[i];
+            unsigned d = (unsigned)((k >> shift) & 255); atomicAdd(&cnt[d], 1u);
+            o |= (unsigned long long)k; a &= (unsigned long long)k;
+        }
+    }
+    if(bits){
+        for(int s = 32; s > 0; s >>= 1){ o |= __shfl_down(o, s, 64); a &= __shfl_down(a, s, 64); }
+        if((threadIdx.x & 63) == 0){ so[threadIdx.x >> 6] = o; sa[threadIdx.x >> 6] = a; }
     }
     __syncthreads();
     hist[(L)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+    if(bits && threadIdx.x == 0){
+        for(int w = 1; w < RS_BLOCK / 64; w++){ o |= so[w]; a &= sa[w]; }
+        bits[2 * blockIdx.x] = o; bits[2 * blockIdx.x + 1] = a;
+    }
+}
+// OR / AND of the tiles' partials (one workgroup)
+__global__ void __launch_bounds__(256) k_rs_bits_final(const unsigned long long *part, L ntiles, unsigned long long *out){
+    __shared__ unsigned long long so[4], sa[4];
+    unsigned long long o = 0, a = ~0ull;
+    for(L t = threadIdx.x; t < ntiles; t += 256){ o |= part[2 * t]; a &= part[2 * t + 1]; }
+    for(int s = 32; s > 0; s >>= 1){ o |= __shfl_down(o, s, 64); a &= __shfl_down(a, s, 64); }
+    if((threadIdx.x & 63) == 0){ so[threadIdx.x >> 6] = o; sa[threadIdx.x >> 6] = a; }
+    __syncthreads();
+    if(threadIdx.x == 0){
+        for(int w = 1; w < 4; w++){ o |= so[w]; a &= sa[w]; }
+        out[0] = o; out[1] = a;
+    }
 }
 
 template<typename K>
@@ -83,22 +113,6 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K *kin, const uin
     }
 }
 
-// OR and AND of all keys (which bits vary): wave and workgroup reductions, one pair of global
-// atomics per workgroup.
-template<typename K>
-__global__ void __launch_bounds__(256) k_rs_bits(const K *keys, L n, unsigned long long *orv, unsigned long long *andv){
-    __shared__ unsigned long long so[4], sa[4];
-    unsigned long long o = 0, a = ~0ull;
-    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){ unsigned long long k = (unsigned long long)keys[i]; o |= k; a &= k; }
-    for(int s = 32; s > 0; s >>= 1){ o |= __shfl_down(o, s, 64); a &= __shfl_down(a, s, 64); }
-    if((threadIdx.x & 63) == 0){ so[threadIdx.x >> 6] = o; sa[threadIdx.x >> 6] = a; }
-    __syncthreads();
-    if(threadIdx.x == 0){
-        for(int w = 1; w < 4; w++){ o |= so[w]; a &= sa[w]; }
-        atomicOr(orv, o); atomicAnd(andv, a);
-    }
-}
-
 // Stable sort of (keys, vals) by key bits [lo, hi).  vals may be null (row ids 0..n-1).
 template<typename K>
 static int32_t radix_sort_t(hpa_ctx *c, K *keys, uint32_t *vals, L n, int lo, int hi, K **kout, uint32_t **vout){
@@ -106,30 +120,35 @@ static int32_t radix_sort_t(hpa_ctx *c, K *keys, uint32_t *vals, L n, int lo, in
     *kout = keys; *vout = vals;
     if(n <= 0) R 0;
     int32_t e;
-    unsigned long long *bits = (unsigned long long*)dalloc(c, 16);
-    if(!bits) R HPA_E_MEMORY;
-    unsigned long long init[2] = {0ull, ~0ull};
-    hipMemcpyAsync(bits, init, 16, hipMemcpyHostToDevice, c->stream);
-    hipLaunchKernelGGL(k_rs_bits<K>, dim3(reduce_grid(n, 256)), dim3(256), 0, c->stream, keys, n, bits, bits + 1);
-    unsigned long long hb[2]; if((e = d2h(c, hb, bits, 16))) R e;
-    unsigned long long vary = hb[0] ^ hb[1];
     L ntiles = ceil_div(n, RS_TILE);
+    unsigned long long *bits = (unsigned long long*)dalloc(c, 16 + 16 * ntiles);
     K *k2 = (K*)dalloc(c, sizeof(K) * n);
     uint32_t *v2 = (uint32_t*)dalloc(c, 4 * n);
     uint32_t *v1 = vals;
     if(!v1){ v1 = (uint32_t*)dalloc(c, 4 * n); if(!v1) R HPA_E_MEMORY; }
     uint32_t *hist = (uint32_t*)dalloc(c, 4 * 256 * ntiles);
     int64_t *off = (int64_t*)dalloc(c, 8 * 256 * ntiles);
-    if(!k2 || !v2 || !hist || !off) R HPA_E_MEMORY;
+    if(!bits || !k2 || !v2 || !hist || !off) R HPA_E_MEMORY;
     // ping-pong: va == nullptr means "row ids" (first pass without given values)
     K *ka = keys, *kb = k2; uint32_t *va = vals, *vb = v2;
     uint32_t *spare = vals ? vals : v1;
     bool any = false;
+    unsigned long long vary = ~0ull;
+    bool first = true;
     for(int s = lo; s < hi; s += 8){
-        if(((vary >> s) & 255ull) == 0) continue;
+        if(!first && ((vary >> s) & 255ull) == 0) continue;
         {
             ProfScope ps("radix_hist", c->stream, (double)n * sizeof(K) + 4.0 * 256 * ntiles);
-            hipLaunchKernelGGL(k_rs_hist<K>, dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream, ka, n, s, hist, ntiles);
+            hipLaunchKernelGGL(k_rs_hist<K>, dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream, ka, n, s, hist, ntiles,
+                               first ? bits + 2 : nullptr);
+        }
+        if(first){
+            // which bits vary, from the first histogram pass: constant digits are skipped
+            hipLaunchKernelGGL(k_rs_bits_final, dim3(1), dim3(256), 0, c->stream, bits + 2, ntiles, bits);
+            unsigned long long hb[2]; if((e = d2h(c, hb, bits, 16))) R e;
+            vary = hb[0] ^ hb[1];
+            first = false;
+            if(((vary >> s) & 255ull) == 0) continue;
         }
         if((e = scan_u32_to_i64(c, hist, off, 256 * ntiles, nullptr))) R e;
         {
