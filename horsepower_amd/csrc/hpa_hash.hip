@@ -69,6 +69,96 @@ static int32_t ht_create(hpa_ctx *c, L n, HTab *t){
     R check_launch("hash_clear");
 }
 
+__global__ void k_fill_i32(int *p, int n, int v){ for(int i = threadIdx.x; i < n; i += blockDim.x) p[i] = v; }
+// Packed table for 4-byte keys: one 8-byte slot per distinct key, (key << 32) | row, inserted
+// with a single CAS; a key met again keeps the smaller row (first occurrence, as index_of needs)
+// and raises `dup` (a join whose build keys repeat then takes its sort-based path, whose output
+// this path equals whenever the keys are unique: each probe row matches at most one build row).
+#define PK_EMPTY 0xffffffffffffffffull
+__global__ void k_pk_clear(unsigned long long *slots, L cap){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += (L)gridDim.x * blockDim.x) slots[i] = PK_EMPTY;
+}
+__global__ void k_pk_insert(const int32_t *x, L n, unsigned long long *slots, uint64_t mask, int *dup){
+    int d = 0;
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
+        const uint32_t k = (uint32_t)x[i];
+        const unsigned long long want = ((unsigned long long)k << 32) | (unsigned long long)(uint32_t)i;
+        uint64_t h = mix64(k) & mask;
+        while(true){
+            unsigned long long cur = slots[h];
+            if(cur == PK_EMPTY){
+                cur = atomicCAS(&slots[h], PK_EMPTY, want);
+                if(cur == PK_EMPTY) break;
+            }
+            if((uint32_t)(cur >> 32) == k){ d = 1; atomicMin(&slots[h], want); break; }
+            h = (h + 1) & mask;
+        }
+    }
+    if(__any(d) && (threadIdx.x & 63) == 0) atomicOr(dup, 1);
+}
+__global__ void k_pk_probe(const int32_t *y, L ny, const unsigned long long *slots, uint64_t mask, uint32_t *cnt, uint32_t *rid){
+    for(L j = (L)blockIdx.x * blockDim.x + threadIdx.x; j < ny; j += (L)gridDim.x * blockDim.x){
+        const uint32_t k = (uint32_t)y[j];
+        uint64_t h = mix64(k) & mask;
+        uint32_t c = 0, r = 0;
+        while(true){
+            const unsigned long long cur = slots[h];
+            if(cur == PK_EMPTY) break;
+            if((uint32_t)(cur >> 32) == k){ c = 1; r = (uint32_t)cur; break; }
+            h = (h + 1) & mask;
+        }
+        cnt[j] = c; rid[j] = r;
+    }
+}
+__global__ void k_pk_write(L ny, const uint32_t *cnt, const uint32_t *rid, const int64_t *off, int64_t *z0, int64_t *z1, int swap){
+    for(L j = (L)blockIdx.x * blockDim.x + threadIdx.x; j < ny; j += (L)gridDim.x * blockDim.x){
+        if(!cnt[j]) continue;
+        const int64_t o = off[j];
+        if(swap){ z0[o] = j; z1[o] = rid[j]; } else { z0[o] = rid[j]; z1[o] = j; }
+    }
+}
+
+__global__ void k_pk_probe_index(const int32_t *y, L ny, const unsigned long long *slots, uint64_t mask, int64_t *z, int64_t miss){
+    for(L j = (L)blockIdx.x * blockDim.x + threadIdx.x; j < ny; j += (L)gridDim.x * blockDim.x){
+        const uint32_t k = (uint32_t)y[j];
+        uint64_t h = mix64(k) & mask;
+        int64_t r = miss;
+        while(true){
+            const unsigned long long cur = slots[h];
+            if(cur == PK_EMPTY) break;
+            if((uint32_t)(cur >> 32) == k){ r = (int64_t)(uint32_t)cur; break; }
+            h = (h + 1) & mask;
+        }
+        z[j] = r;
+    }
+}
+__global__ void k_pk_probe_member(const int32_t *y, L ny, const unsigned long long *slots, uint64_t mask, uint8_t *z){
+    for(L j = (L)blockIdx.x * blockDim.x + threadIdx.x; j < ny; j += (L)gridDim.x * blockDim.x){
+        const uint32_t k = (uint32_t)y[j];
+        uint64_t h = mix64(k) & mask;
+        uint8_t r = 0;
+        while(true){
+            const unsigned long long cur = slots[h];
+            if(cur == PK_EMPTY) break;
+            if((uint32_t)(cur >> 32) == k){ r = 1; break; }
+            h = (h + 1) & mask;
+        }
+        z[j] = r;
+    }
+}
+// packed table over x (4-byte keys): clear + insert; returns the slots and mask
+static int32_t pk_build(hpa_ctx *c, const int32_t *x, L n, unsigned long long **slots, uint64_t *mask, int **dup){
+    L cap = 16; while(cap < 2 * n) cap <<= 1;
+    *slots = (unsigned long long*)dalloc(c, 8 * cap);
+    *dup = (int*)dalloc(c, 16);
+    if(!*slots || !*dup) R HPA_E_MEMORY;
+    *mask = (uint64_t)(cap - 1);
+    hipLaunchKernelGGL(k_pk_clear, dim3(grid_for(cap, 256, 4)), dim3(256), 0, c->stream, *slots, cap);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, *dup, 4, 0);
+    if(n > 0) hipLaunchKernelGGL(k_pk_insert, dim3(grid_for(n, 256, 4)), dim3(256), 0, c->stream, x, n, *slots, *mask, *dup);
+    R check_launch("hash_build");
+}
+
 // ---------------------------------------------------------------- index_of / member build+probe
 template<typename T>
 __global__ void k_ht_insert_min(const T *x, L n, HTab t){
@@ -140,7 +230,15 @@ int32_t is_ordered_dev(hpa_ctx *c, V x, bool *ord);
 
 template<typename T>
 static int32_t index_of_hash(hpa_ctx *c, const T *px, L nx, const T *py, L ny, int64_t *z){
-    HTab t; int32_t e;
+    int32_t e;
+    if(std::is_same<T, int32_t>::value && nx < (L)UINT32_MAX - 1){   // 4-byte keys: packed (key, first row) slots
+        unsigned long long *slots; uint64_t mask; int *dup;
+        if((e = pk_build(c, (const int32_t*)px, nx, &slots, &mask, &dup))) R e;
+        ProfScope ps("index_of", c->stream);
+        if(ny > 0) hipLaunchKernelGGL(k_pk_probe_index, dim3(grid_for(ny, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)py, ny, slots, mask, z, (int64_t)nx);
+        R check_launch("index_of");
+    }
+    HTab t;
     if((e = ht_create(c, nx, &t))) R e;
     ProfScope ps("index_of", c->stream);
     if(nx > 0) hipLaunchKernelGGL(k_ht_insert_min<T>, dim3(grid_for(nx, 256, 4)), dim3(256), 0, c->stream, px, nx, t);
@@ -232,7 +330,14 @@ static int32_t member_t(hpa_ctx *c, const T *py, L ny, const T *px, L nx, uint8_
         if(ny > 0) hipLaunchKernelGGL(k_member_small<T>, dim3(grid_for(ny, 256, 4)), dim3(256), 0, c->stream, py, ny, px, (int)nx, z);
         R check_launch("member");
     }
-    HTab t; int32_t e;
+    int32_t e;
+    if(std::is_same<T, int32_t>::value && nx < (L)UINT32_MAX - 1){
+        unsigned long long *slots; uint64_t mask; int *dup;
+        if((e = pk_build(c, (const int32_t*)px, nx, &slots, &mask, &dup))) R e;
+        if(ny > 0) hipLaunchKernelGGL(k_pk_probe_member, dim3(grid_for(ny, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)py, ny, slots, mask, z);
+        R check_launch("member");
+    }
+    HTab t;
     if((e = ht_create(c, nx, &t))) R e;
     hipLaunchKernelGGL(k_ht_insert_min<T>, dim3(grid_for(nx, 256, 4)), dim3(256), 0, c->stream, px, nx, t);
     if(ny > 0) hipLaunchKernelGGL(k_ht_probe_member<T>, dim3(grid_for(ny, 256, 4)), dim3(256), 0, c->stream, py, ny, t, z);
@@ -359,6 +464,27 @@ static int32_t join_run(hpa_ctx *c, V z, V bx, V py, int32_t t, int op, bool swa
     ProfScope ps("join", c->stream);
     if(nb >= 10){                                    // NUM_JOIN_LINEAR (common.h:152): hash path
         if(op != 4 || t != HPA_I) R HPA_E_NOT_IMPL;  // v3 radix hash join: i32 equality only (h_hash_v3.cc:233,270-277)
+        // unique build keys: packed table, no sort
+        L mark = c->dcur;
+        unsigned long long *slots; uint64_t mask; int *dup;
+        if((e = pk_build(c, (const int32_t*)pb, nb, &slots, &mask, &dup))) R e;
+        int hd = 0; if((e = d2h(c, &hd, dup, 4))) R e;
+        if(!hd){
+            if(np > 0) hipLaunchKernelGGL(k_pk_probe, dim3(grid_for(np, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)pp, np, slots, mask, cnt, rid);
+            if((e = scan_u32_to_i64(c, cnt, off, np, off + np))) R e;
+            int64_t P = 0; if((e = d2h(c, &P, off + np, 8))) R e;
+            int64_t *z0 = (int64_t*)dalloc(c, 8 * (P > 0 ? P : 1)), *z1 = (int64_t*)dalloc(c, 8 * (P > 0 ? P : 1));
+            if(!z0 || !z1) R HPA_E_MEMORY;
+            if(np > 0) hipLaunchKernelGGL(k_pk_write, dim3(grid_for(np, 256, 4)), dim3(256), 0, c->stream, np, cnt, rid, off, z0, z1, swap ? 1 : 0);
+            if((e = check_launch("join"))) R e;
+            V cells = (V)halloc(c, 2 * sizeof(V0)); if(!cells) R HPA_E_MEMORY;
+            z->typ = HPA_G; z->len = 2; z->g = (char*)cells; z->g2 = nullptr;
+            if((e = set_l_vec(c, cells, z0, P))) R e;
+            if((e = set_l_vec(c, cells + 1, z1, P))) R e;
+            note_bound(c, z0, swap ? np : nb); note_bound(c, z1, swap ? nb : np);
+            R 0;
+        }
+        c->dcur = mark;                              // duplicates: the sort-based build below
         uint32_t *k = (uint32_t*)dalloc(c, 4 * nb); if(!k) R HPA_E_MEMORY;
         hipMemcpyAsync(k, pb, 4 * nb, hipMemcpyDeviceToDevice, c->stream);
         uint32_t *sk;

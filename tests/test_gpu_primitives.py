@@ -249,6 +249,20 @@ def test_join(gpu, orc, nx, ny, card):
     assert np.array_equal(g[0], o[0]) and np.array_equal(g[1], o[1])
 
 
+@pytest.mark.parametrize("nb,ny,swap", [(10, 40, False), (5000, 20000, False), (20000, 200000, True), (100000, 800000, False)])
+def test_join_unique_build(gpu, orc, nb, ny, swap):
+    """Unique build keys (the packed-table path), including -1 and keys that never match; the
+    build side is x, or y after the swap."""
+    r = rng(nb + ny)
+    keys = (r.permutation(4 * nb) - 1).astype(np.int32)[:nb]
+    probe = r.integers(-2, 4 * nb, ny).astype(np.int32)
+    x, y = (probe, keys) if swap else (keys, probe)
+    g = gpu.read_list(gpu.pfnJoinIndex(gpu.vec(x), gpu.vec(y), gpu.sym("eq")))
+    o = orc.read_list(orc.pfnJoinIndex(orc.vec(x), orc.vec(y), orc.sym("eq")))
+    assert len(o[0]) > 0
+    assert np.array_equal(g[0], o[0]) and np.array_equal(g[1], o[1])
+
+
 @pytest.mark.parametrize("nx,ny", [(0, 5), (1, 10), (300, 500), (100000, 300000)])
 def test_index_of_member(gpu, orc, nx, ny):
     r = rng(nx + ny + 1)
