@@ -85,6 +85,28 @@ static int32_t jit_get(const std::string &src, const char *kname, hipFunction_t 
     R 0;
 }
 
+// Second-level cache: generated kernels found by a compact description of what they compute
+// (expression code with constants as argument slots, column types, shapes), so a repeated call
+// builds no source text at all.
+static std::map<std::string, hipFunction_t> g_jit_by_key;
+template<typename Gen>
+static int32_t jit_get_keyed(const std::string &key, const char *kname, Gen gen, hipFunction_t *fn){
+    const std::string k = key + "\n" + kname;
+    {
+        std::lock_guard<std::mutex> lk(g_jit_mu);
+        auto it = g_jit_by_key.find(k);
+        if(it != g_jit_by_key.end()){ *fn = it->second; R 0; }
+    }
+    int32_t e = jit_get(gen(), kname, fn);
+    if(!e && !g_compile_only){ std::lock_guard<std::mutex> lk(g_jit_mu); g_jit_by_key[k] = *fn; }
+    R e;
+}
+static std::string types_key(int nx, const int32_t *typ){
+    std::string k;
+    for(int i = 0; i < nx; i++){ k += std::to_string(typ[i]); k += ','; }
+    R k;
+}
+
 template<typename A>
 static int32_t jit_launch(hpa_ctx *c, hipFunction_t fn, unsigned grid, unsigned block, A &args){
     void *params[] = {&args};
@@ -307,6 +329,7 @@ int32_t jit_fused_reduce(hpa_ctx *c, int kind, int nx, V *x, const hpa_expr *con
         for(int k = 0; k < nx; k++) used[k] = uc[k] | ub[k];
         if(!cols_aligned(nx, typ, used, cols)) R HPA_E_NOT_IMPL;
     }
+    auto gen = [&](){
     std::ostringstream s;
     s << PREAMBLE;
     s << "struct A { const void *c[" << EXPR_MAX_COLS << "]; i64 n; void *out; void *aux; " KONST_FIELDS " };\n"
@@ -334,8 +357,11 @@ int32_t jit_fused_reduce(hpa_ctx *c, int kind, int nx, V *x, const hpa_expr *con
          "        ((P*)a.out)[blockIdx.x] = r;\n"
          "    }\n"
          "}\n";
+    return s.str();
+    };
     hipFunction_t fn;
-    if((e = jit_get(s.str(), "fa_reduce", &fn))) R e;
+    const std::string key = "fa|" + std::to_string(nx) + "|" + types_key(nx, typ) + "|" + ce.s + "|" + be.s;
+    if((e = jit_get_keyed(key, "fa_reduce", gen, &fn))) R e;
     JArgs a; memset(&a, 0, sizeof(a));
     for(int k = 0; k < nx; k++) a.c[k] = cols.p[k];
     memcpy(a.ki, kc.i, sizeof(a.ki)); memcpy(a.kf, kc.f, sizeof(a.kf));
@@ -362,6 +388,7 @@ int32_t jit_fused_reduce_many(hpa_ctx *c, int nq, int nx, V *x, const hpa_expr *
     split_used(nx, uc, ub, late);
     for(int k = 0; k < nx; k++) used[k] = uc[k] | ub[k];
     if(!cols_aligned(nx, typ, used, cols)) R HPA_E_NOT_IMPL;
+    auto gen = [&](){
     std::ostringstream s;
     s << PREAMBLE << "#define NQ " << nq << "\n"
       << "struct A { const void *c[" << EXPR_MAX_COLS << "]; i64 n; void *out; void *aux; " KONST_FIELDS " };\n"
@@ -386,8 +413,12 @@ int32_t jit_fused_reduce_many(hpa_ctx *c, int nq, int nx, V *x, const hpa_expr *
          "        r.mn = 1.7976931348623157e308; r.mx = -1.7976931348623157e308; r.imn = 9223372036854775807LL; r.imx = -9223372036854775807LL - 1;\n"
          "        ((P*)a.out)[(i64)q * gridDim.x + blockIdx.x] = r;\n"
          "    }\n}\n";
+    return s.str();
+    };
     hipFunction_t fn;
-    if((e = jit_get(s.str(), "fa_many", &fn))) R e;
+    std::string key = "fam|" + std::to_string(nx) + "|" + types_key(nx, typ) + "|" + be.s;
+    for(int q = 0; q < nq; q++) key += "|" + ce[q].s;
+    if((e = jit_get_keyed(key, "fa_many", gen, &fn))) R e;
     JArgs a; memset(&a, 0, sizeof(a));
     for(int k = 0; k < nx; k++) a.c[k] = cols.p[k];
     memcpy(a.ki, kc.i, sizeof(a.ki)); memcpy(a.kf, kc.f, sizeof(a.kf));
@@ -406,6 +437,7 @@ int32_t jit_fused_map(hpa_ctx *c, int otyp, int nx, V *x, const hpa_expr *ex, co
     Konst kc;
     if((e = expr_to_c(ex, nx, typ, &be, used, &kc))) R e;
     const char *ot = otyp == HPA_B ? "unsigned char" : otyp == HPA_I ? "int" : otyp == HPA_L ? "long long" : "double";
+    auto gen = [&](){
     std::ostringstream s;
     s << PREAMBLE
       << "struct A { const void *c[" << EXPR_MAX_COLS << "]; i64 n; void *out; void *aux; " KONST_FIELDS " };\n"
@@ -415,8 +447,11 @@ int32_t jit_fused_map(hpa_ctx *c, int otyp, int nx, V *x, const hpa_expr *ex, co
     if(otyp == HPA_B) s << "        ((unsigned char*)a.out)[i] = (unsigned char)((" << be.s << ") != 0);\n";
     else s << "        ((" << ot << "*)a.out)[i] = (" << ot << ")(" << be.s << ");\n";
     s << "    }\n}\n";
+    return s.str();
+    };
     hipFunction_t fn;
-    if((e = jit_get(s.str(), "fe_map", &fn))) R e;
+    const std::string key = "fe|" + std::to_string(otyp) + "|" + std::to_string(nx) + "|" + types_key(nx, typ) + "|" + be.s;
+    if((e = jit_get_keyed(key, "fe_map", gen, &fn))) R e;
     JArgs a; memset(&a, 0, sizeof(a));
     for(int k = 0; k < nx; k++) a.c[k] = cols.p[k];
     memcpy(a.ki, kc.i, sizeof(a.ki)); memcpy(a.kf, kc.f, sizeof(a.kf));
@@ -752,9 +787,12 @@ extern "C" int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *
     // that meets more distinct codes than slots reports overflow and the next size reruns
     int hv[2] = {0, 0};
     for(int ns = 4; ns <= FGA_NS; ns *= 2){
-        std::string src = fga_source(nk, ns, nx, typ, ce, used, acc);
+        std::string key = "fga|" + std::to_string(nk) + "|" + std::to_string(ns) + "|" + std::to_string(nx) + "|" + types_key(nx, typ) + "|" + ce.s;
+        for(int k = 0; k < nx; k++) key += used[k] ? '1' : '0';
+        for(auto &ac : acc) key += "|" + ac.body + "#" + std::to_string(ac.cls) + (ac.f ? "f" : "i");
+        auto gen = [&](){ return fga_source(nk, ns, nx, typ, ce, used, acc); };
         hipFunction_t f1, f2;
-        if((e = jit_get(src, "fga", &f1)) || (e = jit_get(src, "fga_merge", &f2))) R e;
+        if((e = jit_get_keyed(key, "fga", gen, &f1)) || (e = jit_get_keyed(key, "fga_merge", gen, &f2))) R e;
         hipMemsetAsync(ovf, 0, 4, c->stream);
         {
             ProfScope ps("fused_group_agg", c->stream, (double)n * cb);

@@ -403,13 +403,30 @@ extern "C" int32_t hpaReadPayload(hpa_V x, void *dst, int64_t bytes){
     hpa_ctx *c = ctx_cur();
     if(bytes <= 0) R 0;
     if(is_inline(x) || !is_device(x->g)){ memcpy(dst, x->g, (size_t)bytes); R 0; }
-    if(hipMemcpyAsync(dst, x->g, (size_t)bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess) R HPA_E_DEVICE;
-    R hipStreamSynchronize(c->stream) == hipSuccess ? 0 : HPA_E_DEVICE;
+    R d2h(c, dst, x->g, bytes);                 // small reads through the pinned staging buffer
 }
 
 // Copy the payloads of n vectors into caller memory with one stream synchronisation.
 extern "C" int32_t hpaReadMany(int32_t n, hpa_V *x, void **dst, const int64_t *bytes){
     hpa_ctx *c = ctx_cur();
+    // small results (the usual case: aggregates over groups) go through the pinned staging
+    // buffer, so the copies are truly asynchronous and one synchronisation covers them all
+    L dev_total = 0;
+    for(int k = 0; k < n; k++)
+        if(bytes[k] > 0 && !is_inline(x[k]) && is_device(x[k]->g)) dev_total += (bytes[k] + 15) & ~(L)15;
+    if(c->pinned && dev_total > 0 && dev_total <= c->pinned_bytes){
+        L off = 0;
+        std::vector<L> at(n, -1);
+        for(int k = 0; k < n; k++){
+            if(bytes[k] <= 0) continue;
+            if(is_inline(x[k]) || !is_device(x[k]->g)){ memcpy(dst[k], x[k]->g, (size_t)bytes[k]); continue; }
+            if(hipMemcpyAsync(c->pinned + off, x[k]->g, (size_t)bytes[k], hipMemcpyDeviceToHost, c->stream) != hipSuccess) R HPA_E_DEVICE;
+            at[k] = off; off += (bytes[k] + 15) & ~(L)15;
+        }
+        if(hipStreamSynchronize(c->stream) != hipSuccess) R HPA_E_DEVICE;
+        for(int k = 0; k < n; k++) if(at[k] >= 0) memcpy(dst[k], c->pinned + at[k], (size_t)bytes[k]);
+        R 0;
+    }
     bool any = false;
     for(int k = 0; k < n; k++){
         if(bytes[k] <= 0) continue;
