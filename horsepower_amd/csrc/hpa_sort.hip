@@ -67,15 +67,23 @@ __global__ void __launch_bounds__(256) k_rs_bits_final(const unsigned long long 
     }
 }
 
+// Scatter of one tile: every key's stable rank inside its wave's quarter of the tile comes from
+// eight ballots (lanes sharing the digit); the tile is then laid out digit by digit in LDS and
+// written out in that order, so consecutive threads write consecutive addresses of one digit's
+// run instead of 4096 scattered 4/8-byte stores.
 template<typename K>
 __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K *kin, const uint32_t *vin, K *kout, uint32_t *vout,
                                                          L n, int shift, const int64_t *off, L ntiles){
-    __shared__ uint32_t wcnt[4][256];
-    __shared__ uint32_t wbase[4][256];
+    __shared__ uint32_t wcnt[4][256];          // per-wave digit counts, then per-wave digit starts
+    __shared__ int64_t gbase[256];             // global position of a digit's run minus its tile-local start
+    __shared__ uint32_t wsum[RS_BLOCK / 64];
+    __shared__ K sk[RS_TILE];
+    __shared__ uint32_t sv[RS_TILE];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for(int q = 0; q < 4; q++) wcnt[q][threadIdx.x] = 0;
     __syncthreads();
-    const L tbase = (L)blockIdx.x * RS_TILE + (L)w * (RS_TILE / 4);
+    const L t0 = (L)blockIdx.x * RS_TILE;
+    const L tbase = t0 + (L)w * (RS_TILE / 4);
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     K kr[RS_ROWS]; uint32_t vr[RS_ROWS]; uint32_t rk[RS_ROWS];
     #pragma unroll
@@ -97,9 +105,19 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K *kin, const uin
     }
     __syncthreads();
     {
-        unsigned d = threadIdx.x;
-        int64_t s = off[(L)d * ntiles + blockIdx.x];
-        for(int q = 0; q < 4; q++){ wbase[q][d] = (uint32_t)s; s += wcnt[q][d]; }
+        // tile-local start of each digit (exclusive scan over the 256 digit totals), then the
+        // start of each wave's share inside its digit
+        const unsigned d = threadIdx.x;
+        const uint32_t c0 = wcnt[0][d], c1 = wcnt[1][d], c2 = wcnt[2][d], c3 = wcnt[3][d];
+        const uint32_t tot = c0 + c1 + c2 + c3;
+        uint32_t inc = tot;
+        for(int o = 1; o < 64; o <<= 1){ uint32_t t = __shfl_up(inc, o, 64); if(lane >= o) inc += t; }
+        if(lane == 63) wsum[w] = inc;
+        __syncthreads();
+        uint32_t pre = inc - tot;
+        for(int q = 0; q < w; q++) pre += wsum[q];
+        wcnt[0][d] = pre; wcnt[1][d] = pre + c0; wcnt[2][d] = pre + c0 + c1; wcnt[3][d] = pre + c0 + c1 + c2;
+        gbase[d] = off[(L)d * ntiles + blockIdx.x] - (int64_t)pre;
     }
     __syncthreads();
     #pragma unroll
@@ -107,9 +125,16 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K *kin, const uin
         L i = tbase + (L)r * 64 + lane;
         if(i < n){
             unsigned d = (unsigned)((kr[r] >> shift) & 255);
-            uint32_t pos = wbase[w][d] + rk[r];
-            kout[pos] = kr[r]; vout[pos] = vr[r];
+            uint32_t lp = wcnt[w][d] + rk[r];
+            sk[lp] = kr[r]; sv[lp] = vr[r];
         }
+    }
+    __syncthreads();
+    const int cnt = (int)(n - t0 < (L)RS_TILE ? n - t0 : (L)RS_TILE);
+    for(int j = threadIdx.x; j < cnt; j += RS_BLOCK){
+        const K k = sk[j];
+        const int64_t pos = gbase[(unsigned)((k >> shift) & 255)] + j;
+        kout[pos] = k; vout[pos] = sv[j];
     }
 }
 
@@ -373,6 +398,620 @@ static int32_t column_key(hpa_ctx *c, V col, const uint32_t *perm, int desc, uin
     R check_launch("order_key");
 }
 
+// ---------------------------------------------------------------- small-code group
+// Groups whose keys map onto at most 256 dense bins in key order are one stable counting sort
+// straight from the key columns, with no key buffer, radix passes or run detection:
+//   trie (two char columns): bin = rank(c0) * nb + rank(c1), ranks among the byte values present,
+//                            nb = number of distinct c1 values (ascending (c0<<8)+c1 order);
+//   i32: bin = (uint32)x - min, when max - min < 256 in the unsigned order the radix key uses.
+// Bins with no rows produce no group.  Passes: presence / range, per-tile bin counts (one read),
+// scan of the (bin, tile) matrix, scatter of row numbers staged in LDS by bin so the 8-byte
+// writes go out as runs, and a one-block pass for the group keys and counts.
+// On large inputs the bins come from a strided sample (one 16-row chunk in SG_SAMPLE_STRIDE);
+// the count pass checks every row against them and raises a flag on a row outside the bins,
+// the scatter then does nothing and the bins are rebuilt from the whole input.
+#define SG_TILE 4096
+#define SG_BINS 256
+#define SG_ABSENT 0x100u                // rank-table entry of a byte value outside the bins
+#define SG_MISS 0x1000u                 // bin of a row outside the bins
+#define SG_SAMPLE_MIN (1 << 20)
+#define SG_SAMPLE_STRIDE 64
+struct SgSrc {
+    const uint8_t *a, *b;        // trie columns (a null: i32 source)
+    const int32_t *x;
+    uint32_t base;               // i32: smallest key in the unsigned order
+    uint32_t nb;                 // trie: distinct c1 values
+    uint32_t nbins;
+    int vec;                     // 16-byte aligned sources: vector loads
+    uint16_t ra[256], rb[256];   // trie: dense rank of each c0 / c1 byte value, or SG_ABSENT
+};
+__device__ __forceinline__ unsigned sg_trie_bin(const uint16_t *ra, const uint16_t *rb, unsigned nb, unsigned va, unsigned vb){
+    const unsigned r1 = ra[va], r2 = rb[vb];
+    R ((r1 | r2) & SG_ABSENT) ? SG_MISS : r1 * nb + r2;
+}
+__device__ __forceinline__ unsigned sg_i32_bin(const SgSrc &s, unsigned x){
+    const unsigned d = x - s.base;
+    R d < s.nbins ? d : SG_MISS;
+}
+// The raw key bytes of one thread's 16 consecutive rows of a tile travel in four uint4
+// registers q0..q3 (trie: q0 = c0 bytes, q1 = c1 bytes; i32: the 16 keys).
+__device__ __forceinline__ bool sg_full(const SgSrc &s, L t0, L n){ R s.vec && t0 + (L)(threadIdx.x + 1) * 16 <= n; }
+__device__ __forceinline__ void sg_fetch(const SgSrc &s, L t0, L n, uint4 &q0, uint4 &q1, uint4 &q2, uint4 &q3){
+    const L r0 = t0 + (L)threadIdx.x * 16;
+    if(!sg_full(s, t0, n)) R;
+    if(s.a){ q0 = *(const uint4*)(s.a + r0); q1 = *(const uint4*)(s.b + r0); }
+    else {
+        q0 = *(const uint4*)(s.x + r0); q1 = *(const uint4*)(s.x + r0 + 4);
+        q2 = *(const uint4*)(s.x + r0 + 8); q3 = *(const uint4*)(s.x + r0 + 12);
+    }
+}
+__device__ __forceinline__ unsigned sg_word(const uint4 &q0, const uint4 &q1, const uint4 &q2, const uint4 &q3, int j){
+    const uint4 v = (j >> 2) == 0 ? q0 : (j >> 2) == 1 ? q1 : (j >> 2) == 2 ? q2 : q3;      // j: a constant after unrolling
+    R (j & 3) == 0 ? v.x : (j & 3) == 1 ? v.y : (j & 3) == 2 ? v.z : v.w;
+}
+__device__ __forceinline__ unsigned sg_byte(const uint4 &v, int j){
+    const unsigned w = j < 4 ? v.x : j < 8 ? v.y : j < 12 ? v.z : v.w;
+    R (w >> (8 * (j & 3))) & 255u;
+}
+// dense bins of the thread's 16 rows into sd[16 * threadIdx.x ...] (rows past n and rows outside
+// the bins: bin 0; the latter set *miss)
+__device__ __forceinline__ void sg_bins(const SgSrc &s, const uint16_t *ra, const uint16_t *rb, L t0, L n,
+                                        const uint4 &q0, const uint4 &q1, const uint4 &q2, const uint4 &q3, uint8_t *sd, bool *miss){
+    const L r0 = t0 + (L)threadIdx.x * 16;
+    unsigned pk[4] = {0, 0, 0, 0};
+    const bool full = sg_full(s, t0, n);
+    #pragma unroll
+    for(int j = 0; j < 16; j++){
+        unsigned v;
+        if(s.a) v = full ? sg_trie_bin(ra, rb, s.nb, sg_byte(q0, j), sg_byte(q1, j))
+                         : (r0 + j < n ? sg_trie_bin(ra, rb, s.nb, s.a[r0 + j], s.b[r0 + j]) : 0u);
+        else v = full ? sg_i32_bin(s, sg_word(q0, q1, q2, q3, j)) : (r0 + j < n ? sg_i32_bin(s, (uint32_t)s.x[r0 + j]) : 0u);
+        if(v == SG_MISS){ *miss = true; v = 0; }
+        pk[j >> 2] |= (v & 255u) << (8 * (j & 3));
+    }
+    *(uint4*)(sd + threadIdx.x * 16) = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+}
+// lanes of this wave holding the same bin (kb ballots), among valid lanes
+__device__ __forceinline__ uint64_t sg_peers(unsigned d, bool valid, int kb){
+    uint64_t peers = __ballot(valid);
+    for(int b = 0; b < kb; b++){
+        uint64_t bb = __ballot((d >> b) & 1);
+        peers &= ((d >> b) & 1) ? bb : ~bb;
+    }
+    return peers;
+}
+// 9..256 bins: one 4096-row tile per workgroup; a row's stable rank inside its wave's quarter of
+// the tile comes from kb ballots (lanes sharing the bin)
+__global__ void __launch_bounds__(256) k_sg_hist(SgSrc s, L n, int nbins, int kb, uint32_t *cnt, L ntiles, int *miss){
+    __shared__ __align__(16) uint8_t sd[SG_TILE];
+    __shared__ uint32_t wc[4][SG_BINS];
+    __shared__ uint16_t ra[256], rb[256];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const L t0 = (L)blockIdx.x * SG_TILE;
+    uint4 q0 = make_uint4(0, 0, 0, 0), q1 = q0, q2 = q0, q3 = q0;
+    sg_fetch(s, t0, n, q0, q1, q2, q3);
+    ra[threadIdx.x] = s.ra[threadIdx.x]; rb[threadIdx.x] = s.rb[threadIdx.x];
+    for(int q = 0; q < 4; q++) wc[q][threadIdx.x] = 0;
+    __syncthreads();
+    bool m = false;
+    sg_bins(s, ra, rb, t0, n, q0, q1, q2, q3, sd, &m);
+    if(__syncthreads_or(m)){ if(threadIdx.x == 0) atomicOr(miss, 1); R; }
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    for(int r = 0; r < SG_TILE / 256; r++){
+        const int j = w * (SG_TILE / 4) + r * 64 + lane;
+        const bool valid = t0 + j < n;
+        const unsigned d = sd[j];
+        const uint64_t peers = sg_peers(d, valid, kb);
+        if(valid && (peers & lt) == 0) wc[w][d] += (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    for(int d = threadIdx.x; d < nbins; d += 256) cnt[(L)d * ntiles + blockIdx.x] = wc[0][d] + wc[1][d] + wc[2][d] + wc[3][d];
+}
+__global__ void __launch_bounds__(256) k_sg_scatter(SgSrc s, L n, int nbins, int kb, const int64_t *off, L ntiles, int64_t *flat,
+                                                    const int *miss){
+    __shared__ __align__(16) uint8_t sd[SG_TILE];
+    __shared__ uint16_t srow[SG_TILE];
+    __shared__ uint32_t wc[4][SG_BINS];
+    __shared__ int64_t gbase[SG_BINS];
+    __shared__ uint32_t wsum[4];
+    __shared__ uint16_t ra[256], rb[256];
+    if(*miss) R;                                    // bins from a sample missed a row: nothing to place
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const L t0 = (L)blockIdx.x * SG_TILE;
+    uint4 q0 = make_uint4(0, 0, 0, 0), q1 = q0, q2 = q0, q3 = q0;
+    sg_fetch(s, t0, n, q0, q1, q2, q3);
+    ra[threadIdx.x] = s.ra[threadIdx.x]; rb[threadIdx.x] = s.rb[threadIdx.x];
+    for(int q = 0; q < 4; q++) wc[q][threadIdx.x] = 0;
+    __syncthreads();
+    bool m = false;
+    sg_bins(s, ra, rb, t0, n, q0, q1, q2, q3, sd, &m);
+    __syncthreads();
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint16_t rk[SG_TILE / 256];
+    #pragma unroll
+    for(int r = 0; r < SG_TILE / 256; r++){
+        const int j = w * (SG_TILE / 4) + r * 64 + lane;
+        const bool valid = t0 + j < n;
+        const unsigned d = sd[j];
+        const uint64_t peers = sg_peers(d, valid, kb);
+        const uint32_t before = (uint32_t)__popcll(peers & lt);
+        const uint32_t cur = wc[w][d];
+        if(valid && before == 0) wc[w][d] = cur + (uint32_t)__popcll(peers);
+        rk[r] = (uint16_t)(cur + before);
+    }
+    __syncthreads();
+    {
+        const unsigned d = threadIdx.x;
+        const uint32_t c0 = wc[0][d], c1 = wc[1][d], c2 = wc[2][d], c3 = wc[3][d];
+        const uint32_t tot = c0 + c1 + c2 + c3;
+        uint32_t inc = tot;
+        for(int o = 1; o < 64; o <<= 1){ uint32_t u = __shfl_up(inc, o, 64); if(lane >= o) inc += u; }
+        if(lane == 63) wsum[w] = inc;
+        __syncthreads();
+        uint32_t pre = inc - tot;
+        for(int q = 0; q < w; q++) pre += wsum[q];
+        wc[0][d] = pre; wc[1][d] = pre + c0; wc[2][d] = pre + c0 + c1; wc[3][d] = pre + c0 + c1 + c2;
+        if((int)d < nbins) gbase[d] = off[(L)d * ntiles + blockIdx.x] - (int64_t)pre;
+    }
+    __syncthreads();
+    #pragma unroll
+    for(int r = 0; r < SG_TILE / 256; r++){
+        const int j = w * (SG_TILE / 4) + r * 64 + lane;
+        if(t0 + j < n) srow[wc[w][sd[j]] + rk[r]] = (uint16_t)j;
+    }
+    __syncthreads();
+    const int cnt = (int)(n - t0 < (L)SG_TILE ? n - t0 : (L)SG_TILE);
+    for(int j = threadIdx.x; j < cnt; j += 256){
+        const int tr = srow[j];
+        flat[gbase[sd[tr]] + j] = t0 + tr;
+    }
+}
+
+// At most 8 bins: a 16384-row tile is four groups of 4096 rows; in group g thread t owns the 16
+// consecutive rows g*4096 + 16t ... (each load instruction reads 1 KiB contiguous per wave).  A
+// thread counts the bins of each chunk in the 8-bit fields of one 64-bit word (bin d in bits
+// 8d..8d+7; a chunk has 16 rows).  For the workgroup scans the word is split into two words of
+// 16-bit fields (even bins, odd bins), wide enough for a whole tile.  Adding each bin's tile start
+// to the scanned prefix gives, per chunk, the tile-local position of the chunk's first row of
+// every bin; a row's stable position is that plus the chunk's rows of its bin already placed.
+#define SG8_G 4
+#define SG8_TILE (SG8_G * 4096)
+// 8-bit fields of bins 0..7 -> 16-bit fields (field i = bin 2i + odd)
+__device__ __forceinline__ unsigned long long sg8_half(unsigned long long c8, int odd){
+    R (c8 >> (8 * odd)) & 0x00FF00FF00FF00FFull;
+}
+__device__ __forceinline__ unsigned sg8_f16(unsigned long long w, unsigned i){ R (unsigned)(w >> (16 * i)) & 0xFFFFu; }
+// bins of the thread's SG8_G chunks (rows past n: 0xFF; rows outside the bins: 0xFF and *miss),
+// counted into c8[g] and, with STORE, kept byte-packed in pk[g] (row j of chunk g in byte j%4 of
+// pk[g][j/4]).  All loads are issued before the first is used.
+template<bool STORE>
+__device__ __forceinline__ void sg8_bins(const SgSrc &s, const uint16_t *ra, const uint16_t *rb, L t0, L n,
+                                         unsigned (*pk)[4], unsigned long long *c8, bool *miss){
+    #pragma unroll
+    for(int g = 0; g < SG8_G; g++) c8[g] = 0;
+    auto put = [&](int g, int j, unsigned v){          // g, j: constants after unrolling
+        if(v == SG_MISS){ *miss = true; v = 0xFFu; }
+        if(v != 0xFFu) c8[g] += 1ull << (8 * v);
+        if(STORE){ if((j & 3) == 0) pk[g][j >> 2] = v & 255u; else pk[g][j >> 2] |= (v & 255u) << (8 * (j & 3)); }
+    };
+    const L rt = t0 + (L)threadIdx.x * 16;
+    if(s.a){
+        uint4 qa[SG8_G], qb[SG8_G];
+        #pragma unroll
+        for(int g = 0; g < SG8_G; g++){
+            const L r = rt + g * 4096;
+            if(s.vec && r + 16 <= n){ qa[g] = *(const uint4*)(s.a + r); qb[g] = *(const uint4*)(s.b + r); }
+        }
+        #pragma unroll
+        for(int g = 0; g < SG8_G; g++){
+            const L r = rt + g * 4096;
+            if(s.vec && r + 16 <= n){
+                #pragma unroll
+                for(int j = 0; j < 16; j++) put(g, j, sg_trie_bin(ra, rb, s.nb, sg_byte(qa[g], j), sg_byte(qb[g], j)));
+            } else {
+                #pragma unroll
+                for(int j = 0; j < 16; j++) put(g, j, r + j < n ? sg_trie_bin(ra, rb, s.nb, s.a[r + j], s.b[r + j]) : 0xFFu);
+            }
+        }
+    } else {
+        uint4 qx[SG8_G][4];
+        #pragma unroll
+        for(int g = 0; g < SG8_G; g++){
+            const L r = rt + g * 4096;
+            if(s.vec && r + 16 <= n){
+                #pragma unroll
+                for(int k = 0; k < 4; k++) qx[g][k] = *(const uint4*)(s.x + r + 4 * k);
+            }
+        }
+        #pragma unroll
+        for(int g = 0; g < SG8_G; g++){
+            const L r = rt + g * 4096;
+            if(s.vec && r + 16 <= n){
+                #pragma unroll
+                for(int j = 0; j < 16; j++){
+                    const uint4 &q = qx[g][j >> 2];
+                    put(g, j, sg_i32_bin(s, (j & 3) == 0 ? q.x : (j & 3) == 1 ? q.y : (j & 3) == 2 ? q.z : q.w));
+                }
+            } else {
+                #pragma unroll
+                for(int j = 0; j < 16; j++) put(g, j, r + j < n ? sg_i32_bin(s, (uint32_t)s.x[r + j]) : 0xFFu);
+            }
+        }
+    }
+}
+__global__ void __launch_bounds__(256) k_sg8_hist(SgSrc s, L n, int nbins, uint32_t *cnt, L ntiles, int *miss){
+    __shared__ unsigned long long ws[4][2];
+    __shared__ uint16_t ra[256], rb[256];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    ra[threadIdx.x] = s.ra[threadIdx.x]; rb[threadIdx.x] = s.rb[threadIdx.x];
+    __syncthreads();
+    unsigned long long c8[SG8_G];
+    bool m = false;
+    sg8_bins<false>(s, ra, rb, (L)blockIdx.x * SG8_TILE, n, nullptr, c8, &m);
+    if(__syncthreads_or(m)){ if(threadIdx.x == 0) atomicOr(miss, 1); R; }
+    unsigned long long e = 0, o = 0;
+    #pragma unroll
+    for(int g = 0; g < SG8_G; g++){ e += sg8_half(c8[g], 0); o += sg8_half(c8[g], 1); }
+    for(int k = 32; k > 0; k >>= 1){ e += __shfl_xor(e, k, 64); o += __shfl_xor(o, k, 64); }
+    if(lane == 0){ ws[w][0] = e; ws[w][1] = o; }
+    __syncthreads();
+    if((int)threadIdx.x < nbins){
+        const unsigned d = threadIdx.x;
+        unsigned t = 0;
+        for(int q = 0; q < 4; q++) t += sg8_f16(ws[q][d & 1], d >> 1);
+        cnt[(L)d * ntiles + blockIdx.x] = t;
+    }
+}
+__global__ void __launch_bounds__(256) k_sg8_scatter(SgSrc s, L n, int nbins, const int64_t *off, L ntiles, int64_t *flat,
+                                                     const int *miss){
+    __shared__ uint16_t srow[SG8_TILE];
+    __shared__ unsigned long long ws[SG8_G][4][2];
+    __shared__ unsigned tst[9];
+    __shared__ int64_t gbase[8];
+    __shared__ uint16_t ra[256], rb[256];
+    if(*miss) R;                                    // bins from a sample missed a row: nothing to place
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    ra[threadIdx.x] = s.ra[threadIdx.x]; rb[threadIdx.x] = s.rb[threadIdx.x];
+    __syncthreads();
+    const L t0 = (L)blockIdx.x * SG8_TILE;
+    unsigned long long c8[SG8_G];
+    unsigned pk[SG8_G][4];
+    bool m = false;
+    sg8_bins<true>(s, ra, rb, t0, n, pk, c8, &m);
+    // per group: exclusive scans of the even-bin and odd-bin words over the workgroup's threads
+    unsigned long long P[SG8_G][2];
+    #pragma unroll
+    for(int g = 0; g < SG8_G; g++){
+        #pragma unroll
+        for(int h = 0; h < 2; h++){
+            const unsigned long long v = sg8_half(c8[g], h);
+            unsigned long long inc = v;
+            for(int k = 1; k < 64; k <<= 1){ const unsigned long long u = __shfl_up(inc, k, 64); if(lane >= k) inc += u; }
+            if(lane == 63) ws[g][w][h] = inc;
+            P[g][h] = inc - v;
+        }
+    }
+    __syncthreads();
+    // rows of each bin before the thread's chunk: earlier groups, earlier waves; then each bin's
+    // tile start (exclusive prefix over bins of the tile totals), all in 16-bit fields
+    unsigned long long tot[2] = {0, 0};
+    #pragma unroll
+    for(int g = 0; g < SG8_G; g++){
+        #pragma unroll
+        for(int h = 0; h < 2; h++){
+            unsigned long long before = 0, gt = 0;
+            #pragma unroll
+            for(int q = 0; q < 4; q++){ const unsigned long long v = ws[g][q][h]; before += q < w ? v : 0ull; gt += v; }
+            P[g][h] += tot[h] + before;
+            tot[h] += gt;
+        }
+    }
+    unsigned long long start[2] = {0, 0};
+    unsigned run = 0;
+    #pragma unroll
+    for(int d = 0; d < 8; d++){
+        start[d & 1] |= (unsigned long long)run << (16 * (d >> 1));
+        run += sg8_f16(tot[d & 1], d >> 1);
+    }
+    if((int)threadIdx.x <= nbins){
+        const unsigned d = threadIdx.x;
+        tst[d] = d < (unsigned)nbins ? sg8_f16(start[d & 1], d >> 1) : 0x7FFFFFFFu;
+        if(d < (unsigned)nbins) gbase[d] = off[(L)d * ntiles + blockIdx.x] - (int64_t)sg8_f16(start[d & 1], d >> 1);
+    }
+    // stable tile-local position of each of the thread's rows
+    #pragma unroll
+    for(int g = 0; g < SG8_G; g++){
+        const unsigned long long pe = P[g][0] + start[0], po = P[g][1] + start[1];
+        const int li = g * 4096 + threadIdx.x * 16;
+        const uint4 b4 = make_uint4(pk[g][0], pk[g][1], pk[g][2], pk[g][3]);
+        unsigned long long runc = 0;
+        #pragma unroll
+        for(int j = 0; j < 16; j++){
+            const unsigned d = sg_byte(b4, j);
+            if(d != 0xFFu){
+                const unsigned p = sg8_f16(d & 1 ? po : pe, d >> 1) + ((unsigned)(runc >> (8 * d)) & 255u);
+                srow[p] = (uint16_t)(li + j);
+                runc += 1ull << (8 * d);
+            }
+        }
+    }
+    __syncthreads();
+    // positions [tst[d], tst[d+1]) hold bin d: each thread follows its bin as j grows
+    const int cnt = (int)(n - t0 < (L)SG8_TILE ? n - t0 : (L)SG8_TILE);
+    int d = 0;
+    #pragma unroll 4
+    for(int j = threadIdx.x; j < cnt; j += 256){
+        while(j >= (int)tst[d + 1]) d++;
+        flat[gbase[d] + j] = t0 + srow[j];
+    }
+}
+// group keys (first row of each non-empty bin) and counts; g = number of groups
+__global__ void __launch_bounds__(256) k_sg_final(const int64_t *off, L ntiles, int nbins, L n, const int64_t *flat,
+                                                  int64_t *keys, int64_t *counts, int64_t *g, const int *miss){
+    __shared__ uint32_t wsum[4];
+    if(*miss){ if(threadIdx.x == 0) *g = -1; R; }
+    const int d = threadIdx.x, lane = d & 63, w = d >> 6;
+    int64_t st = 0, c = 0;
+    if(d < nbins){ st = off[(L)d * ntiles]; c = (d + 1 < nbins ? off[(L)(d + 1) * ntiles] : n) - st; }
+    const uint32_t nz = c > 0 ? 1u : 0u;
+    uint32_t inc = nz;
+    for(int o = 1; o < 64; o <<= 1){ uint32_t t = __shfl_up(inc, o, 64); if(lane >= o) inc += t; }
+    if(lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t pre = inc - nz;
+    for(int q = 0; q < w; q++) pre += wsum[q];
+    if(nz){ keys[pre] = flat[st]; counts[pre] = c; }
+    if(d == 255) *g = (int64_t)(pre + nz);
+}
+
+// Presence of byte values in two char columns (4 x 64-bit masks per column), and the unsigned
+// range of an i32 column: per-workgroup partials, then one workgroup folds them.  Each thread
+// has several 16-byte loads in flight; every load instruction reads one contiguous 1 KiB per wave.
+#define SG_SCAN_UNROLL 4
+// Presence: every row sets its two bytes' flags in LDS (plain stores of 1; no read-modify-write),
+// then each wave turns 64 flags into one mask word with a ballot.
+#define SG_PRESENT_UNROLL 4
+__global__ void __launch_bounds__(256) k_sg_present(const uint8_t *a, const uint8_t *b, L n, int vec, unsigned long long *part){
+    __shared__ uint8_t fa[256], fb[256];
+    fa[threadIdx.x] = 0; fb[threadIdx.x] = 0;
+    __syncthreads();
+    const L S = (L)gridDim.x * 256 * 16;
+    L i = ((L)blockIdx.x * 256 + threadIdx.x) * 16;
+    if(vec){
+        for(; i + (SG_PRESENT_UNROLL - 1) * S + 16 <= n; i += SG_PRESENT_UNROLL * S){
+            uint4 wa[SG_PRESENT_UNROLL], wb[SG_PRESENT_UNROLL];
+            #pragma unroll
+            for(int u = 0; u < SG_PRESENT_UNROLL; u++){ wa[u] = *(const uint4*)(a + i + u * S); wb[u] = *(const uint4*)(b + i + u * S); }
+            #pragma unroll
+            for(int u = 0; u < SG_PRESENT_UNROLL; u++){
+                #pragma unroll
+                for(int j = 0; j < 16; j++){ fa[sg_byte(wa[u], j)] = 1; fb[sg_byte(wb[u], j)] = 1; }
+            }
+        }
+    }
+    for(; i < n; i += S){
+        if(vec && i + 16 <= n){
+            const uint4 wa = *(const uint4*)(a + i), wb = *(const uint4*)(b + i);
+            #pragma unroll
+            for(int j = 0; j < 16; j++){ fa[sg_byte(wa, j)] = 1; fb[sg_byte(wb, j)] = 1; }
+        } else
+            for(int j = 0; j < 16 && i + j < n; j++){ fa[a[i + j]] = 1; fb[b[i + j]] = 1; }
+    }
+    __syncthreads();
+    const unsigned long long ma = __ballot(fa[threadIdx.x] != 0), mb = __ballot(fb[threadIdx.x] != 0);
+    if((threadIdx.x & 63) == 0){
+        part[(L)blockIdx.x * 8 + (threadIdx.x >> 6)] = ma;
+        part[(L)blockIdx.x * 8 + 4 + (threadIdx.x >> 6)] = mb;
+    }
+}
+__global__ void __launch_bounds__(256) k_sg_present_final(const unsigned long long *part, int nblocks, unsigned long long *out){
+    __shared__ unsigned long long sm[4][8];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    #pragma unroll
+    for(int q = 0; q < 8; q++){
+        unsigned long long v = 0;
+        for(int k = threadIdx.x; k < nblocks; k += 256) v |= part[(L)k * 8 + q];
+        for(int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+        if(lane == 0) sm[w][q] = v;
+    }
+    __syncthreads();
+    if(threadIdx.x < 8) out[threadIdx.x] = sm[0][threadIdx.x] | sm[1][threadIdx.x] | sm[2][threadIdx.x] | sm[3][threadIdx.x];
+}
+// The same over a strided sample: thread k looks at rows [16 * stride * k, +16)
+__global__ void __launch_bounds__(256) k_sg_present_sample(const uint8_t *a, const uint8_t *b, L n, L stride, int vec,
+                                                           unsigned long long *part){
+    __shared__ uint8_t fa[256], fb[256];
+    fa[threadIdx.x] = 0; fb[threadIdx.x] = 0;
+    __syncthreads();
+    const L i = ((L)blockIdx.x * 256 + threadIdx.x) * stride * 16;
+    if(i < n){
+        if(vec && i + 16 <= n){
+            const uint4 wa = *(const uint4*)(a + i), wb = *(const uint4*)(b + i);
+            #pragma unroll
+            for(int j = 0; j < 16; j++){ fa[sg_byte(wa, j)] = 1; fb[sg_byte(wb, j)] = 1; }
+        } else
+            for(int j = 0; j < 16 && i + j < n; j++){ fa[a[i + j]] = 1; fb[b[i + j]] = 1; }
+    }
+    __syncthreads();
+    const unsigned long long ma = __ballot(fa[threadIdx.x] != 0), mb = __ballot(fb[threadIdx.x] != 0);
+    if((threadIdx.x & 63) == 0){
+        part[(L)blockIdx.x * 8 + (threadIdx.x >> 6)] = ma;
+        part[(L)blockIdx.x * 8 + 4 + (threadIdx.x >> 6)] = mb;
+    }
+}
+__global__ void __launch_bounds__(256) k_sg_urange_sample(const int32_t *x, L n, L stride, unsigned *part){
+    __shared__ unsigned slo[4], shi[4];
+    unsigned lo = 0xFFFFFFFFu, hi = 0;
+    const L i = ((L)blockIdx.x * 256 + threadIdx.x) * stride * 16;
+    for(int j = 0; j < 16 && i + j < n; j++){ const unsigned v = (unsigned)x[i + j]; lo = v < lo ? v : lo; hi = v > hi ? v : hi; }
+    for(int o = 32; o > 0; o >>= 1){
+        unsigned t = __shfl_xor(lo, o, 64); lo = t < lo ? t : lo;
+        t = __shfl_xor(hi, o, 64); hi = t > hi ? t : hi;
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if(lane == 0){ slo[w] = lo; shi[w] = hi; }
+    __syncthreads();
+    if(threadIdx.x == 0){
+        for(int q = 1; q < 4; q++){ lo = slo[q] < lo ? slo[q] : lo; hi = shi[q] > hi ? shi[q] : hi; }
+        part[2 * blockIdx.x] = lo; part[2 * blockIdx.x + 1] = hi;
+    }
+}
+__global__ void __launch_bounds__(256) k_sg_urange(const int32_t *x, L n, int vec, unsigned *part){
+    __shared__ unsigned slo[4], shi[4];
+    unsigned lo = 0xFFFFFFFFu, hi = 0;
+    const L S = (L)gridDim.x * 256 * 4;
+    L i = ((L)blockIdx.x * 256 + threadIdx.x) * 4;
+    if(vec){
+        for(; i + (SG_SCAN_UNROLL - 1) * S + 4 <= n; i += SG_SCAN_UNROLL * S){
+            uint4 wv[SG_SCAN_UNROLL];
+            #pragma unroll
+            for(int u = 0; u < SG_SCAN_UNROLL; u++) wv[u] = *(const uint4*)(x + i + u * S);
+            #pragma unroll
+            for(int u = 0; u < SG_SCAN_UNROLL; u++){
+                const unsigned a0 = wv[u].x < wv[u].y ? wv[u].x : wv[u].y, a1 = wv[u].z < wv[u].w ? wv[u].z : wv[u].w;
+                const unsigned b0 = wv[u].x > wv[u].y ? wv[u].x : wv[u].y, b1 = wv[u].z > wv[u].w ? wv[u].z : wv[u].w;
+                const unsigned mn = a0 < a1 ? a0 : a1, mx = b0 > b1 ? b0 : b1;
+                lo = mn < lo ? mn : lo; hi = mx > hi ? mx : hi;
+            }
+        }
+    }
+    for(; i < n; i += S){
+        if(vec && i + 4 <= n){
+            const uint4 w4 = *(const uint4*)(x + i);
+            const unsigned a0 = min(w4.x, w4.y), a1 = min(w4.z, w4.w), b0 = max(w4.x, w4.y), b1 = max(w4.z, w4.w);
+            lo = min(lo, min(a0, a1)); hi = max(hi, max(b0, b1));
+        } else
+            for(int j = 0; j < 4 && i + j < n; j++){ const unsigned v = (unsigned)x[i + j]; lo = v < lo ? v : lo; hi = v > hi ? v : hi; }
+    }
+    for(int o = 32; o > 0; o >>= 1){
+        unsigned t = __shfl_xor(lo, o, 64); lo = t < lo ? t : lo;
+        t = __shfl_xor(hi, o, 64); hi = t > hi ? t : hi;
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if(lane == 0){ slo[w] = lo; shi[w] = hi; }
+    __syncthreads();
+    if(threadIdx.x == 0){
+        for(int q = 1; q < 4; q++){ lo = slo[q] < lo ? slo[q] : lo; hi = shi[q] > hi ? shi[q] : hi; }
+        part[2 * blockIdx.x] = lo; part[2 * blockIdx.x + 1] = hi;
+    }
+}
+__global__ void __launch_bounds__(256) k_sg_urange_final(const unsigned *part, int nblocks, unsigned *out){
+    __shared__ unsigned slo[4], shi[4];
+    unsigned lo = 0xFFFFFFFFu, hi = 0;
+    for(int k = threadIdx.x; k < nblocks; k += 256){ lo = part[2 * k] < lo ? part[2 * k] : lo; hi = part[2 * k + 1] > hi ? part[2 * k + 1] : hi; }
+    for(int o = 32; o > 0; o >>= 1){
+        unsigned t = __shfl_xor(lo, o, 64); lo = t < lo ? t : lo;
+        t = __shfl_xor(hi, o, 64); hi = t > hi ? t : hi;
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if(lane == 0){ slo[w] = lo; shi[w] = hi; }
+    __syncthreads();
+    if(threadIdx.x == 0){
+        for(int q = 1; q < 4; q++){ lo = slo[q] < lo ? slo[q] : lo; hi = shi[q] > hi ? shi[q] : hi; }
+        out[0] = lo; out[1] = hi;
+    }
+}
+// workgroups of a whole-column pass that keeps SG_SCAN_UNROLL loads per thread in flight
+static int sg_scan_grid(L n, int rows_per_load){
+    L g = ceil_div(n, (L)256 * rows_per_load * SG_SCAN_UNROLL), cap = (L)num_cus() * 8;
+    R (int)(g < 1 ? 1 : (g > cap ? cap : g));
+}
+
+// *missed: a row fell outside bins taken from a sample (z untouched)
+static int32_t small_group_run(hpa_ctx *c, V z, SgSrc &s, L n, int nbins, bool *missed){
+    int32_t e;
+    *missed = false;
+    s.nbins = (uint32_t)nbins;
+    int kb = 0; while((1 << kb) < nbins) kb++;
+    const bool few = nbins <= 8;                   // packed-count kernels
+    const L ntiles = ceil_div(n, few ? SG8_TILE : SG_TILE);
+    uint32_t *cnt = (uint32_t*)dalloc(c, 4 * nbins * ntiles);
+    int64_t *off = (int64_t*)dalloc(c, 8 * nbins * ntiles);
+    int64_t *flat = (int64_t*)dalloc(c, 8 * n);
+    int64_t *keys = (int64_t*)dalloc(c, 8 * nbins), *counts = (int64_t*)dalloc(c, 8 * nbins);
+    int64_t *gd = (int64_t*)dalloc(c, 16);
+    if(!cnt || !off || !flat || !keys || !counts || !gd) R HPA_E_MEMORY;
+    int *miss = (int*)(gd + 1);
+    hipMemsetAsync(miss, 0, sizeof(int), c->stream);
+    for(int pass = 0; pass < 2; pass++){
+        if(pass == 1 && (e = scan_u32_to_i64(c, cnt, off, (L)nbins * ntiles, nullptr))) R e;
+        ProfScope ps(pass == 0 ? "group_small_hist" : "group_small_scatter", c->stream,
+                     (double)n * ((s.a ? 2 : 4) + (pass == 0 ? 0 : 8)));
+        if(few){
+            if(pass == 0) hipLaunchKernelGGL(k_sg8_hist, dim3((unsigned)ntiles), dim3(256), 0, c->stream, s, n, nbins, cnt, ntiles, miss);
+            else hipLaunchKernelGGL(k_sg8_scatter, dim3((unsigned)ntiles), dim3(256), 0, c->stream, s, n, nbins, off, ntiles, flat, miss);
+        } else {
+            if(pass == 0) hipLaunchKernelGGL(k_sg_hist, dim3((unsigned)ntiles), dim3(256), 0, c->stream, s, n, nbins, kb, cnt, ntiles, miss);
+            else hipLaunchKernelGGL(k_sg_scatter, dim3((unsigned)ntiles), dim3(256), 0, c->stream, s, n, nbins, kb, off, ntiles, flat, miss);
+        }
+    }
+    hipLaunchKernelGGL(k_sg_final, dim3(1), dim3(256), 0, c->stream, off, ntiles, nbins, n, flat, keys, counts, gd, miss);
+    if((e = check_launch("group_small"))) R e;
+    int64_t g = 0; if((e = d2h(c, &g, gd, 8))) R e;
+    if(g < 0){ *missed = true; R 0; }
+    R make_group_dict(c, z, keys, counts, flat, g, n);
+}
+// presence masks (4 x 64 bits per column) of the two char columns, over all rows or a sample
+static int32_t trie_presence(hpa_ctx *c, const uint8_t *a, const uint8_t *b, L n, int vec, bool sample, unsigned long long *hm){
+    int32_t e;
+    int grid;
+    if(sample) grid = (int)ceil_div(ceil_div(n, 16 * (L)SG_SAMPLE_STRIDE), 256);
+    else grid = sg_scan_grid(n, 16);
+    unsigned long long *pm = (unsigned long long*)dalloc(c, 64 * (L)(grid + 1)); if(!pm) R HPA_E_MEMORY;
+    {
+        ProfScope ps("group_small_present", c->stream, (double)n * (sample ? 2.0 / SG_SAMPLE_STRIDE : 2.0));
+        if(sample) hipLaunchKernelGGL(k_sg_present_sample, dim3(grid), dim3(256), 0, c->stream, a, b, n, (L)SG_SAMPLE_STRIDE, vec, pm + 8);
+        else hipLaunchKernelGGL(k_sg_present, dim3(grid), dim3(256), 0, c->stream, a, b, n, vec, pm + 8);
+        hipLaunchKernelGGL(k_sg_present_final, dim3(1), dim3(256), 0, c->stream, pm + 8, grid, pm);
+    }
+    if((e = check_launch("group_small_present"))) R e;
+    R d2h(c, hm, pm, 64);
+}
+// trie group on at most 256 (c0, c1) bins; *done false when the byte values present need more
+static int32_t small_group_trie(hpa_ctx *c, V z, const uint8_t *a, const uint8_t *b, L n, bool *done){
+    *done = false;
+    int32_t e;
+    const int vec = ((uintptr_t)a % 16 == 0 && (uintptr_t)b % 16 == 0) ? 1 : 0;
+    for(int sample = n >= SG_SAMPLE_MIN ? 1 : 0; sample >= 0; sample--){
+        unsigned long long hm[8];
+        if((e = trie_presence(c, a, b, n, vec, sample != 0, hm))) R e;
+        SgSrc s; memset(&s, 0, sizeof(s));
+        int na = 0, nb = 0;
+        for(int v = 0; v < 256; v++){
+            s.ra[v] = (hm[v >> 6] >> (v & 63)) & 1 ? (uint16_t)(na++) : (uint16_t)SG_ABSENT;
+            s.rb[v] = (hm[4 + (v >> 6)] >> (v & 63)) & 1 ? (uint16_t)(nb++) : (uint16_t)SG_ABSENT;
+        }
+        if(na * nb > SG_BINS) R 0;                  // the whole input holds at least the sample's values
+        s.a = a; s.b = b; s.nb = (uint32_t)nb; s.vec = vec;
+        bool missed;
+        if((e = small_group_run(c, z, s, n, na * nb, &missed))) R e;
+        if(!missed){ *done = true; R 0; }
+    }
+    R 0;
+}
+// i32 group on at most 256 consecutive unsigned keys; *done false otherwise
+static int32_t small_group_i32(hpa_ctx *c, V z, const int32_t *x, L n, bool *done){
+    *done = false;
+    int32_t e;
+    const int vec = (uintptr_t)x % 16 == 0 ? 1 : 0;
+    for(int sample = n >= SG_SAMPLE_MIN ? 1 : 0; sample >= 0; sample--){
+        int grid = sample ? (int)ceil_div(ceil_div(n, 16 * (L)SG_SAMPLE_STRIDE), 256) : sg_scan_grid(n, 4);
+        unsigned *rg = (unsigned*)dalloc(c, 8 * (L)(grid + 1)); if(!rg) R HPA_E_MEMORY;
+        if(sample) hipLaunchKernelGGL(k_sg_urange_sample, dim3(grid), dim3(256), 0, c->stream, x, n, (L)SG_SAMPLE_STRIDE, rg + 2);
+        else hipLaunchKernelGGL(k_sg_urange, dim3(grid), dim3(256), 0, c->stream, x, n, vec, rg + 2);
+        hipLaunchKernelGGL(k_sg_urange_final, dim3(1), dim3(256), 0, c->stream, rg + 2, grid, rg);
+        if((e = check_launch("group_small_range"))) R e;
+        unsigned h[2]; if((e = d2h(c, h, rg, 8))) R e;
+        if(h[1] - h[0] >= (unsigned)SG_BINS) R 0;
+        SgSrc s; memset(&s, 0, sizeof(s));
+        s.x = x; s.base = h[0]; s.vec = vec;
+        bool missed;
+        if((e = small_group_run(c, z, s, n, (int)(h[1] - h[0]) + 1, &missed))) R e;
+        if(!missed){ *done = true; R 0; }
+    }
+    R 0;
+}
+
 // ---------------------------------------------------------------- group
 // pfnGroupTrie (h_primitive.cc:2656-2681): buckets by (c0<<8)+c1 in ascending code order
 extern "C" int32_t pfnGroupTrie(hpa_V z, hpa_V x){
@@ -383,6 +1022,9 @@ extern "C" int32_t pfnGroupTrie(hpa_V z, hpa_V x){
     L n = a->len; int32_t e;
     const void *pa = nullptr, *pb = nullptr;
     if(n > 0){ if((e = dev_view(c, a, &pa)) || (e = dev_view(c, b, &pb))) R e; }
+    if(n > 0){
+        bool done; if((e = small_group_trie(c, z, (const uint8_t*)pa, (const uint8_t*)pb, n, &done)) || done) R e;
+    }
     uint32_t *k = (uint32_t*)dalloc(c, 4 * (n > 0 ? n : 1)); if(!k) R HPA_E_MEMORY;
     if(n > 0) hipLaunchKernelGGL(k_key_trie, dim3(grid_for(n, 256, 4)), dim3(256), 0, c->stream, (const uint8_t*)pa, (const uint8_t*)pb, k, n);
     uint32_t *sk, *sv;
@@ -394,6 +1036,9 @@ static int32_t group_int(hpa_ctx *c, V z, V x){      // lib_group_by_normal_int,
     L n = x->len; int32_t e;
     const void *p = nullptr;
     if(n > 0 && (e = dev_view(c, x, &p))) R e;
+    if(n > 0){
+        bool done; if((e = small_group_i32(c, z, (const int32_t*)p, n, &done)) || done) R e;
+    }
     uint32_t *k = (uint32_t*)dalloc(c, 4 * (n > 0 ? n : 1)); if(!k) R HPA_E_MEMORY;
     if(n > 0) hipLaunchKernelGGL(k_key_i32, dim3(grid_for(n, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)p, k, n);
     uint32_t *sk, *sv;

@@ -193,6 +193,53 @@ def test_group_trie(gpu, orc, n):
     assert groups_of(gpu, zg) == groups_of(orc, zo)
 
 
+@pytest.mark.parametrize("n,lo,span", [(1, 7, 1), (4097, 0, 1), (5000, 1000, 256), (300001, 3, 255),
+                                       (70000, 2**31 - 300, 256), (20000, -10, 200), (9000, 0, 257),
+                                       (16385, 100, 7), (50000, 0, 13), (65, 5, 3), (33000, 9, 17)])
+def test_group_int_small_range(gpu, orc, n, lo, span):
+    """Key ranges of at most 256 values take the counting-sort group (packed per-thread counts up to
+    16 bins, ballot ranks above); ranges that wrap in the unsigned order or hold 257 values take the
+    radix group.  All match the oracle."""
+    x = (lo + rng(n + span).integers(0, span, n)).astype(np.int32)
+    assert groups_of(gpu, gpu.pfnGroup(gpu.vec(x))) == groups_of(orc, orc.pfnGroup(orc.vec(x)))
+
+
+@pytest.mark.parametrize("n,na,nb", [(4096, 16, 16), (12345, 17, 16), (300001, 5, 50), (33, 256, 1), (70001, 1, 1),
+                                     (16383, 3, 2), (40000, 4, 4), (100, 2, 2)])
+def test_group_trie_bins(gpu, orc, n, na, nb):
+    """Trie groups over na x nb byte pairs: up to 256 pairs in the counting-sort group, more in the
+    radix group."""
+    r = rng(n + na + nb)
+    a = r.choice(r.choice(256, na, replace=False), n).astype(np.uint8)
+    b = r.choice(r.choice(256, nb, replace=False), n).astype(np.uint8)
+    zg = gpu.pfnGroup(gpu.list([gpu.vec(a, H_C), gpu.vec(b, H_C)]))
+    zo = orc.pfnGroup(orc.list([orc.vec(a, H_C), orc.vec(b, H_C)]))
+    assert groups_of(gpu, zg) == groups_of(orc, zo)
+
+
+def test_group_small_sampled_bins_miss(gpu, orc):
+    """From 2^20 rows on, the counting-sort group takes its bins from a strided sample of the
+    input; values held only by rows outside the sample are caught by the count pass and the group
+    is redone from the whole input (counting sort again, or radix when the bins then overflow)."""
+    n = (1 << 20) + 5
+    a = np.full(n, ord("A"), np.uint8)
+    b = np.full(n, ord("F"), np.uint8)
+    a[17] = ord("Z")
+    b[n - 1] = ord("O")
+    zg = gpu.pfnGroup(gpu.list([gpu.vec(a, H_C), gpu.vec(b, H_C)]))
+    zo = orc.pfnGroup(orc.list([orc.vec(a, H_C), orc.vec(b, H_C)]))
+    assert groups_of(gpu, zg) == groups_of(orc, zo)
+    x = np.full(n, 5, np.int32)
+    x[40] = 9
+    assert groups_of(gpu, gpu.pfnGroup(gpu.vec(x))) == groups_of(orc, orc.pfnGroup(orc.vec(x)))
+    x[40] = 100000
+    x[n - 2] = 7
+    assert groups_of(gpu, gpu.pfnGroup(gpu.vec(x))) == groups_of(orc, orc.pfnGroup(orc.vec(x)))
+    r = rng(21)
+    x = r.integers(0, 6, n).astype(np.int32)                 # every value in the sample: no redo
+    assert groups_of(gpu, gpu.pfnGroup(gpu.vec(x))) == groups_of(orc, orc.pfnGroup(orc.vec(x)))
+
+
 def test_group_list_and_float(gpu, orc):
     r = rng(12)
     a = r.integers(0, 5, 3000).astype(np.int64)
