@@ -110,9 +110,11 @@ int32_t promote_dev(hpa_ctx *c, V x, int32_t to, const void **p);
 // gather: dst[i] = src[idx[i]] for element width w
 int32_t gather_dev(hpa_ctx *c, void *dst, const void *src, const int64_t *idx, L n, L w);
 // stable LSD radix sort of (key, value) pairs by unsigned key bits [lo_bit, hi_bit)
-int32_t radix_sort_pairs_u64(hpa_ctx *c, uint64_t *keys, uint32_t *vals, L n, int lo_bit, int hi_bit,
+// (inputs are only read; vals null = row ids; outputs in fresh buffers, or the inputs
+// themselves when every key digit in the range is constant)
+int32_t radix_sort_pairs_u64(hpa_ctx *c, const uint64_t *keys, const uint32_t *vals, L n, int lo_bit, int hi_bit,
                              uint64_t **keys_out, uint32_t **vals_out);
-int32_t radix_sort_pairs_u32(hpa_ctx *c, uint32_t *keys, uint32_t *vals, L n, int lo_bit, int hi_bit,
+int32_t radix_sort_pairs_u32(hpa_ctx *c, const uint32_t *keys, const uint32_t *vals, L n, int lo_bit, int hi_bit,
                              uint32_t **keys_out, uint32_t **vals_out);
 int32_t iota_u32(hpa_ctx *c, uint32_t *p, L n);
 // Group a sorted key sequence into runs; emits the reference's dict layout with a flat list:

@@ -539,64 +539,100 @@ extern "C" int32_t hpaProfileList(char *buf, int64_t cap){
 // ---------------------------------------------------------------- device-wide exclusive scan
 // Three-phase reduce-then-scan over int64: per-block sums, a single-block scan of the block
 // sums, then per-block scans seeded with the block offset.  Deterministic and order-exact.
+// Thread t of block b owns the SCAN_ITEMS consecutive elements from (b * SCAN_BLOCK + t) *
+// SCAN_ITEMS, read (and, in the down pass, written) with 16-byte vector accesses when aligned,
+// so a block needs one barrier and no dependent loads.
 #define SCAN_BLOCK 256
 #define SCAN_ITEMS 16
 template<typename TI>
-__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_reduce(const TI *in, L n, int64_t *bsum){
-    __shared__ int64_t sh[SCAN_BLOCK / 64];
-    L base = (L)blockIdx.x * SCAN_BLOCK * SCAN_ITEMS;
-    int64_t s = 0;
-    for(int k=0;k<SCAN_ITEMS;k++){ L i = base + (L)k*SCAN_BLOCK + threadIdx.x; if(i < n) s += (int64_t)in[i]; }
-    for(int o=32;o>0;o>>=1) s += __shfl_down(s, o, 64);
-    if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+__device__ __forceinline__ void scan_load(const TI *in, L i0, L n, int vec, int64_t *v){
+    if(vec && i0 + SCAN_ITEMS <= n){
+        if(sizeof(TI) == 4){
+            #pragma unroll
+            for(int q = 0; q < SCAN_ITEMS / 4; q++){
+                const uint4 w = *(const uint4*)(in + i0 + 4 * q);
+                v[4 * q] = (int64_t)(TI)w.x; v[4 * q + 1] = (int64_t)(TI)w.y; v[4 * q + 2] = (int64_t)(TI)w.z; v[4 * q + 3] = (int64_t)(TI)w.w;
+            }
+        } else {
+            #pragma unroll
+            for(int q = 0; q < SCAN_ITEMS / 2; q++){
+                const uint4 w = *(const uint4*)(in + i0 + 2 * q);
+                v[2 * q] = (int64_t)(((uint64_t)w.y << 32) | w.x); v[2 * q + 1] = (int64_t)(((uint64_t)w.w << 32) | w.z);
+            }
+        }
+    } else {
+        #pragma unroll
+        for(int k = 0; k < SCAN_ITEMS; k++) v[k] = i0 + k < n ? (int64_t)in[i0 + k] : 0;
+    }
+}
+// exclusive scan of one value per thread over the block; *total = block sum
+__device__ __forceinline__ int64_t scan_block_excl(int64_t x, int64_t *wsum, int64_t *total){
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int64_t inc = x;
+    for(int o = 1; o < 64; o <<= 1){ int64_t t = __shfl_up(inc, o, 64); if(lane >= o) inc += t; }
+    if(lane == 63) wsum[w] = inc;
     __syncthreads();
-    if(threadIdx.x == 0){ int64_t t = 0; for(int w=0;w<SCAN_BLOCK/64;w++) t += sh[w]; bsum[blockIdx.x] = t; }
+    int64_t pre = inc - x, tot = 0;
+    for(int q = 0; q < SCAN_BLOCK / 64; q++){ pre += q < w ? wsum[q] : 0; tot += wsum[q]; }
+    *total = tot;
+    R pre;
+}
+template<typename TI>
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_reduce(const TI *in, L n, int vec, int64_t *bsum){
+    __shared__ int64_t wsum[SCAN_BLOCK / 64];
+    const L i0 = ((L)blockIdx.x * SCAN_BLOCK + threadIdx.x) * SCAN_ITEMS;
+    int64_t v[SCAN_ITEMS];
+    scan_load<TI>(in, i0, n, vec, v);
+    int64_t s = 0;
+    #pragma unroll
+    for(int k = 0; k < SCAN_ITEMS; k++) s += v[k];
+    for(int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if(threadIdx.x == 0){ int64_t t = 0; for(int w = 0; w < SCAN_BLOCK / 64; w++) t += wsum[w]; bsum[blockIdx.x] = t; }
 }
 __global__ void __launch_bounds__(1024) k_scan_blocks(int64_t *bsum, L nb, int64_t *total){
-    __shared__ int64_t sh[1024];
+    __shared__ int64_t wsum[16];
     __shared__ int64_t carry;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if(threadIdx.x == 0) carry = 0;
     __syncthreads();
     for(L base = 0; base < nb; base += 1024){
-        L i = base + threadIdx.x;
-        int64_t v = i < nb ? bsum[i] : 0;
-        sh[threadIdx.x] = v;
+        const L i = base + threadIdx.x;
+        const int64_t v = i < nb ? bsum[i] : 0;
+        int64_t inc = v;
+        for(int o = 1; o < 64; o <<= 1){ int64_t t = __shfl_up(inc, o, 64); if(lane >= o) inc += t; }
+        if(lane == 63) wsum[w] = inc;
         __syncthreads();
-        for(int o=1;o<1024;o<<=1){
-            int64_t t = threadIdx.x >= (unsigned)o ? sh[threadIdx.x - o] : 0;
-            __syncthreads();
-            sh[threadIdx.x] += t;
-            __syncthreads();
-        }
-        int64_t incl = sh[threadIdx.x];
-        if(i < nb) bsum[i] = carry + incl - v;
+        int64_t pre = carry + inc - v, tot = 0;
+        for(int q = 0; q < 16; q++){ pre += q < w ? wsum[q] : 0; tot += wsum[q]; }
+        if(i < nb) bsum[i] = pre;
         __syncthreads();
-        if(threadIdx.x == 1023) carry += incl;
+        if(threadIdx.x == 0) carry += tot;
         __syncthreads();
     }
     if(threadIdx.x == 0 && total) *total = carry;
 }
 template<typename TI>
-__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_down(const TI *in, int64_t *out, L n, const int64_t *boff){
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_down(const TI *in, int64_t *out, L n, int vec, const int64_t *boff){
     __shared__ int64_t wsum[SCAN_BLOCK / 64];
-    __shared__ int64_t run;
-    L base = (L)blockIdx.x * SCAN_BLOCK * SCAN_ITEMS;
-    if(threadIdx.x == 0) run = boff[blockIdx.x];
-    __syncthreads();
-    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for(int k=0;k<SCAN_ITEMS;k++){
-        L i = base + (L)k*SCAN_BLOCK + threadIdx.x;
-        int64_t v = i < n ? (int64_t)in[i] : 0;
-        int64_t x = v;
-        for(int o=1;o<64;o<<=1){ int64_t t = __shfl_up(x, o, 64); if(lane >= o) x += t; }
-        if(lane == 63) wsum[w] = x;
-        __syncthreads();
-        int64_t pre = run;
-        for(int q=0;q<w;q++) pre += wsum[q];
-        if(i < n) out[i] = pre + x - v;
-        __syncthreads();
-        if(threadIdx.x == SCAN_BLOCK - 1) run = pre + x;
-        __syncthreads();
+    const L i0 = ((L)blockIdx.x * SCAN_BLOCK + threadIdx.x) * SCAN_ITEMS;
+    int64_t v[SCAN_ITEMS];
+    scan_load<TI>(in, i0, n, vec, v);
+    int64_t s = 0;
+    #pragma unroll
+    for(int k = 0; k < SCAN_ITEMS; k++){ const int64_t t = v[k]; v[k] = s; s += t; }
+    int64_t tot;
+    const int64_t pre = boff[blockIdx.x] + scan_block_excl(s, wsum, &tot);
+    if(vec && i0 + SCAN_ITEMS <= n){
+        #pragma unroll
+        for(int q = 0; q < SCAN_ITEMS / 2; q++){
+            const uint64_t a = (uint64_t)(pre + v[2 * q]), b = (uint64_t)(pre + v[2 * q + 1]);
+            *(uint4*)(out + i0 + 2 * q) = make_uint4((unsigned)a, (unsigned)(a >> 32), (unsigned)b, (unsigned)(b >> 32));
+        }
+    } else {
+        #pragma unroll
+        for(int k = 0; k < SCAN_ITEMS; k++) if(i0 + k < n) out[i0 + k] = pre + v[k];
     }
 }
 template<typename TI>
@@ -605,10 +641,11 @@ static int32_t scan_impl(hpa_ctx *c, const TI *in, int64_t *out, L n, int64_t *t
     L nb = ceil_div(n, per); if(nb < 1) nb = 1;
     int64_t *bsum = (int64_t*)dalloc(c, 8 * nb);
     if(!bsum) R HPA_E_MEMORY;
-    if(n > 0) hipLaunchKernelGGL(k_scan_reduce<TI>, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, c->stream, in, n, bsum);
+    const int vec = ((uintptr_t)in % 16 == 0 && (uintptr_t)out % 16 == 0) ? 1 : 0;
+    if(n > 0) hipLaunchKernelGGL(k_scan_reduce<TI>, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, c->stream, in, n, vec, bsum);
     else hipMemsetAsync(bsum, 0, 8, c->stream);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, c->stream, bsum, nb, total_dev);
-    if(n > 0) hipLaunchKernelGGL(k_scan_down<TI>, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, c->stream, in, out, n, bsum);
+    if(n > 0) hipLaunchKernelGGL(k_scan_down<TI>, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, c->stream, in, out, n, vec, bsum);
     R check_launch("scan");
 }
 int32_t scan_i64(hpa_ctx *c, const int64_t *in, int64_t *out, L n, int64_t *total_dev){ R scan_impl<int64_t>(c, in, out, n, total_dev); }

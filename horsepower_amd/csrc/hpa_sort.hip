@@ -14,42 +14,53 @@
 //
 // Radix pass: 4096-key tiles; per-tile digit histogram; exclusive scan in (digit, tile) order;
 // scatter with each key's stable rank inside its tile computed from eight wavefront ballots
-// (lanes sharing the digit) and per-wave LDS counters.  Digits that are constant over all keys
-// are skipped.
+// (lanes sharing the digit), staged through LDS so the writes go out as digit runs.  Digits that
+// are constant over all keys are skipped.  (A single-pass variant with decoupled look-back
+// between tiles was measured slower here: its look-back chains wait on device-scope atomics.)
 #include "hpa_internal.h"
 
 #define RS_BLOCK 256
 #define RS_ROWS  16
 #define RS_TILE  (RS_BLOCK * RS_ROWS)
 
-// Digit histogram of one 4096-key tile.  With `bits` set (the first pass) it also writes the
-// tile's OR and AND of all keys to bits[2*tile], bits[2*tile+1]: which key bits vary.
+// Digit histogram of one 4096-key tile, counted in per-wave LDS histograms; a wave whose valid
+// keys all share the digit adds them with one increment.  hist[d * ntiles + tile].  With `bits`
+// set (the first pass) it also writes the tile's OR and AND of all keys to bits[2*tile],
+// bits[2*tile+1]: which key bits vary.
 template<typename K>
 __global__ void __launch_bounds__(RS_BLOCK) k_rs_hist(const K *keys, L n, int shift, uint32_t *hist, L ntiles,
                                                       unsigned long long *bits){
-    __shared__ uint32_t cnt[256];
+    __shared__ uint32_t cnt[4][256];
     __shared__ unsigned long long so[RS_BLOCK / 64], sa[RS_BLOCK / 64];
-    cnt[threadIdx.x] = 0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for(int q = 0; q < 4; q++) cnt[q][threadIdx.x] = 0;
     __syncthreads();
-    L base = (L)blockIdx.x * RS_TILE;
+    const L base = (L)blockIdx.x * RS_TILE + (L)w * (RS_TILE / 4);
     unsigned long long o = 0, a = ~0ull;
-    #pragma unroll 4
+    K kr[RS_ROWS];
+    #pragma unroll
+    for(int r = 0; r < RS_ROWS; r++){ const L i = base + (L)r * 64 + lane; kr[r] = i < n ? keys[i] : (K)0; }
+    #pragma unroll
     for(int r = 0; r < RS_ROWS; r++){
-        L i = base + (L)r * RS_BLOCK + threadIdx.x;
-        if(i < n){
-            const K k = keys[i];
-            unsigned d = (unsigned)((k >> shift) & 255); atomicAdd(&cnt[d], 1u);
-            o |= (unsigned long long)k; a &= (unsigned long long)k;
-        }
+        const L i = base + (L)r * 64 + lane;
+        const bool valid = i < n;
+        const K k = kr[r];
+        const unsigned d = (unsigned)((k >> shift) & 255);
+        const unsigned d0 = __builtin_amdgcn_readfirstlane(d);
+        const uint64_t vm = __ballot(valid);
+        if(vm == 0) continue;
+        if(__ballot(valid && d == d0) == vm){ if(lane == 0) cnt[w][d0] += (uint32_t)__popcll(vm); }
+        else if(valid) atomicAdd(&cnt[w][d], 1u);
+        if(valid){ o |= (unsigned long long)k; a &= (unsigned long long)k; }
     }
     if(bits){
         for(int s = 32; s > 0; s >>= 1){ o |= __shfl_down(o, s, 64); a &= __shfl_down(a, s, 64); }
-        if((threadIdx.x & 63) == 0){ so[threadIdx.x >> 6] = o; sa[threadIdx.x >> 6] = a; }
+        if(lane == 0){ so[w] = o; sa[w] = a; }
     }
     __syncthreads();
-    hist[(L)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+    hist[(L)threadIdx.x * ntiles + blockIdx.x] = cnt[0][threadIdx.x] + cnt[1][threadIdx.x] + cnt[2][threadIdx.x] + cnt[3][threadIdx.x];
     if(bits && threadIdx.x == 0){
-        for(int w = 1; w < RS_BLOCK / 64; w++){ o |= so[w]; a &= sa[w]; }
+        for(int q = 1; q < RS_BLOCK / 64; q++){ o |= so[q]; a &= sa[q]; }
         bits[2 * blockIdx.x] = o; bits[2 * blockIdx.x + 1] = a;
     }
 }
@@ -70,10 +81,12 @@ __global__ void __launch_bounds__(256) k_rs_bits_final(const unsigned long long 
 // Scatter of one tile: every key's stable rank inside its wave's quarter of the tile comes from
 // eight ballots (lanes sharing the digit); the tile is then laid out digit by digit in LDS and
 // written out in that order, so consecutive threads write consecutive addresses of one digit's
-// run instead of 4096 scattered 4/8-byte stores.
-template<typename K>
+// run instead of 4096 scattered 4/8-byte stores.  off[d * ntiles + tile]: output start of the
+// tile's keys of digit d.  FINAL: only the values are written, widened to i64 into out64 at
+// position pos, or n - 1 - pos with `reverse`.
+template<typename K, bool FINAL>
 __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K *kin, const uint32_t *vin, K *kout, uint32_t *vout,
-                                                         L n, int shift, const int64_t *off, L ntiles){
+                                                         int64_t *out64, int reverse, L n, int shift, const int64_t *off, L ntiles){
     __shared__ uint32_t wcnt[4][256];          // per-wave digit counts, then per-wave digit starts
     __shared__ int64_t gbase[256];             // global position of a digit's run minus its tile-local start
     __shared__ uint32_t wsum[RS_BLOCK / 64];
@@ -90,8 +103,14 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K *kin, const uin
     for(int r = 0; r < RS_ROWS; r++){
         L i = tbase + (L)r * 64 + lane;
         bool valid = i < n;
-        K key = valid ? kin[i] : (K)0;
-        unsigned d = (unsigned)((key >> shift) & 255);
+        kr[r] = valid ? kin[i] : (K)0;
+        vr[r] = valid ? (vin ? vin[i] : (uint32_t)i) : 0u;
+    }
+    #pragma unroll
+    for(int r = 0; r < RS_ROWS; r++){
+        L i = tbase + (L)r * 64 + lane;
+        bool valid = i < n;
+        unsigned d = (unsigned)((kr[r] >> shift) & 255);
         uint64_t peers = __ballot(valid);
         #pragma unroll
         for(int b = 0; b < 8; b++){
@@ -101,7 +120,7 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K *kin, const uin
         uint32_t before = (uint32_t)__popcll(peers & lt);
         uint32_t cur = wcnt[w][d];
         if(valid && before == 0) wcnt[w][d] = cur + (uint32_t)__popcll(peers);
-        kr[r] = key; vr[r] = valid ? (vin ? vin[i] : (uint32_t)i) : 0u; rk[r] = cur + before;
+        rk[r] = cur + before;
     }
     __syncthreads();
     {
@@ -134,72 +153,89 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K *kin, const uin
     for(int j = threadIdx.x; j < cnt; j += RS_BLOCK){
         const K k = sk[j];
         const int64_t pos = gbase[(unsigned)((k >> shift) & 255)] + j;
-        kout[pos] = k; vout[pos] = sv[j];
+        if(FINAL) out64[reverse ? n - 1 - pos : pos] = (int64_t)sv[j];
+        else { kout[pos] = k; vout[pos] = sv[j]; }
     }
+}
+__global__ void k_iota_i64(int64_t *z, L n, int reverse){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x) z[i] = reverse ? n - 1 - i : i;
 }
 
-// Stable sort of (keys, vals) by key bits [lo, hi).  vals may be null (row ids 0..n-1).
+// Stable sort of (keys, vals) by key bits [lo, hi); vals may be null (row ids 0..n-1).  The
+// inputs are only read.  Without out64 the sorted keys and values come back in *kout / *vout
+// (the input keys, and the input values or fresh row ids, when every digit is constant).  With
+// out64 the sorted values are written there as i64, reversed with `reverse`, and *kout / *vout
+// are not set.  Each pass: per-tile digit histograms (the first one also finds the key bits that
+// vary, so passes over constant digits are skipped), a scan in (digit, tile) order, a scatter.
 template<typename K>
-static int32_t radix_sort_t(hpa_ctx *c, K *keys, uint32_t *vals, L n, int lo, int hi, K **kout, uint32_t **vout){
+static int32_t radix_sort_t(hpa_ctx *c, const K *keys, const uint32_t *vals, L n, int lo, int hi, K **kout, uint32_t **vout,
+                            int64_t *out64, bool reverse){
     if(n >= (L)UINT32_MAX){ set_error_detail("radix sort of %lld keys exceeds 32-bit row ids", (long long)n); R HPA_E_NOT_IMPL; }
-    *kout = keys; *vout = vals;
-    if(n <= 0) R 0;
     int32_t e;
-    L ntiles = ceil_div(n, RS_TILE);
+    if(n <= 0){ if(!out64){ *kout = (K*)keys; *vout = (uint32_t*)vals; } R 0; }
+    const L ntiles = ceil_div(n, RS_TILE);
     unsigned long long *bits = (unsigned long long*)dalloc(c, 16 + 16 * ntiles);
-    K *k2 = (K*)dalloc(c, sizeof(K) * n);
-    uint32_t *v2 = (uint32_t*)dalloc(c, 4 * n);
-    uint32_t *v1 = vals;
-    if(!v1){ v1 = (uint32_t*)dalloc(c, 4 * n); if(!v1) R HPA_E_MEMORY; }
     uint32_t *hist = (uint32_t*)dalloc(c, 4 * 256 * ntiles);
     int64_t *off = (int64_t*)dalloc(c, 8 * 256 * ntiles);
-    if(!bits || !k2 || !v2 || !hist || !off) R HPA_E_MEMORY;
-    // ping-pong: va == nullptr means "row ids" (first pass without given values)
-    K *ka = keys, *kb = k2; uint32_t *va = vals, *vb = v2;
-    uint32_t *spare = vals ? vals : v1;
-    bool any = false;
-    unsigned long long vary = ~0ull;
-    bool first = true;
-    for(int s = lo; s < hi; s += 8){
-        if(!first && ((vary >> s) & 255ull) == 0) continue;
-        {
+    if(!bits || !hist || !off) R HPA_E_MEMORY;
+    // digits to sort by: the first histogram pass also finds the bits that vary
+    {
+        ProfScope ps("radix_hist", c->stream, (double)n * sizeof(K) + 4.0 * 256 * ntiles);
+        hipLaunchKernelGGL(k_rs_hist<K>, dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream, keys, n, lo, hist, ntiles, bits + 2);
+        hipLaunchKernelGGL(k_rs_bits_final, dim3(1), dim3(256), 0, c->stream, bits + 2, ntiles, bits);
+    }
+    if((e = check_launch("radix_hist"))) R e;
+    unsigned long long hb[2]; if((e = d2h(c, hb, bits, 16))) R e;
+    const unsigned long long vary = hb[0] ^ hb[1];
+    int run[8], nrun = 0;
+    for(int s = lo; s < hi; s += 8) if((vary >> s) & 255ull) run[nrun++] = s;
+    if(nrun == 0){
+        // every digit is constant: the input order is already sorted
+        if(out64){
+            hipLaunchKernelGGL(k_iota_i64, dim3(grid_for(n, 256, 4)), dim3(256), 0, c->stream, out64, n, reverse ? 1 : 0);
+            R check_launch("radix_sort");
+        }
+        *kout = (K*)keys;
+        if(vals){ *vout = (uint32_t*)vals; R 0; }
+        uint32_t *v1 = (uint32_t*)dalloc(c, 4 * n); if(!v1) R HPA_E_MEMORY;
+        if((e = iota_u32(c, v1, n))) R e;
+        *vout = v1; R 0;
+    }
+    K *kb[2] = {nullptr, nullptr}; uint32_t *vb[2] = {nullptr, nullptr};
+    const int nout = out64 ? nrun - 1 : nrun, nbuf = nout < 2 ? nout : 2;
+    for(int q = 0; q < nbuf; q++){
+        kb[q] = (K*)dalloc(c, sizeof(K) * n); vb[q] = (uint32_t*)dalloc(c, 4 * n);
+        if(!kb[q] || !vb[q]) R HPA_E_MEMORY;
+    }
+    const K *ka = keys; const uint32_t *va = vals;
+    for(int r = 0; r < nrun; r++){
+        const int s = run[r];
+        const bool last = r == nrun - 1;
+        if(!(r == 0 && s == lo)){
             ProfScope ps("radix_hist", c->stream, (double)n * sizeof(K) + 4.0 * 256 * ntiles);
             hipLaunchKernelGGL(k_rs_hist<K>, dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream, ka, n, s, hist, ntiles,
-                               first ? bits + 2 : nullptr);
-        }
-        if(first){
-            // which bits vary, from the first histogram pass: constant digits are skipped
-            hipLaunchKernelGGL(k_rs_bits_final, dim3(1), dim3(256), 0, c->stream, bits + 2, ntiles, bits);
-            unsigned long long hb[2]; if((e = d2h(c, hb, bits, 16))) R e;
-            vary = hb[0] ^ hb[1];
-            first = false;
-            if(((vary >> s) & 255ull) == 0) continue;
+                               (unsigned long long*)nullptr);
         }
         if((e = scan_u32_to_i64(c, hist, off, 256 * ntiles, nullptr))) R e;
-        {
-            ProfScope ps("radix_scatter", c->stream, (double)n * (2 * sizeof(K) + (va ? 8 : 4)) + 8.0 * 256 * ntiles);
-            hipLaunchKernelGGL(k_rs_scatter<K>, dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream,
-                               ka, va, kb, vb, n, s, off, ntiles);
-        }
+        const double rd = (double)n * (sizeof(K) + (va ? 4 : 0)), wr = last && out64 ? 8.0 * n : (double)n * (sizeof(K) + 4);
+        ProfScope ps("radix_scatter", c->stream, rd + wr);
+        if(last && out64)
+            hipLaunchKernelGGL((k_rs_scatter<K, true>), dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream,
+                               ka, va, (K*)nullptr, (uint32_t*)nullptr, out64, reverse ? 1 : 0, n, s, off, ntiles);
+        else
+            hipLaunchKernelGGL((k_rs_scatter<K, false>), dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream,
+                               ka, va, kb[r & 1], vb[r & 1], (int64_t*)nullptr, 0, n, s, off, ntiles);
         if((e = check_launch("radix_sort"))) R e;
-        any = true;
-        K *tk = ka; ka = kb; kb = tk;
-        uint32_t *written = vb;
-        vb = (va != nullptr) ? va : spare;
-        va = written;
+        if(!(last && out64)){ ka = kb[r & 1]; va = vb[r & 1]; }
     }
-    if(!any){
-        // every digit is constant: the input order is already sorted
-        if(!vals){ if((e = iota_u32(c, v1, n))) R e; va = v1; }
-    }
-    *kout = ka; *vout = va;
+    if(!out64){ *kout = (K*)ka; *vout = (uint32_t*)va; }
     R 0;
 }
-int32_t radix_sort_pairs_u64(hpa_ctx *c, uint64_t *keys, uint32_t *vals, L n, int lo, int hi, uint64_t **ko, uint32_t **vo){
-    R radix_sort_t<uint64_t>(c, keys, vals, n, lo, hi, ko, vo);
+int32_t radix_sort_pairs_u64(hpa_ctx *c, const uint64_t *keys, const uint32_t *vals, L n, int lo, int hi, uint64_t **ko, uint32_t **vo){
+    R radix_sort_t<uint64_t>(c, keys, vals, n, lo, hi, ko, vo, nullptr, false);
 }
-int32_t radix_sort_pairs_u32(hpa_ctx *c, uint32_t *keys, uint32_t *vals, L n, int lo, int hi, uint32_t **ko, uint32_t **vo){
-    R radix_sort_t<uint32_t>(c, keys, vals, n, lo, hi, ko, vo);
+int32_t radix_sort_pairs_u32(hpa_ctx *c, const uint32_t *keys, const uint32_t *vals, L n, int lo, int hi, uint32_t **ko, uint32_t **vo){
+    R radix_sort_t<uint32_t>(c, keys, vals, n, lo, hi, ko, vo, nullptr, false);
 }
 
 // ---------------------------------------------------------------- runs -> dict(keys, flat list)
@@ -318,9 +354,6 @@ int32_t runs_to_group_u32(hpa_ctx *c, V z, const uint32_t *sk, const uint32_t *s
 }
 
 // ---------------------------------------------------------------- key builders
-__global__ void k_key_i32(const int32_t *x, uint32_t *k, L n){
-    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x) k[i] = (uint32_t)x[i];
-}
 __global__ void k_key_trie(const uint8_t *a, const uint8_t *b, uint32_t *k, L n){
     for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x) k[i] = ((uint32_t)a[i] << 8) | b[i];
 }
@@ -1039,10 +1072,9 @@ static int32_t group_int(hpa_ctx *c, V z, V x){      // lib_group_by_normal_int,
     if(n > 0){
         bool done; if((e = small_group_i32(c, z, (const int32_t*)p, n, &done)) || done) R e;
     }
-    uint32_t *k = (uint32_t*)dalloc(c, 4 * (n > 0 ? n : 1)); if(!k) R HPA_E_MEMORY;
-    if(n > 0) hipLaunchKernelGGL(k_key_i32, dim3(grid_for(n, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)p, k, n);
+    // the unsigned radix key of an i32 is its bit pattern: the column is sorted in place of keys
     uint32_t *sk, *sv;
-    if((e = radix_sort_pairs_u32(c, k, nullptr, n, 0, 32, &sk, &sv))) R e;
+    if((e = radix_sort_pairs_u32(c, (const uint32_t*)p, nullptr, n, 0, 32, &sk, &sv))) R e;
     R runs_to_group_u32(c, z, sk, sv, n);
 }
 
@@ -1296,24 +1328,36 @@ static int32_t order_emit(hpa_ctx *c, V z, const uint32_t *perm, L n, bool rever
 }
 
 // single i32 / i64 column or the packed pair: stable unsigned radix, desc = reversal
+// order of a single key column by its radix key: the last radix pass writes the i64 permutation
+// (reversed for a descending order) straight into the result
+template<typename K>
+static int32_t order_radix_out(hpa_ctx *c, V z, const K *keys, L n, int hi, bool up){
+    int32_t e;
+    if(n <= 1){
+        uint32_t *pm = (uint32_t*)dalloc(c, 4); if(!pm) R HPA_E_MEMORY;
+        if(n == 1 && (e = iota_u32(c, pm, 1))) R e;
+        R order_emit(c, z, pm, n, !up);
+    }
+    void *dz;
+    if((e = out_init(c, z, HPA_L, n, &dz))) R e;
+    K *sk; uint32_t *sv;
+    if((e = radix_sort_t<K>(c, keys, nullptr, n, 0, hi, &sk, &sv, (int64_t*)dz, !up))) R e;
+    note_bound(c, dz, n);                        // a permutation of 0..n-1
+    R 0;
+}
 static int32_t order_radix_i32(hpa_ctx *c, V z, V col, bool up){
     L n = col->len; int32_t e;
     const void *p = nullptr;
     if(n > 0 && (e = dev_view(c, col, &p))) R e;
-    uint32_t *k = (uint32_t*)dalloc(c, 4 * (n > 0 ? n : 1)); if(!k) R HPA_E_MEMORY;
-    if(n > 0) hipLaunchKernelGGL(k_key_i32, dim3(grid_for(n, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)p, k, n);
-    uint32_t *sk, *sv;
-    if((e = radix_sort_pairs_u32(c, k, nullptr, n, 0, 32, &sk, &sv))) R e;
-    R order_emit(c, z, sv, n, !up);
+    R order_radix_out<uint32_t>(c, z, (const uint32_t*)p, n, 32, up);     // key = the i32 bit pattern
 }
 static int32_t order_radix_i64(hpa_ctx *c, V z, const int64_t *p, L n, bool up){
     int32_t e; bool hi;
     if((e = any_high_i64(c, p, n, &hi))) R e;
+    if(hi) R order_radix_out<uint64_t>(c, z, (const uint64_t*)p, n, 64, up);
     uint64_t *k = (uint64_t*)dalloc(c, 8 * (n > 0 ? n : 1)); if(!k) R HPA_E_MEMORY;
-    if(n > 0) hipLaunchKernelGGL(k_key_i64, dim3(grid_for(n, 256, 4)), dim3(256), 0, c->stream, p, k, n, hi ? 0 : 1);
-    uint64_t *sk; uint32_t *sv;
-    if((e = radix_sort_pairs_u64(c, k, nullptr, n, 0, hi ? 64 : 32, &sk, &sv))) R e;
-    R order_emit(c, z, sv, n, !up);
+    if(n > 0) hipLaunchKernelGGL(k_key_i64, dim3(grid_for(n, 256, 4)), dim3(256), 0, c->stream, p, k, n, 1);
+    R order_radix_out<uint64_t>(c, z, k, n, 32, up);
 }
 
 // lib_quicksort_char on fewer than 10 rows (h_libs.cc:390-392): the all-pairs exchange with
@@ -1352,9 +1396,7 @@ static int32_t order_vector(hpa_ctx *c, V z, V col, bool up){   // lib_order_by_
         uint64_t *k = (uint64_t*)dalloc(c, 8 * n); if(!k) R HPA_E_MEMORY;
         const void *p; if((e = dev_view(c, col, &p))) R e;
         hipLaunchKernelGGL(k_okey_perm<uint8_t>, dim3(grid_for(n, 256, 4)), dim3(256), 0, c->stream, (const uint8_t*)p, (const uint32_t*)nullptr, k, n, 0, 8);
-        uint64_t *sk; uint32_t *sv;
-        if((e = radix_sort_pairs_u64(c, k, nullptr, n, 0, 8, &sk, &sv))) R e;
-        R order_emit(c, z, sv, n, !up);
+        R order_radix_out<uint64_t>(c, z, k, n, 8, up);
     }
     // comparison-sorted types (quicksort in the reference): stable sort on the order key
     V0 lst; memset(&lst, 0, sizeof(lst)); lst.typ = HPA_G; lst.len = 1; lst.g = (char*)col;
