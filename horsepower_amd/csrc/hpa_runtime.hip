@@ -635,8 +635,34 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_down(const TI *in, int64_t 
         for(int k = 0; k < SCAN_ITEMS; k++) if(i0 + k < n) out[i0 + k] = pre + v[k];
     }
 }
+// inputs of at most 1024 * SCAN_ITEMS elements: one workgroup, one launch
+template<typename TI>
+__global__ void __launch_bounds__(1024) k_scan_small(const TI *in, int64_t *out, L n, int vec, int64_t *total){
+    __shared__ int64_t wsum[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const L i0 = (L)threadIdx.x * SCAN_ITEMS;
+    int64_t v[SCAN_ITEMS];
+    scan_load<TI>(in, i0, n, vec, v);
+    int64_t s = 0;
+    #pragma unroll
+    for(int k = 0; k < SCAN_ITEMS; k++){ const int64_t t = v[k]; v[k] = s; s += t; }
+    int64_t inc = s;
+    for(int o = 1; o < 64; o <<= 1){ int64_t t = __shfl_up(inc, o, 64); if(lane >= o) inc += t; }
+    if(lane == 63) wsum[w] = inc;
+    __syncthreads();
+    int64_t pre = inc - s;
+    for(int q = 0; q < w; q++) pre += wsum[q];
+    #pragma unroll
+    for(int k = 0; k < SCAN_ITEMS; k++) if(i0 + k < n) out[i0 + k] = pre + v[k];
+    if(threadIdx.x == 1023 && total) *total = pre + s;
+}
 template<typename TI>
 static int32_t scan_impl(hpa_ctx *c, const TI *in, int64_t *out, L n, int64_t *total_dev){
+    if(n > 0 && n <= 1024 * SCAN_ITEMS){
+        const int vec = ((uintptr_t)in % 16 == 0) ? 1 : 0;
+        hipLaunchKernelGGL(k_scan_small<TI>, dim3(1), dim3(1024), 0, c->stream, in, out, n, vec, total_dev);
+        R check_launch("scan");
+    }
     L per = (L)SCAN_BLOCK * SCAN_ITEMS;
     L nb = ceil_div(n, per); if(nb < 1) nb = 1;
     int64_t *bsum = (int64_t*)dalloc(c, 8 * nb);

@@ -192,9 +192,12 @@ __global__ void __launch_bounds__(CP_BLOCK) k_cp_write_mask(const uint8_t *mask,
 
 // Counting pass: tile offsets on the device and the total on the host.
 template<typename Pred>
+// Per-tile counts and their scan: toff[t] = rows selected before tile t, toff[ntiles] = total.
+// With k_host the total is also read to the host (a synchronisation); without, only *toff_out.
 static int32_t compress_count(hpa_ctx *c, Pred pred, const uint8_t *mask, L n, int64_t **toff_out, int64_t *k_host){
     L ntiles = ceil_div(n, CP_TILE);
-    *k_host = 0;
+    if(k_host) *k_host = 0;
+    *toff_out = nullptr;
     if(ntiles < 1) R 0;
     uint32_t *tcount = (uint32_t*)dalloc(c, 4 * ntiles);
     int64_t *toff = (int64_t*)dalloc(c, 8 * (ntiles + 1));
@@ -207,7 +210,7 @@ static int32_t compress_count(hpa_ctx *c, Pred pred, const uint8_t *mask, L n, i
     if((e = check_launch("compress_count"))) R e;
     if((e = scan_u32_to_i64(c, tcount, toff, ntiles, toff + ntiles))) R e;
     *toff_out = toff;
-    R d2h(c, k_host, toff + ntiles, 8);
+    R k_host ? d2h(c, k_host, toff + ntiles, 8) : 0;
 }
 template<typename Pred>
 static int32_t compress_write(hpa_ctx *c, Pred pred, L n, CpCols cols, int64_t *idx_out, const int64_t *toff){
@@ -255,23 +258,38 @@ static int32_t compress_many(hpa_ctx *c, int ny, V *z, V x, V *y, bool expr_mode
     if(expr_mode) for(int k = 0; k < ep->x.n; k++) wcond += (double)type_width(ep->x.typ[k]);
     ProfScope ps(expr_mode ? "fused_compress" : "compress", c->stream);
     int64_t kk = 0, *toff = nullptr;
-    e = expr_mode ? compress_count(c, *ep, nullptr, n, &toff, &kk) : compress_count(c, MaskPred{m}, m, n, &toff, &kk);
+    // When the arena has room for every output at full length, the write pass is launched right
+    // behind the count pass and the host reads the count while it runs; the arena top is then
+    // pulled back to the end of the last output's selected rows.
+    L maxb = 0; for(int k = 0; k < ny; k++) maxb += (type_bytes(y[k]->typ, n > 0 ? n : 1) + 255) & ~(L)255;
+    const bool spec = n > 0 && c->dcur + maxb + 4096 + 16 * ceil_div(n, CP_TILE) <= c->dtop;
+    e = expr_mode ? compress_count(c, *ep, nullptr, n, &toff, spec ? nullptr : &kk)
+                  : compress_count(c, MaskPred{m}, m, n, &toff, spec ? nullptr : &kk);
     if(e){ c->dcur = mark; R e; }
-    if(kk == n && n > 0 && !expr_mode){                    // copyV alias (h_primitive.cc:1513-1515)
+    if(!spec && kk == n && n > 0 && !expr_mode){           // copyV alias (h_primitive.cc:1513-1515)
         ps.stop(); ps.bytes = (double)n;
         c->dcur = mark;
         for(int k = 0; k < ny; k++) *z[k] = *y[k];
         R 0;
     }
-    // outputs sized exactly: the counting pass already knows how many rows are selected
     for(int k = 0; k < ny; k++){
-        cols.dst[k] = (char*)dalloc(c, type_bytes(y[k]->typ, kk > 0 ? kk : 1));
+        cols.dst[k] = (char*)dalloc(c, type_bytes(y[k]->typ, spec ? n : (kk > 0 ? kk : 1)));
         if(!cols.dst[k]){ c->dcur = mark; R HPA_E_MEMORY; }
     }
     e = expr_mode ? compress_write(c, *ep, n, cols, nullptr, toff) : compress_write(c, MaskPred{m}, n, cols, nullptr, toff);
+    if(e) R e;
+    if(spec){
+        if((e = d2h(c, &kk, toff + ceil_div(n, CP_TILE), 8))) R e;
+        if(kk == n && !expr_mode){                          // copyV alias (h_primitive.cc:1513-1515)
+            ps.stop(); ps.bytes = (double)n;
+            c->dcur = mark;
+            for(int k = 0; k < ny; k++) *z[k] = *y[k];
+            R 0;
+        }
+        c->dcur = (L)(cols.dst[ny - 1] - c->dbase) + ((type_bytes(y[ny - 1]->typ, kk > 0 ? kk : 1) + 255) & ~(L)255);
+    }
     ps.stop();
     ps.bytes = wcond * (double)n + wsum * (double)kk;
-    if(e) R e;
     for(int k = 0; k < ny; k++){
         V zk = z[k];
         zk->typ = y[k]->typ; zk->len = kk;
@@ -311,10 +329,16 @@ extern "C" int32_t pfnWhere(hpa_V z, hpa_V x){
     const uint8_t *m = (const uint8_t*)p;
     ProfScope ps("where", c->stream);
     int64_t kk = 0, *toff = nullptr;
-    if((e = compress_count(c, MaskPred{m}, m, n, &toff, &kk))){ c->dcur = mark; R e; }
-    int64_t *out = (int64_t*)dalloc(c, 8 * (kk > 0 ? kk : 1)); if(!out){ c->dcur = mark; R HPA_E_MEMORY; }
+    // as in compress: with room for n indices the write pass runs while the host reads the count
+    const bool spec = n > 0 && c->dcur + 8 * n + 4096 + 16 * ceil_div(n, CP_TILE) <= c->dtop;
+    if((e = compress_count(c, MaskPred{m}, m, n, &toff, spec ? nullptr : &kk))){ c->dcur = mark; R e; }
+    int64_t *out = (int64_t*)dalloc(c, 8 * (spec ? n : (kk > 0 ? kk : 1))); if(!out){ c->dcur = mark; R HPA_E_MEMORY; }
     CpCols cols; cols.n = 0;
     if((e = compress_write(c, MaskPred{m}, n, cols, out, toff))) R e;
+    if(spec){
+        if((e = d2h(c, &kk, toff + ceil_div(n, CP_TILE), 8))) R e;
+        c->dcur = (L)((char*)out - c->dbase) + ((8 * (kk > 0 ? kk : 1) + 255) & ~(L)255);
+    }
     ps.stop(); ps.bytes = (double)n + 8.0 * (double)kk;
     z->typ = HPA_L; z->len = kk;
     note_bound(c, out, n);
