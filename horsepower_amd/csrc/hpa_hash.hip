@@ -85,11 +85,9 @@ __global__ void k_pk_insert(const int32_t *x, L n, unsigned long long *slots, ui
         const unsigned long long want = ((unsigned long long)k << 32) | (unsigned long long)(uint32_t)i;
         uint64_t h = mix64(k) & mask;
         while(true){
-            unsigned long long cur = slots[h];
-            if(cur == PK_EMPTY){
-                cur = atomicCAS(&slots[h], PK_EMPTY, want);
-                if(cur == PK_EMPTY) break;
-            }
+            // the CAS returns the slot's content: no separate read before it
+            const unsigned long long cur = atomicCAS(&slots[h], PK_EMPTY, want);
+            if(cur == PK_EMPTY) break;
             if((uint32_t)(cur >> 32) == k){ d = 1; atomicMin(&slots[h], want); break; }
             h = (h + 1) & mask;
         }

@@ -622,6 +622,7 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
          "extern \"C\" __global__ void __launch_bounds__(1024) fga_merge(M m){\n"
          "    __shared__ unsigned bits[2048];\n"
          "    __shared__ unsigned pre[1024];\n"
+         "    __shared__ u64 mc[16]; __shared__ i64 mfs[16]; __shared__ double mf[16][2 * NA + 1]; __shared__ i64 mq[16][NA + 1];\n"
          "    const P *part = (const P*)m.part;\n"
          "    for(int k = threadIdx.x; k < 2048; k += 1024) bits[k] = 0;\n"
          "    __syncthreads();\n"
@@ -638,8 +639,14 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
          "    __syncthreads();\n"
          "    __threadfence_block();\n"
          "    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;\n"
-         "    for(int r = wv; r < g; r += 16){\n"
-         "        const unsigned code = m.codes[r];\n"
+         "    // W waves per group (16 / W groups at a time); each wave takes every W-th run of 64\n"
+         "    // partial blocks, reduces it with shuffles, and the group's first wave merges the W\n"
+         "    // wave results in wave order: a fixed association, so runs give identical sums\n"
+         "    const int W = g >= 16 ? 1 : (g >= 8 ? 2 : (g >= 4 ? 4 : (g >= 2 ? 8 : 16)));\n"
+         "    const int sub = wv % W;\n"
+         "    for(int r0 = 0; r0 < g; r0 += 16 / W){\n"
+         "        const int r = r0 + wv / W;\n"
+         "        const unsigned code = r < g ? m.codes[r] : 0xFFFFFFFFu;\n"
          "        u64 c = 0; i64 fs = 9223372036854775807LL;\n";
     for(int k = 0; k < (int)acc.size(); k++){
         if(acc[k].f && acc[k].cls == 0) s << "        KS t" << k << "; t" << k << ".s = 0; t" << k << ".c = 0;\n";
@@ -647,7 +654,7 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
         else if(acc[k].cls == 0) s << "        u64 t" << k << " = 0;\n";
         else s << "        i64 t" << k << " = " << (acc[k].cls == 2 ? "9223372036854775807LL" : "(-9223372036854775807LL-1)") << ";\n";
     }
-    s << "        for(int b = lane; b < m.G; b += 64){\n"
+    s << "        for(int b = sub * 64 + lane; r < g && b < m.G; b += 64 * W){\n"
          "            unsigned u = m.used[b];\n"
          "            for(unsigned sidx = 0; sidx < u; sidx++){\n"
          "                const P &p = part[(i64)b * NS + sidx];\n"
@@ -669,14 +676,32 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
         else s << "            { i64 u = __shfl_down(t" << k << ", o, 64); t" << k << " = u " << (acc[k].cls == 2 ? "<" : ">") << " t" << k << " ? u : t" << k << "; }\n";
     }
     s << "        }\n"
-         "        if(lane == 0){\n"
+         "        if(lane == 0){ mc[wv] = c; mfs[wv] = fs;";
+    for(int k = 0; k < (int)acc.size(); k++){
+        if(acc[k].f && acc[k].cls == 0) s << " mf[wv][2*" << k << "] = t" << k << ".s; mf[wv][2*" << k << "+1] = t" << k << ".c;";
+        else if(acc[k].f) s << " mf[wv][2*" << k << "] = t" << k << ";";
+        else s << " mq[wv][" << k << "] = (i64)t" << k << ";";
+    }
+    s << " }\n"
+         "        __syncthreads();\n"
+         "        if(lane == 0 && sub == 0 && r < g){\n"
+         "            for(int q = wv + 1; q < wv + W; q++){\n"
+         "                c += mc[q]; fs = mfs[q] < fs ? mfs[q] : fs;\n";
+    for(int k = 0; k < (int)acc.size(); k++){
+        if(acc[k].f && acc[k].cls == 0) s << "                { KS u; u.s = mf[q][2*" << k << "]; u.c = mf[q][2*" << k << "+1]; ks_merge(t" << k << ", u); }\n";
+        else if(acc[k].f) s << "                { double u = mf[q][2*" << k << "]; t" << k << " = u " << (acc[k].cls == 2 ? "<" : ">") << " t" << k << " ? u : t" << k << "; }\n";
+        else if(acc[k].cls == 0) s << "                t" << k << " += (u64)mq[q][" << k << "];\n";
+        else s << "                { i64 u = mq[q][" << k << "]; t" << k << " = u " << (acc[k].cls == 2 ? "<" : ">") << " t" << k << " ? u : t" << k << "; }\n";
+    }
+    s << "            }\n"
          "            m.cnt[r] = c; m.first[r] = fs;\n";
     for(int k = 0; k < (int)acc.size(); k++){
         if(acc[k].f && acc[k].cls == 0) s << "            m.f[(i64)r * NA + " << k << "] = t" << k << ".s + t" << k << ".c;\n";
         else if(acc[k].f) s << "            m.f[(i64)r * NA + " << k << "] = t" << k << ";\n";
         else s << "            m.q[(i64)r * NA + " << k << "] = (i64)t" << k << ";\n";
     }
-    s << "        }\n    }\n}\n";
+    s << "        }\n        __syncthreads();\n";
+    s << "    }\n}\n";
     R s.str();
 }
 
@@ -801,18 +826,23 @@ extern "C" int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *
         struct { const void *part; const unsigned *used; int G; unsigned *codes; int *g; unsigned long long *cnt; long long *first; double *f; long long *q; } ma;
         ma.part = part; ma.used = usedb; ma.G = G; ma.codes = codes; ma.g = gdev; ma.cnt = cnt; ma.first = first; ma.f = fo; ma.q = qo;
         if((e = jit_launch(c, f2, 1, 1024, ma))) R e;
+        // the typed outputs are sized for every possible group, so they are written right behind
+        // the merge and one read brings back the overflow flag and the group count
+        if(!o.key0){
+            o.key0 = (uint8_t*)dalloc(c, GM); o.key1 = (uint8_t*)dalloc(c, GM);
+            o.first = (int64_t*)dalloc(c, 8 * GM);
+            if(!o.key0 || !o.key1 || !o.first){ c->dcur = mark; R HPA_E_MEMORY; }
+            for(int a = 0; a < na; a++){ o.out[a] = dalloc(c, 8 * GM); if(!o.out[a]){ c->dcur = mark; R HPA_E_MEMORY; } }
+        }
+        hipLaunchKernelGGL(k_fga_emit, dim3(grid_for(GM, 64, 1)), dim3(64), 0, c->stream, gdev, codes, cnt, first, fo, qo, o);
+        if((e = check_launch("fga_emit"))) R e;
         if((e = d2h(c, hv, ovf, 8))) R e;
         if(!hv[0]) break;
     }
     if(hv[0]){ c->dcur = mark; R HPA_E_NOT_IMPL; }
     const L g = hv[1];
-    uint8_t *k0 = (uint8_t*)dalloc(c, g > 0 ? g : 1), *k1 = (uint8_t*)dalloc(c, g > 0 ? g : 1);
-    int64_t *zf = (int64_t*)dalloc(c, 8 * (g > 0 ? g : 1));
-    if(!k0 || !k1 || !zf) R HPA_E_MEMORY;
-    o.key0 = k0; o.key1 = k1; o.first = zf;
-    for(int a = 0; a < na; a++){ o.out[a] = dalloc(c, 8 * (g > 0 ? g : 1)); if(!o.out[a]) R HPA_E_MEMORY; }
-    if(g > 0) hipLaunchKernelGGL(k_fga_emit, dim3(grid_for(g, 64, 1)), dim3(64), 0, c->stream, gdev, codes, cnt, first, fo, qo, o);
-    if((e = check_launch("fga_emit"))) R e;
+    uint8_t *k0 = o.key0, *k1 = o.key1;
+    int64_t *zf = o.first;
     set_dev_vec(c, zkeys[0], keys[0]->typ, g, k0);
     if(nk == 2) set_dev_vec(c, zkeys[1], keys[1]->typ, g, k1);
     if(zfirst){ set_dev_vec(c, zfirst, HPA_L, g, zf); note_bound(c, zf, n); }

@@ -407,22 +407,38 @@ extern "C" int32_t hpaReadPayload(hpa_V x, void *dst, int64_t bytes){
 }
 
 // Copy the payloads of n vectors into caller memory with one stream synchronisation.
+// Small device ranges copied into the pinned staging buffer by one kernel (the buffer is
+// device-accessible host memory): one launch and one synchronisation for all of them, instead of
+// one copy command each.  Workgroup k copies entry k, 8 bytes a thread when both ends allow.
+#define RM_MAX 32
+struct RmList { const char *src[RM_MAX]; char *dst[RM_MAX]; int64_t bytes[RM_MAX]; };
+__global__ void __launch_bounds__(256) k_read_many(RmList r){
+    const char *s = r.src[blockIdx.x]; char *d = r.dst[blockIdx.x]; const L nb = r.bytes[blockIdx.x];
+    const bool w8 = ((uintptr_t)s % 8 == 0) && ((uintptr_t)d % 8 == 0);
+    L i = (L)threadIdx.x * 8;
+    if(w8) for(; i + 8 <= nb; i += 256 * 8) *(uint64_t*)(d + i) = *(const uint64_t*)(s + i);
+    for(L j = (w8 ? (nb & ~(L)7) : 0) + threadIdx.x; j < nb; j += 256) d[j] = s[j];
+}
 extern "C" int32_t hpaReadMany(int32_t n, hpa_V *x, void **dst, const int64_t *bytes){
     hpa_ctx *c = ctx_cur();
     // small results (the usual case: aggregates over groups) go through the pinned staging
-    // buffer, so the copies are truly asynchronous and one synchronisation covers them all
+    // buffer: one kernel fills it and one synchronisation covers every read
     L dev_total = 0;
     for(int k = 0; k < n; k++)
         if(bytes[k] > 0 && !is_inline(x[k]) && is_device(x[k]->g)) dev_total += (bytes[k] + 15) & ~(L)15;
     if(c->pinned && dev_total > 0 && dev_total <= c->pinned_bytes){
         L off = 0;
         std::vector<L> at(n, -1);
+        RmList rl; int nr = 0;
         for(int k = 0; k < n; k++){
             if(bytes[k] <= 0) continue;
             if(is_inline(x[k]) || !is_device(x[k]->g)){ memcpy(dst[k], x[k]->g, (size_t)bytes[k]); continue; }
-            if(hipMemcpyAsync(c->pinned + off, x[k]->g, (size_t)bytes[k], hipMemcpyDeviceToHost, c->stream) != hipSuccess) R HPA_E_DEVICE;
+            rl.src[nr] = x[k]->g; rl.dst[nr] = c->pinned + off; rl.bytes[nr] = bytes[k]; nr++;
             at[k] = off; off += (bytes[k] + 15) & ~(L)15;
+            if(nr == RM_MAX){ hipLaunchKernelGGL(k_read_many, dim3(nr), dim3(256), 0, c->stream, rl); nr = 0; }
         }
+        if(nr) hipLaunchKernelGGL(k_read_many, dim3(nr), dim3(256), 0, c->stream, rl);
+        if(hipGetLastError() != hipSuccess) R HPA_E_DEVICE;
         if(hipStreamSynchronize(c->stream) != hipSuccess) R HPA_E_DEVICE;
         for(int k = 0; k < n; k++) if(at[k] >= 0) memcpy(dst[k], c->pinned + at[k], (size_t)bytes[k]);
         R 0;
