@@ -654,10 +654,14 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
         else if(acc[k].cls == 0) s << "        u64 t" << k << " = 0;\n";
         else s << "        i64 t" << k << " = " << (acc[k].cls == 2 ? "9223372036854775807LL" : "(-9223372036854775807LL-1)") << ";\n";
     }
-    s << "        for(int b = sub * 64 + lane; r < g && b < m.G; b += 64 * W){\n"
-         "            unsigned u = m.used[b];\n"
-         "            for(unsigned sidx = 0; sidx < u; sidx++){\n"
-         "                const P &p = part[(i64)b * NS + sidx];\n"
+    // partial slots taken in a flat order (slot i = block i / NS, entry i % NS): the loads of
+    // successive iterations do not depend on each other and overlap
+    s << "        const int SL = m.G * NS;\n"
+         "        #pragma unroll 4\n"
+         "        for(int i = sub * 64 + lane; i < SL; i += 64 * W){\n"
+         "            if(r >= g || (unsigned)(i % NS) >= m.used[i / NS]) continue;\n"
+         "            {\n"
+         "                const P &p = part[i];\n"
          "                if(p.code != code) continue;\n"
          "                c += p.cnt; fs = p.first < fs ? p.first : fs;\n";
     for(int k = 0; k < (int)acc.size(); k++){
@@ -739,6 +743,9 @@ static void set_dev_vec(hpa_ctx *c, V z, int32_t typ, L len, void *p){
 
 // hpaFusedGroupAgg: see the public header.  Returns HPA_E_NOT_IMPL (nothing written) when a
 // block meets more distinct key codes than it has slots; the caller then runs the unfused plan.
+// prepared inputs of one distinct fused group-aggregate request (see hpaFusedGroupAgg)
+struct FgaPrep { CExpr ce; Konst kc; std::vector<Acc> acc; std::vector<int> used; FgaOut o; };
+static std::map<std::string, FgaPrep> g_fga_prep;
 extern "C" int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_expr *cond,
                                     int32_t na, const hpa_agg *aggs, hpa_V zfirst, hpa_V *zkeys, hpa_V *zaggs){
     hpa_ctx *c = ctx_cur();
@@ -755,31 +762,56 @@ extern "C" int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *
     if((e = cols_view(c, nx, x, &cols, &nn))) R e;
     if(nx > 0 && nn != n) R HPA_E_LENGTH;
     int32_t typ[EXPR_MAX_COLS]; for(int k = 0; k < nx; k++) typ[k] = x[k]->typ;
-    std::vector<int> uc(nx, 0), ub(nx, 0);
-    CExpr ce{COND_TRUE, false};
-    Konst kc;
-    if(cond && (e = expr_to_c(cond, nx, typ, &ce, uc, &kc))) R e;
-    std::vector<Acc> acc;
-    std::vector<std::string> acc_key;                        // bodies with literal constants
-    FgaOut o; memset(&o, 0, sizeof(o));
-    o.na = na; o.nk = nk;
-    for(int a = 0; a < na; a++){
-        int kd = aggs[a].kind;
-        if(kd < 0 || kd > 4) R HPA_E_DOMAIN;
-        o.kind[a] = kd; o.accid[a] = 0; o.accf[a] = 0;
-        if(kd == 4) continue;
-        CExpr lit; if((e = expr_to_c(aggs[a].body, nx, typ, &lit, ub))) R e;
-        int cls = (kd == 0 || kd == 1) ? 0 : kd;
-        int id = -1;
-        for(int k = 0; k < (int)acc.size(); k++) if(acc_key[k] == lit.s && acc[k].cls == cls){ id = k; break; }
-        if(id < 0){
-            CExpr be; if((e = expr_to_c(aggs[a].body, nx, typ, &be, ub, &kc))) R e;
-            id = (int)acc.size(); acc.push_back({be.s, be.f, cls}); acc_key.push_back(lit.s);
-        }
-        o.accid[a] = id; o.accf[a] = lit.f ? 1 : 0;
+    // The generated-code inputs (condition and accumulator sources, constants, output layout)
+    // depend only on the request's bytes: they are prepared once per distinct request and reused.
+    std::string raw;
+    raw.append((const char*)&nk, 4); raw.append((const char*)&nx, 4); raw.append((const char*)typ, 4 * nx);
+    auto add_expr = [&](const hpa_expr *ex){
+        int32_t m = ex ? ex->n : -1; raw.append((const char*)&m, 4);
+        if(ex && m > 0) raw.append((const char*)ex->code, sizeof(hpa_ins) * (size_t)m);
+    };
+    add_expr(cond);
+    raw.append((const char*)&na, 4);
+    for(int a = 0; a < na; a++){ raw.append((const char*)&aggs[a].kind, 4); add_expr(aggs[a].kind == 4 ? nullptr : aggs[a].body); }
+    FgaPrep prep;
+    bool have = false;
+    {
+        std::lock_guard<std::mutex> lk(g_jit_mu);
+        auto it = g_fga_prep.find(raw);
+        if(it != g_fga_prep.end()){ prep = it->second; have = true; }
     }
-    std::vector<int> used(nx, 0);
-    for(int k = 0; k < nx; k++) used[k] = uc[k] | ub[k];
+    if(!have){
+        std::vector<int> uc(nx, 0), ub(nx, 0);
+        prep.ce = CExpr{COND_TRUE, false};
+        if(cond && (e = expr_to_c(cond, nx, typ, &prep.ce, uc, &prep.kc))) R e;
+        std::vector<std::string> acc_key;                    // bodies with literal constants
+        memset(&prep.o, 0, sizeof(prep.o));
+        prep.o.na = na; prep.o.nk = nk;
+        for(int a = 0; a < na; a++){
+            int kd = aggs[a].kind;
+            if(kd < 0 || kd > 4) R HPA_E_DOMAIN;
+            prep.o.kind[a] = kd; prep.o.accid[a] = 0; prep.o.accf[a] = 0;
+            if(kd == 4) continue;
+            CExpr lit; if((e = expr_to_c(aggs[a].body, nx, typ, &lit, ub))) R e;
+            int cls = (kd == 0 || kd == 1) ? 0 : kd;
+            int id = -1;
+            for(int k = 0; k < (int)prep.acc.size(); k++) if(acc_key[k] == lit.s && prep.acc[k].cls == cls){ id = k; break; }
+            if(id < 0){
+                CExpr be; if((e = expr_to_c(aggs[a].body, nx, typ, &be, ub, &prep.kc))) R e;
+                id = (int)prep.acc.size(); prep.acc.push_back({be.s, be.f, cls}); acc_key.push_back(lit.s);
+            }
+            prep.o.accid[a] = id; prep.o.accf[a] = lit.f ? 1 : 0;
+        }
+        prep.used.assign(nx, 0);
+        for(int k = 0; k < nx; k++) prep.used[k] = uc[k] | ub[k];
+        std::lock_guard<std::mutex> lk(g_jit_mu);
+        g_fga_prep[raw] = prep;
+    }
+    const CExpr &ce = prep.ce;
+    const Konst &kc = prep.kc;
+    const std::vector<Acc> &acc = prep.acc;
+    const std::vector<int> &used = prep.used;
+    FgaOut o = prep.o;
     if(!cols_aligned(nx, typ, used, cols)) R HPA_E_NOT_IMPL;
     const int NA = acc.empty() ? 1 : (int)acc.size();
     o.stride = NA;

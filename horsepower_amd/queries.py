@@ -8,6 +8,8 @@ grouped reductions (optimizer/autofusion.cc:770-935, optimizer/pattern.cc:180-21
 src/horseir/v1/compile-static/opt/q6.h:31-48, cur/q1.h) -- on the HIP fused entry points.
 Both take a backend (backend.Backend) and a dict of column nodes, and return numpy results.
 """
+import functools
+
 import numpy as np
 
 from . import datagen as D
@@ -40,6 +42,7 @@ def q6_plain(B, T, lo=D.Q6_LO, hi=D.Q6_HI, dlo=0.05, dhi=0.07, qmax=24):
     return {"revenue": B.read_scalar(t44)}
 
 
+@functools.lru_cache(maxsize=256)
 def q6_cond(lo=D.Q6_LO, hi=D.Q6_HI, dlo=0.05, dhi=0.07, qmax=24):
     """FA condition over columns (ship_day, quantity, discount, price)."""
     return expr([("COL", 0), ("I", lo), "GEQ", ("COL", 0), ("I", hi), "LT", "AND",
@@ -127,10 +130,12 @@ def q1_fused(B, T, ship_max=D.Q1_SHIP_MAX):
     return dict(zip(Q1_NAMES, B.read_many(B.index_many([kf, ks] + aggs, perm))))
 
 
+@functools.lru_cache(maxsize=64)
 def q1_chf_spec(ship_max=D.Q1_SHIP_MAX):
     """Columns, condition and aggregates of the one-pass Q1 (x = ship_day, quantity, price,
     discount, tax).  disc_price = price*(1-discount), charge = (price*(1-discount))*(1+tax),
-    the association of q1.hir's t36 and t40."""
+    the association of q1.hir's t36 and t40.  Built once per constant (the expression programs
+    are read-only)."""
     cond = expr([("COL", 0), ("I", ship_max), "LEQ"])
     qty = expr([("COL", 1)])
     price = expr([("COL", 2)])
