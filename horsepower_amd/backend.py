@@ -12,6 +12,15 @@ import numpy as np
 from .runtime import Host, cell, load
 from .vnode import H_B, H_E, H_G, H_I, H_L, H_N, NP_OF, V, V0, Expr
 
+_DTYPES = {}
+
+
+def _dtype_of(typ):
+    dt = _DTYPES.get(typ)
+    if dt is None:
+        dt = _DTYPES[typ] = np.dtype(NP_OF[typ])
+    return dt
+
 FUNC_ADDR_NAMES = ("pfnSum", "pfnAvg", "pfnMin", "pfnMax", "pfnLen", "pfnIndex")
 
 
@@ -145,6 +154,7 @@ class GpuBackend(Backend):
     def __init__(self, lib=None):
         super().__init__(lib or load())
         self.dev = self.lib
+        self._aggs = {}
 
     def node(self):
         return self.lib.node()
@@ -157,13 +167,22 @@ class GpuBackend(Backend):
         self.lib.rc("hpaReadPayload", z, C.c_void_p(dst), nbytes)
 
     def read_many(self, zs):
-        """Vector nodes -> numpy arrays with one device synchronisation."""
-        outs = [np.empty(z.contents.len, dtype=np.dtype(NP_OF[z.contents.typ])) for z in zs]
+        """Vector nodes -> numpy arrays with one device synchronisation.  The arrays are views of
+        one host buffer (16-byte aligned slices), so the call costs one allocation."""
         n = len(zs)
-        dst = (C.c_void_p * n)(*[o.ctypes.data for o in outs])
-        nb = (C.c_int64 * n)(*[o.nbytes for o in outs])
+        cs = [z.contents for z in zs]
+        dts = [_dtype_of(v.typ) for v in cs]
+        sizes = [v.len * dt.itemsize for v, dt in zip(cs, dts)]
+        offs, tot = [], 0
+        for sz in sizes:
+            offs.append(tot)
+            tot += (sz + 15) & ~15
+        buf = np.empty(max(tot, 16), dtype=np.uint8)
+        base = buf.__array_interface__["data"][0]
+        dst = (C.c_void_p * n)(*[base + o for o in offs])
+        nb = (C.c_int64 * n)(*sizes)
         self.lib.rc("hpaReadMany", n, (V * n)(*zs), dst, nb)
-        return outs
+        return [buf[o:o + sz].view(dt) for o, sz, dt in zip(offs, sizes, dts)]
 
     def symbol(self, name):
         return self.lib.symbol(name)
@@ -237,7 +256,11 @@ class GpuBackend(Backend):
         zk = [self.node() for _ in range(nk)]
         za = [self.node() for _ in range(na)]
         zf = self.node()
-        ag = aggs(spec)
+        hit = self._aggs.get(id(spec))               # descriptor arrays of a spec seen before
+        if hit is None or hit[0] is not spec:
+            hit = (spec, aggs(spec))
+            self._aggs[id(spec)] = hit
+        ag = hit[1]
         rc = self.lib.dll.hpaFusedGroupAgg(nk, (V * nk)(*keys), len(xs), (V * len(xs))(*xs),
                                            C.byref(cond) if cond is not None else None, na,
                                            C.cast(ag, C.c_void_p), zf, (V * nk)(*zk), (V * na)(*za))

@@ -473,6 +473,19 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
     s << PREAMBLE;
     s << "#define NS " << ns << "\n#define NA " << NA << "\n#define HT 64\n#define U " << U << "\n";
     s << "template<typename T> struct __attribute__((aligned(U * sizeof(T)))) VU { T v[U]; };\n"
+         // streaming column loads marked nontemporal: each row is read once
+         "typedef unsigned nt4 __attribute__((ext_vector_type(4)));\n"
+         "typedef unsigned nt2 __attribute__((ext_vector_type(2)));\n"
+         "template<typename T> __device__ __forceinline__ VU<T> ldnt(const T *p){\n"
+         "    VU<T> r;\n"
+         "    if(sizeof(VU<T>) % 16 == 0){\n"
+         "        #pragma unroll\n"
+         "        for(int k = 0; k < (int)(sizeof(VU<T>) / 16); k++){ nt4 v = __builtin_nontemporal_load((const nt4*)p + k); __builtin_memcpy((char*)&r + 16 * k, &v, 16); }\n"
+         "    } else if(sizeof(VU<T>) == 8){ nt2 v = __builtin_nontemporal_load((const nt2*)p); __builtin_memcpy(&r, &v, 8); }\n"
+         "    else if(sizeof(VU<T>) == 4){ unsigned v = __builtin_nontemporal_load((const unsigned*)p); __builtin_memcpy(&r, &v, 4); }\n"
+         "    else r = *(const VU<T>*)p;\n"
+         "    return r;\n"
+         "}\n"
          "struct A { const void *c[" << EXPR_MAX_COLS << "]; const void *k0; const void *k1; i64 n; void *part; unsigned *used; int *ovf; " KONST_FIELDS " };\n"
          "struct P { unsigned code, valid; u64 cnt; i64 first; double f[2 * NA]; i64 q[NA]; };\n"
          // LDS hash of key code -> slot: entry = code << 8 | slot; 0xFFFFFFFF empty; slot 0xFE busy, 0xFD overflow
@@ -552,9 +565,9 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
          "        if(base + U <= a.n){\n";
     // vector loads of U consecutive rows
     for(int k = 0; k < nx; k++) if(used[k])
-        s << "            const VU<" << ctype(typ[k]) << "> w" << k << " = *(const VU<" << ctype(typ[k]) << ">*)((const " << ctype(typ[k]) << "*)a.c[" << k << "] + base);\n";
-    s << "            const VU<unsigned char> wk0 = *(const VU<unsigned char>*)((const unsigned char*)a.k0 + base);\n";
-    if(nk == 2) s << "            const VU<unsigned char> wk1 = *(const VU<unsigned char>*)((const unsigned char*)a.k1 + base);\n";
+        s << "            const VU<" << ctype(typ[k]) << "> w" << k << " = ldnt<" << ctype(typ[k]) << ">((const " << ctype(typ[k]) << "*)a.c[" << k << "] + base);\n";
+    s << "            const VU<unsigned char> wk0 = ldnt<unsigned char>((const unsigned char*)a.k0 + base);\n";
+    if(nk == 2) s << "            const VU<unsigned char> wk1 = ldnt<unsigned char>((const unsigned char*)a.k1 + base);\n";
     for(int u = 0; u < U; u++){
         s << "          {\n            const unsigned off = it * U + " << u << ";\n";
         for(int k = 0; k < nx; k++) if(used[k]) s << "            const " << ctype(typ[k]) << " v" << k << " = w" << k << ".v[" << u << "];\n";
@@ -816,8 +829,10 @@ extern "C" int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *
     const int NA = acc.empty() ? 1 : (int)acc.size();
     o.stride = NA;
     const size_t psize = 8 + 8 + 8 + 16 * NA + 8 * NA;     // struct P
+    // two workgroups per CU: measured on MI355X (Q1, 60M rows) against 1, 3, 4, 6, 8 and 12,
+    // it streams fastest, and its partials keep the merge short
     int G = (int)ceil_div(n, 256 * FGA_ROWS * 8);
-    if(G > num_cus() * 4) G = num_cus() * 4;
+    if(G > num_cus() * 2) G = num_cus() * 2;
     if(G < 1) G = 1;
     const void *pk0 = nullptr, *pk1 = nullptr;
     if(n > 0){ if((e = dev_view(c, keys[0], &pk0))) R e; if(nk == 2 && (e = dev_view(c, keys[1], &pk1))) R e; }
