@@ -417,15 +417,17 @@ def bench_serve(a):
     from horsepower_amd import serve as S
     from horsepower_amd.backend import GpuBackend
     n_dim = a.rows or 150_000_000
-    # the one-at-a-time path runs on the default context: its arena must hold a Q3 over 600M rows
-    os.environ.setdefault("HPA_ARENA_BYTES", str(120 << 30))
+    # the one-at-a-time paths run on the default context: its arena must hold the interpreter's
+    # Q1 over 600M rows (about 50 GiB); the serving pool takes most of the rest of the 288 GB next
+    # to the 24.6 GB of tables, room for three Q3s at once (about 39 GiB each)
+    os.environ.setdefault("HPA_ARENA_BYTES", str(96 << 30))
     B = GpuBackend()
     t0 = time.time()
     dim, fact = B.gen_serve_tables(n_dim, 1004)
     T = {"dim": dim, "fact": fact}
     nf = fact["fk"].contents.len
     print(f"[bench] serve tables: {n_dim} orders, {nf} lineitems in HBM ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
-    srv = S.Server(B, T, slots=a.slots, pool_bytes=80 << 30, merge=True)
+    srv = S.Server(B, T, slots=a.slots, pool_bytes=128 << 30, merge=True)
     est = srv.calibrate()
     print(f"[bench] calibrated arena estimates (MiB): { {k: v >> 20 for k, v in est.items()} }", file=sys.stderr, flush=True)
     # service time of the per-query fused path sets the offered load

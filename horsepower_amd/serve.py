@@ -130,13 +130,18 @@ class FirstFit:
 
 
 class Server:
-    def __init__(self, B, tables, slots=4, pool_bytes=64 << 30, merge=True, max_skip_s=0.05):
+    def __init__(self, B, tables, slots=4, pool_bytes=64 << 30, merge=True, max_skip_s=0.05,
+                 heavy=("q3",), heavy_slots=1):
         self.B, self.lib, self.T = B, B.lib, tables
         self.slots, self.merge, self.max_skip_s = slots, merge, max_skip_s
         self.base = self.lib.dll.hpaDeviceAlloc(pool_bytes)
         if not self.base:
             raise HorseError(E_MEMORY, "hpaDeviceAlloc", self.lib.detail())
         self.pool_bytes = pool_bytes
+        # heavy requests (whole-table joins and groups) run at most heavy_slots at a time, in
+        # arrival order: two of them sharing the device would each take about twice as long,
+        # while short requests keep filling the other slots
+        self.heavy, self.heavy_slots = tuple(heavy), heavy_slots
         self.ctx = [self.lib.dll.hpaCtxCreate(-1) for _ in range(slots)]
         if not all(self.ctx):
             raise HorseError(E_MEMORY, "hpaCtxCreate", self.lib.detail())
@@ -196,7 +201,7 @@ class Server:
         cv = threading.Condition()
         free_slots = list(range(self.slots))
         pending = collections.deque()
-        state = dict(done=0, error=None)
+        state = dict(done=0, error=None, heavy=0)
         N = len(reqs)
         nxt = 0
         t0 = time.perf_counter()
@@ -217,6 +222,8 @@ class Server:
             with cv:
                 pool.release(off, size)
                 free_slots.append(slot)
+                if job[0].kind in self.heavy:
+                    state["heavy"] -= 1
                 if err is None:
                     state["done"] += len(job)
                 elif err.code == E_MEMORY:
@@ -239,13 +246,18 @@ class Server:
                     nxt += 1
                 while free_slots and pending:
                     pick = None
+                    head = None                              # oldest request the lane allows
                     for i, r in enumerate(pending):
-                        if i > 0 and now - pending[0].arrival > self.max_skip_s:
-                            break
+                        if r.kind in self.heavy and state["heavy"] >= self.heavy_slots:
+                            continue
+                        if head is not None and now - head.arrival > self.max_skip_s:
+                            break                            # it waits for memory: no overtaking
                         got = pool.alloc(r.est)
                         if got is not None:
                             pick = (i, got)
                             break
+                        if head is None:
+                            head = r
                     if pick is None:
                         break
                     i, (off, size) = pick
@@ -261,6 +273,8 @@ class Server:
                                 job.append(q)
                     for q in job:
                         q.admit = now
+                    if r.kind in self.heavy:
+                        state["heavy"] += 1
                     self.peak_in_use = max(self.peak_in_use, pool.in_use())
                     slot = free_slots.pop()
                     th = threading.Thread(target=worker, args=(slot, job, off, size), daemon=True)
