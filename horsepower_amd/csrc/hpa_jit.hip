@@ -486,7 +486,7 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
          "    else r = *(const VU<T>*)p;\n"
          "    return r;\n"
          "}\n"
-         "struct A { const void *c[" << EXPR_MAX_COLS << "]; const void *k0; const void *k1; i64 n; void *part; unsigned *used; int *ovf; " KONST_FIELDS " };\n"
+         "struct A { const void *c[" << EXPR_MAX_COLS << "]; const void *k0; const void *k1; i64 n; void *part; unsigned *used; int *ovf; unsigned kx0, kx1; " KONST_FIELDS " };\n"
          "struct P { unsigned code, valid; u64 cnt; i64 first; double f[2 * NA]; i64 q[NA]; };\n"
          // LDS hash of key code -> slot: entry = code << 8 | slot; 0xFFFFFFFF empty; slot 0xFE busy, 0xFD overflow
          "__device__ __forceinline__ unsigned slot_of(unsigned *ht, unsigned *s_code, unsigned *s_used, unsigned code){\n"
@@ -540,8 +540,9 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
     // one row: values v<k> of the used columns, key bytes kb0/kb1, offset `off` = it*U+u
     std::ostringstream row;
     row << "            if(" << ce.s << "){\n";
-    if(nk == 2) row << "                const unsigned code = ((unsigned)kb0 << 8) | (unsigned)kb1;\n";
-    else row << "                const unsigned code = (unsigned)kb0;\n";
+    // signed key types have their sign bit flipped, so ascending codes are ascending keys
+    if(nk == 2) row << "                const unsigned code = (((unsigned)kb0 ^ a.kx0) << 8) | ((unsigned)kb1 ^ a.kx1);\n";
+    else row << "                const unsigned code = (unsigned)kb0 ^ a.kx0;\n";
     row << "                const unsigned sl = slot_of(ht, s_code, &s_used, code);\n"
            "                if(sl >= NS){ __hip_atomic_store(&s_ovf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); goto fga_done; }\n";
     for(int k = 0; k < (int)acc.size(); k++) row << "                const " << (acc[k].f ? "double" : "i64") << " b" << k << " = " << acc[k].body << ";\n";
@@ -723,14 +724,14 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
 }
 
 // outputs of the merge are laid out [group][accumulator]; this kernel makes typed vectors
-struct FgaOut { int na, stride, nk; int kind[16]; int accid[16]; int accf[16]; void *out[16]; uint8_t *key0, *key1; int64_t *first; };
+struct FgaOut { int na, stride, nk; unsigned kx0, kx1; int kind[16]; int accid[16]; int accf[16]; void *out[16]; uint8_t *key0, *key1; int64_t *first; };
 __global__ void k_fga_emit(const int *gp, const unsigned *codes, const unsigned long long *cnt, const long long *first,
                            const double *f, const long long *q, FgaOut o){
     const int g = *gp;
     for(int r = blockIdx.x * blockDim.x + threadIdx.x; r < g; r += gridDim.x * blockDim.x){
         unsigned cd = codes[r];
-        if(o.nk == 2){ o.key0[r] = (uint8_t)(cd >> 8); o.key1[r] = (uint8_t)(cd & 255); }
-        else o.key0[r] = (uint8_t)cd;
+        if(o.nk == 2){ o.key0[r] = (uint8_t)((cd >> 8) ^ o.kx0); o.key1[r] = (uint8_t)((cd & 255) ^ o.kx1); }
+        else o.key0[r] = (uint8_t)(cd ^ o.kx0);
         o.first[r] = first[r];
         for(int a = 0; a < o.na; a++){
             long long at = (long long)r * o.stride + o.accid[a];
@@ -849,11 +850,15 @@ extern "C" int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *
     long long *qo = (long long*)dalloc(c, 8 * GM * NA);
     if(!part || !usedb || !ovf || !codes || !gdev || !cnt || !first || !fo || !qo){ c->dcur = mark; R HPA_E_MEMORY; }
     struct { const void *c[EXPR_MAX_COLS]; const void *k0; const void *k1; long long n; void *part; unsigned *used; int *ovf;
-             long long ki[JIT_MAX_CONST]; double kf[JIT_MAX_CONST]; } ga;
+             unsigned kx0, kx1; long long ki[JIT_MAX_CONST]; double kf[JIT_MAX_CONST]; } ga;
     memset(&ga, 0, sizeof(ga));
     for(int k = 0; k < nx; k++) ga.c[k] = cols.p[k];
     memcpy(ga.ki, kc.i, sizeof(ga.ki)); memcpy(ga.kf, kc.f, sizeof(ga.kf));
     ga.k0 = pk0; ga.k1 = pk1; ga.n = n; ga.part = part; ga.used = usedb; ga.ovf = ovf;
+    // key bytes compared as their type orders them: C and J signed, B unsigned
+    ga.kx0 = (keys[0]->typ == HPA_B) ? 0u : 0x80u;
+    ga.kx1 = (nk == 2 && keys[1]->typ != HPA_B) ? 0x80u : 0u;
+    o.kx0 = ga.kx0; o.kx1 = ga.kx1;
     double cb = nk; for(int k = 0; k < nx; k++) if(used[k]) cb += (double)type_width(typ[k]);
     // slot counts tried in turn: fewer slots keep the accumulators in fewer registers; a block
     // that meets more distinct codes than slots reports overflow and the next size reruns

@@ -249,7 +249,9 @@ int32_t hpaFusedCompress(int32_t ny, hpa_V *z, hpa_V *y, int32_t nx, hpa_V *x,
 /* Fused filter + group + aggregates in one pass over the table: the FA filter, pfnGroupTrie
  * on one or two byte-wide key columns (h_primitive.cc:2656-2681) and the FP grouped reductions
  * (optimizer/pattern.cc:180-219) of the reference fused into a single scan.  Groups come out in
- * ascending key-code order as pfnGroupTrie orders them; zkeys[k] holds key column k per group,
+ * ascending key order, lexicographic over the key columns with each byte compared as its type
+ * orders it (C and J signed, B unsigned): pfnGroupTrie's bucket order over the byte values its
+ * code defines (0..127), and the order pfnOrderBy gives the key columns; zkeys[k] holds key column k per group,
  * zfirst the first row of each group, zaggs[a] aggregate a (kind 0 sum, 1 avg, 2 min, 3 max,
  * 4 len; body unused for len).  Returns HPA_E_NOT_IMPL without writing outputs when one
  * workgroup meets more than 8 distinct key codes (the caller then runs the unfused primitives). */

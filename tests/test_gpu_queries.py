@@ -83,3 +83,22 @@ def test_fused_group_agg_overflow_and_single_key(gpu, orc):
     assert gpu.read(zc).tolist() == counts.tolist()
     sums = [math.fsum(v[g]) for g in np.split(flat, np.cumsum(counts)[:-1])]
     assert np.allclose(gpu.read(zs), sums, rtol=RTOL, atol=0)
+
+
+def test_fused_group_agg_key_order(gpu):
+    """The one-pass group emits its groups in ascending key order with char bytes compared as
+    signed values, the order pfnOrderBy gives the key columns (bytes >= 128 sort first)."""
+    from horsepower_amd.vnode import H_C, expr
+    r = np.random.default_rng(6)
+    n = 50_000
+    a = r.choice(np.array([3, 100, 200, 250], np.uint8), n)
+    b = r.choice(np.array([1, 130], np.uint8), n)
+    v = np.round(r.uniform(0, 100, n), 2)
+    _, (za, zb), (zs,) = gpu.fused_group_agg([gpu.vec(a, H_C), gpu.vec(b, H_C)], [gpu.vec(v)], None,
+                                             [("sum", expr([("COL", 0)]))])
+    got = list(zip(gpu.read(za).view(np.int8).tolist(), gpu.read(zb).view(np.int8).tolist()))
+    sa, sb = a.view(np.int8), b.view(np.int8)
+    want = sorted(set(zip(sa.tolist(), sb.tolist())))
+    assert got == want
+    sums = [math.fsum(v[(sa == pa) & (sb == pb)]) for pa, pb in want]
+    assert np.allclose(gpu.read(zs), sums, rtol=RTOL, atol=0)
