@@ -157,6 +157,141 @@ static int32_t pk_build(hpa_ctx *c, const int32_t *x, L n, unsigned long long **
     R check_launch("hash_build");
 }
 
+// ---------------------------------------------------------------- dense i32 key ranges
+// Build keys that span at most DENSE_PER_KEY slots per key (surrogate keys, dates, codes) are
+// addressed directly by k - lo: no hashing, no probe chains, and no atomics for unique keys.
+//   index_of / join: tab[k - lo] = first row of k.  Every row stores its index (any one wins),
+//     then every row whose key shows another row lowers it with atomicMin: only repeated keys
+//     pay an atomic, and the table ends at the first occurrence.  `dup` is raised then.  (A
+//     ranked form, a bitmap plus prefix counts indexing 4 bytes per distinct key, probed slower:
+//     its three dependent reads cost more than one read of the larger table.)
+//   member: one flag byte per key value (plain stores), packed into a bitmap of the range that
+//     stays in every XCD's L2 (750 KB for 6M values).
+#define DENSE_PER_KEY 8
+// smallest and largest key: per-workgroup partials part[2 b], part[2 b + 1] (16-byte loads
+// when aligned), folded by one workgroup into out[0] min, out[1] max
+__device__ __forceinline__ void i32_minmax_block(int lo, int hi, int *part){
+    __shared__ int slo[4], shi[4];
+    for(int o = 32; o > 0; o >>= 1){ int t = __shfl_xor(lo, o, 64); lo = t < lo ? t : lo; t = __shfl_xor(hi, o, 64); hi = t > hi ? t : hi; }
+    if((threadIdx.x & 63) == 0){ slo[threadIdx.x >> 6] = lo; shi[threadIdx.x >> 6] = hi; }
+    __syncthreads();
+    if(threadIdx.x == 0){
+        for(int q = 1; q < 4; q++){ lo = slo[q] < lo ? slo[q] : lo; hi = shi[q] > hi ? shi[q] : hi; }
+        part[0] = lo; part[1] = hi;
+    }
+}
+__global__ void __launch_bounds__(256) k_i32_range(const int32_t *x, L n, int vec, int *part){
+    int lo = 2147483647, hi = -2147483647 - 1;
+    const L S = (L)gridDim.x * 256;
+    const L nv = vec ? n / 4 : 0;
+    for(L i = (L)blockIdx.x * 256 + threadIdx.x; i < nv; i += S){
+        const int4 q = ((const int4*)x)[i];
+        lo = min(lo, min(min(q.x, q.y), min(q.z, q.w))); hi = max(hi, max(max(q.x, q.y), max(q.z, q.w)));
+    }
+    for(L i = 4 * nv + (L)blockIdx.x * 256 + threadIdx.x; i < n; i += S){ const int v = x[i]; lo = v < lo ? v : lo; hi = v > hi ? v : hi; }
+    i32_minmax_block(lo, hi, part + 2 * blockIdx.x);
+}
+__global__ void __launch_bounds__(256) k_i32_range_final(const int *part, int np, int *out){
+    int lo = 2147483647, hi = -2147483647 - 1;
+    for(int b = threadIdx.x; b < np; b += 256){ lo = min(lo, part[2 * b]); hi = max(hi, part[2 * b + 1]); }
+    i32_minmax_block(lo, hi, out);
+}
+// [lo, hi] of n i32 keys; *dense when they span at most DENSE_PER_KEY slots per key
+static int32_t dense_range(hpa_ctx *c, const int32_t *x, L n, int *lo, int *hi, bool *dense){
+    *dense = false;
+    if(n < 1024) R 0;                                  // small builds: the hash table is cheap
+    const int np = grid_for(ceil_div(n, 4), 256, 4, num_cus() * 4);
+    int *r = (int*)dalloc(c, 8 * (L)(np + 1)); if(!r) R HPA_E_MEMORY;
+    hipLaunchKernelGGL(k_i32_range, dim3(np), dim3(256), 0, c->stream, x, n, (uintptr_t)x % 16 == 0 ? 1 : 0, r + 2);
+    hipLaunchKernelGGL(k_i32_range_final, dim3(1), dim3(256), 0, c->stream, r + 2, np, r);
+    int32_t e;
+    if((e = check_launch("dense_range"))) R e;
+    int h[2]; if((e = d2h(c, h, r, 8))) R e;
+    *lo = h[0]; *hi = h[1];
+    const L R_ = (L)h[1] - (L)h[0] + 1;
+    *dense = R_ <= (L)DENSE_PER_KEY * n && R_ < (L)INT32_MAX;
+    R 0;
+}
+__global__ void k_dense_flag(const int32_t *x, L n, int lo, uint8_t *flag){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x) flag[(uint32_t)(x[i] - lo)] = 1;
+}
+// bits[w] bit b = flag[32 w + b] (flag zero past m); cnt[w] = its population (cnt may be null)
+__global__ void k_dense_pack(const uint8_t *flag, L m, uint32_t *bits, uint32_t *cnt, L nw){
+    for(L w = (L)blockIdx.x * blockDim.x + threadIdx.x; w < nw; w += (L)gridDim.x * blockDim.x){
+        uint32_t v = 0;
+        if(32 * w + 32 <= m){
+            const uint4 *q = (const uint4*)(flag + 32 * w);
+            const uint4 a = q[0], b = q[1];
+            const unsigned wd[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+            #pragma unroll
+            for(int k = 0; k < 32; k++) v |= ((wd[k >> 2] >> (8 * (k & 3))) & 1u) << k;
+        } else {
+            for(int k = 0; k < 32 && 32 * w + k < m; k++) v |= (uint32_t)(flag[32 * w + k] != 0) << k;
+        }
+        bits[w] = v;
+        if(cnt) cnt[w] = (uint32_t)__popc(v);
+    }
+}
+__global__ void k_dense_clear(int32_t *tab, L m){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (L)gridDim.x * blockDim.x) tab[i] = -1;
+}
+__global__ void k_dense_store(const int32_t *x, L n, int lo, int32_t *tab){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x)
+        tab[(uint32_t)(x[i] - lo)] = (int32_t)i;
+}
+__global__ void k_dense_first(const int32_t *x, L n, int lo, int32_t *tab, int *dup){
+    int d = 0;
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
+        const uint32_t k = (uint32_t)(x[i] - lo);
+        const int32_t t = tab[k];
+        if(t != (int32_t)i){ d = 1; if(t > (int32_t)i) atomicMin(&tab[k], (int32_t)i); }
+    }
+    if(__any(d) && (threadIdx.x & 63) == 0) atomicOr(dup, 1);
+}
+struct DenseTab { int lo; L m; uint32_t *bits; int32_t *tab; int *dup; };
+// first-row table (rows) or bitmap (member) over x[0..n), keys in [lo, lo + m)
+static int32_t dense_build(hpa_ctx *c, const int32_t *x, L n, int lo, L m, bool rows, DenseTab *t){
+    t->lo = lo; t->m = m; t->bits = nullptr; t->tab = nullptr;
+    t->dup = (int*)dalloc(c, 16);
+    if(!t->dup) R HPA_E_MEMORY;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, t->dup, 4, 0);
+    if(rows){
+        t->tab = (int32_t*)dalloc(c, 4 * m); if(!t->tab) R HPA_E_MEMORY;
+        hipLaunchKernelGGL(k_dense_clear, dim3(grid_for(m, 256, 4)), dim3(256), 0, c->stream, t->tab, m);
+        hipLaunchKernelGGL(k_dense_store, dim3(grid_for(n, 256, 1, 1 << 30)), dim3(256), 0, c->stream, x, n, lo, t->tab);
+        hipLaunchKernelGGL(k_dense_first, dim3(grid_for(n, 256, 1, 1 << 30)), dim3(256), 0, c->stream, x, n, lo, t->tab, t->dup);
+        R check_launch("dense_build");
+    }
+    const L nw = ceil_div(m, 32);
+    uint8_t *flag = (uint8_t*)dalloc(c, m + 32);
+    t->bits = (uint32_t*)dalloc(c, 4 * nw);
+    if(!flag || !t->bits) R HPA_E_MEMORY;
+    hipMemsetAsync(flag, 0, (size_t)(m + 32), c->stream);
+    hipLaunchKernelGGL(k_dense_flag, dim3(grid_for(n, 256, 1, 1 << 30)), dim3(256), 0, c->stream, x, n, lo, flag);
+    hipLaunchKernelGGL(k_dense_pack, dim3(grid_for(nw, 256, 1)), dim3(256), 0, c->stream, flag, m, t->bits, (uint32_t*)nullptr, nw);
+    R check_launch("dense_build");
+}
+__global__ void k_dense_probe_index(const int32_t *y, L ny, DenseTab t, int64_t *z, int64_t miss){
+    for(L j = (L)blockIdx.x * blockDim.x + threadIdx.x; j < ny; j += (L)gridDim.x * blockDim.x){
+        const int64_t d = (int64_t)y[j] - t.lo;
+        const int32_t r = (d >= 0 && d < t.m) ? t.tab[d] : -1;
+        z[j] = r >= 0 ? (int64_t)r : miss;
+    }
+}
+__global__ void k_dense_probe_join(const int32_t *y, L ny, DenseTab t, uint32_t *cnt, uint32_t *rid){
+    for(L j = (L)blockIdx.x * blockDim.x + threadIdx.x; j < ny; j += (L)gridDim.x * blockDim.x){
+        const int64_t d = (int64_t)y[j] - t.lo;
+        const int32_t r = (d >= 0 && d < t.m) ? t.tab[d] : -1;
+        cnt[j] = r >= 0 ? 1u : 0u; rid[j] = r >= 0 ? (uint32_t)r : 0u;
+    }
+}
+__global__ void k_dense_probe_member(const int32_t *y, L ny, int lo, L m, const uint32_t *bits, uint8_t *z){
+    for(L j = (L)blockIdx.x * blockDim.x + threadIdx.x; j < ny; j += (L)gridDim.x * blockDim.x){
+        const int64_t d = (int64_t)y[j] - lo;
+        z[j] = (d >= 0 && d < m) ? (uint8_t)((bits[d >> 5] >> (d & 31)) & 1u) : 0;
+    }
+}
+
 // ---------------------------------------------------------------- index_of / member build+probe
 template<typename T>
 __global__ void k_ht_insert_min(const T *x, L n, HTab t){
@@ -229,6 +364,17 @@ int32_t is_ordered_dev(hpa_ctx *c, V x, bool *ord);
 template<typename T>
 static int32_t index_of_hash(hpa_ctx *c, const T *px, L nx, const T *py, L ny, int64_t *z){
     int32_t e;
+    if(std::is_same<T, int32_t>::value && nx < (L)INT32_MAX){
+        int lo, hi; bool dense;
+        if((e = dense_range(c, (const int32_t*)px, nx, &lo, &hi, &dense))) R e;
+        if(dense){
+            DenseTab t;
+            if((e = dense_build(c, (const int32_t*)px, nx, lo, (L)hi - lo + 1, true, &t))) R e;
+            ProfScope ps("index_of", c->stream);
+            if(ny > 0) hipLaunchKernelGGL(k_dense_probe_index, dim3(grid_for(ny, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)py, ny, t, z, (int64_t)nx);
+            R check_launch("index_of");
+        }
+    }
     if(std::is_same<T, int32_t>::value && nx < (L)UINT32_MAX - 1){   // 4-byte keys: packed (key, first row) slots
         unsigned long long *slots; uint64_t mask; int *dup;
         if((e = pk_build(c, (const int32_t*)px, nx, &slots, &mask, &dup))) R e;
@@ -329,6 +475,16 @@ static int32_t member_t(hpa_ctx *c, const T *py, L ny, const T *px, L nx, uint8_
         R check_launch("member");
     }
     int32_t e;
+    if(std::is_same<T, int32_t>::value){
+        int lo, hi; bool dense;
+        if((e = dense_range(c, (const int32_t*)px, nx, &lo, &hi, &dense))) R e;
+        if(dense){
+            DenseTab t;
+            if((e = dense_build(c, (const int32_t*)px, nx, lo, (L)hi - lo + 1, false, &t))) R e;
+            if(ny > 0) hipLaunchKernelGGL(k_dense_probe_member, dim3(grid_for(ny, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)py, ny, lo, t.m, t.bits, z);
+            R check_launch("member");
+        }
+    }
     if(std::is_same<T, int32_t>::value && nx < (L)UINT32_MAX - 1){
         unsigned long long *slots; uint64_t mask; int *dup;
         if((e = pk_build(c, (const int32_t*)px, nx, &slots, &mask, &dup))) R e;
@@ -462,13 +618,19 @@ static int32_t join_run(hpa_ctx *c, V z, V bx, V py, int32_t t, int op, bool swa
     ProfScope ps("join", c->stream);
     if(nb >= 10){                                    // NUM_JOIN_LINEAR (common.h:152): hash path
         if(op != 4 || t != HPA_I) R HPA_E_NOT_IMPL;  // v3 radix hash join: i32 equality only (h_hash_v3.cc:233,270-277)
-        // unique build keys: packed table, no sort
+        // unique build keys: a dense first-row table or a packed hash table, no sort
         L mark = c->dcur;
-        unsigned long long *slots; uint64_t mask; int *dup;
-        if((e = pk_build(c, (const int32_t*)pb, nb, &slots, &mask, &dup))) R e;
+        int lo, hi; bool dense;
+        if((e = dense_range(c, (const int32_t*)pb, nb, &lo, &hi, &dense))) R e;
+        unsigned long long *slots = nullptr; uint64_t mask = 0; DenseTab dt; int *dup;
+        if(dense){ if((e = dense_build(c, (const int32_t*)pb, nb, lo, (L)hi - lo + 1, true, &dt))) R e; dup = dt.dup; }
+        else if((e = pk_build(c, (const int32_t*)pb, nb, &slots, &mask, &dup))) R e;
         int hd = 0; if((e = d2h(c, &hd, dup, 4))) R e;
         if(!hd){
-            if(np > 0) hipLaunchKernelGGL(k_pk_probe, dim3(grid_for(np, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)pp, np, slots, mask, cnt, rid);
+            if(np > 0){
+                if(dense) hipLaunchKernelGGL(k_dense_probe_join, dim3(grid_for(np, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)pp, np, dt, cnt, rid);
+                else hipLaunchKernelGGL(k_pk_probe, dim3(grid_for(np, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)pp, np, slots, mask, cnt, rid);
+            }
             if((e = scan_u32_to_i64(c, cnt, off, np, off + np))) R e;
             int64_t P = 0; if((e = d2h(c, &P, off + np, 8))) R e;
             int64_t *z0 = (int64_t*)dalloc(c, 8 * (P > 0 ? P : 1)), *z1 = (int64_t*)dalloc(c, 8 * (P > 0 ? P : 1));

@@ -328,6 +328,36 @@ def test_index_of_member(gpu, orc, nx, ny):
                               orc.read(orc.pfnMember(orc.vec(y), orc.vec(s))))
 
 
+@pytest.mark.parametrize("nb,ny", [(5000, 20000), (60000, 100000)])
+def test_hash_sparse_keys(gpu, orc, nb, ny):
+    """Build keys spread over the whole i32 range take the packed hash table (dense key ranges
+    take the direct table): index_of with repeated keys, member, and a unique-build join."""
+    r = rng(nb + ny + 3)
+    keys = r.integers(-2**31, 2**31 - 1, nb).astype(np.int32)
+    rep = np.concatenate([keys, keys[: nb // 3]])                      # first occurrences matter
+    probe = np.where(r.random(ny) < 0.5, r.choice(keys, ny), r.integers(-2**31, 2**31 - 1, ny)).astype(np.int32)
+    assert np.array_equal(gpu.read(gpu.pfnIndexOf(gpu.vec(rep), gpu.vec(probe))),
+                          orc.read(orc.pfnIndexOf(orc.vec(rep), orc.vec(probe))))
+    assert np.array_equal(gpu.read(gpu.pfnMember(gpu.vec(probe), gpu.vec(rep))),
+                          orc.read(orc.pfnMember(orc.vec(probe), orc.vec(rep))))
+    u = np.unique(keys)
+    g = gpu.read_list(gpu.pfnJoinIndex(gpu.vec(u), gpu.vec(probe), gpu.sym("eq")))
+    o = orc.read_list(orc.pfnJoinIndex(orc.vec(u), orc.vec(probe), orc.sym("eq")))
+    assert len(o[0]) > 0
+    assert np.array_equal(g[0], o[0]) and np.array_equal(g[1], o[1])
+
+
+def test_dense_keys_repeated_build_join(gpu, orc):
+    """A dense build side with repeated keys: the direct table flags the repeat and the join takes
+    its sort-based path."""
+    r = rng(17)
+    x = r.integers(100, 3000, 5000).astype(np.int32)
+    y = r.integers(50, 3100, 20000).astype(np.int32)
+    g = gpu.read_list(gpu.pfnJoinIndex(gpu.vec(x), gpu.vec(y), gpu.sym("eq")))
+    o = orc.read_list(orc.pfnJoinIndex(orc.vec(x), orc.vec(y), orc.sym("eq")))
+    assert np.array_equal(g[0], o[0]) and np.array_equal(g[1], o[1])
+
+
 def test_index_of_bool_and_char(gpu, orc):
     for x, y in ((np.array([1, 1, 0], np.uint8), np.array([0, 1], np.uint8)),
                  (np.array([1, 1], np.uint8), np.array([0, 1], np.uint8))):
