@@ -196,10 +196,70 @@ __global__ void __launch_bounds__(256) k_i32_range_final(const int *part, int np
     for(int b = threadIdx.x; b < np; b += 256){ lo = min(lo, part[2 * b]); hi = max(hi, part[2 * b + 1]); }
     i32_minmax_block(lo, hi, out);
 }
-// [lo, hi] of n i32 keys; *dense when they span at most DENSE_PER_KEY slots per key
-static int32_t dense_range(hpa_ctx *c, const int32_t *x, L n, int *lo, int *hi, bool *dense){
+// Monotonicity (isOrdered, backend/h_system.cc:678-698) and range of an i32 column in one read:
+// part[4 b ...] = (rises, falls, min, max) of workgroup b's rows and the pair before each
+__global__ void __launch_bounds__(256) k_i32_order_range(const int32_t *x, L n, int *part){
+    __shared__ int s[4][4];
+    int up = 0, down = 0, lo = 2147483647, hi = -2147483647 - 1;
+    for(L i = (L)blockIdx.x * 256 + threadIdx.x; i < n; i += (L)gridDim.x * 256){
+        const int v = x[i];
+        lo = v < lo ? v : lo; hi = v > hi ? v : hi;
+        if(i > 0){ const int a = x[i - 1]; up |= v > a; down |= v < a; }
+    }
+    for(int o = 32; o > 0; o >>= 1){
+        up |= __shfl_xor(up, o, 64); down |= __shfl_xor(down, o, 64);
+        int t = __shfl_xor(lo, o, 64); lo = t < lo ? t : lo; t = __shfl_xor(hi, o, 64); hi = t > hi ? t : hi;
+    }
+    const int w = threadIdx.x >> 6;
+    if((threadIdx.x & 63) == 0){ s[w][0] = up; s[w][1] = down; s[w][2] = lo; s[w][3] = hi; }
+    __syncthreads();
+    if(threadIdx.x == 0){
+        for(int q = 1; q < 4; q++){ up |= s[q][0]; down |= s[q][1]; lo = min(lo, s[q][2]); hi = max(hi, s[q][3]); }
+        part[4 * blockIdx.x] = up; part[4 * blockIdx.x + 1] = down; part[4 * blockIdx.x + 2] = lo; part[4 * blockIdx.x + 3] = hi;
+    }
+}
+__global__ void __launch_bounds__(256) k_i32_order_range_final(const int *part, int np, int *out){
+    __shared__ int s[4][4];
+    int up = 0, down = 0, lo = 2147483647, hi = -2147483647 - 1;
+    for(int b = threadIdx.x; b < np; b += 256){ up |= part[4 * b]; down |= part[4 * b + 1]; lo = min(lo, part[4 * b + 2]); hi = max(hi, part[4 * b + 3]); }
+    for(int o = 32; o > 0; o >>= 1){
+        up |= __shfl_xor(up, o, 64); down |= __shfl_xor(down, o, 64);
+        int t = __shfl_xor(lo, o, 64); lo = t < lo ? t : lo; t = __shfl_xor(hi, o, 64); hi = t > hi ? t : hi;
+    }
+    const int w = threadIdx.x >> 6;
+    if((threadIdx.x & 63) == 0){ s[w][0] = up; s[w][1] = down; s[w][2] = lo; s[w][3] = hi; }
+    __syncthreads();
+    if(threadIdx.x == 0){
+        for(int q = 1; q < 4; q++){ up |= s[q][0]; down |= s[q][1]; lo = min(lo, s[q][2]); hi = max(hi, s[q][3]); }
+        out[0] = up; out[1] = down; out[2] = lo; out[3] = hi;
+    }
+}
+// ordered (monotone in either direction) and [lo, hi] of an i32 column, with one host read
+static int32_t i32_order_range(hpa_ctx *c, const int32_t *x, L n, bool *ord, int *lo, int *hi){
+    *ord = n <= 2; *lo = 0; *hi = -1;
+    if(n <= 0) R 0;
+    const int np = grid_for(n, 256, 8, num_cus() * 4);
+    int *r = (int*)dalloc(c, 16 * (L)(np + 1)); if(!r) R HPA_E_MEMORY;
+    hipLaunchKernelGGL(k_i32_order_range, dim3(np), dim3(256), 0, c->stream, x, n, r + 4);
+    hipLaunchKernelGGL(k_i32_order_range_final, dim3(1), dim3(256), 0, c->stream, r + 4, np, r);
+    int32_t e;
+    if((e = check_launch("order_range"))) R e;
+    int h[4]; if((e = d2h(c, h, r, 16))) R e;
+    if(n > 2) *ord = !(h[0] && h[1]);
+    *lo = h[2]; *hi = h[3];
+    R 0;
+}
+// [lo, hi] of n i32 keys; *dense when they span at most DENSE_PER_KEY slots per key; a range
+// known already comes in through known (lo, hi)
+static int32_t dense_range(hpa_ctx *c, const int32_t *x, L n, int *lo, int *hi, bool *dense, const int *known = nullptr){
     *dense = false;
     if(n < 1024) R 0;                                  // small builds: the hash table is cheap
+    if(known){
+        *lo = known[0]; *hi = known[1];
+        const L R_ = (L)known[1] - (L)known[0] + 1;
+        *dense = R_ <= (L)DENSE_PER_KEY * n && R_ < (L)INT32_MAX;
+        R 0;
+    }
     const int np = grid_for(ceil_div(n, 4), 256, 4, num_cus() * 4);
     int *r = (int*)dalloc(c, 8 * (L)(np + 1)); if(!r) R HPA_E_MEMORY;
     hipLaunchKernelGGL(k_i32_range, dim3(np), dim3(256), 0, c->stream, x, n, (uintptr_t)x % 16 == 0 ? 1 : 0, r + 2);
@@ -362,11 +422,11 @@ __global__ void k_search_ordered(const T *x, L nx, const T *y, L n, int64_t *z){
 int32_t is_ordered_dev(hpa_ctx *c, V x, bool *ord);
 
 template<typename T>
-static int32_t index_of_hash(hpa_ctx *c, const T *px, L nx, const T *py, L ny, int64_t *z){
+static int32_t index_of_hash(hpa_ctx *c, const T *px, L nx, const T *py, L ny, int64_t *z, const int *known = nullptr){
     int32_t e;
     if(std::is_same<T, int32_t>::value && nx < (L)INT32_MAX){
         int lo, hi; bool dense;
-        if((e = dense_range(c, (const int32_t*)px, nx, &lo, &hi, &dense))) R e;
+        if((e = dense_range(c, (const int32_t*)px, nx, &lo, &hi, &dense, known))) R e;
         if(dense){
             DenseTab t;
             if((e = dense_build(c, (const int32_t*)px, nx, lo, (L)hi - lo + 1, true, &t))) R e;
@@ -407,7 +467,13 @@ extern "C" int32_t pfnIndexOf(hpa_V z, hpa_V x, hpa_V y){
     if(is_real(x->typ) && is_real(y->typ)){
         int32_t t = x->typ > y->typ ? x->typ : y->typ;
         bool ord = false;
-        if((e = is_ordered_dev(c, x, &ord))) R e;
+        int known[2]; bool have = false;                 // i32 x: order and key range in one read
+        if(x->typ == HPA_I){
+            const void *p0 = nullptr;
+            if(nx > 0 && (e = dev_view(c, x, &p0))) R e;
+            if((e = i32_order_range(c, (const int32_t*)p0, nx, &ord, &known[0], &known[1]))) R e;
+            have = nx > 0;
+        } else if((e = is_ordered_dev(c, x, &ord))) R e;
         const void *px = nullptr, *py = nullptr;
         if(ord){
             // searchOrdered: y compared in x's integer type, widened when y is wider
@@ -436,7 +502,7 @@ extern "C" int32_t pfnIndexOf(hpa_V z, hpa_V x, hpa_V y){
             case HPA_B: e = index_of_table(c, (const uint8_t*)px, nx, (const uint8_t*)py, ny, zz, 2, true); break;
             case HPA_J: e = index_of_hash<int8_t>(c, (const int8_t*)px, nx, (const int8_t*)py, ny, zz); break;
             case HPA_H: e = index_of_hash<int16_t>(c, (const int16_t*)px, nx, (const int16_t*)py, ny, zz); break;
-            case HPA_I: e = index_of_hash<int32_t>(c, (const int32_t*)px, nx, (const int32_t*)py, ny, zz); break;
+            case HPA_I: e = index_of_hash<int32_t>(c, (const int32_t*)px, nx, (const int32_t*)py, ny, zz, have ? known : nullptr); break;
             case HPA_L: e = index_of_hash<int64_t>(c, (const int64_t*)px, nx, (const int64_t*)py, ny, zz); break;
             case HPA_F: e = index_of_hash<float>(c, (const float*)px, nx, (const float*)py, ny, zz); break;
             case HPA_E: e = index_of_hash<double>(c, (const double*)px, nx, (const double*)py, ny, zz); break;
