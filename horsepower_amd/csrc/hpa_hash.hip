@@ -691,17 +691,34 @@ static int32_t join_run(hpa_ctx *c, V z, V bx, V py, int32_t t, int op, bool swa
         unsigned long long *slots = nullptr; uint64_t mask = 0; DenseTab dt; int *dup;
         if(dense){ if((e = dense_build(c, (const int32_t*)pb, nb, lo, (L)hi - lo + 1, true, &dt))) R e; dup = dt.dup; }
         else if((e = pk_build(c, (const int32_t*)pb, nb, &slots, &mask, &dup))) R e;
-        int hd = 0; if((e = d2h(c, &hd, dup, 4))) R e;
-        if(!hd){
-            if(np > 0){
-                if(dense) hipLaunchKernelGGL(k_dense_probe_join, dim3(grid_for(np, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)pp, np, dt, cnt, rid);
-                else hipLaunchKernelGGL(k_pk_probe, dim3(grid_for(np, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)pp, np, slots, mask, cnt, rid);
-            }
-            if((e = scan_u32_to_i64(c, cnt, off, np, off + np))) R e;
-            int64_t P = 0; if((e = d2h(c, &P, off + np, 8))) R e;
-            int64_t *z0 = (int64_t*)dalloc(c, 8 * (P > 0 ? P : 1)), *z1 = (int64_t*)dalloc(c, 8 * (P > 0 ? P : 1));
+        // Probe, scan and (when the arena has room for np pairs, the most unique build keys can
+        // give) the pair writes are queued before the host reads anything: one read then returns
+        // the repeated-key flag and the pair count (dup[0], dup[2..3]), and the pair buffers are
+        // trimmed.  Repeated keys discard the lot for the sort path below.
+        int64_t *tot = (int64_t*)(dup + 2);
+        if(np > 0){
+            if(dense) hipLaunchKernelGGL(k_dense_probe_join, dim3(grid_for(np, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)pp, np, dt, cnt, rid);
+            else hipLaunchKernelGGL(k_pk_probe, dim3(grid_for(np, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)pp, np, slots, mask, cnt, rid);
+        }
+        if((e = scan_u32_to_i64(c, cnt, off, np, tot))) R e;
+        const bool spec = c->dcur + 16 * (np > 0 ? np : 1) + 1024 <= c->dtop;
+        int64_t *z0 = nullptr, *z1 = nullptr;
+        if(spec){
+            z0 = (int64_t*)dalloc(c, 8 * (np > 0 ? np : 1)); z1 = (int64_t*)dalloc(c, 8 * (np > 0 ? np : 1));
             if(!z0 || !z1) R HPA_E_MEMORY;
             if(np > 0) hipLaunchKernelGGL(k_pk_write, dim3(grid_for(np, 256, 4)), dim3(256), 0, c->stream, np, cnt, rid, off, z0, z1, swap ? 1 : 0);
+        }
+        int64_t st[2] = {0, 0};
+        if((e = d2h(c, st, dup, 16))) R e;
+        const int hd = (int)(st[0] & 0xffffffff);
+        const int64_t P = st[1];
+        if(!hd){
+            if(spec) c->dcur = (L)((char*)z1 - c->dbase) + ((8 * (P > 0 ? P : 1) + 255) & ~(L)255);
+            else {
+                z0 = (int64_t*)dalloc(c, 8 * (P > 0 ? P : 1)); z1 = (int64_t*)dalloc(c, 8 * (P > 0 ? P : 1));
+                if(!z0 || !z1) R HPA_E_MEMORY;
+                if(np > 0) hipLaunchKernelGGL(k_pk_write, dim3(grid_for(np, 256, 4)), dim3(256), 0, c->stream, np, cnt, rid, off, z0, z1, swap ? 1 : 0);
+            }
             if((e = check_launch("join"))) R e;
             V cells = (V)halloc(c, 2 * sizeof(V0)); if(!cells) R HPA_E_MEMORY;
             z->typ = HPA_G; z->len = 2; z->g = (char*)cells; z->g2 = nullptr;
