@@ -257,9 +257,14 @@ __global__ void __launch_bounds__(1024) k_batch_pscan(BPlan p){
 
 // Row tiles of partitioned queries into their partitions: LDS cursors start at the tile's
 // offset in each partition and each selected row takes the next place of its partition.
+// Scatter of a partitioned query's row tile: tile-local 32-bit cursors per partition in LDS
+// (after the partitions' tile bases), four rows per thread per step with their loads and cursor
+// atomics issued together.
+#define SC_ROWS 4
 __global__ void __launch_bounds__(SC_BLOCK) k_batch_scatter(BPlan p){
     __shared__ int64_t s_item;
-    unsigned long long *cur = (unsigned long long*)g_batch_lds;         // [PART_MAX]
+    int64_t *base = (int64_t*)g_batch_lds;                               // [PART_MAX]
+    unsigned *cur = (unsigned*)(base + PART_MAX);                        // [PART_MAX]
     while(true){
         const int64_t item = next_item(&p.ctr[2], &s_item);
         if(item >= p.total_tiles) break;
@@ -269,15 +274,28 @@ __global__ void __launch_bounds__(SC_BLOCK) k_batch_scatter(BPlan p){
         const int64_t r0 = tile * p.tile_rows;
         const int64_t r1 = r0 + p.tile_rows < q.n ? r0 + p.tile_rows : q.n;
         const uint32_t range = (uint32_t)q.range, mask = (uint32_t)p.part_keys - 1u;
-        for(int k = threadIdx.x; k < q.nparts; k += SC_BLOCK) cur[k] = (unsigned long long)q.poff[(int64_t)k * q.ntiles + tile];
+        for(int k = threadIdx.x; k < q.nparts; k += SC_BLOCK){ base[k] = q.poff[(int64_t)k * q.ntiles + tile]; cur[k] = 0u; }
         __syncthreads();
-        for(int64_t i = r0 + threadIdx.x; i < r1; i += SC_BLOCK){
-            if(batch_pass(q.op, q.flt[i], q.thr)){
-                const uint32_t s = (uint32_t)q.key[i] - (uint32_t)q.kmin;
-                if(s < range){
-                    const unsigned long long pos = atomicAdd(&cur[s >> p.part_shift], 1ull);
-                    q.skey[pos] = (uint16_t)(s & mask); q.sval[pos] = q.val[i];
+        for(int64_t i0 = r0 + threadIdx.x; i0 < r1; i0 += SC_ROWS * SC_BLOCK){
+            uint32_t s[SC_ROWS]; double v[SC_ROWS]; bool ok[SC_ROWS];
+            #pragma unroll
+            for(int u = 0; u < SC_ROWS; u++){
+                const int64_t i = i0 + (int64_t)u * SC_BLOCK;
+                ok[u] = false; s[u] = 0; v[u] = 0.0;
+                if(i < r1 && batch_pass(q.op, q.flt[i], q.thr)){
+                    s[u] = (uint32_t)q.key[i] - (uint32_t)q.kmin;
+                    ok[u] = s[u] < range;
+                    if(ok[u]) v[u] = q.val[i];
                 }
+            }
+            unsigned loc[SC_ROWS];
+            #pragma unroll
+            for(int u = 0; u < SC_ROWS; u++) loc[u] = ok[u] ? atomicAdd(&cur[s[u] >> p.part_shift], 1u) : 0u;
+            #pragma unroll
+            for(int u = 0; u < SC_ROWS; u++){
+                if(!ok[u]) continue;
+                const int64_t pos = base[s[u] >> p.part_shift] + loc[u];
+                q.skey[pos] = (uint16_t)(s[u] & mask); q.sval[pos] = v[u];
             }
         }
         __syncthreads();
@@ -472,7 +490,7 @@ static int32_t batch_enqueue(hpa_batch *b, hipStream_t s){
     hipLaunchKernelGGL(k_batch_agg, dim3(b->grid), dim3(b->block), agg_lds, s, p);
     if(b->npq){
         hipLaunchKernelGGL(k_batch_pscan, dim3((unsigned)b->npq), dim3(1024), 0, s, p);
-        hipLaunchKernelGGL(k_batch_scatter, dim3(num_cus() * 4), dim3(SC_BLOCK), (size_t)b->max_parts * 8, s, p);
+        hipLaunchKernelGGL(k_batch_scatter, dim3(num_cus() * 4), dim3(SC_BLOCK), (size_t)PART_MAX * 12, s, p);
         hipLaunchKernelGGL(k_batch_pagg, dim3(num_cus() * 4), dim3(256), lds_bytes(b->part_keys), s, p);
     }
     hipLaunchKernelGGL(k_batch_emit, dim3(num_cus() * 8), dim3(256), 0, s, p);
