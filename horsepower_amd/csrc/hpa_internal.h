@@ -99,6 +99,14 @@ inline int grid_for(L n, int block, int per_thread, int max_blocks = 0){
     if(g < 1) g = 1;
     R (int)g;
 }
+// Workgroups go to the eight XCDs round-robin by block index (block b on XCD b mod 8).  This
+// renumbers block b of a g-block grid so that each XCD takes one contiguous range of tiles:
+// tiles that run side by side then share an L2, and writes that two neighbouring tiles make to
+// one cache line meet there.  A bijection of [0, g).
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned g){
+    const unsigned x = b & 7u, q = b >> 3, per = g >> 3, rem = g & 7u;
+    return x * per + (x < rem ? x : rem) + q;
+}
 
 // ---------------------------------------------------------------- shared device algorithms
 // exclusive scan of n int64 values (in place allowed); total returned through *total_dev

@@ -24,7 +24,8 @@
 #define RS_TILE  (RS_BLOCK * RS_ROWS)
 
 // Digit histogram of one 4096-key tile, counted in per-wave LDS histograms; a wave whose valid
-// keys all share the digit adds them with one increment.  hist[d * ntiles + tile].  With `bits`
+// keys all share the digit adds them with one increment.  hist[tile * 256 + d]: each tile writes
+// one contiguous 1 KB row.  With `bits`
 // set (the first pass) it also writes the tile's OR and AND of all keys to bits[2*tile],
 // bits[2*tile+1]: which key bits vary.
 template<typename K>
@@ -58,7 +59,7 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_hist(const K *keys, L n, int sh
         if(lane == 0){ so[w] = o; sa[w] = a; }
     }
     __syncthreads();
-    hist[(L)threadIdx.x * ntiles + blockIdx.x] = cnt[0][threadIdx.x] + cnt[1][threadIdx.x] + cnt[2][threadIdx.x] + cnt[3][threadIdx.x];
+    hist[(L)blockIdx.x * 256 + threadIdx.x] = cnt[0][threadIdx.x] + cnt[1][threadIdx.x] + cnt[2][threadIdx.x] + cnt[3][threadIdx.x];
     if(bits && threadIdx.x == 0){
         for(int q = 1; q < RS_BLOCK / 64; q++){ o |= so[q]; a &= sa[q]; }
         bits[2 * blockIdx.x] = o; bits[2 * blockIdx.x + 1] = a;
@@ -81,9 +82,10 @@ __global__ void __launch_bounds__(256) k_rs_bits_final(const unsigned long long 
 // Scatter of one tile: every key's stable rank inside its wave's quarter of the tile comes from
 // eight ballots (lanes sharing the digit); the tile is then laid out digit by digit in LDS and
 // written out in that order, so consecutive threads write consecutive addresses of one digit's
-// run instead of 4096 scattered 4/8-byte stores.  off[d * ntiles + tile]: output start of the
-// tile's keys of digit d.  FINAL: only the values are written, widened to i64 into out64 at
-// position pos, or n - 1 - pos with `reverse`.
+// run instead of 4096 scattered 4/8-byte stores.  off[tile * 256 + d]: output start of the
+// tile's keys of digit d.  Block b takes tile xcd_tile(b): the tiles in flight on one XCD are
+// neighbours, and the runs they write for one digit are adjacent in the output.  FINAL: only the
+// values are written, widened to i64 into out64 at position pos, or n - 1 - pos with `reverse`.
 template<typename K, bool FINAL>
 __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K *kin, const uint32_t *vin, K *kout, uint32_t *vout,
                                                          int64_t *out64, int reverse, L n, int shift, const int64_t *off, L ntiles){
@@ -95,7 +97,8 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K *kin, const uin
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for(int q = 0; q < 4; q++) wcnt[q][threadIdx.x] = 0;
     __syncthreads();
-    const L t0 = (L)blockIdx.x * RS_TILE;
+    const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+    const L t0 = (L)tile * RS_TILE;
     const L tbase = t0 + (L)w * (RS_TILE / 4);
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     K kr[RS_ROWS]; uint32_t vr[RS_ROWS]; uint32_t rk[RS_ROWS];
@@ -136,7 +139,7 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K *kin, const uin
         uint32_t pre = inc - tot;
         for(int q = 0; q < w; q++) pre += wsum[q];
         wcnt[0][d] = pre; wcnt[1][d] = pre + c0; wcnt[2][d] = pre + c0 + c1; wcnt[3][d] = pre + c0 + c1 + c2;
-        gbase[d] = off[(L)d * ntiles + blockIdx.x] - (int64_t)pre;
+        gbase[d] = off[(L)tile * 256 + d] - (int64_t)pre;
     }
     __syncthreads();
     #pragma unroll
@@ -157,6 +160,106 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K *kin, const uin
         else { kout[pos] = k; vout[pos] = sv[j]; }
     }
 }
+// Exclusive scan of tile-major digit counts in digit-major order:
+//   off[t * 256 + d] = (counts of all digits below d) + (counts of digit d in tiles before t).
+// Thread (d, q) of a 1024-thread block owns digit d and a quarter q of the rows, so every
+// access is one contiguous 1 KB row of 256 counts, and each thread issues its loads together.
+// Three launches: per-chunk sums of each digit (32 tiles a chunk), chunk bases per digit, then
+// each chunk writes its tiles' offsets.
+#define DS_PER 32
+__global__ void __launch_bounds__(1024) k_ds_part(const uint32_t *cnt, L ntiles, uint32_t *part){
+    __shared__ uint32_t ps[4][256];
+    const unsigned d = threadIdx.x & 255, q = threadIdx.x >> 8;
+    const L t0 = (L)blockIdx.x * DS_PER + q * (DS_PER / 4);
+    uint32_t v[DS_PER / 4];
+    #pragma unroll
+    for(int j = 0; j < DS_PER / 4; j++){ const L t = t0 + j; v[j] = t < ntiles ? cnt[t * 256 + d] : 0u; }
+    uint32_t s = 0;
+    #pragma unroll
+    for(int j = 0; j < DS_PER / 4; j++) s += v[j];
+    ps[q][d] = s;
+    __syncthreads();
+    if(q == 0) part[(L)blockIdx.x * 256 + d] = ps[0][d] + ps[1][d] + ps[2][d] + ps[3][d];
+}
+// chunk bases relative to their digit: cbase[c * 256 + d] = counts of digit d in chunks before
+// c, and tot[d] = all counts of digit d.  Block g takes digits 32g..32g+31; thread (dl, q) sums
+// digit 32g + dl over the q-th of 32 slices of the chunks.
+__global__ void __launch_bounds__(1024) k_ds_mid(const uint32_t *part, int nchunks, uint32_t *cbase, uint32_t *tot){
+    __shared__ uint32_t qs[32][33];
+    const unsigned dl = threadIdx.x & 31, q = threadIdx.x >> 5, d = blockIdx.x * 32 + dl;
+    const int cq = (nchunks + 31) / 32, c0 = (int)q * cq, c1 = c0 + cq < nchunks ? c0 + cq : nchunks;
+    uint32_t s = 0;
+    for(int cb = c0; cb < c1; cb += 8){
+        uint32_t v[8];
+        #pragma unroll
+        for(int j = 0; j < 8; j++) v[j] = cb + j < c1 ? part[(L)(cb + j) * 256 + d] : 0u;
+        #pragma unroll
+        for(int j = 0; j < 8; j++) s += v[j];
+    }
+    qs[q][dl] = s;
+    __syncthreads();
+    uint32_t run = 0;
+    for(unsigned r = 0; r < q; r++) run += qs[r][dl];
+    if(q == 31) tot[d] = run + s;
+    for(int cb = c0; cb < c1; cb += 8){
+        uint32_t v[8];
+        #pragma unroll
+        for(int j = 0; j < 8; j++) v[j] = cb + j < c1 ? part[(L)(cb + j) * 256 + d] : 0u;
+        #pragma unroll
+        for(int j = 0; j < 8; j++) if(cb + j < c1){ cbase[(L)(cb + j) * 256 + d] = run; run += v[j]; }
+    }
+}
+// off of the chunk's 32 tiles: the digit base (exclusive scan of the 256 digit totals, done by
+// every block), the chunk base, and the counts of earlier tiles in the chunk
+__global__ void __launch_bounds__(1024) k_ds_down(const uint32_t *cnt, L ntiles, const uint32_t *cbase, const uint32_t *tot,
+                                                  int64_t *off){
+    __shared__ uint32_t ps[4][256];
+    __shared__ int64_t wtot[4], dbase[256];
+    const unsigned d = threadIdx.x & 255, q = threadIdx.x >> 8, lane = threadIdx.x & 63;
+    const L t0 = (L)blockIdx.x * DS_PER + q * (DS_PER / 4);
+    const uint32_t cb = cbase[(L)blockIdx.x * 256 + d];
+    const uint32_t td = q == 0 ? tot[d] : 0u;
+    uint32_t v[DS_PER / 4];
+    #pragma unroll
+    for(int j = 0; j < DS_PER / 4; j++){ const L t = t0 + j; v[j] = t < ntiles ? cnt[t * 256 + d] : 0u; }
+    uint32_t s = 0;
+    #pragma unroll
+    for(int j = 0; j < DS_PER / 4; j++) s += v[j];
+    ps[q][d] = s;
+    int64_t inc = 0;
+    if(q == 0){
+        inc = td;
+        for(int o = 1; o < 64; o <<= 1){ const int64_t t = __shfl_up(inc, o, 64); if((int)lane >= o) inc += t; }
+        if(lane == 63) wtot[d >> 6] = inc;
+    }
+    __syncthreads();
+    if(q == 0){
+        int64_t pre = inc - td;
+        for(unsigned w = 0; w < (d >> 6); w++) pre += wtot[w];
+        dbase[d] = pre;
+    }
+    __syncthreads();
+    int64_t run = dbase[d] + cb;
+    for(unsigned r = 0; r < q; r++) run += ps[r][d];
+    #pragma unroll
+    for(int j = 0; j < DS_PER / 4; j++){
+        const L t = t0 + j;
+        if(t < ntiles){ off[t * 256 + d] = run; run += v[j]; }
+    }
+}
+static L digit_scan_chunks(L ntiles){ R ceil_div(ntiles, DS_PER); }
+// part: 256 * chunks u32, cbase: 256 * chunks u32 (chunks = digit_scan_chunks), tot: 256 u32
+static int32_t digit_scan(hpa_ctx *c, const uint32_t *cnt, L ntiles, uint32_t *part, uint32_t *cbase, uint32_t *tot, int64_t *off){
+    const L nchunks = digit_scan_chunks(ntiles);
+    if(nchunks > INT32_MAX){ set_error_detail("digit scan of %lld tiles", (long long)ntiles); R HPA_E_NOT_IMPL; }
+    ProfScope ps("radix_scan", c->stream, 12.0 * 256 * ntiles);
+    hipLaunchKernelGGL(k_ds_part, dim3((unsigned)nchunks), dim3(1024), 0, c->stream, cnt, ntiles, part);
+    hipLaunchKernelGGL(k_ds_mid, dim3(8), dim3(1024), 0, c->stream, (const uint32_t*)part, (int)nchunks, cbase, tot);
+    hipLaunchKernelGGL(k_ds_down, dim3((unsigned)nchunks), dim3(1024), 0, c->stream, cnt, ntiles, (const uint32_t*)cbase,
+                       (const uint32_t*)tot, off);
+    R check_launch("radix_scan");
+}
+
 __global__ void k_iota_i64(int64_t *z, L n, int reverse){
     for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x) z[i] = reverse ? n - 1 - i : i;
 }
@@ -177,7 +280,10 @@ static int32_t radix_sort_t(hpa_ctx *c, const K *keys, const uint32_t *vals, L n
     unsigned long long *bits = (unsigned long long*)dalloc(c, 16 + 16 * ntiles);
     uint32_t *hist = (uint32_t*)dalloc(c, 4 * 256 * ntiles);
     int64_t *off = (int64_t*)dalloc(c, 8 * 256 * ntiles);
-    if(!bits || !hist || !off) R HPA_E_MEMORY;
+    uint32_t *part = (uint32_t*)dalloc(c, 4 * 256 * digit_scan_chunks(ntiles));
+    uint32_t *cbase = (uint32_t*)dalloc(c, 4 * 256 * digit_scan_chunks(ntiles));
+    uint32_t *tot = (uint32_t*)dalloc(c, 4 * 256);
+    if(!bits || !hist || !off || !part || !cbase || !tot) R HPA_E_MEMORY;
     // digits to sort by: the first histogram pass also finds the bits that vary
     {
         ProfScope ps("radix_hist", c->stream, (double)n * sizeof(K) + 4.0 * 256 * ntiles);
@@ -216,7 +322,7 @@ static int32_t radix_sort_t(hpa_ctx *c, const K *keys, const uint32_t *vals, L n
             hipLaunchKernelGGL(k_rs_hist<K>, dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream, ka, n, s, hist, ntiles,
                                (unsigned long long*)nullptr);
         }
-        if((e = scan_u32_to_i64(c, hist, off, 256 * ntiles, nullptr))) R e;
+        if((e = digit_scan(c, hist, ntiles, part, cbase, tot, off))) R e;
         const double rd = (double)n * (sizeof(K) + (va ? 4 : 0)), wr = last && out64 ? 8.0 * n : (double)n * (sizeof(K) + 4);
         ProfScope ps("radix_scatter", c->stream, rd + wr);
         if(last && out64)
@@ -1250,7 +1356,7 @@ __global__ void k_count_ties(const uint64_t *k, L n, unsigned long long *cnt){
 // the same order keys -- ties by row -- in a single launch with no host round trip.
 #define SMALL_ORDER 1024
 #define SMALL_COLS 8
-struct SmallCols { int nc; int typ[SMALL_COLS]; int desc[SMALL_COLS]; const void *p[SMALL_COLS]; };
+struct SmallCols { int nc; int typ[SMALL_COLS]; int desc[SMALL_COLS]; int bits[SMALL_COLS]; const void *p[SMALL_COLS]; };
 __device__ __forceinline__ uint64_t okey_any(int t, const void *p, L i){
     switch(t){
         case HPA_B: return okey<uint8_t>(((const uint8_t*)p)[i]);
@@ -1279,6 +1385,118 @@ __global__ void __launch_bounds__(1024) k_order_small(SmallCols s, int n, uint32
     }
 }
 
+// Larger inputs: rows are bucketed on the most significant varying bits of the first column's
+// key -- bucket order is key order -- and placed bucket by bucket; a row's place inside its
+// bucket is its rank among the bucket's rows on the full key, ties by row.  Only rows that share
+// a bucket are compared.  The bucket window (nb bits ending at the highest bit that varies) is
+// worked out on the device from the OR and AND of all keys, so the only host read is the final
+// check: a bucket of more than BR_BUCKET_MAX rows (the placing and ranking then write nothing)
+// sends the order to the radix passes instead.
+#define BR_BUCKET_MAX 1024
+__device__ __forceinline__ uint64_t br_key(const SmallCols &s, int q, L r){
+    const uint64_t v = okey_any(s.typ[q], s.p[q], r);
+    const uint64_t mask = s.bits[q] >= 64 ? ~0ull : ((1ull << s.bits[q]) - 1);
+    return s.desc[q] ? (~v & mask) : v;
+}
+// k0 = first column's key; or_and[0] |= every key, or_and[1] &= every key
+__global__ void __launch_bounds__(256) k_br_key(SmallCols s, L n, uint64_t *k0, unsigned long long *or_and){
+    unsigned long long o = 0, a = ~0ull;
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
+        const uint64_t v = br_key(s, 0, i);
+        k0[i] = v; o |= v; a &= v;
+    }
+    for(int d = 32; d > 0; d >>= 1){ o |= __shfl_down(o, d, 64); a &= __shfl_down(a, d, 64); }
+    if((threadIdx.x & 63) == 0){ atomicOr(or_and, o); atomicAnd(or_and + 1, a); }
+}
+__global__ void k_br_init(uint32_t *p, L n, unsigned long long *or_and){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x) p[i] = 0;
+    if(or_and && blockIdx.x == 0 && threadIdx.x == 0){ or_and[0] = 0; or_and[1] = ~0ull; }
+}
+// low bit of the bucket window: nb bits ending at the highest varying key bit
+__device__ __forceinline__ int br_shift(const unsigned long long *or_and, int nb){
+    const unsigned long long vary = or_and[0] ^ or_and[1];
+    const int top = vary ? 64 - __clzll(vary) : 0;
+    return top > nb ? top - nb : 0;
+}
+__global__ void __launch_bounds__(256) k_br_hist(const uint64_t *k0, L n, const unsigned long long *or_and, int nb, uint32_t *cnt, int *over){
+    const int shift = br_shift(or_and, nb);
+    const uint32_t bmask = (1u << nb) - 1;
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
+        const uint32_t b = (uint32_t)(k0[i] >> shift) & bmask;
+        if(atomicAdd(&cnt[b], 1u) == BR_BUCKET_MAX) atomicOr(over, 1);
+    }
+}
+__global__ void __launch_bounds__(256) k_br_place(const uint64_t *k0, L n, const unsigned long long *or_and, int nb, const int *over,
+                                                  const int64_t *start, uint32_t *cursor, uint32_t *slot_row, uint64_t *slot_k0){
+    if(*over) return;
+    const int shift = br_shift(or_and, nb);
+    const uint32_t bmask = (1u << nb) - 1;
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
+        const uint64_t v = k0[i];
+        const uint32_t b = (uint32_t)(v >> shift) & bmask;
+        const int64_t p = start[b] + atomicAdd(&cursor[b], 1u);
+        slot_row[p] = (uint32_t)i; slot_k0[p] = v;
+    }
+}
+__global__ void __launch_bounds__(256) k_br_rank(SmallCols s, L n, const unsigned long long *or_and, int nb, const int *over,
+                                                 const int64_t *start, const uint32_t *cnt, const uint32_t *slot_row,
+                                                 const uint64_t *slot_k0, uint32_t *perm){
+    if(*over) return;
+    const int shift = br_shift(or_and, nb);
+    const uint32_t bmask = (1u << nb) - 1;
+    for(L p = (L)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (L)gridDim.x * blockDim.x){
+        const uint32_t r = slot_row[p];
+        const uint64_t a = slot_k0[p];
+        const uint32_t b = (uint32_t)(a >> shift) & bmask;
+        const int64_t s0 = start[b], e = s0 + cnt[b];
+        uint32_t rank = 0;
+        for(int64_t j = s0; j < e; j++){
+            const uint64_t kj = slot_k0[j];
+            if(kj != a){ rank += kj < a; continue; }
+            const uint32_t rj = slot_row[j];
+            if(rj == r) continue;
+            int less = rj < r;
+            for(int q = 1; q < s.nc; q++){
+                const uint64_t x = br_key(s, q, rj), y = br_key(s, q, r);
+                if(x != y){ less = x < y; break; }
+            }
+            rank += less;
+        }
+        perm[s0 + rank] = r;
+    }
+}
+// *fallback: a bucket is too large and nothing was written
+static int32_t order_bucket_rank(hpa_ctx *c, const SmallCols &s, L n, uint32_t **perm_out, bool *fallback){
+    int32_t e;
+    *fallback = false;
+    int lg = 0; while((1ll << lg) < n) lg++;
+    int nb = lg - 3; if(nb < 8) nb = 8; if(nb > 22) nb = 22;
+    const L B = 1ll << nb;
+    unsigned long long *or_and = (unsigned long long*)dalloc(c, 16);
+    int *over = (int*)dalloc(c, 16);
+    uint64_t *k0 = (uint64_t*)dalloc(c, 8 * n), *slot_k0 = (uint64_t*)dalloc(c, 8 * n);
+    uint32_t *cnt = (uint32_t*)dalloc(c, 8 * B), *cursor = cnt + B;       // zeroed together
+    int64_t *start = (int64_t*)dalloc(c, 8 * (B + 1));
+    uint32_t *slot_row = (uint32_t*)dalloc(c, 4 * n), *perm = (uint32_t*)dalloc(c, 4 * n);
+    if(!or_and || !over || !k0 || !slot_k0 || !cnt || !start || !slot_row || !perm) R HPA_E_MEMORY;
+    const int g = grid_for(n, 256, 4);
+    hipLaunchKernelGGL(k_br_init, dim3(grid_for(2 * B, 256, 4)), dim3(256), 0, c->stream, cnt, 2 * B, or_and);
+    hipLaunchKernelGGL(k_br_init, dim3(1), dim3(64), 0, c->stream, (uint32_t*)over, (L)1, (unsigned long long*)nullptr);
+    hipLaunchKernelGGL(k_br_key, dim3(g), dim3(256), 0, c->stream, s, n, k0, or_and);
+    hipLaunchKernelGGL(k_br_hist, dim3(g), dim3(256), 0, c->stream, (const uint64_t*)k0, n, (const unsigned long long*)or_and, nb, cnt, over);
+    if((e = check_launch("order_bucket"))) R e;
+    if((e = scan_u32_to_i64(c, cnt, start, B, start + B))) R e;
+    hipLaunchKernelGGL(k_br_place, dim3(g), dim3(256), 0, c->stream, (const uint64_t*)k0, n, (const unsigned long long*)or_and, nb,
+                       (const int*)over, (const int64_t*)start, cursor, slot_row, slot_k0);
+    hipLaunchKernelGGL(k_br_rank, dim3(g), dim3(256), 0, c->stream, s, n, (const unsigned long long*)or_and, nb, (const int*)over,
+                       (const int64_t*)start, (const uint32_t*)cnt, (const uint32_t*)slot_row, (const uint64_t*)slot_k0, perm);
+    if((e = check_launch("order_bucket"))) R e;
+    int ov = 0; if((e = d2h(c, &ov, over, 4))) R e;
+    if(ov){ *fallback = true; R 0; }
+    *perm_out = perm;
+    R 0;
+}
+
 static int32_t order_perm(hpa_ctx *c, V x, const uint8_t *flags, uint32_t **perm_out, bool *ties){
     L nc = x->len; L n = ((V)x->g)->len; int32_t e;
     *ties = false;
@@ -1297,6 +1515,24 @@ static int32_t order_perm(hpa_ctx *c, V x, const uint8_t *flags, uint32_t **perm
         if((e = check_launch("order_small"))) R e;
         *perm_out = perm;
         R 0;
+    }
+    if(n > SMALL_ORDER && n < (L)UINT32_MAX && nc <= SMALL_COLS){
+        SmallCols s; memset(&s, 0, sizeof(s)); s.nc = (int)nc;
+        bool ok = true;
+        for(L q = 0; q < nc; q++){
+            V col = ((V)x->g) + q;
+            if(col->typ == HPA_S || col->typ == HPA_X || col->typ == HPA_Q){ ok = false; break; }
+            const void *p; if((e = dev_view(c, col, &p))) R e;
+            s.typ[q] = col->typ; s.p[q] = p; s.desc[q] = flags[q] ? 0 : 1; s.bits[q] = key_bits_of(col->typ);
+        }
+        if(ok){
+            const L mark = c->dcur;
+            bool fallback;
+            ProfScope ps("order_bucket", c->stream);
+            if((e = order_bucket_rank(c, s, n, perm_out, &fallback))) R e;
+            if(!fallback) R 0;
+            c->dcur = mark;
+        }
     }
     uint64_t *k = (uint64_t*)dalloc(c, 8 * (n > 0 ? n : 1)); if(!k) R HPA_E_MEMORY;
     for(L q = nc - 1; q >= 0; q--){

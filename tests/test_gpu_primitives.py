@@ -264,7 +264,7 @@ def test_order_single(gpu, orc, n, up):
         assert np.array_equal(g, o), x.dtype
 
 
-@pytest.mark.parametrize("n", [2, 1000, 200000])
+@pytest.mark.parametrize("n", [2, 1000, 1025, 30000, 131072, 131073, 200000])
 def test_order_lists(gpu, orc, n):
     r = rng(n + 14)
     a = r.integers(0, 20, n).astype(np.int32)
@@ -283,6 +283,16 @@ def test_order_lists(gpu, orc, n):
     g = gpu.read(gpu.pfnOrderBy(gpu.list([gpu.vec(c1), gpu.vec(c2)]), gpu.bools([1, 0])))
     o = orc.read(orc.pfnOrderBy(orc.list([orc.vec(c1), orc.vec(c2)]), orc.bools([1, 0])))
     assert np.array_equal(g, o)
+    # full-row ties on integer columns (the reference merge-sorts those: rows in ascending
+    # row order) and a constant column
+    t1 = r.integers(0, 3, n).astype(np.int64) - 1
+    t2 = np.full(n, 7, np.int32)
+    t3 = r.integers(-2, 2, n).astype(np.int32)
+    cols = lambda B: B.list([B.vec(t1), B.vec(t2), B.vec(t3)])
+    for fl in ([1, 0, 1], [0, 1, 0]):
+        g = gpu.read(gpu.pfnOrderBy(cols(gpu), gpu.bools(fl)))
+        o = orc.read(orc.pfnOrderBy(cols(orc), orc.bools(fl)))
+        assert np.array_equal(g, o), fl
 
 
 @pytest.mark.parametrize("nx,ny,card", [(40, 300, 30), (300, 40, 30), (100, 100, 10), (5, 50, 8),
