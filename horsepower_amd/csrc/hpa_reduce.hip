@@ -270,22 +270,66 @@ __device__ __forceinline__ T seg_val(const T *x, const int64_t *idx, L j, L nx, 
     return x[k];
 }
 
-// floating segments -> KS partials; integer segments -> int64 sums; min/max -> T
+// floating segments -> KS partials; integer segments -> int64 sums; min/max -> T.  Short
+// segments: a wave takes 64 consecutive segments, whose values occupy one contiguous flat range;
+// the wave loads that range together (coalesced index loads, independent value loads) into LDS,
+// SEG_CAP values at a time, and each lane then folds its own segment's values in ascending
+// order -- the FP1 template's order, one segment per lane.
+#define SEG_CAP 256
 template<typename T, bool FLT>
 __global__ void __launch_bounds__(256) k_seg_thread(const T *x, const int64_t *idx, const int64_t *off, L g,
                                                     int kind, void *out, L nx, int *bad){
-    for(L s = (L)blockIdx.x * blockDim.x + threadIdx.x; s < g; s += (L)gridDim.x * blockDim.x){
-        L a = off[s], b = off[s + 1];
-        if(kind == GK_SUM || kind == GK_AVG){
-            if(FLT){ KS acc = {0, 0}; for(L j = a; j < b; j++) ks_add(acc, (double)seg_val(x, idx, j, nx, bad));
-                     double r = acc.s + acc.c; ((double*)out)[s] = kind == GK_AVG ? r / (double)(b - a) : r; }
-            else { unsigned long long acc = 0; for(L j = a; j < b; j++) acc += (unsigned long long)(int64_t)seg_val(x, idx, j, nx, bad);
-                   if(kind == GK_AVG) ((double*)out)[s] = (int64_t)acc * 1.0 / (double)(b - a); else ((int64_t*)out)[s] = (int64_t)acc; }
-        } else {
-            T r = b > a ? seg_val(x, idx, a, nx, bad) : T();
-            for(L j = a; j < b; j++){ T v = seg_val(x, idx, j, nx, bad); r = kind == GK_MAX ? (r > v ? r : v) : (r < v ? r : v); }
-            ((T*)out)[s] = r;
+    __shared__ T buf[4][SEG_CAP];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const L nw = ((L)gridDim.x * blockDim.x) >> 6;
+    for(L base = (((L)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64; base < g; base += nw * 64){
+        const L s = base + lane;
+        const bool valid = s < g;
+        const L a = valid ? off[s] : 0, b = valid ? off[s + 1] : 0;
+        const L last = g - base < 64 ? g - base - 1 : 63;
+        const L A = __shfl(a, 0, 64), E = __shfl(b, (int)last, 64);
+        KS facc = {0, 0};
+        unsigned long long iacc = 0;
+        T r = T();
+        bool first = true;
+        for(L c0 = A; c0 < E; c0 += SEG_CAP){
+            const L c1 = c0 + SEG_CAP < E ? c0 + SEG_CAP : E;
+            int64_t k[SEG_CAP / 64];
+            #pragma unroll
+            for(int t = 0; t < SEG_CAP / 64; t++){ const L j = c0 + t * 64 + lane; k[t] = j < c1 ? (idx ? idx[j] : j) : 0; }
+            #pragma unroll
+            for(int t = 0; t < SEG_CAP / 64; t++){
+                const L j = c0 + t * 64 + lane;
+                T v = T();
+                if(j < c1){
+                    if(idx && (uint64_t)k[t] >= (uint64_t)nx){ if(bad) *bad = 1; }
+                    else v = x[k[t]];
+                }
+                buf[w][t * 64 + lane] = v;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const L j0 = a > c0 ? a : c0, j1 = b < c1 ? b : c1;
+            for(L j = j0; j < j1; j++){
+                const T v = buf[w][j - c0];
+                if(kind == GK_SUM || kind == GK_AVG){
+                    if(FLT) ks_add(facc, (double)v); else iacc += (unsigned long long)(int64_t)v;
+                } else {
+                    if(first){ r = v; first = false; }
+                    else r = kind == GK_MAX ? (r > v ? r : v) : (r < v ? r : v);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
+        if(!valid) continue;
+        if(kind == GK_SUM || kind == GK_AVG){
+            if(FLT){ double q = facc.s + facc.c; ((double*)out)[s] = kind == GK_AVG ? q / (double)(b - a) : q; }
+            else if(kind == GK_AVG) ((double*)out)[s] = (int64_t)iacc * 1.0 / (double)(b - a);
+            else ((int64_t*)out)[s] = (int64_t)iacc;
+        } else ((T*)out)[s] = r;
     }
 }
 

@@ -195,7 +195,9 @@ def q3_plain(B, Dm, F, day=D.Q3_DAY, limit=10, segment=1):
 def q3_fused(B, Dm, F, day=D.Q3_DAY, limit=10, segment=1):
     """Q3 as the optimising compiler emits it (src/horseir/v1/compile-static/cur/q3.h:1-56):
     one mask per table and the multi-column peephole compress of its filtered columns, the FE
-    revenue map, group, the grouped sum, join and the multi-column gathers."""
+    revenue map, group, the grouped sum as one FP loop over the group lists (q3_peephole_1,
+    cur/q3.h:30-50: each group's revenue summed through its row indices, nothing gathered
+    first), join and the multi-column gathers."""
     d_mask = B.pfnAnd(B.pfnEq(Dm["segment"], B.scalar(segment, H_I)), B.pfnLt(Dm["order_day"], B.scalar(day, H_I)))
     lk, lday = B.compress_many(d_mask, [Dm["key"], Dm["order_day"]])
     f_mask = B.pfnGt(F["ship_day"], B.scalar(day, H_I))
@@ -204,7 +206,7 @@ def q3_fused(B, Dm, F, day=D.Q3_DAY, limit=10, segment=1):
     grp = B.pfnGroup(rk)
     gkeys, gvals = B.pfnKeys(grp), B.pfnValues(grp)
     gk = B.pfnIndex(rk, gkeys)
-    gsum = B.pfnRaze(B.each(B.each_right(rev, gvals, "pfnIndex"), "pfnSum"))
+    gsum = B.group_reduce(0, rev, gvals)
     j = B.pfnJoinIndex(lk, gk, B.sym("eq"))
     j0, j1 = B.pfnIndex(j, B.scalar(0, H_L)), B.pfnIndex(j, B.scalar(1, H_L))
     revenue = B.pfnIndex(gsum, j1)
