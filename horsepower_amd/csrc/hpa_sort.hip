@@ -978,26 +978,41 @@ __global__ void __launch_bounds__(256) k_sg_present_sample(const uint8_t *a, con
         part[(L)blockIdx.x * 8 + 4 + (threadIdx.x >> 6)] = mb;
     }
 }
-__global__ void __launch_bounds__(256) k_sg_urange_sample(const int32_t *x, L n, L stride, unsigned *part){
-    __shared__ unsigned slo[4], shi[4];
-    unsigned lo = 0xFFFFFFFFu, hi = 0;
-    const L i = ((L)blockIdx.x * 256 + threadIdx.x) * stride * 16;
-    for(int j = 0; j < 16 && i + j < n; j++){ const unsigned v = (unsigned)x[i + j]; lo = v < lo ? v : lo; hi = v > hi ? v : hi; }
+// Unsigned min / max of an i32 column -- from every row, or from one 16-row chunk in `stride`
+// -- and, alongside, how many consecutive row pairs seen hold different keys: (lo, hi, changes,
+// pairs), reduced over the workgroup into dst[0..3].
+__device__ __forceinline__ void sg_range_block_to(unsigned lo, unsigned hi, unsigned ch, unsigned pr, unsigned *dst){
+    __shared__ unsigned slo[4], shi[4], sch[4], spr[4];
     for(int o = 32; o > 0; o >>= 1){
         unsigned t = __shfl_xor(lo, o, 64); lo = t < lo ? t : lo;
         t = __shfl_xor(hi, o, 64); hi = t > hi ? t : hi;
+        ch += __shfl_xor(ch, o, 64); pr += __shfl_xor(pr, o, 64);
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if(lane == 0){ slo[w] = lo; shi[w] = hi; }
+    if(lane == 0){ slo[w] = lo; shi[w] = hi; sch[w] = ch; spr[w] = pr; }
     __syncthreads();
     if(threadIdx.x == 0){
-        for(int q = 1; q < 4; q++){ lo = slo[q] < lo ? slo[q] : lo; hi = shi[q] > hi ? shi[q] : hi; }
-        part[2 * blockIdx.x] = lo; part[2 * blockIdx.x + 1] = hi;
+        for(int q = 1; q < 4; q++){ lo = slo[q] < lo ? slo[q] : lo; hi = shi[q] > hi ? shi[q] : hi; ch += sch[q]; pr += spr[q]; }
+        dst[0] = lo; dst[1] = hi; dst[2] = ch; dst[3] = pr;
     }
 }
+// per-workgroup partial into part[4 * block ..]
+__device__ __forceinline__ void sg_range_block(unsigned lo, unsigned hi, unsigned ch, unsigned pr, unsigned *part){
+    sg_range_block_to(lo, hi, ch, pr, part + 4 * blockIdx.x);
+}
+__global__ void __launch_bounds__(256) k_sg_urange_sample(const int32_t *x, L n, L stride, unsigned *part){
+    unsigned lo = 0xFFFFFFFFu, hi = 0, ch = 0, pr = 0;
+    const L i = ((L)blockIdx.x * 256 + threadIdx.x) * stride * 16;
+    unsigned prev = 0;
+    for(int j = 0; j < 16 && i + j < n; j++){
+        const unsigned v = (unsigned)x[i + j]; lo = v < lo ? v : lo; hi = v > hi ? v : hi;
+        if(j > 0){ ch += v != prev; pr++; }
+        prev = v;
+    }
+    sg_range_block(lo, hi, ch, pr, part);
+}
 __global__ void __launch_bounds__(256) k_sg_urange(const int32_t *x, L n, int vec, unsigned *part){
-    __shared__ unsigned slo[4], shi[4];
-    unsigned lo = 0xFFFFFFFFu, hi = 0;
+    unsigned lo = 0xFFFFFFFFu, hi = 0, ch = 0, pr = 0;
     const L S = (L)gridDim.x * 256 * 4;
     L i = ((L)blockIdx.x * 256 + threadIdx.x) * 4;
     if(vec){
@@ -1011,6 +1026,7 @@ __global__ void __launch_bounds__(256) k_sg_urange(const int32_t *x, L n, int ve
                 const unsigned b0 = wv[u].x > wv[u].y ? wv[u].x : wv[u].y, b1 = wv[u].z > wv[u].w ? wv[u].z : wv[u].w;
                 const unsigned mn = a0 < a1 ? a0 : a1, mx = b0 > b1 ? b0 : b1;
                 lo = mn < lo ? mn : lo; hi = mx > hi ? mx : hi;
+                ch += (wv[u].x != wv[u].y) + (wv[u].y != wv[u].z) + (wv[u].z != wv[u].w); pr += 3;
             }
         }
     }
@@ -1019,36 +1035,23 @@ __global__ void __launch_bounds__(256) k_sg_urange(const int32_t *x, L n, int ve
             const uint4 w4 = *(const uint4*)(x + i);
             const unsigned a0 = min(w4.x, w4.y), a1 = min(w4.z, w4.w), b0 = max(w4.x, w4.y), b1 = max(w4.z, w4.w);
             lo = min(lo, min(a0, a1)); hi = max(hi, max(b0, b1));
+            ch += (w4.x != w4.y) + (w4.y != w4.z) + (w4.z != w4.w); pr += 3;
         } else
-            for(int j = 0; j < 4 && i + j < n; j++){ const unsigned v = (unsigned)x[i + j]; lo = v < lo ? v : lo; hi = v > hi ? v : hi; }
+            for(int j = 0; j < 4 && i + j < n; j++){
+                const unsigned v = (unsigned)x[i + j]; lo = v < lo ? v : lo; hi = v > hi ? v : hi;
+                if(j > 0){ ch += v != (unsigned)x[i + j - 1]; pr++; }
+            }
     }
-    for(int o = 32; o > 0; o >>= 1){
-        unsigned t = __shfl_xor(lo, o, 64); lo = t < lo ? t : lo;
-        t = __shfl_xor(hi, o, 64); hi = t > hi ? t : hi;
-    }
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if(lane == 0){ slo[w] = lo; shi[w] = hi; }
-    __syncthreads();
-    if(threadIdx.x == 0){
-        for(int q = 1; q < 4; q++){ lo = slo[q] < lo ? slo[q] : lo; hi = shi[q] > hi ? shi[q] : hi; }
-        part[2 * blockIdx.x] = lo; part[2 * blockIdx.x + 1] = hi;
-    }
+    sg_range_block(lo, hi, ch, pr, part);
 }
+// out[0..3] = (lo, hi, changes, pairs) over the workgroups' partials
 __global__ void __launch_bounds__(256) k_sg_urange_final(const unsigned *part, int nblocks, unsigned *out){
-    __shared__ unsigned slo[4], shi[4];
-    unsigned lo = 0xFFFFFFFFu, hi = 0;
-    for(int k = threadIdx.x; k < nblocks; k += 256){ lo = part[2 * k] < lo ? part[2 * k] : lo; hi = part[2 * k + 1] > hi ? part[2 * k + 1] : hi; }
-    for(int o = 32; o > 0; o >>= 1){
-        unsigned t = __shfl_xor(lo, o, 64); lo = t < lo ? t : lo;
-        t = __shfl_xor(hi, o, 64); hi = t > hi ? t : hi;
+    unsigned lo = 0xFFFFFFFFu, hi = 0, ch = 0, pr = 0;
+    for(int k = threadIdx.x; k < nblocks; k += 256){
+        lo = part[4 * k] < lo ? part[4 * k] : lo; hi = part[4 * k + 1] > hi ? part[4 * k + 1] : hi;
+        ch += part[4 * k + 2]; pr += part[4 * k + 3];
     }
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if(lane == 0){ slo[w] = lo; shi[w] = hi; }
-    __syncthreads();
-    if(threadIdx.x == 0){
-        for(int q = 1; q < 4; q++){ lo = slo[q] < lo ? slo[q] : lo; hi = shi[q] > hi ? shi[q] : hi; }
-        out[0] = lo; out[1] = hi;
-    }
+    sg_range_block_to(lo, hi, ch, pr, out);
 }
 // workgroups of a whole-column pass that keeps SG_SCAN_UNROLL loads per thread in flight
 static int sg_scan_grid(L n, int rows_per_load){
@@ -1129,18 +1132,22 @@ static int32_t small_group_trie(hpa_ctx *c, V z, const uint8_t *a, const uint8_t
     R 0;
 }
 // i32 group on at most 256 consecutive unsigned keys; *done false otherwise
-static int32_t small_group_i32(hpa_ctx *c, V z, const int32_t *x, L n, bool *done){
+// *change_frac: the share of consecutive row pairs (all of them, or those of the sampled chunks)
+// whose keys differ -- low for columns clustered by key
+static int32_t small_group_i32(hpa_ctx *c, V z, const int32_t *x, L n, bool *done, double *change_frac){
     *done = false;
+    *change_frac = 1.0;
     int32_t e;
     const int vec = (uintptr_t)x % 16 == 0 ? 1 : 0;
     for(int sample = n >= SG_SAMPLE_MIN ? 1 : 0; sample >= 0; sample--){
         int grid = sample ? (int)ceil_div(ceil_div(n, 16 * (L)SG_SAMPLE_STRIDE), 256) : sg_scan_grid(n, 4);
-        unsigned *rg = (unsigned*)dalloc(c, 8 * (L)(grid + 1)); if(!rg) R HPA_E_MEMORY;
-        if(sample) hipLaunchKernelGGL(k_sg_urange_sample, dim3(grid), dim3(256), 0, c->stream, x, n, (L)SG_SAMPLE_STRIDE, rg + 2);
-        else hipLaunchKernelGGL(k_sg_urange, dim3(grid), dim3(256), 0, c->stream, x, n, vec, rg + 2);
-        hipLaunchKernelGGL(k_sg_urange_final, dim3(1), dim3(256), 0, c->stream, rg + 2, grid, rg);
+        unsigned *rg = (unsigned*)dalloc(c, 16 * (L)(grid + 1)); if(!rg) R HPA_E_MEMORY;
+        if(sample) hipLaunchKernelGGL(k_sg_urange_sample, dim3(grid), dim3(256), 0, c->stream, x, n, (L)SG_SAMPLE_STRIDE, rg + 4);
+        else hipLaunchKernelGGL(k_sg_urange, dim3(grid), dim3(256), 0, c->stream, x, n, vec, rg + 4);
+        hipLaunchKernelGGL(k_sg_urange_final, dim3(1), dim3(256), 0, c->stream, rg + 4, grid, rg);
         if((e = check_launch("group_small_range"))) R e;
-        unsigned h[2]; if((e = d2h(c, h, rg, 8))) R e;
+        unsigned h[4]; if((e = d2h(c, h, rg, 16))) R e;
+        if(h[3] > 0) *change_frac = (double)h[2] / (double)h[3];
         if(h[1] - h[0] >= (unsigned)SG_BINS) R 0;
         SgSrc s; memset(&s, 0, sizeof(s));
         s.x = x; s.base = h[0]; s.vec = vec;
@@ -1171,12 +1178,193 @@ extern "C" int32_t pfnGroupTrie(hpa_V z, hpa_V x){
     R runs_to_group_u32(c, z, sk, sv, n);
 }
 
+// Group of an i32 column whose equal keys sit in runs of consecutive rows (a fact table stored in
+// key order of another table, say).  The stable sort that defines the group only has to order
+// the runs: runs of one key keep their input order, and each run expands to ascending row
+// numbers.  A run is cut at every cap-th row, cap = 2^cb with cb = min(6, 32 - bits of the row
+// count), so that (start << cb) | (length - 1) fits the 32-bit value the sort carries.  Passes:
+// run heads counted per tile, a scan, the runs' (key, packed start and length) written; a stable
+// radix sort of the run keys; the run lengths and their scan (flat positions); the expansion of
+// each run into the flat list; group heads over the sorted run keys.  Taken when consecutive
+// rows show few key changes; abandoned (nothing written) when the runs turn out to be more than
+// GR_RUNS_MAX_SHARE of the rows.  Its extra launches and host read pay off from about 2^24 rows.
+#define GR_RUNS_MIN (1 << 24)
+#define GR_RUNS_MAX_CHANGE 0.4
+#define GR_RUNS_MAX_SHARE 0.55
+__device__ __forceinline__ bool gr_head(const int32_t *x, L i, unsigned cap_mask){
+    return i == 0 || ((uint64_t)i & cap_mask) == 0 || x[i] != x[i - 1];
+}
+__global__ void __launch_bounds__(256) k_gr_head_count(const int32_t *x, L n, unsigned cap_mask, uint32_t *tcount){
+    const L base = (L)blockIdx.x * RUN_TILE;
+    uint32_t c = 0;
+    #pragma unroll
+    for(int k = 0; k < RUN_ITEMS; k++){
+        const L i = base + (L)k * 256 + threadIdx.x;
+        c += (i < n && gr_head(x, i, cap_mask)) ? 1u : 0u;
+    }
+    __shared__ uint32_t sh[4];
+    for(int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+    if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if(threadIdx.x == 0) tcount[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+// run r (input order): rkey[r] = its key, rval[r] = (first row << cb) | (length - 1)
+__global__ void __launch_bounds__(256) k_gr_head_write(const int32_t *x, L n, unsigned cap_mask, int cb, const int64_t *toff,
+                                                       uint32_t *rkey, uint32_t *rval){
+    __shared__ uint32_t s_cnt[RUN_ITEMS * 4];
+    const L base = (L)blockIdx.x * RUN_TILE;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t bal[RUN_ITEMS];
+    #pragma unroll
+    for(int k = 0; k < RUN_ITEMS; k++){
+        const L i = base + (L)k * 256 + threadIdx.x;
+        bal[k] = __ballot(i < n && gr_head(x, i, cap_mask));
+        if(lane == 0) s_cnt[k * 4 + w] = (uint32_t)__popcll(bal[k]);
+    }
+    __syncthreads();
+    if(w == 0){
+        const uint32_t mine = s_cnt[lane];
+        uint32_t inc = mine;
+        for(int o = 1; o < 64; o <<= 1){ const uint32_t t = __shfl_up(inc, o, 64); if(lane >= o) inc += t; }
+        s_cnt[lane] = inc - mine;
+    }
+    __syncthreads();
+    const int64_t pre = toff[blockIdx.x];
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    #pragma unroll
+    for(int k = 0; k < RUN_ITEMS; k++){
+        if((bal[k] >> lane) & 1ull){
+            const L i = base + (L)k * 256 + threadIdx.x;
+            const int64_t r = pre + s_cnt[k * 4 + w] + __popcll(bal[k] & lt);
+            L d = 1;                                   // run length: rows up to the next head (at most cap)
+            while(i + d < n && !gr_head(x, i + d, cap_mask)) d++;
+            rkey[r] = (uint32_t)x[i]; rval[r] = ((uint32_t)i << cb) | (uint32_t)(d - 1);
+        }
+    }
+}
+__global__ void k_gr_len(const uint32_t *sv, L nr, unsigned len_mask, uint32_t *len){
+    for(L j = (L)blockIdx.x * blockDim.x + threadIdx.x; j < nr; j += (L)gridDim.x * blockDim.x) len[j] = (sv[j] & len_mask) + 1;
+}
+// flat[fpos[j] + t] = (first row of sorted run j) + t; a wave writes the contiguous output range
+// of its 64 runs together, each lane finding its run by a search over the 64 run starts in LDS
+__global__ void __launch_bounds__(256) k_gr_expand(const uint32_t *sv, int cb, const int64_t *fpos, L nr, L n, int64_t *flat){
+    __shared__ int64_t sp[4][65];
+    __shared__ uint32_t ss[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const L nw = ((L)gridDim.x * blockDim.x) >> 6;
+    for(L base = (((L)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64; base < nr; base += nw * 64){
+        const L j = base + lane;
+        const bool valid = j < nr;
+        sp[w][lane] = valid ? fpos[j] : INT64_MAX;
+        ss[w][lane] = valid ? sv[j] >> cb : 0u;
+        if(lane == 0) sp[w][64] = base + 64 < nr ? fpos[base + 64] : n;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int64_t P0 = sp[w][0], P1 = sp[w][64];
+        for(int64_t p = P0 + lane; p < P1; p += 64){
+            int lo = 0, hi = 63;                      // last run whose start is <= p
+            while(lo < hi){ const int mid = (lo + hi + 1) >> 1; if(sp[w][mid] <= p) lo = mid; else hi = mid - 1; }
+            flat[p] = (int64_t)ss[w][lo] + (p - sp[w][lo]);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+// heads of the sorted run keys: group key = the first row of the group's first run, start =
+// its flat position
+__global__ void __launch_bounds__(256) k_gr_group_write(const uint32_t *sk, const uint32_t *sv, int cb, const int64_t *fpos, L nr,
+                                                        const int64_t *toff, int64_t *keys, int64_t *start){
+    __shared__ uint32_t s_cnt[RUN_ITEMS * 4];
+    const L base = (L)blockIdx.x * RUN_TILE;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t bal[RUN_ITEMS];
+    #pragma unroll
+    for(int k = 0; k < RUN_ITEMS; k++){
+        const L i = base + (L)k * 256 + threadIdx.x;
+        bal[k] = __ballot(i < nr && (i == 0 || sk[i] != sk[i - 1]));
+        if(lane == 0) s_cnt[k * 4 + w] = (uint32_t)__popcll(bal[k]);
+    }
+    __syncthreads();
+    if(w == 0){
+        const uint32_t mine = s_cnt[lane];
+        uint32_t inc = mine;
+        for(int o = 1; o < 64; o <<= 1){ const uint32_t t = __shfl_up(inc, o, 64); if(lane >= o) inc += t; }
+        s_cnt[lane] = inc - mine;
+    }
+    __syncthreads();
+    const int64_t pre = toff[blockIdx.x];
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    #pragma unroll
+    for(int k = 0; k < RUN_ITEMS; k++){
+        if((bal[k] >> lane) & 1ull){
+            const L i = base + (L)k * 256 + threadIdx.x;
+            const int64_t g = pre + s_cnt[k * 4 + w] + __popcll(bal[k] & lt);
+            keys[g] = (int64_t)(sv[i] >> cb); start[g] = fpos[i];
+        }
+    }
+}
+static int32_t group_i32_runs(hpa_ctx *c, V z, const int32_t *x, L n, bool *done){
+    *done = false;
+    int32_t e;
+    const int sbits = n > 1 ? 64 - __builtin_clzll((unsigned long long)(n - 1)) : 1;
+    const int cb = 32 - sbits < 6 ? 32 - sbits : 6;
+    if(cb < 2) R 0;                                          // runs of at most 2 rows: not worth it
+    const unsigned cap_mask = (1u << cb) - 1;
+    const L mark = c->dcur;
+    const L nt = ceil_div(n, RUN_TILE);
+    uint32_t *tcount = (uint32_t*)dalloc(c, 4 * nt);
+    int64_t *toff = (int64_t*)dalloc(c, 8 * (nt + 1));
+    if(!tcount || !toff) R HPA_E_MEMORY;
+    ProfScope ps("group_runs", c->stream);
+    hipLaunchKernelGGL(k_gr_head_count, dim3((unsigned)nt), dim3(256), 0, c->stream, x, n, cap_mask, tcount);
+    if((e = scan_u32_to_i64(c, tcount, toff, nt, toff + nt))) R e;
+    int64_t nr = 0; if((e = d2h(c, &nr, toff + nt, 8))) R e;
+    if((double)nr > GR_RUNS_MAX_SHARE * (double)n){ c->dcur = mark; R 0; }
+    uint32_t *rkey = (uint32_t*)dalloc(c, 4 * nr), *rval = (uint32_t*)dalloc(c, 4 * nr);
+    if(!rkey || !rval) R HPA_E_MEMORY;
+    hipLaunchKernelGGL(k_gr_head_write, dim3((unsigned)nt), dim3(256), 0, c->stream, x, n, cap_mask, cb, (const int64_t*)toff, rkey, rval);
+    if((e = check_launch("group_runs"))) R e;
+    uint32_t *sk, *sv;
+    if((e = radix_sort_pairs_u32(c, rkey, rval, nr, 0, 32, &sk, &sv))) R e;
+    uint32_t *len = (uint32_t*)dalloc(c, 4 * nr);
+    int64_t *fpos = (int64_t*)dalloc(c, 8 * (nr + 1));
+    int64_t *flat = (int64_t*)dalloc(c, 8 * n);
+    if(!len || !fpos || !flat) R HPA_E_MEMORY;
+    hipLaunchKernelGGL(k_gr_len, dim3(grid_for(nr, 256, 4)), dim3(256), 0, c->stream, (const uint32_t*)sv, (L)nr, cap_mask, len);
+    if((e = scan_u32_to_i64(c, len, fpos, nr, fpos + nr))) R e;
+    hipLaunchKernelGGL(k_gr_expand, dim3(grid_for(nr, 256, 1)), dim3(256), 0, c->stream, (const uint32_t*)sv, cb,
+                       (const int64_t*)fpos, (L)nr, n, flat);
+    // groups: heads of the sorted run keys
+    const L gt = ceil_div(nr, RUN_TILE);
+    uint32_t *gcount = (uint32_t*)dalloc(c, 4 * gt);
+    int64_t *goff = (int64_t*)dalloc(c, 8 * (gt + 1));
+    if(!gcount || !goff) R HPA_E_MEMORY;
+    hipLaunchKernelGGL(k_run_tilecount<uint32_t>, dim3((unsigned)gt), dim3(256), 0, c->stream, (const uint32_t*)sk, (const uint32_t*)sv,
+                       (const uint64_t*)nullptr, (L)nr, gcount);
+    if((e = scan_u32_to_i64(c, gcount, goff, gt, goff + gt))) R e;
+    int64_t G = 0; if((e = d2h(c, &G, goff + gt, 8))) R e;
+    int64_t *keys = (int64_t*)dalloc(c, 8 * G), *start = (int64_t*)dalloc(c, 8 * G), *cnt = (int64_t*)dalloc(c, 8 * G);
+    if(!keys || !start || !cnt) R HPA_E_MEMORY;
+    hipLaunchKernelGGL(k_gr_group_write, dim3((unsigned)gt), dim3(256), 0, c->stream, (const uint32_t*)sk, (const uint32_t*)sv, cb,
+                       (const int64_t*)fpos, (L)nr, (const int64_t*)goff, keys, start);
+    hipLaunchKernelGGL(k_run_counts, dim3(grid_for(G, 256, 1)), dim3(256), 0, c->stream, (const int64_t*)start, (L)G, n, cnt);
+    if((e = check_launch("group_runs"))) R e;
+    *done = true;
+    R make_group_dict(c, z, keys, cnt, flat, G, n);
+}
+
 static int32_t group_int(hpa_ctx *c, V z, V x){      // lib_group_by_normal_int, h_libs.cc:1295-1329
     L n = x->len; int32_t e;
     const void *p = nullptr;
     if(n > 0 && (e = dev_view(c, x, &p))) R e;
+    double change = 1.0;
     if(n > 0){
-        bool done; if((e = small_group_i32(c, z, (const int32_t*)p, n, &done)) || done) R e;
+        bool done; if((e = small_group_i32(c, z, (const int32_t*)p, n, &done, &change)) || done) R e;
+    }
+    if(n >= GR_RUNS_MIN && change <= GR_RUNS_MAX_CHANGE){
+        bool done; if((e = group_i32_runs(c, z, (const int32_t*)p, n, &done)) || done) R e;
     }
     // the unsigned radix key of an i32 is its bit pattern: the column is sorted in place of keys
     uint32_t *sk, *sv;

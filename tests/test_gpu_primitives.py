@@ -176,6 +176,50 @@ def test_group_int(gpu, orc, n, card):
     assert groups_of(gpu, gpu.pfnGroup(gpu.vec(x))) == groups_of(orc, orc.pfnGroup(orc.vec(x)))
 
 
+@pytest.mark.parametrize("nkeys,maxrun,recur", [(1_000_000, 30, True), (1_300_000, 30, False), (150_000, 200, True),
+                                                 (2000, 20000, True)])
+def test_group_int_clustered(gpu, orc, nkeys, maxrun, recur):
+    """Keys in runs of consecutive rows, over 2^24 rows (the run-compressed group): runs of
+    1..maxrun rows, cut at the 64-row cap, runs far longer than the cap, keys that come back in
+    later runs, negative keys (the group order is the unsigned order of the bit pattern).  Runs
+    of a few rows at 80M rows: test_group_int_clustered_large."""
+    r = rng(nkeys + maxrun)
+    keys = r.permutation(nkeys).astype(np.int64) * 7 - nkeys * 3
+    if recur:
+        keys = np.concatenate([keys, keys[r.integers(0, nkeys, nkeys // 3)]])
+    x = np.repeat(keys, r.integers(1, maxrun + 1, keys.size)).astype(np.int32)
+    assert x.size > (1 << 24)
+    gk, gc, gf = gpu.read_groups(gpu.pfnGroup(gpu.vec(x)))
+    ok, oc, of = orc.read_groups(orc.pfnGroup(orc.vec(x)))
+    assert np.array_equal(gk, ok) and np.array_equal(gc, oc) and np.array_equal(gf, of)
+
+
+def test_group_int_clustered_large(gpu):
+    """80M rows clustered in runs (past 2^26 rows the run cap shrinks to 32 rows): checked by the
+    properties that define the stable-sort group -- the flat list is a permutation of the rows,
+    keys are equal inside a group and strictly ascending (unsigned) from group to group, rows
+    ascend inside a group, and each group's key is its first row."""
+    r = rng(80)
+    keys = (r.permutation(20_000_000).astype(np.int64) * 5 - 50_000_000).astype(np.int32)
+    x = np.repeat(keys, r.integers(1, 8, keys.size))
+    x[r.integers(0, x.size, 1000)] = 12345                      # a key that recurs far apart
+    n = x.size
+    assert n > (1 << 26)
+    k, c, f = gpu.read_groups(gpu.pfnGroup(gpu.vec(x)))
+    assert f.size == n and c.sum() == n and k.size == c.size
+    seen = np.zeros(n, bool)
+    seen[f] = True
+    assert seen.all()
+    starts = np.cumsum(c) - c
+    head = np.zeros(n, bool)
+    head[starts] = True
+    xs = x[f].view(np.uint32)
+    assert np.array_equal(xs[1:] != xs[:-1], head[1:])        # key changes exactly at group heads
+    assert (np.diff(xs[starts].astype(np.int64)) > 0).all()    # groups in ascending unsigned order
+    assert (np.diff(f)[~head[1:]] > 0).all()                   # rows ascend inside a group
+    assert np.array_equal(k, f[starts])
+
+
 def test_group_long(gpu, orc):
     for x in (np.array([-1, 4294967295, -1, 7, 7], np.int64),
               rng(9).integers(-5, 5, 5000).astype(np.int64) * (1 << 33),
