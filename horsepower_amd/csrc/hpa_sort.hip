@@ -1529,14 +1529,6 @@ __global__ void k_perm_to_i64(const uint32_t *perm, int64_t *z, L n, int reverse
     for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x)
         z[i] = reverse ? perm[n - 1 - i] : perm[i];
 }
-// adjacent full-key equality along the final permutation (ties the reference may order
-// differently through its quicksort)
-__global__ void k_count_ties(const uint64_t *k, L n, unsigned long long *cnt){
-    unsigned long long c0 = 0;
-    for(L i = 1 + (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x) c0 += k[i] == k[i - 1];
-    for(int o = 32; o > 0; o >>= 1) c0 += __shfl_down(c0, o, 64);
-    if((threadIdx.x & 63) == 0 && c0) atomicAdd(cnt, c0);
-}
 
 // Stable permutation of x (list of columns, flags per column) in lexicographic order, ties by
 // ascending row.  LSD over columns: sort by the last column, then stably by each earlier one.
@@ -1829,6 +1821,135 @@ static int32_t order_vector(hpa_ctx *c, V z, V col, bool up){   // lib_order_by_
     R order_emit(c, z, perm, n, false);
 }
 
+// ---------------------------------------------------------------- list order: tie direction
+// The reference orders a list column by column (lib_quicksort_list, h_libs.cc:637-697): within a
+// run of rows equal on the earlier columns, a column that is already non-decreasing and not all
+// equal is kept for an ascending flag and *reversed in place* for a descending one
+// (lib_order_by_sorted, h_libs.cc:604-617); otherwise the run is sorted stably (integers) and
+// split into runs of equal values.  So the rows of every run stay in ascending or descending row
+// order, and the result is the lexicographic order with rows equal on every column in ascending
+// or descending row order -- descending where an odd number of such reversals lies above them.
+// order_perm breaks those full ties by ascending row; this pass finds the reversals and turns the
+// affected tie groups around.  It runs only for lists with a descending column and some full tie.
+//   lvl[i]: first column where the rows at positions i-1 and i of the order differ (nc: a full
+//   tie; 0 at i = 0).  Positions i-1, i share a run of level c (equal on columns < c) iff
+//   lvl[i] >= c.
+__global__ void k_ot_lvl(SmallCols s, const uint32_t *perm, L n, uint8_t *lvl, unsigned long long *nties){
+    unsigned long long t = 0;
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
+        int l = 0;
+        if(i > 0){
+            const uint32_t a = perm[i - 1], b = perm[i];
+            while(l < s.nc && okey_any(s.typ[l], s.p[l], a) == okey_any(s.typ[l], s.p[l], b)) l++;
+        }
+        lvl[i] = (uint8_t)l;
+        t += l == s.nc;
+    }
+    for(int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
+    if((threadIdx.x & 63) == 0 && t) atomicAdd(nties, t);
+}
+// b[i] = 1 where position i starts a run of level c (i > 0)
+__global__ void k_ot_bound(const uint8_t *lvl, L n, int c, uint32_t *b){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x) b[i] = i > 0 && lvl[i] < c;
+}
+// run id of each row (inclusive count of run starts up to its position)
+__global__ void k_ot_rowrun(const uint32_t *perm, const uint32_t *b, const int64_t *ex, L n, uint32_t *rowrun){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x) rowrun[perm[i]] = (uint32_t)(ex[i] + b[i]);
+}
+// rows sorted by (run, row): consecutive rows of one run in ascending row order; bit 1 of
+// rflag[run]: column c increases somewhere along them, bit 2: decreases somewhere
+__global__ void k_ot_check(SmallCols s, int c, const uint32_t *srun, const uint32_t *srow, L n, uint32_t *rflag){
+    for(L j = 1 + (L)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (L)gridDim.x * blockDim.x){
+        if(srun[j] != srun[j - 1]) continue;
+        const uint64_t a = okey_any(s.typ[c], s.p[c], srow[j - 1]), b = okey_any(s.typ[c], s.p[c], srow[j]);
+        if(a != b) atomicOr(&rflag[srun[j]], a < b ? 1u : 2u);
+    }
+}
+// a run whose rows run in direction d (0 ascending rows, 1 descending) is reversed when column
+// c, read in that direction, is non-decreasing and not all equal; its sub-runs take the new
+// direction
+__global__ void k_ot_flip(const uint32_t *b, const int64_t *ex, const uint32_t *rflag, L n, uint32_t *dir){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
+        const uint32_t f = rflag[ex[i] + b[i]];
+        const bool inc = f & 1u, dec = f & 2u, d = dir[i] != 0;
+        const bool flip = d ? (dec && !inc) : (inc && !dec);
+        if(flip) dir[i] = d ? 0u : 1u;
+    }
+}
+// group starts of the full-tie groups (runs of level nc)
+__global__ void k_ot_gstart(const uint32_t *b, const int64_t *ex, L n, int64_t *gs){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x)
+        if(i == 0 || b[i]) gs[ex[i] + b[i]] = i;
+}
+// tie groups in descending direction come out reversed
+__global__ void k_ot_apply(const uint32_t *perm, const uint32_t *b, const int64_t *ex, const int64_t *gs, const uint32_t *dir,
+                           L n, L ng, uint32_t *out){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
+        const L g = ex[i] + b[i];
+        const L lo = gs[g], hi = g + 1 < ng ? gs[g + 1] : n;
+        out[dir[i] ? lo + hi - 1 - i : i] = perm[i];
+    }
+}
+static int32_t order_list_ties(hpa_ctx *c, V x, const uint8_t *flags, uint32_t **perm_io, L n){
+    const L nc = x->len;
+    bool any_desc = false;
+    for(L q = 0; q < nc; q++) any_desc |= flags[q] == 0;
+    if(!any_desc || n < 2 || nc > SMALL_COLS) R 0;
+    int32_t e;
+    SmallCols s; memset(&s, 0, sizeof(s)); s.nc = (int)nc;
+    for(L q = 0; q < nc; q++){
+        V col = ((V)x->g) + q;
+        if(col->typ == HPA_S || col->typ == HPA_X || col->typ == HPA_Q) R 0;
+        const void *p; if((e = dev_view(c, col, &p))) R e;
+        s.typ[q] = col->typ; s.p[q] = p; s.desc[q] = flags[q] ? 0 : 1; s.bits[q] = key_bits_of(col->typ);
+    }
+    const uint32_t *perm = *perm_io;
+    const int g = grid_for(n, 256, 4);
+    uint8_t *lvl = (uint8_t*)dalloc(c, n);
+    uint32_t *dir = (uint32_t*)dalloc(c, 4 * n);
+    unsigned long long *nt = (unsigned long long*)dalloc(c, 16);
+    if(!lvl || !dir || !nt) R HPA_E_MEMORY;
+    hipLaunchKernelGGL(k_br_init, dim3(1), dim3(64), 0, c->stream, (uint32_t*)nt, (L)2, (unsigned long long*)nullptr);
+    hipLaunchKernelGGL(k_ot_lvl, dim3(g), dim3(256), 0, c->stream, s, perm, n, lvl, nt);
+    if((e = check_launch("order_ties"))) R e;
+    unsigned long long ties = 0; if((e = d2h(c, &ties, nt, 8))) R e;
+    if(ties == 0) R 0;                                        // the lexicographic order is unique
+    ProfScope ps("order_ties", c->stream);
+    hipLaunchKernelGGL(k_br_init, dim3(g), dim3(256), 0, c->stream, dir, n, (unsigned long long*)nullptr);
+    uint32_t *b = (uint32_t*)dalloc(c, 4 * n), *rowrun = (uint32_t*)dalloc(c, 4 * n), *rflag = (uint32_t*)dalloc(c, 4 * n);
+    int64_t *ex = (int64_t*)dalloc(c, 8 * (n + 1));
+    if(!b || !rowrun || !rflag || !ex) R HPA_E_MEMORY;
+    for(int q = 0; q < (int)nc; q++){
+        if(flags[q]) continue;                               // ascending columns never reverse a run
+        hipLaunchKernelGGL(k_ot_bound, dim3(g), dim3(256), 0, c->stream, (const uint8_t*)lvl, n, q, b);
+        if((e = scan_u32_to_i64(c, b, ex, n, ex + n))) R e;
+        hipLaunchKernelGGL(k_ot_rowrun, dim3(g), dim3(256), 0, c->stream, perm, (const uint32_t*)b, (const int64_t*)ex, n, rowrun);
+        hipLaunchKernelGGL(k_br_init, dim3(g), dim3(256), 0, c->stream, rflag, n, (unsigned long long*)nullptr);
+        const L mark = c->dcur;
+        uint32_t *srun, *srow;
+        if((e = radix_sort_pairs_u32(c, rowrun, nullptr, n, 0, 32, &srun, &srow))) R e;
+        hipLaunchKernelGGL(k_ot_check, dim3(g), dim3(256), 0, c->stream, s, q, (const uint32_t*)srun, (const uint32_t*)srow, n, rflag);
+        hipLaunchKernelGGL(k_ot_flip, dim3(g), dim3(256), 0, c->stream, (const uint32_t*)b, (const int64_t*)ex, (const uint32_t*)rflag,
+                           n, dir);
+        if((e = check_launch("order_ties"))) R e;
+        c->dcur = mark;
+    }
+    // the full-tie groups, each turned around where its direction is descending
+    hipLaunchKernelGGL(k_ot_bound, dim3(g), dim3(256), 0, c->stream, (const uint8_t*)lvl, n, (int)nc, b);
+    if((e = scan_u32_to_i64(c, b, ex, n, ex + n))) R e;
+    int64_t starts = 0; if((e = d2h(c, &starts, ex + n, 8))) R e;
+    const L ng = starts + 1;                                 // group starts after position 0, plus the first
+    int64_t *gs = (int64_t*)dalloc(c, 8 * ng);
+    uint32_t *out = (uint32_t*)dalloc(c, 4 * n);
+    if(!gs || !out) R HPA_E_MEMORY;
+    hipLaunchKernelGGL(k_ot_gstart, dim3(g), dim3(256), 0, c->stream, (const uint32_t*)b, (const int64_t*)ex, n, gs);
+    hipLaunchKernelGGL(k_ot_apply, dim3(g), dim3(256), 0, c->stream, perm, (const uint32_t*)b, (const int64_t*)ex, (const int64_t*)gs,
+                       (const uint32_t*)dir, n, ng, out);
+    if((e = check_launch("order_ties"))) R e;
+    *perm_io = out;
+    R 0;
+}
+
 // pfnOrderBy (h_primitive.cc:1769-1816)
 extern "C" int32_t pfnOrderBy(hpa_V z, hpa_V x, hpa_V y){
     hpa_ctx *c = ctx_cur();
@@ -1854,6 +1975,7 @@ extern "C" int32_t pfnOrderBy(hpa_V z, hpa_V x, hpa_V y){
         uint32_t *perm = nullptr; bool ties;
         if(nc == 0 || n == 0){ void *dz; R out_init(c, z, HPA_L, 0, &dz); }
         if((e = order_perm(c, x, fl.data(), &perm, &ties))) R e;
+        if((e = order_list_ties(c, x, fl.data(), &perm, n))) R e;
         R order_emit(c, z, perm, n, false);
     }
     // lib_order_basic (h_primitive.cc:1760-1767)

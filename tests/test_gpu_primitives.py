@@ -339,6 +339,28 @@ def test_order_lists(gpu, orc, n):
         assert np.array_equal(g, o), fl
 
 
+@pytest.mark.parametrize("n", [500, 3000, 60000])
+def test_order_lists_sorted_runs(gpu, orc, n):
+    """The reference reverses a run in place when its column is already non-decreasing and the
+    flag is descending (lib_order_by_sorted), which puts rows tied on every column in
+    descending row order, and can flip them back one level down.  Integer columns (merge-sorted
+    in the reference, so the tie order is defined): a sorted first column, columns sorted
+    inside the runs of the previous one, constant columns, both flags."""
+    r = rng(n + 15)
+    rows = np.arange(n)
+    cases = [
+        [np.sort(r.integers(0, 10, n)).astype(np.int64), r.integers(0, 3, n).astype(np.int32), np.zeros(n, np.int32)],
+        [r.integers(0, 4, n).astype(np.int32), (rows // 3).astype(np.int64), np.full(n, 5, np.int64)],
+        [np.sort(r.integers(0, 6, n)).astype(np.int32), (-(rows // 4)).astype(np.int64), (rows % 2).astype(np.int32)],
+        [r.integers(0, 3, n).astype(np.int64), r.integers(0, 2, n).astype(np.int32), (rows // 7).astype(np.int32)],
+    ]
+    for cols in cases:
+        for fl in ([0, 0, 0], [0, 1, 0], [1, 0, 0], [0, 0, 1], [1, 1, 0]):
+            g = gpu.read(gpu.pfnOrderBy(gpu.list([gpu.vec(c) for c in cols]), gpu.bools(fl)))
+            o = orc.read(orc.pfnOrderBy(orc.list([orc.vec(c) for c in cols]), orc.bools(fl)))
+            assert np.array_equal(g, o), (fl, [c.dtype for c in cols])
+
+
 @pytest.mark.parametrize("nx,ny,card", [(40, 300, 30), (300, 40, 30), (100, 100, 10), (5, 50, 8),
                                         (50, 5, 8), (100000, 400000, 150000), (0, 10, 3), (20, 0, 3)])
 def test_join(gpu, orc, nx, ny, card):
