@@ -1490,7 +1490,26 @@ int32_t is_ordered_dev(hpa_ctx *c, V x, bool *ord){
     R 0;
 }
 
-// pfnGroupBasic (h_primitive.cc:1002-1057)
+extern "C" int32_t pfnOrderBy(hpa_V z, hpa_V x, hpa_V y);
+__global__ void k_perm64_to_u32(const int64_t *p, L n, uint32_t *q){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x) q[i] = (uint32_t)p[i];
+}
+// pfnGroupBasic (h_primitive.cc:1002-1057).  Lists and the types other than I/L are grouped along
+// the order pfnOrderBy gives them with every flag true (h_primitive.cc:1017-1020) -- its case 1
+// for two i32 columns, its single-vector rules for one column -- cut into runs of equal rows
+// (lib_get_group_by, h_libs.cc:776-809).
+static int32_t group_along_order(hpa_ctx *c, V z, V x, int nc, V cols, L n){
+    int32_t e;
+    std::vector<uint8_t> up((size_t)nc, 1);
+    V0 y; memset(&y, 0, sizeof(y)); y.typ = HPA_B; y.len = nc; y.g = (char*)up.data();
+    V0 t; memset(&t, 0, sizeof(t));
+    if((e = pfnOrderBy(&t, x, &y))) R e;
+    uint32_t *perm = (uint32_t*)dalloc(c, 4 * (n > 0 ? n : 1)); if(!perm) R HPA_E_MEMORY;
+    if(n > 1) hipLaunchKernelGGL(k_perm64_to_u32, dim3(grid_for(n, 256, 4)), dim3(256), 0, c->stream, (const int64_t*)t.g, n, perm);
+    else if(n == 1 && (e = iota_u32(c, perm, 1))) R e;
+    if((e = check_launch("group"))) R e;
+    R runs_along_perm(c, z, nc, cols, perm, n);
+}
 extern "C" int32_t pfnGroupBasic(hpa_V z, hpa_V x){
     hpa_ctx *c = ctx_cur();
     if(x->typ == HPA_I) R group_int(c, z, x);
@@ -1500,18 +1519,11 @@ extern "C" int32_t pfnGroupBasic(hpa_V z, hpa_V x){
         if(nc == 0) R HPA_E_DOMAIN;
         L n = ((V)x->g)->len;
         for(L q = 0; q < nc; q++){ V col = ((V)x->g) + q; if(!is_basic(col->typ)) R HPA_E_DOMAIN; if(col->len != n) R HPA_E_MATCH; }
-        std::vector<uint8_t> up((size_t)nc, 1);
-        uint32_t *perm; bool ties; int32_t e;
-        if((e = order_perm(c, x, up.data(), &perm, &ties))) R e;
-        R runs_along_perm(c, z, (int)nc, (V)x->g, perm, n);
+        R group_along_order(c, z, x, (int)nc, (V)x->g, n);
     }
     if(is_basic(x->typ)){
         if(x->typ == HPA_S || x->typ == HPA_X || x->typ == HPA_Q) R HPA_E_NOT_IMPL;
-        V0 lst; memset(&lst, 0, sizeof(lst));
-        lst.typ = HPA_G; lst.len = 1; lst.g = (char*)x;
-        uint8_t up = 1; uint32_t *perm; bool ties; int32_t e;
-        if((e = order_perm(c, &lst, &up, &perm, &ties))) R e;
-        R runs_along_perm(c, z, 1, x, perm, x->len);
+        R group_along_order(c, z, x, 1, x, x->len);
     }
     R HPA_E_DOMAIN;
 }

@@ -220,6 +220,23 @@ def test_group_int_clustered_large(gpu):
     assert np.array_equal(k, f[starts])
 
 
+@pytest.mark.parametrize("n", [1, 700, 5000, 70000])
+def test_group_along_order(gpu, orc, n):
+    """Lists and non-integer vectors group along pfnOrderBy's order with every flag true: two
+    i32 columns take its case 1 (packed (x0 << 32) | x1, x1 sign-extended, so negative x1 values
+    of different x0 share a key and equal rows can form separate runs), one i32 column its
+    unsigned radix order, chars their byte order."""
+    r = rng(n + 16)
+    a = r.integers(-3, 3, n).astype(np.int32)
+    b = r.integers(-2, 2, n).astype(np.int32)
+    ch = r.integers(0, 256, n).astype(np.uint8)
+    for cols in ([a, b], [a], [b, a], [a, b, a]):
+        zg = gpu.pfnGroup(gpu.list([gpu.vec(v) for v in cols]))
+        zo = orc.pfnGroup(orc.list([orc.vec(v) for v in cols]))
+        assert groups_of(gpu, zg) == groups_of(orc, zo), len(cols)
+    assert groups_of(gpu, gpu.pfnGroup(gpu.vec(ch, H_C))) == groups_of(orc, orc.pfnGroup(orc.vec(ch, H_C)))
+
+
 def test_group_long(gpu, orc):
     for x in (np.array([-1, 4294967295, -1, 7, 7], np.int64),
               rng(9).integers(-5, 5, 5000).astype(np.int64) * (1 << 33),
