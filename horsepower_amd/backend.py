@@ -270,6 +270,42 @@ class GpuBackend(Backend):
             raise HorseError(rc, "hpaFusedGroupAgg", self.lib.detail())
         return zf, zk, za
 
+    def fused_group_agg_host(self, keys, xs, cond, spec, cap=64):
+        """fused_group_agg with its results read back in the same synchronisation
+        (hpaFusedGroupAggHost): ([key arrays], [aggregate arrays]) as numpy arrays, or None when
+        the key cardinality per workgroup exceeds the fused kernel's slots.  More than `cap`
+        groups are read from the result nodes instead."""
+        from .vnode import HorseError, aggs
+        nk, na = len(keys), len(spec)
+        zk = [self.node() for _ in range(nk)]
+        za = [self.node() for _ in range(na)]
+        hit = self._aggs.get(id(spec))
+        if hit is None or hit[0] is not spec:
+            hit = (spec, aggs(spec))
+            self._aggs[id(spec)] = hit
+        buf = np.empty(cap * (nk + 8 * na) + 16, dtype=np.uint8)
+        base = buf.__array_interface__["data"][0]
+        offs = [k * cap for k in range(nk)] + [((nk * cap + 7) & ~7) + 8 * cap * a for a in range(na)]
+        ng = C.c_int64(0)
+        rc = self.lib.dll.hpaFusedGroupAggHost(nk, (V * nk)(*keys), len(xs), (V * len(xs))(*xs),
+                                               C.byref(cond) if cond is not None else None, na,
+                                               C.cast(hit[1], C.c_void_p), None, (V * nk)(*zk), (V * na)(*za),
+                                               C.c_int64(cap), (C.c_void_p * nk)(*[base + o for o in offs[:nk]]), None,
+                                               (C.c_void_p * na)(*[base + o for o in offs[nk:]]), C.byref(ng))
+        if rc == 99:
+            return None
+        if rc != 0:
+            raise HorseError(rc, "hpaFusedGroupAggHost", self.lib.detail())
+        g = ng.value
+        if g > cap:
+            out = self.read_many(zk + za)
+            return out[:nk], out[nk:]
+        out = []
+        for z, o in zip(zk + za, offs):
+            dt = _dtype_of(z.contents.typ)
+            out.append(buf[o:o + g * dt.itemsize].view(dt))
+        return out[:nk], out[nk:]
+
     def group_reduce(self, kind, x, groups):
         z = self.node()
         self.lib.rc("hpaGroupReduce", z, kind, x, groups)

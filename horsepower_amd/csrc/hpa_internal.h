@@ -113,6 +113,8 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned g){
 // (device pointer, may be null).
 int32_t scan_i64(hpa_ctx *c, const int64_t *in, int64_t *out, L n, int64_t *total_dev);
 int32_t scan_u32_to_i64(hpa_ctx *c, const uint32_t *in, int64_t *out, L n, int64_t *total_dev);
+// device ranges into host memory with one synchronisation (pinned staging when they fit)
+int32_t read_staged(hpa_ctx *c, int n, const void *const *src, void *const *dst, const int64_t *bytes);
 // promote a device vector to type `to` (promoteValue, backend/h_system.cc:168-252)
 int32_t promote_dev(hpa_ctx *c, V x, int32_t to, const void **p);
 // gather: dst[i] = src[idx[i]] for element width w

@@ -487,7 +487,9 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
          "    return r;\n"
          "}\n"
          "struct A { const void *c[" << EXPR_MAX_COLS << "]; const void *k0; const void *k1; i64 n; void *part; unsigned *used; int *ovf; unsigned kx0, kx1; " KONST_FIELDS " };\n"
-         "struct P { unsigned code, valid; u64 cnt; i64 first; double f[2 * NA]; i64 q[NA]; };\n"
+         // a partial slot; the slots' codes live in their own array after the G * NS slots, so the
+         // merge tests 64 slots' codes with one line and loads only the slots that match
+         "struct P { u64 cnt; i64 first; double f[2 * NA]; i64 q[NA]; };\n"
          // LDS hash of key code -> slot: entry = code << 8 | slot; 0xFFFFFFFF empty; slot 0xFE busy, 0xFD overflow
          "__device__ __forceinline__ unsigned slot_of(unsigned *ht, unsigned *s_code, unsigned *s_used, unsigned code){\n"
          "    unsigned h = ((code * 2654435761u) >> 26) & (HT - 1);\n"
@@ -615,7 +617,8 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
         }
         s << " }\n        __syncthreads();\n"
              "        if(threadIdx.x == 0){\n"
-             "            P r; r.code = s_code[" << q << "]; r.valid = 1; r.cnt = 0; r.first = 9223372036854775807LL;\n"
+             "            P r; r.cnt = 0; r.first = 9223372036854775807LL;\n"
+             "            ((unsigned*)((P*)a.part + (i64)gridDim.x * NS))[(i64)blockIdx.x * NS + " << q << "] = s_code[" << q << "];\n"
              "            for(int ww = 0; ww < 4; ww++){ r.cnt += (u64)shq[ww][NA]; r.first = shq[ww][NA + 1] < r.first ? shq[ww][NA + 1] : r.first; }\n";
         for(int k = 0; k < (int)acc.size(); k++){
             if(acc[k].f && acc[k].cls == 0)
@@ -638,9 +641,10 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
          "    __shared__ unsigned pre[1024];\n"
          "    __shared__ u64 mc[16]; __shared__ i64 mfs[16]; __shared__ double mf[16][2 * NA + 1]; __shared__ i64 mq[16][NA + 1];\n"
          "    const P *part = (const P*)m.part;\n"
+         "    const unsigned *pcode = (const unsigned*)(part + (i64)m.G * NS);\n"
          "    for(int k = threadIdx.x; k < 2048; k += 1024) bits[k] = 0;\n"
          "    __syncthreads();\n"
-         "    for(int b = threadIdx.x; b < m.G; b += 1024){ unsigned u = m.used[b]; for(unsigned s = 0; s < u; s++){ unsigned cd = part[(i64)b * NS + s].code; atomicOr(&bits[cd >> 5], 1u << (cd & 31)); } }\n"
+         "    for(int b = threadIdx.x; b < m.G; b += 1024){ unsigned u = m.used[b]; for(unsigned s = 0; s < u; s++){ unsigned cd = pcode[(i64)b * NS + s]; atomicOr(&bits[cd >> 5], 1u << (cd & 31)); } }\n"
          "    __syncthreads();\n"
          "    unsigned c0 = __popc(bits[2 * threadIdx.x]), c1 = __popc(bits[2 * threadIdx.x + 1]);\n"
          "    pre[threadIdx.x] = c0 + c1;\n"
@@ -675,8 +679,8 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
          "        for(int i = sub * 64 + lane; i < SL; i += 64 * W){\n"
          "            if(r >= g || (unsigned)(i % NS) >= m.used[i / NS]) continue;\n"
          "            {\n"
+         "                if(pcode[i] != code) continue;\n"
          "                const P &p = part[i];\n"
-         "                if(p.code != code) continue;\n"
          "                c += p.cnt; fs = p.first < fs ? p.first : fs;\n";
     for(int k = 0; k < (int)acc.size(); k++){
         if(acc[k].f && acc[k].cls == 0) s << "                { KS u; u.s = p.f[2*" << k << "]; u.c = p.f[2*" << k << "+1]; ks_merge(t" << k << ", u); }\n";
@@ -749,19 +753,25 @@ __global__ void k_fga_emit(const int *gp, const unsigned *codes, const unsigned 
     }
 }
 
-static void set_dev_vec(hpa_ctx *c, V z, int32_t typ, L len, void *p){
+// a one-element result is inline: its value comes from `host` when given, else from the device
+static void set_dev_vec(hpa_ctx *c, V z, int32_t typ, L len, void *p, const void *host = nullptr){
     z->typ = typ; z->len = len;
-    if(len <= 1){ z->g = (char*)&z->b; z->i64 = 0; if(len == 1) d2h(c, &z->b, p, type_width(typ)); }
+    if(len <= 1){
+        z->g = (char*)&z->b; z->i64 = 0;
+        if(len == 1){ if(host) memcpy(&z->b, host, (size_t)type_width(typ)); else d2h(c, &z->b, p, type_width(typ)); }
+    }
     else z->g = (char*)p;
 }
+// host destinations of hpaFusedGroupAggHost
+struct FgaHost { L cap; void **keys; int64_t *first; void **aggs; int64_t *ng; };
 
 // hpaFusedGroupAgg: see the public header.  Returns HPA_E_NOT_IMPL (nothing written) when a
 // block meets more distinct key codes than it has slots; the caller then runs the unfused plan.
 // prepared inputs of one distinct fused group-aggregate request (see hpaFusedGroupAgg)
 struct FgaPrep { CExpr ce; Konst kc; std::vector<Acc> acc; std::vector<int> used; FgaOut o; };
 static std::map<std::string, FgaPrep> g_fga_prep;
-extern "C" int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_expr *cond,
-                                    int32_t na, const hpa_agg *aggs, hpa_V zfirst, hpa_V *zkeys, hpa_V *zaggs){
+static int32_t fga_run(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_expr *cond,
+                       int32_t na, const hpa_agg *aggs, hpa_V zfirst, hpa_V *zkeys, hpa_V *zaggs, const FgaHost *hd){
     hpa_ctx *c = ctx_cur();
     if(nk < 1 || nk > 2 || na < 1 || na > 16 || nx < 0 || nx > EXPR_MAX_COLS) R HPA_E_DOMAIN;
     if(!jit_enabled()) R HPA_E_NOT_IMPL;
@@ -829,7 +839,7 @@ extern "C" int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *
     if(!cols_aligned(nx, typ, used, cols)) R HPA_E_NOT_IMPL;
     const int NA = acc.empty() ? 1 : (int)acc.size();
     o.stride = NA;
-    const size_t psize = 8 + 8 + 8 + 16 * NA + 8 * NA;     // struct P
+    const size_t psize = 8 + 8 + 16 * NA + 8 * NA + 4;     // struct P, and the slot's code
     // two workgroups per CU: measured on MI355X (Q1, 60M rows) against 1, 3, 4, 6, 8 and 12,
     // it streams fastest, and its partials keep the merge short
     int G = (int)ceil_div(n, 256 * FGA_ROWS * 8);
@@ -888,22 +898,45 @@ extern "C" int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *
         }
         hipLaunchKernelGGL(k_fga_emit, dim3(grid_for(GM, 64, 1)), dim3(64), 0, c->stream, gdev, codes, cnt, first, fo, qo, o);
         if((e = check_launch("fga_emit"))) R e;
-        if((e = d2h(c, hv, ovf, 8))) R e;
+        if(hd){
+            // the flag, the group count and the first cap groups of every output in one read
+            const L cap = hd->cap < GM ? hd->cap : GM;
+            const void *src[3 + 2 * 16]; void *dst[3 + 2 * 16]; int64_t nb[3 + 2 * 16]; int m = 0;
+            src[m] = ovf; dst[m] = hv; nb[m++] = 8;
+            src[m] = o.key0; dst[m] = hd->keys[0]; nb[m++] = cap;
+            if(nk == 2){ src[m] = o.key1; dst[m] = hd->keys[1]; nb[m++] = cap; }
+            if(hd->first){ src[m] = o.first; dst[m] = hd->first; nb[m++] = 8 * cap; }
+            for(int a = 0; a < na; a++){ src[m] = o.out[a]; dst[m] = hd->aggs[a]; nb[m++] = 8 * cap; }
+            if((e = read_staged(c, m, src, dst, nb))) R e;
+        } else if((e = d2h(c, hv, ovf, 8))) R e;
         if(!hv[0]) break;
     }
     if(hv[0]){ c->dcur = mark; R HPA_E_NOT_IMPL; }
     const L g = hv[1];
+    const bool staged = hd && g <= (hd->cap < GM ? hd->cap : GM);
     uint8_t *k0 = o.key0, *k1 = o.key1;
     int64_t *zf = o.first;
-    set_dev_vec(c, zkeys[0], keys[0]->typ, g, k0);
-    if(nk == 2) set_dev_vec(c, zkeys[1], keys[1]->typ, g, k1);
-    if(zfirst){ set_dev_vec(c, zfirst, HPA_L, g, zf); note_bound(c, zf, n); }
+    set_dev_vec(c, zkeys[0], keys[0]->typ, g, k0, staged ? hd->keys[0] : nullptr);
+    if(nk == 2) set_dev_vec(c, zkeys[1], keys[1]->typ, g, k1, staged ? hd->keys[1] : nullptr);
+    if(zfirst){ set_dev_vec(c, zfirst, HPA_L, g, zf, staged ? hd->first : nullptr); note_bound(c, zf, n); }
     for(int a = 0; a < na; a++){
         int kd = aggs[a].kind;
         int32_t t = kd == 4 ? HPA_L : kd == 1 ? HPA_E : (o.accf[a] ? HPA_E : HPA_L);
-        set_dev_vec(c, zaggs[a], t, g, o.out[a]);
+        set_dev_vec(c, zaggs[a], t, g, o.out[a], staged ? hd->aggs[a] : nullptr);
     }
+    if(hd) *hd->ng = g;
     R 0;
+}
+extern "C" int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_expr *cond,
+                                    int32_t na, const hpa_agg *aggs, hpa_V zfirst, hpa_V *zkeys, hpa_V *zaggs){
+    R fga_run(nk, keys, nx, x, cond, na, aggs, zfirst, zkeys, zaggs, nullptr);
+}
+extern "C" int32_t hpaFusedGroupAggHost(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_expr *cond,
+                                        int32_t na, const hpa_agg *aggs, hpa_V zfirst, hpa_V *zkeys, hpa_V *zaggs,
+                                        int64_t cap, void **hkeys, int64_t *hfirst, void **haggs, int64_t *ng){
+    if(cap < 0 || !hkeys || !haggs || !ng || (nk == 2 && !hkeys[1])) R HPA_E_DOMAIN;
+    FgaHost hd{cap, hkeys, zfirst ? hfirst : nullptr, haggs, ng};
+    R fga_run(nk, keys, nx, x, cond, na, aggs, zfirst, zkeys, zaggs, &hd);
 }
 
 // Diagnostic for hosts without a GPU: emit the generated sources of one fused group-aggregate,

@@ -419,21 +419,20 @@ __global__ void __launch_bounds__(256) k_read_many(RmList r){
     if(w8) for(; i + 8 <= nb; i += 256 * 8) *(uint64_t*)(d + i) = *(const uint64_t*)(s + i);
     for(L j = (w8 ? (nb & ~(L)7) : 0) + threadIdx.x; j < nb; j += 256) d[j] = s[j];
 }
-extern "C" int32_t hpaReadMany(int32_t n, hpa_V *x, void **dst, const int64_t *bytes){
-    hpa_ctx *c = ctx_cur();
-    // small results (the usual case: aggregates over groups) go through the pinned staging
-    // buffer: one kernel fills it and one synchronisation covers every read
+// Device ranges src[k] (bytes[k] each, device pointers) into host memory dst[k], after the work
+// queued before: through the pinned staging buffer with one kernel and one synchronisation when
+// they fit, else one copy command each and one synchronisation.
+int32_t read_staged(hpa_ctx *c, int n, const void *const *src, void *const *dst, const int64_t *bytes){
     L dev_total = 0;
-    for(int k = 0; k < n; k++)
-        if(bytes[k] > 0 && !is_inline(x[k]) && is_device(x[k]->g)) dev_total += (bytes[k] + 15) & ~(L)15;
-    if(c->pinned && dev_total > 0 && dev_total <= c->pinned_bytes){
+    for(int k = 0; k < n; k++) if(bytes[k] > 0) dev_total += (bytes[k] + 15) & ~(L)15;
+    if(dev_total == 0) R 0;
+    if(c->pinned && dev_total <= c->pinned_bytes){
         L off = 0;
         std::vector<L> at(n, -1);
         RmList rl; int nr = 0;
         for(int k = 0; k < n; k++){
             if(bytes[k] <= 0) continue;
-            if(is_inline(x[k]) || !is_device(x[k]->g)){ memcpy(dst[k], x[k]->g, (size_t)bytes[k]); continue; }
-            rl.src[nr] = x[k]->g; rl.dst[nr] = c->pinned + off; rl.bytes[nr] = bytes[k]; nr++;
+            rl.src[nr] = (const char*)src[k]; rl.dst[nr] = c->pinned + off; rl.bytes[nr] = bytes[k]; nr++;
             at[k] = off; off += (bytes[k] + 15) & ~(L)15;
             if(nr == RM_MAX){ hipLaunchKernelGGL(k_read_many, dim3(nr), dim3(256), 0, c->stream, rl); nr = 0; }
         }
@@ -443,15 +442,22 @@ extern "C" int32_t hpaReadMany(int32_t n, hpa_V *x, void **dst, const int64_t *b
         for(int k = 0; k < n; k++) if(at[k] >= 0) memcpy(dst[k], c->pinned + at[k], (size_t)bytes[k]);
         R 0;
     }
-    bool any = false;
+    for(int k = 0; k < n; k++)
+        if(bytes[k] > 0 && hipMemcpyAsync(dst[k], src[k], (size_t)bytes[k], hipMemcpyDeviceToHost, c->stream) != hipSuccess) R HPA_E_DEVICE;
+    if(hipStreamSynchronize(c->stream) != hipSuccess) R HPA_E_DEVICE;
+    R 0;
+}
+extern "C" int32_t hpaReadMany(int32_t n, hpa_V *x, void **dst, const int64_t *bytes){
+    hpa_ctx *c = ctx_cur();
+    // small results (the usual case: aggregates over groups) go through the pinned staging
+    // buffer: one kernel fills it and one synchronisation covers every read
+    std::vector<const void*> s; std::vector<void*> d; std::vector<int64_t> b;
     for(int k = 0; k < n; k++){
         if(bytes[k] <= 0) continue;
         if(is_inline(x[k]) || !is_device(x[k]->g)){ memcpy(dst[k], x[k]->g, (size_t)bytes[k]); continue; }
-        if(hipMemcpyAsync(dst[k], x[k]->g, (size_t)bytes[k], hipMemcpyDeviceToHost, c->stream) != hipSuccess) R HPA_E_DEVICE;
-        any = true;
+        s.push_back(x[k]->g); d.push_back(dst[k]); b.push_back(bytes[k]);
     }
-    if(any && hipStreamSynchronize(c->stream) != hipSuccess) R HPA_E_DEVICE;
-    R 0;
+    R read_staged(c, (int)s.size(), s.data(), d.data(), b.data());
 }
 
 extern "C" int32_t hpaSync(void){

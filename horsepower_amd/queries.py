@@ -151,15 +151,15 @@ def q1_chf(B, T, ship_max=D.Q1_SHIP_MAX):
     """Q1 in one pass over the table (filter, two-char group, eight aggregates).  The fused
     kernel emits its groups in ascending (flag, status) order, each char compared as the order
     by compares it, so q1.hir's order by (t75) is the identity on its output and is not run.
-    Falls back to q1_fused when the key cardinality exceeds the fused kernel's per-workgroup
-    slots."""
+    The results come back with the kernel's own completion check (one synchronisation).  Falls
+    back to q1_fused when the key cardinality exceeds the fused kernel's per-workgroup slots."""
     cond, spec = q1_chf_spec(ship_max)
     xs = [T["ship_day"], T["quantity"], T["price"], T["discount"], T["tax"]]
-    r = B.fused_group_agg([T["return_flag"], T["line_status"]], xs, cond, spec)
+    r = B.fused_group_agg_host([T["return_flag"], T["line_status"]], xs, cond, spec)
     if r is None:
         return q1_fused(B, T, ship_max)
-    _, (kf, ks), aggs = r
-    return dict(zip(Q1_NAMES, B.read_many([kf, ks] + aggs)))
+    keys, aggs = r
+    return dict(zip(Q1_NAMES, keys + aggs))
 
 
 # ------------------------------------------------------------------------------------- Q3

@@ -254,10 +254,20 @@ int32_t hpaFusedCompress(int32_t ny, hpa_V *z, hpa_V *y, int32_t nx, hpa_V *x,
  * code defines (0..127), and the order pfnOrderBy gives the key columns; zkeys[k] holds key column k per group,
  * zfirst the first row of each group, zaggs[a] aggregate a (kind 0 sum, 1 avg, 2 min, 3 max,
  * 4 len; body unused for len).  Returns HPA_E_NOT_IMPL without writing outputs when one
- * workgroup meets more than 8 distinct key codes (the caller then runs the unfused primitives). */
+ * workgroup meets more than 16 distinct key codes (the caller then runs the unfused primitives). */
 typedef struct hpa_agg { int32_t kind; int32_t _pad; const hpa_expr *body; } hpa_agg;
 int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_expr *cond,
                          int32_t na, const hpa_agg *aggs, hpa_V zfirst, hpa_V *zkeys, hpa_V *zaggs);
+/* hpaFusedGroupAgg that also delivers its results to host memory, for a plan whose next step is
+ * reading them: hkeys[k] (cap bytes each), hfirst (cap i64, used when zfirst is given) and
+ * haggs[a] (cap 8-byte values: i64 for len and integer sums, f64 otherwise) receive the first
+ * min(cap, groups) groups, and *ng the group count, with one stream synchronisation in all
+ * (flag, count and outputs are read together).  The z* nodes are set as by hpaFusedGroupAgg;
+ * when *ng > cap the host copies are incomplete and the nodes hold the results.  No counterpart
+ * in the reference, whose compiled plans end in host memory already (cur/q1.h). */
+int32_t hpaFusedGroupAggHost(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_expr *cond,
+                             int32_t na, const hpa_agg *aggs, hpa_V zfirst, hpa_V *zkeys, hpa_V *zaggs,
+                             int64_t cap, void **hkeys, int64_t *hfirst, void **haggs, int64_t *ng);
 /* Diagnostic (no device needed): generate the fused group-aggregate kernel for these column
  * types and expressions and compile it with hipRTC; 0 when the generated code compiles. */
 int32_t hpaJitCompileCheck(int32_t nx, const int32_t *coltyp, const hpa_expr *cond,

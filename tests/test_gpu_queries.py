@@ -85,6 +85,28 @@ def test_fused_group_agg_overflow_and_single_key(gpu, orc):
     assert np.allclose(gpu.read(zs), sums, rtol=RTOL, atol=0)
 
 
+@pytest.mark.parametrize("keys,cap", [(b"xyz", 64), (b"xyz", 2), (b"q", 64), (b"abcdefgh", 8)])
+def test_fused_group_agg_host(gpu, keys, cap):
+    """The host-delivering form returns what the node form returns, read back: with every group
+    staged, with more groups than `cap` (read from the nodes), with a single group."""
+    from horsepower_amd.vnode import H_C, expr
+    r = np.random.default_rng(len(keys) + cap)
+    n = 300_000
+    k0 = r.choice(np.frombuffer(keys, np.uint8), n)
+    k1 = r.choice(np.frombuffer(b"A" if len(keys) == 1 else b"AB", np.uint8), n)
+    v = np.round(r.uniform(0, 100, n), 2)
+    w = r.integers(0, 50, n).astype(np.int32)
+    spec = [("sum", expr([("COL", 0)])), ("avg", expr([("COL", 1)])), ("sum", expr([("COL", 1)])),
+            ("max", expr([("COL", 0)])), ("len", None)]
+    cond = expr([("COL", 1), ("I", 40), "LT"])
+    ks, xs = [gpu.vec(k0, H_C), gpu.vec(k1, H_C)], [gpu.vec(v), gpu.vec(w)]
+    _, zk, za = gpu.fused_group_agg(ks, xs, cond, spec)
+    want = gpu.read_many(zk + za)
+    hk, ha = gpu.fused_group_agg_host(ks, xs, cond, spec, cap=cap)
+    for a, b in zip(hk + ha, want):
+        assert a.dtype == b.dtype and np.array_equal(a, b)
+
+
 def test_fused_group_agg_key_order(gpu):
     """The one-pass group emits its groups in ascending key order with char bytes compared as
     signed values, the order pfnOrderBy gives the key columns (bytes >= 128 sort first)."""
