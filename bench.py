@@ -329,34 +329,28 @@ def bench_batch(a):
     b.close()
 
 
-def bench_prims(a):
-    import ctypes as C
+def prims_cases(B, n, gpu):
+    """Inputs and calls of the per-primitive measurements on backend B: n-row Q1 columns, n/4-row
+    sort/group columns (a 1e6-key i32 column), a join/index_of/member pair of n/40 unique sparse
+    build keys and n/10 probe keys.  Each case: (name, algorithmic bytes, call).  The fused
+    reduction has no oracle counterpart and is listed on the GPU only."""
     import numpy as np
     from horsepower_amd import datagen as D, queries as Q
-    from horsepower_amd.backend import GpuBackend
-    from horsepower_amd.vnode import H_E, H_I, H_L
-    n = a.rows or 60_000_000
+    from horsepower_amd.vnode import H_E, H_I, H_L, expr
     ns = max(n // 4, 1)                       # sorts, groups and hash builds: config 2's 15M rows
-    t0 = time.time()
     t = D.gen_q1(n)
-    B = GpuBackend()
     T = Q.load_table(B, t)
     key = B.vec(D.uniform(7, 1, ns, 0, 999_999))                   # 1e6-key i32 group column
     dim_keys = D.gen_q3(ns // 10)[0]["key"]
     build = B.vec(dim_keys)                                        # unique sparse keys
     probe = B.vec(dim_keys[D.uniform(7, 2, 4 * len(dim_keys), 0, len(dim_keys) - 1, np.int64)])
     idx = B.vec(D.uniform(7, 3, n // 2, 0, n - 1, np.int64))
-    print(f"[bench] prims inputs ready in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-    lib = B.lib
     m1 = B.pfnLeq(T["ship_day"], B.scalar(2434, H_I))
     m6 = B.pfnAnd(B.pfnGeq(T["ship_day"], B.scalar(730, H_I)), B.pfnLt(T["ship_day"], B.scalar(1095, H_I)))
     k1 = int(B.read_scalar(B.pfnSum(m1)))
     k6 = int(B.read_scalar(B.pfnSum(m6)))
     B.sync()
-    mark = B.mark()
     nb, npb = len(dim_keys), 4 * len(dim_keys)
-    cond6 = Q.q6_cond()
-    from horsepower_amd.vnode import expr
     cases = [
         ("compare_i32", 5 * n, lambda: B.pfnLeq(T["ship_day"], B.scalar(2434, H_I))),
         ("compare_f64", 9 * n, lambda: B.pfnGeq(T["discount"], B.scalar(0.05, H_E))),
@@ -374,9 +368,24 @@ def bench_prims(a):
         ("join_i32", 4 * (nb + npb) + 16 * npb, lambda: B.pfnJoinIndex(build, probe, B.sym("eq"))),
         ("index_of_i32", 4 * (nb + npb) + 8 * npb, lambda: B.pfnIndexOf(build, probe)),
         ("member_i32", 4 * (nb + npb) + npb, lambda: B.pfnMember(probe, build)),
-        ("fused_q6_reduce", 24 * n, lambda: B.fused_reduce(0, [T["ship_day"], T["quantity"], T["discount"], T["price"]],
-                                                          cond6, expr([("COL", 3), ("COL", 2), "MUL"]))),
     ]
+    if gpu:
+        cond6 = Q.q6_cond()
+        cases.append(("fused_q6_reduce", 24 * n, lambda: B.fused_reduce(0, [T["ship_day"], T["quantity"], T["discount"], T["price"]],
+                                                                        cond6, expr([("COL", 3), ("COL", 2), "MUL"]))))
+    return cases, {"ns": ns, "nb": nb, "npb": npb}
+
+
+def bench_prims(a):
+    import math
+    from horsepower_amd.backend import GpuBackend
+    n = a.rows or 60_000_000
+    t0 = time.time()
+    B = GpuBackend()
+    cases, sz = prims_cases(B, n, True)
+    print(f"[bench] prims inputs ready in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    lib = B.lib
+    mark = B.mark()
     reps = max(a.steps // 4, 3)
     res = {}
     for name, nbytes, fn in cases:
@@ -399,17 +408,54 @@ def bench_prims(a):
         res[name] = {"bytes": nbytes, "us": round(ms * 1e3, 2), "GBs": round(gbs, 1),
                      "frac": round(gbs / PEAK_HBM_GBS, 4), "launch_scopes_us": inner}
         print(f"[bench] {name:20s} {ms * 1e3:9.1f} us {gbs:8.1f} GB/s {gbs / PEAK_HBM_GBS:6.1%}  {inner}", file=sys.stderr, flush=True)
-    import math
+    cpu = None
+    if not a.no_cpu:
+        cpu = prims_cpu(a, res)
     gm = math.exp(sum(math.log(v["GBs"]) for v in res.values()) / len(res))
     out = {"metric": "rows/s + % HBM roofline per query; batched queries/s & p99 latency vs plain",
            "value": round(gm, 1), "unit": "GB/s (geometric mean over primitives)", "n_gpus": 1,
            "steps": reps, "warmup": 1, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "mixed (i32/f64/u8 as each primitive)",
            "data": "synthetic (seeded counter-based generator)",
-           "config": {"workload": f"each primitive alone: {n}-row columns, {ns}-row sorts/groups, "
-                                  f"join/index_of/member {nb} x {npb}", "rows": n},
-           "primitives": res}
+           "config": {"workload": f"each primitive alone: {n}-row columns, {sz['ns']}-row sorts/groups, "
+                                  f"join/index_of/member {sz['nb']} x {sz['npb']}", "rows": n},
+           "cpu_baseline": cpu, "primitives": res}
     print(json.dumps(out), flush=True)
+
+
+def prims_cpu(a, res):
+    """The same primitives on the oracle (C + OpenMP restatement of the reference's, all host
+    cores; the reference's serial phases stay serial) over a smaller sample: one discarded call,
+    then the mean of up to three.  Adds cpu_us / cpu_GBs / gpu_over_cpu to each primitive."""
+    import math
+    import oracle
+    threads = os.cpu_count() or 1
+    ob = oracle.backend(threads)
+    n = a.cpu_rows or 6_000_000
+    cases, sz = prims_cases(ob, n, False)
+    mark = ob.mark()
+    rates = []
+    for name, nbytes, fn in cases:
+        fn()
+        ob.release(mark)
+        times = []
+        t_start = time.perf_counter()
+        while len(times) < 3 and time.perf_counter() - t_start < 5.0:
+            t1 = time.perf_counter()
+            fn()
+            times.append(time.perf_counter() - t1)
+            ob.release(mark)
+        s = sum(times) / len(times)
+        gbs = nbytes / s / 1e9
+        rates.append(gbs)
+        if name in res:
+            res[name].update({"cpu_us": round(s * 1e6, 1), "cpu_GBs": round(gbs, 2),
+                              "gpu_over_cpu": round(res[name]["GBs"] / gbs, 1)})
+        print(f"[bench] cpu {name:20s} {s * 1e6:11.1f} us {gbs:8.2f} GB/s", file=sys.stderr, flush=True)
+    gm = math.exp(sum(math.log(r) for r in rates) / len(rates))
+    return {"value": round(gm, 2), "unit": "GB/s (geometric mean over primitives)", "cores": ob.threads(),
+            "kind": "port", "sample": f"{n}-row columns, {sz['ns']}-row sorts/groups, join/index_of/member "
+                                      f"{sz['nb']} x {sz['npb']}; one discarded call, mean of up to three"}
 
 
 def bench_serve(a):
