@@ -657,10 +657,13 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_down(const TI *in, int64_t 
         for(int k = 0; k < SCAN_ITEMS; k++) if(i0 + k < n) out[i0 + k] = pre + v[k];
     }
 }
-// inputs of at most 1024 * SCAN_ITEMS elements: one workgroup, one launch
+// inputs of at most 1024 * SCAN_ITEMS elements: one workgroup, one launch.  The results pass
+// through LDS (rows padded by one word against bank conflicts) so that consecutive threads write
+// consecutive elements.
 template<typename TI>
 __global__ void __launch_bounds__(1024) k_scan_small(const TI *in, int64_t *out, L n, int vec, int64_t *total){
     __shared__ int64_t wsum[16];
+    __shared__ int64_t so[1024 * (SCAN_ITEMS + 1)];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const L i0 = (L)threadIdx.x * SCAN_ITEMS;
     int64_t v[SCAN_ITEMS];
@@ -675,8 +678,10 @@ __global__ void __launch_bounds__(1024) k_scan_small(const TI *in, int64_t *out,
     int64_t pre = inc - s;
     for(int q = 0; q < w; q++) pre += wsum[q];
     #pragma unroll
-    for(int k = 0; k < SCAN_ITEMS; k++) if(i0 + k < n) out[i0 + k] = pre + v[k];
+    for(int k = 0; k < SCAN_ITEMS; k++) so[threadIdx.x * (SCAN_ITEMS + 1) + k] = pre + v[k];
     if(threadIdx.x == 1023 && total) *total = pre + s;
+    __syncthreads();
+    for(L j = threadIdx.x; j < n; j += 1024) out[j] = so[(j / SCAN_ITEMS) * (SCAN_ITEMS + 1) + (j % SCAN_ITEMS)];
 }
 template<typename TI>
 static int32_t scan_impl(hpa_ctx *c, const TI *in, int64_t *out, L n, int64_t *total_dev){
