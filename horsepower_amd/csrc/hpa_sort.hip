@@ -1191,34 +1191,42 @@ extern "C" int32_t pfnGroupTrie(hpa_V z, hpa_V x){
 #define GR_RUNS_MIN (1 << 24)
 #define GR_RUNS_MAX_CHANGE 0.4
 #define GR_RUNS_MAX_SHARE 0.55
-__device__ __forceinline__ bool gr_head(const int32_t *x, L i, unsigned cap_mask){
-    return i == 0 || ((uint64_t)i & cap_mask) == 0 || x[i] != x[i - 1];
+// Run heads of row i (threads of a wave hold consecutive rows): the previous row's key comes from
+// the lane below; lane 0's row is a multiple of 64, so a forced head, and needs none.
+__device__ __forceinline__ bool gr_head(int32_t xi, L i, L n, unsigned cap_mask){
+    const int32_t xp = __shfl_up(xi, 1, 64);
+    return i < n && (((uint64_t)i & cap_mask) == 0 || xi != xp);
 }
 __global__ void __launch_bounds__(256) k_gr_head_count(const int32_t *x, L n, unsigned cap_mask, uint32_t *tcount){
     const L base = (L)blockIdx.x * RUN_TILE;
+    int32_t xv[RUN_ITEMS];
+    #pragma unroll
+    for(int k = 0; k < RUN_ITEMS; k++){ const L i = base + (L)k * 256 + threadIdx.x; xv[k] = i < n ? x[i] : 0; }
     uint32_t c = 0;
     #pragma unroll
-    for(int k = 0; k < RUN_ITEMS; k++){
-        const L i = base + (L)k * 256 + threadIdx.x;
-        c += (i < n && gr_head(x, i, cap_mask)) ? 1u : 0u;
-    }
+    for(int k = 0; k < RUN_ITEMS; k++) c += gr_head(xv[k], base + (L)k * 256 + threadIdx.x, n, cap_mask) ? 1u : 0u;
     __shared__ uint32_t sh[4];
     for(int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
     if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
     __syncthreads();
     if(threadIdx.x == 0) tcount[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
-// run r (input order): rkey[r] = its key, rval[r] = (first row << cb) | (length - 1)
+// run r (input order): rkey[r] = its key, rval[r] = (first row << cb) | (length - 1).  A head is
+// forced at every multiple of the cap (a divisor of 64), so the next head after a row is in the
+// rest of its wave's 64-row ballot word or at the start of the next word (the next tile's first
+// row, for the tile's last word).
 __global__ void __launch_bounds__(256) k_gr_head_write(const int32_t *x, L n, unsigned cap_mask, int cb, const int64_t *toff,
                                                        uint32_t *rkey, uint32_t *rval){
     __shared__ uint32_t s_cnt[RUN_ITEMS * 4];
     const L base = (L)blockIdx.x * RUN_TILE;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint64_t bal[RUN_ITEMS];
+    int32_t xv[RUN_ITEMS];
+    #pragma unroll
+    for(int k = 0; k < RUN_ITEMS; k++){ const L i = base + (L)k * 256 + threadIdx.x; xv[k] = i < n ? x[i] : 0; }
     #pragma unroll
     for(int k = 0; k < RUN_ITEMS; k++){
-        const L i = base + (L)k * 256 + threadIdx.x;
-        bal[k] = __ballot(i < n && gr_head(x, i, cap_mask));
+        bal[k] = __ballot(gr_head(xv[k], base + (L)k * 256 + threadIdx.x, n, cap_mask));
         if(lane == 0) s_cnt[k * 4 + w] = (uint32_t)__popcll(bal[k]);
     }
     __syncthreads();
@@ -1236,9 +1244,11 @@ __global__ void __launch_bounds__(256) k_gr_head_write(const int32_t *x, L n, un
         if((bal[k] >> lane) & 1ull){
             const L i = base + (L)k * 256 + threadIdx.x;
             const int64_t r = pre + s_cnt[k * 4 + w] + __popcll(bal[k] & lt);
-            L d = 1;                                   // run length: rows up to the next head (at most cap)
-            while(i + d < n && !gr_head(x, i + d, cap_mask)) d++;
-            rkey[r] = (uint32_t)x[i]; rval[r] = ((uint32_t)i << cb) | (uint32_t)(d - 1);
+            const int word = k * 4 + w;                // rows base + 64 * word ..
+            const uint64_t above = lane == 63 ? 0ull : (bal[k] & (~0ull << (lane + 1)));
+            L next = above ? base + 64 * (L)word + __builtin_ctzll(above) : base + 64 * (L)(word + 1);
+            if(next > n) next = n;
+            rkey[r] = (uint32_t)xv[k]; rval[r] = ((uint32_t)i << cb) | (uint32_t)(next - i - 1);
         }
     }
 }
