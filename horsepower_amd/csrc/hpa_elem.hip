@@ -399,6 +399,11 @@ extern "C" int32_t pfnIndex(hpa_V z, hpa_V x, hpa_V y){
         if((e = promote_dev(c, y, HPA_L, &py))) R e;
         const void *px = nullptr;
         if(nx > 0 && (e = dev_view(c, x, &px))) R e;
+        // the keys of a group of this very i32 column: the group kept each key's value
+        if(x->typ == HPA_I && y->typ == HPA_L && nz > 1){
+            const void *gv = group_vals(c, py, px, nx);
+            if(gv){ c->dcur = mark; z->typ = HPA_I; z->len = nz; z->g = (char*)gv; R 0; }
+        }
         // index vectors made here (order, group, join, where) are known to be in range
         const bool known = y->typ == HPA_L && within_bound(c, py, nx);
         int *bad = nullptr;

@@ -33,9 +33,16 @@ struct hpa_ctx {
     // index vectors produced here, with an exclusive upper bound of their (non-negative) values;
     // pfnIndex skips its device bounds check for them (entries die with their arena range)
     std::map<const void*, L> bounds;
+    // group keys vectors made here over an arena i32 column, with the key value of each group:
+    // index(column, keys) is those values (entries die with their arena range; the values are
+    // allocated before the keys, so a rewind that frees either frees the keys)
+    struct GroupVals { const void *src; L n; const void *vals; };
+    std::map<const void*, GroupVals> gvals;
 };
 void note_bound(hpa_ctx *c, const void *p, L bound);
 bool within_bound(hpa_ctx *c, const void *p, L limit);
+void note_group_vals(hpa_ctx *c, const void *keys, const void *src, L n, const void *vals);
+const void *group_vals(hpa_ctx *c, const void *keys, const void *src, L n);
 int32_t resident_alloc(V x, int32_t typ, L len);   // long-lived HBM payload (hpaFreeResident)
 
 hpa_ctx *ctx_cur();
@@ -131,7 +138,7 @@ int32_t iota_u32(hpa_ctx *c, uint32_t *p, L n);
 // keys = first original index of each run, values = flat list (counts, i64 indices).
 int32_t runs_to_group_u64(hpa_ctx *c, V z, const uint64_t *skeys, const uint32_t *sidx, L n,
                           const uint64_t *eqkeys);
-int32_t runs_to_group_u32(hpa_ctx *c, V z, const uint32_t *skeys, const uint32_t *sidx, L n);
+int32_t runs_to_group_u32(hpa_ctx *c, V z, const uint32_t *skeys, const uint32_t *sidx, L n, const void *src = nullptr);
 // Build the dict(keys, flat list) node structure from device arrays.
 int32_t make_group_dict(hpa_ctx *c, V z, int64_t *keys, int64_t *counts, int64_t *flat, L g, L n);
 

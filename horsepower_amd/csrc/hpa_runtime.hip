@@ -165,7 +165,7 @@ extern "C" int32_t hpaCtxSetArena(hpa_ctx *c, void *base, int64_t bytes){
     if(base && !(is_device(base) && is_device((char*)base + bytes - 1))){ set_error_detail("hpaCtxSetArena: not a device pool range"); R HPA_E_DOMAIN; }
     if(hipStreamSynchronize(c->stream) != hipSuccess) R HPA_E_DEVICE;
     c->dbase = (char*)base; c->dtop = base ? bytes : 0; c->dcur = 0; c->dpeak = 0;
-    c->hcur = 0; c->marks.clear(); c->bounds.clear();
+    c->hcur = 0; c->marks.clear(); c->bounds.clear(); c->gvals.clear();
     for(void *p : c->host_bufs) free(p);
     c->host_bufs.clear();
     R 0;
@@ -222,6 +222,7 @@ extern "C" int64_t hpaSetHeapOffset(int64_t cur){
     for(void *p : c->host_bufs) free(p);
     c->host_bufs.clear();
     c->bounds.erase(c->bounds.lower_bound((const void*)(c->dbase + cur)), c->bounds.end());
+    c->gvals.erase(c->gvals.lower_bound((const void*)(c->dbase + cur)), c->gvals.end());
     R c->dcur = cur;
 }
 
@@ -231,6 +232,14 @@ void note_bound(hpa_ctx *c, const void *p, L bound){
 bool within_bound(hpa_ctx *c, const void *p, L limit){
     auto it = c->bounds.find(p);
     R it != c->bounds.end() && it->second <= limit;
+}
+static bool in_arena(hpa_ctx *c, const void *p){ R p && (const char*)p >= c->dbase && (const char*)p < c->dbase + c->dtop; }
+void note_group_vals(hpa_ctx *c, const void *keys, const void *src, L n, const void *vals){
+    if(in_arena(c, keys) && in_arena(c, src) && in_arena(c, vals) && vals < keys && src < vals) c->gvals[keys] = {src, n, vals};
+}
+const void *group_vals(hpa_ctx *c, const void *keys, const void *src, L n){
+    auto it = c->gvals.find(keys);
+    R it != c->gvals.end() && it->second.src == src && it->second.n == n ? it->second.vals : nullptr;
 }
 
 // ---------------------------------------------------------------- value model

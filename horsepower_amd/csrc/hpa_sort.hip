@@ -373,7 +373,8 @@ __global__ void __launch_bounds__(256) k_run_tilecount(const K *sk, const uint32
 }
 template<typename K>
 __global__ void __launch_bounds__(256) k_run_write(const K *sk, const uint32_t *sidx, const uint64_t *eq, L n,
-                                                   const int64_t *toff, int64_t *keys, int64_t *start, int64_t *flat){
+                                                   const int64_t *toff, int64_t *keys, int64_t *start, int64_t *flat,
+                                                   uint32_t *kval){
     __shared__ uint32_t s_cnt[RUN_ITEMS * 4];
     const L base = (L)blockIdx.x * RUN_TILE;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -402,6 +403,7 @@ __global__ void __launch_bounds__(256) k_run_write(const K *sk, const uint32_t *
             L i = base + (L)k * 256 + threadIdx.x;
             int64_t r = pre + s_cnt[k * 4 + w] + __popcll(bal[k] & lt);
             keys[r] = (int64_t)sidx[i]; start[r] = i;
+            if(kval) kval[r] = (uint32_t)sk[i];
         }
     }
 }
@@ -427,8 +429,10 @@ int32_t make_group_dict(hpa_ctx *c, V z, int64_t *keys, int64_t *counts, int64_t
     R 0;
 }
 
+// src: the i32 column whose bit patterns sk holds (null otherwise); its groups' key values are
+// then kept beside the keys for index(column, keys)
 template<typename K>
-static int32_t runs_to_group_t(hpa_ctx *c, V z, const K *sk, const uint32_t *sidx, L n, const uint64_t *eq){
+static int32_t runs_to_group_t(hpa_ctx *c, V z, const K *sk, const uint32_t *sidx, L n, const uint64_t *eq, const void *src = nullptr){
     int32_t e;
     const L nt = ceil_div(n, RUN_TILE) > 0 ? ceil_div(n, RUN_TILE) : 1;
     uint32_t *tcount = (uint32_t*)dalloc(c, 4 * nt);
@@ -440,23 +444,25 @@ static int32_t runs_to_group_t(hpa_ctx *c, V z, const K *sk, const uint32_t *sid
         if((e = scan_u32_to_i64(c, tcount, toff, nt, toff + nt))) R e;
         if((e = d2h(c, &g, toff + nt, 8))) R e;
     }
+    uint32_t *kval = src && g > 1 ? (uint32_t*)dalloc(c, 4 * g) : nullptr;       // before keys: dies with them
     int64_t *keys = (int64_t*)dalloc(c, 8 * (g > 0 ? g : 1));
     int64_t *start = (int64_t*)dalloc(c, 8 * (g > 0 ? g : 1));
     int64_t *cnt = (int64_t*)dalloc(c, 8 * (g > 0 ? g : 1));
     int64_t *flat = (int64_t*)dalloc(c, 8 * (n > 0 ? n : 1));
     if(!keys || !start || !cnt || !flat) R HPA_E_MEMORY;
     if(n > 0){
-        hipLaunchKernelGGL(k_run_write<K>, dim3((unsigned)nt), dim3(256), 0, c->stream, sk, sidx, eq, n, toff, keys, start, flat);
+        hipLaunchKernelGGL(k_run_write<K>, dim3((unsigned)nt), dim3(256), 0, c->stream, sk, sidx, eq, n, toff, keys, start, flat, kval);
         hipLaunchKernelGGL(k_run_counts, dim3(grid_for(g, 256, 1)), dim3(256), 0, c->stream, start, g, n, cnt);
     }
     if((e = check_launch("group_runs"))) R e;
+    if(kval) note_group_vals(c, keys, src, n, kval);
     R make_group_dict(c, z, keys, cnt, flat, g, n);
 }
 int32_t runs_to_group_u64(hpa_ctx *c, V z, const uint64_t *sk, const uint32_t *sidx, L n, const uint64_t *eq){
     R runs_to_group_t<uint64_t>(c, z, sk, sidx, n, eq);
 }
-int32_t runs_to_group_u32(hpa_ctx *c, V z, const uint32_t *sk, const uint32_t *sidx, L n){
-    R runs_to_group_t<uint32_t>(c, z, sk, sidx, n, nullptr);
+int32_t runs_to_group_u32(hpa_ctx *c, V z, const uint32_t *sk, const uint32_t *sidx, L n, const void *src){
+    R runs_to_group_t<uint32_t>(c, z, sk, sidx, n, nullptr, src);
 }
 
 // ---------------------------------------------------------------- key builders
@@ -1285,7 +1291,7 @@ __global__ void __launch_bounds__(256) k_gr_expand(const uint32_t *sv, int cb, c
 // heads of the sorted run keys: group key = the first row of the group's first run, start =
 // its flat position
 __global__ void __launch_bounds__(256) k_gr_group_write(const uint32_t *sk, const uint32_t *sv, int cb, const int64_t *fpos, L nr,
-                                                        const int64_t *toff, int64_t *keys, int64_t *start){
+                                                        const int64_t *toff, int64_t *keys, int64_t *start, uint32_t *kval){
     __shared__ uint32_t s_cnt[RUN_ITEMS * 4];
     const L base = (L)blockIdx.x * RUN_TILE;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1311,7 +1317,7 @@ __global__ void __launch_bounds__(256) k_gr_group_write(const uint32_t *sk, cons
         if((bal[k] >> lane) & 1ull){
             const L i = base + (L)k * 256 + threadIdx.x;
             const int64_t g = pre + s_cnt[k * 4 + w] + __popcll(bal[k] & lt);
-            keys[g] = (int64_t)(sv[i] >> cb); start[g] = fpos[i];
+            keys[g] = (int64_t)(sv[i] >> cb); start[g] = fpos[i]; kval[g] = sk[i];
         }
     }
 }
@@ -1355,13 +1361,15 @@ static int32_t group_i32_runs(hpa_ctx *c, V z, const int32_t *x, L n, bool *done
                        (const uint64_t*)nullptr, (L)nr, gcount);
     if((e = scan_u32_to_i64(c, gcount, goff, gt, goff + gt))) R e;
     int64_t G = 0; if((e = d2h(c, &G, goff + gt, 8))) R e;
+    uint32_t *kval = (uint32_t*)dalloc(c, 4 * G);           // before keys: dies with them
     int64_t *keys = (int64_t*)dalloc(c, 8 * G), *start = (int64_t*)dalloc(c, 8 * G), *cnt = (int64_t*)dalloc(c, 8 * G);
-    if(!keys || !start || !cnt) R HPA_E_MEMORY;
+    if(!kval || !keys || !start || !cnt) R HPA_E_MEMORY;
     hipLaunchKernelGGL(k_gr_group_write, dim3((unsigned)gt), dim3(256), 0, c->stream, (const uint32_t*)sk, (const uint32_t*)sv, cb,
-                       (const int64_t*)fpos, (L)nr, (const int64_t*)goff, keys, start);
+                       (const int64_t*)fpos, (L)nr, (const int64_t*)goff, keys, start, kval);
     hipLaunchKernelGGL(k_run_counts, dim3(grid_for(G, 256, 1)), dim3(256), 0, c->stream, (const int64_t*)start, (L)G, n, cnt);
     if((e = check_launch("group_runs"))) R e;
     *done = true;
+    if(G > 1) note_group_vals(c, keys, x, n, kval);
     R make_group_dict(c, z, keys, cnt, flat, G, n);
 }
 
@@ -1379,7 +1387,7 @@ static int32_t group_int(hpa_ctx *c, V z, V x){      // lib_group_by_normal_int,
     // the unsigned radix key of an i32 is its bit pattern: the column is sorted in place of keys
     uint32_t *sk, *sv;
     if((e = radix_sort_pairs_u32(c, (const uint32_t*)p, nullptr, n, 0, 32, &sk, &sv))) R e;
-    R runs_to_group_u32(c, z, sk, sv, n);
+    R runs_to_group_u32(c, z, sk, sv, n, p);
 }
 
 static int32_t any_high_i64(hpa_ctx *c, const int64_t *p, L n, bool *hi){

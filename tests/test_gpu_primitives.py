@@ -237,6 +237,26 @@ def test_group_along_order(gpu, orc, n):
     assert groups_of(gpu, gpu.pfnGroup(gpu.vec(ch, H_C))) == groups_of(orc, orc.pfnGroup(orc.vec(ch, H_C)))
 
 
+@pytest.mark.parametrize("n,clustered", [(300_000, False), (20_000_000, True)])
+def test_index_by_group_keys(gpu, n, clustered):
+    """index(x, keys(group(x))) on an i32 column made in the arena returns the groups' key values
+    (kept by the group, row path and run-compressed path); the same keys over another column of
+    the same length are gathered as usual."""
+    r = rng(n)
+    if clustered:
+        base = r.permutation(n // 4).astype(np.int32) - 5
+        x = np.repeat(base, r.integers(1, 8, base.size))[:n]
+    else:
+        x = r.integers(-1000, 100000, n).astype(np.int32)
+    other = r.integers(0, 7, x.size).astype(np.int32)
+    xa = gpu.pfnPlus(gpu.vec(x), gpu.scalar(0, H_I))            # arena columns
+    oa = gpu.pfnPlus(gpu.vec(other), gpu.scalar(0, H_I))
+    keys = gpu.pfnKeys(gpu.pfnGroup(xa))
+    k = gpu.read(keys)
+    assert np.array_equal(gpu.read(gpu.pfnIndex(xa, keys)), x[k])
+    assert np.array_equal(gpu.read(gpu.pfnIndex(oa, keys)), other[k])
+
+
 def test_group_long(gpu, orc):
     for x in (np.array([-1, 4294967295, -1, 7, 7], np.int64),
               rng(9).integers(-5, 5, 5000).astype(np.int64) * (1 << 33),
