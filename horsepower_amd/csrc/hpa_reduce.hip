@@ -116,13 +116,20 @@ __global__ void __launch_bounds__(RD_BLOCK) k_minmax(const T *x, L n, T *part){
         part[blockIdx.x] = r;
     }
 }
+// min / max of the partials: one workgroup, each thread over a strided slice, then a tree (the
+// result does not depend on the order)
 template<typename T, int MAXK>
-__global__ void k_minmax_final(const T *part, int np, T *out){
-    if(threadIdx.x == 0 && blockIdx.x == 0){
-        T r = part[0];
-        for(int q = 1; q < np; q++){ T v = part[q]; r = MAXK ? (r > v ? r : v) : (r < v ? r : v); }
-        *out = r;
+__global__ void __launch_bounds__(RD_BLOCK) k_minmax_final(const T *part, int np, T *out){
+    __shared__ T sh[RD_BLOCK];
+    T r = part[0];
+    for(int q = threadIdx.x; q < np; q += RD_BLOCK){ const T v = part[q]; r = MAXK ? (r > v ? r : v) : (r < v ? r : v); }
+    sh[threadIdx.x] = r;
+    __syncthreads();
+    for(int s = RD_BLOCK / 2; s > 0; s >>= 1){
+        if((int)threadIdx.x < s){ const T v = sh[threadIdx.x + s], u = sh[threadIdx.x]; sh[threadIdx.x] = MAXK ? (u > v ? u : v) : (u < v ? u : v); }
+        __syncthreads();
     }
+    if(threadIdx.x == 0) *out = sh[0];
 }
 
 static int sum_grid(L n){
@@ -212,8 +219,8 @@ static int32_t minmax_typed(hpa_ctx *c, const void *p, L n, int op, T *res){
         if(op == 0) hipLaunchKernelGGL((k_minmax<T,0>), dim3(G), dim3(RD_BLOCK), 0, c->stream, (const T*)p, n, part);
         else hipLaunchKernelGGL((k_minmax<T,1>), dim3(G), dim3(RD_BLOCK), 0, c->stream, (const T*)p, n, part);
     }
-    if(op == 0) hipLaunchKernelGGL((k_minmax_final<T,0>), dim3(1), dim3(64), 0, c->stream, part, G, out);
-    else hipLaunchKernelGGL((k_minmax_final<T,1>), dim3(1), dim3(64), 0, c->stream, part, G, out);
+    if(op == 0) hipLaunchKernelGGL((k_minmax_final<T,0>), dim3(1), dim3(RD_BLOCK), 0, c->stream, part, G, out);
+    else hipLaunchKernelGGL((k_minmax_final<T,1>), dim3(1), dim3(RD_BLOCK), 0, c->stream, part, G, out);
     int32_t e = check_launch("minmax"); if(e) R e;
     R d2h(c, res, out, sizeof(T));
 }
@@ -787,21 +794,22 @@ __device__ __forceinline__ void fa_merge(FaPart &r, const FaPart &b){
     r.mn = b.mn < r.mn ? b.mn : r.mn; r.mx = b.mx > r.mx ? b.mx : r.mx;
     r.imn = b.imn < r.imn ? b.imn : r.imn; r.imx = b.imx > r.imx ? b.imx : r.imx;
 }
-// np partials merged by 64 lanes over fixed contiguous slices, then in lane order (one wave
-// per result: block q merges part[q * np .. q * np + np)).
-__global__ void __launch_bounds__(64) k_fused_reduce_final(const FaPart *part, int np, FaPart *out){
-    __shared__ FaPart sh[64];
+// np partials merged in a fixed order (block q merges part[q * np .. q * np + np)): thread t
+// folds partials t, t + 1024, ... (coalesced loads, all in flight together), then a pairwise
+// tree in LDS.  Deterministic: the same association on every run.
+#define FA_FINAL 1024
+__global__ void __launch_bounds__(FA_FINAL) k_fused_reduce_final(const FaPart *part, int np, FaPart *out){
+    __shared__ FaPart sh[FA_FINAL];
     const FaPart *p = part + (L)blockIdx.x * np;
-    const int per = (np + 63) / 64;
     FaPart a; a.s = {0, 0}; a.i = 0; a.cnt = 0; a.mn = DBL_MAX; a.mx = -DBL_MAX; a.imn = LLONG_MAX; a.imx = LLONG_MIN;
-    for(int q = threadIdx.x * per; q < np && q < (int)(threadIdx.x + 1) * per; q++) fa_merge(a, p[q]);
+    for(int q = threadIdx.x; q < np; q += FA_FINAL) fa_merge(a, p[q]);
     sh[threadIdx.x] = a;
     __syncthreads();
-    if(threadIdx.x == 0){
-        FaPart r = sh[0];
-        for(int t = 1; t < 64; t++) fa_merge(r, sh[t]);
-        out[blockIdx.x] = r;
+    for(int s = FA_FINAL / 2; s > 0; s >>= 1){
+        if((int)threadIdx.x < s) fa_merge(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
     }
+    if(threadIdx.x == 0) out[blockIdx.x] = sh[0];
 }
 
 // Shared-scan FA reductions: z[q] = sum (kind 0) or avg (kind 1) of body over the rows where
@@ -826,7 +834,7 @@ extern "C" int32_t hpaFusedReduceMany(int32_t nq, hpa_V *z, int32_t kind, int32_
         for(int q = 0; q < nq; q++) if((e = hpaFusedReduce(z[q], kind, nx, x, &conds[q], body))) R e;
         R 0;
     }
-    hipLaunchKernelGGL(k_fused_reduce_final, dim3(nq), dim3(64), 0, c->stream, part, G, out);
+    hipLaunchKernelGGL(k_fused_reduce_final, dim3(nq), dim3(FA_FINAL), 0, c->stream, part, G, out);
     if((e = check_launch("fused_reduce_many"))) R e;
     FaPart h[8]; if((e = d2h(c, h, out, sizeof(FaPart) * nq))) R e;
     c->dcur = mark;
@@ -863,7 +871,7 @@ extern "C" int32_t hpaFusedReduce(hpa_V z, int32_t kind, int32_t nx, hpa_V *x, c
         ProfScope ps("fused_reduce_interp", c->stream, (double)n * cb);
         hipLaunchKernelGGL(k_fused_reduce, dim3(G), dim3(256), 0, c->stream, ce, cond != nullptr, be, cols, n, part);
     }
-    hipLaunchKernelGGL(k_fused_reduce_final, dim3(1), dim3(64), 0, c->stream, part, G, out);
+    hipLaunchKernelGGL(k_fused_reduce_final, dim3(1), dim3(FA_FINAL), 0, c->stream, part, G, out);
     if((e = check_launch("fused_reduce"))) R e;
     FaPart h; if((e = d2h(c, &h, out, sizeof(FaPart)))) R e;
     c->dcur = mark;
