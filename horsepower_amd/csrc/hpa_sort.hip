@@ -1445,7 +1445,7 @@ __global__ void k_perm_flags_f(int nc, const void *const *cols, const int *typ, 
     }
 }
 
-static int32_t order_perm(hpa_ctx *c, V x, const uint8_t *flags, uint32_t **perm_out, bool *ties);
+static int32_t order_perm(hpa_ctx *c, V x, const uint8_t *flags, uint32_t **perm_out, bool *ties, bool *known_ties);
 // run heads flagged per row (list keys): scatter each head's original index and position
 __global__ void k_run_emit(const uint32_t *flag, const int64_t *pos, const uint32_t *sidx, L n,
                            int64_t *keys, int64_t *start, int64_t *flat){
@@ -1648,10 +1648,11 @@ __global__ void __launch_bounds__(256) k_br_place(const uint64_t *k0, L n, const
         slot_row[p] = (uint32_t)i; slot_k0[p] = v;
     }
 }
-__global__ void __launch_bounds__(256) k_br_rank(SmallCols s, L n, const unsigned long long *or_and, int nb, const int *over,
+// over[1] is set when two rows are equal on every column (a full tie)
+__global__ void __launch_bounds__(256) k_br_rank(SmallCols s, L n, const unsigned long long *or_and, int nb, int *over,
                                                  const int64_t *start, const uint32_t *cnt, const uint32_t *slot_row,
                                                  const uint64_t *slot_k0, uint32_t *perm){
-    if(*over) return;
+    if(over[0]) return;
     const int shift = br_shift(or_and, nb);
     const uint32_t bmask = (1u << nb) - 1;
     for(L p = (L)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (L)gridDim.x * blockDim.x){
@@ -1660,23 +1661,27 @@ __global__ void __launch_bounds__(256) k_br_rank(SmallCols s, L n, const unsigne
         const uint32_t b = (uint32_t)(a >> shift) & bmask;
         const int64_t s0 = start[b], e = s0 + cnt[b];
         uint32_t rank = 0;
+        bool tie = false;
         for(int64_t j = s0; j < e; j++){
             const uint64_t kj = slot_k0[j];
             if(kj != a){ rank += kj < a; continue; }
             const uint32_t rj = slot_row[j];
             if(rj == r) continue;
-            int less = rj < r;
-            for(int q = 1; q < s.nc; q++){
+            int less = rj < r, q = 1;
+            for(; q < s.nc; q++){
                 const uint64_t x = br_key(s, q, rj), y = br_key(s, q, r);
                 if(x != y){ less = x < y; break; }
             }
+            tie |= q == s.nc;
             rank += less;
         }
         perm[s0 + rank] = r;
+        if(tie) over[1] = 1;
     }
 }
-// *fallback: a bucket is too large and nothing was written
-static int32_t order_bucket_rank(hpa_ctx *c, const SmallCols &s, L n, uint32_t **perm_out, bool *fallback){
+// *fallback: a bucket is too large and nothing was written; *ties: two rows are equal on every
+// column
+static int32_t order_bucket_rank(hpa_ctx *c, const SmallCols &s, L n, uint32_t **perm_out, bool *fallback, bool *ties){
     int32_t e;
     *fallback = false;
     int lg = 0; while((1ll << lg) < n) lg++;
@@ -1691,25 +1696,27 @@ static int32_t order_bucket_rank(hpa_ctx *c, const SmallCols &s, L n, uint32_t *
     if(!or_and || !over || !k0 || !slot_k0 || !cnt || !start || !slot_row || !perm) R HPA_E_MEMORY;
     const int g = grid_for(n, 256, 4);
     hipLaunchKernelGGL(k_br_init, dim3(grid_for(2 * B, 256, 4)), dim3(256), 0, c->stream, cnt, 2 * B, or_and);
-    hipLaunchKernelGGL(k_br_init, dim3(1), dim3(64), 0, c->stream, (uint32_t*)over, (L)1, (unsigned long long*)nullptr);
+    hipLaunchKernelGGL(k_br_init, dim3(1), dim3(64), 0, c->stream, (uint32_t*)over, (L)2, (unsigned long long*)nullptr);
     hipLaunchKernelGGL(k_br_key, dim3(g), dim3(256), 0, c->stream, s, n, k0, or_and);
     hipLaunchKernelGGL(k_br_hist, dim3(g), dim3(256), 0, c->stream, (const uint64_t*)k0, n, (const unsigned long long*)or_and, nb, cnt, over);
     if((e = check_launch("order_bucket"))) R e;
     if((e = scan_u32_to_i64(c, cnt, start, B, start + B))) R e;
     hipLaunchKernelGGL(k_br_place, dim3(g), dim3(256), 0, c->stream, (const uint64_t*)k0, n, (const unsigned long long*)or_and, nb,
                        (const int*)over, (const int64_t*)start, cursor, slot_row, slot_k0);
-    hipLaunchKernelGGL(k_br_rank, dim3(g), dim3(256), 0, c->stream, s, n, (const unsigned long long*)or_and, nb, (const int*)over,
+    hipLaunchKernelGGL(k_br_rank, dim3(g), dim3(256), 0, c->stream, s, n, (const unsigned long long*)or_and, nb, over,
                        (const int64_t*)start, (const uint32_t*)cnt, (const uint32_t*)slot_row, (const uint64_t*)slot_k0, perm);
     if((e = check_launch("order_bucket"))) R e;
-    int ov = 0; if((e = d2h(c, &ov, over, 4))) R e;
-    if(ov){ *fallback = true; R 0; }
+    int ov[2] = {0, 0}; if((e = d2h(c, ov, over, 8))) R e;
+    if(ov[0]){ *fallback = true; R 0; }
+    *ties = ov[1] != 0;
     *perm_out = perm;
     R 0;
 }
 
-static int32_t order_perm(hpa_ctx *c, V x, const uint8_t *flags, uint32_t **perm_out, bool *ties){
+// *ties / *known_ties: whether two rows are equal on every column, when the path found out
+static int32_t order_perm(hpa_ctx *c, V x, const uint8_t *flags, uint32_t **perm_out, bool *ties, bool *known_ties){
     L nc = x->len; L n = ((V)x->g)->len; int32_t e;
-    *ties = false;
+    *ties = false; *known_ties = false;
     uint32_t *perm = nullptr;
     if(n > 0 && n <= SMALL_ORDER && nc <= SMALL_COLS){
         SmallCols s; memset(&s, 0, sizeof(s)); s.nc = (int)nc;
@@ -1739,8 +1746,9 @@ static int32_t order_perm(hpa_ctx *c, V x, const uint8_t *flags, uint32_t **perm
             const L mark = c->dcur;
             bool fallback;
             ProfScope ps("order_bucket", c->stream);
-            if((e = order_bucket_rank(c, s, n, perm_out, &fallback))) R e;
-            if(!fallback) R 0;
+            bool bt = true;
+            if((e = order_bucket_rank(c, s, n, perm_out, &fallback, &bt))) R e;
+            if(!fallback){ *ties = bt; *known_ties = true; R 0; }
             c->dcur = mark;
         }
     }
@@ -1847,7 +1855,8 @@ static int32_t order_vector(hpa_ctx *c, V z, V col, bool up){   // lib_order_by_
     // comparison-sorted types (quicksort in the reference): stable sort on the order key
     V0 lst; memset(&lst, 0, sizeof(lst)); lst.typ = HPA_G; lst.len = 1; lst.g = (char*)col;
     uint8_t f = up ? 1 : 0; uint32_t *perm; bool ties;
-    if((e = order_perm(c, &lst, &f, &perm, &ties))) R e;
+    bool known;
+    if((e = order_perm(c, &lst, &f, &perm, &ties, &known))) R e;
     R order_emit(c, z, perm, n, false);
 }
 
@@ -1920,11 +1929,12 @@ __global__ void k_ot_apply(const uint32_t *perm, const uint32_t *b, const int64_
         out[dir[i] ? lo + hi - 1 - i : i] = perm[i];
     }
 }
-static int32_t order_list_ties(hpa_ctx *c, V x, const uint8_t *flags, uint32_t **perm_io, L n){
+// known: whether the order found full ties (1), found none (0), or did not look (-1)
+static int32_t order_list_ties(hpa_ctx *c, V x, const uint8_t *flags, uint32_t **perm_io, L n, int known){
     const L nc = x->len;
     bool any_desc = false;
     for(L q = 0; q < nc; q++) any_desc |= flags[q] == 0;
-    if(!any_desc || n < 2 || nc > SMALL_COLS) R 0;
+    if(!any_desc || n < 2 || nc > SMALL_COLS || known == 0) R 0;
     int32_t e;
     SmallCols s; memset(&s, 0, sizeof(s)); s.nc = (int)nc;
     for(L q = 0; q < nc; q++){
@@ -1942,8 +1952,10 @@ static int32_t order_list_ties(hpa_ctx *c, V x, const uint8_t *flags, uint32_t *
     hipLaunchKernelGGL(k_br_init, dim3(1), dim3(64), 0, c->stream, (uint32_t*)nt, (L)2, (unsigned long long*)nullptr);
     hipLaunchKernelGGL(k_ot_lvl, dim3(g), dim3(256), 0, c->stream, s, perm, n, lvl, nt);
     if((e = check_launch("order_ties"))) R e;
-    unsigned long long ties = 0; if((e = d2h(c, &ties, nt, 8))) R e;
-    if(ties == 0) R 0;                                        // the lexicographic order is unique
+    if(known < 0){
+        unsigned long long ties = 0; if((e = d2h(c, &ties, nt, 8))) R e;
+        if(ties == 0) R 0;                                    // the lexicographic order is unique
+    }
     ProfScope ps("order_ties", c->stream);
     hipLaunchKernelGGL(k_br_init, dim3(g), dim3(256), 0, c->stream, dir, n, (unsigned long long*)nullptr);
     uint32_t *b = (uint32_t*)dalloc(c, 4 * n), *rowrun = (uint32_t*)dalloc(c, 4 * n), *rflag = (uint32_t*)dalloc(c, 4 * n);
@@ -2002,10 +2014,10 @@ extern "C" int32_t pfnOrderBy(hpa_V z, hpa_V x, hpa_V y){
         }
         if(nc == 1) R order_vector(c, z, a, fl[0] != 0);
         L n = nc == 0 ? 0 : a[0].len;
-        uint32_t *perm = nullptr; bool ties;
+        uint32_t *perm = nullptr; bool ties, known;
         if(nc == 0 || n == 0){ void *dz; R out_init(c, z, HPA_L, 0, &dz); }
-        if((e = order_perm(c, x, fl.data(), &perm, &ties))) R e;
-        if((e = order_list_ties(c, x, fl.data(), &perm, n))) R e;
+        if((e = order_perm(c, x, fl.data(), &perm, &ties, &known))) R e;
+        if((e = order_list_ties(c, x, fl.data(), &perm, n, known ? (ties ? 1 : 0) : -1))) R e;
         R order_emit(c, z, perm, n, false);
     }
     // lib_order_basic (h_primitive.cc:1760-1767)

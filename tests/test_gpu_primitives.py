@@ -390,6 +390,8 @@ def test_order_lists_sorted_runs(gpu, orc, n):
         [r.integers(0, 4, n).astype(np.int32), (rows // 3).astype(np.int64), np.full(n, 5, np.int64)],
         [np.sort(r.integers(0, 6, n)).astype(np.int32), (-(rows // 4)).astype(np.int64), (rows % 2).astype(np.int32)],
         [r.integers(0, 3, n).astype(np.int64), r.integers(0, 2, n).astype(np.int32), (rows // 7).astype(np.int32)],
+        # many first-column values (small buckets of the bucket-rank path), full ties on the rest
+        [r.integers(0, max(n // 3, 1), n).astype(np.int64), r.integers(0, 2, n).astype(np.int32), np.zeros(n, np.int32)],
     ]
     for cols in cases:
         for fl in ([0, 0, 0], [0, 1, 0], [1, 0, 0], [0, 0, 1], [1, 1, 0]):
