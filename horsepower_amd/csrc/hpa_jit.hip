@@ -847,7 +847,10 @@ static int32_t fga_run(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_
     if(G < 1) G = 1;
     const void *pk0 = nullptr, *pk1 = nullptr;
     if(n > 0){ if((e = dev_view(c, keys[0], &pk0))) R e; if(nk == 2 && (e = dev_view(c, keys[1], &pk1))) R e; }
-    if(((uintptr_t)pk0 | (uintptr_t)pk1) % 4){ c->dcur = mark; R HPA_E_NOT_IMPL; }     // 4-byte key vector loads
+    // FGA_ROWS-row vector loads of every column
+    if(((uintptr_t)pk0 | (uintptr_t)pk1) % FGA_ROWS){ c->dcur = mark; R HPA_E_NOT_IMPL; }
+    for(int k = 0; k < nx; k++)
+        if(used[k] && (uintptr_t)cols.p[k] % (uintptr_t)(FGA_ROWS * type_width(typ[k]))){ c->dcur = mark; R HPA_E_NOT_IMPL; }
     void *part = dalloc(c, (L)psize * FGA_NS * G);
     unsigned *usedb = (unsigned*)dalloc(c, 4 * G);
     int *ovf = (int*)dalloc(c, 16);
