@@ -1628,12 +1628,30 @@ __device__ __forceinline__ int br_shift(const unsigned long long *or_and, int nb
     const int top = vary ? 64 - __clzll(vary) : 0;
     return top > nb ? top - nb : 0;
 }
+// bucket counts; a wave adds the lanes that share its leading lanes' buckets with one atomic each
+// (up to four rounds), so a few crowded buckets do not serialise the kernel
 __global__ void __launch_bounds__(256) k_br_hist(const uint64_t *k0, L n, const unsigned long long *or_and, int nb, uint32_t *cnt, int *over){
     const int shift = br_shift(or_and, nb);
     const uint32_t bmask = (1u << nb) - 1;
-    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
-        const uint32_t b = (uint32_t)(k0[i] >> shift) & bmask;
-        if(atomicAdd(&cnt[b], 1u) == BR_BUCKET_MAX) atomicOr(over, 1);
+    const int lane = threadIdx.x & 63;
+    for(L i0 = (L)blockIdx.x * blockDim.x; i0 < n; i0 += (L)gridDim.x * blockDim.x){
+        const L i = i0 + threadIdx.x;
+        bool pending = i < n;
+        const uint32_t b = pending ? (uint32_t)(k0[i] >> shift) & bmask : 0u;
+        for(int it = 0; it < 4; it++){
+            const uint64_t pm = __ballot(pending);
+            if(!pm) break;
+            const int leader = __ffsll((unsigned long long)pm) - 1;
+            const uint32_t bl = __shfl(b, leader, 64);
+            const bool mine = pending && b == bl;
+            const uint64_t mm = __ballot(mine);
+            if(lane == leader){
+                const uint32_t k = (uint32_t)__popcll(mm);
+                if(atomicAdd(&cnt[bl], k) + k > BR_BUCKET_MAX) atomicOr(over, 1);
+            }
+            if(mine) pending = false;
+        }
+        if(pending && atomicAdd(&cnt[b], 1u) + 1 > BR_BUCKET_MAX) atomicOr(over, 1);
     }
 }
 __global__ void __launch_bounds__(256) k_br_place(const uint64_t *k0, L n, const unsigned long long *or_and, int nb, const int *over,
@@ -1897,11 +1915,24 @@ __global__ void k_ot_rowrun(const uint32_t *perm, const uint32_t *b, const int64
 }
 // rows sorted by (run, row): consecutive rows of one run in ascending row order; bit 1 of
 // rflag[run]: column c increases somewhere along them, bit 2: decreases somewhere
+// (one run can hold every row: a wave whose rows share a run ORs its bits first, and a bit
+// already set is not set again, so the flag word does not serialise the kernel)
 __global__ void k_ot_check(SmallCols s, int c, const uint32_t *srun, const uint32_t *srow, L n, uint32_t *rflag){
-    for(L j = 1 + (L)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (L)gridDim.x * blockDim.x){
-        if(srun[j] != srun[j - 1]) continue;
-        const uint64_t a = okey_any(s.typ[c], s.p[c], srow[j - 1]), b = okey_any(s.typ[c], s.p[c], srow[j]);
-        if(a != b) atomicOr(&rflag[srun[j]], a < b ? 1u : 2u);
+    const L stride = (L)gridDim.x * blockDim.x;
+    for(L j0 = (L)blockIdx.x * blockDim.x; j0 < n; j0 += stride){
+        const L j = j0 + threadIdx.x + 1;                     // rows j-1, j: the pair at hand
+        uint32_t bit = 0, run = 0xFFFFFFFFu;
+        if(j < n && srun[j] == srun[j - 1]){
+            run = srun[j];
+            const uint64_t a = okey_any(s.typ[c], s.p[c], srow[j - 1]), b = okey_any(s.typ[c], s.p[c], srow[j]);
+            if(a != b) bit = a < b ? 1u : 2u;
+        }
+        const uint32_t r0 = __builtin_amdgcn_readfirstlane(run);
+        if(__all(run == r0)){
+            uint32_t w = bit;
+            for(int o = 32; o > 0; o >>= 1) w |= __shfl_xor(w, o, 64);
+            if((threadIdx.x & 63) == 0 && w && r0 != 0xFFFFFFFFu && (rflag[r0] & w) != w) atomicOr(&rflag[r0], w);
+        } else if(bit && (rflag[run] & bit) == 0) atomicOr(&rflag[run], bit);
     }
 }
 // a run whose rows run in direction d (0 ascending rows, 1 descending) is reversed when column
