@@ -213,7 +213,7 @@ template<typename T>
 __global__ void k_any_zero(const T *p, L n, int *flag){
     int f = 0;
     for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x) f |= (p[i] == (T)0);
-    if(__any(f) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+    if(__any(f) && (threadIdx.x & 63) == 0 && !*(volatile int*)flag) atomicOr(flag, 1);
 }
 static int32_t check_zero(hpa_ctx *c, V y, bool *found){
     *found = false;
@@ -368,7 +368,7 @@ __global__ void __launch_bounds__(256) k_gather(const T *src, const int64_t *idx
         if(k < 0 || k >= nsrc){ f = 1; continue; }
         dst[i] = src[k];
     }
-    if(bad && __any(f) && (threadIdx.x & 63) == 0) atomicOr(bad, 1);
+    if(bad && __any(f) && (threadIdx.x & 63) == 0 && !*(volatile int*)bad) atomicOr(bad, 1);
 }
 static int32_t gather_checked(hpa_ctx *c, void *dst, const void *src, const int64_t *idx, L n, L nsrc, L w, int *bad){
     if(n <= 0) R 0;
@@ -464,7 +464,7 @@ __global__ void __launch_bounds__(256) k_gather_many(GatherCols g, const int64_t
             }
         }
     }
-    if(bad && __any(f) && (threadIdx.x & 63) == 0) atomicOr(bad, 1);
+    if(bad && __any(f) && (threadIdx.x & 63) == 0 && !*(volatile int*)bad) atomicOr(bad, 1);
 }
 
 // The multi-column peephole of index (the gathers after a group or an order, as in

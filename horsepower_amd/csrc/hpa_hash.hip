@@ -92,7 +92,7 @@ __global__ void k_pk_insert(const int32_t *x, L n, unsigned long long *slots, ui
             h = (h + 1) & mask;
         }
     }
-    if(__any(d) && (threadIdx.x & 63) == 0) atomicOr(dup, 1);
+    if(__any(d) && (threadIdx.x & 63) == 0 && !*(volatile int*)dup) atomicOr(dup, 1);
 }
 __global__ void k_pk_probe(const int32_t *y, L ny, const unsigned long long *slots, uint64_t mask, uint32_t *cnt, uint32_t *rid){
     for(L j = (L)blockIdx.x * blockDim.x + threadIdx.x; j < ny; j += (L)gridDim.x * blockDim.x){
@@ -306,7 +306,7 @@ __global__ void k_dense_first(const int32_t *x, L n, int lo, int32_t *tab, int *
         const int32_t t = tab[k];
         if(t != (int32_t)i){ d = 1; if(t > (int32_t)i) atomicMin(&tab[k], (int32_t)i); }
     }
-    if(__any(d) && (threadIdx.x & 63) == 0) atomicOr(dup, 1);
+    if(__any(d) && (threadIdx.x & 63) == 0 && !*(volatile int*)dup) atomicOr(dup, 1);
 }
 struct DenseTab { int lo; L m; uint32_t *bits; int32_t *tab; int *dup; };
 // first-row table (rows) or bitmap (member) over x[0..n), keys in [lo, lo + m)

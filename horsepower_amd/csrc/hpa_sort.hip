@@ -476,7 +476,7 @@ __global__ void k_key_i64(const int64_t *x, uint64_t *k, L n, int low32){
 __global__ void k_any_high(const int64_t *x, L n, int *flag){
     int f = 0;
     for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x) f |= (x[i] >> 32) > 0;
-    if(__any(f) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+    if(__any(f) && (threadIdx.x & 63) == 0 && !*(volatile int*)flag) atomicOr(flag, 1);
 }
 // checkOrderCase1 (h_libs.cc:1912-1940): ((0|x0) << 32) | x1 with both sign-extended
 __global__ void k_key_case1(const int32_t *x0, const int32_t *x1, int64_t *k, L n){
