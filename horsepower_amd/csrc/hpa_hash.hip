@@ -78,17 +78,39 @@ __global__ void k_fill_i32(int *p, int n, int v){ for(int i = threadIdx.x; i < n
 __global__ void k_pk_clear(unsigned long long *slots, L cap){
     for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += (L)gridDim.x * blockDim.x) slots[i] = PK_EMPTY;
 }
+// Lanes of a wave hold consecutive rows.  Returns whether this lane is the lowest of the lanes
+// holding its key (up to four keys a wave are matched this way; lanes left over return true),
+// and sets *d for a lane whose key a lower lane holds: one table update per key and wave
+// instead of one per row, so a key repeated in many rows does not serialise on its slot.
+__device__ __forceinline__ bool wave_key_leader(bool valid, uint32_t k, int *d){
+    const int lane = threadIdx.x & 63;
+    bool pending = valid, lead = false;
+    for(int it = 0; it < 4; it++){
+        const uint64_t pm = __ballot(pending);
+        if(!pm) break;
+        const int leader = __ffsll((unsigned long long)pm) - 1;
+        const uint32_t kl = __shfl(k, leader, 64);
+        if(pending && k == kl){
+            if(lane == leader) lead = true; else *d = 1;
+            pending = false;
+        }
+    }
+    return lead || pending;
+}
 __global__ void k_pk_insert(const int32_t *x, L n, unsigned long long *slots, uint64_t mask, int *dup){
     int d = 0;
-    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
-        const uint32_t k = (uint32_t)x[i];
+    for(L i0 = (L)blockIdx.x * blockDim.x; i0 < n; i0 += (L)gridDim.x * blockDim.x){
+        const L i = i0 + threadIdx.x;
+        const bool valid = i < n;
+        const uint32_t k = valid ? (uint32_t)x[i] : 0u;
+        if(!wave_key_leader(valid, k, &d)) continue;
         const unsigned long long want = ((unsigned long long)k << 32) | (unsigned long long)(uint32_t)i;
         uint64_t h = mix64(k) & mask;
         while(true){
             // the CAS returns the slot's content: no separate read before it
             const unsigned long long cur = atomicCAS(&slots[h], PK_EMPTY, want);
             if(cur == PK_EMPTY) break;
-            if((uint32_t)(cur >> 32) == k){ d = 1; atomicMin(&slots[h], want); break; }
+            if((uint32_t)(cur >> 32) == k){ d = 1; if(cur > want) atomicMin(&slots[h], want); break; }
             h = (h + 1) & mask;
         }
     }
@@ -301,10 +323,14 @@ __global__ void k_dense_store(const int32_t *x, L n, int lo, int32_t *tab){
 }
 __global__ void k_dense_first(const int32_t *x, L n, int lo, int32_t *tab, int *dup){
     int d = 0;
-    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
-        const uint32_t k = (uint32_t)(x[i] - lo);
+    for(L i0 = (L)blockIdx.x * blockDim.x; i0 < n; i0 += (L)gridDim.x * blockDim.x){
+        const L i = i0 + threadIdx.x;
+        const bool valid = i < n;
+        const uint32_t k = valid ? (uint32_t)(x[i] - lo) : 0u;
+        const bool lead = wave_key_leader(valid, k, &d);
+        if(!valid) continue;
         const int32_t t = tab[k];
-        if(t != (int32_t)i){ d = 1; if(t > (int32_t)i) atomicMin(&tab[k], (int32_t)i); }
+        if(t != (int32_t)i){ d = 1; if(lead && t > (int32_t)i) atomicMin(&tab[k], (int32_t)i); }
     }
     if(__any(d) && (threadIdx.x & 63) == 0 && !*(volatile int*)dup) atomicOr(dup, 1);
 }

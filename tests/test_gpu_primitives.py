@@ -400,6 +400,25 @@ def test_order_lists_sorted_runs(gpu, orc, n):
             assert np.array_equal(g, o), (fl, [c.dtype for c in cols])
 
 
+@pytest.mark.parametrize("n,sparse", [(300_000, False), (300_000, True), (2_000_000, False)])
+def test_repeated_build_keys(gpu, orc, n, sparse):
+    """Build sides whose keys repeat in many rows -- a constant column, a two-valued one -- for
+    index_of (first occurrence), member and join, on the dense and the hashed tables."""
+    r = rng(n + (7 if sparse else 0))
+    scale = 100_000_007 if sparse else 1
+    for x in (np.full(n, 5 * scale, np.int32), (r.integers(0, 2, n) * scale).astype(np.int32)):
+        y = (r.integers(0, 4, n // 2) * scale).astype(np.int32)
+        assert np.array_equal(gpu.read(gpu.pfnIndexOf(gpu.vec(x), gpu.vec(y))),
+                              orc.read(orc.pfnIndexOf(orc.vec(x), orc.vec(y))))
+        assert np.array_equal(gpu.read(gpu.pfnMember(gpu.vec(y), gpu.vec(x))),
+                              orc.read(orc.pfnMember(orc.vec(y), orc.vec(x))))
+    xb = np.concatenate([np.full(50, 9 * scale, np.int32), r.integers(0, 100, 2000).astype(np.int32) * scale])
+    yb = (r.integers(0, 100, 3000) * scale).astype(np.int32)
+    g = gpu.read_list(gpu.pfnJoinIndex(gpu.vec(xb), gpu.vec(yb), gpu.sym("eq")))
+    o = orc.read_list(orc.pfnJoinIndex(orc.vec(xb), orc.vec(yb), orc.sym("eq")))
+    assert np.array_equal(g[0], o[0]) and np.array_equal(g[1], o[1])
+
+
 @pytest.mark.parametrize("nx,ny,card", [(40, 300, 30), (300, 40, 30), (100, 100, 10), (5, 50, 8),
                                         (50, 5, 8), (100000, 400000, 150000), (0, 10, 3), (20, 0, 3)])
 def test_join(gpu, orc, nx, ny, card):
