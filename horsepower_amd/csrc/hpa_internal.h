@@ -82,7 +82,7 @@ int32_t check_launch(const char *what);
 struct ProfScope {
     const char *name; hipStream_t s; hipEvent_t a = nullptr, b = nullptr; bool on; double bytes = 0;
     bool stopped = false;
-    ProfScope(const char *n, hipStream_t st, double algo_bytes = 0);
+    ProfScope(const char *n, hipStream_t st, double algo_bytes = 0, bool user = false);
     void stop();                      // record the end event now (before later stream work)
     ~ProfScope();
 };

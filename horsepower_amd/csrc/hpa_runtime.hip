@@ -489,7 +489,8 @@ extern "C" const char *hpaSymbolName(int64_t id){
 }
 
 // ---------------------------------------------------------------- kernel timing
-static bool g_prof = false;
+static bool g_prof = false;        // the library's launch scopes
+static bool g_prof_user = false;   // caller scopes (hpaProfileBegin/End)
 struct PendingEv { std::string name; hipEvent_t a, b; double bytes; };
 static std::vector<PendingEv> g_pending;
 struct Stat { int64_t count = 0; double ms = 0, bytes = 0; };
@@ -506,7 +507,7 @@ static hipEvent_t ev_get(){
     hipEvent_t e = nullptr; hipEventCreate(&e); R e;
 }
 static void ev_put(hipEvent_t e){ std::lock_guard<std::mutex> lk(g_mu); g_evpool.push_back(e); }
-ProfScope::ProfScope(const char *n, hipStream_t st, double algo_bytes) : name(n), s(st), on(g_prof), bytes(algo_bytes){
+ProfScope::ProfScope(const char *n, hipStream_t st, double algo_bytes, bool user) : name(n), s(st), on(user ? g_prof_user : g_prof), bytes(algo_bytes){
     if(!on) R;
     a = ev_get(); b = ev_get();
     hipEventRecord(a, s);
@@ -533,16 +534,17 @@ static void prof_resolve(){
         auto &s = g_stats[e.name]; s.count++; s.ms += ms; s.bytes += e.bytes;
     }
 }
-extern "C" void hpaProfileEnable(int32_t on){ g_prof = on != 0; }
+// 0 off, 1 every scope, 2 caller scopes only (no events inside the library's calls)
+extern "C" void hpaProfileEnable(int32_t on){ g_prof = on == 1; g_prof_user = on == 1 || on == 2; }
 
 // Caller-named scopes: HIP events on the current context's stream around any sequence of calls
 // (a whole primitive, a query), reported beside the library's own launch scopes.
 struct UserScope { std::string name; ProfScope *ps; };
 static thread_local std::deque<UserScope> t_scopes;     // deque: names keep their address
 extern "C" void hpaProfileBegin(const char *name, double algo_bytes){
-    if(!g_prof) R;
+    if(!g_prof_user) R;
     t_scopes.push_back({name, nullptr});
-    t_scopes.back().ps = new ProfScope(t_scopes.back().name.c_str(), ctx_cur()->stream, algo_bytes);
+    t_scopes.back().ps = new ProfScope(t_scopes.back().name.c_str(), ctx_cur()->stream, algo_bytes, true);
 }
 extern "C" void hpaProfileEnd(void){
     if(t_scopes.empty()) R;

@@ -155,8 +155,10 @@ int64_t  hpaCtxArenaBytes(hpa_ctx *c);
 int64_t  hpaCtxArenaPeak(hpa_ctx *c);
 void     hpaCtxResetPeak(hpa_ctx *c);
 
-/* Kernel timing: while enabled, launches named by the library are bracketed with HIP events
- * on the launching stream; stats are (launch count, total ms). */
+/* Kernel timing: while enabled (on = 1), launches named by the library are bracketed with HIP
+ * events on the launching stream; stats are (launch count, total ms).  on = 2 keeps only the
+ * caller's scopes (hpaProfileBegin/End), so no events are recorded inside the library's calls;
+ * 0 turns timing off. */
 void     hpaProfileEnable(int32_t on);
 void     hpaProfileReset(void);
 int32_t  hpaProfileStats(const char *kernel, int64_t *count, double *total_ms);
