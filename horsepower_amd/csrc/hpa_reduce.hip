@@ -69,14 +69,20 @@ __global__ void __launch_bounds__(RD_BLOCK) k_sum_f(const T *x, L n, KS *part){
     KS r = ks_block(a0, sh);
     if(threadIdx.x == 0) part[blockIdx.x] = r;
 }
-// Partials merged by 256 threads over fixed contiguous slices, then in thread order.
-__global__ void __launch_bounds__(RD_BLOCK) k_sum_f_final(const KS *part, int np, double *out){
-    __shared__ KS sh[RD_BLOCK / 64];
-    const int per = (np + RD_BLOCK - 1) / RD_BLOCK;
+// Partials merged in a fixed order: thread t folds partials t, t + 1024, ... (coalesced loads,
+// all in flight together), then a pairwise tree in LDS.
+#define SUM_FINAL 1024
+__global__ void __launch_bounds__(SUM_FINAL) k_sum_f_final(const KS *part, int np, double *out){
+    __shared__ KS sh[SUM_FINAL];
     KS a = {0, 0};
-    for(int q = threadIdx.x * per; q < np && q < (int)(threadIdx.x + 1) * per; q++) ks_merge(a, part[q]);
-    KS r = ks_block(a, sh);
-    if(threadIdx.x == 0) *out = r.s + r.c;
+    for(int q = threadIdx.x; q < np; q += SUM_FINAL) ks_merge(a, part[q]);
+    sh[threadIdx.x] = a;
+    __syncthreads();
+    for(int s = SUM_FINAL / 2; s > 0; s >>= 1){
+        if((int)threadIdx.x < s) ks_merge(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if(threadIdx.x == 0) *out = sh[0].s + sh[0].c;
 }
 // Exact int64 sum (wrapping, as the reference's i64 accumulator): 4 elements per thread and
 // iteration, one global atomic per workgroup.
@@ -149,7 +155,7 @@ static int32_t sum_fp(hpa_ctx *c, V x, const void *p, double *res){
         if(x->typ == HPA_E) hipLaunchKernelGGL(k_sum_f<double>, dim3(G), dim3(RD_BLOCK), 0, c->stream, (const double*)p, n, part);
         else hipLaunchKernelGGL(k_sum_f<float>, dim3(G), dim3(RD_BLOCK), 0, c->stream, (const float*)p, n, part);
     }
-    hipLaunchKernelGGL(k_sum_f_final, dim3(1), dim3(RD_BLOCK), 0, c->stream, part, G, out);
+    hipLaunchKernelGGL(k_sum_f_final, dim3(1), dim3(SUM_FINAL), 0, c->stream, part, G, out);
     int32_t e = check_launch("sum"); if(e) R e;
     R d2h(c, res, out, 8);
 }
