@@ -26,21 +26,26 @@
 // Digit histogram of one 4096-key tile, counted in per-wave LDS histograms; a wave whose valid
 // keys all share the digit adds them with one increment.  hist[tile * 256 + d]: each tile writes
 // one contiguous 1 KB row.  With `bits`
-// set (the first pass) it also writes the tile's OR and AND of all keys to bits[2*tile],
-// bits[2*tile+1]: which key bits vary.
+// set (the first pass) it also writes the tile's OR and AND of all keys, and whether some key is
+// below the key before it (on the sort's bits, `kmask`), to bits[4*tile .. 4*tile+2]: which key
+// bits vary, and whether the input is already in key order.
 template<typename K>
 __global__ void __launch_bounds__(RS_BLOCK) k_rs_hist(const K *keys, L n, int shift, uint32_t *hist, L ntiles,
-                                                      unsigned long long *bits){
+                                                      unsigned long long *bits, K kmask){
     __shared__ uint32_t cnt[4][256];
     __shared__ unsigned long long so[RS_BLOCK / 64], sa[RS_BLOCK / 64];
+    __shared__ int sd[RS_BLOCK / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for(int q = 0; q < 4; q++) cnt[q][threadIdx.x] = 0;
     __syncthreads();
     const L base = (L)blockIdx.x * RS_TILE + (L)w * (RS_TILE / 4);
     unsigned long long o = 0, a = ~0ull;
+    bool desc = false;
     K kr[RS_ROWS];
     #pragma unroll
     for(int r = 0; r < RS_ROWS; r++){ const L i = base + (L)r * 64 + lane; kr[r] = i < n ? keys[i] : (K)0; }
+    // key of the row before this wave's first row (from the wave or tile before)
+    K prev = (bits && base > 0 && base <= n) ? keys[base - 1] : (K)0;
     #pragma unroll
     for(int r = 0; r < RS_ROWS; r++){
         const L i = base + (L)r * 64 + lane;
@@ -53,29 +58,38 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_hist(const K *keys, L n, int sh
         if(__ballot(valid && d == d0) == vm){ if(lane == 0) cnt[w][d0] += (uint32_t)__popcll(vm); }
         else if(valid) atomicAdd(&cnt[w][d], 1u);
         if(valid){ o |= (unsigned long long)k; a &= (unsigned long long)k; }
+        if(bits){
+            // the row before row i: lane - 1 of this row, or lane 63 of the row before
+            const K up = __shfl_up(k, 1, 64);
+            const K kp = lane == 0 ? prev : up;
+            if(valid && i > 0 && (k & kmask) < (kp & kmask)) desc = true;
+            prev = __shfl(k, 63, 64);
+        }
     }
     if(bits){
         for(int s = 32; s > 0; s >>= 1){ o |= __shfl_down(o, s, 64); a &= __shfl_down(a, s, 64); }
-        if(lane == 0){ so[w] = o; sa[w] = a; }
+        const bool anyd = __ballot(desc) != 0;
+        if(lane == 0){ so[w] = o; sa[w] = a; sd[w] = anyd ? 1 : 0; }
     }
     __syncthreads();
     hist[(L)blockIdx.x * 256 + threadIdx.x] = cnt[0][threadIdx.x] + cnt[1][threadIdx.x] + cnt[2][threadIdx.x] + cnt[3][threadIdx.x];
     if(bits && threadIdx.x == 0){
-        for(int q = 1; q < RS_BLOCK / 64; q++){ o |= so[q]; a &= sa[q]; }
-        bits[2 * blockIdx.x] = o; bits[2 * blockIdx.x + 1] = a;
+        int dd = sd[0];
+        for(int q = 1; q < RS_BLOCK / 64; q++){ o |= so[q]; a &= sa[q]; dd |= sd[q]; }
+        bits[4 * blockIdx.x] = o; bits[4 * blockIdx.x + 1] = a; bits[4 * blockIdx.x + 2] = (unsigned long long)dd;
     }
 }
-// OR / AND of the tiles' partials (one workgroup)
+// OR / AND / any-descent of the tiles' partials (one workgroup)
 __global__ void __launch_bounds__(256) k_rs_bits_final(const unsigned long long *part, L ntiles, unsigned long long *out){
-    __shared__ unsigned long long so[4], sa[4];
-    unsigned long long o = 0, a = ~0ull;
-    for(L t = threadIdx.x; t < ntiles; t += 256){ o |= part[2 * t]; a &= part[2 * t + 1]; }
-    for(int s = 32; s > 0; s >>= 1){ o |= __shfl_down(o, s, 64); a &= __shfl_down(a, s, 64); }
-    if((threadIdx.x & 63) == 0){ so[threadIdx.x >> 6] = o; sa[threadIdx.x >> 6] = a; }
+    __shared__ unsigned long long so[4], sa[4], sd[4];
+    unsigned long long o = 0, a = ~0ull, dd = 0;
+    for(L t = threadIdx.x; t < ntiles; t += 256){ o |= part[4 * t]; a &= part[4 * t + 1]; dd |= part[4 * t + 2]; }
+    for(int s = 32; s > 0; s >>= 1){ o |= __shfl_down(o, s, 64); a &= __shfl_down(a, s, 64); dd |= __shfl_down(dd, s, 64); }
+    if((threadIdx.x & 63) == 0){ so[threadIdx.x >> 6] = o; sa[threadIdx.x >> 6] = a; sd[threadIdx.x >> 6] = dd; }
     __syncthreads();
     if(threadIdx.x == 0){
-        for(int w = 1; w < 4; w++){ o |= so[w]; a &= sa[w]; }
-        out[0] = o; out[1] = a;
+        for(int w = 1; w < 4; w++){ o |= so[w]; a &= sa[w]; dd |= sd[w]; }
+        out[0] = o; out[1] = a; out[2] = dd;
     }
 }
 
@@ -277,26 +291,30 @@ static int32_t radix_sort_t(hpa_ctx *c, const K *keys, const uint32_t *vals, L n
     int32_t e;
     if(n <= 0){ if(!out64){ *kout = (K*)keys; *vout = (uint32_t*)vals; } R 0; }
     const L ntiles = ceil_div(n, RS_TILE);
-    unsigned long long *bits = (unsigned long long*)dalloc(c, 16 + 16 * ntiles);
+    unsigned long long *bits = (unsigned long long*)dalloc(c, 32 + 32 * ntiles);
     uint32_t *hist = (uint32_t*)dalloc(c, 4 * 256 * ntiles);
     int64_t *off = (int64_t*)dalloc(c, 8 * 256 * ntiles);
     uint32_t *part = (uint32_t*)dalloc(c, 4 * 256 * digit_scan_chunks(ntiles));
     uint32_t *cbase = (uint32_t*)dalloc(c, 4 * 256 * digit_scan_chunks(ntiles));
     uint32_t *tot = (uint32_t*)dalloc(c, 4 * 256);
     if(!bits || !hist || !off || !part || !cbase || !tot) R HPA_E_MEMORY;
-    // digits to sort by: the first histogram pass also finds the bits that vary
+    // digits to sort by: the first histogram pass also finds the bits that vary, and whether
+    // the keys are already in order on the bits the digit passes see: [lo, lo + 8 * digits)
+    const int kw = (int)(8 * sizeof(K)), dw = (int)ceil_div(hi - lo, 8) * 8;
+    const K kmask = (K)((lo + dw >= kw ? ~(K)0 >> lo : (((K)1 << dw) - 1)) << lo);
     {
         ProfScope ps("radix_hist", c->stream, (double)n * sizeof(K) + 4.0 * 256 * ntiles);
-        hipLaunchKernelGGL(k_rs_hist<K>, dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream, keys, n, lo, hist, ntiles, bits + 2);
-        hipLaunchKernelGGL(k_rs_bits_final, dim3(1), dim3(256), 0, c->stream, bits + 2, ntiles, bits);
+        hipLaunchKernelGGL(k_rs_hist<K>, dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream, keys, n, lo, hist, ntiles, bits + 4, kmask);
+        hipLaunchKernelGGL(k_rs_bits_final, dim3(1), dim3(256), 0, c->stream, bits + 4, ntiles, bits);
     }
     if((e = check_launch("radix_hist"))) R e;
-    unsigned long long hb[2]; if((e = d2h(c, hb, bits, 16))) R e;
+    unsigned long long hb[3]; if((e = d2h(c, hb, bits, 24))) R e;
     const unsigned long long vary = hb[0] ^ hb[1];
     int run[8], nrun = 0;
     for(int s = lo; s < hi; s += 8) if((vary >> s) & 255ull) run[nrun++] = s;
-    if(nrun == 0){
-        // every digit is constant: the input order is already sorted
+    if(nrun == 0 || hb[2] == 0){
+        // every digit is constant, or no key is below the one before: the input order is the
+        // stable sorted order
         if(out64){
             hipLaunchKernelGGL(k_iota_i64, dim3(grid_for(n, 256, 4)), dim3(256), 0, c->stream, out64, n, reverse ? 1 : 0);
             R check_launch("radix_sort");
@@ -320,7 +338,7 @@ static int32_t radix_sort_t(hpa_ctx *c, const K *keys, const uint32_t *vals, L n
         if(!(r == 0 && s == lo)){
             ProfScope ps("radix_hist", c->stream, (double)n * sizeof(K) + 4.0 * 256 * ntiles);
             hipLaunchKernelGGL(k_rs_hist<K>, dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream, ka, n, s, hist, ntiles,
-                               (unsigned long long*)nullptr);
+                               (unsigned long long*)nullptr, (K)0);
         }
         if((e = digit_scan(c, hist, ntiles, part, cbase, tot, off))) R e;
         const double rd = (double)n * (sizeof(K) + (va ? 4 : 0)), wr = last && out64 ? 8.0 * n : (double)n * (sizeof(K) + 4);
@@ -1702,8 +1720,10 @@ __global__ void __launch_bounds__(256) k_br_rank(SmallCols s, L n, const unsigne
 static int32_t order_bucket_rank(hpa_ctx *c, const SmallCols &s, L n, uint32_t **perm_out, bool *fallback, bool *ties){
     int32_t e;
     *fallback = false;
+    // about two buckets per row: real keys crowd into a few stretches of the window (a sum's
+    // exponent takes a handful of values), and a row's ranking work is its bucket's size
     int lg = 0; while((1ll << lg) < n) lg++;
-    int nb = lg - 3; if(nb < 8) nb = 8; if(nb > 22) nb = 22;
+    int nb = lg + 1; if(nb < 8) nb = 8; if(nb > 22) nb = 22;
     const L B = 1ll << nb;
     unsigned long long *or_and = (unsigned long long*)dalloc(c, 16);
     int *over = (int*)dalloc(c, 16);
@@ -1721,7 +1741,7 @@ static int32_t order_bucket_rank(hpa_ctx *c, const SmallCols &s, L n, uint32_t *
     if((e = scan_u32_to_i64(c, cnt, start, B, start + B))) R e;
     hipLaunchKernelGGL(k_br_place, dim3(g), dim3(256), 0, c->stream, (const uint64_t*)k0, n, (const unsigned long long*)or_and, nb,
                        (const int*)over, (const int64_t*)start, cursor, slot_row, slot_k0);
-    hipLaunchKernelGGL(k_br_rank, dim3(g), dim3(256), 0, c->stream, s, n, (const unsigned long long*)or_and, nb, over,
+    hipLaunchKernelGGL(k_br_rank, dim3(grid_for(n, 256, 1)), dim3(256), 0, c->stream, s, n, (const unsigned long long*)or_and, nb, over,
                        (const int64_t*)start, (const uint32_t*)cnt, (const uint32_t*)slot_row, (const uint64_t*)slot_k0, perm);
     if((e = check_launch("order_bucket"))) R e;
     int ov[2] = {0, 0}; if((e = d2h(c, ov, over, 8))) R e;

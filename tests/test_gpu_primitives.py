@@ -257,6 +257,33 @@ def test_index_by_group_keys(gpu, n, clustered):
     assert np.array_equal(gpu.read(gpu.pfnIndex(oa, keys)), other[k])
 
 
+@pytest.mark.parametrize("n", [5000, 70001])
+def test_sorted_inputs(gpu, orc, n):
+    """Keys already in the sort's order skip the radix scatter passes.  Inputs: sorted in the
+    unsigned bit-pattern order the group uses; sorted as signed values (negatives then sit below
+    the positives, a descent in that order); sorted but for one descent at a lane, wave or
+    4096-row tile boundary; i64 keys; single-column orders of sorted and reverse-sorted columns."""
+    r = rng(n + 77)
+    v = r.integers(-10**6, 10**6, n).astype(np.int32)
+    uns = np.sort(v.view(np.uint32)).view(np.int32)
+    cases = [uns, np.sort(v)]
+    uq = np.unique(v.view(np.uint32)).view(np.int32)              # strictly ascending (unsigned)
+    for p in (64, 1024, 4096, 3 * 4096 + 1024 + 64):
+        if p < uq.size:
+            w = uq.copy()
+            w[p - 1], w[p] = uq[p], uq[p - 1]                     # the one descent: row p
+            cases.append(w)
+    for x in cases:
+        assert groups_of(gpu, gpu.pfnGroup(gpu.vec(x))) == groups_of(orc, orc.pfnGroup(orc.vec(x)))
+    l64 = np.sort(r.integers(0, 2**40, n).astype(np.int64))
+    assert groups_of(gpu, gpu.pfnGroup(gpu.vec(l64))) == groups_of(orc, orc.pfnGroup(orc.vec(l64)))
+    for x in (np.sort(v), np.sort(v)[::-1].copy(), l64, np.sort(r.random(n))):
+        for up in (1, 0):
+            g = gpu.read(gpu.pfnOrderBy(gpu.vec(x), gpu.bools([up])))
+            o = orc.read(orc.pfnOrderBy(orc.vec(x), orc.bools([up])))
+            assert np.array_equal(g, o), (x.dtype, up)
+
+
 def test_group_long(gpu, orc):
     for x in (np.array([-1, 4294967295, -1, 7, 7], np.int64),
               rng(9).integers(-5, 5, 5000).astype(np.int64) * (1 << 33),
