@@ -14,7 +14,9 @@
 //              a 64K-entry LDS table maps each key code to a block-local slot (first come,
 //              first served), each thread keeps register accumulators per slot, slots are
 //              reduced per block into partials; fga_merge combines the partials per key code
-//              in block order (deterministic) and emits groups in ascending code order, the
+//              (eight blocks over slices of the partials, folded by fga_merge2, when there
+//              are few groups) in a fixed order (deterministic) and emits groups in ascending
+//              code order, the
 //              order of pfnGroupTrie (h_primitive.cc:2656-2681).  More than NS distinct codes
 //              in one block is reported as an overflow and the caller takes the unfused path.
 #include "hpa_internal.h"
@@ -27,6 +29,9 @@
 #define FGA_NS 16          // largest slot count tried before the unfused fallback
 #define FGA_ROWS 4         // consecutive rows per thread and iteration (one vector load per column)
 #define FGA_BLOCK 256
+#define FGA_MS 8           // merge blocks, each over one slice of the partial slots
+#define FGA_MCAP 64        // most groups the sliced merge takes (more: one block merges all); fga_merge2
+                           // folds them with one 64-thread block
 
 static std::mutex g_jit_mu;
 struct JitFn { hipModule_t mod; hipFunction_t fn; };
@@ -634,49 +639,80 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
     }
     s << "    if(threadIdx.x == 0) a.used[blockIdx.x] = used;\n}\n";
 
-    // merge kernel: one block of 1024 threads
-    s << "struct M { const void *part; const unsigned *used; int G; unsigned *codes; int *g; u64 *cnt; i64 *first; double *f; i64 *q; };\n"
+    // merge kernels.  fga_merge: FGA_MS blocks of 1024 threads.  Every block lists the codes
+    // present (the partials' codes, one thread per slot); with at most FGA_MCAP groups each block
+    // takes one slice of the slots and writes a record per group, and fga_merge2 folds a group's
+    // FGA_MS records in block order (a fixed association, so runs give identical sums).  More
+    // groups: block 0 merges every slot and writes the results itself.
+    auto acc_init = [&](std::ostringstream &o, const char *ind){
+        o << ind << "u64 c = 0; i64 fs = 9223372036854775807LL;\n";
+        for(int k = 0; k < (int)acc.size(); k++){
+            if(acc[k].f && acc[k].cls == 0) o << ind << "KS t" << k << "; t" << k << ".s = 0; t" << k << ".c = 0;\n";
+            else if(acc[k].f) o << ind << "double t" << k << " = " << (acc[k].cls == 2 ? "1.7976931348623157e308" : "-1.7976931348623157e308") << ";\n";
+            else if(acc[k].cls == 0) o << ind << "u64 t" << k << " = 0;\n";
+            else o << ind << "i64 t" << k << " = " << (acc[k].cls == 2 ? "9223372036854775807LL" : "(-9223372036854775807LL-1)") << ";\n";
+        }
+    };
+    auto acc_final = [&](std::ostringstream &o, const char *ind){
+        o << ind << "m.cnt[r] = c; m.first[r] = fs;\n";
+        for(int k = 0; k < (int)acc.size(); k++){
+            if(acc[k].f && acc[k].cls == 0) o << ind << "m.f[(i64)r * NA + " << k << "] = t" << k << ".s + t" << k << ".c;\n";
+            else if(acc[k].f) o << ind << "m.f[(i64)r * NA + " << k << "] = t" << k << ";\n";
+            else o << ind << "m.q[(i64)r * NA + " << k << "] = (i64)t" << k << ";\n";
+        }
+    };
+    s << "#define MS " << FGA_MS << "\n#define MCAP " << FGA_MCAP << "\n"
+         "struct M { const void *part; const unsigned *used; int G; unsigned *codes; int *g; u64 *cnt; i64 *first; double *f; i64 *q;\n"
+         "           u64 *rc; i64 *rfs; double *rf; i64 *rq; };\n"
          "extern \"C\" __global__ void __launch_bounds__(1024) fga_merge(M m){\n"
          "    __shared__ unsigned bits[2048];\n"
          "    __shared__ unsigned pre[1024];\n"
+         "    __shared__ unsigned scodes[MCAP];\n"
          "    __shared__ u64 mc[16]; __shared__ i64 mfs[16]; __shared__ double mf[16][2 * NA + 1]; __shared__ i64 mq[16][NA + 1];\n"
          "    const P *part = (const P*)m.part;\n"
          "    const unsigned *pcode = (const unsigned*)(part + (i64)m.G * NS);\n"
+         "    const int SL = m.G * NS;\n"
          "    for(int k = threadIdx.x; k < 2048; k += 1024) bits[k] = 0;\n"
          "    __syncthreads();\n"
-         "    for(int b = threadIdx.x; b < m.G; b += 1024){ unsigned u = m.used[b]; for(unsigned s = 0; s < u; s++){ unsigned cd = pcode[(i64)b * NS + s]; atomicOr(&bits[cd >> 5], 1u << (cd & 31)); } }\n"
+         "    // one thread per slot: the used count and the code load together\n"
+         "    #pragma unroll 4\n"
+         "    for(int i = threadIdx.x; i < SL; i += 1024){\n"
+         "        const unsigned u = m.used[i / NS], cd = pcode[i];\n"
+         "        if((unsigned)(i % NS) < u) atomicOr(&bits[cd >> 5], 1u << (cd & 31));\n"
+         "    }\n"
          "    __syncthreads();\n"
          "    unsigned c0 = __popc(bits[2 * threadIdx.x]), c1 = __popc(bits[2 * threadIdx.x + 1]);\n"
          "    pre[threadIdx.x] = c0 + c1;\n"
          "    __syncthreads();\n"
          "    for(int o = 1; o < 1024; o <<= 1){ unsigned t = threadIdx.x >= (unsigned)o ? pre[threadIdx.x - o] : 0; __syncthreads(); pre[threadIdx.x] += t; __syncthreads(); }\n"
          "    unsigned pos = pre[threadIdx.x] - c0 - c1;\n"
-         "    for(int h = 0; h < 2; h++){ unsigned wd = bits[2 * threadIdx.x + h]; while(wd){ int bt = __ffs(wd) - 1; wd &= wd - 1; m.codes[pos++] = (unsigned)((2 * threadIdx.x + h) * 32 + bt); } }\n"
+         "    const bool b0 = blockIdx.x == 0;\n"
+         "    for(int h = 0; h < 2; h++){ unsigned wd = bits[2 * threadIdx.x + h]; while(wd){ int bt = __ffs(wd) - 1; wd &= wd - 1;\n"
+         "        const unsigned cv = (unsigned)((2 * threadIdx.x + h) * 32 + bt); if(b0) m.codes[pos] = cv; if(pos < MCAP) scodes[pos] = cv; pos++; } }\n"
          "    const int g = (int)pre[1023];\n"
-         "    if(threadIdx.x == 0) *m.g = g;\n"
+         "    const bool split = g <= MCAP && gridDim.x > 1;\n"
+         "    if(b0 && threadIdx.x == 0) *m.g = g;\n"
+         "    if(!split && !b0) return;\n"
          "    __syncthreads();\n"
          "    __threadfence_block();\n"
+         "    const int per = split ? (SL + (int)gridDim.x - 1) / (int)gridDim.x : SL;\n"
+         "    const int lo = split ? (int)blockIdx.x * per : 0, hi = lo + per < SL ? lo + per : SL;\n"
          "    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;\n"
          "    // W waves per group (16 / W groups at a time); each wave takes every W-th run of 64\n"
-         "    // partial blocks, reduces it with shuffles, and the group's first wave merges the W\n"
-         "    // wave results in wave order: a fixed association, so runs give identical sums\n"
+         "    // partial slots of the block's range, reduces it with shuffles, and the group's first\n"
+         "    // wave merges the W wave results in wave order\n"
          "    const int W = g >= 16 ? 1 : (g >= 8 ? 2 : (g >= 4 ? 4 : (g >= 2 ? 8 : 16)));\n"
          "    const int sub = wv % W;\n"
          "    for(int r0 = 0; r0 < g; r0 += 16 / W){\n"
          "        const int r = r0 + wv / W;\n"
-         "        const unsigned code = r < g ? m.codes[r] : 0xFFFFFFFFu;\n"
-         "        u64 c = 0; i64 fs = 9223372036854775807LL;\n";
-    for(int k = 0; k < (int)acc.size(); k++){
-        if(acc[k].f && acc[k].cls == 0) s << "        KS t" << k << "; t" << k << ".s = 0; t" << k << ".c = 0;\n";
-        else if(acc[k].f) s << "        double t" << k << " = " << (acc[k].cls == 2 ? "1.7976931348623157e308" : "-1.7976931348623157e308") << ";\n";
-        else if(acc[k].cls == 0) s << "        u64 t" << k << " = 0;\n";
-        else s << "        i64 t" << k << " = " << (acc[k].cls == 2 ? "9223372036854775807LL" : "(-9223372036854775807LL-1)") << ";\n";
+         "        const unsigned code = r < g ? (split ? scodes[r] : m.codes[r]) : 0xFFFFFFFFu;\n";
+    {
+        std::ostringstream o; acc_init(o, "        "); s << o.str();
     }
     // partial slots taken in a flat order (slot i = block i / NS, entry i % NS): the loads of
     // successive iterations do not depend on each other and overlap
-    s << "        const int SL = m.G * NS;\n"
-         "        #pragma unroll 4\n"
-         "        for(int i = sub * 64 + lane; i < SL; i += 64 * W){\n"
+    s << "        #pragma unroll 4\n"
+         "        for(int i = lo + sub * 64 + lane; i < hi; i += 64 * W){\n"
          "            if(r >= g || (unsigned)(i % NS) >= m.used[i / NS]) continue;\n"
          "            {\n"
          "                if(pcode[i] != code) continue;\n"
@@ -716,43 +752,80 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
         else s << "                { i64 u = mq[q][" << k << "]; t" << k << " = u " << (acc[k].cls == 2 ? "<" : ">") << " t" << k << " ? u : t" << k << "; }\n";
     }
     s << "            }\n"
-         "            m.cnt[r] = c; m.first[r] = fs;\n";
+         "            if(split){\n"
+         "                const i64 rec = (i64)r * MS + blockIdx.x;\n"
+         "                m.rc[rec] = c; m.rfs[rec] = fs;\n";
     for(int k = 0; k < (int)acc.size(); k++){
-        if(acc[k].f && acc[k].cls == 0) s << "            m.f[(i64)r * NA + " << k << "] = t" << k << ".s + t" << k << ".c;\n";
-        else if(acc[k].f) s << "            m.f[(i64)r * NA + " << k << "] = t" << k << ";\n";
-        else s << "            m.q[(i64)r * NA + " << k << "] = (i64)t" << k << ";\n";
+        if(acc[k].f && acc[k].cls == 0) s << "                m.rf[rec * (2 * NA) + 2*" << k << "] = t" << k << ".s; m.rf[rec * (2 * NA) + 2*" << k << "+1] = t" << k << ".c;\n";
+        else if(acc[k].f) s << "                m.rf[rec * (2 * NA) + 2*" << k << "] = t" << k << ";\n";
+        else s << "                m.rq[rec * NA + " << k << "] = (i64)t" << k << ";\n";
     }
-    s << "        }\n        __syncthreads();\n";
+    s << "            } else {\n";
+    {
+        std::ostringstream o; acc_final(o, "                "); s << o.str();
+    }
+    s << "            }\n"
+         "        }\n        __syncthreads();\n";
     s << "    }\n}\n";
+    // fga_merge2: with at most MCAP groups, block 0 folds each group's FGA_MS records in block
+    // order (one thread per group); then every group's typed outputs are written (the layout of
+    // FgaOut: key bytes, first row, and per output a count, an average, or the accumulator)
+    s << "struct EO { int na, stride, nk; unsigned kx0, kx1; int kind[16]; int accid[16]; int accf[16]; void *out[16];\n"
+         "            unsigned char *key0, *key1; i64 *first; };\n"
+         "struct M2 { M m; EO o; };\n"
+         "__device__ __forceinline__ void fga_emit(const M &m, const EO &o, int r){\n"
+         "    const unsigned cd = m.codes[r];\n"
+         "    if(o.nk == 2){ o.key0[r] = (unsigned char)((cd >> 8) ^ o.kx0); o.key1[r] = (unsigned char)((cd & 255) ^ o.kx1); }\n"
+         "    else o.key0[r] = (unsigned char)(cd ^ o.kx0);\n"
+         "    o.first[r] = m.first[r];\n"
+         "    for(int a = 0; a < o.na; a++){\n"
+         "        const i64 at = (i64)r * o.stride + o.accid[a];\n"
+         "        switch(o.kind[a]){\n"
+         "            case 4: ((i64*)o.out[a])[r] = (i64)m.cnt[r]; break;\n"
+         "            case 1:      // avg = sum / len\n"
+         "                if(o.accf[a]) ((double*)o.out[a])[r] = m.f[at] / (double)m.cnt[r];\n"
+         "                else ((double*)o.out[a])[r] = (double)m.q[at] * 1.0 / (double)m.cnt[r];\n"
+         "                break;\n"
+         "            default:\n"
+         "                if(o.accf[a]) ((double*)o.out[a])[r] = m.f[at];\n"
+         "                else ((i64*)o.out[a])[r] = m.q[at];\n"
+         "        }\n"
+         "    }\n"
+         "}\n"
+         "extern \"C\" __global__ void __launch_bounds__(64) fga_merge2(M2 a){\n"
+         "    const M &m = a.m;\n"
+         "    const int g = *m.g;\n"
+         "    if(g > MCAP){\n"
+         "        for(int r = blockIdx.x * 64 + threadIdx.x; r < g; r += gridDim.x * 64) fga_emit(m, a.o, r);\n"
+         "        return;\n"
+         "    }\n"
+         "    const int r = threadIdx.x;\n"
+         "    if(blockIdx.x != 0 || r >= g) return;\n";
+    {
+        std::ostringstream o; acc_init(o, "    "); s << o.str();
+    }
+    s << "    #pragma unroll\n"
+         "    for(int b = 0; b < MS; b++){\n"
+         "        const i64 rec = (i64)r * MS + b;\n"
+         "        c += m.rc[rec]; fs = m.rfs[rec] < fs ? m.rfs[rec] : fs;\n";
+    for(int k = 0; k < (int)acc.size(); k++){
+        if(acc[k].f && acc[k].cls == 0) s << "        { KS u; u.s = m.rf[rec * (2 * NA) + 2*" << k << "]; u.c = m.rf[rec * (2 * NA) + 2*" << k << "+1]; ks_merge(t" << k << ", u); }\n";
+        else if(acc[k].f) s << "        { double u = m.rf[rec * (2 * NA) + 2*" << k << "]; t" << k << " = u " << (acc[k].cls == 2 ? "<" : ">") << " t" << k << " ? u : t" << k << "; }\n";
+        else if(acc[k].cls == 0) s << "        t" << k << " += (u64)m.rq[rec * NA + " << k << "];\n";
+        else s << "        { i64 u = m.rq[rec * NA + " << k << "]; t" << k << " = u " << (acc[k].cls == 2 ? "<" : ">") << " t" << k << " ? u : t" << k << "; }\n";
+    }
+    s << "    }\n";
+    {
+        std::ostringstream o; acc_final(o, "    "); s << o.str();
+    }
+    s << "    fga_emit(m, a.o, r);\n";
+    s << "}\n";
     R s.str();
 }
 
-// outputs of the merge are laid out [group][accumulator]; this kernel makes typed vectors
+// outputs of the merge are laid out [group][accumulator]; fga_merge2 makes typed vectors of
+// them through this layout (mirrored by EO in the generated source)
 struct FgaOut { int na, stride, nk; unsigned kx0, kx1; int kind[16]; int accid[16]; int accf[16]; void *out[16]; uint8_t *key0, *key1; int64_t *first; };
-__global__ void k_fga_emit(const int *gp, const unsigned *codes, const unsigned long long *cnt, const long long *first,
-                           const double *f, const long long *q, FgaOut o){
-    const int g = *gp;
-    for(int r = blockIdx.x * blockDim.x + threadIdx.x; r < g; r += gridDim.x * blockDim.x){
-        unsigned cd = codes[r];
-        if(o.nk == 2){ o.key0[r] = (uint8_t)((cd >> 8) ^ o.kx0); o.key1[r] = (uint8_t)((cd & 255) ^ o.kx1); }
-        else o.key0[r] = (uint8_t)(cd ^ o.kx0);
-        o.first[r] = first[r];
-        for(int a = 0; a < o.na; a++){
-            long long at = (long long)r * o.stride + o.accid[a];
-            switch(o.kind[a]){
-                case 4: ((int64_t*)o.out[a])[r] = (int64_t)cnt[r]; break;
-                case 1:      // avg = sum / len (pfnAvg, h_primitive.cc:788-800)
-                    if(o.accf[a]) ((double*)o.out[a])[r] = f[at] / (double)cnt[r];
-                    else ((double*)o.out[a])[r] = (double)q[at] * 1.0 / (double)cnt[r];
-                    break;
-                default:
-                    if(o.accf[a]) ((double*)o.out[a])[r] = f[at];
-                    else ((int64_t*)o.out[a])[r] = q[at];
-            }
-        }
-    }
-}
-
 // a one-element result is inline: its value comes from `host` when given, else from the device
 static void set_dev_vec(hpa_ctx *c, V z, int32_t typ, L len, void *p, const void *host = nullptr){
     z->typ = typ; z->len = len;
@@ -861,7 +934,13 @@ static int32_t fga_run(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_
     long long *first = (long long*)dalloc(c, 8 * GM);
     double *fo = (double*)dalloc(c, 8 * GM * NA);
     long long *qo = (long long*)dalloc(c, 8 * GM * NA);
-    if(!part || !usedb || !ovf || !codes || !gdev || !cnt || !first || !fo || !qo){ c->dcur = mark; R HPA_E_MEMORY; }
+    // per-(group, merge block) records of the sliced merge
+    const L NREC = (L)FGA_MCAP * FGA_MS;
+    unsigned long long *rc = (unsigned long long*)dalloc(c, 8 * NREC);
+    long long *rfs = (long long*)dalloc(c, 8 * NREC);
+    double *rf = (double*)dalloc(c, 16 * NREC * NA);
+    long long *rq = (long long*)dalloc(c, 8 * NREC * NA);
+    if(!part || !usedb || !ovf || !codes || !gdev || !cnt || !first || !fo || !qo || !rc || !rfs || !rf || !rq){ c->dcur = mark; R HPA_E_MEMORY; }
     struct { const void *c[EXPR_MAX_COLS]; const void *k0; const void *k1; long long n; void *part; unsigned *used; int *ovf;
              unsigned kx0, kx1; long long ki[JIT_MAX_CONST]; double kf[JIT_MAX_CONST]; } ga;
     memset(&ga, 0, sizeof(ga));
@@ -881,16 +960,19 @@ static int32_t fga_run(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_
         for(int k = 0; k < nx; k++) key += used[k] ? '1' : '0';
         for(auto &ac : acc) key += "|" + ac.body + "#" + std::to_string(ac.cls) + (ac.f ? "f" : "i");
         auto gen = [&](){ return fga_source(nk, ns, nx, typ, ce, used, acc); };
-        hipFunction_t f1, f2;
-        if((e = jit_get_keyed(key, "fga", gen, &f1)) || (e = jit_get_keyed(key, "fga_merge", gen, &f2))) R e;
+        hipFunction_t f1, f2, f3;
+        if((e = jit_get_keyed(key, "fga", gen, &f1)) || (e = jit_get_keyed(key, "fga_merge", gen, &f2)) ||
+           (e = jit_get_keyed(key, "fga_merge2", gen, &f3))) R e;
         hipMemsetAsync(ovf, 0, 4, c->stream);
         {
             ProfScope ps("fused_group_agg", c->stream, (double)n * cb);
             if((e = jit_launch(c, f1, (unsigned)G, 256, ga))) R e;
         }
-        struct { const void *part; const unsigned *used; int G; unsigned *codes; int *g; unsigned long long *cnt; long long *first; double *f; long long *q; } ma;
+        struct { const void *part; const unsigned *used; int G; unsigned *codes; int *g; unsigned long long *cnt; long long *first; double *f; long long *q;
+                 unsigned long long *rc; long long *rfs; double *rf; long long *rq; } ma;
         ma.part = part; ma.used = usedb; ma.G = G; ma.codes = codes; ma.g = gdev; ma.cnt = cnt; ma.first = first; ma.f = fo; ma.q = qo;
-        if((e = jit_launch(c, f2, 1, 1024, ma))) R e;
+        ma.rc = rc; ma.rfs = rfs; ma.rf = rf; ma.rq = rq;
+        if((e = jit_launch(c, f2, FGA_MS, 1024, ma))) R e;
         // the typed outputs are sized for every possible group, so they are written right behind
         // the merge and one read brings back the overflow flag and the group count
         if(!o.key0){
@@ -899,8 +981,9 @@ static int32_t fga_run(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_
             if(!o.key0 || !o.key1 || !o.first){ c->dcur = mark; R HPA_E_MEMORY; }
             for(int a = 0; a < na; a++){ o.out[a] = dalloc(c, 8 * GM); if(!o.out[a]){ c->dcur = mark; R HPA_E_MEMORY; } }
         }
-        hipLaunchKernelGGL(k_fga_emit, dim3(grid_for(GM, 64, 1)), dim3(64), 0, c->stream, gdev, codes, cnt, first, fo, qo, o);
-        if((e = check_launch("fga_emit"))) R e;
+        struct { decltype(ma) m; FgaOut o; } m2;
+        m2.m = ma; m2.o = o;
+        if((e = jit_launch(c, f3, (unsigned)grid_for(GM, 64, 1), 64, m2))) R e;
         if(hd){
             // the flag, the group count and the first cap groups of every output in one read
             const L cap = hd->cap < GM ? hd->cap : GM;

@@ -107,6 +107,40 @@ def test_fused_group_agg_host(gpu, keys, cap):
         assert a.dtype == b.dtype and np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("ngroups", [64, 65, 128])
+def test_fused_group_agg_many_groups(gpu, ngroups):
+    """Groups in contiguous row ranges: each workgroup meets a few codes, the whole column up to
+    128 (the merge slices the partials up to 64 groups and takes them in one block beyond)."""
+    from horsepower_amd.vnode import H_C, expr
+    r = np.random.default_rng(ngroups)
+    n = 1 << 22
+    idx = (np.arange(n, dtype=np.int64) * ngroups) // n
+    k0 = (65 + idx % 16).astype(np.uint8)
+    k1 = (65 + idx // 16).astype(np.uint8)
+    v = np.round(r.uniform(0, 100, n), 2)
+    w = r.integers(-50, 50, n).astype(np.int32)
+    spec = [("sum", expr([("COL", 0)])), ("avg", expr([("COL", 0)])), ("min", expr([("COL", 1)])),
+            ("max", expr([("COL", 0)])), ("sum", expr([("COL", 1)])), ("len", None)]
+    res = gpu.fused_group_agg([gpu.vec(k0, H_C), gpu.vec(k1, H_C)], [gpu.vec(v), gpu.vec(w)], None, spec)
+    assert res is not None
+    zf, (za, zb), (zs, zav, zmn, zmx, zis, zc) = res
+    want = sorted(set(zip(k0.tolist(), k1.tolist())))
+    assert len(want) == ngroups
+    assert list(zip(gpu.read(za).tolist(), gpu.read(zb).tolist())) == want
+    code = k0.astype(np.int64) * 256 + k1
+    first, sums, mins, maxs, isums, cnts = [], [], [], [], [], []
+    for a, b in want:
+        rows = np.flatnonzero(code == a * 256 + b)
+        first.append(rows[0]); cnts.append(rows.size); sums.append(math.fsum(v[rows]))
+        mins.append(w[rows].min()); maxs.append(v[rows].max()); isums.append(int(w[rows].sum()))
+    assert gpu.read(zf).tolist() == first
+    assert gpu.read(zc).tolist() == cnts
+    assert np.allclose(gpu.read(zs), sums, rtol=RTOL, atol=0)
+    assert np.allclose(gpu.read(zav), np.array(sums) / np.array(cnts), rtol=RTOL, atol=0)
+    assert gpu.read(zmn).tolist() == mins and gpu.read(zmx).tolist() == maxs
+    assert gpu.read(zis).tolist() == isums
+
+
 def test_fused_group_agg_key_order(gpu):
     """The one-pass group emits its groups in ascending key order with char bytes compared as
     signed values, the order pfnOrderBy gives the key columns (bytes >= 128 sort first)."""
