@@ -491,7 +491,7 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
          "    else r = *(const VU<T>*)p;\n"
          "    return r;\n"
          "}\n"
-         "struct A { const void *c[" << EXPR_MAX_COLS << "]; const void *k0; const void *k1; i64 n; void *part; unsigned *used; int *ovf; unsigned kx0, kx1; " KONST_FIELDS " };\n"
+         "struct A { const void *c[" << EXPR_MAX_COLS << "]; const void *k0; const void *k1; i64 n; void *part; unsigned *used; unsigned kx0, kx1; " KONST_FIELDS " };\n"
          // a partial slot; the slots' codes live in their own array after the G * NS slots, so the
          // merge tests 64 slots' codes with one line and loads only the slots that match
          "struct P { u64 cnt; i64 first; double f[2 * NA]; i64 q[NA]; };\n"
@@ -591,7 +591,6 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
       << row.str()
       << "            }\n        }\n    }\n"
          "fga_done:\n"
-         "    if(__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) && threadIdx.x == 0) atomicOr(a.ovf, 1);\n"
          "    __syncthreads();\n"
          "    const unsigned used = s_used < NS ? s_used : NS;\n"
          "    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;\n"
@@ -637,7 +636,8 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
         }
         s << "            out[" << q << "] = r;\n        }\n        __syncthreads();\n    }\n";
     }
-    s << "    if(threadIdx.x == 0) a.used[blockIdx.x] = used;\n}\n";
+    // bit 31 of the used count: this block met more codes than slots (the merge reports it)
+    s << "    if(threadIdx.x == 0) a.used[blockIdx.x] = used | (__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) ? 0x80000000u : 0u);\n}\n";
 
     // merge kernels.  fga_merge: FGA_MS blocks of 1024 threads.  Every block lists the codes
     // present (the partials' codes, one thread per slot); with at most FGA_MCAP groups each block
@@ -663,41 +663,48 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
     };
     s << "#define MS " << FGA_MS << "\n#define MCAP " << FGA_MCAP << "\n"
          "struct M { const void *part; const unsigned *used; int G; unsigned *codes; int *g; u64 *cnt; i64 *first; double *f; i64 *q;\n"
-         "           u64 *rc; i64 *rfs; double *rf; i64 *rq; };\n"
+         "           u64 *rc; i64 *rfs; double *rf; i64 *rq; int *ovf; };\n"
          "extern \"C\" __global__ void __launch_bounds__(1024) fga_merge(M m){\n"
          "    __shared__ unsigned bits[2048];\n"
-         "    __shared__ unsigned pre[1024];\n"
+         "    __shared__ unsigned wsum[16];\n"
+         "    __shared__ int sovf;\n"
          "    __shared__ unsigned scodes[MCAP];\n"
          "    __shared__ u64 mc[16]; __shared__ i64 mfs[16]; __shared__ double mf[16][2 * NA + 1]; __shared__ i64 mq[16][NA + 1];\n"
          "    const P *part = (const P*)m.part;\n"
          "    const unsigned *pcode = (const unsigned*)(part + (i64)m.G * NS);\n"
          "    const int SL = m.G * NS;\n"
          "    for(int k = threadIdx.x; k < 2048; k += 1024) bits[k] = 0;\n"
+         "    if(threadIdx.x == 0) sovf = 0;\n"
          "    __syncthreads();\n"
-         "    // one thread per slot: the used count and the code load together\n"
+         "    // one thread per slot: the used count and the code load together; bit 31 of a used\n"
+         "    // count is the block's overflow\n"
          "    #pragma unroll 4\n"
          "    for(int i = threadIdx.x; i < SL; i += 1024){\n"
-         "        const unsigned u = m.used[i / NS], cd = pcode[i];\n"
-         "        if((unsigned)(i % NS) < u) atomicOr(&bits[cd >> 5], 1u << (cd & 31));\n"
+         "        const unsigned uw = m.used[i / NS], cd = pcode[i];\n"
+         "        if((uw >> 31) && (i % NS) == 0) sovf = 1;\n"
+         "        if((unsigned)(i % NS) < (uw & 0x7FFFFFFFu)) atomicOr(&bits[cd >> 5], 1u << (cd & 31));\n"
          "    }\n"
          "    __syncthreads();\n"
-         "    unsigned c0 = __popc(bits[2 * threadIdx.x]), c1 = __popc(bits[2 * threadIdx.x + 1]);\n"
-         "    pre[threadIdx.x] = c0 + c1;\n"
+         "    // list position of this thread's codes: a scan inside each wave, then the wave totals\n"
+         "    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;\n"
+         "    const unsigned cc = __popc(bits[2 * threadIdx.x]) + __popc(bits[2 * threadIdx.x + 1]);\n"
+         "    unsigned inc = cc;\n"
+         "    for(int o = 1; o < 64; o <<= 1){ const unsigned t = __shfl_up(inc, o, 64); if(lane >= o) inc += t; }\n"
+         "    if(lane == 63) wsum[wv] = inc;\n"
          "    __syncthreads();\n"
-         "    for(int o = 1; o < 1024; o <<= 1){ unsigned t = threadIdx.x >= (unsigned)o ? pre[threadIdx.x - o] : 0; __syncthreads(); pre[threadIdx.x] += t; __syncthreads(); }\n"
-         "    unsigned pos = pre[threadIdx.x] - c0 - c1;\n"
+         "    unsigned pos = inc - cc, gt = 0;\n"
+         "    for(int q = 0; q < 16; q++){ const unsigned t = wsum[q]; if(q < wv) pos += t; gt += t; }\n"
          "    const bool b0 = blockIdx.x == 0;\n"
          "    for(int h = 0; h < 2; h++){ unsigned wd = bits[2 * threadIdx.x + h]; while(wd){ int bt = __ffs(wd) - 1; wd &= wd - 1;\n"
          "        const unsigned cv = (unsigned)((2 * threadIdx.x + h) * 32 + bt); if(b0) m.codes[pos] = cv; if(pos < MCAP) scodes[pos] = cv; pos++; } }\n"
-         "    const int g = (int)pre[1023];\n"
+         "    const int g = (int)gt;\n"
          "    const bool split = g <= MCAP && gridDim.x > 1;\n"
-         "    if(b0 && threadIdx.x == 0) *m.g = g;\n"
+         "    if(b0 && threadIdx.x == 0){ *m.g = g; *m.ovf = sovf; }\n"
          "    if(!split && !b0) return;\n"
          "    __syncthreads();\n"
          "    __threadfence_block();\n"
          "    const int per = split ? (SL + (int)gridDim.x - 1) / (int)gridDim.x : SL;\n"
          "    const int lo = split ? (int)blockIdx.x * per : 0, hi = lo + per < SL ? lo + per : SL;\n"
-         "    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;\n"
          "    // W waves per group (16 / W groups at a time); each wave takes every W-th run of 64\n"
          "    // partial slots of the block's range, reduces it with shuffles, and the group's first\n"
          "    // wave merges the W wave results in wave order\n"
@@ -713,9 +720,9 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
     // successive iterations do not depend on each other and overlap
     s << "        #pragma unroll 4\n"
          "        for(int i = lo + sub * 64 + lane; i < hi; i += 64 * W){\n"
-         "            if(r >= g || (unsigned)(i % NS) >= m.used[i / NS]) continue;\n"
+         "            const unsigned u = m.used[i / NS] & 0x7FFFFFFFu, cd = pcode[i];\n"
+         "            if(r >= g || (unsigned)(i % NS) >= u || cd != code) continue;\n"
          "            {\n"
-         "                if(pcode[i] != code) continue;\n"
          "                const P &p = part[i];\n"
          "                c += p.cnt; fs = p.first < fs ? p.first : fs;\n";
     for(int k = 0; k < (int)acc.size(); k++){
@@ -941,12 +948,12 @@ static int32_t fga_run(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_
     double *rf = (double*)dalloc(c, 16 * NREC * NA);
     long long *rq = (long long*)dalloc(c, 8 * NREC * NA);
     if(!part || !usedb || !ovf || !codes || !gdev || !cnt || !first || !fo || !qo || !rc || !rfs || !rf || !rq){ c->dcur = mark; R HPA_E_MEMORY; }
-    struct { const void *c[EXPR_MAX_COLS]; const void *k0; const void *k1; long long n; void *part; unsigned *used; int *ovf;
+    struct { const void *c[EXPR_MAX_COLS]; const void *k0; const void *k1; long long n; void *part; unsigned *used;
              unsigned kx0, kx1; long long ki[JIT_MAX_CONST]; double kf[JIT_MAX_CONST]; } ga;
     memset(&ga, 0, sizeof(ga));
     for(int k = 0; k < nx; k++) ga.c[k] = cols.p[k];
     memcpy(ga.ki, kc.i, sizeof(ga.ki)); memcpy(ga.kf, kc.f, sizeof(ga.kf));
-    ga.k0 = pk0; ga.k1 = pk1; ga.n = n; ga.part = part; ga.used = usedb; ga.ovf = ovf;
+    ga.k0 = pk0; ga.k1 = pk1; ga.n = n; ga.part = part; ga.used = usedb;
     // key bytes compared as their type orders them: C and J signed, B unsigned
     ga.kx0 = (keys[0]->typ == HPA_B) ? 0u : 0x80u;
     ga.kx1 = (nk == 2 && keys[1]->typ != HPA_B) ? 0x80u : 0u;
@@ -963,15 +970,14 @@ static int32_t fga_run(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_
         hipFunction_t f1, f2, f3;
         if((e = jit_get_keyed(key, "fga", gen, &f1)) || (e = jit_get_keyed(key, "fga_merge", gen, &f2)) ||
            (e = jit_get_keyed(key, "fga_merge2", gen, &f3))) R e;
-        hipMemsetAsync(ovf, 0, 4, c->stream);
         {
             ProfScope ps("fused_group_agg", c->stream, (double)n * cb);
             if((e = jit_launch(c, f1, (unsigned)G, 256, ga))) R e;
         }
         struct { const void *part; const unsigned *used; int G; unsigned *codes; int *g; unsigned long long *cnt; long long *first; double *f; long long *q;
-                 unsigned long long *rc; long long *rfs; double *rf; long long *rq; } ma;
+                 unsigned long long *rc; long long *rfs; double *rf; long long *rq; int *ovf; } ma;
         ma.part = part; ma.used = usedb; ma.G = G; ma.codes = codes; ma.g = gdev; ma.cnt = cnt; ma.first = first; ma.f = fo; ma.q = qo;
-        ma.rc = rc; ma.rfs = rfs; ma.rf = rf; ma.rq = rq;
+        ma.rc = rc; ma.rfs = rfs; ma.rf = rf; ma.rq = rq; ma.ovf = ovf;
         if((e = jit_launch(c, f2, FGA_MS, 1024, ma))) R e;
         // the typed outputs are sized for every possible group, so they are written right behind
         // the merge and one read brings back the overflow flag and the group count
