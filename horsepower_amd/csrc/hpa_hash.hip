@@ -189,7 +189,14 @@ static int32_t pk_build(hpa_ctx *c, const int32_t *x, L n, unsigned long long **
 //     its three dependent reads cost more than one read of the larger table.)
 //   member: one flag byte per key value (plain stores), packed into a bitmap of the range that
 //     stays in every XCD's L2 (750 KB for 6M values).
+// Sparser ranges, up to GUARD_PER_KEY slots per key and GUARD_MAX_RANGE slots, keep the direct
+// table for index_of and join but guard it with a presence bitmap: only the bitmap is cleared
+// (one bit per slot), the table is written at the keys present and read only where the bit is
+// set.  The bitmap is small enough to stay in cache (7.5 MB for a 60M-key range), and probes
+// that miss touch nothing else.  Taken when the arena has room for the table's address range.
 #define DENSE_PER_KEY 8
+#define GUARD_PER_KEY 512
+#define GUARD_MAX_RANGE (1ll << 28)
 // smallest and largest key: per-workgroup partials part[2 b], part[2 b + 1] (16-byte loads
 // when aligned), folded by one workgroup into out[0] min, out[1] max
 __device__ __forceinline__ void i32_minmax_block(int lo, int hi, int *part){
@@ -271,15 +278,19 @@ static int32_t i32_order_range(hpa_ctx *c, const int32_t *x, L n, bool *ord, int
     *lo = h[2]; *hi = h[3];
     R 0;
 }
-// [lo, hi] of n i32 keys; *dense when they span at most DENSE_PER_KEY slots per key; a range
-// known already comes in through known (lo, hi)
-static int32_t dense_range(hpa_ctx *c, const int32_t *x, L n, int *lo, int *hi, bool *dense, const int *known = nullptr){
+// [lo, hi] of n i32 keys; *dense when they span at most DENSE_PER_KEY slots per key, *guard
+// (when asked) when they span at most GUARD_PER_KEY slots per key and GUARD_MAX_RANGE slots; a
+// range known already comes in through known (lo, hi)
+static int32_t dense_range(hpa_ctx *c, const int32_t *x, L n, int *lo, int *hi, bool *dense, const int *known = nullptr,
+                           bool *guard = nullptr){
     *dense = false;
+    if(guard) *guard = false;
     if(n < 1024) R 0;                                  // small builds: the hash table is cheap
     if(known){
         *lo = known[0]; *hi = known[1];
         const L R_ = (L)known[1] - (L)known[0] + 1;
         *dense = R_ <= (L)DENSE_PER_KEY * n && R_ < (L)INT32_MAX;
+        if(guard) *guard = !*dense && R_ <= (L)GUARD_PER_KEY * n && R_ <= GUARD_MAX_RANGE;
         R 0;
     }
     const int np = grid_for(ceil_div(n, 4), 256, 4, num_cus() * 4);
@@ -292,6 +303,7 @@ static int32_t dense_range(hpa_ctx *c, const int32_t *x, L n, int *lo, int *hi, 
     *lo = h[0]; *hi = h[1];
     const L R_ = (L)h[1] - (L)h[0] + 1;
     *dense = R_ <= (L)DENSE_PER_KEY * n && R_ < (L)INT32_MAX;
+    if(guard) *guard = !*dense && R_ <= (L)GUARD_PER_KEY * n && R_ <= GUARD_MAX_RANGE;
     R 0;
 }
 __global__ void k_dense_flag(const int32_t *x, L n, int lo, uint8_t *flag){
@@ -334,10 +346,22 @@ __global__ void k_dense_first(const int32_t *x, L n, int lo, int32_t *tab, int *
     }
     if(__any(d) && (threadIdx.x & 63) == 0 && !*(volatile int*)dup) atomicOr(dup, 1);
 }
-struct DenseTab { int lo; L m; uint32_t *bits; int32_t *tab; int *dup; };
+// pres: presence bitmap of a guarded table (null: every slot of tab is written)
+struct DenseTab { int lo; L m; uint32_t *bits; int32_t *tab; int *dup; const uint32_t *pres; };
+__global__ void k_fill_u32(uint32_t *p, L n, uint32_t v){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x) p[i] = v;
+}
+// guarded table: every build row stores its index and sets its key's presence bit
+__global__ void k_guard_store(const int32_t *x, L n, int lo, int32_t *tab, uint32_t *pres){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
+        const uint32_t k = (uint32_t)(x[i] - lo);
+        tab[k] = (int32_t)i;
+        atomicOr(&pres[k >> 5], 1u << (k & 31));
+    }
+}
 // first-row table (rows) or bitmap (member) over x[0..n), keys in [lo, lo + m)
 static int32_t dense_build(hpa_ctx *c, const int32_t *x, L n, int lo, L m, bool rows, DenseTab *t){
-    t->lo = lo; t->m = m; t->bits = nullptr; t->tab = nullptr;
+    t->lo = lo; t->m = m; t->bits = nullptr; t->tab = nullptr; t->pres = nullptr;
     t->dup = (int*)dalloc(c, 16);
     if(!t->dup) R HPA_E_MEMORY;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, t->dup, 4, 0);
@@ -357,17 +381,40 @@ static int32_t dense_build(hpa_ctx *c, const int32_t *x, L n, int lo, L m, bool 
     hipLaunchKernelGGL(k_dense_pack, dim3(grid_for(nw, 256, 1)), dim3(256), 0, c->stream, flag, m, t->bits, (uint32_t*)nullptr, nw);
     R check_launch("dense_build");
 }
+// guarded first-row table over keys [lo, lo + m); *room false (nothing kept) when the arena
+// cannot hold the table's address range
+static int32_t guard_build(hpa_ctx *c, const int32_t *x, L n, int lo, L m, DenseTab *t, bool *room){
+    *room = false;
+    const L mark = c->dcur, nw = ceil_div(m, 32);
+    t->lo = lo; t->m = m; t->bits = nullptr;
+    t->dup = (int*)dalloc(c, 16);
+    uint32_t *pres = (uint32_t*)dalloc(c, 4 * nw);
+    t->tab = (int32_t*)dalloc(c, 4 * m);
+    if(!t->dup || !pres || !t->tab){ c->dcur = mark; R 0; }
+    *room = true;
+    t->pres = pres;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, t->dup, 4, 0);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nw, 256, 4)), dim3(256), 0, c->stream, pres, nw, 0u);
+    hipLaunchKernelGGL(k_guard_store, dim3(grid_for(n, 256, 1, 1 << 30)), dim3(256), 0, c->stream, x, n, lo, t->tab, pres);
+    hipLaunchKernelGGL(k_dense_first, dim3(grid_for(n, 256, 1, 1 << 30)), dim3(256), 0, c->stream, x, n, lo, t->tab, t->dup);
+    R check_launch("guard_build");
+}
+// row of key y under the table: -1 when absent (a guarded table is read only where its bit is set)
+__device__ __forceinline__ int32_t dense_row(const DenseTab &t, int32_t y){
+    const int64_t d = (int64_t)y - t.lo;
+    if(d < 0 || d >= t.m) return -1;
+    if(t.pres && !((t.pres[d >> 5] >> (d & 31)) & 1u)) return -1;
+    return t.tab[d];
+}
 __global__ void k_dense_probe_index(const int32_t *y, L ny, DenseTab t, int64_t *z, int64_t miss){
     for(L j = (L)blockIdx.x * blockDim.x + threadIdx.x; j < ny; j += (L)gridDim.x * blockDim.x){
-        const int64_t d = (int64_t)y[j] - t.lo;
-        const int32_t r = (d >= 0 && d < t.m) ? t.tab[d] : -1;
+        const int32_t r = dense_row(t, y[j]);
         z[j] = r >= 0 ? (int64_t)r : miss;
     }
 }
 __global__ void k_dense_probe_join(const int32_t *y, L ny, DenseTab t, uint32_t *cnt, uint32_t *rid){
     for(L j = (L)blockIdx.x * blockDim.x + threadIdx.x; j < ny; j += (L)gridDim.x * blockDim.x){
-        const int64_t d = (int64_t)y[j] - t.lo;
-        const int32_t r = (d >= 0 && d < t.m) ? t.tab[d] : -1;
+        const int32_t r = dense_row(t, y[j]);
         cnt[j] = r >= 0 ? 1u : 0u; rid[j] = r >= 0 ? (uint32_t)r : 0u;
     }
 }
@@ -451,11 +498,12 @@ template<typename T>
 static int32_t index_of_hash(hpa_ctx *c, const T *px, L nx, const T *py, L ny, int64_t *z, const int *known = nullptr){
     int32_t e;
     if(std::is_same<T, int32_t>::value && nx < (L)INT32_MAX){
-        int lo, hi; bool dense;
-        if((e = dense_range(c, (const int32_t*)px, nx, &lo, &hi, &dense, known))) R e;
-        if(dense){
-            DenseTab t;
-            if((e = dense_build(c, (const int32_t*)px, nx, lo, (L)hi - lo + 1, true, &t))) R e;
+        int lo, hi; bool dense, guard;
+        if((e = dense_range(c, (const int32_t*)px, nx, &lo, &hi, &dense, known, &guard))) R e;
+        DenseTab t; bool room = false;
+        if(guard && (e = guard_build(c, (const int32_t*)px, nx, lo, (L)hi - lo + 1, &t, &room))) R e;
+        if(dense || room){
+            if(dense && (e = dense_build(c, (const int32_t*)px, nx, lo, (L)hi - lo + 1, true, &t))) R e;
             ProfScope ps("index_of", c->stream);
             if(ny > 0) hipLaunchKernelGGL(k_dense_probe_index, dim3(grid_for(ny, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)py, ny, t, z, (int64_t)nx);
             R check_launch("index_of");
@@ -712,11 +760,15 @@ static int32_t join_run(hpa_ctx *c, V z, V bx, V py, int32_t t, int op, bool swa
         if(op != 4 || t != HPA_I) R HPA_E_NOT_IMPL;  // v3 radix hash join: i32 equality only (h_hash_v3.cc:233,270-277)
         // unique build keys: a dense first-row table or a packed hash table, no sort
         L mark = c->dcur;
-        int lo, hi; bool dense;
-        if((e = dense_range(c, (const int32_t*)pb, nb, &lo, &hi, &dense))) R e;
-        unsigned long long *slots = nullptr; uint64_t mask = 0; DenseTab dt; int *dup;
-        if(dense){ if((e = dense_build(c, (const int32_t*)pb, nb, lo, (L)hi - lo + 1, true, &dt))) R e; dup = dt.dup; }
+        int lo, hi; bool dense, guard;
+        if((e = dense_range(c, (const int32_t*)pb, nb, &lo, &hi, &dense, nullptr, &guard))) R e;
+        unsigned long long *slots = nullptr; uint64_t mask = 0; DenseTab dt; int *dup = nullptr;
+        bool room = false;
+        if(guard && (e = guard_build(c, (const int32_t*)pb, nb, lo, (L)hi - lo + 1, &dt, &room))) R e;
+        if(room) dense = true;                       // probed like a dense table
+        else if(dense){ if((e = dense_build(c, (const int32_t*)pb, nb, lo, (L)hi - lo + 1, true, &dt))) R e; }
         else if((e = pk_build(c, (const int32_t*)pb, nb, &slots, &mask, &dup))) R e;
+        if(dense) dup = dt.dup;
         // Probe, scan and (when the arena has room for np pairs, the most unique build keys can
         // give) the pair writes are queued before the host reads anything: one read then returns
         // the repeated-key flag and the pair count (dup[0], dup[2..3]), and the pair buffers are

@@ -489,6 +489,30 @@ def test_index_of_member(gpu, orc, nx, ny):
                               orc.read(orc.pfnMember(orc.vec(y), orc.vec(s))))
 
 
+@pytest.mark.parametrize("nb,spread", [(5000, 40), (5000, 400), (100003, 40)])
+def test_guarded_tables(gpu, orc, nb, spread):
+    """Build keys spanning 8..512 slots per key take the direct table behind a presence bitmap:
+    index_of (unique and repeated keys) and join (unique build; a repeated build falls to the
+    sort path), probed in ascending order as group keys come and in random order, with keys
+    absent inside the range and keys outside it."""
+    r = rng(nb + spread)
+    lo = -7
+    span = spread * nb
+    u = np.unique(r.integers(1, span - 1, nb + nb // 4))[: nb - 2]
+    keys = np.concatenate([[lo, lo + span - 1], lo + u]).astype(np.int32)
+    r.shuffle(keys)
+    absent = np.setdiff1d(np.arange(lo, lo + span, max(1, span // 5000)), keys)[:3000]
+    probe = np.concatenate([r.choice(keys, 3 * nb), absent, [lo - 1, lo - 5, lo + span, 2**31 - 1]]).astype(np.int32)
+    for y in (np.sort(probe), r.permutation(probe)):
+        for x in (keys, np.concatenate([keys, keys[: nb // 3]])):
+            assert np.array_equal(gpu.read(gpu.pfnIndexOf(gpu.vec(x), gpu.vec(y))),
+                                  orc.read(orc.pfnIndexOf(orc.vec(x), orc.vec(y))))
+            g = gpu.read_list(gpu.pfnJoinIndex(gpu.vec(x), gpu.vec(y), gpu.sym("eq")))
+            o = orc.read_list(orc.pfnJoinIndex(orc.vec(x), orc.vec(y), orc.sym("eq")))
+            assert len(o[0]) > 0
+            assert np.array_equal(g[0], o[0]) and np.array_equal(g[1], o[1])
+
+
 @pytest.mark.parametrize("nb,ny", [(5000, 20000), (60000, 100000)])
 def test_hash_sparse_keys(gpu, orc, nb, ny):
     """Build keys spread over the whole i32 range take the packed hash table (dense key ranges
