@@ -412,10 +412,58 @@ __global__ void k_dense_probe_index(const int32_t *y, L ny, DenseTab t, int64_t 
         z[j] = r >= 0 ? (int64_t)r : miss;
     }
 }
-__global__ void k_dense_probe_join(const int32_t *y, L ny, DenseTab t, uint32_t *cnt, uint32_t *rid){
-    for(L j = (L)blockIdx.x * blockDim.x + threadIdx.x; j < ny; j += (L)gridDim.x * blockDim.x){
-        const int32_t r = dense_row(t, y[j]);
-        cnt[j] = r >= 0 ? 1u : 0u; rid[j] = r >= 0 ? (uint32_t)r : 0u;
+// Join on a direct table with unique build keys: every probe row matches at most once, so the
+// pairs are a compaction of the probe rows.  k_dj_count counts each 4096-row tile's matches;
+// after a scan of the tile counts, k_dj_write probes the tile again (the table and its bitmap
+// are in cache by then) and writes its pairs at their rank in row order: per row of 64 lanes a
+// wave ballot, the wave counts of the tile's 16 rows x 4 waves scanned in (row, wave) order.
+#define DJ_ITEMS 16
+#define DJ_TILE (256 * DJ_ITEMS)
+__global__ void __launch_bounds__(256) k_dj_count(const int32_t *y, L n, DenseTab t, uint32_t *tcount){
+    const L base = (L)blockIdx.x * DJ_TILE;
+    uint32_t c = 0;
+    #pragma unroll
+    for(int k = 0; k < DJ_ITEMS; k++){
+        const L i = base + (L)k * 256 + threadIdx.x;
+        if(i < n && dense_row(t, y[i]) >= 0) c++;
+    }
+    __shared__ uint32_t sh[4];
+    for(int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+    if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if(threadIdx.x == 0) tcount[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+__global__ void __launch_bounds__(256) k_dj_write(const int32_t *y, L n, DenseTab t, const int64_t *toff,
+                                                  int64_t *z0, int64_t *z1, int swap){
+    __shared__ uint32_t s_cnt[DJ_ITEMS * 4];
+    const L base = (L)blockIdx.x * DJ_TILE;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t bal[DJ_ITEMS];
+    int32_t rr[DJ_ITEMS];
+    #pragma unroll
+    for(int k = 0; k < DJ_ITEMS; k++){
+        const L i = base + (L)k * 256 + threadIdx.x;
+        rr[k] = i < n ? dense_row(t, y[i]) : -1;
+        bal[k] = __ballot(rr[k] >= 0);
+        if(lane == 0) s_cnt[k * 4 + w] = (uint32_t)__popcll(bal[k]);
+    }
+    __syncthreads();
+    if(w == 0){
+        const uint32_t mine = s_cnt[lane];
+        uint32_t inc = mine;
+        for(int o = 1; o < 64; o <<= 1){ const uint32_t tt = __shfl_up(inc, o, 64); if(lane >= o) inc += tt; }
+        s_cnt[lane] = inc - mine;
+    }
+    __syncthreads();
+    const int64_t pre = toff[blockIdx.x];
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    #pragma unroll
+    for(int k = 0; k < DJ_ITEMS; k++){
+        if(rr[k] >= 0){
+            const L i = base + (L)k * 256 + threadIdx.x;
+            const int64_t o = pre + s_cnt[k * 4 + w] + __popcll(bal[k] & lt);
+            if(swap){ z0[o] = i; z1[o] = rr[k]; } else { z0[o] = rr[k]; z1[o] = i; }
+        }
     }
 }
 __global__ void k_dense_probe_member(const int32_t *y, L ny, int lo, L m, const uint32_t *bits, uint8_t *z){
@@ -773,18 +821,34 @@ static int32_t join_run(hpa_ctx *c, V z, V bx, V py, int32_t t, int op, bool swa
         // give) the pair writes are queued before the host reads anything: one read then returns
         // the repeated-key flag and the pair count (dup[0], dup[2..3]), and the pair buffers are
         // trimmed.  Repeated keys discard the lot for the sort path below.
-        int64_t *tot = (int64_t*)(dup + 2);
-        if(np > 0){
-            if(dense) hipLaunchKernelGGL(k_dense_probe_join, dim3(grid_for(np, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)pp, np, dt, cnt, rid);
-            else hipLaunchKernelGGL(k_pk_probe, dim3(grid_for(np, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)pp, np, slots, mask, cnt, rid);
+        // A direct table probes each 4096-row tile twice (count, then write at the tile's offset);
+        // the hash table probes once into per-row counts and build rows, scanned per row.
+        int64_t *tot = (int64_t*)(dup + 2);          // zero until a scan writes it
+        const L ntl = ceil_div(np, DJ_TILE);
+        uint32_t *tcount = nullptr; int64_t *toff = nullptr;
+        if(dense){
+            tcount = (uint32_t*)dalloc(c, 4 * (ntl > 0 ? ntl : 1)); toff = (int64_t*)dalloc(c, 8 * (ntl + 1));
+            if(!tcount || !toff) R HPA_E_MEMORY;
+            if(np > 0){
+                hipLaunchKernelGGL(k_dj_count, dim3((unsigned)ntl), dim3(256), 0, c->stream, (const int32_t*)pp, np, dt, tcount);
+                if((e = scan_u32_to_i64(c, tcount, toff, ntl, tot))) R e;
+            }
+        } else {
+            if(np > 0) hipLaunchKernelGGL(k_pk_probe, dim3(grid_for(np, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)pp, np, slots, mask, cnt, rid);
+            if((e = scan_u32_to_i64(c, cnt, off, np, tot))) R e;
         }
-        if((e = scan_u32_to_i64(c, cnt, off, np, tot))) R e;
+        auto write_pairs = [&](int64_t *z0, int64_t *z1){
+            if(np <= 0) return;
+            if(dense) hipLaunchKernelGGL(k_dj_write, dim3((unsigned)ntl), dim3(256), 0, c->stream, (const int32_t*)pp, np, dt,
+                                         (const int64_t*)toff, z0, z1, swap ? 1 : 0);
+            else hipLaunchKernelGGL(k_pk_write, dim3(grid_for(np, 256, 4)), dim3(256), 0, c->stream, np, cnt, rid, off, z0, z1, swap ? 1 : 0);
+        };
         const bool spec = c->dcur + 16 * (np > 0 ? np : 1) + 1024 <= c->dtop;
         int64_t *z0 = nullptr, *z1 = nullptr;
         if(spec){
             z0 = (int64_t*)dalloc(c, 8 * (np > 0 ? np : 1)); z1 = (int64_t*)dalloc(c, 8 * (np > 0 ? np : 1));
             if(!z0 || !z1) R HPA_E_MEMORY;
-            if(np > 0) hipLaunchKernelGGL(k_pk_write, dim3(grid_for(np, 256, 4)), dim3(256), 0, c->stream, np, cnt, rid, off, z0, z1, swap ? 1 : 0);
+            write_pairs(z0, z1);
         }
         int64_t st[2] = {0, 0};
         if((e = d2h(c, st, dup, 16))) R e;
@@ -795,7 +859,7 @@ static int32_t join_run(hpa_ctx *c, V z, V bx, V py, int32_t t, int op, bool swa
             else {
                 z0 = (int64_t*)dalloc(c, 8 * (P > 0 ? P : 1)); z1 = (int64_t*)dalloc(c, 8 * (P > 0 ? P : 1));
                 if(!z0 || !z1) R HPA_E_MEMORY;
-                if(np > 0) hipLaunchKernelGGL(k_pk_write, dim3(grid_for(np, 256, 4)), dim3(256), 0, c->stream, np, cnt, rid, off, z0, z1, swap ? 1 : 0);
+                write_pairs(z0, z1);
             }
             if((e = check_launch("join"))) R e;
             V cells = (V)halloc(c, 2 * sizeof(V0)); if(!cells) R HPA_E_MEMORY;
