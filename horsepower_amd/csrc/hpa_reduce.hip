@@ -506,7 +506,8 @@ static int32_t seg_reduce(hpa_ctx *c, V x, const void *px, const int64_t *idx, c
     if(!dst) R HPA_E_MEMORY;
     int32_t e;
     int *bad = nullptr;
-    if(idx){ bad = (int*)dalloc(c, 16); if(!bad) R HPA_E_MEMORY; hipMemsetAsync(bad, 0, 4, c->stream); }
+    // row lists made here (a group's flat list) carry a bound on their rows: no check to read back
+    if(idx && !within_bound(c, idx, x->len)){ bad = (int*)dalloc(c, 16); if(!bad) R HPA_E_MEMORY; hipMemsetAsync(bad, 0, 4, c->stream); }
     L nx = x->len;
     switch(t){
         case HPA_B: e = seg_reduce_t<uint8_t,false>(c, (const uint8_t*)px, idx, off, g, n, kind, dst, nx, bad); break;

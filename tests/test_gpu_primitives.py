@@ -584,6 +584,19 @@ def test_grouped_aggregates(gpu, orc, n, card):
         assert np.allclose(fp, res["g"]["v" + fn], rtol=RTOL, atol=0)
 
 
+def test_group_reduce_row_bounds(gpu):
+    """A group's row lists are known to index the grouped column; reducing a shorter column
+    through them still checks every row and fails with HPA_E_INDEX (code 3)."""
+    k = (np.arange(100_000) % 7).astype(np.int32)
+    vals = gpu.pfnValues(gpu.pfnGroup(gpu.vec(k)))
+    v = np.round(rng(31).uniform(0, 100, k.size), 2)
+    full = gpu.read(gpu.group_reduce(0, gpu.vec(v), vals))
+    assert np.allclose(full, [math.fsum(v[k == g]) for g in range(7)], rtol=1e-12, atol=0)
+    with pytest.raises(HorseError) as ei:
+        gpu.group_reduce(0, gpu.vec(v[: k.size // 2]), vals)
+    assert ei.value.code == 3
+
+
 def test_arena_rewind_and_contexts(gpu):
     """Heap offset rewinds the arena; a context whose arena is too small returns HPA_E_MEMORY
     (the reference exits on a full heap) and leaves the default context usable."""
