@@ -52,20 +52,30 @@ __global__ void __launch_bounds__(CP_BLOCK) k_cp_count(Pred pred, L n, uint32_t 
     __syncthreads();
     if(threadIdx.x == 0) tcount[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
-// the same for a byte mask: 16 consecutive mask bytes per thread, one 16-byte load
-__global__ void __launch_bounds__(CP_BLOCK) k_cp_count_mask(const uint8_t *m, L n, uint32_t *tcount){
-    __shared__ uint32_t sh[CP_BLOCK / 64];
-    const L i0 = (L)blockIdx.x * CP_TILE + (L)threadIdx.x * 16;
-    uint32_t c = 0;
+// bit k of the result: row i0 + k of the 16 consecutive rows at i0 is selected
+__device__ __forceinline__ uint32_t mask16(const uint8_t *mask, L i0, L n){
+    uint32_t m = 0;
     if(i0 + 16 <= n){
-        const uint4 v = *(const uint4*)(m + i0);
-        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+        const uint4 v = *(const uint4*)(mask + i0);
+        const uint32_t x[4] = {v.x, v.y, v.z, v.w};
         #pragma unroll
         for(int q = 0; q < 4; q++)
-            c += ((w[q] & 0xFFu) != 0) + ((w[q] & 0xFF00u) != 0) + ((w[q] & 0xFF0000u) != 0) + ((w[q] & 0xFF000000u) != 0);
+            #pragma unroll
+            for(int b = 0; b < 4; b++) if((x[q] >> (8 * b)) & 0xFFu) m |= 1u << (4 * q + b);
     } else {
-        for(L i = i0; i < n && i < i0 + 16; i++) c += m[i] != 0;
+        for(int k = 0; k < 16 && i0 + k < n; k++) if(mask[i0 + k]) m |= 1u << k;
     }
+    return m;
+}
+// the same for a byte mask: 16 consecutive mask bytes per thread, one 16-byte load.  With `mb`
+// the thread's 16 selection bits are kept (mb[tile * 256 + thread]), so the write pass reads
+// n / 8 bytes instead of the n-byte mask again.
+__global__ void __launch_bounds__(CP_BLOCK) k_cp_count_mask(const uint8_t *m, L n, uint32_t *tcount, uint16_t *mb){
+    __shared__ uint32_t sh[CP_BLOCK / 64];
+    const L i0 = (L)blockIdx.x * CP_TILE + (L)threadIdx.x * 16;
+    const uint32_t bits = mask16(m, i0, n);
+    if(mb) mb[(L)blockIdx.x * CP_BLOCK + threadIdx.x] = (uint16_t)bits;
+    uint32_t c = (uint32_t)__popc(bits);
     for(int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
     if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
     __syncthreads();
@@ -138,23 +148,14 @@ __device__ __forceinline__ void cp_flush(const uint64_t *buf, char *dst, int64_t
     for(uint32_t j = threadIdx.x; j < cnt; j += CP_BLOCK) ((E*)dst)[pre + j] = lb[j];
 }
 
-__global__ void __launch_bounds__(CP_BLOCK) k_cp_write_mask(const uint8_t *mask, L n, CpCols cols, int64_t *idx_out, const int64_t *toff){
+__global__ void __launch_bounds__(CP_BLOCK) k_cp_write_mask(const uint8_t *mask, const uint16_t *mb, L n, CpCols cols,
+                                                          int64_t *idx_out, const int64_t *toff){
     __shared__ uint64_t buf[CP_TILE];
     __shared__ uint32_t sw[CP_BLOCK / 64];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const L i0 = (L)blockIdx.x * CP_TILE + (L)t * 16;
-    uint32_t m = 0;
     const bool full = i0 + 16 <= n;
-    if(full){
-        const uint4 v = *(const uint4*)(mask + i0);
-        const uint32_t x[4] = {v.x, v.y, v.z, v.w};
-        #pragma unroll
-        for(int q = 0; q < 4; q++)
-            #pragma unroll
-            for(int b = 0; b < 4; b++) if((x[q] >> (8 * b)) & 0xFFu) m |= 1u << (4 * q + b);
-    } else {
-        for(int k = 0; k < 16 && i0 + k < n; k++) if(mask[i0 + k]) m |= 1u << k;
-    }
+    const uint32_t m = mb ? (uint32_t)mb[(L)blockIdx.x * CP_BLOCK + t] : mask16(mask, i0, n);
     // exclusive scan of the per-thread counts in thread (= row) order
     const uint32_t c = (uint32_t)__popc(m);
     uint32_t inc = c;
@@ -194,17 +195,23 @@ __global__ void __launch_bounds__(CP_BLOCK) k_cp_write_mask(const uint8_t *mask,
 template<typename Pred>
 // Per-tile counts and their scan: toff[t] = rows selected before tile t, toff[ntiles] = total.
 // With k_host the total is also read to the host (a synchronisation); without, only *toff_out.
-static int32_t compress_count(hpa_ctx *c, Pred pred, const uint8_t *mask, L n, int64_t **toff_out, int64_t *k_host){
+// With mb_out, the byte-mask count also keeps the selection bits for the write pass (*mb_out;
+// null when the mask is not 16-byte aligned).
+static int32_t compress_count(hpa_ctx *c, Pred pred, const uint8_t *mask, L n, int64_t **toff_out, int64_t *k_host,
+                              uint16_t **mb_out = nullptr){
     L ntiles = ceil_div(n, CP_TILE);
     if(k_host) *k_host = 0;
     *toff_out = nullptr;
+    if(mb_out) *mb_out = nullptr;
     if(ntiles < 1) R 0;
     uint32_t *tcount = (uint32_t*)dalloc(c, 4 * ntiles);
     int64_t *toff = (int64_t*)dalloc(c, 8 * (ntiles + 1));
     if(!tcount || !toff) R HPA_E_MEMORY;
-    if(mask && ((uintptr_t)mask & 15) == 0)
-        hipLaunchKernelGGL(k_cp_count_mask, dim3((unsigned)ntiles), dim3(CP_BLOCK), 0, c->stream, mask, n, tcount);
-    else
+    if(mask && ((uintptr_t)mask & 15) == 0){
+        uint16_t *mb = nullptr;
+        if(mb_out){ mb = (uint16_t*)dalloc(c, 2 * CP_BLOCK * ntiles); if(!mb) R HPA_E_MEMORY; *mb_out = mb; }
+        hipLaunchKernelGGL(k_cp_count_mask, dim3((unsigned)ntiles), dim3(CP_BLOCK), 0, c->stream, mask, n, tcount, mb);
+    } else
         hipLaunchKernelGGL(k_cp_count<Pred>, dim3((unsigned)ntiles), dim3(CP_BLOCK), 0, c->stream, pred, n, tcount);
     int32_t e;
     if((e = check_launch("compress_count"))) R e;
@@ -220,13 +227,15 @@ static int32_t compress_write(hpa_ctx *c, Pred pred, L n, CpCols cols, int64_t *
     R check_launch("compress_write");
 }
 // the byte-mask form: 16 consecutive rows per thread when every operand is 16-element aligned
-static int32_t compress_write(hpa_ctx *c, MaskPred pred, L n, CpCols cols, int64_t *idx_out, const int64_t *toff){
+// (mb: the selection bits the count pass kept, or null)
+static int32_t compress_write(hpa_ctx *c, MaskPred pred, L n, CpCols cols, int64_t *idx_out, const int64_t *toff,
+                              const uint16_t *mb = nullptr){
     L ntiles = ceil_div(n, CP_TILE);
     if(ntiles < 1) R 0;
     bool aligned = ((uintptr_t)pred.m & 15) == 0;
     for(int k = 0; k < cols.n; k++)
         aligned = aligned && ((uintptr_t)cols.src[k] % (uintptr_t)(16 * cols.w[k])) == 0 && cols.w[k] <= 8;
-    if(aligned) hipLaunchKernelGGL(k_cp_write_mask, dim3((unsigned)ntiles), dim3(CP_BLOCK), 0, c->stream, pred.m, n, cols, idx_out, toff);
+    if(aligned) hipLaunchKernelGGL(k_cp_write_mask, dim3((unsigned)ntiles), dim3(CP_BLOCK), 0, c->stream, pred.m, mb, n, cols, idx_out, toff);
     else hipLaunchKernelGGL(k_cp_write<MaskPred>, dim3((unsigned)ntiles), dim3(CP_BLOCK), 0, c->stream, pred, n, cols, idx_out, toff);
     R check_launch("compress_write");
 }
@@ -262,9 +271,10 @@ static int32_t compress_many(hpa_ctx *c, int ny, V *z, V x, V *y, bool expr_mode
     // behind the count pass and the host reads the count while it runs; the arena top is then
     // pulled back to the end of the last output's selected rows.
     L maxb = 0; for(int k = 0; k < ny; k++) maxb += (type_bytes(y[k]->typ, n > 0 ? n : 1) + 255) & ~(L)255;
-    const bool spec = n > 0 && c->dcur + maxb + 4096 + 16 * ceil_div(n, CP_TILE) <= c->dtop;
+    const bool spec = n > 0 && c->dcur + maxb + 4096 + (16 + 2 * CP_BLOCK) * ceil_div(n, CP_TILE) <= c->dtop;
+    uint16_t *mb = nullptr;
     e = expr_mode ? compress_count(c, *ep, nullptr, n, &toff, spec ? nullptr : &kk)
-                  : compress_count(c, MaskPred{m}, m, n, &toff, spec ? nullptr : &kk);
+                  : compress_count(c, MaskPred{m}, m, n, &toff, spec ? nullptr : &kk, &mb);
     if(e){ c->dcur = mark; R e; }
     if(!spec && kk == n && n > 0 && !expr_mode){           // copyV alias (h_primitive.cc:1513-1515)
         ps.stop(); ps.bytes = (double)n;
@@ -276,7 +286,7 @@ static int32_t compress_many(hpa_ctx *c, int ny, V *z, V x, V *y, bool expr_mode
         cols.dst[k] = (char*)dalloc(c, type_bytes(y[k]->typ, spec ? n : (kk > 0 ? kk : 1)));
         if(!cols.dst[k]){ c->dcur = mark; R HPA_E_MEMORY; }
     }
-    e = expr_mode ? compress_write(c, *ep, n, cols, nullptr, toff) : compress_write(c, MaskPred{m}, n, cols, nullptr, toff);
+    e = expr_mode ? compress_write(c, *ep, n, cols, nullptr, toff) : compress_write(c, MaskPred{m}, n, cols, nullptr, toff, mb);
     if(e) R e;
     if(spec){
         if((e = d2h(c, &kk, toff + ceil_div(n, CP_TILE), 8))) R e;
@@ -330,11 +340,12 @@ extern "C" int32_t pfnWhere(hpa_V z, hpa_V x){
     ProfScope ps("where", c->stream);
     int64_t kk = 0, *toff = nullptr;
     // as in compress: with room for n indices the write pass runs while the host reads the count
-    const bool spec = n > 0 && c->dcur + 8 * n + 4096 + 16 * ceil_div(n, CP_TILE) <= c->dtop;
-    if((e = compress_count(c, MaskPred{m}, m, n, &toff, spec ? nullptr : &kk))){ c->dcur = mark; R e; }
+    const bool spec = n > 0 && c->dcur + 8 * n + 4096 + (16 + 2 * CP_BLOCK) * ceil_div(n, CP_TILE) <= c->dtop;
+    uint16_t *mb = nullptr;
+    if((e = compress_count(c, MaskPred{m}, m, n, &toff, spec ? nullptr : &kk, &mb))){ c->dcur = mark; R e; }
     int64_t *out = (int64_t*)dalloc(c, 8 * (spec ? n : (kk > 0 ? kk : 1))); if(!out){ c->dcur = mark; R HPA_E_MEMORY; }
     CpCols cols; cols.n = 0;
-    if((e = compress_write(c, MaskPred{m}, n, cols, out, toff))) R e;
+    if((e = compress_write(c, MaskPred{m}, n, cols, out, toff, mb))) R e;
     if(spec){
         if((e = d2h(c, &kk, toff + ceil_div(n, CP_TILE), 8))) R e;
         c->dcur = (L)((char*)out - c->dbase) + ((8 * (kk > 0 ? kk : 1) + 255) & ~(L)255);
