@@ -816,7 +816,8 @@ static int32_t join_run(hpa_ctx *c, V z, V bx, V py, int32_t t, int op, bool swa
     ProfScope ps("join", c->stream);
     if(nb >= 10){                                    // NUM_JOIN_LINEAR (common.h:152): hash path
         if(op != 4 || t != HPA_I) R HPA_E_NOT_IMPL;  // v3 radix hash join: i32 equality only (h_hash_v3.cc:233,270-277)
-        // unique build keys: a dense first-row table or a packed hash table, no sort
+        // unique build keys: a direct first-row table (cleared, or guarded by a presence bitmap)
+        // or a packed hash table, no sort
         L mark = c->dcur;
         int lo, hi; bool dense, guard;
         if((e = dense_range(c, (const int32_t*)pb, nb, &lo, &hi, &dense, nullptr, &guard))) R e;
