@@ -38,11 +38,16 @@ struct hpa_ctx {
     // allocated before the keys, so a rewind that frees either frees the keys)
     struct GroupVals { const void *src; L n; const void *vals; };
     std::map<const void*, GroupVals> gvals;
+    // group counts vectors made here, with the g + 1 group offsets a flat list over them needs
+    // (allocated before the counts, so a rewind that frees them frees the counts too)
+    struct GroupOffs { L g; const int64_t *offs; };
+    std::map<const void*, GroupOffs> goffs;
 };
 void note_bound(hpa_ctx *c, const void *p, L bound);
 bool within_bound(hpa_ctx *c, const void *p, L limit);
 void note_group_vals(hpa_ctx *c, const void *keys, const void *src, L n, const void *vals);
 const void *group_vals(hpa_ctx *c, const void *keys, const void *src, L n);
+void note_group_offsets(hpa_ctx *c, const void *counts, L g, const int64_t *offs);
 int32_t resident_alloc(V x, int32_t typ, L len);   // long-lived HBM payload (hpaFreeResident)
 
 hpa_ctx *ctx_cur();

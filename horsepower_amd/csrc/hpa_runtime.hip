@@ -165,7 +165,7 @@ extern "C" int32_t hpaCtxSetArena(hpa_ctx *c, void *base, int64_t bytes){
     if(base && !(is_device(base) && is_device((char*)base + bytes - 1))){ set_error_detail("hpaCtxSetArena: not a device pool range"); R HPA_E_DOMAIN; }
     if(hipStreamSynchronize(c->stream) != hipSuccess) R HPA_E_DEVICE;
     c->dbase = (char*)base; c->dtop = base ? bytes : 0; c->dcur = 0; c->dpeak = 0;
-    c->hcur = 0; c->marks.clear(); c->bounds.clear(); c->gvals.clear();
+    c->hcur = 0; c->marks.clear(); c->bounds.clear(); c->gvals.clear(); c->goffs.clear();
     for(void *p : c->host_bufs) free(p);
     c->host_bufs.clear();
     R 0;
@@ -223,6 +223,7 @@ extern "C" int64_t hpaSetHeapOffset(int64_t cur){
     c->host_bufs.clear();
     c->bounds.erase(c->bounds.lower_bound((const void*)(c->dbase + cur)), c->bounds.end());
     c->gvals.erase(c->gvals.lower_bound((const void*)(c->dbase + cur)), c->gvals.end());
+    c->goffs.erase(c->goffs.lower_bound((const void*)(c->dbase + cur)), c->goffs.end());
     R c->dcur = cur;
 }
 
@@ -240,6 +241,13 @@ void note_group_vals(hpa_ctx *c, const void *keys, const void *src, L n, const v
 const void *group_vals(hpa_ctx *c, const void *keys, const void *src, L n){
     auto it = c->gvals.find(keys);
     R it != c->gvals.end() && it->second.src == src && it->second.n == n ? it->second.vals : nullptr;
+}
+void note_group_offsets(hpa_ctx *c, const void *counts, L g, const int64_t *offs){
+    if(g > 0 && in_arena(c, counts) && in_arena(c, offs) && (const void*)offs < counts) c->goffs[counts] = {g, offs};
+}
+static const int64_t *group_offsets(hpa_ctx *c, const void *counts, L g){
+    auto it = c->goffs.find(counts);
+    R it != c->goffs.end() && it->second.g == g ? it->second.offs : nullptr;
 }
 
 // ---------------------------------------------------------------- value model
@@ -743,6 +751,8 @@ int32_t flat_offsets(hpa_ctx *c, V y, const int64_t **offs, const int64_t **idx,
     // the inner vector is the i64 row list of a group result, or the gathered values of
     // each_right(@index) (any element type); only the former is returned as an index
     if(t->typ == HPA_L){ if((e = dev_view(c, t, &pi))) R e; }
+    // a group made here kept its offsets: no scan of the counts
+    if(const int64_t *go = group_offsets(c, pc, *g)){ *offs = go; *idx = (const int64_t*)pi; R 0; }
     int64_t *o = (int64_t*)dalloc(c, 8 * (*g + 1));
     if(!o) R HPA_E_MEMORY;
     if((e = scan_i64(c, (const int64_t*)pc, o, *g, o + *g))) R e;

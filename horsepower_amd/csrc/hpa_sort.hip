@@ -425,9 +425,13 @@ __global__ void __launch_bounds__(256) k_run_write(const K *sk, const uint32_t *
         }
     }
 }
-__global__ void k_run_counts(const int64_t *start, L g, L n, int64_t *cnt){
-    for(L r = (L)blockIdx.x * blockDim.x + threadIdx.x; r < g; r += (L)gridDim.x * blockDim.x)
+// cnt[r] = start[r + 1] - start[r] with start[g] = n, which is written too (start holds g + 1
+// entries: the groups' offsets in the flat list)
+__global__ void k_run_counts(int64_t *start, L g, L n, int64_t *cnt){
+    for(L r = (L)blockIdx.x * blockDim.x + threadIdx.x; r < g; r += (L)gridDim.x * blockDim.x){
         cnt[r] = (r + 1 < g ? start[r + 1] : n) - start[r];
+        if(r == 0) start[g] = n;
+    }
 }
 
 int32_t make_group_dict(hpa_ctx *c, V z, int64_t *keys, int64_t *counts, int64_t *flat, L g, L n){
@@ -464,7 +468,7 @@ static int32_t runs_to_group_t(hpa_ctx *c, V z, const K *sk, const uint32_t *sid
     }
     uint32_t *kval = src && g > 1 ? (uint32_t*)dalloc(c, 4 * g) : nullptr;       // before keys: dies with them
     int64_t *keys = (int64_t*)dalloc(c, 8 * (g > 0 ? g : 1));
-    int64_t *start = (int64_t*)dalloc(c, 8 * (g > 0 ? g : 1));
+    int64_t *start = (int64_t*)dalloc(c, 8 * (g + 1));
     int64_t *cnt = (int64_t*)dalloc(c, 8 * (g > 0 ? g : 1));
     int64_t *flat = (int64_t*)dalloc(c, 8 * (n > 0 ? n : 1));
     if(!keys || !start || !cnt || !flat) R HPA_E_MEMORY;
@@ -474,6 +478,7 @@ static int32_t runs_to_group_t(hpa_ctx *c, V z, const K *sk, const uint32_t *sid
     }
     if((e = check_launch("group_runs"))) R e;
     if(kval) note_group_vals(c, keys, src, n, kval);
+    note_group_offsets(c, cnt, g, start);
     R make_group_dict(c, z, keys, cnt, flat, g, n);
 }
 int32_t runs_to_group_u64(hpa_ctx *c, V z, const uint64_t *sk, const uint32_t *sidx, L n, const uint64_t *eq){
@@ -1380,14 +1385,15 @@ static int32_t group_i32_runs(hpa_ctx *c, V z, const int32_t *x, L n, bool *done
     if((e = scan_u32_to_i64(c, gcount, goff, gt, goff + gt))) R e;
     int64_t G = 0; if((e = d2h(c, &G, goff + gt, 8))) R e;
     uint32_t *kval = (uint32_t*)dalloc(c, 4 * G);           // before keys: dies with them
-    int64_t *keys = (int64_t*)dalloc(c, 8 * G), *start = (int64_t*)dalloc(c, 8 * G), *cnt = (int64_t*)dalloc(c, 8 * G);
+    int64_t *keys = (int64_t*)dalloc(c, 8 * G), *start = (int64_t*)dalloc(c, 8 * (G + 1)), *cnt = (int64_t*)dalloc(c, 8 * G);
     if(!kval || !keys || !start || !cnt) R HPA_E_MEMORY;
     hipLaunchKernelGGL(k_gr_group_write, dim3((unsigned)gt), dim3(256), 0, c->stream, (const uint32_t*)sk, (const uint32_t*)sv, cb,
                        (const int64_t*)fpos, (L)nr, (const int64_t*)goff, keys, start, kval);
-    hipLaunchKernelGGL(k_run_counts, dim3(grid_for(G, 256, 1)), dim3(256), 0, c->stream, (const int64_t*)start, (L)G, n, cnt);
+    hipLaunchKernelGGL(k_run_counts, dim3(grid_for(G, 256, 1)), dim3(256), 0, c->stream, start, (L)G, n, cnt);
     if((e = check_launch("group_runs"))) R e;
     *done = true;
     if(G > 1) note_group_vals(c, keys, x, n, kval);
+    note_group_offsets(c, cnt, G, start);
     R make_group_dict(c, z, keys, cnt, flat, G, n);
 }
 
@@ -1490,7 +1496,7 @@ static int32_t runs_along_perm(hpa_ctx *c, V z, int nc, V cols, const uint32_t *
     if((e = scan_u32_to_i64(c, flag, pos, n, pos + n))) R e;
     int64_t g = 0; if((e = d2h(c, &g, pos + n, 8))) R e;
     int64_t *keys = (int64_t*)dalloc(c, 8 * (g > 0 ? g : 1));
-    int64_t *start = (int64_t*)dalloc(c, 8 * (g > 0 ? g : 1));
+    int64_t *start = (int64_t*)dalloc(c, 8 * (g + 1));
     int64_t *cnt = (int64_t*)dalloc(c, 8 * (g > 0 ? g : 1));
     int64_t *flat = (int64_t*)dalloc(c, 8 * (n > 0 ? n : 1));
     if(!keys || !start || !cnt || !flat) R HPA_E_MEMORY;
@@ -1499,6 +1505,7 @@ static int32_t runs_along_perm(hpa_ctx *c, V z, int nc, V cols, const uint32_t *
         hipLaunchKernelGGL(k_run_counts, dim3(grid_for(g, 256, 1)), dim3(256), 0, c->stream, start, g, n, cnt);
     }
     if((e = check_launch("group_runs"))) R e;
+    if(n > 0) note_group_offsets(c, cnt, g, start);
     R make_group_dict(c, z, keys, cnt, flat, g, n);
 }
 
