@@ -597,6 +597,24 @@ def test_group_reduce_row_bounds(gpu):
     assert ei.value.code == 3
 
 
+def test_group_offsets_after_rewind(gpu):
+    """A group keeps its flat-list offsets for the reductions over it; after the arena is rewound
+    a new group with as many groups but other sizes lands at the same addresses and must be
+    reduced over its own offsets."""
+    r = rng(41)
+    keys = (np.arange(300) * 1000 - 7).astype(np.int32)       # a range over 256: the radix path
+    m = gpu.mark()
+    s1 = r.integers(1, 60, 300)
+    for sizes in (s1, r.permutation(s1)):                   # same row count: same addresses
+        k = r.permutation(np.repeat(keys, sizes)).astype(np.int32)
+        v = np.round(r.uniform(0, 100, k.size), 2)
+        vals = gpu.pfnValues(gpu.pfnGroup(gpu.vec(k)))
+        got = gpu.read(gpu.group_reduce(0, gpu.vec(v), vals))
+        want = [math.fsum(v[k == key]) for key in np.sort(keys.view(np.uint32)).view(np.int32)]
+        assert np.allclose(got, want, rtol=1e-12, atol=0)
+        gpu.release(m)
+
+
 def test_arena_rewind_and_contexts(gpu):
     """Heap offset rewinds the arena; a context whose arena is too small returns HPA_E_MEMORY
     (the reference exits on a full heap) and leaves the default context usable."""
