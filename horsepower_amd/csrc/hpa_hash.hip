@@ -333,7 +333,9 @@ __global__ void k_dense_store(const int32_t *x, L n, int lo, int32_t *tab){
     for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x)
         tab[(uint32_t)(x[i] - lo)] = (int32_t)i;
 }
-__global__ void k_dense_first(const int32_t *x, L n, int lo, int32_t *tab, int *dup){
+// gated: run only when *dup is raised already (a guarded build counted fewer keys than rows)
+__global__ void k_dense_first(const int32_t *x, L n, int lo, int32_t *tab, int *dup, int gated){
+    if(gated && !*(volatile int*)dup) return;
     int d = 0;
     for(L i0 = (L)blockIdx.x * blockDim.x; i0 < n; i0 += (L)gridDim.x * blockDim.x){
         const L i = i0 + threadIdx.x;
@@ -350,6 +352,27 @@ __global__ void k_dense_first(const int32_t *x, L n, int lo, int32_t *tab, int *
 struct DenseTab { int lo; L m; uint32_t *bits; int32_t *tab; int *dup; const uint32_t *pres; };
 __global__ void k_fill_u32(uint32_t *p, L n, uint32_t v){
     for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x) p[i] = v;
+}
+// set bits of the presence bitmap, one partial per workgroup (no atomics on one address)
+#define POPC_PARTS 1024
+__global__ void __launch_bounds__(256) k_popc_part(const uint32_t *bits, L nw, unsigned long long *part){
+    __shared__ unsigned long long sh[4];
+    unsigned long long s = 0;
+    for(L w = (L)blockIdx.x * blockDim.x + threadIdx.x; w < nw; w += (L)gridDim.x * blockDim.x) s += (unsigned)__popc(bits[w]);
+    for(int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if(threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+// fewer distinct keys (set bits) than build rows: some key repeats
+__global__ void __launch_bounds__(256) k_dup_if_fewer(const unsigned long long *part, int np, L n, int *dup){
+    __shared__ unsigned long long sh[4];
+    unsigned long long s = 0;
+    for(int b = threadIdx.x; b < np; b += 256) s += part[b];
+    for(int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if(threadIdx.x == 0 && sh[0] + sh[1] + sh[2] + sh[3] != (unsigned long long)n) *dup = 1;
 }
 // guarded table: every build row stores its index and sets its key's presence bit
 __global__ void k_guard_store(const int32_t *x, L n, int lo, int32_t *tab, uint32_t *pres){
@@ -369,7 +392,7 @@ static int32_t dense_build(hpa_ctx *c, const int32_t *x, L n, int lo, L m, bool 
         t->tab = (int32_t*)dalloc(c, 4 * m); if(!t->tab) R HPA_E_MEMORY;
         hipLaunchKernelGGL(k_dense_clear, dim3(grid_for(m, 256, 4)), dim3(256), 0, c->stream, t->tab, m);
         hipLaunchKernelGGL(k_dense_store, dim3(grid_for(n, 256, 1, 1 << 30)), dim3(256), 0, c->stream, x, n, lo, t->tab);
-        hipLaunchKernelGGL(k_dense_first, dim3(grid_for(n, 256, 1, 1 << 30)), dim3(256), 0, c->stream, x, n, lo, t->tab, t->dup);
+        hipLaunchKernelGGL(k_dense_first, dim3(grid_for(n, 256, 1, 1 << 30)), dim3(256), 0, c->stream, x, n, lo, t->tab, t->dup, 0);
         R check_launch("dense_build");
     }
     const L nw = ceil_div(m, 32);
@@ -388,15 +411,21 @@ static int32_t guard_build(hpa_ctx *c, const int32_t *x, L n, int lo, L m, Dense
     const L mark = c->dcur, nw = ceil_div(m, 32);
     t->lo = lo; t->m = m; t->bits = nullptr;
     t->dup = (int*)dalloc(c, 16);
+    unsigned long long *part = (unsigned long long*)dalloc(c, 8 * POPC_PARTS);
     uint32_t *pres = (uint32_t*)dalloc(c, 4 * nw);
     t->tab = (int32_t*)dalloc(c, 4 * m);
-    if(!t->dup || !pres || !t->tab){ c->dcur = mark; R 0; }
+    if(!t->dup || !part || !pres || !t->tab){ c->dcur = mark; R 0; }
     *room = true;
     t->pres = pres;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, t->dup, 4, 0);
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nw, 256, 4)), dim3(256), 0, c->stream, pres, nw, 0u);
     hipLaunchKernelGGL(k_guard_store, dim3(grid_for(n, 256, 1, 1 << 30)), dim3(256), 0, c->stream, x, n, lo, t->tab, pres);
-    hipLaunchKernelGGL(k_dense_first, dim3(grid_for(n, 256, 1, 1 << 30)), dim3(256), 0, c->stream, x, n, lo, t->tab, t->dup);
+    // keys are unique exactly when the bitmap holds as many bits as there are rows; only then is
+    // the first-row pass skipped
+    const int np = grid_for(nw, 256, 8, POPC_PARTS);
+    hipLaunchKernelGGL(k_popc_part, dim3(np), dim3(256), 0, c->stream, (const uint32_t*)pres, nw, part);
+    hipLaunchKernelGGL(k_dup_if_fewer, dim3(1), dim3(256), 0, c->stream, (const unsigned long long*)part, np, n, t->dup);
+    hipLaunchKernelGGL(k_dense_first, dim3(grid_for(n, 256, 1, 1 << 30)), dim3(256), 0, c->stream, x, n, lo, t->tab, t->dup, 1);
     R check_launch("guard_build");
 }
 // row of key y under the table: -1 when absent (a guarded table is read only where its bit is set)
