@@ -920,10 +920,13 @@ static int32_t fga_run(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_
     const int NA = acc.empty() ? 1 : (int)acc.size();
     o.stride = NA;
     const size_t psize = 8 + 8 + 16 * NA + 8 * NA + 4;     // struct P, and the slot's code
-    // two workgroups per CU: measured on MI355X (Q1, 60M rows) against 1, 3, 4, 6, 8 and 12,
-    // it streams fastest, and its partials keep the merge short
+    // Workgroups per CU, measured on MI355X (Q1): with one workgroup merging every partial, two
+    // per CU was fastest of 1-12 at 60M rows.  With the sliced merge, three per CU take the 60M
+    // query from 0.409-0.414 to 0.399-0.400 ms end to end (the kernel itself is no faster), and
+    // four are slower (0.420); at 6M rows two stay best.  So three from 2^25 rows, two below.
+    const int per_cu = n >= (1ll << 25) ? 3 : 2;
     int G = (int)ceil_div(n, 256 * FGA_ROWS * 8);
-    if(G > num_cus() * 2) G = num_cus() * 2;
+    if(G > num_cus() * per_cu) G = num_cus() * per_cu;
     if(G < 1) G = 1;
     const void *pk0 = nullptr, *pk1 = nullptr;
     if(n > 0){ if((e = dev_view(c, keys[0], &pk0))) R e; if(nk == 2 && (e = dev_view(c, keys[1], &pk1))) R e; }

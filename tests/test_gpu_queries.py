@@ -66,6 +66,24 @@ def test_q1_one_pass(gpu, orc, n):
         close(g[k], o[k], k)
 
 
+def test_q1_one_pass_large(gpu):
+    """34M rows, where the one-pass kernel runs three workgroups per CU.  The oracle's plan is
+    sized for a few million rows (at 34M it stops with a segmentation fault), so the check is
+    the GPU's interpreter sequence (group + per-group sums, checked against the oracle at 1M
+    rows in test_q1) and exact per-group row counts from numpy."""
+    n = 34_000_001
+    t = D.gen_q1(n)
+    T = Q.load_table(gpu, t)
+    o = Q.q1_plain(gpu, T)
+    g = Q.q1_chf(gpu, T)
+    for k in Q.Q1_NAMES:
+        close(g[k], o[k], k)
+    sel = t["ship_day"] <= D.Q1_SHIP_MAX
+    code = t["return_flag"][sel].astype(np.int64) * 256 + t["line_status"][sel]
+    _, counts = np.unique(code, return_counts=True)
+    assert np.asarray(g["count_order"]).tolist() == counts.tolist()
+
+
 def test_fused_group_agg_overflow_and_single_key(gpu, orc):
     from horsepower_amd.vnode import H_C, expr
     r = np.random.default_rng(5)
