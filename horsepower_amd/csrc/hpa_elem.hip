@@ -148,7 +148,7 @@ extern "C" int32_t pfnCompare(hpa_V z, hpa_V x, hpa_V y, int32_t op){
         default: e = HPA_E_NOT_IMPL;
     }
     if(e) R e;
-    if(is_inline(z)){ e = commit_inline(c, z, dz); c->dcur = mark; }
+    if(is_inline(z)){ e = commit_inline(c, z, dz); arena_set(c, mark); }
     R e;
 }
 extern "C" int32_t pfnLt (hpa_V z, hpa_V x, hpa_V y){ R pfnCompare(z,x,y,0); }
@@ -254,7 +254,7 @@ extern "C" int32_t pfnArith(hpa_V z, hpa_V x, hpa_V y, int32_t op){
     if(!(x->len == 1 || y->len == 1 || x->len == y->len)) R HPA_E_LENGTH;
     if(op < 0 || op > 3) R HPA_E_DOMAIN;
     L mark0 = c->dcur;
-    if(op == 3){ bool zf; int32_t e = check_zero(c, y, &zf); if(e) R e; c->dcur = mark0; if(zf) R HPA_E_DIV_ZERO; }
+    if(op == 3){ bool zf; int32_t e = check_zero(c, y, &zf); if(e) R e; arena_set(c, mark0); if(zf) R HPA_E_DIV_ZERO; }
     L n = x->len == 1 ? y->len : x->len;
     int32_t t = x->typ > y->typ ? x->typ : y->typ;
     void *dz; int32_t e;
@@ -276,7 +276,7 @@ extern "C" int32_t pfnArith(hpa_V z, hpa_V x, hpa_V y, int32_t op){
         case HPA_E: e = arith_flt<double>(c, x, y, t, (double*)dz, n, op); break;
     }
     if(e) R e;
-    if(is_inline(z)){ e = commit_inline(c, z, dz); c->dcur = mark; }
+    if(is_inline(z)){ e = commit_inline(c, z, dz); arena_set(c, mark); }
     R e;
 }
 extern "C" int32_t pfnPlus (hpa_V z, hpa_V x, hpa_V y){ R pfnArith(z,x,y,0); }
@@ -311,7 +311,7 @@ static int32_t logic(hpa_V z, hpa_V x, hpa_V y, int w){
         default: e = launch_map2(c, "logic", a, b, zz, n, Logic<1>()); break;
     }
     if(e) R e;
-    if(is_inline(z)){ e = commit_inline(c, z, dz); c->dcur = mark; }
+    if(is_inline(z)){ e = commit_inline(c, z, dz); arena_set(c, mark); }
     R e;
 }
 extern "C" int32_t pfnAnd (hpa_V z, hpa_V x, hpa_V y){ R logic(z,x,y,0); }
@@ -402,24 +402,24 @@ extern "C" int32_t pfnIndex(hpa_V z, hpa_V x, hpa_V y){
         // the keys of a group of this very i32 column: the group kept each key's value
         if(x->typ == HPA_I && y->typ == HPA_L && nz > 1){
             const void *gv = group_vals(c, py, px, nx);
-            if(gv){ c->dcur = mark; z->typ = HPA_I; z->len = nz; z->g = (char*)gv; R 0; }
+            if(gv){ arena_set(c, mark); z->typ = HPA_I; z->len = nz; z->g = (char*)gv; R 0; }
         }
         // index vectors made here (order, group, join, where) are known to be in range
         const bool known = y->typ == HPA_L && within_bound(c, py, nx);
         int *bad = nullptr;
         if(!known){ bad = (int*)dalloc(c, 16); if(!bad) R HPA_E_MEMORY; hipMemsetAsync(bad, 0, 4, c->stream); }
         void *dst = dalloc(c, type_bytes(x->typ, nz > 0 ? nz : 1)); if(!dst) R HPA_E_MEMORY;
-        if(nz > 0 && nx == 0){ c->dcur = mark; R HPA_E_INDEX; }
+        if(nz > 0 && nx == 0){ arena_set(c, mark); R HPA_E_INDEX; }
         if((e = gather_checked(c, dst, px, (const int64_t*)py, nz, nx, w, bad))) R e;
         if(!known){
             int h = 0; if((e = d2h(c, &h, bad, 4))) R e;
-            if(h){ c->dcur = mark; R HPA_E_INDEX; }
+            if(h){ arena_set(c, mark); R HPA_E_INDEX; }
         }
         z->typ = x->typ; z->len = nz;
         if(nz <= 1){
             z->g = (char*)&z->b; z->i64 = 0;
             if(nz == 1 && (e = d2h(c, &z->b, dst, w))) R e;
-            c->dcur = mark;
+            arena_set(c, mark);
         } else {
             z->g = (char*)dst;
             // gathering a bounded index vector keeps its bound
@@ -483,17 +483,17 @@ extern "C" int32_t hpaIndexMany(int32_t nx, hpa_V *z, hpa_V *x, hpa_V y){
     bool known = y->typ == HPA_L;
     for(int k = 0; k < nx; k++){
         const void *px = nullptr;
-        if(x[k]->len > 0 && (e = dev_view(c, x[k], &px))){ c->dcur = mark; R e; }
-        if(nz > 0 && x[k]->len == 0){ c->dcur = mark; R HPA_E_INDEX; }
+        if(x[k]->len > 0 && (e = dev_view(c, x[k], &px))){ arena_set(c, mark); R e; }
+        if(nz > 0 && x[k]->len == 0){ arena_set(c, mark); R HPA_E_INDEX; }
         g.src[k] = (const char*)px; g.w[k] = (int)type_width(x[k]->typ); g.nsrc[k] = x[k]->len;
         known = known && within_bound(c, py, x[k]->len);
     }
     for(int k = 0; k < nx; k++){
         g.dst[k] = (char*)dalloc(c, type_bytes(x[k]->typ, nz > 0 ? nz : 1));
-        if(!g.dst[k]){ c->dcur = mark; R HPA_E_MEMORY; }
+        if(!g.dst[k]){ arena_set(c, mark); R HPA_E_MEMORY; }
     }
     int *bad = nullptr;
-    if(!known){ bad = (int*)dalloc(c, 16); if(!bad){ c->dcur = mark; R HPA_E_MEMORY; } hipMemsetAsync(bad, 0, 4, c->stream); }
+    if(!known){ bad = (int*)dalloc(c, 16); if(!bad){ arena_set(c, mark); R HPA_E_MEMORY; } hipMemsetAsync(bad, 0, 4, c->stream); }
     if(nz > 0){
         double by = 8.0 * nz; for(int k = 0; k < nx; k++) by += 2.0 * g.w[k] * nz;
         ProfScope ps("gather_many", c->stream, by);
@@ -502,7 +502,7 @@ extern "C" int32_t hpaIndexMany(int32_t nx, hpa_V *z, hpa_V *x, hpa_V y){
     }
     if(!known){
         int h = 0; if((e = d2h(c, &h, bad, 4))) R e;
-        if(h){ c->dcur = mark; R HPA_E_INDEX; }
+        if(h){ arena_set(c, mark); R HPA_E_INDEX; }
     }
     // one read back for every inline (single-row) result
     if(nz == 1){
@@ -514,7 +514,7 @@ extern "C" int32_t hpaIndexMany(int32_t nx, hpa_V *z, hpa_V *x, hpa_V y){
             V zk = z[k]; zk->typ = x[k]->typ; zk->len = 1; zk->g = (char*)&zk->b; zk->i64 = 0;
             memcpy(&zk->b, &tmp[k], (size_t)g.w[k]);
         }
-        c->dcur = mark;
+        arena_set(c, mark);
         R 0;
     }
     for(int k = 0; k < nx; k++){

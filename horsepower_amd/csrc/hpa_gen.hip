@@ -108,7 +108,7 @@ extern "C" int32_t hpaGenServeTables(int64_t n_dim, uint64_t seed, hpa_V *dim, h
     for(int k = 0; k < 3; k++) if((e = resident_alloc(dim[k], HPA_I, n_dim))) R e;
     L mark = c->dcur;
     int64_t *per = (int64_t*)dalloc(c, 8 * n_dim), *off = (int64_t*)dalloc(c, 8 * n_dim), *tot = (int64_t*)dalloc(c, 8);
-    if(!per || !off || !tot){ c->dcur = mark; R HPA_E_MEMORY; }
+    if(!per || !off || !tot){ arena_set(c, mark); R HPA_E_MEMORY; }
     // affine permutation of [0, n): a odd and coprime with n (datagen._affine_perm)
     L a = (L)(mix_seed(seed, 99) % (uint64_t)n_dim) | 1;
     while(std::gcd(a, (L)n_dim) != 1) a += 2;
@@ -116,14 +116,14 @@ extern "C" int32_t hpaGenServeTables(int64_t n_dim, uint64_t seed, hpa_V *dim, h
     const int G = grid_for(n_dim, 256, 4);
     hipLaunchKernelGGL(k_gen_dim, dim3(G), dim3(256), 0, c->stream, (L)n_dim, a, b, mix_seed(seed, 11), mix_seed(seed, 12),
                        mix_seed(seed, 13), mix_seed(seed, 14), (int32_t*)dim[0]->g, (int32_t*)dim[1]->g, (int32_t*)dim[2]->g, per);
-    if((e = check_launch("gen_dim")) || (e = scan_i64(c, per, off, n_dim, tot))) { c->dcur = mark; R e; }
+    if((e = check_launch("gen_dim")) || (e = scan_i64(c, per, off, n_dim, tot))) { arena_set(c, mark); R e; }
     int64_t nf = 0;
-    if((e = d2h(c, &nf, tot, 8))){ c->dcur = mark; R e; }
+    if((e = d2h(c, &nf, tot, 8))){ arena_set(c, mark); R e; }
     int64_t *owner = (int64_t*)dalloc(c, 8 * nf);
-    if(!owner){ c->dcur = mark; R HPA_E_MEMORY; }
+    if(!owner){ arena_set(c, mark); R HPA_E_MEMORY; }
     hipLaunchKernelGGL(k_gen_owner, dim3(G), dim3(256), 0, c->stream, (L)n_dim, per, off, owner);
     static const int32_t ftyp[8] = {HPA_I, HPA_I, HPA_E, HPA_E, HPA_I, HPA_E, HPA_C, HPA_C};
-    for(int k = 0; k < 8; k++) if((e = resident_alloc(fact[k], ftyp[k], nf))){ c->dcur = mark; R e; }
+    for(int k = 0; k < 8; k++) if((e = resident_alloc(fact[k], ftyp[k], nf))){ arena_set(c, mark); R e; }
     FactOut o = {(int32_t*)fact[0]->g, (int32_t*)fact[1]->g, (int32_t*)fact[4]->g, (double*)fact[2]->g,
                  (double*)fact[3]->g, (double*)fact[5]->g, fact[6]->g, fact[7]->g};
     FactSeeds s = {mix_seed(seed, 15), mix_seed(seed, 16), mix_seed(seed, 17), mix_seed(seed, 18),
@@ -132,6 +132,6 @@ extern "C" int32_t hpaGenServeTables(int64_t n_dim, uint64_t seed, hpa_V *dim, h
                        (const int32_t*)dim[1]->g, s, o);
     e = check_launch("gen_fact");
     if(!e && hipStreamSynchronize(c->stream) != hipSuccess) e = HPA_E_DEVICE;
-    c->dcur = mark;
+    arena_set(c, mark);
     R e;
 }

@@ -414,7 +414,7 @@ static int32_t guard_build(hpa_ctx *c, const int32_t *x, L n, int lo, L m, Dense
     unsigned long long *part = (unsigned long long*)dalloc(c, 8 * POPC_PARTS);
     uint32_t *pres = (uint32_t*)dalloc(c, 4 * nw);
     t->tab = (int32_t*)dalloc(c, 4 * m);
-    if(!t->dup || !part || !pres || !t->tab){ c->dcur = mark; R 0; }
+    if(!t->dup || !part || !pres || !t->tab){ arena_set(c, mark); R 0; }
     *room = true;
     t->pres = pres;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, t->dup, 4, 0);
@@ -651,7 +651,7 @@ extern "C" int32_t pfnIndexOf(hpa_V z, hpa_V x, hpa_V y){
                 else hipLaunchKernelGGL(k_search_ordered<int64_t>, dim3(grid_for(ny, 256, 4)), dim3(256), 0, c->stream, (const int64_t*)px, nx, (const int64_t*)py, ny, (int64_t*)dz);
                 if((e = check_launch("index_of"))) R e;
             }
-            if(is_inline(z)){ e = commit_inline(c, z, dz); c->dcur = mark; }
+            if(is_inline(z)){ e = commit_inline(c, z, dz); arena_set(c, mark); }
             R e;
         }
         if(nx > 0 && (e = promote_dev(c, x, t, &px))) R e;
@@ -669,7 +669,7 @@ extern "C" int32_t pfnIndexOf(hpa_V z, hpa_V x, hpa_V y){
             case HPA_E: e = index_of_hash<double>(c, (const double*)px, nx, (const double*)py, ny, zz); break;
         }
         if(e) R e;
-        if(is_inline(z)){ e = commit_inline(c, z, dz); c->dcur = mark; }
+        if(is_inline(z)){ e = commit_inline(c, z, dz); arena_set(c, mark); }
         R e;
     }
     if(x->typ == y->typ && is_basic(x->typ)){
@@ -688,7 +688,7 @@ extern "C" int32_t pfnIndexOf(hpa_V z, hpa_V x, hpa_V y){
             default: e = HPA_E_NOT_IMPL;
         }
         if(e) R e;
-        if(is_inline(z)){ e = commit_inline(c, z, dz); c->dcur = mark; }
+        if(is_inline(z)){ e = commit_inline(c, z, dz); arena_set(c, mark); }
         R e;
     }
     R HPA_E_TYPE;
@@ -758,7 +758,7 @@ extern "C" int32_t pfnMember(hpa_V z, hpa_V y, hpa_V x){
             case HPA_E: e = member_t<double>(c, (const double*)py, ny, (const double*)px, nx, zz); break;
         }
         if(e) R e;
-        if(is_inline(z)){ e = commit_inline(c, z, dz); c->dcur = mark; }
+        if(is_inline(z)){ e = commit_inline(c, z, dz); arena_set(c, mark); }
         R e;
     }
     if(x->typ == y->typ && is_basic(x->typ)){
@@ -778,7 +778,7 @@ extern "C" int32_t pfnMember(hpa_V z, hpa_V y, hpa_V x){
             default: e = HPA_E_NOT_IMPL;
         }
         if(e) R e;
-        if(is_inline(z)){ e = commit_inline(c, z, dz); c->dcur = mark; }
+        if(is_inline(z)){ e = commit_inline(c, z, dz); arena_set(c, mark); }
         R e;
     }
     R HPA_E_DOMAIN;
@@ -899,7 +899,7 @@ static int32_t join_run(hpa_ctx *c, V z, V bx, V py, int32_t t, int op, bool swa
         const int hd = (int)(st[0] & 0xffffffff);
         const int64_t P = st[1];
         if(!hd){
-            if(spec) c->dcur = (L)((char*)z1 - c->dbase) + ((8 * (P > 0 ? P : 1) + 255) & ~(L)255);
+            if(spec) arena_set(c, (L)((char*)z1 - c->dbase) + ((8 * (P > 0 ? P : 1) + 255) & ~(L)255));
             else {
                 z0 = (int64_t*)dalloc(c, 8 * (P > 0 ? P : 1)); z1 = (int64_t*)dalloc(c, 8 * (P > 0 ? P : 1));
                 if(!z0 || !z1) R HPA_E_MEMORY;
@@ -913,7 +913,7 @@ static int32_t join_run(hpa_ctx *c, V z, V bx, V py, int32_t t, int op, bool swa
             note_bound(c, z0, swap ? np : nb); note_bound(c, z1, swap ? nb : np);
             R 0;
         }
-        c->dcur = mark;                              // duplicates: the sort-based build below
+        arena_set(c, mark);                              // duplicates: the sort-based build below
         uint32_t *k = (uint32_t*)dalloc(c, 4 * nb); if(!k) R HPA_E_MEMORY;
         hipMemcpyAsync(k, pb, 4 * nb, hipMemcpyDeviceToDevice, c->stream);
         uint32_t *sk;

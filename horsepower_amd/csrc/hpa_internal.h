@@ -25,11 +25,16 @@ struct hpa_ctx {
     char *dbase = nullptr; L dcur = 0, dtop = 0, dpeak = 0;
     // host arena for V0 nodes and list cells
     char *hbase = nullptr; L hcur = 0, htop = 0;
-    std::map<L, L> marks;         // device watermark -> host watermark when it was taken
+    // device watermark -> (host watermark, host-buffer sequence number) when it was taken
+    struct Mark { L hcur; uint64_t seq; };
+    std::map<L, Mark> marks;
     // pinned host staging for small device->host reads
     char *pinned = nullptr; L pinned_bytes = 0;
-    // host buffers handed out by hpaToHost (released with the context)
-    std::vector<void*> host_bufs;
+    // host buffers handed out by hpaToHost, each with the creation sequence number and the
+    // device watermark at creation; a rewind releases the ones made after its mark
+    struct HostBuf { void *p; uint64_t seq; L dcur; };
+    std::vector<HostBuf> host_bufs;
+    uint64_t host_seq = 0;
     // index vectors produced here, with an exclusive upper bound of their (non-negative) values;
     // pfnIndex skips its device bounds check for them (entries die with their arena range)
     std::map<const void*, L> bounds;
@@ -43,6 +48,8 @@ struct hpa_ctx {
     struct GroupOffs { L g; const int64_t *offs; };
     std::map<const void*, GroupOffs> goffs;
 };
+// set the arena top of c (rewind or truncate), dropping side-table entries above it
+void arena_set(hpa_ctx *c, L cur);
 void note_bound(hpa_ctx *c, const void *p, L bound);
 bool within_bound(hpa_ctx *c, const void *p, L limit);
 void note_group_vals(hpa_ctx *c, const void *keys, const void *src, L n, const void *vals);

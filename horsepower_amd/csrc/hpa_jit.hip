@@ -931,9 +931,9 @@ static int32_t fga_run(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_
     const void *pk0 = nullptr, *pk1 = nullptr;
     if(n > 0){ if((e = dev_view(c, keys[0], &pk0))) R e; if(nk == 2 && (e = dev_view(c, keys[1], &pk1))) R e; }
     // FGA_ROWS-row vector loads of every column
-    if(((uintptr_t)pk0 | (uintptr_t)pk1) % FGA_ROWS){ c->dcur = mark; R HPA_E_NOT_IMPL; }
+    if(((uintptr_t)pk0 | (uintptr_t)pk1) % FGA_ROWS){ arena_set(c, mark); R HPA_E_NOT_IMPL; }
     for(int k = 0; k < nx; k++)
-        if(used[k] && (uintptr_t)cols.p[k] % (uintptr_t)(FGA_ROWS * type_width(typ[k]))){ c->dcur = mark; R HPA_E_NOT_IMPL; }
+        if(used[k] && (uintptr_t)cols.p[k] % (uintptr_t)(FGA_ROWS * type_width(typ[k]))){ arena_set(c, mark); R HPA_E_NOT_IMPL; }
     void *part = dalloc(c, (L)psize * FGA_NS * G);
     unsigned *usedb = (unsigned*)dalloc(c, 4 * G);
     int *ovf = (int*)dalloc(c, 16);
@@ -950,7 +950,7 @@ static int32_t fga_run(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_
     long long *rfs = (long long*)dalloc(c, 8 * NREC);
     double *rf = (double*)dalloc(c, 16 * NREC * NA);
     long long *rq = (long long*)dalloc(c, 8 * NREC * NA);
-    if(!part || !usedb || !ovf || !codes || !gdev || !cnt || !first || !fo || !qo || !rc || !rfs || !rf || !rq){ c->dcur = mark; R HPA_E_MEMORY; }
+    if(!part || !usedb || !ovf || !codes || !gdev || !cnt || !first || !fo || !qo || !rc || !rfs || !rf || !rq){ arena_set(c, mark); R HPA_E_MEMORY; }
     struct { const void *c[EXPR_MAX_COLS]; const void *k0; const void *k1; long long n; void *part; unsigned *used;
              unsigned kx0, kx1; long long ki[JIT_MAX_CONST]; double kf[JIT_MAX_CONST]; } ga;
     memset(&ga, 0, sizeof(ga));
@@ -987,8 +987,8 @@ static int32_t fga_run(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_
         if(!o.key0){
             o.key0 = (uint8_t*)dalloc(c, GM); o.key1 = (uint8_t*)dalloc(c, GM);
             o.first = (int64_t*)dalloc(c, 8 * GM);
-            if(!o.key0 || !o.key1 || !o.first){ c->dcur = mark; R HPA_E_MEMORY; }
-            for(int a = 0; a < na; a++){ o.out[a] = dalloc(c, 8 * GM); if(!o.out[a]){ c->dcur = mark; R HPA_E_MEMORY; } }
+            if(!o.key0 || !o.key1 || !o.first){ arena_set(c, mark); R HPA_E_MEMORY; }
+            for(int a = 0; a < na; a++){ o.out[a] = dalloc(c, 8 * GM); if(!o.out[a]){ arena_set(c, mark); R HPA_E_MEMORY; } }
         }
         struct { decltype(ma) m; FgaOut o; } m2;
         m2.m = ma; m2.o = o;
@@ -1006,7 +1006,7 @@ static int32_t fga_run(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_
         } else if((e = d2h(c, hv, ovf, 8))) R e;
         if(!hv[0]) break;
     }
-    if(hv[0]){ c->dcur = mark; R HPA_E_NOT_IMPL; }
+    if(hv[0]){ arena_set(c, mark); R HPA_E_NOT_IMPL; }
     const L g = hv[1];
     const bool staged = hd && g <= (hd->cap < GM ? hd->cap : GM);
     uint8_t *k0 = o.key0, *k1 = o.key1;

@@ -192,7 +192,7 @@ extern "C" int32_t pfnSum(hpa_V z, hpa_V x){
         if(tz == HPA_L) e = sum_int(c, x, p, &iv);
         else e = sum_fp(c, x, p, &dv);
     }
-    c->dcur = mark;
+    arena_set(c, mark);
     if(e) R e;
     z->typ = tz; z->len = 1; z->g = (char*)&z->b; z->i64 = 0;
     if(tz == HPA_L) z->i64 = iv;
@@ -259,7 +259,7 @@ extern "C" int32_t pfnReduce(hpa_V z, hpa_V x, int32_t op){
         case HPA_F: e = minmax_typed<float>(c, p, n, op, (float*)&bits); break;
         case HPA_E: e = minmax_typed<double>(c, p, n, op, (double*)&bits); break;
     }
-    c->dcur = mark;
+    arena_set(c, mark);
     if(e) R e;
     z->typ = x->typ; z->len = 1; z->g = (char*)&z->b; z->i64 = bits;
     R 0;
@@ -716,7 +716,7 @@ extern "C" int32_t pfnRaze(hpa_V z, hpa_V x){
             pos += v->len;
         }
         if(hipStreamSynchronize(c->stream) != hipSuccess) R HPA_E_DEVICE;
-        if(is_inline(z)){ e = commit_inline(c, z, dz); c->dcur = mark; }
+        if(is_inline(z)){ e = commit_inline(c, z, dz); arena_set(c, mark); }
         R e;
     }
     if(is_basic(x->typ)){ *z = *x; R 0; }
@@ -758,7 +758,7 @@ extern "C" int32_t hpaFusedMap(hpa_V z, int32_t typ, int32_t nx, hpa_V *x, const
         }
         if((e = check_launch("fused_map"))) R e;
     }
-    if(is_inline(z)){ e = commit_inline(c, z, dz); c->dcur = mark; }
+    if(is_inline(z)){ e = commit_inline(c, z, dz); arena_set(c, mark); }
     R e;
 }
 
@@ -835,16 +835,16 @@ extern "C" int32_t hpaFusedReduceMany(int32_t nq, hpa_V *z, int32_t kind, int32_
     bool bf = be.ins[be.n - 1].kf;
     int G = sum_grid(n);
     FaPart *part = (FaPart*)dalloc(c, sizeof(FaPart) * G * nq), *out = (FaPart*)dalloc(c, sizeof(FaPart) * nq);
-    if(!part || !out){ c->dcur = mark; R HPA_E_MEMORY; }
+    if(!part || !out){ arena_set(c, mark); R HPA_E_MEMORY; }
     if(jit_fused_reduce_many(c, nq, nx, x, conds, body, cols, n, part, G) != 0){
-        c->dcur = mark;
+        arena_set(c, mark);
         for(int q = 0; q < nq; q++) if((e = hpaFusedReduce(z[q], kind, nx, x, &conds[q], body))) R e;
         R 0;
     }
     hipLaunchKernelGGL(k_fused_reduce_final, dim3(nq), dim3(FA_FINAL), 0, c->stream, part, G, out);
     if((e = check_launch("fused_reduce_many"))) R e;
     FaPart h[8]; if((e = d2h(c, h, out, sizeof(FaPart) * nq))) R e;
-    c->dcur = mark;
+    arena_set(c, mark);
     for(int q = 0; q < nq; q++){
         V zq = z[q];
         zq->len = 1; zq->g = (char*)&zq->b; zq->i64 = 0;
@@ -881,7 +881,7 @@ extern "C" int32_t hpaFusedReduce(hpa_V z, int32_t kind, int32_t nx, hpa_V *x, c
     hipLaunchKernelGGL(k_fused_reduce_final, dim3(1), dim3(FA_FINAL), 0, c->stream, part, G, out);
     if((e = check_launch("fused_reduce"))) R e;
     FaPart h; if((e = d2h(c, &h, out, sizeof(FaPart)))) R e;
-    c->dcur = mark;
+    arena_set(c, mark);
     z->len = 1; z->g = (char*)&z->b; z->i64 = 0;
     if(bf){
         z->typ = HPA_E;

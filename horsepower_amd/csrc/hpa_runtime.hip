@@ -19,6 +19,7 @@ static thread_local char t_err[512];
 struct Range { uintptr_t lo, hi; };
 static std::vector<Range> g_ranges;            // device ranges: arenas and resident payloads
 static std::map<uintptr_t, L> g_resident;       // resident payload -> bytes
+static std::map<uintptr_t, uintptr_t> g_res_host; // hpaToHost copy of a resident payload -> the payload
 
 int num_cus(){ R g_cus; }
 
@@ -118,7 +119,7 @@ extern "C" void hpaCtxDestroy(hpa_ctx *c){
     if(c->dbase && c->own_arena){ unregister_device_range(c->dbase); hipFree(c->dbase); }
     if(c->own_stream) hipStreamDestroy(c->stream);
     if(c->pinned) hipHostFree(c->pinned);
-    for(void *p : c->host_bufs) free(p);
+    for(auto &hb : c->host_bufs) free(hb.p);
     free(c->hbase);
     if(t_cur == c) t_cur = nullptr;
     delete c;
@@ -131,7 +132,8 @@ extern "C" void hpaShutdown(void){
     {
         std::lock_guard<std::mutex> lk(g_mu);
         for(auto &kv : g_resident) hipFree((void*)kv.first);
-        g_resident.clear(); g_ranges.clear();
+        for(auto &kv : g_res_host) free((void*)kv.first);
+        g_resident.clear(); g_res_host.clear(); g_ranges.clear();
     }
     g_init = false;
 }
@@ -166,7 +168,7 @@ extern "C" int32_t hpaCtxSetArena(hpa_ctx *c, void *base, int64_t bytes){
     if(hipStreamSynchronize(c->stream) != hipSuccess) R HPA_E_DEVICE;
     c->dbase = (char*)base; c->dtop = base ? bytes : 0; c->dcur = 0; c->dpeak = 0;
     c->hcur = 0; c->marks.clear(); c->bounds.clear(); c->gvals.clear(); c->goffs.clear();
-    for(void *p : c->host_bufs) free(p);
+    for(auto &hb : c->host_bufs) free(hb.p);
     c->host_bufs.clear();
     R 0;
 }
@@ -210,21 +212,38 @@ extern "C" hpa_V hpaAllocNode(void){ R node_alloc(ctx_cur()); }
 
 extern "C" int64_t hpaGetHeapOffset(void){
     hpa_ctx *c = ctx_cur();
-    c->marks[c->dcur] = c->hcur;
+    c->marks[c->dcur] = {c->hcur, c->host_seq};
     R c->dcur;
 }
 extern "C" int64_t hpaSetHeapOffset(int64_t cur){
     hpa_ctx *c = ctx_cur();
     hipStreamSynchronize(c->stream);          // outstanding kernels may still use the payloads
     auto it = c->marks.find(cur);
-    if(it != c->marks.end()) c->hcur = it->second;
+    // host copies made after the mark die with it; without a recorded mark, those made while the
+    // arena stood above the new offset
+    const bool known = it != c->marks.end();
+    const uint64_t seq = known ? it->second.seq : 0;
+    if(known) c->hcur = it->second.hcur;
     c->marks.erase(c->marks.upper_bound(cur), c->marks.end());
-    for(void *p : c->host_bufs) free(p);
-    c->host_bufs.clear();
-    c->bounds.erase(c->bounds.lower_bound((const void*)(c->dbase + cur)), c->bounds.end());
-    c->gvals.erase(c->gvals.lower_bound((const void*)(c->dbase + cur)), c->gvals.end());
-    c->goffs.erase(c->goffs.lower_bound((const void*)(c->dbase + cur)), c->goffs.end());
-    R c->dcur = cur;
+    size_t k = 0;
+    for(auto &hb : c->host_bufs){
+        if(known ? hb.seq >= seq : hb.dcur > cur) free(hb.p);
+        else c->host_bufs[k++] = hb;
+    }
+    c->host_bufs.resize(k);
+    arena_set(c, cur);
+    R cur;
+}
+
+// Every move of the arena top goes through here: the side tables keyed by arena address (index
+// bounds, group key values, group offsets) lose the entries at or above the new top, so a later
+// allocation at a freed address never inherits a dead buffer's facts.
+void arena_set(hpa_ctx *c, L cur){
+    const void *top = (const void*)(c->dbase + cur);
+    c->bounds.erase(c->bounds.lower_bound(top), c->bounds.end());
+    c->gvals.erase(c->gvals.lower_bound(top), c->gvals.end());
+    c->goffs.erase(c->goffs.lower_bound(top), c->goffs.end());
+    c->dcur = cur;
 }
 
 void note_bound(hpa_ctx *c, const void *p, L bound){
@@ -368,14 +387,14 @@ extern "C" int32_t hpaUpload(hpa_V x){ R upload_rec(ctx_cur(), x); }
 
 extern "C" int32_t hpaFreeResident(hpa_V x){
     void *p = x->g;
-    L bytes = 0;
     {
         std::lock_guard<std::mutex> lk(g_mu);
+        auto hc = g_res_host.find((uintptr_t)p);     // a node that hpaToHost moved to host memory
+        if(hc != g_res_host.end()){ free(p); p = (void*)hc->second; g_res_host.erase(hc); }
         auto it = g_resident.find((uintptr_t)p);
         if(it == g_resident.end()) R HPA_E_DOMAIN;
-        bytes = it->second; g_resident.erase(it);
+        g_resident.erase(it);
     }
-    (void)bytes;
     unregister_device_range(p);
     hipDeviceSynchronize();
     hipFree(p);
@@ -383,28 +402,33 @@ extern "C" int32_t hpaFreeResident(hpa_V x){
     R 0;
 }
 
+// Host copy of one device payload of `bytes`, repointing x->g at it.  A copy of an arena payload
+// belongs to the context and dies with the first rewind below its creation (hpaSetHeapOffset);
+// a copy of a resident payload (hpaUpload, generated tables) keeps the payload's handle and lives
+// until hpaFreeResident of the node.
+static int32_t tohost_payload(hpa_ctx *c, V x, L bytes){
+    void *h = malloc((size_t)(bytes > 0 ? bytes : 1));
+    if(!h){ set_error_detail("hpaToHost: malloc(%lld) failed", (long long)bytes); R HPA_E_MEMORY; }
+    if(hipMemcpyAsync(h, x->g, (size_t)bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess){ free(h); R HPA_E_DEVICE; }
+    if(hipStreamSynchronize(c->stream) != hipSuccess){ free(h); R HPA_E_DEVICE; }
+    bool res;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        res = g_resident.count((uintptr_t)x->g) != 0;
+        if(res) g_res_host[(uintptr_t)h] = (uintptr_t)x->g;
+    }
+    if(!res) c->host_bufs.push_back({h, c->host_seq++, c->dcur});
+    x->g = (char*)h;
+    R 0;
+}
 static int32_t tohost_rec(hpa_ctx *c, V x){
     if(is_basic(x->typ)){
         if(is_inline(x) || !is_device(x->g)) R 0;
-        L bytes = type_bytes(x->typ, x->len);
-        void *h = malloc((size_t)(bytes > 0 ? bytes : 1));
-        if(!h) R HPA_E_MEMORY;
-        if(hipMemcpyAsync(h, x->g, (size_t)bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess){ free(h); R HPA_E_DEVICE; }
-        if(hipStreamSynchronize(c->stream) != hipSuccess){ free(h); R HPA_E_DEVICE; }
-        c->host_bufs.push_back(h);
-        x->g = (char*)h;
-        R 0;
+        R tohost_payload(c, x, type_bytes(x->typ, x->len));
     }
     if(x->typ == HPA_G && is_flat(x)){
         // counts vector lives on the device too
-        if(!is_inline(x) && is_device(x->g)){
-            L bytes = 8 * x->len;
-            void *h = malloc((size_t)(bytes > 0 ? bytes : 1));
-            if(hipMemcpyAsync(h, x->g, (size_t)bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess){ free(h); R HPA_E_DEVICE; }
-            if(hipStreamSynchronize(c->stream) != hipSuccess){ free(h); R HPA_E_DEVICE; }
-            c->host_bufs.push_back(h);
-            x->g = (char*)h;
-        }
+        if(!is_inline(x) && is_device(x->g)){ int32_t e = tohost_payload(c, x, 8 * x->len); if(e) R e; }
         R tohost_rec(c, x->g2);
     }
     if(x->typ == HPA_G || x->typ == HPA_N || x->typ == HPA_A){

@@ -275,16 +275,16 @@ static int32_t compress_many(hpa_ctx *c, int ny, V *z, V x, V *y, bool expr_mode
     uint16_t *mb = nullptr;
     e = expr_mode ? compress_count(c, *ep, nullptr, n, &toff, spec ? nullptr : &kk)
                   : compress_count(c, MaskPred{m}, m, n, &toff, spec ? nullptr : &kk, &mb);
-    if(e){ c->dcur = mark; R e; }
+    if(e){ arena_set(c, mark); R e; }
     if(!spec && kk == n && n > 0 && !expr_mode){           // copyV alias (h_primitive.cc:1513-1515)
         ps.stop(); ps.bytes = (double)n;
-        c->dcur = mark;
+        arena_set(c, mark);
         for(int k = 0; k < ny; k++) *z[k] = *y[k];
         R 0;
     }
     for(int k = 0; k < ny; k++){
         cols.dst[k] = (char*)dalloc(c, type_bytes(y[k]->typ, spec ? n : (kk > 0 ? kk : 1)));
-        if(!cols.dst[k]){ c->dcur = mark; R HPA_E_MEMORY; }
+        if(!cols.dst[k]){ arena_set(c, mark); R HPA_E_MEMORY; }
     }
     e = expr_mode ? compress_write(c, *ep, n, cols, nullptr, toff) : compress_write(c, MaskPred{m}, n, cols, nullptr, toff, mb);
     if(e) R e;
@@ -292,11 +292,11 @@ static int32_t compress_many(hpa_ctx *c, int ny, V *z, V x, V *y, bool expr_mode
         if((e = d2h(c, &kk, toff + ceil_div(n, CP_TILE), 8))) R e;
         if(kk == n && !expr_mode){                          // copyV alias (h_primitive.cc:1513-1515)
             ps.stop(); ps.bytes = (double)n;
-            c->dcur = mark;
+            arena_set(c, mark);
             for(int k = 0; k < ny; k++) *z[k] = *y[k];
             R 0;
         }
-        c->dcur = (L)(cols.dst[ny - 1] - c->dbase) + ((type_bytes(y[ny - 1]->typ, kk > 0 ? kk : 1) + 255) & ~(L)255);
+        arena_set(c, (L)(cols.dst[ny - 1] - c->dbase) + ((type_bytes(y[ny - 1]->typ, kk > 0 ? kk : 1) + 255) & ~(L)255));
     }
     ps.stop();
     ps.bytes = wcond * (double)n + wsum * (double)kk;
@@ -342,22 +342,21 @@ extern "C" int32_t pfnWhere(hpa_V z, hpa_V x){
     // as in compress: with room for n indices the write pass runs while the host reads the count
     const bool spec = n > 0 && c->dcur + 8 * n + 4096 + (16 + 2 * CP_BLOCK) * ceil_div(n, CP_TILE) <= c->dtop;
     uint16_t *mb = nullptr;
-    if((e = compress_count(c, MaskPred{m}, m, n, &toff, spec ? nullptr : &kk, &mb))){ c->dcur = mark; R e; }
-    int64_t *out = (int64_t*)dalloc(c, 8 * (spec ? n : (kk > 0 ? kk : 1))); if(!out){ c->dcur = mark; R HPA_E_MEMORY; }
+    if((e = compress_count(c, MaskPred{m}, m, n, &toff, spec ? nullptr : &kk, &mb))){ arena_set(c, mark); R e; }
+    int64_t *out = (int64_t*)dalloc(c, 8 * (spec ? n : (kk > 0 ? kk : 1))); if(!out){ arena_set(c, mark); R HPA_E_MEMORY; }
     CpCols cols; cols.n = 0;
     if((e = compress_write(c, MaskPred{m}, n, cols, out, toff, mb))) R e;
     if(spec){
         if((e = d2h(c, &kk, toff + ceil_div(n, CP_TILE), 8))) R e;
-        c->dcur = (L)((char*)out - c->dbase) + ((8 * (kk > 0 ? kk : 1) + 255) & ~(L)255);
+        arena_set(c, (L)((char*)out - c->dbase) + ((8 * (kk > 0 ? kk : 1) + 255) & ~(L)255));
     }
     ps.stop(); ps.bytes = (double)n + 8.0 * (double)kk;
     z->typ = HPA_L; z->len = kk;
-    note_bound(c, out, n);
     if(kk <= 1){
         z->g = (char*)&z->b; z->i64 = 0;
         if(kk == 1 && (e = d2h(c, &z->b, out, 8))) R e;
-        c->dcur = mark;
-    } else z->g = (char*)out;
+        arena_set(c, mark);
+    } else { z->g = (char*)out; note_bound(c, out, n); }
     R 0;
 }
 
@@ -431,7 +430,7 @@ extern "C" int32_t hpaFusedCompress(int32_t ny, hpa_V *z, hpa_V *y, int32_t nx, 
             m.typ = HPA_B; m.len = n; m.g = (char*)mask;
             R compress_many(c, ny, z, &m, y, false, nullptr, 0);
         }
-        c->dcur = mark;
+        arena_set(c, mark);
     }
     R compress_many(c, ny, z, nullptr, y, true, &ep, n);
 }

@@ -1360,7 +1360,7 @@ static int32_t group_i32_runs(hpa_ctx *c, V z, const int32_t *x, L n, bool *done
     hipLaunchKernelGGL(k_gr_head_count, dim3((unsigned)nt), dim3(256), 0, c->stream, x, n, cap_mask, tcount);
     if((e = scan_u32_to_i64(c, tcount, toff, nt, toff + nt))) R e;
     int64_t nr = 0; if((e = d2h(c, &nr, toff + nt, 8))) R e;
-    if((double)nr > GR_RUNS_MAX_SHARE * (double)n){ c->dcur = mark; R 0; }
+    if((double)nr > GR_RUNS_MAX_SHARE * (double)n){ arena_set(c, mark); R 0; }
     uint32_t *rkey = (uint32_t*)dalloc(c, 4 * nr), *rval = (uint32_t*)dalloc(c, 4 * nr);
     if(!rkey || !rval) R HPA_E_MEMORY;
     hipLaunchKernelGGL(k_gr_head_write, dim3((unsigned)nt), dim3(256), 0, c->stream, x, n, cap_mask, cb, (const int64_t*)toff, rkey, rval);
@@ -1794,7 +1794,7 @@ static int32_t order_perm(hpa_ctx *c, V x, const uint8_t *flags, uint32_t **perm
             bool bt = true;
             if((e = order_bucket_rank(c, s, n, perm_out, &fallback, &bt))) R e;
             if(!fallback){ *ties = bt; *known_ties = true; R 0; }
-            c->dcur = mark;
+            arena_set(c, mark);
         }
     }
     uint64_t *k = (uint64_t*)dalloc(c, 8 * (n > 0 ? n : 1)); if(!k) R HPA_E_MEMORY;
@@ -1821,7 +1821,7 @@ static int32_t order_emit(hpa_ctx *c, V z, const uint32_t *perm, L n, bool rever
     if((e = out_init(c, z, HPA_L, n, &dz))) R e;
     if(n > 0) hipLaunchKernelGGL(k_perm_to_i64, dim3(grid_for(n, 256, 4)), dim3(256), 0, c->stream, perm, (int64_t*)dz, n, reverse ? 1 : 0);
     if((e = check_launch("order"))) R e;
-    if(is_inline(z)){ e = commit_inline(c, z, dz); c->dcur = mark; }
+    if(is_inline(z)){ e = commit_inline(c, z, dz); arena_set(c, mark); }
     else note_bound(c, dz, n);                   // a permutation of 0..n-1
     R e;
 }
@@ -1888,7 +1888,7 @@ static int32_t order_vector(hpa_ctx *c, V z, V col, bool up){   // lib_order_by_
             if((e = out_init(c, z, HPA_L, n, &dz))) R e;
             if(n > 0) hipLaunchKernelGGL(k_char_small, dim3(1), dim3(64), 0, c->stream, (const char*)p, (int64_t*)dz, (int)n, up ? 0 : 1);
             if((e = check_launch("order"))) R e;
-            if(is_inline(z)){ e = commit_inline(c, z, dz); c->dcur = mark; }
+            if(is_inline(z)){ e = commit_inline(c, z, dz); arena_set(c, mark); }
             R e;
         }
         // counting sort by byte (stable, ascending), then reversal for desc
@@ -2032,7 +2032,7 @@ static int32_t order_list_ties(hpa_ctx *c, V x, const uint8_t *flags, uint32_t *
         hipLaunchKernelGGL(k_ot_flip, dim3(g), dim3(256), 0, c->stream, (const uint32_t*)b, (const int64_t*)ex, (const uint32_t*)rflag,
                            n, dir);
         if((e = check_launch("order_ties"))) R e;
-        c->dcur = mark;
+        arena_set(c, mark);
     }
     // the full-tie groups, each turned around where its direction is descending
     hipLaunchKernelGGL(k_ot_bound, dim3(g), dim3(256), 0, c->stream, (const uint8_t*)lvl, n, (int)nc, b);

@@ -18,7 +18,25 @@ def build():
     return LIB
 
 
-def load(threads=0, heap_bytes=1 << 32):
+def _heap_reservation():
+    """Bytes of address space for the oracle's bump heap: half the host's memory (at least
+    4 GiB), or HPA_ORACLE_HEAP_GB.  Pages are committed only when touched, and a heap that
+    fills up makes every primitive return HPA_E_MEMORY."""
+    gb = os.environ.get("HPA_ORACLE_HEAP_GB")
+    if gb:
+        return int(float(gb) * (1 << 30))
+    try:
+        phys = os.sysconf("SC_PHYS_PAGES") * os.sysconf("SC_PAGE_SIZE")
+    except (ValueError, OSError):
+        phys = 8 << 30
+    return max(4 << 30, phys // 2)
+
+
+def heap_capacity():
+    return load().orc_heap_capacity()
+
+
+def load(threads=0, heap_bytes=None):
     global _dll
     if _dll is None:
         if not os.path.exists(LIB):
@@ -26,7 +44,10 @@ def load(threads=0, heap_bytes=1 << 32):
         _dll = C.CDLL(LIB)
         _dll.orc_init.argtypes = [C.c_int64, C.c_int32]
         _dll.orc_init.restype = C.c_int32
-        if _dll.orc_init(heap_bytes, threads) != 0:
+        _dll.orc_heap_capacity.restype = C.c_int64
+        _dll.orc_set_heap_top.argtypes = [C.c_int64]
+        _dll.orc_set_heap_top.restype = C.c_int64
+        if _dll.orc_init(heap_bytes or _heap_reservation(), threads) != 0:
             raise RuntimeError("oracle heap allocation failed")
     elif threads:
         _dll.orc_init(0, threads)

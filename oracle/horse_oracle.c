@@ -24,6 +24,7 @@
 #include <float.h>
 #include <limits.h>
 #include <omp.h>
+#include <sys/mman.h>
 
 #define R return
 #define H_EPS 1e-13                   /* h_system.cc:9 */
@@ -34,7 +35,7 @@ typedef int64_t L;
 typedef double  E;
 
 static char   *g_heap = NULL;
-static L       g_cur = 0, g_top = 0;
+static L       g_cur = 0, g_top = 0, g_cap = 0;   /* bump pointer, usable top, reservation */
 static int     g_threads = 1;
 
 /* ------------------------------------------------------------------ memory (h_memory.cc) */
@@ -61,12 +62,17 @@ static int is_comparable(int32_t t){                /* h_system.cc:963 */
     R is_real(t) || t==HPA_C || t==HPA_Q || (t>=HPA_M && t<=HPA_T);
 }
 
+/* The heap is one anonymous mapping reserved without swap accounting: pages are committed as
+ * the bump pointer first touches them, so a large reservation (the caller sizes it from its
+ * inputs) costs only what is used. */
 int32_t orc_init(int64_t heap_bytes, int32_t threads){
     if(!g_heap){
         if(heap_bytes <= 0) heap_bytes = 1LL<<32;
-        g_heap = (char*)malloc((size_t)heap_bytes);
-        if(!g_heap) R HPA_E_MEMORY;
-        g_top = heap_bytes; g_cur = 0;
+        void *p = mmap(NULL, (size_t)heap_bytes, PROT_READ | PROT_WRITE,
+                       MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE, -1, 0);
+        if(p == MAP_FAILED) R HPA_E_MEMORY;
+        g_heap = (char*)p;
+        g_top = g_cap = heap_bytes; g_cur = 0;
     }
     if(threads > 0) g_threads = threads;
     else g_threads = omp_get_max_threads();
@@ -75,6 +81,14 @@ int32_t orc_init(int64_t heap_bytes, int32_t threads){
 }
 int32_t orc_threads(void){ R g_threads; }
 int64_t orc_get_heap(void){ R g_cur; }
+int64_t orc_heap_capacity(void){ R g_cap; }
+/* Lower (or restore) the heap's usable top, within the reservation; returns the previous top.
+ * Lets a test run a primitive against a nearly full heap. */
+int64_t orc_set_heap_top(int64_t top){
+    L old = g_top;
+    g_top = (top <= 0 || top > g_cap) ? g_cap : top;
+    R old;
+}
 int64_t orc_set_heap(int64_t cur){ R g_cur = cur; }
 
 static char *heap_alloc(L size){                   /* allocMem, h_memory.cc:93-112 */
@@ -89,13 +103,17 @@ OV orc_alloc_node(void){
     R x;
 }
 
-OV orc_initV(OV x, int32_t typ, int64_t len){      /* initV, h_memory.cc:182-196 */
+/* initV (h_memory.cc:182-196).  Returns NULL when the payload does not fit the heap (the
+ * reference stops the process with "Heap full!!", h_memory.cc:110); every caller turns that
+ * into HPA_E_MEMORY through IV. */
+OV orc_initV(OV x, int32_t typ, int64_t len){
     x->typ = typ; x->len = len;
     if(is_basic(typ)) x->g = (len > 1) ? heap_alloc(type_size(typ,len)) : (char*)&x->b;
     else x->g = heap_alloc(type_size(typ,len));
     if(typ==HPA_G) x->g2 = NULL;
-    R x;
+    R x->g ? x : NULL;
 }
+#define IV(x, t, n) do{ if(!orc_initV((x), (t), (n))) R HPA_E_MEMORY; }while(0)
 static void copyV(OV z, OV x){ *z = *x; }         /* h_system.cc:70-73 */
 #define vV(x,k) (((OV)(x)->g)+(k))
 
@@ -128,7 +146,7 @@ static int op_from_symbol(int64_t q){             /* getOpFromName, h_system.cc:
 static int32_t promote(OV z, OV x, int32_t typMax){
     if(x->typ == typMax){ copyV(z,x); R 0; }
     L n = x->len;
-    orc_initV(z, typMax, n);
+    IV(z, typMax, n);
     if(n > 1 && !z->g) R HPA_E_MEMORY;
 #define PROM(T, ctype) { ctype *d=(ctype*)z->g; switch(x->typ){ \
         case HPA_B: for(L i=0;i<n;i++) d[i]=(ctype)sB(x)[i]; break; \
@@ -171,7 +189,7 @@ int32_t orc_pfnCompare(OV z, OV x, OV y, int32_t op){
         O0 tx, ty; int32_t e;
         if((e=promote(&tx,x,t))) R e;
         if((e=promote(&ty,y,t))) R e;
-        orc_initV(z, HPA_B, n); uint8_t *zz = sB(z);
+        IV(z, HPA_B, n); uint8_t *zz = sB(z);
         switch(t){
 #define CMPI(ct) { ct *a=(ct*)tx.g, *b=(ct*)ty.g; _Pragma("omp parallel for simd") for(L i=0;i<n;i++) zz[i]=COMP(op,a[i*sx],b[i*sy]); } break
             case HPA_B: CMPI(uint8_t);
@@ -192,7 +210,7 @@ int32_t orc_pfnCompare(OV z, OV x, OV y, int32_t op){
         R 0;
     }
     else if(x->typ == y->typ){
-        orc_initV(z, HPA_B, n); uint8_t *zz = sB(z);
+        IV(z, HPA_B, n); uint8_t *zz = sB(z);
         switch(x->typ){
 #define CMPS(ct) { ct *a=(ct*)x->g, *b=(ct*)y->g; for(L i=0;i<n;i++) zz[i]=COMP(op,a[i*sx],b[i*sy]); } break
             case HPA_C: CMPS(char);
@@ -239,7 +257,7 @@ int32_t orc_pfnArith(OV z, OV x, OV y, int32_t op){
     O0 tx, ty; int32_t e;
     if((e=promote(&tx,x,t))) R e;
     if((e=promote(&ty,y,t))) R e;
-    orc_initV(z, t, n);
+    IV(z, t, n);
     switch(t){
         case HPA_B: { uint8_t *a=sB(&tx),*b=sB(&ty),*d=sB(z);
             #pragma omp parallel for simd
@@ -269,7 +287,7 @@ static int32_t logic(OV z, OV x, OV y, int which){
     L n = (x->len==1) ? y->len : x->len;
     L sx = (x->len==1)?0:1, sy = (y->len==1)?0:1;
     uint8_t *a=sB(x), *b=sB(y);
-    orc_initV(z, HPA_B, n); uint8_t *d = sB(z);
+    IV(z, HPA_B, n); uint8_t *d = sB(z);
     #pragma omp parallel for simd
     for(L i=0;i<n;i++){
         int p=a[i*sx], q=b[i*sy], r;
@@ -315,7 +333,7 @@ int32_t orc_pfnCompress(OV z, OV x, OV y){        /* h_primitive.cc:1479-1521 */
             case HPA_L: case HPA_E: case HPA_Q: case HPA_S: case HPA_Y: w=8; break;
             default: free(par); free(off); R HPA_E_NOT_IMPL;
         }
-        orc_initV(z, typ, k);
+        if(!orc_initV(z, typ, k)){ free(par); free(off); R HPA_E_MEMORY; }
         char *dst = z->g; const char *src = y->g; const uint8_t *m = sB(x);
         L seg = n / T;
         #pragma omp parallel num_threads(T)
@@ -326,7 +344,7 @@ int32_t orc_pfnCompress(OV z, OV x, OV y){        /* h_primitive.cc:1479-1521 */
         }
     }
     else if(k > 0) copyV(z, y);
-    else orc_initV(z, typ, 0);
+    else if(!orc_initV(z, typ, 0)){ free(par); free(off); R HPA_E_MEMORY; }
     free(par); free(off);
     R 0;
 }
@@ -335,7 +353,7 @@ int32_t orc_pfnWhere(OV z, OV x){                  /* h_primitive.cc:728-762 */
     if(x->typ != HPA_B) R HPA_E_DOMAIN;
     L n = x->len, k = 0; const uint8_t *m = sB(x);
     for(L i=0;i<n;i++) k += m[i]?1:0;
-    orc_initV(z, HPA_L, k);
+    IV(z, HPA_L, k);
     L c = 0; for(L i=0;i<n;i++) if(m[i]) sL(z)[c++] = i;
     R 0;
 }
@@ -350,7 +368,7 @@ int32_t orc_pfnIndex(OV z, OV x, OV y){
     for(L i=0;i<nz;i++) if(iy[i] >= nx || iy[i] < 0) R HPA_E_INDEX;
     if(is_basic(x->typ)){
         L w = type_size(x->typ, 1); if(x->typ==HPA_C) w = 1;
-        orc_initV(z, x->typ, nz);
+        IV(z, x->typ, nz);
         char *d = z->g; const char *s = x->g;
         #pragma omp parallel for
         for(L i=0;i<nz;i++) memcpy(d + i*w, s + iy[i]*w, (size_t)w);
@@ -364,13 +382,13 @@ int32_t orc_pfnIndex(OV z, OV x, OV y){
 }
 
 int32_t orc_pfnList(OV z, int64_t n, OV x[]){      /* h_primitive.cc:32-36 */
-    orc_initV(z, HPA_G, n); z->g2 = NULL;
+    IV(z, HPA_G, n); z->g2 = NULL;
     for(L i=0;i<n;i++) *vV(z,i) = *x[i];
     R 0;
 }
 
 int32_t orc_pfnLen(OV z, OV x){                    /* h_primitive.cc:656-662 */
-    orc_initV(z, HPA_L, 1);
+    IV(z, HPA_L, 1);
     z->i64 = x->len;
     R 0;
 }
@@ -380,7 +398,7 @@ int32_t orc_pfnSum(OV z, OV x){
     if(!is_real(x->typ)) R HPA_E_DOMAIN;
     int32_t tz = x->typ==HPA_F?HPA_F:x->typ==HPA_E?HPA_E:HPA_L;
     L n = x->len;
-    orc_initV(z, tz, 1);
+    IV(z, tz, 1);
     switch(x->typ){
 #define SUMI(ct) { L t=0; ct *a=(ct*)x->g; _Pragma("omp parallel for simd reduction(+:t)") for(L i=0;i<n;i++) t+=a[i]; z->i64=t; } break
         case HPA_B: SUMI(uint8_t);
@@ -405,7 +423,7 @@ int32_t orc_pfnAvg(OV z, OV x){
     O0 t; int32_t e;
     if((e=orc_pfnSum(&t, x))) R e;
     int32_t tz = t.typ==HPA_F ? HPA_F : HPA_E;
-    orc_initV(z, tz, 1);
+    IV(z, tz, 1);
     switch(t.typ){
         case HPA_L: z->f64 = t.i64*1.0/x->len; break;
         case HPA_F: z->f32 = t.f32/x->len; break;
@@ -419,7 +437,7 @@ int32_t orc_pfnReduce(OV z, OV x, int32_t op){
     L n = x->len;
     if(n < 1){
         int32_t tz = is_int(x->typ) ? HPA_L : x->typ;
-        orc_initV(z, tz, 1);
+        IV(z, tz, 1);
         switch(tz){
             case HPA_L: z->i64 = op==0 ? LLONG_MAX : LLONG_MIN; break;
             case HPA_F: z->f32 = op==0 ? FLT_MAX : FLT_MIN; break;
@@ -427,7 +445,7 @@ int32_t orc_pfnReduce(OV z, OV x, int32_t op){
         }
         R 0;
     }
-    orc_initV(z, x->typ, 1);
+    IV(z, x->typ, 1);
     switch(x->typ){
 #define RED(ct, fld) { ct *a=(ct*)x->g; ct t=a[0]; for(L i=0;i<n;i++) t = op==0?MINV(t,a[i]):MAXV(t,a[i]); z->fld=t; } break
         case HPA_B: { uint8_t *a=sB(x); uint8_t t=a[0]; for(L i=0;i<n;i++) t = op==0?MINV(t,a[i]):MAXV(t,a[i]); z->b=t; } break;
@@ -655,25 +673,27 @@ static void order_by_vector(L *targ, OV val, const uint8_t *isUp, L n){ /* h_lib
 }
 
 /* checkOrderCase1, h_libs.cc:1912-1940: both columns i32 and equal flags.  The packed key is
- * ((0|x0)<<32)|x1 with x0, x1 sign-extended, so a negative x1 sets all high bits. */
-static int check_order_case1(OV z, OV x, OV f){
+ * ((0|x0)<<32)|x1 with x0, x1 sign-extended, so a negative x1 sets all high bits.
+ * Returns -1 when the case does not apply, else 0 or an error code. */
+static int32_t check_order_case1(OV z, OV x, OV f){
     if(x->len==2 && vV(x,0)->typ==HPA_I && vV(x,1)->typ==HPA_I && sB(f)[0]==sB(f)[1]){
         OV x0 = vV(x,0), x1 = vV(x,1);
         L n = x0->len;
         int64_t *k = (int64_t*)malloc(sizeof(int64_t)*(n>0?n:1));
+        if(!k) R HPA_E_MEMORY;
         for(L i=0;i<n;i++) k[i] = (int64_t)((uint64_t)(int64_t)sI(x0)[i] << 32) | (int64_t)sI(x1)[i];
-        orc_initV(z, HPA_L, n);
+        if(!orc_initV(z, HPA_L, n)){ free(k); R HPA_E_MEMORY; }
         radixsort_long_keys(sL(z), k, n, sB(f)[0]);
         free(k);
-        R 1;
+        R 0;
     }
-    R 0;
+    R -1;
 }
 
 static int32_t order_basic(OV z, OV x, OV y){      /* lib_order_basic, h_primitive.cc:1760-1767 */
     if(is_basic(x->typ) && y->typ==HPA_B && y->len==1){
         if(x->typ==HPA_S || x->typ==HPA_X) R HPA_E_NOT_IMPL;
-        orc_initV(z, HPA_L, x->len);
+        IV(z, HPA_L, x->len);
         order_by_vector(sL(z), x, sB(y), x->len);
         R 0;
     }
@@ -684,11 +704,12 @@ int32_t orc_pfnOrderBy(OV z, OV x, OV y){          /* h_primitive.cc:1769-1816 *
     if(x->typ==HPA_G && y->typ==HPA_B && x->len==y->len){
         for(L i=0;i<x->len;i++) if(!is_basic(vV(x,i)->typ)) R HPA_E_DOMAIN;
         if(x->len > 0){ L n=vV(x,0)->len; for(L i=0;i<x->len;i++) if(vV(x,i)->len!=n) R HPA_E_MATCH; }
-        if(check_order_case1(z, x, y)) { ; }
-        else if(x->len == 1) R order_basic(z, vV(x,0), y);
+        int32_t c1 = check_order_case1(z, x, y);
+        if(c1 >= 0) R c1;
+        if(x->len == 1) R order_basic(z, vV(x,0), y);
         else {
             L n = x->len==0 ? 0 : vV(x,0)->len;
-            orc_initV(z, HPA_L, n);
+            IV(z, HPA_L, n);
             order_by_list(sL(z), x, sB(y), n);
         }
         R 0;
@@ -697,26 +718,28 @@ int32_t orc_pfnOrderBy(OV z, OV x, OV y){          /* h_primitive.cc:1769-1816 *
 }
 
 /* ------------------------------------------------------------------ group */
-static void init_dict(OV z, L g){
-    orc_initV(z, HPA_N, 2);
-    orc_initV(vV(z,0), HPA_L, g);
-    orc_initV(vV(z,1), HPA_G, g);
+static int32_t init_dict(OV z, L g){
+    IV(z, HPA_N, 2);
+    IV(vV(z,0), HPA_L, g);
+    IV(vV(z,1), HPA_G, g);
+    R 0;
 }
 
 /* lib_group_by_normal_int, h_libs.cc:1295-1329 */
 static int32_t group_int(OV z, OV x){
     L n = x->len;
     PosI *p = (PosI*)malloc(sizeof(PosI)*(n>0?n:1));
+    if(!p) R HPA_E_MEMORY;
     for(L i=0;i<n;i++){ p[i].x = sI(x)[i]; p[i].i = i; }
     radix_core_int(p, n);
     L g = n>0 ? 1 : 0;
     for(L i=1;i<n;i++) if(p[i-1].x != p[i].x) g++;
-    init_dict(z, g);
+    if(init_dict(z, g)){ free(p); R HPA_E_MEMORY; }
     OV keys = vV(z,0), vals = vV(z,1);
     L k = 0, s = 0;
     for(L i=1;i<=n;i++){
         if(i==n || p[i-1].x != p[i].x){
-            OV d = vV(vals,k); orc_initV(d, HPA_L, i-s);
+            OV d = vV(vals,k); if(!orc_initV(d, HPA_L, i-s)){ free(p); R HPA_E_MEMORY; }
             for(L j=s;j<i;j++) sL(d)[j-s] = p[j].i;
             sL(keys)[k] = p[s].i; k++; s = i;
         }
@@ -727,16 +750,17 @@ static int32_t group_int(OV z, OV x){
 static int32_t group_long(OV z, OV x){
     L n = x->len;
     PosL *p = (PosL*)malloc(sizeof(PosL)*(n>0?n:1));
+    if(!p) R HPA_E_MEMORY;
     for(L i=0;i<n;i++){ p[i].x = sL(x)[i]; p[i].i = (int32_t)i; }
     radix_core_long(p, n);
     L g = n>0 ? 1 : 0;
     for(L i=1;i<n;i++) if(p[i-1].x != p[i].x) g++;
-    init_dict(z, g);
+    if(init_dict(z, g)){ free(p); R HPA_E_MEMORY; }
     OV keys = vV(z,0), vals = vV(z,1);
     L k = 0, s = 0;
     for(L i=1;i<=n;i++){
         if(i==n || p[i-1].x != p[i].x){
-            OV d = vV(vals,k); orc_initV(d, HPA_L, i-s);
+            OV d = vV(vals,k); if(!orc_initV(d, HPA_L, i-s)){ free(p); R HPA_E_MEMORY; }
             for(L j=s;j<i;j++) sL(d)[j-s] = p[j].i;
             sL(keys)[k] = p[s].i; k++; s = i;
         }
@@ -750,12 +774,12 @@ static int32_t group_runs(OV z, OV val, const L *index, L n, int isList){
 #define IX(i) (index ? index[i] : (i))
 #define CMPR(a,b) (isList ? cmp_list(val,(a),(b),NULL) : cmp_item(val,(a),(b),NULL))
     for(L i=1;i<n;i++) if(CMPR(IX(i-1), IX(i)) != 0) g++;
-    init_dict(z, g);
+    if(init_dict(z, g)) R HPA_E_MEMORY;
     OV keys = vV(z,0), vals = vV(z,1);
     L k = 0, s = 0;
     for(L i=1;i<=n;i++){
         if(i==n || CMPR(IX(i-1), IX(i)) != 0){
-            OV d = vV(vals,k); orc_initV(d, HPA_L, i-s);
+            OV d = vV(vals,k); IV(d, HPA_L, i-s);
             for(L j=s;j<i;j++) sL(d)[j-s] = IX(j);
             sL(keys)[k] = IX(s); k++; s = i;
         }
@@ -784,12 +808,14 @@ int32_t orc_pfnGroupTrie(OV z, OV x){              /* h_primitive.cc:2656-2681 *
     OV c0 = vV(x,0), c1 = vV(x,1);
     L n = c0->len;
     L *cnt = (L*)calloc(65536, sizeof(L)), *id = (L*)malloc(sizeof(L)*65536);
+    if(!cnt || !id){ free(cnt); free(id); R HPA_E_MEMORY; }
 #define CODE(i) ((((int)(unsigned char)sC(c0)[i])<<8) + (int)(unsigned char)sC(c1)[i])
     for(L i=0;i<n;i++) cnt[CODE(i)]++;
     L g = 0; for(int i=0;i<65536;i++) if(cnt[i]) id[i] = g++;
-    init_dict(z, g);
+    if(init_dict(z, g)){ free(cnt); free(id); R HPA_E_MEMORY; }
     OV keys = vV(z,0), vals = vV(z,1);
-    L c = 0; for(int i=0;i<65536;i++) if(cnt[i]) orc_initV(vV(vals,c++), HPA_L, cnt[i]);
+    L c = 0;
+    for(int i=0;i<65536;i++) if(cnt[i] && !orc_initV(vV(vals,c++), HPA_L, cnt[i])){ free(cnt); free(id); R HPA_E_MEMORY; }
     L *fill = (L*)calloc(g>0?g:1, sizeof(L));
     for(L i=0;i<n;i++){ L k = id[CODE(i)]; sL(vV(vals,k))[fill[k]++] = i; }
     for(L k=0;k<g;k++) sL(keys)[k] = sL(vV(vals,k))[0];
@@ -804,7 +830,7 @@ int32_t orc_pfnGroupBasic(OV z, OV x){             /* h_primitive.cc:1002-1057 *
     else if(is_integers(x)) { ; }
     else {
         L lenZ = x->typ==HPA_G ? x->len : 1;
-        O0 y; orc_initV(&y, HPA_B, lenZ);
+        O0 y; IV(&y, HPA_B, lenZ);
         for(L i=0;i<lenZ;i++) sB(&y)[i] = 1;
         int32_t e = orc_pfnOrderBy(&t, x, &y); if(e) R e;
         order = sL(&t);
@@ -837,13 +863,13 @@ int32_t orc_pfnValues(OV z, OV x){                 /* h_primitive.cc:205-218 */
 /* ------------------------------------------------------------------ each / raze */
 int32_t orc_pfnEach(OV z, OV x, int32_t (*foo)(OV, OV)){      /* h_primitive.cc:1818-1832 */
     if(x->typ != HPA_G) R HPA_E_DOMAIN;
-    orc_initV(z, HPA_G, x->len);
+    IV(z, HPA_G, x->len);
     for(L i=0;i<x->len;i++){ int32_t e = foo(vV(z,i), vV(x,i)); if(e) R e; }
     R 0;
 }
 int32_t orc_pfnEachRight(OV z, OV x, OV y, int32_t (*foo)(OV, OV, OV)){ /* h_primitive.cc:1882-1904 */
     if(y->typ == HPA_G){
-        orc_initV(z, HPA_G, y->len);
+        IV(z, HPA_G, y->len);
         for(L i=0;i<y->len;i++){ int32_t e = foo(vV(z,i), x, vV(y,i)); if(e) R e; }
         R 0;
     }
@@ -855,7 +881,7 @@ int32_t orc_pfnRaze(OV z, OV x){                   /* h_primitive.cc:935-955, fi
         int32_t t = vV(x,0)->typ; L n = 0;
         for(L i=0;i<x->len;i++){ if(vV(x,i)->typ != t) R HPA_E_TYPE; n += vV(x,i)->len; }
         L w = type_size(t,1); if(t==HPA_C) w = 1;
-        orc_initV(z, t, n);
+        IV(z, t, n);
         char *d = z->g; L c = 0;
         for(L i=0;i<x->len;i++){ OV v = vV(x,i); memcpy(d + c*w, v->g, (size_t)(w*v->len)); c += v->len; }
         R 0;
@@ -885,7 +911,7 @@ static int32_t join_hash(OV z0, OV z1, OV x, OV y){
         L s = a; while(a < nx && b[2*a] == key) a++;
         lo[j] = s; cnt[j] = a - s; c += a - s;
     }
-    orc_initV(z0, HPA_L, c); orc_initV(z1, HPA_L, c);
+    if(!orc_initV(z0, HPA_L, c) || !orc_initV(z1, HPA_L, c)){ free(b); free(lo); free(cnt); R HPA_E_MEMORY; }
     c = 0;
     for(L j=0;j<ny;j++) for(L k=0;k<cnt[j];k++){ sL(z0)[c] = b[2*(lo[j]+k)+1]; sL(z1)[c] = j; c++; }
     free(b); free(lo); free(cnt);
@@ -898,7 +924,7 @@ static int32_t join_linear(OV z0, OV z1, OV x, OV y, int isEq){
 #define MATCH(i,j) (x->typ==HPA_L ? (isEq ? sL(y)[i]==sL(x)[j] : sL(y)[i]!=sL(x)[j]) : (isEq ? KV(y,i)==KV(x,j) : KV(y,i)!=KV(x,j)))
     if(!(x->typ==HPA_I || x->typ==HPA_L || x->typ==HPA_E)) R HPA_E_NOT_IMPL;
     for(L i=0;i<ny;i++) for(L j=0;j<nx;j++) if(MATCH(i,j)){ size++; break; }
-    orc_initV(z0, HPA_L, size); orc_initV(z1, HPA_L, size);
+    IV(z0, HPA_L, size); IV(z1, HPA_L, size);
     size = 0;
     for(L i=0;i<ny;i++) for(L j=0;j<nx;j++) if(MATCH(i,j)){ sL(z0)[size] = j; sL(z1)[size++] = i; break; }
 #undef KV
@@ -913,7 +939,7 @@ static int32_t join_single(OV z, OV x, OV y, OV f){
     O0 tx, ty; int32_t e;
     if((e=promote(&tx,x,t))) R e;
     if((e=promote(&ty,y,t))) R e;
-    orc_initV(z, HPA_G, 2);
+    IV(z, HPA_G, 2);
     if(op==4 || op==5){
         if(tx.len >= 10){                          /* NUM_JOIN_LINEAR, common.h:152 */
             if(op != 4 || t != HPA_I) R HPA_E_NOT_IMPL;   /* v3 hash path: i32 eq only, h_hash_v3.cc:233,272-275 */
@@ -970,7 +996,7 @@ int32_t orc_pfnIndexOf(OV z, OV x, OV y){          /* h_primitive.cc:1525-1579 *
         if(is_ordered(x)){
             O0 ty; int32_t e;
             if((e=promote(&ty, y, x->typ > y->typ ? x->typ : y->typ))) R e;
-            L n = y->len; orc_initV(z, HPA_L, n);
+            L n = y->len; IV(z, HPA_L, n);
             for(L i=0;i<n;i++){
                 int64_t v = ty.typ==HPA_I ? sI(&ty)[i] : ty.typ==HPA_L ? sL(&ty)[i] : 0;
                 if(!is_int(ty.typ)){ R HPA_E_NOT_IMPL; }
@@ -984,7 +1010,7 @@ int32_t orc_pfnIndexOf(OV z, OV x, OV y){          /* h_primitive.cc:1525-1579 *
         if((e=promote(&tx,x,t))) R e;
         if((e=promote(&ty,y,t))) R e;
         L nx = tx.len, ny = ty.len;
-        orc_initV(z, HPA_L, ny);
+        IV(z, HPA_L, ny);
         if(t == HPA_B){                            /* lib_index_of_B, h_libs.cc:309-314: flag[1] starts at 0 */
             L flag[2] = {-1, 0}; int c = 0;
             for(L i=0;i<nx;i++){ int s = sB(&tx)[i]; if(c<2 && flag[s]==-1){ flag[s]=i; c++; if(c==2) break; } }
@@ -1006,7 +1032,7 @@ int32_t orc_pfnIndexOf(OV z, OV x, OV y){          /* h_primitive.cc:1525-1579 *
         L nx = x->len, ny = y->len, buf[256];
         for(int i=0;i<256;i++) buf[i] = -1;
         for(L i=0;i<nx;i++){ unsigned char c = sC(x)[i]; if(buf[c]==-1) buf[c]=i; }
-        orc_initV(z, HPA_L, ny);
+        IV(z, HPA_L, ny);
         for(L i=0;i<ny;i++){ L q = buf[(unsigned char)sC(y)[i]]; sL(z)[i] = q==-1 ? nx : q; }
         R 0;
     }
@@ -1020,7 +1046,7 @@ int32_t orc_pfnMember(OV z, OV y, OV x){           /* h_primitive.cc:2288-2336: 
         if((e=promote(&tx,x,t))) R e;
         if((e=promote(&ty,y,t))) R e;
         L sLen = tx.len, vLen = ty.len;
-        orc_initV(z, HPA_B, vLen);
+        IV(z, HPA_B, vLen);
         if(t == HPA_B){                            /* lib_member_B, h_libs.cc:969-974 */
             int flag[2] = {0,0};
             for(L i=0;i<sLen;i++) flag[sB(&tx)[i]] = 1;
@@ -1056,7 +1082,7 @@ int32_t orc_pfnMember(OV z, OV y, OV x){           /* h_primitive.cc:2288-2336: 
     if(x->typ == y->typ && x->typ == HPA_C){       /* lib_member_C, h_libs.cc:1005-1011 */
         uint8_t buf[256] = {0};
         for(L i=0;i<x->len;i++) buf[(unsigned char)sC(x)[i]] = 1;
-        orc_initV(z, HPA_B, y->len);
+        IV(z, HPA_B, y->len);
         for(L i=0;i<y->len;i++) sB(z)[i] = buf[(unsigned char)sC(y)[i]];
         R 0;
     }

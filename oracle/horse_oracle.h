@@ -18,6 +18,8 @@ typedef hpa_V  OV;
 int32_t orc_init(int64_t heap_bytes, int32_t threads);
 int32_t orc_threads(void);
 int64_t orc_get_heap(void);
+int64_t orc_heap_capacity(void);
+int64_t orc_set_heap_top(int64_t top);
 int64_t orc_set_heap(int64_t cur);
 OV      orc_alloc_node(void);
 OV      orc_initV(OV x, int32_t typ, int64_t len);

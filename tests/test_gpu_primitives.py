@@ -639,3 +639,51 @@ def test_arena_rewind_and_contexts(gpu):
         lib.dll.hpaCtxDestroy(ctx)
     assert np.array_equal(gpu.read(gpu.pfnPlus(x, x)), 2 * np.arange(200_000))
     gpu.release(m)
+
+
+def test_where_one_match_leaves_no_index_bound(gpu):
+    """pfnWhere with at most one match returns an inline scalar and rewinds the arena; the row
+    bound of the freed index buffer must not pass to the next vector allocated at its address,
+    so pfnIndex through that vector still checks its rows (HPA_E_INDEX, code 3)."""
+    x = gpu.vec(np.arange(500, dtype=np.int64))
+    src = gpu.vec(np.array([0, 1, 1000], np.int64))       # row 1000 is out of range for x
+    zero = gpu.scalar(0, H_L)
+    m = gpu.mark()
+    mask2 = np.zeros(64, np.uint8)
+    mask2[[5, 9]] = 1
+    addr = gpu.pfnWhere(gpu.vec(mask2)).contents.g        # where the index buffer lands
+    gpu.release(m)
+    mask1 = np.zeros(64, np.uint8)
+    mask1[5] = 1
+    assert gpu.read(gpu.pfnWhere(gpu.vec(mask1))).tolist() == [5]
+    t = None
+    for _ in range(4096):                                  # 256-byte steps up to that address
+        t = gpu.pfnPlus(src, zero)
+        if t.contents.g >= addr:
+            break
+    assert t.contents.g == addr
+    with pytest.raises(HorseError) as ei:
+        gpu.pfnIndex(x, t)
+    assert ei.value.code == 3
+    gpu.release(m)
+
+
+def test_tohost_copies_and_rewinds(gpu):
+    """hpaToHost copies made before a heap mark survive a rewind to it, copies made after it are
+    released, and a resident payload copied to the host keeps its handle for hpaFreeResident."""
+    dll = gpu.lib.dll
+    one = gpu.scalar(1, H_L)
+    a = gpu.pfnPlus(gpu.vec(np.arange(1000, dtype=np.int64)), one)
+    assert dll.hpaToHost(a) == 0
+    m = gpu.mark()
+    b = gpu.pfnPlus(a, one)
+    assert dll.hpaToHost(b) == 0
+    assert np.array_equal(gpu.read(b), np.arange(1000) + 2)
+    gpu.release(m)
+    assert np.array_equal(gpu.read(a), np.arange(1000) + 1)
+    assert np.array_equal(gpu.read(gpu.pfnPlus(a, one)), np.arange(1000) + 2)
+    gpu.release(m)
+    r = gpu.vec(np.arange(10, dtype=np.int64) * 3)
+    assert dll.hpaToHost(r) == 0
+    assert np.array_equal(gpu.read(r), np.arange(10) * 3)
+    assert dll.hpaFreeResident(r) == 0

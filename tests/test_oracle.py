@@ -251,3 +251,33 @@ def test_q1_matches_numpy(orc):
         assert r["count_order"][gi] == sel.sum()
         assert r["sum_qty"][gi] == t["quantity"][m][sel].astype(np.int64).sum()
         assert abs(r["sum_charge"][gi] - math.fsum(p * (1 - d) * (1 + tx))) <= 1e-12 * r["sum_charge"][gi]
+
+
+def test_heap_full_returns_memory_error(orc):
+    """A primitive whose result does not fit the oracle's heap returns HPA_E_MEMORY (100)
+    instead of writing through a null payload (the reference stops with "Heap full!!",
+    h_memory.cc:110)."""
+    import oracle
+    dll = oracle.load()
+    n = 1 << 20
+    m = orc.vec(np.ones(n, np.uint8))
+    m_np = np.ones(n, np.uint8)
+    m_np[::3] = 0
+    mask = orc.vec(m_np)
+    col = orc.vec(np.arange(n, dtype=np.float64))
+    ints = orc.vec(rng.integers(0, 1000, n).astype(np.int32))
+    mark = orc.mark()
+    old = dll.orc_set_heap_top(dll.orc_get_heap() + (1 << 16))
+    try:
+        for name, args in (("pfnCompress", (mask, col)), ("pfnGroup", (ints,)), ("pfnWhere", (mask,)),
+                           ("pfnMul", (col, col)), ("pfnOrderBy", None)):
+            with pytest.raises(HorseError) as ei:
+                if args is None:
+                    orc.pfnOrderBy(orc.list([ints, col]), orc.bools([True, False]))
+                else:
+                    orc.new(name, *args)
+            assert ei.value.code == 100, name
+    finally:
+        dll.orc_set_heap_top(old)
+        orc.release(mark)
+    assert orc.read(orc.pfnSum(m)).tolist() == [n]
