@@ -19,7 +19,7 @@ ARCH = os.environ.get("HPA_OFFLOAD_ARCH", "gfx950")
 HIPFLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wno-unused-result", "-Wno-unused-variable", "-Wno-unused-function",
+    "-Wno-unused-result", "-Wno-unused-value", "-Wno-unused-variable", "-Wno-unused-function",
 ]
 
 

@@ -1534,6 +1534,8 @@ int32_t is_ordered_dev(hpa_ctx *c, V x, bool *ord){
 }
 
 extern "C" int32_t pfnOrderBy(hpa_V z, hpa_V x, hpa_V y);
+static int32_t order_vector_qsort(hpa_ctx *c, V z, V col, bool up, bool expect_ties);
+bool qsort_ref_type(int32_t t);
 __global__ void k_perm64_to_u32(const int64_t *p, L n, uint32_t *q){
     for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x) q[i] = (uint32_t)p[i];
 }
@@ -1566,6 +1568,18 @@ extern "C" int32_t pfnGroupBasic(hpa_V z, hpa_V x){
     }
     if(is_basic(x->typ)){
         if(x->typ == HPA_S || x->typ == HPA_X || x->typ == HPA_Q) R HPA_E_NOT_IMPL;
+        if(qsort_ref_type(x->typ)){
+            // pfnOrderBy of the vector with a true flag (h_primitive.cc:1017-1020): a group's keys
+            // repeat, so the reference's partitions run at once
+            V0 t; memset(&t, 0, sizeof(t)); int32_t e;
+            if((e = order_vector_qsort(c, &t, x, true, true))) R e;
+            L n = x->len;
+            uint32_t *perm = (uint32_t*)dalloc(c, 4 * (n > 0 ? n : 1)); if(!perm) R HPA_E_MEMORY;
+            if(n > 1) hipLaunchKernelGGL(k_perm64_to_u32, dim3(grid_for(n, 256, 4)), dim3(256), 0, c->stream, (const int64_t*)t.g, n, perm);
+            else if(n == 1 && (e = iota_u32(c, perm, 1))) R e;
+            if((e = check_launch("group"))) R e;
+            R runs_along_perm(c, z, 1, x, perm, n);
+        }
         R group_along_order(c, z, x, 1, x, x->len);
     }
     R HPA_E_DOMAIN;
@@ -1871,6 +1885,70 @@ __global__ void k_char_small(const char *s, int64_t *z, int n, int desc){
     for(int i = 0; i < n; i++) z[i] = r[i];
 }
 
+// the reference's quicksort restated on the device (hpa_qsort.hip)
+int32_t qsort_ref_perm(hpa_ctx *c, int32_t typ, const void *col, uint32_t *perm, L n, const uint32_t *seg_lo,
+                       const uint32_t *seg_hi, L nseg, bool up);
+int32_t order_list_ref(hpa_ctx *c, int nc, const int32_t *typ, const void *const *cols, const uint8_t *flags, L n,
+                       uint32_t **perm_out);
+
+// whether two rows adjacent in perm are equal on every column (flag[0]), and whether an F/E
+// column holds a NaN (flag[1])
+__global__ void __launch_bounds__(256) k_full_ties(SmallCols s, const uint32_t *perm, L n, int *flag){
+    bool tie = false, nan = false;
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
+        const L r = perm ? perm[i] : i, r0 = i > 0 ? (perm ? perm[i - 1] : i - 1) : 0;
+        bool eq = i > 0;
+        for(int q = 0; q < s.nc; q++){
+            if(s.typ[q] == HPA_E){ const double v = ((const double*)s.p[q])[r]; nan |= v != v; }
+            if(s.typ[q] == HPA_F){ const float v = ((const float*)s.p[q])[r]; nan |= v != v; }
+            if(eq) eq = okey_any(s.typ[q], s.p[q], r0) == okey_any(s.typ[q], s.p[q], r);
+        }
+        tie |= eq;
+    }
+    if(__ballot(tie) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+    if(__ballot(nan) && (threadIdx.x & 63) == 0) atomicOr(flag + 1, 1);
+}
+// *ties: some adjacent rows of perm tie on every column; HPA_E_DOMAIN for a NaN key
+static int32_t full_ties(hpa_ctx *c, int nc, V cols, const uint32_t *perm, L n, bool *ties){
+    int32_t e;
+    *ties = false;
+    if(n < 1) R 0;
+    SmallCols s; memset(&s, 0, sizeof(s)); s.nc = nc;
+    for(int q = 0; q < nc; q++){ const void *p; if((e = dev_view(c, cols + q, &p))) R e; s.typ[q] = cols[q].typ; s.p[q] = p; }
+    int *f = (int*)dalloc(c, 16); if(!f) R HPA_E_MEMORY;
+    hipMemsetAsync(f, 0, 8, c->stream);
+    hipLaunchKernelGGL(k_full_ties, dim3(grid_for(n, 256, 4)), dim3(256), 0, c->stream, s, perm, n, f);
+    if((e = check_launch("order_ties"))) R e;
+    int h[2]; if((e = d2h(c, h, f, 8))) R e;
+    if(h[1]){ set_error_detail("order: NaN key in a quicksort-ordered column"); R HPA_E_DOMAIN; }
+    *ties = h[0] != 0;
+    R 0;
+}
+// A column ordered by comparison in the reference (lib_quicksort_other).  Without ties every
+// correct order is its order, so the stable sort on the order key answers; keys with ties run
+// the reference's partitions (hpa_qsort.hip) from the identity, as lib_order_by_vector starts.
+// expect_ties (groups, whose keys repeat by design) goes to the partitions directly.
+static int32_t order_vector_qsort(hpa_ctx *c, V z, V col, bool up, bool expect_ties){
+    L n = col->len; int32_t e;
+    const void *p = nullptr;
+    if(n > 0 && (e = dev_view(c, col, &p))) R e;
+    const L mark = c->dcur;
+    if(!expect_ties || n < 2){
+        V0 lst; memset(&lst, 0, sizeof(lst)); lst.typ = HPA_G; lst.len = 1; lst.g = (char*)col;
+        uint8_t f = up ? 1 : 0; uint32_t *perm; bool ties, known;
+        if((e = order_perm(c, &lst, &f, &perm, &ties, &known))) R e;
+        bool t2; if((e = full_ties(c, 1, col, perm, n, &t2))) R e;
+        if(!t2) R order_emit(c, z, perm, n, false);
+        arena_set(c, mark);
+    } else {
+        bool t2; if((e = full_ties(c, 1, col, nullptr, n, &t2))) R e;      // the NaN check
+    }
+    uint32_t *perm = (uint32_t*)dalloc(c, 4 * n); if(!perm) R HPA_E_MEMORY;
+    if((e = iota_u32(c, perm, n))) R e;
+    if((e = qsort_ref_perm(c, col->typ, p, perm, n, nullptr, nullptr, 0, up))) R e;
+    R order_emit(c, z, perm, n, false);
+}
+
 static int32_t order_vector(hpa_ctx *c, V z, V col, bool up){   // lib_order_by_vector, h_libs.cc:730-736
     L n = col->len; int32_t e;
     if(col->typ == HPA_I) R order_radix_i32(c, z, col, up);
@@ -1897,12 +1975,8 @@ static int32_t order_vector(hpa_ctx *c, V z, V col, bool up){   // lib_order_by_
         hipLaunchKernelGGL(k_okey_perm<uint8_t>, dim3(grid_for(n, 256, 4)), dim3(256), 0, c->stream, (const uint8_t*)p, (const uint32_t*)nullptr, k, n, 0, 8);
         R order_radix_out<uint64_t>(c, z, k, n, 8, up);
     }
-    // comparison-sorted types (quicksort in the reference): stable sort on the order key
-    V0 lst; memset(&lst, 0, sizeof(lst)); lst.typ = HPA_G; lst.len = 1; lst.g = (char*)col;
-    uint8_t f = up ? 1 : 0; uint32_t *perm; bool ties;
-    bool known;
-    if((e = order_perm(c, &lst, &f, &perm, &ties, &known))) R e;
-    R order_emit(c, z, perm, n, false);
+    if(qsort_ref_type(col->typ)) R order_vector_qsort(c, z, col, up, false);
+    R HPA_E_NOT_IMPL;
 }
 
 // ---------------------------------------------------------------- list order: tie direction
@@ -2074,7 +2148,20 @@ extern "C" int32_t pfnOrderBy(hpa_V z, hpa_V x, hpa_V y){
         L n = nc == 0 ? 0 : a[0].len;
         uint32_t *perm = nullptr; bool ties, known;
         if(nc == 0 || n == 0){ void *dz; R out_init(c, z, HPA_L, 0, &dz); }
+        // columns sorted by lib_quicksort_other or lib_quicksort_char: with rows tied on every
+        // column the reference's column-by-column passes decide their order (hpa_qsort.hip)
+        bool ref_cols = false;
+        for(L q = 0; q < nc; q++) ref_cols |= qsort_ref_type(a[q].typ) || a[q].typ == HPA_C;
         if((e = order_perm(c, x, fl.data(), &perm, &ties, &known))) R e;
+        if(ref_cols){
+            bool t2 = true;
+            if(nc <= SMALL_COLS && (e = full_ties(c, (int)nc, a, perm, n, &t2))) R e;
+            if(!t2) R order_emit(c, z, perm, n, false);          // the lexicographic order is unique
+            std::vector<int32_t> ty((size_t)nc); std::vector<const void*> cp((size_t)nc);
+            for(L q = 0; q < nc; q++){ ty[q] = a[q].typ; if((e = dev_view(c, a + q, &cp[q]))) R e; }
+            if((e = order_list_ref(c, (int)nc, ty.data(), cp.data(), fl.data(), n, &perm))) R e;
+            R order_emit(c, z, perm, n, false);
+        }
         if((e = order_list_ties(c, x, fl.data(), &perm, n, known ? (ties ? 1 : 0) : -1))) R e;
         R order_emit(c, z, perm, n, false);
     }

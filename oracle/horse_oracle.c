@@ -700,7 +700,17 @@ static int32_t order_basic(OV z, OV x, OV y){      /* lib_order_basic, h_primiti
     R 1;
 }
 
+/* A NaN key makes lib_partition's j scan leave its segment (a NaN pivot compares greater than
+ * itself, h_libs.cc:419), so orders and groups over F/E columns holding NaN are rejected with
+ * E_DOMAIN instead (defined behaviour chosen where the reference's is undefined). */
+static int has_nan(OV v){
+    if(v->typ == HPA_E){ for(L i=0;i<v->len;i++) if(sE(v)[i] != sE(v)[i]) R 1; }
+    if(v->typ == HPA_F){ for(L i=0;i<v->len;i++) if(sF(v)[i] != sF(v)[i]) R 1; }
+    R 0;
+}
 int32_t orc_pfnOrderBy(OV z, OV x, OV y){          /* h_primitive.cc:1769-1816 */
+    if(x->typ==HPA_G){ for(L i=0;i<x->len;i++) if(has_nan(vV(x,i))) R HPA_E_DOMAIN; }
+    else if(has_nan(x)) R HPA_E_DOMAIN;
     if(x->typ==HPA_G && y->typ==HPA_B && x->len==y->len){
         for(L i=0;i<x->len;i++) if(!is_basic(vV(x,i)->typ)) R HPA_E_DOMAIN;
         if(x->len > 0){ L n=vV(x,0)->len; for(L i=0;i<x->len;i++) if(vV(x,i)->len!=n) R HPA_E_MATCH; }

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import kat
-from horsepower_amd.vnode import H_B, H_C, H_E, H_I, H_L, HorseError
+from horsepower_amd.vnode import H_B, H_C, H_D, H_E, H_I, H_L, HorseError
 
 pytestmark = pytest.mark.gpu
 RTOL = 1e-12
@@ -355,21 +355,30 @@ def test_group_list_and_float(gpu, orc):
     zg = gpu.pfnGroup(gpu.list([gpu.vec(a), gpu.vec(b)]))
     zo = orc.pfnGroup(orc.list([orc.vec(a), orc.vec(b)]))
     assert groups_of(gpu, zg) == groups_of(orc, zo)
-    d = r.permutation(2000).astype(np.float64) / 3          # distinct keys: no quicksort ties
+    d = r.permutation(2000).astype(np.float64) / 3          # distinct keys
     assert groups_of(gpu, gpu.pfnGroup(gpu.vec(d))) == groups_of(orc, orc.pfnGroup(orc.vec(d)))
+    for n in (3000, 300_000):                                # keys with ties: the quicksort's order
+        t = r.integers(0, 50, n) / 4.0
+        assert groups_of(gpu, gpu.pfnGroup(gpu.vec(t))) == groups_of(orc, orc.pfnGroup(orc.vec(t)))
+        dt = (19940101 + r.integers(0, 500, n)).astype(np.int32)
+        assert groups_of(gpu, gpu.pfnGroup(gpu.vec(dt, H_D))) == groups_of(orc, orc.pfnGroup(orc.vec(dt, H_D)))
+        zg = gpu.pfnGroup(gpu.list([gpu.vec(t), gpu.vec(dt, H_D)]))
+        zo = orc.pfnGroup(orc.list([orc.vec(t), orc.vec(dt, H_D)]))
+        assert groups_of(gpu, zg) == groups_of(orc, zo)
 
 
 @pytest.mark.parametrize("n", [0, 1, 9, 10, 5000, 300001])
 @pytest.mark.parametrize("up", [1, 0])
 def test_order_single(gpu, orc, n, up):
     r = rng(n + 13 + up)
-    for x in (r.integers(-50, 50, n).astype(np.int32), r.integers(-2**40, 2**40, n).astype(np.int64),
-              r.integers(0, 1000, n).astype(np.int64), r.permutation(n).astype(np.float64) - n / 2,
-              r.integers(65, 91, n).astype(np.uint8)):
-        typ = H_C if x.dtype == np.uint8 else None
+    for x, typ in ((r.integers(-50, 50, n).astype(np.int32), None), (r.integers(-2**40, 2**40, n).astype(np.int64), None),
+                   (r.integers(0, 1000, n).astype(np.int64), None), (r.permutation(n).astype(np.float64) - n / 2, None),
+                   (r.integers(65, 91, n).astype(np.uint8), H_C),
+                   (r.integers(0, 40, n) / 8.0, None),                                  # f64 with ties
+                   ((19940101 + r.integers(0, 300, n)).astype(np.int32), H_D)):         # dates with ties
         g = gpu.read(gpu.pfnOrderBy(gpu.vec(x, typ), gpu.bools([up])))
         o = orc.read(orc.pfnOrderBy(orc.vec(x, typ), orc.bools([up])))
-        assert np.array_equal(g, o), x.dtype
+        assert np.array_equal(g, o), (x.dtype, typ)
 
 
 @pytest.mark.parametrize("n", [2, 1000, 1025, 30000, 131072, 131073, 200000])
@@ -386,6 +395,12 @@ def test_order_lists(gpu, orc, n):
     g = gpu.read(gpu.pfnOrderBy(gpu.list([gpu.vec(rev), gpu.vec(day)]), gpu.bools([0, 1])))
     o = orc.read(orc.pfnOrderBy(orc.list([orc.vec(rev), orc.vec(day)]), orc.bools([0, 1])))
     assert np.array_equal(g, o)
+    rev_t = np.round(r.integers(0, max(n // 4, 2), n) * 0.5, 1)   # tied revenues, tied dates
+    dte = (19950101 + r.integers(0, 60, n)).astype(np.int32)
+    for fl in ([0, 1], [1, 0]):
+        g = gpu.read(gpu.pfnOrderBy(gpu.list([gpu.vec(rev_t), gpu.vec(dte, H_D)]), gpu.bools(fl)))
+        o = orc.read(orc.pfnOrderBy(orc.list([orc.vec(rev_t), orc.vec(dte, H_D)]), orc.bools(fl)))
+        assert np.array_equal(g, o), fl
     c1 = r.integers(0, 30, n).astype(np.int64)
     c2 = r.permutation(n).astype(np.int64)
     g = gpu.read(gpu.pfnOrderBy(gpu.list([gpu.vec(c1), gpu.vec(c2)]), gpu.bools([1, 0])))
