@@ -58,6 +58,8 @@ void note_group_offsets(hpa_ctx *c, const void *counts, L g, const int64_t *offs
 int32_t resident_alloc(V x, int32_t typ, L len);   // long-lived HBM payload (hpaFreeResident)
 
 hpa_ctx *ctx_cur();
+bool     hpa_is_initialized();
+int32_t  noalloc_scratch_init(int device);        // address of hpaSumNoAlloc's static scratch
 void    *dalloc(hpa_ctx *c, L bytes);               // nullptr when the arena is full
 void    *halloc(hpa_ctx *c, L bytes);
 V        node_alloc(hpa_ctx *c);

@@ -201,6 +201,61 @@ extern "C" int32_t pfnSum(hpa_V z, hpa_V x){
     R 0;
 }
 
+// ---------------------------------------------------------------- allocation-free FP64 sum
+// hpaSumNoAlloc: pfnSum's kernels for an E vector, with the workgroup partials and the final
+// value's staging in `scratch`, a file-scope device array instead of arena memory.  Host code
+// never passes the array's name to a kernel (on the host it is only a stand-in for the symbol):
+// noalloc_scratch_init resolves the array's address on each device with hipGetSymbolAddress, with
+// that device current, and the kernels get that address.  The partials are written whole by every
+// call before they are read, so nothing needs clearing; calls that overlap on one GPU race on it.
+#define NOALLOC_PARTS 4096
+static __device__ __align__(256) KS scratch[NOALLOC_PARTS];
+#define NOALLOC_MAX_DEV 64
+static KS *g_scratch_dev[NOALLOC_MAX_DEV];
+
+int32_t noalloc_scratch_init(int device){
+    if(device < 0 || device >= NOALLOC_MAX_DEV) R HPA_E_DEVICE;
+    if(g_scratch_dev[device]) R 0;
+    void *ptr = nullptr;
+    const hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if(err != hipSuccess || !ptr){
+        set_error_detail("hipGetSymbolAddress(scratch) failed on device %d: %s", device, hipGetErrorString(err));
+        R HPA_E_DEVICE;
+    }
+    g_scratch_dev[device] = (KS*)ptr;
+    R 0;
+}
+// the value k_sum_f and k_sum_f_final give for zero or one element: (0 + v) + 0
+__global__ void k_store_sum1(double *out, double v){ *out = (0.0 + v) + 0.0; }
+
+extern "C" int32_t hpaSumNoAlloc(hpa_V x, double *out){
+    if(!hpa_is_initialized()){ set_error_detail("hpaSumNoAlloc: call hpaInit first"); R HPA_E_DOMAIN; }
+    if(!x || !out) R HPA_E_DOMAIN;
+    if(x->typ != HPA_E){ set_error_detail("hpaSumNoAlloc: an E (f64) vector is required"); R HPA_E_TYPE; }
+    hpa_ctx *c = ctx_cur();
+    int dev = 0;
+    if(hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEV || !g_scratch_dev[dev]){
+        set_error_detail("hpaSumNoAlloc: no scratch resolved for the current device");
+        R HPA_E_DEVICE;
+    }
+    const L n = x->len;
+    if(n <= 0 || is_inline(x)){                           // no element, or one held in the node
+        hipLaunchKernelGGL(k_store_sum1, dim3(1), dim3(1), 0, c->stream, out, n > 0 ? x->f64 : 0.0);
+        R check_launch("sum_noalloc");
+    }
+    if(!is_device(x->g)){ set_error_detail("hpaSumNoAlloc: the vector's payload must be in HBM"); R HPA_E_DOMAIN; }
+    if(((uintptr_t)x->g & 31) != 0){ set_error_detail("hpaSumNoAlloc: payload not 32-byte aligned"); R HPA_E_DOMAIN; }
+    const int G = sum_grid(n);
+    if(G > NOALLOC_PARTS){
+        set_error_detail("hpaSumNoAlloc: %d workgroup partials exceed the %d the static scratch holds", G, NOALLOC_PARTS);
+        R HPA_E_MEMORY;
+    }
+    KS *part = g_scratch_dev[dev];
+    hipLaunchKernelGGL(k_sum_f<double>, dim3(G), dim3(RD_BLOCK), 0, c->stream, (const double*)x->g, n, part);
+    hipLaunchKernelGGL(k_sum_f_final, dim3(1), dim3(SUM_FINAL), 0, c->stream, (const KS*)part, G, out);
+    R check_launch("sum_noalloc");
+}
+
 // pfnAvg (h_primitive.cc:788-800)
 extern "C" int32_t pfnAvg(hpa_V z, hpa_V x){
     V0 t; memset(&t, 0, sizeof(t));

@@ -109,9 +109,12 @@ extern "C" int32_t hpaInit(int32_t device, int64_t arena_bytes){
     if(env && arena_bytes <= 0) arena_bytes = atoll(env);
     g_default = ctx_new(arena_bytes, 256LL << 20);
     if(!g_default){ set_error_detail("default context allocation failed"); R HPA_E_MEMORY; }
+    int32_t e = noalloc_scratch_init(device);
+    if(e){ hpaCtxDestroy(g_default); g_default = nullptr; R e; }
     g_init = true;
     R 0;
 }
+bool hpa_is_initialized(){ R g_init; }
 
 extern "C" void hpaCtxDestroy(hpa_ctx *c){
     if(!c) R;

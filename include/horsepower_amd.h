@@ -284,6 +284,17 @@ int32_t hpaCompressMany(int32_t ny, hpa_V *z, hpa_V mask, hpa_V *y);
  * kind 0 sum, 1 avg, 2 min, 3 max, 4 len.  z is a vector over groups. */
 int32_t hpaGroupReduce(hpa_V z, int32_t kind, hpa_V x, hpa_V y);
 
+/* ---- allocation-free entry point -----------------------------------------------------------
+ * No counterpart in the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * hpaSumNoAlloc is pfnSum (h_primitive.cc:764-786) of an E vector whose payload is in HBM: the
+ * same kernels and the same FP64 result, written to the device address `out` on the current
+ * context's stream, with no allocation and no host synchronisation.  Its workgroup partials live
+ * in a static device scratch of 4096 partials (64 KiB per GPU, held until the process ends);
+ * a call that would need more returns HPA_E_MEMORY.  hpaInit must have run on the device.  Calls
+ * that overlap on one GPU share that scratch and race. */
+int32_t hpaSumNoAlloc(hpa_V x, double *out);
+
 /* ---- synthetic tables generated in HBM (benchmarks, serving): counter-based splitmix64
  * columns, the values of horsepower_amd/datagen.py.  Payloads are resident (hpaFreeResident). */
 int32_t hpaGenUniform(hpa_V z, int32_t typ, int64_t n, int64_t start, uint64_t seed, uint64_t stream,
