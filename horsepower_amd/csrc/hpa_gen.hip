@@ -59,13 +59,16 @@ __global__ void k_gen_owner(L n, const int64_t *per, const int64_t *off, int64_t
     for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x)
         for(int64_t k = 0; k < per[i]; k++) owner[off[i] + k] = i;
 }
+// days from an order to its lineitems' shipping (datagen.Q3_SHIP_SPAN: about 30K result groups
+// of the join + group query per 1.5M orders)
+#define GEN_SHIP_SPAN 318
 struct FactOut { int32_t *fk, *ship, *qty; double *price, *disc, *tax; char *flag, *status; };
 struct FactSeeds { uint64_t s15, s16, s17, s18, s19, s20, s21; };
 __global__ void k_gen_fact(L nf, const int64_t *owner, const int32_t *key, const int32_t *day, FactSeeds s, FactOut o){
     for(L r = (L)blockIdx.x * blockDim.x + threadIdx.x; r < nf; r += (L)gridDim.x * blockDim.x){
         const uint64_t c = (uint64_t)r + 1;
         const int64_t w = owner[r];
-        const int32_t ship = day[w] + (int32_t)sm_uniform(s.s15, c, 1, 121);
+        const int32_t ship = day[w] + (int32_t)sm_uniform(s.s15, c, 1, GEN_SHIP_SPAN);
         o.fk[r] = key[w];
         o.ship[r] = ship;
         o.price[r] = (double)sm_uniform(s.s16, c, 90000, 10500000 - 90000 + 1) / 100.0;
