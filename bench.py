@@ -63,7 +63,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="q1", choices=["q1", "q6", "q3", "batch", "prims", "serve", "copy"])
+    p.add_argument("--workload", default="all", choices=["all", "q1", "q6", "q3", "batch", "prims", "serve", "copy"])
     p.add_argument("--rows", type=int, default=0)
     p.add_argument("--plan", default="best", choices=["best", "fused", "plain"])
     p.add_argument("--cpu-rows", type=int, default=0)
@@ -98,34 +98,38 @@ def run_steps(B, plan, tables, steps, mark):
         B.release(mark)
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {a.gpus}); this build runs on a single "
-              "MI355X and has no multi-GPU path", file=sys.stderr)
-        sys.exit(2)
-    import ctypes as C
-    import numpy as np
-    from horsepower_amd import queries as Q
-    from horsepower_amd.backend import GpuBackend
+def free_tables(B, tables):
+    """Release the resident column payloads of the given tables (dicts of column nodes)."""
+    for T in tables:
+        for node in T.values():
+            v = node.contents
+            if v.len > 1 and v.g and B.lib.dll.hpaIsDevicePtr(v.g):
+                B.lib.dll.hpaFreeResident(node)
 
-    if a.workload == "batch":
-        return bench_batch(a)
-    if a.workload == "prims":
-        return bench_prims(a)
-    if a.workload == "serve":
-        return bench_serve(a)
-    if a.workload == "copy":
-        return bench_copy(a)
-    W = workload(a.workload, a.rows)
-    t0 = time.time()
-    cols = W["gen"](W["n"] if W["n"] else W["nd"])
-    print(f"[bench] generated {a.workload} in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-    B = GpuBackend()
+
+HEAD = {"metric": "rows/s + % HBM roofline per query; batched queries/s & p99 latency vs plain",
+        "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None}
+
+
+def bench_query(a, B, name, rows=0, tables=None, desc=None, cpu=True, plain=True):
+    """One query workload (q1 / q6 / q3): K timed steps over resident inputs, a kernel-timing
+    pass, the interpreter sequence on the GPU and the oracle on a bounded sample."""
+    import ctypes as C
+    W = workload(name, rows)
+    own = tables is None
+    if own:
+        t0 = time.time()
+        cols = W["gen"](W["n"] if W["n"] else W["nd"])
+        print(f"[bench] generated {name} in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+        from horsepower_amd import queries as Q
+        tables = tuple(Q.load_table(B, c) for c in cols)
+        nrows = W["n"] if W["n"] else sum(len(next(iter(c.values()))) for c in cols)
+        del cols
+    else:
+        nrows = sum(next(iter(T.values())).contents.len for T in tables) if not W["n"] else W["n"]
+        if name == "q3":
+            nrows = sum(T[k].contents.len for T, k in zip(tables, ("key", "fk")))
     lib = B.lib
-    tables = tuple(Q.load_table(B, c) for c in cols)
-    rows = W["n"] if W["n"] else sum(len(next(iter(c.values()))) for c in cols)
-    del cols
     plan = W["plans"][a.plan]
     B.sync()
     mark = B.mark()
@@ -136,62 +140,98 @@ def main():
     B.sync()
     dt = time.perf_counter() - t0
     ms = dt * 1000.0 / a.steps
-    value = rows * a.steps / dt
-    print(f"[bench] timed {a.steps} steps: {ms:.3f} ms/step", file=sys.stderr, flush=True)
-
-    # kernel timing pass (events on every named launch)
-    lib.dll.hpaProfileReset()
-    lib.dll.hpaProfileEnable(1)
-    run_steps(B, plan, tables, a.profile_steps, mark)
-    B.sync()
-    lib.dll.hpaProfileEnable(0)
-    buf = C.create_string_buffer(1 << 16)
-    lib.dll.hpaProfileList(buf, 1 << 16)
-    kern = {}
-    for line in buf.value.decode().splitlines():
-        name, cnt, tms, byt = line.split()
-        kern[name] = dict(count=int(cnt), ms=float(tms), bytes=float(byt))
+    print(f"[bench] {name} {nrows} rows: {a.steps} steps, {ms:.3f} ms/step", file=sys.stderr, flush=True)
+    kern = profile_kernels(lib, lambda: run_steps(B, plan, tables, 1, mark), a.profile_steps)
     dom = max(kern, key=lambda k: kern[k]["ms"]) if kern else None
     roof = None
     if dom:
-        k = kern[dom]
-        avg_ms = k["ms"] / k["count"]
-        per_launch = k["bytes"] / k["count"]
-        achieved = per_launch / (avg_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                "traffic": measured_traffic(a.workload, rows, dom),
-                "bytes_per_launch": per_launch, "avg_launch_us": round(avg_ms * 1e3, 2)}
+        roof = roofline_of(kern, dom)
+        roof["traffic"] = measured_traffic(name, nrows, dom)
     per_kernel = {k: {"launches_per_step": v["count"] / a.profile_steps,
                       "us_per_step": round(v["ms"] * 1e3 / a.profile_steps, 2),
-                      "GBs": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None}
+                      "GBs": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 and v["bytes"] > 0 else None}
                   for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["ms"])}
-
-    # the interpreter's per-primitive sequence on the GPU, for comparison
-    plain = None
-    if a.plan != "plain" and not a.no_plain:
-        pl = W["plans"]["plain"]
-        run_steps(B, pl, tables, 1, mark)
+    pl = None
+    if plain and a.plan != "plain" and not a.no_plain:
+        pp = W["plans"]["plain"]
+        run_steps(B, pp, tables, 1, mark)
         B.sync()
         t1 = time.perf_counter()
         ps = max(2, a.steps // 4)
-        run_steps(B, pl, tables, ps, mark)
+        run_steps(B, pp, tables, ps, mark)
         B.sync()
         pdt = (time.perf_counter() - t1) / ps
-        plain = {"ms_per_step": round(pdt * 1e3, 3), "rows_per_s": rows / pdt}
+        pl = {"ms_per_step": round(pdt * 1e3, 3), "rows_per_s": nrows / pdt}
+    cb = cpu_baseline(a, W) if cpu and not a.no_cpu else None
+    B.release(mark)
+    if own:
+        free_tables(B, tables)
+    return {"value": nrows * a.steps / dt, "unit": "rows/s", "steps": a.steps, "warmup": a.warmup,
+            "ms_per_step": round(ms, 4), "dtype": "f64", "data": "synthetic (seeded counter-based generator)",
+            "config": {"workload": desc or W["desc"], "rows": nrows, "plan": a.plan},
+            "roofline": roof, "cpu_baseline": cb, "plain_gpu": pl, "kernels": per_kernel}
 
-    cpu = None
-    if not a.no_cpu:
-        cpu = cpu_baseline(a, W)
 
-    out = {
-        "metric": "rows/s + % HBM roofline per query; batched queries/s & p99 latency vs plain",
-        "value": value, "unit": "rows/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded counter-based generator)",
-        "config": {"workload": W["desc"], "rows": rows, "plan": a.plan},
-        "roofline": roof, "cpu_baseline": cpu, "plain_gpu": plain, "kernels": per_kernel,
-    }
+def bench_all(a):
+    """The default run: the 60M-row group-by query as the headline line, and every other
+    workload of the metric measured in the same process as sub-objects: the filter query at 6M
+    rows, the join query at 15M and 60M lineitems, the 64-query batch and the serving stream
+    over 600M + 150M rows, and each primitive alone."""
+    os.environ.setdefault("HPA_ARENA_BYTES", str(96 << 30))
+    from horsepower_amd.backend import GpuBackend
+    B = GpuBackend()
+    t_all = time.time()
+    head = bench_query(a, B, "q1", a.rows or 60_000_000)
+    subs = {}
+    q6 = workload("q6", 6_000_000)
+    subs["q6_6M"] = bench_query(a, B, "q6", 6_000_000)
+    for nd, tag in ((3_750_000, "q3_15M"), (15_000_000, "q3_60M")):
+        dim, fact = B.gen_serve_tables(nd, 1002)
+        nf = fact["fk"].contents.len
+        subs[tag] = bench_query(a, B, "q3", nd * 4, tables=(dim, fact), cpu=(tag == "q3_15M"),
+                                desc=f"q3 join+group+top-10, {nd} orders, {nf} lineitems (generated in HBM)")
+        free_tables(B, (dim, fact))
+    print(f"[bench] query workloads done at {time.time() - t_all:.0f}s", file=sys.stderr, flush=True)
+    subs["batch_64"] = bench_batch(a, B, cpu_budget=10.0)
+    print(f"[bench] batch done at {time.time() - t_all:.0f}s", file=sys.stderr, flush=True)
+    subs["serve_600M"] = bench_serve(a, B)
+    print(f"[bench] serve done at {time.time() - t_all:.0f}s", file=sys.stderr, flush=True)
+    subs["prims"] = bench_prims(a, B, cpu_budget=1.5)
+    print(f"[bench] all workloads in {time.time() - t_all:.0f}s", file=sys.stderr, flush=True)
+    for v in subs.values():
+        for k in HEAD:
+            v.pop(k, None)
+    out = dict(HEAD)
+    out.update(head)
+    out["workloads"] = subs
+    del q6
+    return out
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {a.gpus}); this build runs on a single "
+              "MI355X and has no multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+    from horsepower_amd.backend import GpuBackend
+    if a.workload == "all":
+        out = bench_all(a)
+    elif a.workload == "batch":
+        out = bench_batch(a)
+    elif a.workload == "prims":
+        out = bench_prims(a)
+    elif a.workload == "serve":
+        out = bench_serve(a)
+    elif a.workload == "copy":
+        out = bench_copy(a)
+    else:
+        if a.workload == "q3":
+            os.environ.setdefault("HPA_ARENA_BYTES", str(64 << 30))
+        out = dict(HEAD)
+        out.update(bench_query(a, GpuBackend(), a.workload, a.rows))
+    for k, v in HEAD.items():
+        out.setdefault(k, v)
     print(json.dumps(out), flush=True)
 
 
@@ -232,7 +272,7 @@ def pct(xs, p):
     return float(np.percentile(np.asarray(xs), p))
 
 
-def bench_batch(a):
+def bench_batch(a, B=None, cpu_budget=25.0):
     import numpy as np
     from horsepower_amd import datagen as D, queries as Q
     from horsepower_amd.backend import Batch, GpuBackend
@@ -242,7 +282,8 @@ def bench_batch(a):
     cols = [D.gen_batch_query(q, int(ns[q]), int(gs[q])) for q in range(nq)]
     rows = int(ns.sum())
     print(f"[bench] generated {nq} queries, {rows} rows in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-    B = GpuBackend()
+    if B is None:
+        B = GpuBackend()
     tables = [Q.load_table(B, c) for c in cols]
     B.sync()
     b = Batch(B, tables, thr)
@@ -298,7 +339,7 @@ def bench_batch(a):
         om = ob.mark()
         k = 0
         t0 = time.perf_counter()
-        while k < nq and time.perf_counter() - t0 < 25.0:
+        while k < nq and time.perf_counter() - t0 < cpu_budget:
             Q.batch_plain(ob, Q.load_table(ob, cols[k]), thr[k])
             ob.release(om)
             k += 1
@@ -325,8 +366,9 @@ def bench_batch(a):
                         "us_per_step": round(v["ms"] * 1e3 / a.profile_steps, 2)} for k, v in kern.items()},
         "autotune_ms": [round(x, 4) for x in tune],
     }
-    print(json.dumps(out), flush=True)
     b.close()
+    free_tables(B, tables)
+    return out
 
 
 def prims_cases(B, n, gpu):
@@ -373,15 +415,17 @@ def prims_cases(B, n, gpu):
         cond6 = Q.q6_cond()
         cases.append(("fused_q6_reduce", 24 * n, lambda: B.fused_reduce(0, [T["ship_day"], T["quantity"], T["discount"], T["price"]],
                                                                         cond6, expr([("COL", 3), ("COL", 2), "MUL"]))))
-    return cases, {"ns": ns, "nb": nb, "npb": npb}
+    return cases, {"ns": ns, "nb": nb, "npb": npb,
+                   "tables": [T, {"key": key, "build": build, "probe": probe, "idx": idx}]}
 
 
-def bench_prims(a):
+def bench_prims(a, B=None, cpu_budget=5.0):
     import math
     from horsepower_amd.backend import GpuBackend
-    n = a.rows or 60_000_000
+    n = 60_000_000 if (B is not None or not a.rows) else a.rows
     t0 = time.time()
-    B = GpuBackend()
+    if B is None:
+        B = GpuBackend()
     cases, sz = prims_cases(B, n, True)
     print(f"[bench] prims inputs ready in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     lib = B.lib
@@ -416,7 +460,7 @@ def bench_prims(a):
         print(f"[bench] {name:20s} {ms * 1e3:9.1f} us {gbs:8.1f} GB/s {gbs / PEAK_HBM_GBS:6.1%}  {inner}", file=sys.stderr, flush=True)
     cpu = None
     if not a.no_cpu:
-        cpu = prims_cpu(a, res)
+        cpu = prims_cpu(a, res, cpu_budget)
     gm = math.exp(sum(math.log(v["GBs"]) for v in res.values()) / len(res))
     out = {"metric": "rows/s + % HBM roofline per query; batched queries/s & p99 latency vs plain",
            "value": round(gm, 1), "unit": "GB/s (geometric mean over primitives)", "n_gpus": 1,
@@ -426,10 +470,12 @@ def bench_prims(a):
            "config": {"workload": f"each primitive alone: {n}-row columns, {sz['ns']}-row sorts/groups, "
                                   f"join/index_of/member {sz['nb']} x {sz['npb']}", "rows": n},
            "cpu_baseline": cpu, "primitives": res}
-    print(json.dumps(out), flush=True)
+    B.release(mark)
+    free_tables(B, sz["tables"])
+    return out
 
 
-def prims_cpu(a, res):
+def prims_cpu(a, res, budget=5.0):
     """The same primitives on the oracle (C + OpenMP restatement of the reference's, all host
     cores; the reference's serial phases stay serial) over a smaller sample: one discarded call,
     then the mean of up to three.  Adds cpu_us / cpu_GBs / gpu_over_cpu to each primitive."""
@@ -446,7 +492,7 @@ def prims_cpu(a, res):
         ob.release(mark)
         times = []
         t_start = time.perf_counter()
-        while len(times) < 3 and time.perf_counter() - t_start < 5.0:
+        while len(times) < 3 and time.perf_counter() - t_start < budget:
             t1 = time.perf_counter()
             fn()
             times.append(time.perf_counter() - t1)
@@ -464,7 +510,7 @@ def prims_cpu(a, res):
                                       f"{sz['nb']} x {sz['npb']}; one discarded call, mean of up to three"}
 
 
-def bench_serve(a):
+def bench_serve(a, B=None):
     import numpy as np
     from horsepower_amd import serve as S
     from horsepower_amd.backend import GpuBackend
@@ -473,7 +519,8 @@ def bench_serve(a):
     # Q1 over 600M rows (about 50 GiB); the serving pool takes most of the rest of the 288 GB next
     # to the 24.6 GB of tables, room for three Q3s at once (about 39 GiB each)
     os.environ.setdefault("HPA_ARENA_BYTES", str(96 << 30))
-    B = GpuBackend()
+    if B is None:
+        B = GpuBackend()
     t0 = time.time()
     dim, fact = B.gen_serve_tables(n_dim, 1004)
     T = {"dim": dim, "fact": fact}
@@ -515,8 +562,10 @@ def bench_serve(a):
                       "slots": a.slots, "requests": N},
            "latency_ms": {"p50": runs["server"]["p50_ms"], "p99": runs["server"]["p99_ms"]},
            "runs": runs, "equal_results": equal, "arena_estimates_mib": {k: v >> 20 for k, v in est.items()},
-           "peak_pool_in_use_mib": srv.peak_in_use >> 20}
-    print(json.dumps(out), flush=True)
+           "peak_pool_in_use_mib": srv.peak_in_use >> 20, "cpu_baseline": None,
+           "cpu_baseline_note": "the reference has no serving loop; its per-request primitives are timed in the query workloads"}
+    free_tables(B, (dim, fact))
+    return out
 
 
 def bench_copy(a):
@@ -541,7 +590,7 @@ def bench_copy(a):
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
            "data": "synthetic", "config": {"workload": f"copy of {n} f64 (read {8 * n} B, write {8 * n} B per launch)",
                                             "rows": n}}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def cpu_baseline(a, W):

@@ -591,7 +591,7 @@ static int32_t index_of_hash(hpa_ctx *c, const T *px, L nx, const T *py, L ny, i
         if(guard && (e = guard_build(c, (const int32_t*)px, nx, lo, (L)hi - lo + 1, &t, &room))) R e;
         if(dense || room){
             if(dense && (e = dense_build(c, (const int32_t*)px, nx, lo, (L)hi - lo + 1, true, &t))) R e;
-            ProfScope ps("index_of", c->stream);
+            ProfScope ps("index_of", c->stream, 4.0 * (nx + ny) + 8.0 * ny);
             if(ny > 0) hipLaunchKernelGGL(k_dense_probe_index, dim3(grid_for(ny, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)py, ny, t, z, (int64_t)nx);
             R check_launch("index_of");
         }
@@ -599,13 +599,13 @@ static int32_t index_of_hash(hpa_ctx *c, const T *px, L nx, const T *py, L ny, i
     if(std::is_same<T, int32_t>::value && nx < (L)UINT32_MAX - 1){   // 4-byte keys: packed (key, first row) slots
         unsigned long long *slots; uint64_t mask; int *dup;
         if((e = pk_build(c, (const int32_t*)px, nx, &slots, &mask, &dup))) R e;
-        ProfScope ps("index_of", c->stream);
+        ProfScope ps("index_of", c->stream, 4.0 * (nx + ny) + 8.0 * ny);
         if(ny > 0) hipLaunchKernelGGL(k_pk_probe_index, dim3(grid_for(ny, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)py, ny, slots, mask, z, (int64_t)nx);
         R check_launch("index_of");
     }
     HTab t;
     if((e = ht_create(c, nx, &t))) R e;
-    ProfScope ps("index_of", c->stream);
+    ProfScope ps("index_of", c->stream, (double)sizeof(T) * (nx + ny) + 8.0 * ny);
     if(nx > 0) hipLaunchKernelGGL(k_ht_insert_min<T>, dim3(grid_for(nx, 256, 4)), dim3(256), 0, c->stream, px, nx, t);
     if(ny > 0) hipLaunchKernelGGL(k_ht_probe_index<T>, dim3(grid_for(ny, 256, 4)), dim3(256), 0, c->stream, py, ny, t, z, (int64_t)nx);
     R check_launch("index_of");
@@ -646,7 +646,7 @@ extern "C" int32_t pfnIndexOf(hpa_V z, hpa_V x, hpa_V y){
             void *dz; L mark = c->dcur;
             if((e = out_init(c, z, HPA_L, ny, &dz))) R e;
             if(ny > 0){
-                ProfScope ps("index_of", c->stream);
+                ProfScope ps("index_of", c->stream, (double)type_width(st) * (nx + ny) + 8.0 * ny);
                 if(st == HPA_I) hipLaunchKernelGGL(k_search_ordered<int32_t>, dim3(grid_for(ny, 256, 4)), dim3(256), 0, c->stream, (const int32_t*)px, nx, (const int32_t*)py, ny, (int64_t*)dz);
                 else hipLaunchKernelGGL(k_search_ordered<int64_t>, dim3(grid_for(ny, 256, 4)), dim3(256), 0, c->stream, (const int64_t*)px, nx, (const int64_t*)py, ny, (int64_t*)dz);
                 if((e = check_launch("index_of"))) R e;
@@ -696,7 +696,7 @@ extern "C" int32_t pfnIndexOf(hpa_V z, hpa_V x, hpa_V y){
 
 template<typename T>
 static int32_t member_t(hpa_ctx *c, const T *py, L ny, const T *px, L nx, uint8_t *z){
-    ProfScope ps("member", c->stream);
+    ProfScope ps("member", c->stream, (double)sizeof(T) * (nx + ny) + (double)ny);
     if(nx <= 2){
         if(ny > 0) hipLaunchKernelGGL(k_member_small<T>, dim3(grid_for(ny, 256, 4)), dim3(256), 0, c->stream, py, ny, px, (int)nx, z);
         R check_launch("member");
@@ -911,6 +911,7 @@ static int32_t join_run(hpa_ctx *c, V z, V bx, V py, int32_t t, int op, bool swa
             if((e = set_l_vec(c, cells, z0, P))) R e;
             if((e = set_l_vec(c, cells + 1, z1, P))) R e;
             note_bound(c, z0, swap ? np : nb); note_bound(c, z1, swap ? nb : np);
+            ps.stop(); ps.bytes = 4.0 * (nb + np) + 16.0 * P;          // keys read once, pairs written
             R 0;
         }
         arena_set(c, mark);                              // duplicates: the sort-based build below
@@ -955,6 +956,7 @@ static int32_t join_run(hpa_ctx *c, V z, V bx, V py, int32_t t, int op, bool swa
     if((e = set_l_vec(c, cells, z0, P))) R e;
     if((e = set_l_vec(c, cells + 1, z1, P))) R e;
     note_bound(c, z0, swap ? np : nb); note_bound(c, z1, swap ? nb : np);   // rows of x and of y
+    ps.stop(); ps.bytes = 4.0 * (nb + np) + 16.0 * P;
     R 0;
 }
 

@@ -1391,6 +1391,7 @@ static int32_t group_i32_runs(hpa_ctx *c, V z, const int32_t *x, L n, bool *done
                        (const int64_t*)fpos, (L)nr, (const int64_t*)goff, keys, start, kval);
     hipLaunchKernelGGL(k_run_counts, dim3(grid_for(G, 256, 1)), dim3(256), 0, c->stream, start, (L)G, n, cnt);
     if((e = check_launch("group_runs"))) R e;
+    ps.stop(); ps.bytes = 12.0 * n + 8.0 * G;        // int group: keys read, row list and keys written
     *done = true;
     if(G > 1) note_group_vals(c, keys, x, n, kval);
     note_group_offsets(c, cnt, G, start);
