@@ -53,7 +53,28 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_c_driver(verbose)
     return LIB
+
+
+ROOT = os.path.dirname(HERE)
+DRIVER_SRC = os.path.join(ROOT, "tests", "c_driver", "driver.c")
+DRIVER = os.path.join(ROOT, "tests", "c_driver", "hpa_c_driver")
+
+
+def build_c_driver(verbose=False):
+    """The C caller the GPU tests run (tests/c_driver/driver.c): plain gcc against the public
+    header, linked to the library with a run path relative to the binary."""
+    if not os.path.exists(DRIVER_SRC):
+        return None
+    cmd = ["gcc", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), DRIVER_SRC, "-L", HERE, "-lhorsepower_amd",
+           "-Wl,-rpath,$ORIGIN/../../horsepower_amd", "-o", DRIVER]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C driver build failed:\n{r.stderr[-4000:]}")
+    if verbose:
+        print(f"built {DRIVER}")
+    return DRIVER
 
 
 if __name__ == "__main__":

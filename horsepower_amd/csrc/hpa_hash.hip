@@ -833,7 +833,9 @@ static int32_t set_l_vec(hpa_ctx *c, V z, int64_t *p, L n){
 }
 
 // build on bx (shorter), probe py; `swap` writes (probe, build) into (z0, z1)
-static int32_t join_run(hpa_ctx *c, V z, V bx, V py, int32_t t, int op, bool swap){
+// hash_only: the hash path whatever the build size (joinOneColumn calls lib_join_index_hash
+// directly, h_system.cc:860-868, with no linear path below NUM_JOIN_LINEAR rows)
+static int32_t join_run(hpa_ctx *c, V z, V bx, V py, int32_t t, int op, bool swap, bool hash_only = false){
     L nb = bx->len, np = py->len; int32_t e;
     const void *pb = nullptr, *pp = nullptr;
     if(nb > 0 && (e = promote_dev(c, bx, t, &pb))) R e;
@@ -843,7 +845,7 @@ static int32_t join_run(hpa_ctx *c, V z, V bx, V py, int32_t t, int op, bool swa
     if(!cnt || !rid || !off) R HPA_E_MEMORY;
     int64_t *start = nullptr; uint32_t *sidx = nullptr;
     ProfScope ps("join", c->stream);
-    if(nb >= 10){                                    // NUM_JOIN_LINEAR (common.h:152): hash path
+    if(nb >= 10 || hash_only){                       // NUM_JOIN_LINEAR (common.h:152): hash path
         if(op != 4 || t != HPA_I) R HPA_E_NOT_IMPL;  // v3 radix hash join: i32 equality only (h_hash_v3.cc:233,270-277)
         // unique build keys: a direct first-row table (cleared, or guarded by a presence bitmap)
         // or a packed hash table, no sort
@@ -960,6 +962,112 @@ static int32_t join_run(hpa_ctx *c, V z, V bx, V py, int32_t t, int op, bool swa
     R 0;
 }
 
+// ---------------------------------------------------------------- multi-column join
+// pfnJoinIndexMultiple (h_primitive.cc:2095-2110): the first "eq" column (else the first "neq",
+// else column 0; getFirstEqual, :2088-2092) is joined alone by joinOneColumn -- the radix hash
+// join built on x, probed with y, no swap of the sides (h_system.cc:860-881) -- and then
+// joinOtherColumns keeps, column by column and in order, the pairs whose values satisfy that
+// column's operator (compareOneColumn, h_system.cc:819-843: exact comparisons COMP, :791-794, in
+// the column pair's larger type).  Keeping the pairs that pass every column at once is the same
+// filter.
+#define JOIN_MAX_COLS 16
+struct JoinPred {
+    int n;
+    int op[JOIN_MAX_COLS], cls[JOIN_MAX_COLS], tx[JOIN_MAX_COLS], ty[JOIN_MAX_COLS];
+    const void *px[JOIN_MAX_COLS], *py[JOIN_MAX_COLS];
+};
+__device__ __forceinline__ int64_t jp_int(int t, const void *p, int64_t i){
+    switch(t){
+        case HPA_B: return ((const uint8_t*)p)[i];
+        case HPA_J: return ((const int8_t*)p)[i];
+        case HPA_H: return ((const int16_t*)p)[i];
+        case HPA_I: return ((const int32_t*)p)[i];
+        default:    return ((const int64_t*)p)[i];
+    }
+}
+template<typename R_> __device__ __forceinline__ R_ jp_real(int t, const void *p, int64_t i){
+    switch(t){
+        case HPA_F: return (R_)((const float*)p)[i];
+        case HPA_E: return (R_)((const double*)p)[i];
+        default:    return (R_)jp_int(t, p, i);
+    }
+}
+template<typename T> __device__ __forceinline__ bool jp_comp(int op, T x, T y){   // COMP, h_system.cc:791-794
+    return op < 2 ? (op == 0 ? x < y : x <= y) : op < 4 ? (op == 2 ? x > y : x >= y) : op < 6 ? (op == 4 ? x == y : x != y) : false;
+}
+__global__ void __launch_bounds__(256) k_join_filter(JoinPred p, const int64_t *z0, const int64_t *z1, L np, uint8_t *keep){
+    for(L k = (L)blockIdx.x * blockDim.x + threadIdx.x; k < np; k += (L)gridDim.x * blockDim.x){
+        const int64_t a = z0[k], b = z1[k];
+        bool ok = true;
+        for(int q = 0; q < p.n && ok; q++){
+            switch(p.cls[q]){
+                case 0: ok = jp_comp(p.op[q], jp_int(p.tx[q], p.px[q], a), jp_int(p.ty[q], p.py[q], b)); break;
+                case 1: ok = jp_comp(p.op[q], jp_real<float>(p.tx[q], p.px[q], a), jp_real<float>(p.ty[q], p.py[q], b)); break;
+                default: ok = jp_comp(p.op[q], jp_real<double>(p.tx[q], p.px[q], a), jp_real<double>(p.ty[q], p.py[q], b)); break;
+            }
+        }
+        keep[k] = ok ? 1 : 0;
+    }
+}
+static int op_from_name(const char *s);
+static int32_t join_multi(hpa_ctx *c, V z, V x, V y, V f){
+    int32_t e;
+    const L nc = f->len;
+    if(x->typ != HPA_G || y->typ != HPA_G || x->len != nc || y->len != nc) R HPA_E_LENGTH;
+    if(nc > JOIN_MAX_COLS){ set_error_detail("join: more than %d key columns", JOIN_MAX_COLS); R HPA_E_NOT_IMPL; }
+    std::vector<int64_t> q((size_t)nc);
+    if((e = hpaReadPayload(f, q.data(), 8 * nc))) R e;
+    std::vector<int> ops((size_t)nc);
+    for(L i = 0; i < nc; i++){ ops[i] = op_from_name(hpaSymbolName(q[i])); if(ops[i] < 0) R HPA_E_DOMAIN; }
+    L fx = -1;
+    for(L i = 0; i < nc && fx < 0; i++) if(ops[i] == 4) fx = i;
+    for(L i = 0; i < nc && fx < 0; i++) if(ops[i] == 5) fx = i;
+    if(fx < 0) fx = 0;
+    V xc = (V)x->g, yc = (V)y->g;
+    const L nx = xc[0].len, ny = yc[0].len;
+    for(L i = 0; i < nc; i++){
+        if(!is_real(xc[i].typ) || !is_real(yc[i].typ)) R HPA_E_NOT_IMPL;   // compareOpWithIndex: B/H/J/I/L/F/E
+        if(xc[i].len != nx || yc[i].len != ny) R HPA_E_LENGTH;
+    }
+    const int32_t t0 = xc[fx].typ > yc[fx].typ ? xc[fx].typ : yc[fx].typ;
+    // joinOneColumn: eq takes the hash join (i32 keys only in v3, h_hash_v3.cc:233-259); its neq
+    // and ordered operators leave this path
+    if(ops[fx] != 4 || t0 != HPA_I){ set_error_detail("join: the first key column must be an i32 eq"); R HPA_E_NOT_IMPL; }
+    V0 one; memset(&one, 0, sizeof(one));
+    if((e = join_run(c, &one, xc + fx, yc + fx, t0, 4, false, true))) R e;
+    V pc = (V)one.g;                                   // list(x rows, y rows)
+    const L P = pc[0].len;
+    JoinPred jp; memset(&jp, 0, sizeof(jp));
+    for(L i = 0; i < nc; i++){
+        if(i == fx) continue;
+        const int k = jp.n++;
+        const int32_t tm = xc[i].typ > yc[i].typ ? xc[i].typ : yc[i].typ;
+        jp.op[k] = ops[i]; jp.tx[k] = xc[i].typ; jp.ty[k] = yc[i].typ;
+        jp.cls[k] = tm == HPA_F ? 1 : tm == HPA_E ? 2 : 0;
+        if(P > 0){
+            if(nx > 0 && (e = dev_view(c, xc + i, &jp.px[k]))) R e;
+            if(ny > 0 && (e = dev_view(c, yc + i, &jp.py[k]))) R e;
+        }
+    }
+    if(jp.n == 0 || P == 0){ *z = one; R 0; }
+    const void *p0 = nullptr, *p1 = nullptr;
+    if((e = dev_view(c, pc, &p0)) || (e = dev_view(c, pc + 1, &p1))) R e;
+    V0 mask; memset(&mask, 0, sizeof(mask));
+    void *dm; if((e = out_init(c, &mask, HPA_B, P, &dm))) R e;
+    {
+        ProfScope ps("join_filter", c->stream, 16.0 * P + (double)P);
+        hipLaunchKernelGGL(k_join_filter, dim3(grid_for(P, 256, 4)), dim3(256), 0, c->stream, jp, (const int64_t*)p0,
+                           (const int64_t*)p1, P, (uint8_t*)dm);
+        if((e = check_launch("join_filter"))) R e;
+        if(is_inline(&mask) && (e = commit_inline(c, &mask, dm))) R e;
+    }
+    V cells = (V)halloc(c, 2 * sizeof(V0)); if(!cells) R HPA_E_MEMORY;
+    V zs[2] = {cells, cells + 1}, ys[2] = {pc, pc + 1};
+    if((e = hpaCompressMany(2, zs, &mask, ys))) R e;
+    z->typ = HPA_G; z->len = 2; z->g = (char*)cells; z->g2 = nullptr;
+    R 0;
+}
+
 static int op_from_name(const char *s){            // backend/h_system.cc:779-789
     if(!s) R -1;
     if(!strcmp(s, "lt")) R 0; if(!strcmp(s, "leq")) R 1; if(!strcmp(s, "gt")) R 2;
@@ -971,7 +1079,7 @@ static int op_from_name(const char *s){            // backend/h_system.cc:779-78
 extern "C" int32_t pfnJoinIndex(hpa_V z, hpa_V x, hpa_V y, hpa_V f){
     hpa_ctx *c = ctx_cur();
     if(f->typ != HPA_Q || f->len < 1) R HPA_E_DOMAIN;
-    if(f->len > 1) R HPA_E_NOT_IMPL;                 // multi-column joins (joinOtherColumns) are not on this path
+    if(f->len > 1) R join_multi(c, z, x, y, f);
     int64_t q; int32_t e = read_scalar(c, f, &q, 8); if(e) R e;
     int op = op_from_name(hpaSymbolName(q));
     if(op < 0) R HPA_E_DOMAIN;

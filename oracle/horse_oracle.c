@@ -115,6 +115,8 @@ OV orc_initV(OV x, int32_t typ, int64_t len){
 }
 #define IV(x, t, n) do{ if(!orc_initV((x), (t), (n))) R HPA_E_MEMORY; }while(0)
 static void copyV(OV z, OV x){ *z = *x; }         /* h_system.cc:70-73 */
+/* *z = *x for a node that moves (x is going away): an inline scalar's payload moves with it */
+static void move_node(OV z, OV x){ *z = *x; if(x->g == (char*)&x->b) z->g = (char*)&z->b; }
 #define vV(x,k) (((OV)(x)->g)+(k))
 
 /* ------------------------------------------------------------------ symbols (h_symbol.cc) */
@@ -959,17 +961,84 @@ static int32_t join_single(OV z, OV x, OV y, OV f){
     }
     R HPA_E_NOT_IMPL;
 }
+/* COMP, h_system.cc:791-794 */
+#define JCOMP(op,a,b) ((op)<2 ? ((op)==0 ? (a)<(b) : (a)<=(b)) : (op)<4 ? ((op)==2 ? (a)>(b) : (a)>=(b)) : (op)<6 ? ((op)==4 ? (a)==(b) : (a)!=(b)) : 0)
+static int64_t jval_i(OV v, L k){
+    switch(v->typ){ case HPA_B: R sB(v)[k]; case HPA_J: R sJ(v)[k]; case HPA_H: R sH(v)[k]; case HPA_I: R sI(v)[k]; default: R sL(v)[k]; }
+}
+static double jval_e(OV v, L k){ R v->typ==HPA_E ? sE(v)[k] : v->typ==HPA_F ? (double)sF(v)[k] : (double)jval_i(v,k); }
+static float  jval_f(OV v, L k){ R v->typ==HPA_F ? sF(v)[k] : (float)jval_i(v,k); }
+/* compareOneColumn, h_system.cc:819-843: keep the pairs (i0[k], i1[k]) with x[i0[k]] op y[i1[k]],
+ * both read in the larger of the two types, in pair order */
+static int32_t join_compare_column(OV z0, OV z1, OV x, OV y, OV i0, OV i1, int op){
+    L size = i0->len, tot = 0;
+    int32_t tm = MAXV(x->typ, y->typ);
+    uint8_t *t = (uint8_t*)malloc(size > 0 ? size : 1);
+    if(!t) R HPA_E_MEMORY;
+    for(L k=0;k<size;k++){
+        L a = sL(i0)[k], b = sL(i1)[k];
+        int r;
+        if(tm == HPA_E){ double p = jval_e(x,a), q = jval_e(y,b); r = JCOMP(op,p,q); }
+        else if(tm == HPA_F){ float p = jval_f(x,a), q = jval_f(y,b); r = JCOMP(op,p,q); }
+        else { int64_t p = jval_i(x,a), q = jval_i(y,b); r = JCOMP(op,p,q); }
+        t[k] = (uint8_t)r; tot += r;
+    }
+    if(!orc_initV(z0, HPA_L, tot) || !orc_initV(z1, HPA_L, tot)){ free(t); R HPA_E_MEMORY; }
+    tot = 0;
+    for(L k=0;k<size;k++) if(t[k]){ sL(z0)[tot] = sL(i0)[k]; sL(z1)[tot++] = sL(i1)[k]; }
+    free(t);
+    R 0;
+}
+/* pfnJoinIndexMultiple, h_primitive.cc:2095-2110: the first eq column (getFirstEqual,
+ * :2088-2092; else the first neq, else column 0) through joinOneColumn -- the radix hash join on
+ * x and y as given, no side swap and no linear path (h_system.cc:860-881) -- then every other
+ * column in order through compareOneColumn (joinOtherColumns, :845-852).  The v3 hash join takes
+ * i32 keys only; other first-column cases stop the reference (EP/TODO) and are NOT_IMPL here. */
+static int32_t join_multi(OV z, OV x, OV y, OV f){
+    L nc = f->len;
+    if(x->typ != HPA_G || y->typ != HPA_G || x->len != nc || y->len != nc) R HPA_E_LENGTH;
+    int ops[64];
+    if(nc > 64) R HPA_E_NOT_IMPL;
+    for(L i=0;i<nc;i++){ ops[i] = op_from_symbol(((int64_t*)f->g)[i]); if(ops[i] < 0) R HPA_E_DOMAIN; }
+    L fx = -1;
+    for(L i=0;i<nc && fx<0;i++) if(ops[i]==4) fx = i;
+    for(L i=0;i<nc && fx<0;i++) if(ops[i]==5) fx = i;
+    if(fx < 0) fx = 0;
+    L nx = vV(x,0)->len, ny = vV(y,0)->len;
+    for(L i=0;i<nc;i++){
+        if(!is_real(vV(x,i)->typ) || !is_real(vV(y,i)->typ)) R HPA_E_NOT_IMPL;
+        if(vV(x,i)->len != nx || vV(y,i)->len != ny) R HPA_E_LENGTH;
+    }
+    int32_t t0 = MAXV(vV(x,fx)->typ, vV(y,fx)->typ);
+    if(ops[fx] != 4 || t0 != HPA_I) R HPA_E_NOT_IMPL;
+    O0 tx, ty; int32_t e;
+    if((e=promote(&tx, vV(x,fx), t0))) R e;
+    if((e=promote(&ty, vV(y,fx), t0))) R e;
+    O0 a0, a1;
+    if((e=join_hash(&a0, &a1, &tx, &ty))) R e;
+    for(L i=0;i<nc;i++){
+        if(i == fx) continue;
+        O0 b0, b1;
+        if((e=join_compare_column(&b0, &b1, vV(x,i), vV(y,i), &a0, &a1, ops[i]))) R e;
+        move_node(&a0, &b0); move_node(&a1, &b1);
+    }
+    IV(z, HPA_G, 2);
+    move_node(vV(z,0), &a0); move_node(vV(z,1), &a1);
+    R 0;
+}
 int32_t orc_pfnJoinIndex(OV z, OV x, OV y, OV f){  /* h_primitive.cc:2122-2155 */
     if(f->len == 1){
         if(x->len > y->len){
             int32_t e = join_single(z, y, x, f); if(e) R e;
-            O0 a = *vV(z,0), b = *vV(z,1);               /* swap2, h_system.cc:47-56 */
-            *vV(z,0) = b; *vV(z,1) = a;
+            O0 a, b;                                     /* swap2, h_system.cc:47-56 */
+            move_node(&a, vV(z,0)); move_node(&b, vV(z,1));
+            move_node(vV(z,0), &b); move_node(vV(z,1), &a);
             R 0;
         }
         R join_single(z, x, y, f);
     }
-    R HPA_E_NOT_IMPL;
+    if(f->len > 1) R join_multi(z, x, y, f);
+    R HPA_E_DOMAIN;
 }
 
 /* ------------------------------------------------------------------ index_of / member */

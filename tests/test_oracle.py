@@ -281,3 +281,46 @@ def test_heap_full_returns_memory_error(orc):
         dll.orc_set_heap_top(old)
         orc.release(mark)
     assert orc.read(orc.pfnSum(m)).tolist() == [n]
+
+
+def _join_multi_brute(xs, ys, ops):
+    """All pairs (i, j) satisfying every column's operator, probe-major: sorted by (j, i)."""
+    f = {"eq": np.equal, "neq": np.not_equal, "lt": np.less, "leq": np.less_equal,
+         "gt": np.greater, "geq": np.greater_equal}
+    ok = np.ones((len(xs[0]), len(ys[0])), bool)
+    for a, b, op in zip(xs, ys, ops):
+        ok &= f[op](a.astype(np.result_type(a, b))[:, None], b.astype(np.result_type(a, b))[None, :])
+    i, j = np.nonzero(ok)
+    o = np.lexsort((i, j))
+    return i[o], j[o]
+
+
+@pytest.mark.parametrize("ops", [("eq", "lt"), ("lt", "eq", "neq"), ("eq", "eq", "geq"), ("gt", "leq", "eq")])
+def test_join_multi_brute_force(orc, ops):
+    """pfnJoinIndexMultiple (h_primitive.cc:2095-2110): the first eq column hash-joined (build on
+    x, probe y, no swap), the rest filtered pair by pair; the pairs are exactly the nested-loop
+    matches in probe-major order."""
+    r = np.random.default_rng(len(ops) * 7 + len(ops[0]))
+    for nx, ny in ((40, 70), (300, 25), (5, 9)):
+        xs, ys = [], []
+        for q, op in enumerate(ops):
+            if op == "eq" and q == ops.index("eq"):
+                xs.append(r.integers(0, 12, nx).astype(np.int32)); ys.append(r.integers(0, 12, ny).astype(np.int32))
+            elif q % 2:
+                xs.append(np.round(r.uniform(0, 4, nx), 1)); ys.append(r.integers(0, 4, ny).astype(np.int64))
+            else:
+                xs.append(r.integers(0, 4, nx).astype(np.int32)); ys.append(r.integers(0, 4, ny).astype(np.int16))
+        z = orc.pfnJoinIndex(orc.list([orc.vec(a) for a in xs]), orc.list([orc.vec(b) for b in ys]),
+                             orc.vec(np.array([orc.symbol(o) for o in ops], np.int64), 9))
+        got = orc.read_list(z)
+        i, j = _join_multi_brute(xs, ys, ops)
+        assert got[0].tolist() == i.tolist() and got[1].tolist() == j.tolist(), (ops, nx, ny)
+
+
+def test_join_single_pair_after_swap(orc):
+    """A one-pair join result is two inline scalars; swapping the sides (swap2,
+    h_system.cc:47-56, when x is the longer input) keeps each value with its side."""
+    x = np.arange(100, 140, dtype=np.int32)
+    y = np.array([117], np.int32)
+    z = orc.read_list(orc.pfnJoinIndex(orc.vec(x), orc.vec(y), orc.sym("eq")))
+    assert z[0].tolist() == [17] and z[1].tolist() == [0]
