@@ -666,11 +666,15 @@ extern "C" int32_t pfnList(hpa_V z, int64_t n, hpa_V x[]){
     R 0;
 }
 
+int32_t enum_values(hpa_ctx *c, V z, V x);          // hpa_enum.hip
+int32_t enum_keys(V z, V x);
 extern "C" int32_t pfnKeys(hpa_V z, hpa_V x){      // h_primitive.cc:190-203
+    if(x->typ == HPA_Y && x->p) R enum_keys(z, x);
     if(x->typ == HPA_N || x->typ == HPA_A){ *z = *((V)x->g); R 0; }
     R HPA_E_DOMAIN;
 }
 extern "C" int32_t pfnValues(hpa_V z, hpa_V x){    // h_primitive.cc:205-218
+    if(x->typ == HPA_Y && x->p) R enum_values(ctx_cur(), z, x);     // copyEnumIndex
     if(x->typ == HPA_N || x->typ == HPA_A){ *z = *(((V)x->g) + 1); R 0; }
     R HPA_E_DOMAIN;
 }

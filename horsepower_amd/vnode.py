@@ -114,7 +114,8 @@ _ARITY = {
     "pfnGt": 2, "pfnGeq": 2, "pfnEq": 2, "pfnNeq": 2, "pfnPlus": 2, "pfnMinus": 2, "pfnMul": 2,
     "pfnDiv": 2, "pfnAnd": 2, "pfnNand": 2, "pfnOr": 2, "pfnNor": 2, "pfnXor": 2,
     "pfnCompress": 2, "pfnIndexOf": 2, "pfnOrderBy": 2, "pfnMember": 2, "pfnKeys": 1,
-    "pfnValues": 1, "pfnJoinIndex": 3,
+    "pfnValues": 1, "pfnJoinIndex": 3, "pfnEnum": 2, "pfnFetch": 1, "pfnColumnValue": 2,
+    "pfnLoadTable": 1, "pfnAddFKey": 3,
 }
 
 

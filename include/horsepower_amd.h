@@ -209,6 +209,25 @@ int32_t pfnMember     (hpa_V z, hpa_V y, hpa_V x);         /* h_primitive.cc:228
 int32_t pfnKeys       (hpa_V z, hpa_V x);                  /* h_primitive.cc:190-203 */
 int32_t pfnValues     (hpa_V z, hpa_V x);                  /* h_primitive.cc:205-218 */
 
+/* ---- enumerations and foreign keys ---------------------------------------------------------
+ * An enum node (typ HPA_Y) has the reference's layout (backend/common.h:19, h_primitive.h:12-16,
+ * h_system.h:12-14): len = number of values, vg = i64 index of each value among the keys (in
+ * HBM), and the union's pointer `p` -> hpa_Y0 {source, ek = keys node, ev = values node}.
+ * Tables (typ HPA_A) are V0[2] (column-name symbols, list of columns) with p -> hpa_A0 {rows,
+ * columns, foreign-key list} (h_system.h:17-31); the list holds the enums pfnAddFKey made. */
+typedef struct hpa_Y0 { int64_t source; hpa_V ek; hpa_V ev; } hpa_Y0;
+typedef struct hpa_ListY { void *y; void *cn; struct hpa_ListY *next; } hpa_ListY;
+typedef struct hpa_A0 { int64_t row; int64_t col; hpa_ListY *fkey; } hpa_A0;
+int32_t pfnEnum       (hpa_V z, hpa_V x, hpa_V y);         /* h_primitive.cc:2198-2222 */
+int32_t pfnFetch      (hpa_V z, hpa_V x);                  /* h_primitive.cc:220-225 */
+int32_t pfnAddFKey    (hpa_V x, hpa_V xKey, hpa_V y, hpa_V yKey); /* h_primitive.cc:2425-2466 */
+int32_t pfnColumnValue(hpa_V z, hpa_V x, hpa_V y);         /* h_primitive.cc:283-313 */
+int32_t pfnLoadTable  (hpa_V z, hpa_V x);                  /* h_primitive.cc:20-30 */
+/* registerTable (h_system.cc:58-62): the catalog entry pfnLoadTable and pfnAddFKey look up by
+ * name; the reference makes it when it reads a table from disk (load_db.cc), which this library
+ * does not do, so the caller registers the tables it holds. */
+int32_t hpaRegisterTable(int64_t name, hpa_V table);
+
 /* Join operator names.  pfnJoinIndex's `f` is a symbol vector of operator names
  * ("eq", "neq", ...; backend/h_system.cc:779-789).  Symbols are interned ids in the reference
  * (backend/h_symbol.cc); this library interns the names it understands here. */

@@ -864,12 +864,101 @@ int32_t orc_pfnGroup(OV z, OV x){                  /* h_primitive.cc:993-999 */
 }
 
 int32_t orc_pfnKeys(OV z, OV x){                   /* h_primitive.cc:190-203 */
+    if(x->typ==HPA_Y){ copyV(z, ((hpa_Y0*)x->p)->ek); R 0; }      /* getEnumKey */
     if(x->typ==HPA_N || x->typ==HPA_A){ copyV(z, vV(x,0)); R 0; }
     R HPA_E_DOMAIN;
 }
 int32_t orc_pfnValues(OV z, OV x){                 /* h_primitive.cc:205-218 */
+    if(x->typ==HPA_Y){                             /* copyEnumIndex, h_system.cc:501-506 */
+        IV(z, HPA_L, x->len);
+        if(x->len == 1) z->i64 = *(int64_t*)x->g;
+        else z->g = x->g;
+        R 0;
+    }
     if(x->typ==HPA_N || x->typ==HPA_A){ copyV(z, vV(x,1)); R 0; }
     R HPA_E_DOMAIN;
+}
+int32_t orc_pfnFetch(OV z, OV x){                  /* h_primitive.cc:220-225 */
+    if(x->typ==HPA_Y){ copyV(z, ((hpa_Y0*)x->p)->ev); R 0; }      /* getEnumVal */
+    R HPA_E_DOMAIN;
+}
+/* pfnEnum, h_primitive.cc:2198-2222: index_of the values in the keys; a value that is not
+ * among the keys is E_ENUM_INDEX.  The index vector is the enum's payload (initEnum). */
+int32_t orc_pfnEnum(OV z, OV x, OV y){
+    if(is_basic(x->typ)){
+        OV ind = orc_alloc_node(); if(!ind) R HPA_E_MEMORY;
+        int32_t e = orc_pfnIndexOf(ind, x, y); if(e) R e;
+        L n = ind->len;
+        for(L i=0;i<n;i++) if(sL(ind)[i] >= x->len) R HPA_E_ENUM_INDEX;
+        hpa_Y0 *yy = (hpa_Y0*)heap_alloc(sizeof(hpa_Y0)); if(!yy) R HPA_E_MEMORY;
+        yy->source = -1; yy->ek = x; yy->ev = y;
+        z->typ = HPA_Y; z->len = n; z->g = ind->g; z->p = yy;     /* ind is a heap node: inline ok */
+        R 0;
+    }
+    if(x->typ==HPA_G && y->typ==HPA_G) R HPA_E_NOT_IMPL;
+    R HPA_E_DOMAIN;
+}
+/* the catalog (registerTable / findTableByName, h_system.cc:58-67) */
+static struct { int64_t sid; OV t; } g_tabs[64]; static int g_ntab = 0;
+int32_t orc_register_table(int64_t sid, OV t){
+    for(int i=0;i<g_ntab;i++) if(g_tabs[i].sid == sid){ g_tabs[i].t = t; R 0; }
+    if(g_ntab >= 64) R HPA_E_MEMORY;
+    g_tabs[g_ntab].sid = sid; g_tabs[g_ntab].t = t; g_ntab++;
+    R 0;
+}
+static OV find_table(OV x){                        /* fetchTableByName, h_primitive.cc:257-262 */
+    if(x->typ != HPA_Q || x->len != 1) R NULL;
+    for(int i=0;i<g_ntab;i++) if(g_tabs[i].sid == x->q) R g_tabs[i].t;
+    R NULL;
+}
+int32_t orc_pfnLoadTable(OV z, OV x){              /* h_primitive.cc:20-30 (catalog lookup only) */
+    if(x->typ != HPA_Q) R HPA_E_DOMAIN;
+    OV t = find_table(x); if(!t) R HPA_E_TABLE_NOT_FOUND;
+    *z = *t; R 0;
+}
+static const int64_t *sym_of(OV s){ R s->len <= 1 ? &s->q : (const int64_t*)s->g; }
+static OV fkey_by_name(OV t, OV m){               /* fetchFKeyByName, h_primitive.cc:264-281 */
+    if(t->typ != HPA_A || !t->p || m->typ != HPA_Q) R NULL;
+    for(hpa_ListY *y = ((hpa_A0*)t->p)->fkey; y; y = y->next){
+        OV col = (OV)y->cn;
+        if(col->typ != m->typ || col->len != m->len) continue;
+        int f = 1; for(L i=0;i<m->len;i++) if(sym_of(m)[i] != sym_of(col)[i]){ f = 0; break; }
+        if(f) R (OV)y->y;
+    }
+    R NULL;
+}
+static L find_col(OV t, int64_t cid){              /* findColFromTable, h_system.cc:157-166 */
+    OV k = vV(t,0);
+    for(L i=0;i<k->len;i++) if(sym_of(k)[i] == cid) R i;
+    R -1;
+}
+static int32_t column_value(OV z, OV x, OV y, int fk){  /* fetchColumnValue, h_primitive.cc:283-308 */
+    if(!(x->typ==HPA_A || x->typ==HPA_K) || y->typ != HPA_Q) R HPA_E_DOMAIN;
+    if(fk){ OV e0 = fkey_by_name(x, y); if(e0){ copyV(z, e0); R 0; } }
+    OV vals = vV(x,1);
+    if(y->len == 1){ L k = find_col(x, y->q); if(k < 0) R HPA_E_COL_NOT_FOUND; copyV(z, vV(vals,k)); R 0; }
+    IV(z, HPA_G, y->len);
+    for(L i=0;i<y->len;i++){ L k = find_col(x, sym_of(y)[i]); if(k < 0) R HPA_E_COL_NOT_FOUND; copyV(vV(z,i), vV(vals,k)); }
+    R 0;
+}
+int32_t orc_pfnColumnValue(OV z, OV x, OV y){ R column_value(z, x, y, 1); }   /* h_primitive.cc:310-313 */
+/* pfnAddFKey, h_primitive.cc:2425-2466, with setFKey/insertFKey (h_system.cc:652-675) */
+int32_t orc_pfnAddFKey(OV x, OV xKey, OV y, OV yKey){
+    OV tx = find_table(x); if(!tx) R HPA_E_DOMAIN;
+    OV ty = find_table(y); if(!ty) R HPA_E_DOMAIN;
+    if(!(xKey->typ==HPA_Q && yKey->typ==xKey->typ && yKey->len==xKey->len)) R HPA_E_DOMAIN;
+    if(ty->typ != HPA_A || !ty->p) R HPA_E_DOMAIN;
+    OV xc = orc_alloc_node(), yc = orc_alloc_node(), fk = orc_alloc_node();
+    if(!xc || !yc || !fk) R HPA_E_MEMORY;
+    int32_t e;
+    if((e = column_value(xc, tx, xKey, 0)) || (e = column_value(yc, ty, yKey, 0))) R e;
+    if((e = orc_pfnEnum(fk, xc, yc))) R e;
+    hpa_ListY *n = (hpa_ListY*)heap_alloc(sizeof(hpa_ListY)); if(!n) R HPA_E_MEMORY;
+    n->y = fk; n->cn = yKey; n->next = NULL;
+    hpa_A0 *a = (hpa_A0*)ty->p;
+    if(!a->fkey) a->fkey = n;
+    else { hpa_ListY *t = a->fkey; while(t->next) t = t->next; t->next = n; }
+    R 0;
 }
 
 /* ------------------------------------------------------------------ each / raze */

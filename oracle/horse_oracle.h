@@ -61,6 +61,12 @@ int32_t orc_pfnEach(OV z, OV x, int32_t (*foo)(OV, OV));
 int32_t orc_pfnEachRight(OV z, OV x, OV y, int32_t (*foo)(OV, OV, OV));
 int32_t orc_pfnJoinIndex(OV z, OV x, OV y, OV f);
 int32_t orc_pfnMember(OV z, OV y, OV x);
+int32_t orc_pfnEnum(OV z, OV x, OV y);
+int32_t orc_pfnFetch(OV z, OV x);
+int32_t orc_pfnAddFKey(OV x, OV xKey, OV y, OV yKey);
+int32_t orc_pfnColumnValue(OV z, OV x, OV y);
+int32_t orc_pfnLoadTable(OV z, OV x);
+int32_t orc_register_table(int64_t sid, OV t);
 int32_t orc_pfnKeys(OV z, OV x);
 int32_t orc_pfnValues(OV z, OV x);
 int64_t orc_symbol(const char *name);

@@ -324,3 +324,22 @@ def test_join_single_pair_after_swap(orc):
     y = np.array([117], np.int32)
     z = orc.read_list(orc.pfnJoinIndex(orc.vec(x), orc.vec(y), orc.sym("eq")))
     assert z[0].tolist() == [17] and z[1].tolist() == [0]
+
+
+def test_enum_closed_form(orc):
+    """pfnEnum (h_primitive.cc:2198-2222): the index of every value among the keys (index_of,
+    first occurrence), E_ENUM_INDEX (10) for a value that is not a key; keys/values/fetch give
+    back the keys, the index vector and the values."""
+    r = np.random.default_rng(21)
+    keys = r.permutation(50_000).astype(np.int32) * 3 + 1
+    vals = keys[r.integers(0, keys.size, 200_000)]
+    e = orc.pfnEnum(orc.vec(keys), orc.vec(vals))
+    pos = {int(k): i for i, k in enumerate(keys)}
+    assert orc.read(orc.pfnValues(e)).tolist() == [pos[int(v)] for v in vals]
+    assert np.array_equal(orc.read(orc.pfnKeys(e)), keys)
+    assert np.array_equal(orc.read(orc.pfnFetch(e)), vals)
+    bad = vals.copy()
+    bad[777] = 2                                     # not of the form 3k + 1
+    with pytest.raises(HorseError) as ei:
+        orc.pfnEnum(orc.vec(keys), orc.vec(bad))
+    assert ei.value.code == 10
