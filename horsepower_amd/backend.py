@@ -212,6 +212,17 @@ class GpuBackend(Backend):
         self.lib.rc("hpaCompressMany", n, (V * n)(*zs), mask, (V * n)(*ys))
         return zs
 
+    def compress_map(self, mask, ys, maps, xs):
+        """Compress of plain columns ys and of FE maps [(typ, expr)] over columns xs, one pass."""
+        n, nm = len(ys), len(maps)
+        zs = [self.node() for _ in range(n)]
+        zm = [self.node() for _ in range(nm)]
+        typs = (C.c_int32 * max(nm, 1))(*[t for t, _ in maps])
+        exprs = (Expr * max(nm, 1))(*[e for _, e in maps])
+        self.lib.rc("hpaCompressMap", n, (V * max(n, 1))(*zs), mask, (V * max(n, 1))(*ys), nm,
+                    (V * max(nm, 1))(*zm), typs, exprs, len(xs), (V * max(len(xs), 1))(*xs))
+        return zs, zm
+
     def index_many(self, xs, y):
         """Peephole index: [index(x, y) for x in xs] in one launch."""
         n = len(xs)

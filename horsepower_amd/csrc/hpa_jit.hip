@@ -466,6 +466,77 @@ int32_t jit_fused_map(hpa_ctx *c, int otyp, int nx, V *x, const hpa_expr *ex, co
     R jit_launch(c, fn, (unsigned)grid_for(n, 256, 4), 256, a);
 }
 
+// ---------------------------------------------------------------- FE map of a compress (cp_map)
+// The byte-mask write pass of the stream compaction (4096-row tiles, thread t owns the 16
+// consecutive rows [16t, 16t + 16), its selection bits `mb` and the tile offsets `toff` from the
+// count pass) with each selected row's value computed from the map's columns: the columns are
+// read with 16-row vector loads, the selected values compacted in LDS and written out by all
+// threads together.  Every column must start on a 16-byte boundary.
+int32_t jit_compress_map(hpa_ctx *c, int otyp, const DevCols &cols, const hpa_expr *ex, const uint16_t *mb,
+                         const int64_t *toff, L n, void *dz){
+    if(!jit_enabled()) R HPA_E_NOT_IMPL;
+    const int nx = cols.n;
+    for(int k = 0; k < nx; k++) if((uintptr_t)cols.p[k] & 15) R HPA_E_NOT_IMPL;
+    std::vector<int> used(nx, 0);
+    CExpr be; int32_t e;
+    Konst kc;
+    if((e = expr_to_c(ex, nx, cols.typ, &be, used, &kc))) R e;
+    const char *ot = otyp == HPA_B ? "unsigned char" : otyp == HPA_I ? "int" : otyp == HPA_L ? "long long" : "double";
+    const std::string val = otyp == HPA_B ? "(unsigned char)((" + be.s + ") != 0)" : "(" + std::string(ot) + ")(" + be.s + ")";
+    auto gen = [&](){
+    std::ostringstream s;
+    s << PREAMBLE
+      << "template<typename T> struct __attribute__((aligned(16))) R16 { T v[16]; };\n"
+         "struct A { const void *c[" << EXPR_MAX_COLS << "]; i64 n; void *out; const unsigned short *mb; const i64 *toff; " KONST_FIELDS " };\n"
+         "extern \"C\" __global__ void __launch_bounds__(256) cp_map(A a){\n"
+         "    __shared__ " << ot << " buf[4096];\n"
+         "    __shared__ unsigned sw[4];\n"
+         "    const int t = threadIdx.x, lane = t & 63, w = t >> 6;\n"
+         "    const i64 i0 = (i64)blockIdx.x * 4096 + (i64)t * 16;\n"
+         "    const unsigned m = a.mb[(i64)blockIdx.x * 256 + t];\n"
+         "    const unsigned cnt = __popc(m);\n"
+         "    unsigned inc = cnt;\n"
+         "    for(int o = 1; o < 64; o <<= 1){ unsigned y = __shfl_up(inc, o, 64); if(lane >= o) inc += y; }\n"
+         "    if(lane == 63) sw[w] = inc;\n"
+         "    __syncthreads();\n"
+         "    unsigned wbase = 0, total = 0;\n"
+         "    for(int q = 0; q < 4; q++){ if(q < w) wbase += sw[q]; total += sw[q]; }\n"
+         "    if(total == 0) return;\n"
+         "    unsigned off = wbase + inc - cnt;\n"
+         "    if(m && i0 + 16 <= a.n){\n";
+    for(int k = 0; k < nx; k++) if(used[k])
+        s << "        const R16<" << ctype(cols.typ[k]) << "> r" << k << " = *(const R16<" << ctype(cols.typ[k]) << ">*)((const "
+          << ctype(cols.typ[k]) << "*)a.c[" << k << "] + i0);\n";
+    s << "        #pragma unroll\n"
+         "        for(int k = 0; k < 16; k++) if((m >> k) & 1u){\n";
+    for(int k = 0; k < nx; k++) if(used[k]) s << "            const " << ctype(cols.typ[k]) << " v" << k << " = r" << k << ".v[k];\n";
+    s << "            buf[off++] = " << val << ";\n"
+         "        }\n"
+         "    } else if(m){\n"
+         "        for(int k = 0; k < 16; k++) if((m >> k) & 1u){\n"
+         "            const i64 i = i0 + k;\n"
+      << loads_code(nx, cols.typ, used)
+      << "            buf[off++] = " << val << ";\n"
+         "        }\n"
+         "    }\n"
+         "    __syncthreads();\n"
+         "    const i64 pre = a.toff[blockIdx.x];\n"
+         "    for(unsigned j = t; j < total; j += 256) ((" << ot << "*)a.out)[pre + j] = buf[j];\n"
+         "}\n";
+    return s.str();
+    };
+    hipFunction_t fn;
+    const std::string key = "cpm|" + std::to_string(otyp) + "|" + std::to_string(nx) + "|" + types_key(nx, cols.typ) + "|" + be.s;
+    if((e = jit_get_keyed(key, "cp_map", gen, &fn))) R e;
+    struct CArgs { const void *c[EXPR_MAX_COLS]; long long n; void *out; const uint16_t *mb; const int64_t *toff;
+                   long long ki[JIT_MAX_CONST]; double kf[JIT_MAX_CONST]; } a;
+    memset(&a, 0, sizeof(a));
+    for(int k = 0; k < nx; k++) a.c[k] = cols.p[k];
+    memcpy(a.ki, kc.i, sizeof(a.ki)); memcpy(a.kf, kc.f, sizeof(a.kf));
+    a.n = n; a.out = dz; a.mb = mb; a.toff = toff;
+    R jit_launch(c, fn, (unsigned)ceil_div(n, 4096), 256, a);
+}
+
 // ---------------------------------------------------------------- fused group + aggregates (fga)
 // Accumulator classes: sums (and avgs, which share the sum of the same body), mins, maxes.
 struct Acc { std::string body; bool f; int cls; };   // cls 0 sum, 2 min, 3 max

@@ -17,6 +17,18 @@ static_assert(CP_BLOCK == 256 && CP_ITEMS * 4 == 64, "the tile scan gives one (i
 
 
 struct CpCols { int n; int w[CP_MAXC]; const char *src[CP_MAXC]; char *dst[CP_MAXC]; };
+// FE map outputs of a compress: dst[j][k] = e[j](x) at the k-th selected row, as type typ[j]
+#define CP_MAXM 2
+struct CpMaps { int n; int typ[CP_MAXM]; char *dst[CP_MAXM]; DevCols x; DevExpr e[CP_MAXM]; const hpa_expr *hx[CP_MAXM]; };
+// the FE map's conversion of a result to its output type (k_fused_map)
+__device__ __forceinline__ uint64_t cp_map_bits(int typ, bool fk, Slot s){
+    switch(typ){
+        case HPA_B: return (uint64_t)(fk ? s.f != 0.0 : s.i != 0);
+        case HPA_I: return (uint64_t)(uint32_t)(fk ? (int32_t)s.f : (int32_t)s.i);
+        case HPA_L: return (uint64_t)(fk ? (int64_t)s.f : s.i);
+        default: { const double d = fk ? s.f : (double)s.i; return (uint64_t)__double_as_longlong(d); }
+    }
+}
 
 struct MaskPred {
     const uint8_t *m;
@@ -84,8 +96,9 @@ __global__ void __launch_bounds__(CP_BLOCK) k_cp_count_mask(const uint8_t *m, L 
 
 // Pass 2: each tile takes its output offset from the scanned tile counts, ranks its selected
 // rows with wave ballots (row order) and scatters every column (and/or the row index).
-template<typename Pred>
-__global__ void __launch_bounds__(CP_BLOCK) k_cp_write(Pred pred, L n, CpCols cols, int64_t *idx_out, const int64_t *toff){
+template<typename Pred, bool MAPS = false>
+__global__ void __launch_bounds__(CP_BLOCK) k_cp_write(Pred pred, L n, CpCols cols, int64_t *idx_out, const int64_t *toff,
+                                                       CpMaps maps = CpMaps()){
     __shared__ uint32_t s_cnt[CP_ITEMS * 4];
     const L base = (L)blockIdx.x * CP_TILE;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -114,6 +127,15 @@ __global__ void __launch_bounds__(CP_BLOCK) k_cp_write(Pred pred, L n, CpCols co
             int64_t pos = pre + s_cnt[k * 4 + w] + __popcll(bal[k] & lt);
             for(int c = 0; c < cols.n; c++) copy_elem(cols.dst[c], cols.src[c], cols.w[c], pos, i);
             if(idx_out) idx_out[pos] = i;
+            if(MAPS) for(int j = 0; j < maps.n; j++){
+                // lanes hold consecutive rows, so the map's column loads are coalesced
+                const uint64_t v = cp_map_bits(maps.typ[j], maps.e[j].ins[maps.e[j].n - 1].kf, eval_expr(maps.e[j], maps.x, i));
+                switch(maps.typ[j]){
+                    case HPA_B: ((uint8_t*)maps.dst[j])[pos] = (uint8_t)v; break;
+                    case HPA_I: ((uint32_t*)maps.dst[j])[pos] = (uint32_t)v; break;
+                    default: ((uint64_t*)maps.dst[j])[pos] = v; break;
+                }
+            }
         }
     }
 }
@@ -229,20 +251,36 @@ static int32_t compress_write(hpa_ctx *c, Pred pred, L n, CpCols cols, int64_t *
 // the byte-mask form: 16 consecutive rows per thread when every operand is 16-element aligned
 // (mb: the selection bits the count pass kept, or null)
 static int32_t compress_write(hpa_ctx *c, MaskPred pred, L n, CpCols cols, int64_t *idx_out, const int64_t *toff,
-                              const uint16_t *mb = nullptr){
+                              const uint16_t *mb = nullptr, const CpMaps *maps = nullptr){
     L ntiles = ceil_div(n, CP_TILE);
     if(ntiles < 1) R 0;
     bool aligned = ((uintptr_t)pred.m & 15) == 0;
     for(int k = 0; k < cols.n; k++)
         aligned = aligned && ((uintptr_t)cols.src[k] % (uintptr_t)(16 * cols.w[k])) == 0 && cols.w[k] <= 8;
-    if(aligned) hipLaunchKernelGGL(k_cp_write_mask, dim3((unsigned)ntiles), dim3(CP_BLOCK), 0, c->stream, pred.m, mb, n, cols, idx_out, toff);
+    if(maps && aligned && mb){
+        // generated map kernels over the kept selection bits, then the plain columns; with the
+        // code generator switched off, the interpreted write pass below
+        int32_t e = 0;
+        for(int j = 0; j < maps->n && !e; j++) e = jit_compress_map(c, maps->typ[j], maps->x, maps->hx[j], mb, toff, n, maps->dst[j]);
+        if(!e){
+            if(cols.n > 0 || idx_out)
+                hipLaunchKernelGGL(k_cp_write_mask, dim3((unsigned)ntiles), dim3(CP_BLOCK), 0, c->stream, pred.m, mb, n, cols, idx_out, toff);
+            R check_launch("compress_write");
+        }
+        if(e != HPA_E_NOT_IMPL) R e;
+    }
+    if(maps) hipLaunchKernelGGL((k_cp_write<MaskPred, true>), dim3((unsigned)ntiles), dim3(CP_BLOCK), 0, c->stream, pred, n, cols, idx_out, toff, *maps);
+    else if(aligned) hipLaunchKernelGGL(k_cp_write_mask, dim3((unsigned)ntiles), dim3(CP_BLOCK), 0, c->stream, pred.m, mb, n, cols, idx_out, toff);
     else hipLaunchKernelGGL(k_cp_write<MaskPred>, dim3((unsigned)ntiles), dim3(CP_BLOCK), 0, c->stream, pred, n, cols, idx_out, toff);
     R check_launch("compress_write");
 }
 
 // pfnCompress (h_primitive.cc:1479-1521)
-static int32_t compress_many(hpa_ctx *c, int ny, V *z, V x, V *y, bool expr_mode, const ExprPred *ep, L n_expr){
+// nm > 0: also zm[j] = compress(x, maps->e[j](maps->x)) (byte-mask mode only; no aliasing)
+static int32_t compress_many(hpa_ctx *c, int ny, V *z, V x, V *y, bool expr_mode, const ExprPred *ep, L n_expr,
+                             int nm = 0, V *zm = nullptr, CpMaps *maps = nullptr){
     L n = expr_mode ? n_expr : x->len;
+    const bool alias_ok = nm == 0;
     for(int k = 0; k < ny; k++){
         if(y[k]->len != n) R HPA_E_LENGTH;
         switch(y[k]->typ){
@@ -263,20 +301,23 @@ static int32_t compress_many(hpa_ctx *c, int ny, V *z, V x, V *y, bool expr_mode
     const uint8_t *m = nullptr;
     if(!expr_mode && n > 0){ const void *p; if((e = dev_view(c, x, &p))) R e; m = (const uint8_t*)p; }
     double wsum = 0; for(int k = 0; k < ny; k++) wsum += cols.w[k];
+    for(int j = 0; j < nm; j++) wsum += (double)type_width(maps->typ[j]);
     double wcond = expr_mode ? 0 : 1.0;
     if(expr_mode) for(int k = 0; k < ep->x.n; k++) wcond += (double)type_width(ep->x.typ[k]);
+    for(int k = 0; k < (nm > 0 ? maps->x.n : 0); k++) wcond += (double)type_width(maps->x.typ[k]);
     ProfScope ps(expr_mode ? "fused_compress" : "compress", c->stream);
     int64_t kk = 0, *toff = nullptr;
     // When the arena has room for every output at full length, the write pass is launched right
     // behind the count pass and the host reads the count while it runs; the arena top is then
     // pulled back to the end of the last output's selected rows.
     L maxb = 0; for(int k = 0; k < ny; k++) maxb += (type_bytes(y[k]->typ, n > 0 ? n : 1) + 255) & ~(L)255;
+    for(int j = 0; j < nm; j++) maxb += (type_bytes(maps->typ[j], n > 0 ? n : 1) + 255) & ~(L)255;
     const bool spec = n > 0 && c->dcur + maxb + 4096 + (16 + 2 * CP_BLOCK) * ceil_div(n, CP_TILE) <= c->dtop;
     uint16_t *mb = nullptr;
     e = expr_mode ? compress_count(c, *ep, nullptr, n, &toff, spec ? nullptr : &kk)
                   : compress_count(c, MaskPred{m}, m, n, &toff, spec ? nullptr : &kk, &mb);
     if(e){ arena_set(c, mark); R e; }
-    if(!spec && kk == n && n > 0 && !expr_mode){           // copyV alias (h_primitive.cc:1513-1515)
+    if(!spec && kk == n && n > 0 && !expr_mode && alias_ok){   // copyV alias (h_primitive.cc:1513-1515)
         ps.stop(); ps.bytes = (double)n;
         arena_set(c, mark);
         for(int k = 0; k < ny; k++) *z[k] = *y[k];
@@ -286,17 +327,25 @@ static int32_t compress_many(hpa_ctx *c, int ny, V *z, V x, V *y, bool expr_mode
         cols.dst[k] = (char*)dalloc(c, type_bytes(y[k]->typ, spec ? n : (kk > 0 ? kk : 1)));
         if(!cols.dst[k]){ arena_set(c, mark); R HPA_E_MEMORY; }
     }
-    e = expr_mode ? compress_write(c, *ep, n, cols, nullptr, toff) : compress_write(c, MaskPred{m}, n, cols, nullptr, toff, mb);
+    for(int j = 0; j < nm; j++){
+        maps->dst[j] = (char*)dalloc(c, type_bytes(maps->typ[j], spec ? n : (kk > 0 ? kk : 1)));
+        if(!maps->dst[j]){ arena_set(c, mark); R HPA_E_MEMORY; }
+    }
+    e = expr_mode ? compress_write(c, *ep, n, cols, nullptr, toff)
+                  : compress_write(c, MaskPred{m}, n, cols, nullptr, toff, mb, nm > 0 ? maps : nullptr);
     if(e) R e;
     if(spec){
         if((e = d2h(c, &kk, toff + ceil_div(n, CP_TILE), 8))) R e;
-        if(kk == n && !expr_mode){                          // copyV alias (h_primitive.cc:1513-1515)
+        if(kk == n && !expr_mode && alias_ok){              // copyV alias (h_primitive.cc:1513-1515)
             ps.stop(); ps.bytes = (double)n;
             arena_set(c, mark);
             for(int k = 0; k < ny; k++) *z[k] = *y[k];
             R 0;
         }
-        arena_set(c, (L)(cols.dst[ny - 1] - c->dbase) + ((type_bytes(y[ny - 1]->typ, kk > 0 ? kk : 1) + 255) & ~(L)255));
+        // pull the arena top back to the end of the last output's selected rows
+        const char *last = nm > 0 ? maps->dst[nm - 1] : cols.dst[ny - 1];
+        const int32_t lt = nm > 0 ? maps->typ[nm - 1] : y[ny - 1]->typ;
+        arena_set(c, (L)(last - c->dbase) + ((type_bytes(lt, kk > 0 ? kk : 1) + 255) & ~(L)255));
     }
     ps.stop();
     ps.bytes = wcond * (double)n + wsum * (double)kk;
@@ -311,6 +360,15 @@ static int32_t compress_many(hpa_ctx *c, int ny, V *z, V x, V *y, bool expr_mode
             if(zk->typ == HPA_Y) zk->p = y[k]->p;            // enum keeps its target (vy(z)=vy(y))
         }
     }
+    for(int j = 0; j < nm; j++){
+        V zj = zm[j];
+        const L w = type_width(maps->typ[j]);
+        zj->typ = maps->typ[j]; zj->len = kk;
+        if(kk <= 1){
+            zj->g = (char*)&zj->b; zj->i64 = 0;
+            if(kk == 1 && (e = d2h(c, &zj->b, maps->dst[j], w))) R e;
+        } else zj->g = maps->dst[j];
+    }
     R 0;
 }
 
@@ -319,6 +377,30 @@ extern "C" int32_t pfnCompress(hpa_V z, hpa_V x, hpa_V y){
     if(x->typ != HPA_B) R HPA_E_DOMAIN;
     if(x->len != y->len) R HPA_E_LENGTH;
     R compress_many(c, 1, &z, x, &y, false, nullptr, 0);
+}
+
+// The FA compress scan with FE maps fused (optimizer/autofusion.cc:814-817, 886-899): plain
+// columns z[k] = compress(mask, y[k]) and computed ones zm[j] = compress(mask, maps[j](x)) in the
+// one write pass, each map evaluated at the selected rows only.
+extern "C" int32_t hpaCompressMap(int32_t ny, hpa_V *z, hpa_V mask, hpa_V *y, int32_t nm, hpa_V *zm,
+                                  const int32_t *mtyp, const hpa_expr *maps, int32_t nx, hpa_V *x){
+    hpa_ctx *c = ctx_cur();
+    if(mask->typ != HPA_B) R HPA_E_DOMAIN;
+    if(nm < 0 || nm > CP_MAXM || ny < 0) R HPA_E_NOT_IMPL;
+    for(int k = 0; k < ny; k++) if(y[k]->len != mask->len) R HPA_E_LENGTH;
+    CpMaps mp; memset(&mp, 0, sizeof(mp)); mp.n = nm;
+    L n; int32_t e;
+    if((e = cols_view(c, nx, x, &mp.x, &n))) R e;
+    if(nx > 0 && n != mask->len) R HPA_E_LENGTH;
+    int32_t ct[EXPR_MAX_COLS]; for(int k = 0; k < nx; k++) ct[k] = x[k]->typ;
+    for(int j = 0; j < nm; j++){
+        if(!(mtyp[j] == HPA_B || mtyp[j] == HPA_I || mtyp[j] == HPA_L || mtyp[j] == HPA_E)) R HPA_E_TYPE;
+        mp.typ[j] = mtyp[j];
+        mp.hx[j] = maps + j;
+        if((e = expr_compile(maps + j, nx, ct, &mp.e[j]))) R e;
+    }
+    if(ny == 0 && nm == 0) R 0;
+    R compress_many(c, ny, z, mask, y, false, nullptr, 0, nm, zm, &mp);
 }
 
 extern "C" int32_t hpaCompressMany(int32_t ny, hpa_V *z, hpa_V mask, hpa_V *y){

@@ -192,17 +192,22 @@ def q3_plain(B, Dm, F, day=D.Q3_DAY, limit=10, segment=1):
     return res
 
 
+# (1 - discount) * price over columns (discount, price): q3.hir's revenue, rounded per operation
+Q3_REVENUE = expr([("I", 1), ("COL", 0), "SUB", ("COL", 1), "MUL"])
+
+
 def q3_fused(B, Dm, F, day=D.Q3_DAY, limit=10, segment=1):
     """Q3 as the optimising compiler emits it (src/horseir/v1/compile-static/cur/q3.h:1-56):
-    one mask per table and the multi-column peephole compress of its filtered columns, the FE
-    revenue map, group, the grouped sum as one FP loop over the group lists (q3_peephole_1,
+    one mask per table and the multi-column peephole compress of its filtered columns with the FE
+    revenue map fused into the fact table's compress, group, the grouped sum as one FP loop over the group lists (q3_peephole_1,
     cur/q3.h:30-50: each group's revenue summed through its row indices, nothing gathered
     first), join and the multi-column gathers."""
     d_mask = B.pfnAnd(B.pfnEq(Dm["segment"], B.scalar(segment, H_I)), B.pfnLt(Dm["order_day"], B.scalar(day, H_I)))
     lk, lday = B.compress_many(d_mask, [Dm["key"], Dm["order_day"]])
     f_mask = B.pfnGt(F["ship_day"], B.scalar(day, H_I))
-    rk, rp, rd = B.compress_many(f_mask, [F["fk"], F["price"], F["discount"]])
-    rev = B.fused_map(H_E, [rd, rp], expr([("I", 1), ("COL", 0), "SUB", ("COL", 1), "MUL"]))
+    # the fact compress with the FE revenue map fused: rk and (1 - discount) * price written in
+    # the one pass, price and discount never materialised compressed
+    (rk,), (rev,) = B.compress_map(f_mask, [F["fk"]], [(H_E, Q3_REVENUE)], [F["discount"], F["price"]])
     grp = B.pfnGroup(rk)
     gkeys, gvals = B.pfnKeys(grp), B.pfnValues(grp)
     gk = B.pfnIndex(rk, gkeys)

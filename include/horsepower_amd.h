@@ -298,6 +298,12 @@ int32_t hpaJitCompileCheck(int32_t nx, const int32_t *coltyp, const hpa_expr *co
 int32_t hpaIndexMany(int32_t nx, hpa_V *z, hpa_V *x, hpa_V y);
 /* Peephole compress with a ready mask: z[k] = compress(mask, y[k]). */
 int32_t hpaCompressMany(int32_t ny, hpa_V *z, hpa_V mask, hpa_V *y);
+/* The FA compress scan with FE maps fused (optimizer/autofusion.cc:814-817,886-899): z[k] =
+ * compress(mask, y[k]) for k < ny and zm[j] = compress(mask, maps[j](x)) as type mtyp[j] (B, I,
+ * L or E) for j < nm <= 2, in one write pass; each map is evaluated at the selected rows only.
+ * Outputs never alias the inputs (an all-true mask copies). */
+int32_t hpaCompressMap(int32_t ny, hpa_V *z, hpa_V mask, hpa_V *y, int32_t nm, hpa_V *zm,
+                       const int32_t *mtyp, const hpa_expr *maps, int32_t nx, hpa_V *x);
 /* FP1/FP2/FP5/FP6/FP7 grouped reductions (optimizer/pattern.cc:180-219): for each group list
  * t of y (flat or normal list of i64 row indices), z[g] = reduce_{j in t}(x[j]);
  * kind 0 sum, 1 avg, 2 min, 3 max, 4 len.  z is a vector over groups. */

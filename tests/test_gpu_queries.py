@@ -176,3 +176,27 @@ def test_fused_group_agg_key_order(gpu):
     assert got == want
     sums = [math.fsum(v[(sa == pa) & (sb == pb)]) for pa, pb in want]
     assert np.allclose(gpu.read(zs), sums, rtol=RTOL, atol=0)
+
+
+@pytest.mark.parametrize("n,sel", [(0, 0.5), (1, 1.0), (5000, 0.5), (1_000_003, 0.54), (200_000, 1.0), (70_000, 0.0)])
+def test_compress_map(gpu, orc, n, sel):
+    """hpaCompressMap: plain columns compressed and FE maps of other columns evaluated at the
+    selected rows, against the unfused compress + map sequence on the oracle (per-operation IEEE,
+    so bit for bit)."""
+    from horsepower_amd.vnode import H_B, H_E, H_I, H_L, expr
+    r = np.random.default_rng(n + 3)
+    m = (r.random(n) < sel).astype(np.uint8)
+    k = r.integers(0, 1 << 30, n).astype(np.int32)
+    d = r.integers(0, 11, n) / 100.0
+    p = r.integers(90000, 10500000, n) / 100.0
+    q = r.integers(1, 50, n).astype(np.int32)
+    rev = expr([("I", 1), ("COL", 0), "SUB", ("COL", 1), "MUL"])
+    big = expr([("COL", 2), ("I", 24), "LT"])
+    (zk,), (zr, zb) = gpu.compress_map(gpu.vec(m), [gpu.vec(k)], [(H_E, rev), (H_B, big)],
+                                       [gpu.vec(d), gpu.vec(p), gpu.vec(q)])
+    mo = orc.vec(m)
+    ok = orc.read(orc.pfnCompress(mo, orc.vec(k)))
+    orv = orc.read(orc.pfnMul(orc.pfnMinus(orc.scalar(1, H_L), orc.pfnCompress(mo, orc.vec(d))), orc.pfnCompress(mo, orc.vec(p))))
+    assert np.array_equal(gpu.read(zk), ok)
+    assert np.array_equal(gpu.read(zr), orv)
+    assert np.array_equal(gpu.read(zb), (q[m != 0] < 24).astype(np.uint8))
