@@ -3,7 +3,8 @@
 // The reference groups and orders with serial radix/merge/quick sorts (backend/h_libs.cc:375-736,
 // 1184-1403, 1912-1940; h_primitive.cc:993-1057, 1760-1816, 2656-2681).  Every group and order
 // output is a function of a *stable* sort on a per-case key:
-//   - i32 group / i32 order:  unsigned 32-bit key (4 byte-digit LSD on two's complement bytes)
+//   - i32 group / i32 order:  unsigned 32-bit key (the bit pattern: the reference's byte-digit
+//                             LSD on two's complement bytes orders the same way)
 //   - i64 group / i64 order:  unsigned 64-bit key, or its low 32 bits when no key has
 //                             (x >> 32) > 0 (lib_radixsort_core_long, h_libs.cc:1240-1266)
 //   - two i32 columns, equal flags: packed ((x0 << 32) | x1), x1 sign-extended
@@ -13,9 +14,10 @@
 // out in descending row order as in the reference.
 //
 // Radix pass: 4096-key tiles; per-tile digit histogram; exclusive scan in (digit, tile) order;
-// scatter with each key's stable rank inside its tile computed from eight wavefront ballots
-// (lanes sharing the digit), staged through LDS so the writes go out as digit runs.  Digits that
-// are constant over all keys are skipped.  (A single-pass variant with decoupled look-back
+// scatter with each key's stable rank inside its tile computed from one wavefront ballot per
+// digit bit (lanes sharing the digit), staged through LDS so the writes go out as digit runs.
+// 32-bit keys take digits of up to 10 bits (a 20-bit key range sorts in two passes), 64-bit keys
+// bytes; digits that are constant over all keys are skipped.  (A single-pass variant with decoupled look-back
 // between tiles was measured slower here: its look-back chains wait on device-scope atomics.)
 #include "hpa_internal.h"
 
@@ -23,20 +25,21 @@
 #define RS_ROWS  16
 #define RS_TILE  (RS_BLOCK * RS_ROWS)
 
-// Digit histogram of one 4096-key tile, counted in per-wave LDS histograms; a wave whose valid
-// keys all share the digit adds them with one increment.  hist[tile * 256 + d]: each tile writes
-// one contiguous 1 KB row.  With `bits`
-// set (the first pass) it also writes the tile's OR and AND of all keys, and whether some key is
-// below the key before it (on the sort's bits, `kmask`), to bits[4*tile .. 4*tile+2]: which key
-// bits vary, and whether the input is already in key order.
-template<typename K>
-__global__ void __launch_bounds__(RS_BLOCK) k_rs_hist(const K *keys, L n, int shift, uint32_t *hist, L ntiles,
+// Digit histogram of one 4096-key tile (2^DB bins; the digit is (key >> shift) & dmask, dmask at
+// most 2^DB - 1), counted in per-wave LDS histograms; a wave whose valid keys all share the digit
+// adds them with one increment.  hist[tile * 2^DB + d]: each tile writes one contiguous row.  With
+// `bits` set (the first pass) it also writes the tile's OR and AND of all keys, and whether some
+// key is below the key before it (on the sort's bits, `kmask`), to bits[4*tile .. 4*tile+2]:
+// which key bits vary, and whether the input is already in key order.
+template<typename K, int DB>
+__global__ void __launch_bounds__(RS_BLOCK) k_rs_hist(const K *keys, L n, int shift, unsigned dmask, uint32_t *hist, L ntiles,
                                                       unsigned long long *bits, K kmask){
-    __shared__ uint32_t cnt[4][256];
+    constexpr int NB = 1 << DB;
+    __shared__ uint32_t cnt[4][NB];
     __shared__ unsigned long long so[RS_BLOCK / 64], sa[RS_BLOCK / 64];
     __shared__ int sd[RS_BLOCK / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for(int q = 0; q < 4; q++) cnt[q][threadIdx.x] = 0;
+    for(int q = 0; q < 4; q++) for(int d = threadIdx.x; d < NB; d += RS_BLOCK) cnt[q][d] = 0;
     __syncthreads();
     const L base = (L)blockIdx.x * RS_TILE + (L)w * (RS_TILE / 4);
     unsigned long long o = 0, a = ~0ull;
@@ -51,7 +54,7 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_hist(const K *keys, L n, int sh
         const L i = base + (L)r * 64 + lane;
         const bool valid = i < n;
         const K k = kr[r];
-        const unsigned d = (unsigned)((k >> shift) & 255);
+        const unsigned d = (unsigned)((k >> shift) & (K)dmask);
         const unsigned d0 = __builtin_amdgcn_readfirstlane(d);
         const uint64_t vm = __ballot(valid);
         if(vm == 0) continue;
@@ -72,7 +75,7 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_hist(const K *keys, L n, int sh
         if(lane == 0){ so[w] = o; sa[w] = a; sd[w] = anyd ? 1 : 0; }
     }
     __syncthreads();
-    hist[(L)blockIdx.x * 256 + threadIdx.x] = cnt[0][threadIdx.x] + cnt[1][threadIdx.x] + cnt[2][threadIdx.x] + cnt[3][threadIdx.x];
+    for(int d = threadIdx.x; d < NB; d += RS_BLOCK) hist[(L)blockIdx.x * NB + d] = cnt[0][d] + cnt[1][d] + cnt[2][d] + cnt[3][d];
     if(bits && threadIdx.x == 0){
         int dd = sd[0];
         for(int q = 1; q < RS_BLOCK / 64; q++){ o |= so[q]; a &= sa[q]; dd |= sd[q]; }
@@ -94,22 +97,26 @@ __global__ void __launch_bounds__(256) k_rs_bits_final(const unsigned long long 
 }
 
 // Scatter of one tile: every key's stable rank inside its wave's quarter of the tile comes from
-// eight ballots (lanes sharing the digit); the tile is then laid out digit by digit in LDS and
+// DB ballots (lanes sharing the digit); the tile is then laid out digit by digit in LDS and
 // written out in that order, so consecutive threads write consecutive addresses of one digit's
-// run instead of 4096 scattered 4/8-byte stores.  off[tile * 256 + d]: output start of the
-// tile's keys of digit d.  Block b takes tile xcd_tile(b): the tiles in flight on one XCD are
-// neighbours, and the runs they write for one digit are adjacent in the output.  FINAL: only the
-// values are written, widened to i64 into out64 at position pos, or n - 1 - pos with `reverse`.
-template<typename K, bool FINAL>
+// run instead of 4096 scattered 4/8-byte stores.  off[tile * 2^DB + d]: output start of the
+// tile's keys of digit d (positions fit 32 bits: sorts take fewer than 2^32 keys).  Block b takes
+// tile xcd_tile(b): the tiles in flight on one XCD are neighbours, and the runs they write for one
+// digit are adjacent in the output.  MODE 0: keys and values to kout / vout; MODE 1: only the
+// values, widened to i64 into out64 at position pos, or n - 1 - pos with `reverse`; MODE 2: the
+// keys to kout and the widened values to out64 (no reversal).
+template<typename K, int MODE, int DB>
 __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K *kin, const uint32_t *vin, K *kout, uint32_t *vout,
-                                                         int64_t *out64, int reverse, L n, int shift, const int64_t *off, L ntiles){
-    __shared__ uint32_t wcnt[4][256];          // per-wave digit counts, then per-wave digit starts
-    __shared__ int64_t gbase[256];             // global position of a digit's run minus its tile-local start
+                                                         int64_t *out64, int reverse, L n, int shift, unsigned dmask,
+                                                         const uint32_t *off, L ntiles){
+    constexpr int NB = 1 << DB, PER = NB / RS_BLOCK;   // digits per thread in the tile's digit scan
+    __shared__ uint32_t wcnt[4][NB];           // per-wave digit counts, then per-wave digit starts
+    __shared__ uint32_t gbase[NB];             // output position of a digit's run minus its tile-local start (>= 0)
     __shared__ uint32_t wsum[RS_BLOCK / 64];
     __shared__ K sk[RS_TILE];
     __shared__ uint32_t sv[RS_TILE];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for(int q = 0; q < 4; q++) wcnt[q][threadIdx.x] = 0;
+    for(int q = 0; q < 4; q++) for(int d = threadIdx.x; d < NB; d += RS_BLOCK) wcnt[q][d] = 0;
     __syncthreads();
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
     const L t0 = (L)tile * RS_TILE;
@@ -127,10 +134,10 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K *kin, const uin
     for(int r = 0; r < RS_ROWS; r++){
         L i = tbase + (L)r * 64 + lane;
         bool valid = i < n;
-        unsigned d = (unsigned)((kr[r] >> shift) & 255);
+        unsigned d = (unsigned)((kr[r] >> shift) & (K)dmask);
         uint64_t peers = __ballot(valid);
         #pragma unroll
-        for(int b = 0; b < 8; b++){
+        for(int b = 0; b < DB; b++){
             uint64_t bb = __ballot((d >> b) & 1);
             peers &= ((d >> b) & 1) ? bb : ~bb;
         }
@@ -141,26 +148,32 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K *kin, const uin
     }
     __syncthreads();
     {
-        // tile-local start of each digit (exclusive scan over the 256 digit totals), then the
-        // start of each wave's share inside its digit
-        const unsigned d = threadIdx.x;
-        const uint32_t c0 = wcnt[0][d], c1 = wcnt[1][d], c2 = wcnt[2][d], c3 = wcnt[3][d];
-        const uint32_t tot = c0 + c1 + c2 + c3;
+        // tile-local start of each digit (exclusive scan over the digit totals; thread t owns
+        // digits [t * PER, t * PER + PER)), then the start of each wave's share inside its digit
+        uint32_t tot = 0;
+        #pragma unroll
+        for(int j = 0; j < PER; j++){ const int d = threadIdx.x * PER + j; tot += wcnt[0][d] + wcnt[1][d] + wcnt[2][d] + wcnt[3][d]; }
         uint32_t inc = tot;
         for(int o = 1; o < 64; o <<= 1){ uint32_t t = __shfl_up(inc, o, 64); if(lane >= o) inc += t; }
         if(lane == 63) wsum[w] = inc;
         __syncthreads();
         uint32_t pre = inc - tot;
         for(int q = 0; q < w; q++) pre += wsum[q];
-        wcnt[0][d] = pre; wcnt[1][d] = pre + c0; wcnt[2][d] = pre + c0 + c1; wcnt[3][d] = pre + c0 + c1 + c2;
-        gbase[d] = off[(L)tile * 256 + d] - (int64_t)pre;
+        #pragma unroll
+        for(int j = 0; j < PER; j++){
+            const int d = threadIdx.x * PER + j;
+            const uint32_t c0 = wcnt[0][d], c1 = wcnt[1][d], c2 = wcnt[2][d], c3 = wcnt[3][d];
+            wcnt[0][d] = pre; wcnt[1][d] = pre + c0; wcnt[2][d] = pre + c0 + c1; wcnt[3][d] = pre + c0 + c1 + c2;
+            gbase[d] = off[(L)tile * NB + d] - pre;
+            pre += c0 + c1 + c2 + c3;
+        }
     }
     __syncthreads();
     #pragma unroll
     for(int r = 0; r < RS_ROWS; r++){
         L i = tbase + (L)r * 64 + lane;
         if(i < n){
-            unsigned d = (unsigned)((kr[r] >> shift) & 255);
+            unsigned d = (unsigned)((kr[r] >> shift) & (K)dmask);
             uint32_t lp = wcnt[w][d] + rk[r];
             sk[lp] = kr[r]; sv[lp] = vr[r];
         }
@@ -169,35 +182,44 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K *kin, const uin
     const int cnt = (int)(n - t0 < (L)RS_TILE ? n - t0 : (L)RS_TILE);
     for(int j = threadIdx.x; j < cnt; j += RS_BLOCK){
         const K k = sk[j];
-        const int64_t pos = gbase[(unsigned)((k >> shift) & 255)] + j;
-        if(FINAL) out64[reverse ? n - 1 - pos : pos] = (int64_t)sv[j];
+        const L pos = (L)gbase[(unsigned)((k >> shift) & (K)dmask)] + j;
+        if(MODE == 1) out64[reverse ? n - 1 - pos : pos] = (int64_t)sv[j];
+        else if(MODE == 2){ kout[pos] = k; out64[pos] = (int64_t)sv[j]; }
         else { kout[pos] = k; vout[pos] = sv[j]; }
     }
 }
-// Exclusive scan of tile-major digit counts in digit-major order:
-//   off[t * 256 + d] = (counts of all digits below d) + (counts of digit d in tiles before t).
-// Thread (d, q) of a 1024-thread block owns digit d and a quarter q of the rows, so every
-// access is one contiguous 1 KB row of 256 counts, and each thread issues its loads together.
-// Three launches: per-chunk sums of each digit (32 tiles a chunk), chunk bases per digit, then
-// each chunk writes its tiles' offsets.
+// Exclusive scan of tile-major digit counts (NB digits) in digit-major order:
+//   off[t * NB + d] = (counts of all digits below d) + (counts of digit d in tiles before t).
+// Thread (d, q) of a 1024-thread block owns digit d and the q-th of Q = 1024 / NB slices of the
+// block's tiles, so every access is one contiguous row of NB counts, and each thread issues its
+// loads together.  Three launches: per-chunk sums of each digit (32 tiles a chunk), chunk bases
+// per digit, then each chunk writes its tiles' offsets.
 #define DS_PER 32
+template<int NB>
 __global__ void __launch_bounds__(1024) k_ds_part(const uint32_t *cnt, L ntiles, uint32_t *part){
-    __shared__ uint32_t ps[4][256];
-    const unsigned d = threadIdx.x & 255, q = threadIdx.x >> 8;
-    const L t0 = (L)blockIdx.x * DS_PER + q * (DS_PER / 4);
-    uint32_t v[DS_PER / 4];
+    constexpr int Q = 1024 / NB, TQ = DS_PER / Q;
+    __shared__ uint32_t ps[Q][NB];
+    const unsigned d = threadIdx.x % NB, q = threadIdx.x / NB;
+    const L t0 = (L)blockIdx.x * DS_PER + q * TQ;
+    uint32_t v[TQ];
     #pragma unroll
-    for(int j = 0; j < DS_PER / 4; j++){ const L t = t0 + j; v[j] = t < ntiles ? cnt[t * 256 + d] : 0u; }
+    for(int j = 0; j < TQ; j++){ const L t = t0 + j; v[j] = t < ntiles ? cnt[t * NB + d] : 0u; }
     uint32_t s = 0;
     #pragma unroll
-    for(int j = 0; j < DS_PER / 4; j++) s += v[j];
+    for(int j = 0; j < TQ; j++) s += v[j];
     ps[q][d] = s;
     __syncthreads();
-    if(q == 0) part[(L)blockIdx.x * 256 + d] = ps[0][d] + ps[1][d] + ps[2][d] + ps[3][d];
+    if(q == 0){
+        uint32_t t = 0;
+        #pragma unroll
+        for(int r = 0; r < Q; r++) t += ps[r][d];
+        part[(L)blockIdx.x * NB + d] = t;
+    }
 }
-// chunk bases relative to their digit: cbase[c * 256 + d] = counts of digit d in chunks before
+// chunk bases relative to their digit: cbase[c * NB + d] = counts of digit d in chunks before
 // c, and tot[d] = all counts of digit d.  Block g takes digits 32g..32g+31; thread (dl, q) sums
 // digit 32g + dl over the q-th of 32 slices of the chunks.
+template<int NB>
 __global__ void __launch_bounds__(1024) k_ds_mid(const uint32_t *part, int nchunks, uint32_t *cbase, uint32_t *tot){
     __shared__ uint32_t qs[32][33];
     const unsigned dl = threadIdx.x & 31, q = threadIdx.x >> 5, d = blockIdx.x * 32 + dl;
@@ -206,7 +228,7 @@ __global__ void __launch_bounds__(1024) k_ds_mid(const uint32_t *part, int nchun
     for(int cb = c0; cb < c1; cb += 8){
         uint32_t v[8];
         #pragma unroll
-        for(int j = 0; j < 8; j++) v[j] = cb + j < c1 ? part[(L)(cb + j) * 256 + d] : 0u;
+        for(int j = 0; j < 8; j++) v[j] = cb + j < c1 ? part[(L)(cb + j) * NB + d] : 0u;
         #pragma unroll
         for(int j = 0; j < 8; j++) s += v[j];
     }
@@ -218,58 +240,61 @@ __global__ void __launch_bounds__(1024) k_ds_mid(const uint32_t *part, int nchun
     for(int cb = c0; cb < c1; cb += 8){
         uint32_t v[8];
         #pragma unroll
-        for(int j = 0; j < 8; j++) v[j] = cb + j < c1 ? part[(L)(cb + j) * 256 + d] : 0u;
+        for(int j = 0; j < 8; j++) v[j] = cb + j < c1 ? part[(L)(cb + j) * NB + d] : 0u;
         #pragma unroll
-        for(int j = 0; j < 8; j++) if(cb + j < c1){ cbase[(L)(cb + j) * 256 + d] = run; run += v[j]; }
+        for(int j = 0; j < 8; j++) if(cb + j < c1){ cbase[(L)(cb + j) * NB + d] = run; run += v[j]; }
     }
 }
-// off of the chunk's 32 tiles: the digit base (exclusive scan of the 256 digit totals, done by
+// off of the chunk's 32 tiles: the digit base (exclusive scan of the NB digit totals, done by
 // every block), the chunk base, and the counts of earlier tiles in the chunk
+template<int NB>
 __global__ void __launch_bounds__(1024) k_ds_down(const uint32_t *cnt, L ntiles, const uint32_t *cbase, const uint32_t *tot,
-                                                  int64_t *off){
-    __shared__ uint32_t ps[4][256];
-    __shared__ int64_t wtot[4], dbase[256];
-    const unsigned d = threadIdx.x & 255, q = threadIdx.x >> 8, lane = threadIdx.x & 63;
-    const L t0 = (L)blockIdx.x * DS_PER + q * (DS_PER / 4);
-    const uint32_t cb = cbase[(L)blockIdx.x * 256 + d];
+                                                  uint32_t *off){
+    constexpr int Q = 1024 / NB, TQ = DS_PER / Q;
+    __shared__ uint32_t ps[Q][NB];
+    __shared__ uint32_t wtot[NB / 64], dbase[NB];
+    const unsigned d = threadIdx.x % NB, q = threadIdx.x / NB, lane = threadIdx.x & 63;
+    const L t0 = (L)blockIdx.x * DS_PER + q * TQ;
+    const uint32_t cb = cbase[(L)blockIdx.x * NB + d];
     const uint32_t td = q == 0 ? tot[d] : 0u;
-    uint32_t v[DS_PER / 4];
+    uint32_t v[TQ];
     #pragma unroll
-    for(int j = 0; j < DS_PER / 4; j++){ const L t = t0 + j; v[j] = t < ntiles ? cnt[t * 256 + d] : 0u; }
+    for(int j = 0; j < TQ; j++){ const L t = t0 + j; v[j] = t < ntiles ? cnt[t * NB + d] : 0u; }
     uint32_t s = 0;
     #pragma unroll
-    for(int j = 0; j < DS_PER / 4; j++) s += v[j];
+    for(int j = 0; j < TQ; j++) s += v[j];
     ps[q][d] = s;
-    int64_t inc = 0;
+    uint32_t inc = 0;
     if(q == 0){
         inc = td;
-        for(int o = 1; o < 64; o <<= 1){ const int64_t t = __shfl_up(inc, o, 64); if((int)lane >= o) inc += t; }
+        for(int o = 1; o < 64; o <<= 1){ const uint32_t t = __shfl_up(inc, o, 64); if((int)lane >= o) inc += t; }
         if(lane == 63) wtot[d >> 6] = inc;
     }
     __syncthreads();
     if(q == 0){
-        int64_t pre = inc - td;
+        uint32_t pre = inc - td;
         for(unsigned w = 0; w < (d >> 6); w++) pre += wtot[w];
         dbase[d] = pre;
     }
     __syncthreads();
-    int64_t run = dbase[d] + cb;
+    uint32_t run = dbase[d] + cb;
     for(unsigned r = 0; r < q; r++) run += ps[r][d];
     #pragma unroll
-    for(int j = 0; j < DS_PER / 4; j++){
+    for(int j = 0; j < TQ; j++){
         const L t = t0 + j;
-        if(t < ntiles){ off[t * 256 + d] = run; run += v[j]; }
+        if(t < ntiles){ off[t * NB + d] = run; run += v[j]; }
     }
 }
 static L digit_scan_chunks(L ntiles){ R ceil_div(ntiles, DS_PER); }
-// part: 256 * chunks u32, cbase: 256 * chunks u32 (chunks = digit_scan_chunks), tot: 256 u32
-static int32_t digit_scan(hpa_ctx *c, const uint32_t *cnt, L ntiles, uint32_t *part, uint32_t *cbase, uint32_t *tot, int64_t *off){
+// part, cbase: NB * chunks u32 (chunks = digit_scan_chunks), tot: NB u32
+template<int NB>
+static int32_t digit_scan(hpa_ctx *c, const uint32_t *cnt, L ntiles, uint32_t *part, uint32_t *cbase, uint32_t *tot, uint32_t *off){
     const L nchunks = digit_scan_chunks(ntiles);
     if(nchunks > INT32_MAX){ set_error_detail("digit scan of %lld tiles", (long long)ntiles); R HPA_E_NOT_IMPL; }
-    ProfScope ps("radix_scan", c->stream, 12.0 * 256 * ntiles);
-    hipLaunchKernelGGL(k_ds_part, dim3((unsigned)nchunks), dim3(1024), 0, c->stream, cnt, ntiles, part);
-    hipLaunchKernelGGL(k_ds_mid, dim3(8), dim3(1024), 0, c->stream, (const uint32_t*)part, (int)nchunks, cbase, tot);
-    hipLaunchKernelGGL(k_ds_down, dim3((unsigned)nchunks), dim3(1024), 0, c->stream, cnt, ntiles, (const uint32_t*)cbase,
+    ProfScope ps("radix_scan", c->stream, 12.0 * NB * ntiles);
+    hipLaunchKernelGGL(k_ds_part<NB>, dim3((unsigned)nchunks), dim3(1024), 0, c->stream, cnt, ntiles, part);
+    hipLaunchKernelGGL(k_ds_mid<NB>, dim3(NB / 32), dim3(1024), 0, c->stream, (const uint32_t*)part, (int)nchunks, cbase, tot);
+    hipLaunchKernelGGL(k_ds_down<NB>, dim3((unsigned)nchunks), dim3(1024), 0, c->stream, cnt, ntiles, (const uint32_t*)cbase,
                        (const uint32_t*)tot, off);
     R check_launch("radix_scan");
 }
@@ -278,44 +303,82 @@ __global__ void k_iota_i64(int64_t *z, L n, int reverse){
     for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x) z[i] = reverse ? n - 1 - i : i;
 }
 
+// One LSD pass over digit (key >> s) & dmask with 2^DB bins: histogram (unless `have_hist`), digit
+// scan, scatter (to kb / vb; with out64 the values to out64 as i64, and the keys to kb when kb is
+// set).
+template<typename K, int DB>
+static int32_t radix_pass(hpa_ctx *c, const K *ka, const uint32_t *va, K *kb, uint32_t *vb, int64_t *out64, bool reverse,
+                          L n, L ntiles, int s, unsigned dmask, bool have_hist, uint32_t *hist, uint32_t *part, uint32_t *cbase,
+                          uint32_t *tot, uint32_t *off){
+    constexpr int NB = 1 << DB;
+    int32_t e;
+    if(!have_hist){
+        ProfScope ps("radix_hist", c->stream, (double)n * sizeof(K) + 4.0 * NB * ntiles);
+        hipLaunchKernelGGL((k_rs_hist<K, DB>), dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream, ka, n, s, dmask, hist, ntiles,
+                           (unsigned long long*)nullptr, (K)0);
+    }
+    if((e = digit_scan<NB>(c, hist, ntiles, part, cbase, tot, off))) R e;
+    const double rd = (double)n * (sizeof(K) + (va ? 4 : 0)),
+                 wr = out64 ? 8.0 * n + (kb ? (double)n * sizeof(K) : 0.0) : (double)n * (sizeof(K) + 4);
+    ProfScope ps("radix_scatter", c->stream, rd + wr);
+    if(out64 && kb)
+        hipLaunchKernelGGL((k_rs_scatter<K, 2, DB>), dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream,
+                           ka, va, kb, (uint32_t*)nullptr, out64, 0, n, s, dmask, (const uint32_t*)off, ntiles);
+    else if(out64)
+        hipLaunchKernelGGL((k_rs_scatter<K, 1, DB>), dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream,
+                           ka, va, (K*)nullptr, (uint32_t*)nullptr, out64, reverse ? 1 : 0, n, s, dmask, (const uint32_t*)off, ntiles);
+    else
+        hipLaunchKernelGGL((k_rs_scatter<K, 0, DB>), dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream,
+                           ka, va, kb, vb, (int64_t*)nullptr, 0, n, s, dmask, (const uint32_t*)off, ntiles);
+    R check_launch("radix_sort");
+}
+
 // Stable sort of (keys, vals) by key bits [lo, hi); vals may be null (row ids 0..n-1).  The
 // inputs are only read.  Without out64 the sorted keys and values come back in *kout / *vout
 // (the input keys, and the input values or fresh row ids, when every digit is constant).  With
 // out64 the sorted values are written there as i64, reversed with `reverse`, and *kout / *vout
-// are not set.  Each pass: per-tile digit histograms (the first one also finds the key bits that
-// vary, so passes over constant digits are skipped), a scan in (digit, tile) order, a scatter.
+// are not set -- unless `keep_keys` (no reversal): the sorted keys then come back in *kout too.
+//
+// Digits: 32-bit keys are sorted with digits of up to RS_WIDE bits (1024 bins), 64-bit keys with
+// bytes.  The first histogram pass, over the lowest digit, also finds the key bits that vary and
+// whether the keys are already in order; the remaining varying bits are then covered by as few
+// passes as possible (bytes when wide digits would not save a pass), and digits whose bits are
+// all constant are skipped.  Each pass: per-tile digit histograms, a scan in (digit, tile) order,
+// a scatter.
+#define RS_WIDE 10
 template<typename K>
 static int32_t radix_sort_t(hpa_ctx *c, const K *keys, const uint32_t *vals, L n, int lo, int hi, K **kout, uint32_t **vout,
-                            int64_t *out64, bool reverse){
+                            int64_t *out64, bool reverse, bool keep_keys = false){
+    if(keep_keys && (!out64 || reverse)) R HPA_E_DOMAIN;
     if(n >= (L)UINT32_MAX){ set_error_detail("radix sort of %lld keys exceeds 32-bit row ids", (long long)n); R HPA_E_NOT_IMPL; }
     int32_t e;
-    if(n <= 0){ if(!out64){ *kout = (K*)keys; *vout = (uint32_t*)vals; } R 0; }
+    if(n <= 0){ if(!out64){ *kout = (K*)keys; *vout = (uint32_t*)vals; } else if(keep_keys) *kout = (K*)keys; R 0; }
+    constexpr int kw = (int)(8 * sizeof(K)), wide = sizeof(K) == 4 ? RS_WIDE : 8, NBmax = 1 << wide;
+    if(lo < 0 || hi > kw || lo >= hi){ set_error_detail("radix sort of key bits [%d, %d)", lo, hi); R HPA_E_DOMAIN; }
     const L ntiles = ceil_div(n, RS_TILE);
     unsigned long long *bits = (unsigned long long*)dalloc(c, 32 + 32 * ntiles);
-    uint32_t *hist = (uint32_t*)dalloc(c, 4 * 256 * ntiles);
-    int64_t *off = (int64_t*)dalloc(c, 8 * 256 * ntiles);
-    uint32_t *part = (uint32_t*)dalloc(c, 4 * 256 * digit_scan_chunks(ntiles));
-    uint32_t *cbase = (uint32_t*)dalloc(c, 4 * 256 * digit_scan_chunks(ntiles));
-    uint32_t *tot = (uint32_t*)dalloc(c, 4 * 256);
+    uint32_t *hist = (uint32_t*)dalloc(c, 4 * NBmax * ntiles);
+    uint32_t *off = (uint32_t*)dalloc(c, 4 * NBmax * ntiles);
+    uint32_t *part = (uint32_t*)dalloc(c, 4 * NBmax * digit_scan_chunks(ntiles));
+    uint32_t *cbase = (uint32_t*)dalloc(c, 4 * NBmax * digit_scan_chunks(ntiles));
+    uint32_t *tot = (uint32_t*)dalloc(c, 4 * NBmax);
     if(!bits || !hist || !off || !part || !cbase || !tot) R HPA_E_MEMORY;
-    // digits to sort by: the first histogram pass also finds the bits that vary, and whether
-    // the keys are already in order on the bits the digit passes see: [lo, lo + 8 * digits)
-    const int kw = (int)(8 * sizeof(K)), dw = (int)ceil_div(hi - lo, 8) * 8;
-    const K kmask = (K)((lo + dw >= kw ? ~(K)0 >> lo : (((K)1 << dw) - 1)) << lo);
+    const K kmask = (K)((hi - lo >= kw ? ~(K)0 : (((K)1 << (hi - lo)) - 1)) << lo);
+    auto dmask_of = [&](int s, int db) -> unsigned { const int w = hi - s < db ? hi - s : db; R (1u << w) - 1u; };
     {
-        ProfScope ps("radix_hist", c->stream, (double)n * sizeof(K) + 4.0 * 256 * ntiles);
-        hipLaunchKernelGGL(k_rs_hist<K>, dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream, keys, n, lo, hist, ntiles, bits + 4, kmask);
+        ProfScope ps("radix_hist", c->stream, (double)n * sizeof(K) + 4.0 * NBmax * ntiles);
+        hipLaunchKernelGGL((k_rs_hist<K, wide>), dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream, keys, n, lo, dmask_of(lo, wide),
+                           hist, ntiles, bits + 4, kmask);
         hipLaunchKernelGGL(k_rs_bits_final, dim3(1), dim3(256), 0, c->stream, bits + 4, ntiles, bits);
     }
     if((e = check_launch("radix_hist"))) R e;
     unsigned long long hb[3]; if((e = d2h(c, hb, bits, 24))) R e;
-    const unsigned long long vary = hb[0] ^ hb[1];
-    int run[8], nrun = 0;
-    for(int s = lo; s < hi; s += 8) if((vary >> s) & 255ull) run[nrun++] = s;
-    if(nrun == 0 || hb[2] == 0){
+    const unsigned long long vary = (hb[0] ^ hb[1]) & (unsigned long long)kmask;
+    if(vary == 0 || hb[2] == 0){
         // every digit is constant, or no key is below the one before: the input order is the
         // stable sorted order
         if(out64){
+            if(keep_keys) *kout = (K*)keys;
             hipLaunchKernelGGL(k_iota_i64, dim3(grid_for(n, 256, 4)), dim3(256), 0, c->stream, out64, n, reverse ? 1 : 0);
             R check_launch("radix_sort");
         }
@@ -325,32 +388,34 @@ static int32_t radix_sort_t(hpa_ctx *c, const K *keys, const uint32_t *vals, L n
         if((e = iota_u32(c, v1, n))) R e;
         *vout = v1; R 0;
     }
-    K *kb[2] = {nullptr, nullptr}; uint32_t *vb[2] = {nullptr, nullptr};
-    const int nout = out64 ? nrun - 1 : nrun, nbuf = nout < 2 ? nout : 2;
-    for(int q = 0; q < nbuf; q++){
-        kb[q] = (K*)dalloc(c, sizeof(K) * n); vb[q] = (uint32_t*)dalloc(c, 4 * n);
-        if(!kb[q] || !vb[q]) R HPA_E_MEMORY;
+    // the digit plan: the first digit [lo, lo + wide), then digits of `step` bits up to the top
+    // varying bit
+    const int vhi = 64 - __builtin_clzll(vary);
+    struct Digit { int s, db; };
+    Digit plan[64]; int nrun = 0;
+    if((vary >> lo) & dmask_of(lo, wide)) plan[nrun++] = {lo, wide};
+    const int r0 = lo + wide, rest = vhi - r0;
+    if(rest > 0){
+        const int step = (int)ceil_div(rest, 8) == (int)ceil_div(rest, wide) ? 8 : wide;
+        for(int s = r0; s < vhi; s += step) if((vary >> s) & dmask_of(s, step)) plan[nrun++] = {s, step};
     }
+    K *kb[2] = {nullptr, nullptr}; uint32_t *vb[2] = {nullptr, nullptr};
+    const int nout = out64 ? nrun - 1 : nrun, nbuf = nout < 2 ? nout : 2, nkb = keep_keys ? (nrun < 2 ? nrun : 2) : nbuf;
+    for(int q = 0; q < nkb; q++){ kb[q] = (K*)dalloc(c, sizeof(K) * n); if(!kb[q]) R HPA_E_MEMORY; }
+    for(int q = 0; q < nbuf; q++){ vb[q] = (uint32_t*)dalloc(c, 4 * n); if(!vb[q]) R HPA_E_MEMORY; }
     const K *ka = keys; const uint32_t *va = vals;
     for(int r = 0; r < nrun; r++){
-        const int s = run[r];
-        const bool last = r == nrun - 1;
-        if(!(r == 0 && s == lo)){
-            ProfScope ps("radix_hist", c->stream, (double)n * sizeof(K) + 4.0 * 256 * ntiles);
-            hipLaunchKernelGGL(k_rs_hist<K>, dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream, ka, n, s, hist, ntiles,
-                               (unsigned long long*)nullptr, (K)0);
-        }
-        if((e = digit_scan(c, hist, ntiles, part, cbase, tot, off))) R e;
-        const double rd = (double)n * (sizeof(K) + (va ? 4 : 0)), wr = last && out64 ? 8.0 * n : (double)n * (sizeof(K) + 4);
-        ProfScope ps("radix_scatter", c->stream, rd + wr);
-        if(last && out64)
-            hipLaunchKernelGGL((k_rs_scatter<K, true>), dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream,
-                               ka, va, (K*)nullptr, (uint32_t*)nullptr, out64, reverse ? 1 : 0, n, s, off, ntiles);
-        else
-            hipLaunchKernelGGL((k_rs_scatter<K, false>), dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream,
-                               ka, va, kb[r & 1], vb[r & 1], (int64_t*)nullptr, 0, n, s, off, ntiles);
-        if((e = check_launch("radix_sort"))) R e;
-        if(!(last && out64)){ ka = kb[r & 1]; va = vb[r & 1]; }
+        const int s = plan[r].s, db = plan[r].db;
+        const bool last = r == nrun - 1, fin = last && out64 != nullptr;
+        const bool have = r == 0 && s == lo && db == wide;
+        K *kd = fin && !keep_keys ? nullptr : kb[r & 1];
+        e = db == 8 ? radix_pass<K, 8>(c, ka, va, kd, vb[r & 1], fin ? out64 : nullptr, reverse, n, ntiles, s, dmask_of(s, 8),
+                                       have, hist, part, cbase, tot, off)
+                    : radix_pass<K, wide>(c, ka, va, kd, vb[r & 1], fin ? out64 : nullptr, reverse, n, ntiles, s,
+                                          dmask_of(s, wide), have, hist, part, cbase, tot, off);
+        if(e) R e;
+        if(!fin){ ka = kb[r & 1]; va = vb[r & 1]; }
+        else if(keep_keys) *kout = kd;
     }
     if(!out64){ *kout = (K*)ka; *vout = (uint32_t*)va; }
     R 0;
@@ -360,6 +425,11 @@ int32_t radix_sort_pairs_u64(hpa_ctx *c, const uint64_t *keys, const uint32_t *v
 }
 int32_t radix_sort_pairs_u32(hpa_ctx *c, const uint32_t *keys, const uint32_t *vals, L n, int lo, int hi, uint32_t **ko, uint32_t **vo){
     R radix_sort_t<uint32_t>(c, keys, vals, n, lo, hi, ko, vo, nullptr, false);
+}
+// the sorted keys in *ko and the stable permutation (row ids) as i64 in flat[0, n)
+static int32_t radix_sort_flat_u32(hpa_ctx *c, const uint32_t *keys, L n, int lo, int hi, uint32_t **ko, int64_t *flat){
+    uint32_t *vo = nullptr;
+    R radix_sort_t<uint32_t>(c, keys, nullptr, n, lo, hi, ko, &vo, flat, false, true);
 }
 
 // ---------------------------------------------------------------- runs -> dict(keys, flat list)
@@ -389,10 +459,12 @@ __global__ void __launch_bounds__(256) k_run_tilecount(const K *sk, const uint32
     __syncthreads();
     if(threadIdx.x == 0) tcount[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
+// (with flat_in: the flat list is already written -- the sort's last pass wrote it -- and a
+// head's key is read from it)
 template<typename K>
 __global__ void __launch_bounds__(256) k_run_write(const K *sk, const uint32_t *sidx, const uint64_t *eq, L n,
                                                    const int64_t *toff, int64_t *keys, int64_t *start, int64_t *flat,
-                                                   uint32_t *kval){
+                                                   uint32_t *kval, const int64_t *flat_in){
     __shared__ uint32_t s_cnt[RUN_ITEMS * 4];
     const L base = (L)blockIdx.x * RUN_TILE;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -401,7 +473,7 @@ __global__ void __launch_bounds__(256) k_run_write(const K *sk, const uint32_t *
     for(int k = 0; k < RUN_ITEMS; k++){
         L i = base + (L)k * 256 + threadIdx.x;
         bool h = false;
-        if(i < n){ flat[i] = (int64_t)sidx[i]; h = run_head(sk, sidx, eq, i); }
+        if(i < n){ if(!flat_in) flat[i] = (int64_t)sidx[i]; h = run_head(sk, sidx, eq, i); }
         bal[k] = __ballot(h);
         if(lane == 0) s_cnt[k * 4 + w] = (uint32_t)__popcll(bal[k]);
     }
@@ -420,7 +492,7 @@ __global__ void __launch_bounds__(256) k_run_write(const K *sk, const uint32_t *
         if((bal[k] >> lane) & 1ull){
             L i = base + (L)k * 256 + threadIdx.x;
             int64_t r = pre + s_cnt[k * 4 + w] + __popcll(bal[k] & lt);
-            keys[r] = (int64_t)sidx[i]; start[r] = i;
+            keys[r] = flat_in ? flat_in[i] : (int64_t)sidx[i]; start[r] = i;
             if(kval) kval[r] = (uint32_t)sk[i];
         }
     }
@@ -454,7 +526,8 @@ int32_t make_group_dict(hpa_ctx *c, V z, int64_t *keys, int64_t *counts, int64_t
 // src: the i32 column whose bit patterns sk holds (null otherwise); its groups' key values are
 // then kept beside the keys for index(column, keys)
 template<typename K>
-static int32_t runs_to_group_t(hpa_ctx *c, V z, const K *sk, const uint32_t *sidx, L n, const uint64_t *eq, const void *src = nullptr){
+static int32_t runs_to_group_t(hpa_ctx *c, V z, const K *sk, const uint32_t *sidx, L n, const uint64_t *eq, const void *src = nullptr,
+                               int64_t *flat_in = nullptr){
     int32_t e;
     const L nt = ceil_div(n, RUN_TILE) > 0 ? ceil_div(n, RUN_TILE) : 1;
     uint32_t *tcount = (uint32_t*)dalloc(c, 4 * nt);
@@ -470,10 +543,11 @@ static int32_t runs_to_group_t(hpa_ctx *c, V z, const K *sk, const uint32_t *sid
     int64_t *keys = (int64_t*)dalloc(c, 8 * (g > 0 ? g : 1));
     int64_t *start = (int64_t*)dalloc(c, 8 * (g + 1));
     int64_t *cnt = (int64_t*)dalloc(c, 8 * (g > 0 ? g : 1));
-    int64_t *flat = (int64_t*)dalloc(c, 8 * (n > 0 ? n : 1));
+    int64_t *flat = flat_in ? flat_in : (int64_t*)dalloc(c, 8 * (n > 0 ? n : 1));
     if(!keys || !start || !cnt || !flat) R HPA_E_MEMORY;
     if(n > 0){
-        hipLaunchKernelGGL(k_run_write<K>, dim3((unsigned)nt), dim3(256), 0, c->stream, sk, sidx, eq, n, toff, keys, start, flat, kval);
+        hipLaunchKernelGGL(k_run_write<K>, dim3((unsigned)nt), dim3(256), 0, c->stream, sk, sidx, eq, n, toff, keys, start, flat, kval,
+                           (const int64_t*)flat_in);
         hipLaunchKernelGGL(k_run_counts, dim3(grid_for(g, 256, 1)), dim3(256), 0, c->stream, start, g, n, cnt);
     }
     if((e = check_launch("group_runs"))) R e;
@@ -486,6 +560,10 @@ int32_t runs_to_group_u64(hpa_ctx *c, V z, const uint64_t *sk, const uint32_t *s
 }
 int32_t runs_to_group_u32(hpa_ctx *c, V z, const uint32_t *sk, const uint32_t *sidx, L n, const void *src){
     R runs_to_group_t<uint32_t>(c, z, sk, sidx, n, nullptr, src);
+}
+// the same over a flat list the sort already wrote (radix_sort_flat_u32)
+static int32_t runs_to_group_flat_u32(hpa_ctx *c, V z, const uint32_t *sk, int64_t *flat, L n, const void *src){
+    R runs_to_group_t<uint32_t>(c, z, sk, nullptr, n, nullptr, src, flat);
 }
 
 // ---------------------------------------------------------------- key builders
@@ -1409,10 +1487,12 @@ static int32_t group_int(hpa_ctx *c, V z, V x){      // lib_group_by_normal_int,
     if(n >= GR_RUNS_MIN && change <= GR_RUNS_MAX_CHANGE){
         bool done; if((e = group_i32_runs(c, z, (const int32_t*)p, n, &done)) || done) R e;
     }
-    // the unsigned radix key of an i32 is its bit pattern: the column is sorted in place of keys
-    uint32_t *sk, *sv;
-    if((e = radix_sort_pairs_u32(c, (const uint32_t*)p, nullptr, n, 0, 32, &sk, &sv))) R e;
-    R runs_to_group_u32(c, z, sk, sv, n, p);
+    // the unsigned radix key of an i32 is its bit pattern: the column is sorted in place of keys;
+    // the sort's last pass writes the flat list
+    int64_t *flat = (int64_t*)dalloc(c, 8 * (n > 0 ? n : 1)); if(!flat) R HPA_E_MEMORY;
+    uint32_t *sk = nullptr;
+    if((e = radix_sort_flat_u32(c, (const uint32_t*)p, n, 0, 32, &sk, flat))) R e;
+    R runs_to_group_flat_u32(c, z, sk, flat, n, p);
 }
 
 static int32_t any_high_i64(hpa_ctx *c, const int64_t *p, L n, bool *hi){

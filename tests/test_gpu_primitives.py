@@ -284,6 +284,26 @@ def test_sorted_inputs(gpu, orc, n):
             assert np.array_equal(g, o), (x.dtype, up)
 
 
+@pytest.mark.parametrize("span", [9, 10, 11, 16, 19, 20, 21, 26, 30, 32])
+@pytest.mark.parametrize("low", [0, 3])
+def test_radix_digit_plans(gpu, orc, span, low):
+    """i32 keys whose varying bits span `span` bits above `low` constant low bits: the radix sort
+    covers them with a 10-bit first digit and then 10-bit or byte digits (2 passes up to 20 bits,
+    3 up to 30), skipping constant digits.  Group and ascending/descending order against the
+    oracle; 32-bit spans include negative keys (unsigned bit-pattern order)."""
+    n = 300_000
+    r = rng(span * 10 + low)
+    hi = min(span + low, 32)
+    v = r.integers(0, 1 << (hi - low), n, dtype=np.uint64) << np.uint64(low)
+    x = (v & np.uint64(0xFFFFFFFF)).astype(np.uint32).view(np.int32)
+    x[r.integers(0, n, n // 4)] = x[r.integers(0, n, n // 4)]              # repeated keys
+    assert groups_of(gpu, gpu.pfnGroup(gpu.vec(x))) == groups_of(orc, orc.pfnGroup(orc.vec(x)))
+    for up in (1, 0):
+        g = gpu.read(gpu.pfnOrderBy(gpu.vec(x), gpu.bools([up])))
+        o = orc.read(orc.pfnOrderBy(orc.vec(x), orc.bools([up])))
+        assert np.array_equal(g, o), up
+
+
 def test_group_long(gpu, orc):
     for x in (np.array([-1, 4294967295, -1, 7, 7], np.int64),
               rng(9).integers(-5, 5, 5000).astype(np.int64) * (1 << 33),
