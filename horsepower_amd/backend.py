@@ -58,6 +58,17 @@ class Backend:
     def list(self, items):
         return self.host.list(items)
 
+    def strs(self, items):
+        return self.resident(self.host.strs(items))
+
+    def to_host(self, z):
+        return z
+
+    def read_strs(self, z):
+        """String (H_S) node -> list of bytes."""
+        self.to_host(z)
+        return [C.string_at(int(p)) for p in self.read(z)]
+
     def bools(self, flags):
         return self.host.vec(np.array(flags, dtype=np.uint8), H_B)
 
@@ -165,6 +176,10 @@ class GpuBackend(Backend):
 
     def read_raw(self, z, nbytes, dst):
         self.lib.rc("hpaReadPayload", z, C.c_void_p(dst), nbytes)
+
+    def to_host(self, z):
+        self.lib.rc("hpaToHost", z)
+        return z
 
     def read_many(self, zs):
         """Vector nodes -> numpy arrays with one device synchronisation.  The arrays are views of

@@ -127,7 +127,8 @@ extern "C" int32_t pfnCompare(hpa_V z, hpa_V x, hpa_V y, int32_t op){
     if(is_real(x->typ) && is_real(y->typ)) t = x->typ > y->typ ? x->typ : y->typ;
     else if(x->typ == y->typ){
         t = x->typ;
-        if(t == HPA_Q && op != 4 && op != 5) R HPA_E_NOT_IMPL;   // symbol order needs the string table
+        // symbol order: strcmp of the names (compareSymbol), over the symbol table
+        if(t == HPA_Q && op != 4 && op != 5) R compare_sym(c, z, x, y, op);
     }
     else R HPA_E_TYPE;
     void *dz; int32_t e;

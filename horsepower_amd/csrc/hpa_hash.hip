@@ -761,6 +761,7 @@ extern "C" int32_t pfnMember(hpa_V z, hpa_V y, hpa_V x){
         if(is_inline(z)){ e = commit_inline(c, z, dz); arena_set(c, mark); }
         R e;
     }
+    if(x->typ == HPA_S && y->typ == HPA_S) R member_str(c, z, y, x);   // by the strings' bytes
     if(x->typ == y->typ && is_basic(x->typ)){
         if(nx > 0 && (e = dev_view(c, x, &px))) R e;
         if(ny > 0 && (e = dev_view(c, y, &py))) R e;

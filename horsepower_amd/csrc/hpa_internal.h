@@ -148,6 +148,12 @@ int32_t radix_sort_pairs_u64(hpa_ctx *c, const uint64_t *keys, const uint32_t *v
 int32_t radix_sort_pairs_u32(hpa_ctx *c, const uint32_t *keys, const uint32_t *vals, L n, int lo_bit, int hi_bit,
                              uint32_t **keys_out, uint32_t **vals_out);
 int32_t iota_u32(hpa_ctx *c, uint32_t *p, L n);
+// strings and symbols (hpa_string.hip)
+int32_t compare_sym(hpa_ctx *c, V z, V x, V y, int op);
+int32_t member_str(hpa_ctx *c, V z, V y, V x);
+int32_t str_upload(hpa_ctx *c, V x);
+int32_t str_tohost(hpa_ctx *c);
+void str_prune(hpa_ctx *c, L cur);
 // Group a sorted key sequence into runs; emits the reference's dict layout with a flat list:
 // keys = first original index of each run, values = flat list (counts, i64 indices).
 int32_t runs_to_group_u64(hpa_ctx *c, V z, const uint64_t *skeys, const uint32_t *sidx, L n,

@@ -246,6 +246,7 @@ void arena_set(hpa_ctx *c, L cur){
     c->bounds.erase(c->bounds.lower_bound(top), c->bounds.end());
     c->gvals.erase(c->gvals.lower_bound(top), c->gvals.end());
     c->goffs.erase(c->goffs.lower_bound(top), c->goffs.end());
+    str_prune(c, cur);
     c->dcur = cur;
 }
 
@@ -369,6 +370,7 @@ int32_t resident_alloc(V x, int32_t typ, L len){
 static int32_t upload_rec(hpa_ctx *c, V x){
     if(is_basic(x->typ)){
         if(is_inline(x) || x->len <= 1 || is_device(x->g)) R 0;
+        if(x->typ == HPA_S){ int32_t e = str_upload(c, x); if(e) R e; }   // the strings' bytes first
         L bytes = type_bytes(x->typ, x->len);
         void *d = nullptr;
         if(hipMalloc(&d, (size_t)bytes) != hipSuccess){ set_error_detail("hipMalloc(%lld) failed", (long long)bytes); R HPA_E_MEMORY; }
@@ -426,6 +428,7 @@ static int32_t tohost_payload(hpa_ctx *c, V x, L bytes){
 }
 static int32_t tohost_rec(hpa_ctx *c, V x){
     if(is_basic(x->typ)){
+        if(x->typ == HPA_S){ int32_t e = str_tohost(c); if(e) R e; }      // bytes the device wrote
         if(is_inline(x) || !is_device(x->g)) R 0;
         R tohost_payload(c, x, type_bytes(x->typ, x->len));
     }
@@ -507,20 +510,6 @@ extern "C" int32_t hpaReadMany(int32_t n, hpa_V *x, void **dst, const int64_t *b
 extern "C" int32_t hpaSync(void){
     hpa_ctx *c = ctx_cur();
     R hipStreamSynchronize(c->stream) == hipSuccess ? 0 : HPA_E_DEVICE;
-}
-
-// ---------------------------------------------------------------- symbols
-static std::vector<std::string> g_syms;
-extern "C" int64_t hpaSymbol(const char *name){
-    std::lock_guard<std::mutex> lk(g_mu);
-    for(size_t i=0;i<g_syms.size();i++) if(g_syms[i] == name) R (int64_t)i;
-    g_syms.push_back(name);
-    R (int64_t)g_syms.size() - 1;
-}
-extern "C" const char *hpaSymbolName(int64_t id){
-    std::lock_guard<std::mutex> lk(g_mu);
-    if(id < 0 || id >= (int64_t)g_syms.size()) R nullptr;
-    R g_syms[id].c_str();
 }
 
 // ---------------------------------------------------------------- kernel timing

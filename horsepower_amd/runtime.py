@@ -8,7 +8,7 @@ import os
 
 import numpy as np
 
-from .vnode import (V0, V, H_B, H_C, H_G, H_L, H_N, H_Q, NP_OF, TYP_OF, Expr, HorseError,
+from .vnode import (V0, V, H_B, H_S, H_C, H_G, H_L, H_N, H_Q, NP_OF, TYP_OF, Expr, HorseError,
                     PrimitiveLib)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -77,6 +77,7 @@ class Device(PrimitiveLib):
         _bind(dll, "hpaProfileEnd", None, [])
         _bind(dll, "hpaSymbol", i64, [C.c_char_p])
         _bind(dll, "hpaSymbolName", C.c_char_p, [i64])
+        _bind(dll, "hpaImportSymbols", i32, [i64, C.POINTER(C.c_char_p)])
         _bind(dll, "hpaFusedMap", i32, [V, i32, i32, C.POINTER(V), C.POINTER(Expr)])
         _bind(dll, "hpaFusedReduce", i32, [V, i32, i32, C.POINTER(V), C.POINTER(Expr), C.POINTER(Expr)])
         _bind(dll, "hpaFusedReduceMany", i32, [i32, C.POINTER(V), i32, i32, C.POINTER(V), C.POINTER(Expr), C.POINTER(Expr)])
@@ -179,6 +180,14 @@ class Host:
 
     def scalar(self, v, typ):
         return self.vec(np.array([v], dtype=NP_OF[typ]), typ)
+
+    def strs(self, items):
+        """A string (H_S) node: char* values into one NUL-separated buffer kept alive here."""
+        bs = [s.encode() if isinstance(s, str) else bytes(s) for s in items]
+        buf = np.frombuffer(b"".join(b + b"\0" for b in bs) or b"\0", dtype=np.uint8).copy()
+        offs = np.cumsum([0] + [len(b) + 1 for b in bs[:-1]]).astype(np.uint64) if bs else np.zeros(0, np.uint64)
+        self._keep.append(buf)
+        return self.vec(offs + np.uint64(buf.ctypes.data), H_S)
 
     def sym(self, symid):
         z = self.node()

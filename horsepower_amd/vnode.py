@@ -18,6 +18,7 @@ NP_OF = {
     H_B: np.uint8, H_J: np.int8, H_H: np.int16, H_I: np.int32, H_L: np.int64,
     H_F: np.float32, H_E: np.float64, H_C: np.uint8, H_Q: np.int64, H_M: np.int32,
     H_D: np.int32, H_Z: np.int64, H_U: np.int32, H_W: np.int32, H_T: np.int32,
+    H_S: np.uint64,                                   # char* values (host addresses)
 }
 TYP_OF = {np.dtype(np.uint8): H_B, np.dtype(np.bool_): H_B, np.dtype(np.int8): H_J,
           np.dtype(np.int16): H_H, np.dtype(np.int32): H_I, np.dtype(np.int64): H_L,
@@ -115,7 +116,7 @@ _ARITY = {
     "pfnDiv": 2, "pfnAnd": 2, "pfnNand": 2, "pfnOr": 2, "pfnNor": 2, "pfnXor": 2,
     "pfnCompress": 2, "pfnIndexOf": 2, "pfnOrderBy": 2, "pfnMember": 2, "pfnKeys": 1,
     "pfnValues": 1, "pfnJoinIndex": 3, "pfnEnum": 2, "pfnFetch": 1, "pfnColumnValue": 2,
-    "pfnLoadTable": 1, "pfnAddFKey": 3,
+    "pfnLoadTable": 1, "pfnAddFKey": 3, "pfnLike": 2, "pfnSubString": 2,
 }
 
 

@@ -233,6 +233,20 @@ int32_t hpaRegisterTable(int64_t name, hpa_V table);
  * (backend/h_symbol.cc); this library interns the names it understands here. */
 int64_t hpaSymbol(const char *name);
 const char *hpaSymbolName(int64_t id);
+/* The caller's symbol table becomes the library's: names[i] gets id i, so the ids of symbol
+ * values agree with the interpreter's getSymbol (backend/h_symbol.cc:148-157).  Call it once the
+ * host's symbols exist (and again after it makes more); names the library interned before and
+ * that are not among them move to ids after n. */
+int32_t hpaImportSymbols(int64_t n, const char *const *names);
+
+/* ---- symbols and strings -----------------------------------------------------------------
+ * Symbols are ids into that table; strings (HPA_S) are vectors of char* holding host addresses,
+ * as in the reference.  hpaUpload of an S column mirrors its strings' bytes in HBM; kernels read
+ * the mirror.  Symbol order compares (pfnLt/Leq/Gt/Geq on HPA_Q, compareSymbol,
+ * h_symbol.cc:241-244) and every string compare (strcmp, h_primitive.cc:1121-1160) go through
+ * pfnCompare as the other types do. */
+int32_t pfnLike       (hpa_V z, hpa_V x, hpa_V y);         /* h_primitive.cc:1670-1757 */
+int32_t pfnSubString  (hpa_V z, hpa_V x, hpa_V y);         /* h_primitive.cc:2468-2489 */
 
 /* ---- fused loops emitted by the HorseIR optimizer ---------------------------------------
  * An expression is postfix code over the input columns x[0..nx).  Each instruction yields an

@@ -120,11 +120,19 @@ static void move_node(OV z, OV x){ *z = *x; if(x->g == (char*)&x->b) z->g = (cha
 #define vV(x,k) (((OV)(x)->g)+(k))
 
 /* ------------------------------------------------------------------ symbols (h_symbol.cc) */
-static const char *g_syms[64]; static int g_nsym = 0;
-int64_t orc_symbol(const char *name){
-    for(int i=0;i<g_nsym;i++) if(!strcmp(g_syms[i],name)) R i;
-    if(g_nsym >= 64) R -1;
+static char **g_syms = NULL; static int64_t g_nsym = 0, g_capsym = 0;
+int64_t orc_symbol(const char *name){               /* getSymbol: one id per distinct name */
+    for(int64_t i=0;i<g_nsym;i++) if(!strcmp(g_syms[i],name)) R i;
+    if(g_nsym == g_capsym){
+        int64_t c = g_capsym ? 2*g_capsym : 64;
+        char **t = (char**)realloc(g_syms, sizeof(char*)*(size_t)c); if(!t) R -1;
+        g_syms = t; g_capsym = c;
+    }
     g_syms[g_nsym] = strdup(name); R g_nsym++;
+}
+static const char *sym_str(int64_t q){ R (q>=0 && q<g_nsym) ? g_syms[q] : NULL; }
+static int compare_symbol(int64_t x, int64_t y){   /* compareSymbol, h_symbol.cc:241-244 */
+    R x==y ? 0 : strcmp(sym_str(x), sym_str(y));
 }
 static int op_from_symbol(int64_t q){             /* getOpFromName, h_system.cc:779-789 */
     if(q < 0 || q >= g_nsym) R -1;
@@ -218,7 +226,11 @@ int32_t orc_pfnCompare(OV z, OV x, OV y, int32_t op){
             case HPA_C: CMPS(char);
             case HPA_M: case HPA_D: case HPA_U: case HPA_W: case HPA_T: CMPS(int32_t);
             case HPA_Z: CMPS(int64_t);
-            case HPA_Q: if(op==4||op==5) CMPS(int64_t); R HPA_E_NOT_IMPL;
+            case HPA_Q: { int64_t *a=(int64_t*)x->g, *b=(int64_t*)y->g;
+                for(L i=0;i<n;i++){
+                    int64_t p=a[i*sx], q=b[i*sy];
+                    if(p!=q && (!sym_str(p) || !sym_str(q))) R HPA_E_DOMAIN;
+                    zz[i]=comp_op(op, compare_symbol(p,q)); } } break;
 #undef CMPS
             default: R HPA_E_NOT_IMPL;
         }
@@ -1207,6 +1219,7 @@ int32_t orc_pfnIndexOf(OV z, OV x, OV y){          /* h_primitive.cc:1525-1579 *
     R HPA_E_NOT_IMPL;
 }
 
+static int cmp_str_ptr(const void *p, const void *q){ R strcmp(*(char *const*)p, *(char *const*)q); }
 int32_t orc_pfnMember(OV z, OV y, OV x){           /* h_primitive.cc:2288-2336: z[i] = y[i] in x */
     if(is_real(x->typ) && is_real(y->typ)){
         int32_t t = MAXV(x->typ, y->typ);
@@ -1254,7 +1267,103 @@ int32_t orc_pfnMember(OV z, OV y, OV x){           /* h_primitive.cc:2288-2336: 
         for(L i=0;i<y->len;i++) sB(z)[i] = buf[(unsigned char)sC(y)[i]];
         R 0;
     }
+    if(x->typ == y->typ && x->typ == HPA_S){       /* lib_member_S, h_libs.cc:1015-1018: equal strings */
+        char **b = (char**)y->g;
+        char **a = (char**)malloc(sizeof(char*)*(size_t)(x->len>0?x->len:1));
+        memcpy(a, x->g, sizeof(char*)*(size_t)x->len);
+        qsort(a, (size_t)x->len, sizeof(char*), cmp_str_ptr);
+        IV(z, HPA_B, y->len);
+        for(L i=0;i<y->len;i++) sB(z)[i] = bsearch(&b[i], a, (size_t)x->len, sizeof(char*), cmp_str_ptr) != NULL;
+        free(a);
+        R 0;
+    }
     R HPA_E_NOT_IMPL;
+}
+
+/* ------------------------------------------------------------------ like, sub_string */
+/* genLikeString (h_system.cc:319-334): ^ + pattern with % -> .*, _ -> ., . and \ escaped + $.
+ * NULL for an empty pattern.  Bytes that are regular-expression operators stay operators in the
+ * reference; the oracle (like the GPU library) takes only patterns without them. */
+static char *gen_like_string(const char *in, L n){
+    if(n <= 0) R NULL;
+    char *o = (char*)malloc((size_t)(2*n + 3)); L c = 0;
+    o[c++] = '^';
+    for(L i=0;i<n;i++){
+        if(in[i]=='%'){ o[c++]='.'; o[c++]='*'; }
+        else if(in[i]=='.' || in[i]=='\\'){ o[c++]='\\'; o[c++]=in[i]; }
+        else o[c++] = in[i]=='_' ? '.' : in[i];
+    }
+    o[c++] = '$'; o[c] = 0;
+    R o;
+}
+/* backtracking match of the translated expression at s[i..n) (PCRE2 defaults: . and .* never
+ * match a newline; $ matches at the end or before a final newline) */
+static int re_here(const char *re, const char *s, L i, L n){
+    if(re[0] == '$' && re[1] == 0) R i == n || (i == n-1 && s[i] == '\n');
+    if(re[0] == '.' && re[1] == '*'){
+        for(L k=i; ; k++){                         /* .* as short as possible, then longer */
+            if(re_here(re+2, s, k, n)) R 1;
+            if(k >= n || s[k] == '\n') R 0;
+        }
+    }
+    if(i >= n) R 0;
+    if(re[0] == '.'){ if(s[i] == '\n') R 0; R re_here(re+1, s, i+1, n); }
+    if(re[0] == '\\'){ if(s[i] != re[1]) R 0; R re_here(re+2, s, i+1, n); }
+    if(s[i] != re[0]) R 0;
+    R re_here(re+1, s, i+1, n);
+}
+static int like_one(const char *re, const char *s, L n){ R re_here(re+1, s, 0, n); }   /* skip ^ */
+
+/* pfnLike (h_primitive.cc:1670-1757), one pattern y (C, Q or S) over x (C: one string, Q, S) */
+int32_t orc_pfnLike(OV z, OV x, OV y){
+    int strish_x = x->typ==HPA_C || x->typ==HPA_Q || x->typ==HPA_S;
+    int strish_y = y->typ==HPA_C || y->typ==HPA_Q || y->typ==HPA_S;
+    if(!strish_x || !strish_y) R HPA_E_DOMAIN;
+    if(!(y->len == 1 || y->typ == HPA_C)) R HPA_E_NOT_IMPL;
+    char *pat; L pn;
+    if(y->typ == HPA_C){ pat = (char*)calloc((size_t)y->len+1, 1); memcpy(pat, y->g, (size_t)y->len); }
+    else if(y->typ == HPA_Q){ const char *t = sym_str(sL(y)[0]); if(!t) R HPA_E_DOMAIN; pat = strdup(t); }
+    else pat = strdup(((char**)y->g)[0]);
+    pn = (L)strlen(pat);
+    for(L i=0;i<pn;i++) if(strchr("^$[|()?*+{", pat[i])){ free(pat); R HPA_E_NOT_IMPL; }
+    char *re = gen_like_string(pat, pn);
+    free(pat);
+    if(!re) R HPA_E_NULL_VALUE;                    /* getLikePatten NULL -> E_NULL_VALUE */
+    L n = x->typ == HPA_C ? 1 : x->len;
+    IV(z, HPA_B, n);
+    if(x->typ == HPA_C){
+        L l = 0; while(l < x->len && ((char*)x->g)[l]) l++;
+        sB(z)[0] = (uint8_t)like_one(re, (char*)x->g, l);
+    } else if(x->typ == HPA_Q){
+        for(L i=0;i<n;i++){ const char *t = sym_str(sL(x)[i]); if(!t){ free(re); R HPA_E_DOMAIN; }
+            sB(z)[i] = (uint8_t)like_one(re, t, (L)strlen(t)); }
+    } else {
+        char **a = (char**)x->g;
+        for(L i=0;i<n;i++) sB(z)[i] = (uint8_t)like_one(re, a[i], (L)strlen(a[i]));
+    }
+    free(re);
+    R 0;
+}
+
+/* pfnSubString (h_primitive.cc:2468-2489): seg bytes from 1-based start, the room counted as
+ * strlen - start, padded with spaces; start < 1 (a read before the string there) is E_DOMAIN */
+int32_t orc_pfnSubString(OV z, OV x, OV y){
+    if(x->typ != HPA_S || !is_int(y->typ)) R HPA_E_DOMAIN;
+    if(y->len != 2) R HPA_E_NOT_IMPL;
+    L start = (L)key_bits(y, 0), seg = (L)key_bits(y, 1);
+    if(start < 1 || seg < 0) R HPA_E_DOMAIN;
+    L n = x->len;
+    IV(z, HPA_S, n);
+    char **a = (char**)x->g, **o = (char**)z->g;
+    for(L i=0;i<n;i++){
+        char *p = (char*)heap_alloc(seg+1); if(!p) R HPA_E_MEMORY;
+        L t0 = (L)strlen(a[i]), space = t0 - start, len = seg < space ? seg : space;
+        if(len < 0) len = 0;
+        for(L j=0;j<len;j++) p[j] = a[i][start+j-1];
+        while(len < seg) p[len++] = ' ';
+        p[seg] = 0; o[i] = p;
+    }
+    R 0;
 }
 
 /* ------------------------------------------------------------------ fused Q6 (opt/q6.h:31-48) */

@@ -70,6 +70,8 @@ int32_t orc_register_table(int64_t sid, OV t);
 int32_t orc_pfnKeys(OV z, OV x);
 int32_t orc_pfnValues(OV z, OV x);
 int64_t orc_symbol(const char *name);
+int32_t orc_pfnLike(OV z, OV x, OV y);
+int32_t orc_pfnSubString(OV z, OV x, OV y);
 
 /* compensated FP64 sums (Neumaier), the accuracy reference for every FP64 aggregate */
 double  orc_sum_neumaier(const double *x, int64_t n);

@@ -13,18 +13,18 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SYMS = os.path.join(ROOT, "include", "horsepower_amd.syms")
 LIBDIR = os.path.join(ROOT, "horsepower_amd")
 
-# h_primitive.o's shape: a replaced primitive (pfnSum) and a host-only one (pfnLike) that
+# h_primitive.o's shape: a replaced primitive (pfnSum) and a host-only one (pfnTable) that
 # calls it internally
 STAND_IN = r"""
 typedef struct V0 { long len; char *g; int typ; int pad; long u; } V0, *V;
 int pfnSum(V z, V x){ (void)z; (void)x; return 42; }
-int pfnLike(V z, V x){ return pfnSum(z, x); }
+int pfnTable(V z, V x){ return pfnSum(z, x); }
 """
 CALLER = r"""
 typedef struct V0 { long len; char *g; int typ; int pad; long u; } V0, *V;
 int pfnSum(V z, V x);
-int pfnLike(V z, V x);
-int (*table[])(V, V) = { pfnSum, pfnLike };   /* the interpreter's dyaFunc-style table */
+int pfnTable(V z, V x);
+int (*table[])(V, V) = { pfnSum, pfnTable };   /* the interpreter's dyaFunc-style table */
 int main(void){ return table[0] == 0; }
 """
 
@@ -47,11 +47,11 @@ def test_localized_object_does_not_shadow_library(tmp_path):
     run("gcc", "-c", "-O1", "h_primitive.c", "-o", "h_primitive.o")
     run("objcopy", f"--localize-symbols={SYMS}", "h_primitive.o", "h_primitive.local.o")
     local = run("nm", "h_primitive.local.o")
-    assert re.search(r"\bt pfnSum\b", local) and re.search(r"\bT pfnLike\b", local)
+    assert re.search(r"\bt pfnSum\b", local) and re.search(r"\bT pfnTable\b", local)
     run("gcc", "-c", "main.c", "-o", "main.o")
     run("gcc", "main.o", "h_primitive.local.o", "-L", LIBDIR, "-lhorsepower_amd", "-o", "q")
     # the table's pfnSum is the library's (undefined here, bound at load time); the local copy
     # stays for the host-only primitive's own call
     dyn = run("nm", "-D", "q")
     assert re.search(r"\bU pfnSum\b", dyn), dyn
-    assert not re.search(r"\bU pfnLike\b", dyn)
+    assert not re.search(r"\bU pfnTable\b", dyn)
