@@ -78,6 +78,9 @@ class Device(PrimitiveLib):
         _bind(dll, "hpaSymbol", i64, [C.c_char_p])
         _bind(dll, "hpaSymbolName", C.c_char_p, [i64])
         _bind(dll, "hpaImportSymbols", i32, [i64, C.POINTER(C.c_char_p)])
+        _bind(dll, "hpaReadSerialized", i32, [V, C.c_char_p, C.POINTER(i64)])
+        _bind(dll, "hpaInitTableFromBin", i32, [C.c_char_p, C.c_char_p, C.POINTER(V)])
+        _bind(dll, "hpaWriteSerialized", i32, [V, C.c_char_p])
         _bind(dll, "hpaFusedMap", i32, [V, i32, i32, C.POINTER(V), C.POINTER(Expr)])
         _bind(dll, "hpaFusedReduce", i32, [V, i32, i32, C.POINTER(V), C.POINTER(Expr), C.POINTER(Expr)])
         _bind(dll, "hpaFusedReduceMany", i32, [i32, C.POINTER(V), i32, i32, C.POINTER(V), C.POINTER(Expr), C.POINTER(Expr)])
@@ -174,6 +177,10 @@ class Host:
             z.contents.u.i64 = int(raw.view(np.int64)[0])
             z.contents.g = C.addressof(z.contents) + 24
         else:
+            if typ == H_C:                                 # room for the NUL getTypeSize counts
+                b = np.zeros(a.shape[0] + 1, np.uint8)
+                b[:-1] = a
+                a = b
             self._keep.append(a)
             z.contents.g = a.ctypes.data
         return z

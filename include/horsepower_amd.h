@@ -228,6 +228,17 @@ int32_t pfnLoadTable  (hpa_V z, hpa_V x);                  /* h_primitive.cc:20-
  * does not do, so the caller registers the tables it holds. */
 int32_t hpaRegisterTable(int64_t name, hpa_V table);
 
+/* Table snapshots in the reference's .bin format (serializeV / readSerializeV,
+ * backend/h_io.cc:630-776): the reader streams every column payload from the file into resident
+ * HBM through pinned staging (the table never exists as host payloads), re-interns symbol names
+ * and mirrors strings as hpaUpload does; *payload_bytes (may be null) gets the payload bytes
+ * moved.  hpaInitTableFromBin is initTableFromBin (load_db.cc:224-234): read, then registered
+ * under `name` for pfnLoadTable, the table node in *z.  hpaWriteSerialized is serializeV of a
+ * node whose payloads are in HBM or on the host. */
+int32_t hpaReadSerialized(hpa_V z, const char *path, int64_t *payload_bytes);
+int32_t hpaInitTableFromBin(const char *name, const char *path, hpa_V *z);
+int32_t hpaWriteSerialized(hpa_V x, const char *path);
+
 /* Join operator names.  pfnJoinIndex's `f` is a symbol vector of operator names
  * ("eq", "neq", ...; backend/h_system.cc:779-789).  Symbols are interned ids in the reference
  * (backend/h_symbol.cc); this library interns the names it understands here. */

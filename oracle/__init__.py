@@ -67,6 +67,12 @@ def backend(threads=0):
     dll.orc_symbol.argtypes = [C.c_char_p]
     dll.orc_symbol.restype = C.c_int64
     dll.orc_threads.restype = C.c_int32
+    dll.orc_symbol_name.argtypes = [C.c_int64]
+    dll.orc_symbol_name.restype = C.c_char_p
+    dll.orc_serializeV.argtypes = [V, C.c_char_p]
+    dll.orc_serializeV.restype = C.c_int32
+    dll.orc_readSerializeV.argtypes = [V, C.c_char_p]
+    dll.orc_readSerializeV.restype = C.c_int32
     dll.orc_sum_neumaier.argtypes = [C.c_void_p, C.c_int64]
     dll.orc_sum_neumaier.restype = C.c_double
     dll.orc_sum_neumaier_idx.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]

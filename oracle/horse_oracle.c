@@ -20,6 +20,7 @@
 #include "horse_oracle.h"
 #include <stdlib.h>
 #include <string.h>
+#include <stdio.h>
 #include <math.h>
 #include <float.h>
 #include <limits.h>
@@ -131,6 +132,7 @@ int64_t orc_symbol(const char *name){               /* getSymbol: one id per dis
     g_syms[g_nsym] = strdup(name); R g_nsym++;
 }
 static const char *sym_str(int64_t q){ R (q>=0 && q<g_nsym) ? g_syms[q] : NULL; }
+const char *orc_symbol_name(int64_t q){ R sym_str(q); }
 static int compare_symbol(int64_t x, int64_t y){   /* compareSymbol, h_symbol.cc:241-244 */
     R x==y ? 0 : strcmp(sym_str(x), sym_str(y));
 }
@@ -1364,6 +1366,89 @@ int32_t orc_pfnSubString(OV z, OV x, OV y){
         p[seg] = 0; o[i] = p;
     }
     R 0;
+}
+
+/* ------------------------------------------------------------------ serializeV / readSerializeV */
+/* h_io.cc:630-776: the V0 header as it sits in memory, then the payload per type (see
+ * serialize_node); the reader re-makes symbol ids (getSymbol) and string storage (allocStrMem) */
+static int ser_node(OV x, FILE *fp){
+    if(fwrite(x, sizeof(O0), 1, fp) != 1) R HPA_E_GENERAL;
+    L n = x->len;
+    switch(x->typ){
+        case HPA_Q: case HPA_S: {
+            for(L i=0;i<n;i++){
+                const char *t = x->typ==HPA_Q ? sym_str(sL(x)[i]) : ((char**)x->g)[i];
+                if(!t) R HPA_E_DOMAIN;
+                int64_t l = (int64_t)strlen(t); fwrite(&l, 8, 1, fp);
+            }
+            for(L i=0;i<n;i++){
+                const char *t = x->typ==HPA_Q ? sym_str(sL(x)[i]) : ((char**)x->g)[i];
+                fwrite(t, 1, strlen(t)+1, fp);
+            }
+            R 0;
+        }
+        case HPA_G: for(L i=0;i<n;i++){ int32_t e = ser_node(vV(x,i), fp); if(e) R e; } R 0;
+        case HPA_A: {
+            hpa_A0 a; memset(&a, 0, sizeof(a)); if(x->p) a = *(hpa_A0*)x->p;
+            fwrite(&a, sizeof(a), 1, fp);
+            int32_t e = ser_node(vV(x,0), fp); if(e) R e;
+            R ser_node(vV(x,1), fp);
+        }
+        default:
+            if(!is_basic(x->typ)) R HPA_E_TYPE;
+            fwrite(x->g, 1, (size_t)type_size(x->typ, n), fp);   /* getTypeSize: a char vector + 1 */
+            R 0;
+    }
+}
+int32_t orc_serializeV(OV x, const char *path){
+    FILE *fp = fopen(path, "wb"); if(!fp) R HPA_E_GENERAL;
+    int32_t e = ser_node(x, fp);
+    if(fclose(fp) != 0 && !e) e = HPA_E_GENERAL;
+    R e;
+}
+static int rd_exact(void *p, size_t n, FILE *fp){ R (n == 0 || fread(p, 1, n, fp) == n) ? 0 : HPA_E_GENERAL; }
+static int rd_node(OV x, FILE *fp, int depth){
+    O0 h; int32_t e;
+    if(depth > 64) R HPA_E_DOMAIN;
+    if((e = rd_exact(&h, sizeof(h), fp))) R e;
+    L n = h.len;
+    if(n < 0) R HPA_E_DOMAIN;
+    if(h.typ == HPA_A){
+        x->typ = HPA_A; x->len = h.len;
+        x->g = heap_alloc(2*sizeof(O0)); if(!x->g) R HPA_E_MEMORY;
+        hpa_A0 *a = (hpa_A0*)heap_alloc(sizeof(hpa_A0)); if(!a) R HPA_E_MEMORY;
+        if((e = rd_exact(a, sizeof(*a), fp))) R e;
+        a->fkey = NULL;                           /* a pointer from another process */
+        x->p = a;
+        if((e = rd_node(vV(x,0), fp, depth+1))) R e;
+        R rd_node(vV(x,1), fp, depth+1);
+    }
+    if(h.typ == HPA_G){
+        x->typ = HPA_G; x->len = n; x->g2 = NULL;
+        x->g = heap_alloc(sizeof(O0)*(size_t)(n>0?n:1)); if(!x->g) R HPA_E_MEMORY;
+        for(L i=0;i<n;i++) if((e = rd_node(vV(x,i), fp, depth+1))) R e;
+        R 0;
+    }
+    if(!is_basic(h.typ)) R HPA_E_TYPE;
+    IV(x, h.typ, n);
+    if(h.typ == HPA_Q || h.typ == HPA_S){
+        int64_t *len = (int64_t*)malloc(sizeof(int64_t)*(size_t)(n>0?n:1));
+        if((e = rd_exact(len, 8*(size_t)n, fp))){ free(len); R e; }
+        for(L i=0;i<n;i++){
+            char *t = heap_alloc(len[i]+1); if(!t){ free(len); R HPA_E_MEMORY; }
+            if((e = rd_exact(t, (size_t)len[i]+1, fp))){ free(len); R e; }
+            if(h.typ == HPA_Q) sL(x)[i] = orc_symbol(t); else ((char**)x->g)[i] = t;
+        }
+        free(len);
+        R 0;
+    }
+    R rd_exact(x->g, (size_t)type_size(h.typ, n), fp);
+}
+int32_t orc_readSerializeV(OV z, const char *path){
+    FILE *fp = fopen(path, "rb"); if(!fp) R HPA_E_GENERAL;
+    int32_t e = rd_node(z, fp, 0);
+    fclose(fp);
+    R e;
 }
 
 /* ------------------------------------------------------------------ fused Q6 (opt/q6.h:31-48) */

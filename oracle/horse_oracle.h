@@ -70,8 +70,11 @@ int32_t orc_register_table(int64_t sid, OV t);
 int32_t orc_pfnKeys(OV z, OV x);
 int32_t orc_pfnValues(OV z, OV x);
 int64_t orc_symbol(const char *name);
+const char *orc_symbol_name(int64_t q);
 int32_t orc_pfnLike(OV z, OV x, OV y);
 int32_t orc_pfnSubString(OV z, OV x, OV y);
+int32_t orc_serializeV(OV x, const char *path);
+int32_t orc_readSerializeV(OV z, const char *path);
 
 /* compensated FP64 sums (Neumaier), the accuracy reference for every FP64 aggregate */
 double  orc_sum_neumaier(const double *x, int64_t n);
