@@ -45,7 +45,6 @@
 #define PART_MAX 4096          // partitions per query (LDS counters of the count and scatter kernels)
 #define PART_MIN_KEYS 1024     // smallest partition width the tunables may choose
 #define PART_MIN_TILE 8192     // smallest row tile the tunables may choose (sizes the count matrix)
-#define SC_BLOCK 256
 
 struct BQuery {
     const int32_t *key; const double *val; const int32_t *flt;
@@ -255,16 +254,27 @@ __global__ void __launch_bounds__(1024) k_batch_pscan(BPlan p){
     if(t == 0) q.poff[N] = (int64_t)s_carry;
 }
 
-// Row tiles of partitioned queries into their partitions: LDS cursors start at the tile's
-// offset in each partition and each selected row takes the next place of its partition.
-// Scatter of a partitioned query's row tile: tile-local 32-bit cursors per partition in LDS
-// (after the partitions' tile bases), four rows per thread per step with their loads and cursor
-// atomics issued together.
-#define SC_ROWS 4
+// Scatter of a partitioned query's row tile, staged through LDS so that the writes go out as
+// runs: the tile is taken 8192 selected-or-not rows at a time (eight per thread); each row gets
+// its rank inside its partition for this chunk from an LDS counter, a block scan of the
+// partitions' chunk counts places every partition's rows contiguously in LDS, and the block
+// then writes the chunk out slot by slot -- consecutive threads to consecutive places of one
+// partition's run -- at each partition's running output position (its tile base from the
+// (partition, tile) scan plus the rows of earlier chunks).  The next chunk continues the same
+// runs, so a partition's bytes reach L2 in order from one workgroup instead of one 2-byte key
+// and one 8-byte value per lane to as many lines.
+#define SC_BLOCK 1024
+#define SC_ROWS 8
+#define SC_CHUNK (SC_ROWS * SC_BLOCK)
 __global__ void __launch_bounds__(SC_BLOCK) k_batch_scatter(BPlan p){
+    // 144 KB of LDS: one workgroup per CU
     __shared__ int64_t s_item;
-    int64_t *base = (int64_t*)g_batch_lds;                               // [PART_MAX]
-    unsigned *cur = (unsigned*)(base + PART_MAX);                        // [PART_MAX]
+    __shared__ uint32_t s_wsum[SC_BLOCK / 64];
+    __shared__ int64_t pos0[PART_MAX];                                   // next output place per partition
+    __shared__ uint32_t coff[PART_MAX];                                  // chunk counts, then offsets
+    __shared__ double st_val[SC_CHUNK];
+    __shared__ uint16_t st_key[SC_CHUNK], st_part[SC_CHUNK];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     while(true){
         const int64_t item = next_item(&p.ctr[2], &s_item);
         if(item >= p.total_tiles) break;
@@ -274,13 +284,15 @@ __global__ void __launch_bounds__(SC_BLOCK) k_batch_scatter(BPlan p){
         const int64_t r0 = tile * p.tile_rows;
         const int64_t r1 = r0 + p.tile_rows < q.n ? r0 + p.tile_rows : q.n;
         const uint32_t range = (uint32_t)q.range, mask = (uint32_t)p.part_keys - 1u;
-        for(int k = threadIdx.x; k < q.nparts; k += SC_BLOCK){ base[k] = q.poff[(int64_t)k * q.ntiles + tile]; cur[k] = 0u; }
-        __syncthreads();
-        for(int64_t i0 = r0 + threadIdx.x; i0 < r1; i0 += SC_ROWS * SC_BLOCK){
-            uint32_t s[SC_ROWS]; double v[SC_ROWS]; bool ok[SC_ROWS];
+        const int P = q.nparts;
+        for(int k = threadIdx.x; k < P; k += SC_BLOCK) pos0[k] = q.poff[(int64_t)k * q.ntiles + tile];
+        for(int64_t c0 = r0; c0 < r1; c0 += SC_CHUNK){
+            for(int k = threadIdx.x; k < P; k += SC_BLOCK) coff[k] = 0u;
+            __syncthreads();
+            uint32_t s[SC_ROWS], rk[SC_ROWS]; double v[SC_ROWS]; bool ok[SC_ROWS];
             #pragma unroll
             for(int u = 0; u < SC_ROWS; u++){
-                const int64_t i = i0 + (int64_t)u * SC_BLOCK;
+                const int64_t i = c0 + (int64_t)u * SC_BLOCK + threadIdx.x;
                 ok[u] = false; s[u] = 0; v[u] = 0.0;
                 if(i < r1 && batch_pass(q.op, q.flt[i], q.thr)){
                     s[u] = (uint32_t)q.key[i] - (uint32_t)q.kmin;
@@ -288,17 +300,38 @@ __global__ void __launch_bounds__(SC_BLOCK) k_batch_scatter(BPlan p){
                     if(ok[u]) v[u] = q.val[i];
                 }
             }
-            unsigned loc[SC_ROWS];
             #pragma unroll
-            for(int u = 0; u < SC_ROWS; u++) loc[u] = ok[u] ? atomicAdd(&cur[s[u] >> p.part_shift], 1u) : 0u;
+            for(int u = 0; u < SC_ROWS; u++) rk[u] = ok[u] ? atomicAdd(&coff[s[u] >> p.part_shift], 1u) : 0u;
+            __syncthreads();
+            // exclusive scan of the P chunk counts: thread t owns partitions [4t, 4t + 4)
+            uint32_t c[4], tot = 0;
+            #pragma unroll
+            for(int j = 0; j < 4; j++){ const int k = 4 * threadIdx.x + j; c[j] = k < P ? coff[k] : 0u; tot += c[j]; }
+            uint32_t inc = tot;
+            for(int o = 1; o < 64; o <<= 1){ const uint32_t y = __shfl_up(inc, o, 64); if(lane >= o) inc += y; }
+            if(lane == 63) s_wsum[w] = inc;
+            __syncthreads();
+            uint32_t pre = inc - tot, total = 0;
+            for(int q2 = 0; q2 < SC_BLOCK / 64; q2++){ if(q2 < w) pre += s_wsum[q2]; total += s_wsum[q2]; }
+            #pragma unroll
+            for(int j = 0; j < 4; j++){ const int k = 4 * threadIdx.x + j; if(k < P) coff[k] = pre; pre += c[j]; }
+            __syncthreads();
             #pragma unroll
             for(int u = 0; u < SC_ROWS; u++){
                 if(!ok[u]) continue;
-                const int64_t pos = base[s[u] >> p.part_shift] + loc[u];
-                q.skey[pos] = (uint16_t)(s[u] & mask); q.sval[pos] = v[u];
+                const uint32_t part = s[u] >> p.part_shift, slot = coff[part] + rk[u];
+                st_key[slot] = (uint16_t)(s[u] & mask); st_part[slot] = (uint16_t)part; st_val[slot] = v[u];
             }
+            __syncthreads();
+            for(uint32_t j = threadIdx.x; j < total; j += SC_BLOCK){
+                const uint32_t k = st_part[j];
+                const int64_t pos = pos0[k] + (int64_t)(j - coff[k]);
+                q.skey[pos] = st_key[j]; q.sval[pos] = st_val[j];
+            }
+            __syncthreads();
+            for(int k = threadIdx.x; k < P; k += SC_BLOCK) pos0[k] += (int64_t)((k + 1 < P ? coff[k + 1] : total) - coff[k]);
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 
@@ -490,7 +523,7 @@ static int32_t batch_enqueue(hpa_batch *b, hipStream_t s){
     hipLaunchKernelGGL(k_batch_agg, dim3(b->grid), dim3(b->block), agg_lds, s, p);
     if(b->npq){
         hipLaunchKernelGGL(k_batch_pscan, dim3((unsigned)b->npq), dim3(1024), 0, s, p);
-        hipLaunchKernelGGL(k_batch_scatter, dim3(num_cus() * 4), dim3(SC_BLOCK), (size_t)PART_MAX * 12, s, p);
+        hipLaunchKernelGGL(k_batch_scatter, dim3(num_cus()), dim3(SC_BLOCK), 0, s, p);
         hipLaunchKernelGGL(k_batch_pagg, dim3(num_cus() * 4), dim3(256), lds_bytes(b->part_keys), s, p);
     }
     hipLaunchKernelGGL(k_batch_emit, dim3(num_cus() * 8), dim3(256), 0, s, p);

@@ -110,11 +110,13 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K *kin, const uin
                                                          int64_t *out64, int reverse, L n, int shift, unsigned dmask,
                                                          const uint32_t *off, L ntiles){
     constexpr int NB = 1 << DB, PER = NB / RS_BLOCK;   // digits per thread in the tile's digit scan
-    __shared__ uint32_t wcnt[4][NB];           // per-wave digit counts, then per-wave digit starts
+    // LDS kept small for occupancy: 16-bit per-wave counts (at most 4096), one staging buffer
+    // that holds the tile's keys and then its values
+    __shared__ uint16_t wcnt[4][NB];           // per-wave digit counts, then per-wave digit starts
     __shared__ uint32_t gbase[NB];             // output position of a digit's run minus its tile-local start (>= 0)
     __shared__ uint32_t wsum[RS_BLOCK / 64];
     __shared__ K sk[RS_TILE];
-    __shared__ uint32_t sv[RS_TILE];
+    uint32_t *sv = (uint32_t*)sk;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for(int q = 0; q < 4; q++) for(int d = threadIdx.x; d < NB; d += RS_BLOCK) wcnt[q][d] = 0;
     __syncthreads();
@@ -143,7 +145,7 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K *kin, const uin
         }
         uint32_t before = (uint32_t)__popcll(peers & lt);
         uint32_t cur = wcnt[w][d];
-        if(valid && before == 0) wcnt[w][d] = cur + (uint32_t)__popcll(peers);
+        if(valid && before == 0) wcnt[w][d] = (uint16_t)(cur + (uint32_t)__popcll(peers));
         rk[r] = cur + before;
     }
     __syncthreads();
@@ -152,7 +154,7 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K *kin, const uin
         // digits [t * PER, t * PER + PER)), then the start of each wave's share inside its digit
         uint32_t tot = 0;
         #pragma unroll
-        for(int j = 0; j < PER; j++){ const int d = threadIdx.x * PER + j; tot += wcnt[0][d] + wcnt[1][d] + wcnt[2][d] + wcnt[3][d]; }
+        for(int j = 0; j < PER; j++){ const int d = threadIdx.x * PER + j; tot += (uint32_t)wcnt[0][d] + wcnt[1][d] + wcnt[2][d] + wcnt[3][d]; }
         uint32_t inc = tot;
         for(int o = 1; o < 64; o <<= 1){ uint32_t t = __shfl_up(inc, o, 64); if(lane >= o) inc += t; }
         if(lane == 63) wsum[w] = inc;
@@ -163,9 +165,33 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K *kin, const uin
         for(int j = 0; j < PER; j++){
             const int d = threadIdx.x * PER + j;
             const uint32_t c0 = wcnt[0][d], c1 = wcnt[1][d], c2 = wcnt[2][d], c3 = wcnt[3][d];
-            wcnt[0][d] = pre; wcnt[1][d] = pre + c0; wcnt[2][d] = pre + c0 + c1; wcnt[3][d] = pre + c0 + c1 + c2;
+            wcnt[0][d] = (uint16_t)pre; wcnt[1][d] = (uint16_t)(pre + c0);
+            wcnt[2][d] = (uint16_t)(pre + c0 + c1); wcnt[3][d] = (uint16_t)(pre + c0 + c1 + c2);
             gbase[d] = off[(L)tile * NB + d] - pre;
             pre += c0 + c1 + c2 + c3;
+        }
+    }
+    __syncthreads();
+    // keys laid out digit by digit; each thread then takes slots tid + 256 k, keeping their output
+    // positions for the values
+    #pragma unroll
+    for(int r = 0; r < RS_ROWS; r++){
+        L i = tbase + (L)r * 64 + lane;
+        if(i < n){
+            unsigned d = (unsigned)((kr[r] >> shift) & (K)dmask);
+            sk[wcnt[w][d] + rk[r]] = kr[r];
+        }
+    }
+    __syncthreads();
+    const int cnt = (int)(n - t0 < (L)RS_TILE ? n - t0 : (L)RS_TILE);
+    uint32_t pos[RS_ROWS];
+    #pragma unroll
+    for(int k = 0; k < RS_ROWS; k++){
+        const int j = threadIdx.x + k * RS_BLOCK;
+        if(j < cnt){
+            const K key = sk[j];
+            pos[k] = gbase[(unsigned)((key >> shift) & (K)dmask)] + (uint32_t)j;
+            if(MODE != 1) kout[pos[k]] = key;
         }
     }
     __syncthreads();
@@ -174,18 +200,20 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K *kin, const uin
         L i = tbase + (L)r * 64 + lane;
         if(i < n){
             unsigned d = (unsigned)((kr[r] >> shift) & (K)dmask);
-            uint32_t lp = wcnt[w][d] + rk[r];
-            sk[lp] = kr[r]; sv[lp] = vr[r];
+            sv[wcnt[w][d] + rk[r]] = vr[r];
         }
     }
     __syncthreads();
-    const int cnt = (int)(n - t0 < (L)RS_TILE ? n - t0 : (L)RS_TILE);
-    for(int j = threadIdx.x; j < cnt; j += RS_BLOCK){
-        const K k = sk[j];
-        const L pos = (L)gbase[(unsigned)((k >> shift) & (K)dmask)] + j;
-        if(MODE == 1) out64[reverse ? n - 1 - pos : pos] = (int64_t)sv[j];
-        else if(MODE == 2){ kout[pos] = k; out64[pos] = (int64_t)sv[j]; }
-        else { kout[pos] = k; vout[pos] = sv[j]; }
+    #pragma unroll
+    for(int k = 0; k < RS_ROWS; k++){
+        const int j = threadIdx.x + k * RS_BLOCK;
+        if(j < cnt){
+            const uint32_t v = sv[j];
+            const L p = (L)pos[k];
+            if(MODE == 1) out64[reverse ? n - 1 - p : p] = (int64_t)v;
+            else if(MODE == 2) out64[p] = (int64_t)v;
+            else vout[p] = v;
+        }
     }
 }
 // Exclusive scan of tile-major digit counts (NB digits) in digit-major order:
