@@ -357,13 +357,13 @@ class Batch:
     def run(self, graph=True):
         self.lib.rc("hpaBatchRun", self.h, 1 if graph else 0)
 
-    def results(self):
+    def results(self, fn="hpaBatchFetch"):
         """[{key, count, sum}] per query: numpy views of the plan's pinned staging, filled with
         one synchronisation (valid until the next call)."""
         nq = self.nq
         ng = (C.c_int64 * nq)()
         k, c, s = (C.c_void_p * nq)(), (C.c_void_p * nq)(), (C.c_void_p * nq)()
-        self.lib.rc("hpaBatchFetch", self.h, ng, k, c, s)
+        self.lib.rc(fn, self.h, ng, k, c, s)
 
         def view(p, n, ct, dt):
             if n == 0:
@@ -371,6 +371,14 @@ class Batch:
             return np.ctypeslib.as_array(C.cast(C.c_void_p(p), C.POINTER(ct)), shape=(n,))
         return [dict(key=view(k[q], ng[q], C.c_int32, np.int32), count=view(c[q], ng[q], C.c_int64, np.int64),
                      sum=view(s[q], ng[q], C.c_double, np.float64)) for q in range(nq)]
+
+    def fetch_begin(self):
+        """Start copying the last run's results to the host; the next run may start at once."""
+        self.lib.rc("hpaBatchFetchBegin", self.h)
+
+    def fetch_end(self):
+        """The results of the oldest begun fetch (valid until two more fetch_begin calls)."""
+        return self.results("hpaBatchFetchEnd")
 
     def result_nodes(self, q):
         """Query q's results as value nodes (device payloads) for further primitives."""

@@ -163,10 +163,18 @@ __global__ void k_batch_agg(BPlan p){
             unsigned *lc = (unsigned*)g_batch_lds;
             for(int k = threadIdx.x; k < q.nparts; k += blockDim.x) lc[k] = 0u;
             __syncthreads();
-            for(int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x){
-                if(batch_pass(op, q.flt[i], thr)){
-                    uint32_t s = (uint32_t)q.key[i] - (uint32_t)kmin;
-                    if(s < range) atomicAdd(&lc[s >> p.part_shift], 1u); else bad = true;
+            // four rows per thread per step, their loads issued together
+            const int64_t step = (int64_t)blockDim.x;
+            for(int64_t i0 = r0 + threadIdx.x; i0 < r1; i0 += 4 * step){
+                int32_t f[4], k[4];
+                #pragma unroll
+                for(int u = 0; u < 4; u++){ const int64_t i = i0 + u * step; f[u] = i < r1 ? q.flt[i] : 0; k[u] = i < r1 ? q.key[i] : 0; }
+                #pragma unroll
+                for(int u = 0; u < 4; u++){
+                    if(i0 + u * step < r1 && batch_pass(op, f[u], thr)){
+                        const uint32_t s = (uint32_t)k[u] - (uint32_t)kmin;
+                        if(s < range) atomicAdd(&lc[s >> p.part_shift], 1u); else bad = true;
+                    }
                 }
             }
             __syncthreads();
@@ -181,15 +189,25 @@ __global__ void k_batch_agg(BPlan p){
             for(int k = threadIdx.x; k < tot; k += blockDim.x){ lhi[k] = 0.0; llo[k] = 0.0; lcnt[k] = 0u; }
             __syncthreads();
             const int base = (w % reps) * q.range;
-            for(int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x){
-                if(batch_pass(op, q.flt[i], thr)){
-                    uint32_t s = (uint32_t)q.key[i] - (uint32_t)kmin;
-                    if(s < range){
-                        double v = q.val[i];
-                        double old = atomicAdd(&lhi[base + s], v);
-                        atomicAdd(&llo[base + s], two_sum_err(old, v, old + v));
-                        atomicAdd(&lcnt[base + s], 1u);
-                    } else bad = true;
+            const int64_t step = (int64_t)blockDim.x;
+            for(int64_t i0 = r0 + threadIdx.x; i0 < r1; i0 += 4 * step){
+                int32_t f[4], k[4]; double v[4];
+                #pragma unroll
+                for(int u = 0; u < 4; u++){
+                    const int64_t i = i0 + u * step;
+                    const bool in = i < r1;
+                    f[u] = in ? q.flt[i] : 0; k[u] = in ? q.key[i] : 0; v[u] = in ? q.val[i] : 0.0;
+                }
+                #pragma unroll
+                for(int u = 0; u < 4; u++){
+                    if(i0 + u * step < r1 && batch_pass(op, f[u], thr)){
+                        const uint32_t s = (uint32_t)k[u] - (uint32_t)kmin;
+                        if(s < range){
+                            double old = atomicAdd(&lhi[base + s], v[u]);
+                            atomicAdd(&llo[base + s], two_sum_err(old, v[u], old + v[u]));
+                            atomicAdd(&lcnt[base + s], 1u);
+                        } else bad = true;
+                    }
                 }
             }
             __syncthreads();
@@ -224,31 +242,31 @@ __global__ void k_batch_agg(BPlan p){
 }
 
 // Exclusive scan of one partitioned query's (partition, tile) counts per workgroup, in rounds of
-// 4096 entries (four consecutive per thread), carrying the running total between rounds.
+// 16384 entries (sixteen consecutive per thread: a wave scan of the thread sums, then the 16
+// wave totals), carrying the running total between rounds.
+#define PS_PER 16
 __global__ void __launch_bounds__(1024) k_batch_pscan(BPlan p){
-    __shared__ unsigned long long sh[1024];
+    __shared__ unsigned long long s_w[16];
     __shared__ unsigned long long s_carry;
     const BQuery &q = p.q[p.pq[blockIdx.x]];
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int64_t N = (int64_t)q.nparts * q.ntiles;
     if(t == 0) s_carry = 0;
     __syncthreads();
-    for(int64_t r0 = 0; r0 < N; r0 += 4096){
-        unsigned long long v[4], s = 0;
-        for(int k = 0; k < 4; k++){ int64_t i = r0 + t * 4 + k; v[k] = i < N ? q.pcnt[i] : 0ull; s += v[k]; }
-        sh[t] = s;
+    for(int64_t r0 = 0; r0 < N; r0 += 1024 * PS_PER){
+        unsigned v[PS_PER]; unsigned long long s = 0;
+        #pragma unroll
+        for(int k = 0; k < PS_PER; k++){ const int64_t i = r0 + (int64_t)t * PS_PER + k; v[k] = i < N ? q.pcnt[i] : 0u; s += v[k]; }
+        unsigned long long inc = s;
+        for(int o = 1; o < 64; o <<= 1){ const unsigned long long y = __shfl_up(inc, o, 64); if(lane >= o) inc += y; }
+        if(lane == 63) s_w[w] = inc;
         __syncthreads();
-        for(int o = 1; o < 1024; o <<= 1){
-            unsigned long long a = t >= o ? sh[t - o] : 0ull;
-            __syncthreads();
-            sh[t] += a;
-            __syncthreads();
-        }
-        const unsigned long long carry = s_carry;
-        unsigned long long run = carry + sh[t] - s;
-        for(int k = 0; k < 4; k++){ int64_t i = r0 + t * 4 + k; if(i < N) q.poff[i] = (int64_t)run; run += v[k]; }
+        unsigned long long pre = s_carry + inc - s, tot = 0;
+        for(int k = 0; k < 16; k++){ if(k < w) pre += s_w[k]; tot += s_w[k]; }
+        #pragma unroll
+        for(int k = 0; k < PS_PER; k++){ const int64_t i = r0 + (int64_t)t * PS_PER + k; if(i < N) q.poff[i] = (int64_t)pre; pre += v[k]; }
         __syncthreads();
-        if(t == 1023) s_carry = carry + sh[1023];
+        if(t == 0) s_carry += tot;
         __syncthreads();
     }
     if(t == 0) q.poff[N] = (int64_t)s_carry;
@@ -455,6 +473,14 @@ struct hpa_batch {
     std::vector<int64_t> ng; bool ng_valid = false;
     double scan_bytes = 0;         // algorithmic bytes of one run: the three input columns, once
     char *pinned = nullptr;        // host staging of every query's results (hpaBatchFetch)
+    // pipelined fetch (hpaBatchFetchBegin / End): two snapshot slots in HBM and pinned memory,
+    // copied to the host on a stream of their own while the next run computes
+    hipStream_t cstream = nullptr;
+    hipEvent_t ev_run[2] = {nullptr, nullptr}, ev_copy[2] = {nullptr, nullptr};
+    char *dsnap[2] = {nullptr, nullptr}, *hsnap[2] = {nullptr, nullptr};
+    int64_t snap_cap = 0;          // groups a slot holds
+    int64_t gknown = -1;           // groups of the last run whose count was read
+    int64_t runs = 0, begun = 0, ended = 0, slot_run[2] = {-1, -1};
 };
 
 static BPlan plan_of(hpa_batch *b){
@@ -533,6 +559,15 @@ static int32_t batch_enqueue(hpa_batch *b, hipStream_t s){
 extern "C" void hpaBatchDestroy(hpa_batch *b){
     if(!b) R;
     drop_graph(b);
+    if(b->cstream) hipStreamSynchronize(b->cstream);
+    hipStreamSynchronize(b->c->stream);
+    for(int k = 0; k < 2; k++){
+        if(b->dsnap[k]) hipFree(b->dsnap[k]);
+        if(b->hsnap[k]) hipHostFree(b->hsnap[k]);
+        if(b->ev_run[k]) hipEventDestroy(b->ev_run[k]);
+        if(b->ev_copy[k]) hipEventDestroy(b->ev_copy[k]);
+    }
+    if(b->cstream) hipStreamDestroy(b->cstream);
     if(b->pinned) hipHostFree(b->pinned);
     delete b;
 }
@@ -620,6 +655,7 @@ extern "C" int32_t hpaBatchRun(hpa_batch *b, int32_t use_graph){
     hpa_ctx *c = b->c;
     int32_t e;
     b->ng_valid = false;
+    b->runs++;
     if(use_graph){
         if(!b->captured){
             if(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) R HPA_E_DEVICE;
@@ -696,6 +732,93 @@ extern "C" int32_t hpaBatchFetch(hpa_batch *b, int64_t *ng, void **keys, void **
         ng[qi] = z - a; keys[qi] = hk + a; cnts[qi] = hc + a; sums[qi] = hs + a;
     }
     R hipStreamSynchronize(c->stream) == hipSuccess ? 0 : HPA_E_DEVICE;
+}
+
+// Pipelined fetch.  Begin: on the plan's stream, a snapshot of the last run's group counts and
+// results (as many groups as the last known count; the first call reads that count) into slot
+// `begun & 1` in HBM (a device copy), then on the copy stream the snapshot's copy into the
+// slot's pinned buffer; returns at once, so the next hpaBatchRun computes while the results
+// travel.  End: waits for the oldest begun slot and hands out its results as hpaBatchFetch does
+// (valid until that slot is begun again, two begins later).  A run with more groups than its
+// snapshot took (its tables changed since the count was read) fails with HPA_E_LENGTH.
+static int32_t ensure_snap(hpa_batch *b, int64_t cap){
+    if(!b->cstream){
+        if(hipStreamCreateWithFlags(&b->cstream, hipStreamNonBlocking) != hipSuccess) R HPA_E_DEVICE;
+        for(int k = 0; k < 2; k++)
+            if(hipEventCreateWithFlags(&b->ev_run[k], hipEventDisableTiming) != hipSuccess ||
+               hipEventCreateWithFlags(&b->ev_copy[k], hipEventDisableTiming) != hipSuccess) R HPA_E_DEVICE;
+    }
+    if(cap <= b->snap_cap) R 0;
+    hipStreamSynchronize(b->cstream); hipStreamSynchronize(b->c->stream);
+    for(int k = 0; k < 2; k++){
+        if(b->dsnap[k]){ hipFree(b->dsnap[k]); b->dsnap[k] = nullptr; }
+        if(b->hsnap[k]){ hipHostFree(b->hsnap[k]); b->hsnap[k] = nullptr; }
+    }
+    const size_t bytes = (size_t)(16 * b->nq + 20 * (cap > 0 ? cap : 1) + 64);
+    for(int k = 0; k < 2; k++){
+        if(hipMalloc((void**)&b->dsnap[k], bytes) != hipSuccess){ set_error_detail("batch snapshot of %zu bytes", bytes); R HPA_E_MEMORY; }
+        if(hipHostMalloc((void**)&b->hsnap[k], bytes, hipHostMallocDefault) != hipSuccess){ set_error_detail("batch pinned snapshot"); R HPA_E_MEMORY; }
+    }
+    b->snap_cap = cap;
+    R 0;
+}
+// slot layout: ng (16 nq bytes), then keys (4 cap), counts (8 cap), sums (8 cap)
+static void snap_parts(hpa_batch *b, char *base, int64_t **ng, int32_t **k, int64_t **c, double **s){
+    *ng = (int64_t*)base;
+    char *p = base + 16 * b->nq;
+    *c = (int64_t*)p; *s = (double*)(p + 8 * b->snap_cap); *k = (int32_t*)(p + 16 * b->snap_cap);
+}
+extern "C" int32_t hpaBatchFetchBegin(hpa_batch *b){
+    hpa_ctx *c = b->c;
+    int32_t e;
+    if(b->begun - b->ended >= 2){ set_error_detail("batch: two fetches begun and not ended"); R HPA_E_DOMAIN; }
+    if(b->gknown < 0){
+        if((e = batch_ends(b))) R e;
+        b->gknown = b->ng[2 * (b->nq - 1)];
+    }
+    const int64_t cap = b->gknown;
+    if((e = ensure_snap(b, cap))) R e;
+    const int slot = (int)(b->begun & 1);
+    // the slot's previous copy to the host has finished before the snapshot overwrites it
+    if(b->slot_run[slot] >= 0 && hipStreamWaitEvent(c->stream, b->ev_copy[slot], 0) != hipSuccess) R HPA_E_DEVICE;
+    int64_t *ng; int32_t *k; int64_t *cn; double *sm;
+    snap_parts(b, b->dsnap[slot], &ng, &k, &cn, &sm);
+    if(hipMemcpyAsync(ng, b->ngdev, (size_t)(16 * b->nq), hipMemcpyDeviceToDevice, c->stream) != hipSuccess) R HPA_E_DEVICE;
+    if(cap > 0){
+        if(hipMemcpyAsync(k, b->gkey, (size_t)(4 * cap), hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+           hipMemcpyAsync(cn, b->gcnt, (size_t)(8 * cap), hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+           hipMemcpyAsync(sm, b->gsum, (size_t)(8 * cap), hipMemcpyDeviceToDevice, c->stream) != hipSuccess) R HPA_E_DEVICE;
+    }
+    if(hipEventRecord(b->ev_run[slot], c->stream) != hipSuccess) R HPA_E_DEVICE;
+    if(hipStreamWaitEvent(b->cstream, b->ev_run[slot], 0) != hipSuccess) R HPA_E_DEVICE;
+    const size_t bytes = (size_t)(16 * b->nq + 20 * cap);
+    if(hipMemcpyAsync(b->hsnap[slot], b->dsnap[slot], bytes, hipMemcpyDeviceToHost, b->cstream) != hipSuccess) R HPA_E_DEVICE;
+    if(hipEventRecord(b->ev_copy[slot], b->cstream) != hipSuccess) R HPA_E_DEVICE;
+    b->slot_run[slot] = b->runs;
+    b->begun++;
+    R 0;
+}
+extern "C" int32_t hpaBatchFetchEnd(hpa_batch *b, int64_t *ng, void **keys, void **cnts, void **sums){
+    if(b->ended >= b->begun){ set_error_detail("batch: no fetch begun"); R HPA_E_DOMAIN; }
+    const int slot = (int)(b->ended & 1);
+    if(hipEventSynchronize(b->ev_copy[slot]) != hipSuccess) R HPA_E_DEVICE;
+    b->ended++;
+    int64_t *hng; int32_t *hk; int64_t *hc; double *hs;
+    snap_parts(b, b->hsnap[slot], &hng, &hk, &hc, &hs);
+    for(int qi = 0; qi < b->nq; qi++)
+        if(hng[2 * qi + 1]){ set_error_detail("batch query %d: key outside the planned range", qi); R HPA_E_DOMAIN; }
+    const int64_t G = hng[2 * (b->nq - 1)];
+    if(G > b->snap_cap){
+        b->gknown = G;
+        set_error_detail("batch: the run made %lld groups, its snapshot took %lld", (long long)G, (long long)b->snap_cap);
+        R HPA_E_LENGTH;
+    }
+    b->gknown = G;
+    for(int qi = 0; qi < b->nq; qi++){
+        const int64_t a = qi > 0 ? hng[2 * (qi - 1)] : 0, z = hng[2 * qi];
+        ng[qi] = z - a; keys[qi] = hk + a; cnts[qi] = hc + a; sums[qi] = hs + a;
+    }
+    R 0;
 }
 
 // Autotuner: time each candidate (block size, tile rows, LDS key budget, partition width) over

@@ -104,6 +104,8 @@ class Device(PrimitiveLib):
         _bind(dll, "hpaBatchRun", i32, [vp, i32])
         _bind(dll, "hpaBatchResult", i32, [vp, i32, V, V, V])
         _bind(dll, "hpaBatchFetch", i32, [vp, C.POINTER(i64), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)])
+        _bind(dll, "hpaBatchFetchBegin", i32, [vp])
+        _bind(dll, "hpaBatchFetchEnd", i32, [vp, C.POINTER(i64), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)])
         _bind(dll, "hpaBatchSetConfig", i32, [vp, i32, i32, i32, i32])
         _bind(dll, "hpaBatchGetConfig", i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)])
         _bind(dll, "hpaBatchAutotune", i32, [vp, i32, C.POINTER(C.c_float), i32, C.POINTER(i32)])

@@ -371,6 +371,14 @@ int32_t  hpaBatchResult(hpa_batch *b, int32_t qi, hpa_V zkey, hpa_V zcnt, hpa_V 
 /* Every query's results of the last run in pinned host memory owned by the plan (valid until
  * the next fetch): ng[q] groups, keys[q] (i32), cnts[q] (i64), sums[q] (f64). */
 int32_t  hpaBatchFetch(hpa_batch *b, int64_t *ng, void **keys, void **cnts, void **sums);
+/* The same fetch pipelined with the next run: Begin snapshots the last run's results in HBM and
+ * starts their copy to pinned host memory on a stream of its own (it returns at once, so a
+ * following hpaBatchRun computes while they travel); End waits for the oldest begun fetch and
+ * gives its results as hpaBatchFetch does, valid until two more Begins.  At most two fetches
+ * are outstanding.  The snapshot takes as many groups as the last counted run made; a run that
+ * makes more (its tables changed) fails in End with HPA_E_LENGTH. */
+int32_t  hpaBatchFetchBegin(hpa_batch *b);
+int32_t  hpaBatchFetchEnd(hpa_batch *b, int64_t *ng, void **keys, void **cnts, void **sums);
 /* Tunables: block size, rows per work item, LDS key budget, and the key-partition width of
  * queries whose key range exceeds the LDS budget (0 = compensated global atomics instead;
  * otherwise a power of two in [1024, 2048]). */

@@ -72,6 +72,30 @@ def test_batch_vs_oracle(gpu, orc):
         gpu.release(m)
 
 
+def test_batch_pipelined_fetch(gpu, orc):
+    """hpaBatchFetchBegin/End: results of run k copied to the host while run k+1 computes, two
+    fetches in flight; every run's results equal the oracle's; a run with more groups than the
+    snapshot took fails in End with HPA_E_LENGTH."""
+    from horsepower_amd.vnode import HorseError
+    ts, thrs = make_tables()
+    exp = expected(orc, ts, thrs)
+    m = gpu.mark()
+    b = Batch(gpu, [Q.load_table(gpu, t) for t in ts], thrs)
+    try:
+        b.run(True)
+        b.fetch_begin()
+        for k in range(6):
+            b.run(True)
+            b.fetch_begin()
+            check(b.fetch_end(), exp, ("pipelined", k))
+        check(b.fetch_end(), exp, "last")
+        with pytest.raises(HorseError):
+            b.fetch_end()                                   # nothing begun
+    finally:
+        b.close()
+        gpu.release(m)
+
+
 def test_batch_graph_replays_with_eager_work_between(gpu, orc):
     """Replays of the captured plan, with eager primitive work and synchronisations between
     them, all give the eager result."""
