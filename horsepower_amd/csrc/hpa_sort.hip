@@ -473,13 +473,32 @@ __device__ __forceinline__ bool run_head(const K *sk, const uint32_t *sidx, cons
     return eq ? eq[sidx[i]] != eq[sidx[i - 1]] : sk[i] != sk[i - 1];
 }
 template<typename K>
+__device__ __forceinline__ K shfl_up1(K v){
+    if(sizeof(K) == 8){
+        const uint64_t u = (uint64_t)v;
+        const uint32_t lo = __shfl_up((uint32_t)u, 1, 64), hi = __shfl_up((uint32_t)(u >> 32), 1, 64);
+        return (K)(((uint64_t)hi << 32) | lo);
+    }
+    return (K)__shfl_up((uint32_t)v, 1, 64);
+}
+// run head of row i where the lanes of the wave hold consecutive rows: the previous key comes
+// from the lane below (lane 0 loads it); every lane of the wave must call it
+template<typename K>
+__device__ __forceinline__ bool run_head_w(const K *sk, const uint32_t *sidx, const uint64_t *eq, L i, L n){
+    if(eq) return i < n && run_head(sk, sidx, eq, i);
+    const K cur = i < n ? sk[i] : (K)0;
+    K prev = shfl_up1(cur);
+    if((threadIdx.x & 63) == 0 && i > 0 && i <= n) prev = sk[i - 1];
+    return i < n && (i == 0 || cur != prev);
+}
+template<typename K>
 __global__ void __launch_bounds__(256) k_run_tilecount(const K *sk, const uint32_t *sidx, const uint64_t *eq, L n, uint32_t *tcount){
     const L base = (L)blockIdx.x * RUN_TILE;
     uint32_t c = 0;
     #pragma unroll
     for(int k = 0; k < RUN_ITEMS; k++){
         L i = base + (L)k * 256 + threadIdx.x;
-        c += (i < n && run_head(sk, sidx, eq, i)) ? 1u : 0u;
+        c += run_head_w(sk, sidx, eq, i, n) ? 1u : 0u;
     }
     __shared__ uint32_t sh[4];
     for(int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
@@ -501,7 +520,8 @@ __global__ void __launch_bounds__(256) k_run_write(const K *sk, const uint32_t *
     for(int k = 0; k < RUN_ITEMS; k++){
         L i = base + (L)k * 256 + threadIdx.x;
         bool h = false;
-        if(i < n){ if(!flat_in) flat[i] = (int64_t)sidx[i]; h = run_head(sk, sidx, eq, i); }
+        if(i < n && !flat_in) flat[i] = (int64_t)sidx[i];
+        h = run_head_w(sk, sidx, eq, i, n);
         bal[k] = __ballot(h);
         if(lane == 0) s_cnt[k * 4 + w] = (uint32_t)__popcll(bal[k]);
     }
