@@ -16,6 +16,9 @@ typedef int64_t L;
 #define R return
 
 // ---------------------------------------------------------------- context and arenas
+// mapped completion buffer of a context: a 64-byte head holding the sequence number, then the data
+#define PUB_HEAD 64
+#define PUB_BYTES (64 << 10)
 struct hpa_ctx {
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -30,6 +33,9 @@ struct hpa_ctx {
     std::map<L, Mark> marks;
     // pinned host staging for small device->host reads
     char *pinned = nullptr; L pinned_bytes = 0;
+    // coherent host memory mapped into the device: small reads are published there by a kernel
+    // followed by a sequence number the host spins on (no blocking stream synchronisation)
+    char *mapped = nullptr; char *mapped_dev = nullptr; uint64_t pub_seq = 0;
     // host buffers handed out by hpaToHost, each with the creation sequence number and the
     // device watermark at creation; a rewind releases the ones made after its mark
     struct HostBuf { void *p; uint64_t seq; L dcur; };

@@ -7,6 +7,8 @@
 #include <stdlib.h>
 #include <mutex>
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 
 static std::mutex g_mu;
 static bool g_init = false;
@@ -94,6 +96,9 @@ static hpa_ctx *ctx_new(L arena_bytes, L host_bytes){
     c->hbase = (char*)malloc((size_t)host_bytes); c->htop = host_bytes;
     c->pinned_bytes = 1 << 20;
     if(hipHostMalloc((void**)&c->pinned, (size_t)c->pinned_bytes, hipHostMallocDefault) != hipSuccess) c->pinned = nullptr;
+    if(hipHostMalloc((void**)&c->mapped, PUB_HEAD + PUB_BYTES, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) c->mapped = nullptr;
+    else if(hipHostGetDevicePointer((void**)&c->mapped_dev, c->mapped, 0) != hipSuccess){ hipHostFree(c->mapped); c->mapped = nullptr; }
+    else memset(c->mapped, 0, PUB_HEAD);
     R c;
 }
 
@@ -122,6 +127,7 @@ extern "C" void hpaCtxDestroy(hpa_ctx *c){
     if(c->dbase && c->own_arena){ unregister_device_range(c->dbase); hipFree(c->dbase); }
     if(c->own_stream) hipStreamDestroy(c->stream);
     if(c->pinned) hipHostFree(c->pinned);
+    if(c->mapped) hipHostFree(c->mapped);
     for(auto &hb : c->host_bufs) free(hb.p);
     free(c->hbase);
     if(t_cur == c) t_cur = nullptr;
@@ -292,8 +298,74 @@ bool is_comparable(int32_t t){ R is_real(t) || t==HPA_C || t==HPA_Q || (t>=HPA_M
 bool is_inline(V x){ R x->g == (char*)&x->b; }
 bool is_flat(V x){ R x->typ == HPA_G && x->g2 != nullptr; }
 
+// ---------------------------------------------------------------- small device->host reads
+// A one-workgroup kernel copies the device ranges into the context's coherent mapped host buffer
+// and then writes a new sequence number at its head; the host spins on that number and copies
+// out.  Measured on MI355X: a kernel followed by an 8-byte read takes 7.3 us a round trip this
+// way against 16.6 us with a copy command and a blocking stream synchronisation.  A spin longer
+// than PUB_SPIN_US falls back to a blocking synchronisation, which also reports a failed stream.
+#define PUB_MAX 32
+struct PubList { const char *src[PUB_MAX]; int64_t off[PUB_MAX]; int64_t bytes[PUB_MAX]; int n; };
+__global__ void __launch_bounds__(256) k_publish(PubList r, char *dst, unsigned long long *seqp, unsigned long long seq){
+    for(int k = 0; k < r.n; k++){
+        const char *s = r.src[k]; char *d = dst + r.off[k]; const L nb = r.bytes[k];
+        const bool w8 = ((uintptr_t)s % 8 == 0) && ((uintptr_t)d % 8 == 0);
+        if(w8) for(L i = (L)threadIdx.x * 8; i + 8 <= nb; i += 256 * 8) *(uint64_t*)(d + i) = *(const uint64_t*)(s + i);
+        for(L j = (w8 ? (nb & ~(L)7) : 0) + threadIdx.x; j < nb; j += 256) d[j] = s[j];
+    }
+    __syncthreads();
+    if(threadIdx.x == 0){
+        __threadfence_system();
+        __hip_atomic_store(seqp, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+#define PUB_SPIN_US 4000
+static int32_t pub_wait(hpa_ctx *c, uint64_t seq){
+    volatile uint64_t *f = (volatile uint64_t*)c->mapped;
+    const auto t0 = std::chrono::steady_clock::now();
+    for(uint64_t it = 1; *f != seq; it++){
+        __builtin_ia32_pause();
+        if((it & 255) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(PUB_SPIN_US)){
+            if(hipStreamSynchronize(c->stream) != hipSuccess){ set_error_detail("stream failed before a device read"); R HPA_E_DEVICE; }
+            if(*f != seq){ set_error_detail("device read: completion number %llu not published", (unsigned long long)seq); R HPA_E_DEVICE; }
+            break;
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    R 0;
+}
+// n device ranges into the mapped buffer at 16-byte aligned offsets (returned in off[]); false
+// when they do not fit one publish
+static bool pub_fits(int n, const int64_t *bytes){
+    if(n > PUB_MAX) R false;
+    L tot = 0; for(int k = 0; k < n; k++) tot += (bytes[k] + 15) & ~(L)15;
+    R tot <= PUB_BYTES;
+}
+static int32_t pub_read(hpa_ctx *c, int n, const void *const *src, void *const *dst, const int64_t *bytes){
+    PubList r; r.n = 0; L off = 0;
+    for(int k = 0; k < n; k++){
+        if(bytes[k] <= 0) continue;
+        r.src[r.n] = (const char*)src[k]; r.off[r.n] = off; r.bytes[r.n] = bytes[k]; r.n++;
+        off += (bytes[k] + 15) & ~(L)15;
+    }
+    if(r.n == 0) R 0;
+    const uint64_t seq = ++c->pub_seq;
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(256), 0, c->stream, r, c->mapped_dev + PUB_HEAD,
+                       (unsigned long long*)c->mapped_dev, (unsigned long long)seq);
+    int32_t e;
+    if((e = check_launch("device read"))) R e;
+    if((e = pub_wait(c, seq))) R e;
+    int j = 0;
+    for(int k = 0; k < n; k++) if(bytes[k] > 0){ memcpy(dst[k], c->mapped + PUB_HEAD + r.off[j], (size_t)bytes[k]); j++; }
+    R 0;
+}
+
 int32_t d2h(hpa_ctx *c, void *dst, const void *src, L bytes){
     if(bytes <= 0) R 0;
+    if(c->mapped && bytes <= PUB_BYTES && is_device(src)){
+        const void *s1[1] = {src}; void *d1[1] = {dst}; const int64_t b1[1] = {bytes};
+        R pub_read(c, 1, s1, d1, b1);
+    }
     if(c->pinned && bytes <= c->pinned_bytes){
         if(hipMemcpyAsync(c->pinned, src, (size_t)bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess) R HPA_E_DEVICE;
         if(hipStreamSynchronize(c->stream) != hipSuccess) R HPA_E_DEVICE;
@@ -473,6 +545,7 @@ int32_t read_staged(hpa_ctx *c, int n, const void *const *src, void *const *dst,
     L dev_total = 0;
     for(int k = 0; k < n; k++) if(bytes[k] > 0) dev_total += (bytes[k] + 15) & ~(L)15;
     if(dev_total == 0) R 0;
+    if(c->mapped && pub_fits(n, bytes)) R pub_read(c, n, src, dst, bytes);
     if(c->pinned && dev_total <= c->pinned_bytes){
         L off = 0;
         std::vector<L> at(n, -1);
