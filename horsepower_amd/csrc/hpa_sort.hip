@@ -374,9 +374,12 @@ static int32_t radix_pass(hpa_ctx *c, const K *ka, const uint32_t *va, K *kb, ui
 // all constant are skipped.  Each pass: per-tile digit histograms, a scan in (digit, tile) order,
 // a scatter.
 #define RS_WIDE 10
+// hint (optional): the OR and AND of all keys and whether some key is below the one before it,
+// known from an earlier pass -- the first histogram pass then finds nothing and no read is made
+struct RsHint { unsigned long long orv, andv; bool desc; };
 template<typename K>
 static int32_t radix_sort_t(hpa_ctx *c, const K *keys, const uint32_t *vals, L n, int lo, int hi, K **kout, uint32_t **vout,
-                            int64_t *out64, bool reverse, bool keep_keys = false){
+                            int64_t *out64, bool reverse, bool keep_keys = false, const RsHint *hint = nullptr){
     if(keep_keys && (!out64 || reverse)) R HPA_E_DOMAIN;
     if(n >= (L)UINT32_MAX){ set_error_detail("radix sort of %lld keys exceeds 32-bit row ids", (long long)n); R HPA_E_NOT_IMPL; }
     int32_t e;
@@ -393,14 +396,18 @@ static int32_t radix_sort_t(hpa_ctx *c, const K *keys, const uint32_t *vals, L n
     if(!bits || !hist || !off || !part || !cbase || !tot) R HPA_E_MEMORY;
     const K kmask = (K)((hi - lo >= kw ? ~(K)0 : (((K)1 << (hi - lo)) - 1)) << lo);
     auto dmask_of = [&](int s, int db) -> unsigned { const int w = hi - s < db ? hi - s : db; R (1u << w) - 1u; };
-    {
-        ProfScope ps("radix_hist", c->stream, (double)n * sizeof(K) + 4.0 * NBmax * ntiles);
-        hipLaunchKernelGGL((k_rs_hist<K, wide>), dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream, keys, n, lo, dmask_of(lo, wide),
-                           hist, ntiles, bits + 4, kmask);
-        hipLaunchKernelGGL(k_rs_bits_final, dim3(1), dim3(256), 0, c->stream, bits + 4, ntiles, bits);
+    unsigned long long hb[3];
+    if(hint){ hb[0] = hint->orv; hb[1] = hint->andv; hb[2] = hint->desc ? 1 : 0; }
+    else {
+        {
+            ProfScope ps("radix_hist", c->stream, (double)n * sizeof(K) + 4.0 * NBmax * ntiles);
+            hipLaunchKernelGGL((k_rs_hist<K, wide>), dim3((unsigned)ntiles), dim3(RS_BLOCK), 0, c->stream, keys, n, lo, dmask_of(lo, wide),
+                               hist, ntiles, bits + 4, kmask);
+            hipLaunchKernelGGL(k_rs_bits_final, dim3(1), dim3(256), 0, c->stream, bits + 4, ntiles, bits);
+        }
+        if((e = check_launch("radix_hist"))) R e;
+        if((e = d2h(c, hb, bits, 24))) R e;
     }
-    if((e = check_launch("radix_hist"))) R e;
-    unsigned long long hb[3]; if((e = d2h(c, hb, bits, 24))) R e;
     const unsigned long long vary = (hb[0] ^ hb[1]) & (unsigned long long)kmask;
     if(vary == 0 || hb[2] == 0){
         // every digit is constant, or no key is below the one before: the input order is the
@@ -435,7 +442,7 @@ static int32_t radix_sort_t(hpa_ctx *c, const K *keys, const uint32_t *vals, L n
     for(int r = 0; r < nrun; r++){
         const int s = plan[r].s, db = plan[r].db;
         const bool last = r == nrun - 1, fin = last && out64 != nullptr;
-        const bool have = r == 0 && s == lo && db == wide;
+        const bool have = !hint && r == 0 && s == lo && db == wide;
         K *kd = fin && !keep_keys ? nullptr : kb[r & 1];
         e = db == 8 ? radix_pass<K, 8>(c, ka, va, kd, vb[r & 1], fin ? out64 : nullptr, reverse, n, ntiles, s, dmask_of(s, 8),
                                        have, hist, part, cbase, tot, off)
@@ -455,9 +462,10 @@ int32_t radix_sort_pairs_u32(hpa_ctx *c, const uint32_t *keys, const uint32_t *v
     R radix_sort_t<uint32_t>(c, keys, vals, n, lo, hi, ko, vo, nullptr, false);
 }
 // the sorted keys in *ko and the stable permutation (row ids) as i64 in flat[0, n)
-static int32_t radix_sort_flat_u32(hpa_ctx *c, const uint32_t *keys, L n, int lo, int hi, uint32_t **ko, int64_t *flat){
+static int32_t radix_sort_flat_u32(hpa_ctx *c, const uint32_t *keys, L n, int lo, int hi, uint32_t **ko, int64_t *flat,
+                                   const RsHint *hint = nullptr){
     uint32_t *vo = nullptr;
-    R radix_sort_t<uint32_t>(c, keys, nullptr, n, lo, hi, ko, &vo, flat, false, true);
+    R radix_sort_t<uint32_t>(c, keys, nullptr, n, lo, hi, ko, &vo, flat, false, true, hint);
 }
 
 // ---------------------------------------------------------------- runs -> dict(keys, flat list)
@@ -553,6 +561,13 @@ __global__ void k_run_counts(int64_t *start, L g, L n, int64_t *cnt){
         if(r == 0) start[g] = n;
     }
 }
+// the same with the group count read on the device (*gp), for outputs allocated at a bound
+__global__ void k_run_counts_dev(int64_t *start, const int64_t *gp, L n, int64_t *cnt){
+    const L g = *gp;
+    for(L r = (L)blockIdx.x * blockDim.x + threadIdx.x; r < g; r += (L)gridDim.x * blockDim.x)
+        cnt[r] = (r + 1 < g ? start[r + 1] : n) - start[r];
+    if(blockIdx.x == 0 && threadIdx.x == 0) start[g] = n;
+}
 
 int32_t make_group_dict(hpa_ctx *c, V z, int64_t *keys, int64_t *counts, int64_t *flat, L g, L n){
     V cells = (V)halloc(c, 2 * sizeof(V0));
@@ -573,14 +588,34 @@ int32_t make_group_dict(hpa_ctx *c, V z, int64_t *keys, int64_t *counts, int64_t
 
 // src: the i32 column whose bit patterns sk holds (null otherwise); its groups' key values are
 // then kept beside the keys for index(column, keys)
+// gmax > 0: a bound on the group count; when the arena holds the outputs at that size, every pass
+// is queued at once and the group count is read once at the end
 template<typename K>
 static int32_t runs_to_group_t(hpa_ctx *c, V z, const K *sk, const uint32_t *sidx, L n, const uint64_t *eq, const void *src = nullptr,
-                               int64_t *flat_in = nullptr){
+                               int64_t *flat_in = nullptr, L gmax = 0){
     int32_t e;
     const L nt = ceil_div(n, RUN_TILE) > 0 ? ceil_div(n, RUN_TILE) : 1;
     uint32_t *tcount = (uint32_t*)dalloc(c, 4 * nt);
     int64_t *toff = (int64_t*)dalloc(c, 8 * (nt + 1));
     if(!tcount || !toff) R HPA_E_MEMORY;
+    if(gmax > 0 && n > 0 && c->dcur + 36 * gmax + (flat_in ? 0 : 8 * n) + 4096 <= c->dtop){
+        uint32_t *kval = src ? (uint32_t*)dalloc(c, 4 * gmax) : nullptr;          // before keys: dies with them
+        int64_t *keys = (int64_t*)dalloc(c, 8 * gmax), *start = (int64_t*)dalloc(c, 8 * (gmax + 1));
+        int64_t *cnt = (int64_t*)dalloc(c, 8 * gmax);
+        int64_t *flat = flat_in ? flat_in : (int64_t*)dalloc(c, 8 * n);
+        if(!keys || !start || !cnt || !flat || (src && !kval)) R HPA_E_MEMORY;
+        hipLaunchKernelGGL(k_run_tilecount<K>, dim3((unsigned)nt), dim3(256), 0, c->stream, sk, sidx, eq, n, tcount);
+        if((e = scan_u32_to_i64(c, tcount, toff, nt, toff + nt))) R e;
+        hipLaunchKernelGGL(k_run_write<K>, dim3((unsigned)nt), dim3(256), 0, c->stream, sk, sidx, eq, n, (const int64_t*)toff, keys, start,
+                           flat, kval, (const int64_t*)flat_in);
+        hipLaunchKernelGGL(k_run_counts_dev, dim3(grid_for(gmax, 256, 1)), dim3(256), 0, c->stream, start, (const int64_t*)(toff + nt), n, cnt);
+        if((e = check_launch("group_runs"))) R e;
+        int64_t g = 0; if((e = d2h(c, &g, toff + nt, 8))) R e;
+        if(g > gmax){ set_error_detail("group count %lld above its bound %lld", (long long)g, (long long)gmax); R HPA_E_DOMAIN; }
+        if(kval && g > 1) note_group_vals(c, keys, src, n, kval);
+        note_group_offsets(c, cnt, g, start);
+        R make_group_dict(c, z, keys, cnt, flat, g, n);
+    }
     int64_t g = 0;
     if(n > 0){
         hipLaunchKernelGGL(k_run_tilecount<K>, dim3((unsigned)nt), dim3(256), 0, c->stream, sk, sidx, eq, n, tcount);
@@ -610,8 +645,8 @@ int32_t runs_to_group_u32(hpa_ctx *c, V z, const uint32_t *sk, const uint32_t *s
     R runs_to_group_t<uint32_t>(c, z, sk, sidx, n, nullptr, src);
 }
 // the same over a flat list the sort already wrote (radix_sort_flat_u32)
-static int32_t runs_to_group_flat_u32(hpa_ctx *c, V z, const uint32_t *sk, int64_t *flat, L n, const void *src){
-    R runs_to_group_t<uint32_t>(c, z, sk, nullptr, n, nullptr, src, flat);
+static int32_t runs_to_group_flat_u32(hpa_ctx *c, V z, const uint32_t *sk, int64_t *flat, L n, const void *src, L gmax = 0){
+    R runs_to_group_t<uint32_t>(c, z, sk, nullptr, n, nullptr, src, flat, gmax);
 }
 
 // ---------------------------------------------------------------- key builders
@@ -1133,81 +1168,6 @@ __global__ void __launch_bounds__(256) k_sg_present_sample(const uint8_t *a, con
         part[(L)blockIdx.x * 8 + 4 + (threadIdx.x >> 6)] = mb;
     }
 }
-// Unsigned min / max of an i32 column -- from every row, or from one 16-row chunk in `stride`
-// -- and, alongside, how many consecutive row pairs seen hold different keys: (lo, hi, changes,
-// pairs), reduced over the workgroup into dst[0..3].
-__device__ __forceinline__ void sg_range_block_to(unsigned lo, unsigned hi, unsigned ch, unsigned pr, unsigned *dst){
-    __shared__ unsigned slo[4], shi[4], sch[4], spr[4];
-    for(int o = 32; o > 0; o >>= 1){
-        unsigned t = __shfl_xor(lo, o, 64); lo = t < lo ? t : lo;
-        t = __shfl_xor(hi, o, 64); hi = t > hi ? t : hi;
-        ch += __shfl_xor(ch, o, 64); pr += __shfl_xor(pr, o, 64);
-    }
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if(lane == 0){ slo[w] = lo; shi[w] = hi; sch[w] = ch; spr[w] = pr; }
-    __syncthreads();
-    if(threadIdx.x == 0){
-        for(int q = 1; q < 4; q++){ lo = slo[q] < lo ? slo[q] : lo; hi = shi[q] > hi ? shi[q] : hi; ch += sch[q]; pr += spr[q]; }
-        dst[0] = lo; dst[1] = hi; dst[2] = ch; dst[3] = pr;
-    }
-}
-// per-workgroup partial into part[4 * block ..]
-__device__ __forceinline__ void sg_range_block(unsigned lo, unsigned hi, unsigned ch, unsigned pr, unsigned *part){
-    sg_range_block_to(lo, hi, ch, pr, part + 4 * blockIdx.x);
-}
-__global__ void __launch_bounds__(256) k_sg_urange_sample(const int32_t *x, L n, L stride, unsigned *part){
-    unsigned lo = 0xFFFFFFFFu, hi = 0, ch = 0, pr = 0;
-    const L i = ((L)blockIdx.x * 256 + threadIdx.x) * stride * 16;
-    unsigned prev = 0;
-    for(int j = 0; j < 16 && i + j < n; j++){
-        const unsigned v = (unsigned)x[i + j]; lo = v < lo ? v : lo; hi = v > hi ? v : hi;
-        if(j > 0){ ch += v != prev; pr++; }
-        prev = v;
-    }
-    sg_range_block(lo, hi, ch, pr, part);
-}
-__global__ void __launch_bounds__(256) k_sg_urange(const int32_t *x, L n, int vec, unsigned *part){
-    unsigned lo = 0xFFFFFFFFu, hi = 0, ch = 0, pr = 0;
-    const L S = (L)gridDim.x * 256 * 4;
-    L i = ((L)blockIdx.x * 256 + threadIdx.x) * 4;
-    if(vec){
-        for(; i + (SG_SCAN_UNROLL - 1) * S + 4 <= n; i += SG_SCAN_UNROLL * S){
-            uint4 wv[SG_SCAN_UNROLL];
-            #pragma unroll
-            for(int u = 0; u < SG_SCAN_UNROLL; u++) wv[u] = *(const uint4*)(x + i + u * S);
-            #pragma unroll
-            for(int u = 0; u < SG_SCAN_UNROLL; u++){
-                const unsigned a0 = wv[u].x < wv[u].y ? wv[u].x : wv[u].y, a1 = wv[u].z < wv[u].w ? wv[u].z : wv[u].w;
-                const unsigned b0 = wv[u].x > wv[u].y ? wv[u].x : wv[u].y, b1 = wv[u].z > wv[u].w ? wv[u].z : wv[u].w;
-                const unsigned mn = a0 < a1 ? a0 : a1, mx = b0 > b1 ? b0 : b1;
-                lo = mn < lo ? mn : lo; hi = mx > hi ? mx : hi;
-                ch += (wv[u].x != wv[u].y) + (wv[u].y != wv[u].z) + (wv[u].z != wv[u].w); pr += 3;
-            }
-        }
-    }
-    for(; i < n; i += S){
-        if(vec && i + 4 <= n){
-            const uint4 w4 = *(const uint4*)(x + i);
-            const unsigned a0 = min(w4.x, w4.y), a1 = min(w4.z, w4.w), b0 = max(w4.x, w4.y), b1 = max(w4.z, w4.w);
-            lo = min(lo, min(a0, a1)); hi = max(hi, max(b0, b1));
-            ch += (w4.x != w4.y) + (w4.y != w4.z) + (w4.z != w4.w); pr += 3;
-        } else
-            for(int j = 0; j < 4 && i + j < n; j++){
-                const unsigned v = (unsigned)x[i + j]; lo = v < lo ? v : lo; hi = v > hi ? v : hi;
-                if(j > 0){ ch += v != (unsigned)x[i + j - 1]; pr++; }
-            }
-    }
-    sg_range_block(lo, hi, ch, pr, part);
-}
-// out[0..3] = (lo, hi, changes, pairs) over the workgroups' partials
-__global__ void __launch_bounds__(256) k_sg_urange_final(const unsigned *part, int nblocks, unsigned *out){
-    unsigned lo = 0xFFFFFFFFu, hi = 0, ch = 0, pr = 0;
-    for(int k = threadIdx.x; k < nblocks; k += 256){
-        lo = part[4 * k] < lo ? part[4 * k] : lo; hi = part[4 * k + 1] > hi ? part[4 * k + 1] : hi;
-        ch += part[4 * k + 2]; pr += part[4 * k + 3];
-    }
-    sg_range_block_to(lo, hi, ch, pr, out);
-}
 // workgroups of a whole-column pass that keeps SG_SCAN_UNROLL loads per thread in flight
 static int sg_scan_grid(L n, int rows_per_load){
     L g = ceil_div(n, (L)256 * rows_per_load * SG_SCAN_UNROLL), cap = (L)num_cus() * 8;
@@ -1286,33 +1246,6 @@ static int32_t small_group_trie(hpa_ctx *c, V z, const uint8_t *a, const uint8_t
     }
     R 0;
 }
-// i32 group on at most 256 consecutive unsigned keys; *done false otherwise
-// *change_frac: the share of consecutive row pairs (all of them, or those of the sampled chunks)
-// whose keys differ -- low for columns clustered by key
-static int32_t small_group_i32(hpa_ctx *c, V z, const int32_t *x, L n, bool *done, double *change_frac){
-    *done = false;
-    *change_frac = 1.0;
-    int32_t e;
-    const int vec = (uintptr_t)x % 16 == 0 ? 1 : 0;
-    for(int sample = n >= SG_SAMPLE_MIN ? 1 : 0; sample >= 0; sample--){
-        int grid = sample ? (int)ceil_div(ceil_div(n, 16 * (L)SG_SAMPLE_STRIDE), 256) : sg_scan_grid(n, 4);
-        unsigned *rg = (unsigned*)dalloc(c, 16 * (L)(grid + 1)); if(!rg) R HPA_E_MEMORY;
-        if(sample) hipLaunchKernelGGL(k_sg_urange_sample, dim3(grid), dim3(256), 0, c->stream, x, n, (L)SG_SAMPLE_STRIDE, rg + 4);
-        else hipLaunchKernelGGL(k_sg_urange, dim3(grid), dim3(256), 0, c->stream, x, n, vec, rg + 4);
-        hipLaunchKernelGGL(k_sg_urange_final, dim3(1), dim3(256), 0, c->stream, rg + 4, grid, rg);
-        if((e = check_launch("group_small_range"))) R e;
-        unsigned h[4]; if((e = d2h(c, h, rg, 16))) R e;
-        if(h[3] > 0) *change_frac = (double)h[2] / (double)h[3];
-        if(h[1] - h[0] >= (unsigned)SG_BINS) R 0;
-        SgSrc s; memset(&s, 0, sizeof(s));
-        s.x = x; s.base = h[0]; s.vec = vec;
-        bool missed;
-        if((e = small_group_run(c, z, s, n, (int)(h[1] - h[0]) + 1, &missed))) R e;
-        if(!missed){ *done = true; R 0; }
-    }
-    R 0;
-}
-
 // ---------------------------------------------------------------- group
 // pfnGroupTrie (h_primitive.cc:2656-2681): buckets by (c0<<8)+c1 in ascending code order
 extern "C" int32_t pfnGroupTrie(hpa_V z, hpa_V x){
@@ -1332,17 +1265,127 @@ extern "C" int32_t pfnGroupTrie(hpa_V z, hpa_V x){
     if((e = radix_sort_pairs_u32(c, k, nullptr, n, 0, 16, &sk, &sv))) R e;
     R runs_to_group_u32(c, z, sk, sv, n);
 }
+// ---------------------------------------------------------------- i32 group: one scan of the keys
+// Before an i32 group every key is read once: per 4096-row tile the unsigned minimum and maximum,
+// the OR and AND of the keys, whether some key is below the key before it, how many consecutive
+// rows hold different keys, and the tile's run heads (a row whose key differs from the row
+// before, or whose row number is a multiple of the run cap).  One read returns the summary and
+// the run count, and picks the path: a counting sort over the key range when it spans fewer than
+// SG_BINS values, the run-compressed group below, or the radix sort -- whose digit plan (which
+// key bits vary, whether the input is already in key order) then needs no pass or read of its own.
+struct GiSum { unsigned lo, hi, orv, andv; unsigned long long ch; unsigned desc, pad; };
+__device__ __forceinline__ void gi_fold(GiSum &a, const GiSum &b){
+    a.lo = b.lo < a.lo ? b.lo : a.lo; a.hi = b.hi > a.hi ? b.hi : a.hi;
+    a.orv |= b.orv; a.andv &= b.andv; a.ch += b.ch; a.desc |= b.desc;
+}
+__device__ __forceinline__ void gi_wave_fold(GiSum &a){
+    for(int o = 32; o > 0; o >>= 1){
+        GiSum b;
+        b.lo = __shfl_down(a.lo, o, 64); b.hi = __shfl_down(a.hi, o, 64);
+        b.orv = __shfl_down(a.orv, o, 64); b.andv = __shfl_down(a.andv, o, 64);
+        b.ch = __shfl_down(a.ch, o, 64); b.desc = __shfl_down(a.desc, o, 64);
+        gi_fold(a, b);
+    }
+}
+// Chunks of GI_CHUNK rows, one wave each (lane l holds rows [16 l, 16 l + 16) of the chunk: four
+// 16-byte loads when aligned; the key before its first row is the lane below's last, lane 0
+// reads it); waves take chunks independently, so no barrier holds back the next loads.  The run
+// heads of each chunk go to ccount[chunk]; the summary is folded per workgroup into part.
+#define GI_CHUNK (64 * RUN_ITEMS)
+__global__ void __launch_bounds__(256) k_gi_scan(const int32_t *x, L n, int vec, unsigned cap_mask, uint32_t *ccount, L nchunks,
+                                                 GiSum *part){
+    __shared__ GiSum sp[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    GiSum a = {0xFFFFFFFFu, 0u, 0u, 0xFFFFFFFFu, 0ull, 0u, 0u};
+    const L nw = (L)gridDim.x * 4;
+    for(L ch = (L)blockIdx.x * 4 + w; ch < nchunks; ch += nw){
+        const L i0 = ch * GI_CHUNK + (L)lane * RUN_ITEMS;
+        unsigned v[RUN_ITEMS];
+        if(vec && i0 + RUN_ITEMS <= n){
+            #pragma unroll
+            for(int q = 0; q < RUN_ITEMS / 4; q++){
+                const uint4 u = *(const uint4*)(x + i0 + 4 * q);
+                v[4 * q] = u.x; v[4 * q + 1] = u.y; v[4 * q + 2] = u.z; v[4 * q + 3] = u.w;
+            }
+        } else {
+            #pragma unroll
+            for(int j = 0; j < RUN_ITEMS; j++) v[j] = i0 + j < n ? (unsigned)x[i0 + j] : 0u;
+        }
+        unsigned prev = __shfl_up(v[RUN_ITEMS - 1], 1, 64);
+        if(lane == 0) prev = (i0 > 0 && i0 <= n) ? (unsigned)x[i0 - 1] : 0u;
+        uint32_t hc = 0;
+        #pragma unroll
+        for(int j = 0; j < RUN_ITEMS; j++){
+            const L i = i0 + j;
+            if(i < n){
+                const unsigned k = v[j], pk = j == 0 ? prev : v[j - 1];
+                a.lo = k < a.lo ? k : a.lo; a.hi = k > a.hi ? k : a.hi; a.orv |= k; a.andv &= k;
+                const bool dif = i > 0 && k != pk;
+                a.ch += dif ? 1ull : 0ull;
+                a.desc |= (i > 0 && k < pk) ? 1u : 0u;
+                hc += (((uint64_t)i & cap_mask) == 0 || dif) ? 1u : 0u;
+            }
+        }
+        for(int o = 32; o > 0; o >>= 1) hc += __shfl_down(hc, o, 64);
+        if(lane == 0) ccount[ch] = hc;
+    }
+    gi_wave_fold(a);
+    if(lane == 0) sp[w] = a;
+    __syncthreads();
+    if(threadIdx.x == 0){
+        for(int q = 1; q < 4; q++) gi_fold(a, sp[q]);
+        part[blockIdx.x] = a;
+    }
+}
+__global__ void __launch_bounds__(256) k_gi_final(const GiSum *part, int nb, GiSum *out){
+    __shared__ GiSum sp[4];
+    GiSum a = {0xFFFFFFFFu, 0u, 0u, 0xFFFFFFFFu, 0ull, 0u, 0u};
+    for(int k = threadIdx.x; k < nb; k += 256) gi_fold(a, part[k]);
+    gi_wave_fold(a);
+    if((threadIdx.x & 63) == 0) sp[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if(threadIdx.x == 0){ for(int q = 1; q < 4; q++) gi_fold(a, sp[q]); *out = a; }
+}
+struct GiScan { GiSum s; int64_t nr; int64_t *toff; };
+// summary and run count of x (cap_mask: run cap - 1); toff[k] = runs before chunk k, so the
+// runs before a RUN_TILE tile t are toff[t * (RUN_TILE / GI_CHUNK)]
+static int32_t gi_scan(hpa_ctx *c, const int32_t *x, L n, unsigned cap_mask, GiScan *r){
+    int32_t e;
+    const L nch = ceil_div(n, GI_CHUNK);
+    const L cap = (L)num_cus() * 8;
+    const L nb = ceil_div(nch, 4);
+    const int grid = (int)(nb < cap ? nb : cap);
+    uint32_t *ccount = (uint32_t*)dalloc(c, 4 * nch);
+    int64_t *toff = (int64_t*)dalloc(c, 8 * (nch + 1));
+    GiSum *part = (GiSum*)dalloc(c, (L)sizeof(GiSum) * (grid + 1));
+    if(!ccount || !toff || !part) R HPA_E_MEMORY;
+    {
+        ProfScope ps("group_scan", c->stream, 4.0 * n);
+        hipLaunchKernelGGL(k_gi_scan, dim3(grid), dim3(256), 0, c->stream, x, n, (uintptr_t)x % 16 == 0 ? 1 : 0, cap_mask, ccount, nch, part + 1);
+        hipLaunchKernelGGL(k_gi_final, dim3(1), dim3(256), 0, c->stream, (const GiSum*)(part + 1), grid, part);
+    }
+    if((e = check_launch("group_scan"))) R e;
+    if((e = scan_u32_to_i64(c, ccount, toff, nch, toff + nch))) R e;
+    const void *src[2] = {part, toff + nch};
+    void *dst[2] = {&r->s, &r->nr};
+    const int64_t bytes[2] = {(int64_t)sizeof(GiSum), 8};
+    if((e = read_staged(c, 2, src, dst, bytes))) R e;
+    r->toff = toff;
+    R 0;
+}
 
 // Group of an i32 column whose equal keys sit in runs of consecutive rows (a fact table stored in
 // key order of another table, say).  The stable sort that defines the group only has to order
 // the runs: runs of one key keep their input order, and each run expands to ascending row
 // numbers.  A run is cut at every cap-th row, cap = 2^cb with cb = min(6, 32 - bits of the row
-// count), so that (start << cb) | (length - 1) fits the 32-bit value the sort carries.  Passes:
-// run heads counted per tile, a scan, the runs' (key, packed start and length) written; a stable
-// radix sort of the run keys; the run lengths and their scan (flat positions); the expansion of
-// each run into the flat list; group heads over the sorted run keys.  Taken when consecutive
-// rows show few key changes; abandoned (nothing written) when the runs turn out to be more than
-// GR_RUNS_MAX_SHARE of the rows.  Its extra launches and host read pay off from about 2^24 rows.
+// count), so that (start << cb) | (length - 1) fits the 32-bit value the sort carries.  Passes
+// after the scan: the runs' (key, packed start and length) written at the scan's run offsets; a
+// stable radix sort of the runs (digit plan from the scan); per tile of sorted runs, its group
+// heads and rows; one launch that expands every run into the flat list at its position and
+// writes each group's first row, flat offset and key value; the counts.  The outputs are
+// allocated for as many groups as runs, so the only read is the group count at the end.  Taken
+// from 2^24 rows when consecutive rows show few key changes and the runs are at most
+// GR_RUNS_MAX_SHARE of the rows.
 #define GR_RUNS_MIN (1 << 24)
 #define GR_RUNS_MAX_CHANGE 0.4
 #define GR_RUNS_MAX_SHARE 0.55
@@ -1352,25 +1395,11 @@ __device__ __forceinline__ bool gr_head(int32_t xi, L i, L n, unsigned cap_mask)
     const int32_t xp = __shfl_up(xi, 1, 64);
     return i < n && (((uint64_t)i & cap_mask) == 0 || xi != xp);
 }
-__global__ void __launch_bounds__(256) k_gr_head_count(const int32_t *x, L n, unsigned cap_mask, uint32_t *tcount){
-    const L base = (L)blockIdx.x * RUN_TILE;
-    int32_t xv[RUN_ITEMS];
-    #pragma unroll
-    for(int k = 0; k < RUN_ITEMS; k++){ const L i = base + (L)k * 256 + threadIdx.x; xv[k] = i < n ? x[i] : 0; }
-    uint32_t c = 0;
-    #pragma unroll
-    for(int k = 0; k < RUN_ITEMS; k++) c += gr_head(xv[k], base + (L)k * 256 + threadIdx.x, n, cap_mask) ? 1u : 0u;
-    __shared__ uint32_t sh[4];
-    for(int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
-    if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
-    __syncthreads();
-    if(threadIdx.x == 0) tcount[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
-}
 // run r (input order): rkey[r] = its key, rval[r] = (first row << cb) | (length - 1).  A head is
 // forced at every multiple of the cap (a divisor of 64), so the next head after a row is in the
 // rest of its wave's 64-row ballot word or at the start of the next word (the next tile's first
 // row, for the tile's last word).
-__global__ void __launch_bounds__(256) k_gr_head_write(const int32_t *x, L n, unsigned cap_mask, int cb, const int64_t *toff,
+__global__ void __launch_bounds__(256) k_gr_head_write(const int32_t *x, L n, unsigned cap_mask, int cb, const int64_t *toff, int tstride,
                                                        uint32_t *rkey, uint32_t *rval){
     __shared__ uint32_t s_cnt[RUN_ITEMS * 4];
     const L base = (L)blockIdx.x * RUN_TILE;
@@ -1392,7 +1421,7 @@ __global__ void __launch_bounds__(256) k_gr_head_write(const int32_t *x, L n, un
         s_cnt[lane] = inc - mine;
     }
     __syncthreads();
-    const int64_t pre = toff[blockIdx.x];
+    const int64_t pre = toff[(L)blockIdx.x * tstride];
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     #pragma unroll
     for(int k = 0; k < RUN_ITEMS; k++){
@@ -1407,118 +1436,180 @@ __global__ void __launch_bounds__(256) k_gr_head_write(const int32_t *x, L n, un
         }
     }
 }
-__global__ void k_gr_len(const uint32_t *sv, L nr, unsigned len_mask, uint32_t *len){
-    for(L j = (L)blockIdx.x * blockDim.x + threadIdx.x; j < nr; j += (L)gridDim.x * blockDim.x) len[j] = (sv[j] & len_mask) + 1;
-}
-// flat[fpos[j] + t] = (first row of sorted run j) + t; a wave writes the contiguous output range
-// of its 64 runs together, each lane finding its run by a search over the 64 run starts in LDS
-__global__ void __launch_bounds__(256) k_gr_expand(const uint32_t *sv, int cb, const int64_t *fpos, L nr, L n, int64_t *flat){
-    __shared__ int64_t sp[4][65];
-    __shared__ uint32_t ss[4][64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const L nw = ((L)gridDim.x * blockDim.x) >> 6;
-    for(L base = (((L)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64; base < nr; base += nw * 64){
-        const L j = base + lane;
-        const bool valid = j < nr;
-        sp[w][lane] = valid ? fpos[j] : INT64_MAX;
-        ss[w][lane] = valid ? sv[j] >> cb : 0u;
-        if(lane == 0) sp[w][64] = base + 64 < nr ? fpos[base + 64] : n;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int64_t P0 = sp[w][0], P1 = sp[w][64];
-        for(int64_t p = P0 + lane; p < P1; p += 64){
-            int lo = 0, hi = 63;                      // last run whose start is <= p
-            while(lo < hi){ const int mid = (lo + hi + 1) >> 1; if(sp[w][mid] <= p) lo = mid; else hi = mid - 1; }
-            flat[p] = (int64_t)ss[w][lo] + (p - sp[w][lo]);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-}
-// heads of the sorted run keys: group key = the first row of the group's first run, start =
-// its flat position
-__global__ void __launch_bounds__(256) k_gr_group_write(const uint32_t *sk, const uint32_t *sv, int cb, const int64_t *fpos, L nr,
-                                                        const int64_t *toff, int64_t *keys, int64_t *start, uint32_t *kval){
-    __shared__ uint32_t s_cnt[RUN_ITEMS * 4];
-    const L base = (L)blockIdx.x * RUN_TILE;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint64_t bal[RUN_ITEMS];
+// Sorted runs (sk: keys, sv: (first row << cb) | (length - 1)) in tiles of GRF_TILE runs, thread
+// t taking runs t, t + 256, ... of its tile.  k_grf_count: the group heads (a run whose key
+// differs from the run before) and the rows of each tile.  k_grf_write: every run's position in
+// the flat list (the tile's row offset, then the runs before it in the tile: a wave scan per
+// step and a scan over the (step, wave) sums), the run expanded there as ascending row numbers,
+// and for each group head the group's first row, flat offset and key value at its rank.
+#define GRF_ITEMS 8
+#define GRF_TILE (256 * GRF_ITEMS)
+__global__ void __launch_bounds__(256) k_grf_count(const uint32_t *sk, const uint32_t *sv, L nr, unsigned len_mask,
+                                                   uint32_t *gcount, uint32_t *rcount){
+    __shared__ uint32_t sh[4], sr[4];
+    const L base = (L)blockIdx.x * GRF_TILE;
+    uint32_t h = 0, rows = 0;
     #pragma unroll
-    for(int k = 0; k < RUN_ITEMS; k++){
+    for(int k = 0; k < GRF_ITEMS; k++){
         const L i = base + (L)k * 256 + threadIdx.x;
-        bal[k] = __ballot(i < nr && (i == 0 || sk[i] != sk[i - 1]));
-        if(lane == 0) s_cnt[k * 4 + w] = (uint32_t)__popcll(bal[k]);
+        h += run_head_w<uint32_t>(sk, nullptr, nullptr, i, nr) ? 1u : 0u;
+        if(i < nr) rows += (sv[i] & len_mask) + 1u;
+    }
+    for(int o = 32; o > 0; o >>= 1){ h += __shfl_down(h, o, 64); rows += __shfl_down(rows, o, 64); }
+    if((threadIdx.x & 63) == 0){ sh[threadIdx.x >> 6] = h; sr[threadIdx.x >> 6] = rows; }
+    __syncthreads();
+    if(threadIdx.x == 0){ gcount[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3]; rcount[blockIdx.x] = sr[0] + sr[1] + sr[2] + sr[3]; }
+}
+// Tiles holding at most GRF_STAGE rows expand through LDS: a bitmap of the run starts over the
+// tile's output range and a prefix count per 64-bit word give each output row its run, so the
+// flat list is written with consecutive threads on consecutive rows; larger tiles (long runs)
+// let each lane write its own run.  A head whose group ends inside the tile writes its count
+// (the next head's position minus its own); the last group of each tile gets its count from
+// k_grf_fix.
+#define GRF_STAGE 16384
+__global__ void __launch_bounds__(256) k_grf_write(const uint32_t *sk, const uint32_t *sv, L nr, int cb, unsigned len_mask,
+                                                   const int64_t *goff, const int64_t *roff, int64_t *flat, int64_t *keys,
+                                                   int64_t *start, int64_t *cnt, uint32_t *kval){
+    __shared__ uint32_t s_h[GRF_ITEMS * 4], s_r[GRF_ITEMS * 4];
+    __shared__ uint64_t s_hb[GRF_ITEMS * 4];              // head flags of the tile's runs, 64 a word
+    __shared__ uint32_t s_fpos[GRF_TILE], s_row[GRF_TILE];  // each run's tile-relative flat position, first row
+    __shared__ unsigned long long s_bm[GRF_STAGE / 64];   // run starts over the tile's output range
+    __shared__ uint32_t s_pre[GRF_STAGE / 64], s_wsum[4];
+    const L base = (L)blockIdx.x * GRF_TILE;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const int64_t gpre = goff[blockIdx.x], rpre = roff[blockIdx.x];
+    const L rows = roff[blockIdx.x + 1] - rpre;
+    const bool staged = rows <= GRF_STAGE;
+    if(staged) for(int q = threadIdx.x; q < GRF_STAGE / 64; q += 256) s_bm[q] = 0ull;
+    uint32_t kv[GRF_ITEMS], vv[GRF_ITEMS], ex[GRF_ITEMS];
+    uint64_t bal[GRF_ITEMS];
+    #pragma unroll
+    for(int k = 0; k < GRF_ITEMS; k++){
+        const L i = base + (L)k * 256 + threadIdx.x;
+        const bool valid = i < nr;
+        kv[k] = valid ? sk[i] : 0u; vv[k] = valid ? sv[i] : 0u;
+        bal[k] = __ballot(run_head_w<uint32_t>(sk, nullptr, nullptr, i, nr));
+        const uint32_t len = valid ? (vv[k] & len_mask) + 1u : 0u;
+        uint32_t inc = len;
+        for(int o = 1; o < 64; o <<= 1){ const uint32_t t = __shfl_up(inc, o, 64); if(lane >= o) inc += t; }
+        ex[k] = inc - len;
+        if(lane == 63) s_r[k * 4 + w] = inc;
+        if(lane == 0){ s_h[k * 4 + w] = (uint32_t)__popcll(bal[k]); s_hb[k * 4 + w] = bal[k]; }
     }
     __syncthreads();
     if(w == 0){
-        const uint32_t mine = s_cnt[lane];
-        uint32_t inc = mine;
-        for(int o = 1; o < 64; o <<= 1){ const uint32_t t = __shfl_up(inc, o, 64); if(lane >= o) inc += t; }
-        s_cnt[lane] = inc - mine;
+        const uint32_t mh = lane < GRF_ITEMS * 4 ? s_h[lane] : 0u, mr = lane < GRF_ITEMS * 4 ? s_r[lane] : 0u;
+        uint32_t ih = mh, ir = mr;
+        for(int o = 1; o < 64; o <<= 1){
+            const uint32_t a = __shfl_up(ih, o, 64), b = __shfl_up(ir, o, 64);
+            if(lane >= o){ ih += a; ir += b; }
+        }
+        if(lane < GRF_ITEMS * 4){ s_h[lane] = ih - mh; s_r[lane] = ir - mr; }
     }
     __syncthreads();
-    const int64_t pre = toff[blockIdx.x];
-    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     #pragma unroll
-    for(int k = 0; k < RUN_ITEMS; k++){
-        if((bal[k] >> lane) & 1ull){
-            const L i = base + (L)k * 256 + threadIdx.x;
-            const int64_t g = pre + s_cnt[k * 4 + w] + __popcll(bal[k] & lt);
-            keys[g] = (int64_t)(sv[i] >> cb); start[g] = fpos[i]; kval[g] = sk[i];
+    for(int k = 0; k < GRF_ITEMS; k++){
+        const L i = base + (L)k * 256 + threadIdx.x;
+        const int idx = k * 256 + threadIdx.x;
+        if(i < nr){
+            const uint32_t fr = s_r[k * 4 + w] + ex[k];
+            s_fpos[idx] = fr; s_row[idx] = vv[k] >> cb;
+            if(staged) atomicOr(&s_bm[fr >> 6], 1ull << (fr & 63));
+        }
+    }
+    __syncthreads();
+    if(staged){
+        // run count before each 64-row word of the output range: thread t owns word t
+        static_assert(GRF_STAGE / 64 == 256, "one bitmap word per thread");
+        const int nwd = (int)((rows + 63) >> 6);
+        const uint32_t pc = (int)threadIdx.x < nwd ? (uint32_t)__popcll(s_bm[threadIdx.x]) : 0u;
+        uint32_t inc = pc;
+        for(int o = 1; o < 64; o <<= 1){ const uint32_t t = __shfl_up(inc, o, 64); if(lane >= o) inc += t; }
+        if(lane == 63) s_wsum[w] = inc;
+        __syncthreads();
+        uint32_t pre = inc - pc;
+        for(int q = 0; q < w; q++) pre += s_wsum[q];
+        s_pre[threadIdx.x] = pre;
+        __syncthreads();
+        for(L p = threadIdx.x; p < rows; p += 256){
+            const int wi = (int)(p >> 6);
+            const unsigned long long m = s_bm[wi] & (~0ull >> (63 - (p & 63)));
+            const int r = (int)s_pre[wi] + __popcll(m) - 1;
+            flat[rpre + p] = (int64_t)s_row[r] + (p - (L)s_fpos[r]);
+        }
+    }
+    #pragma unroll
+    for(int k = 0; k < GRF_ITEMS; k++){
+        const L i = base + (L)k * 256 + threadIdx.x;
+        if(i < nr){
+            const int idx = k * 256 + threadIdx.x;
+            const int64_t fpos = rpre + (int64_t)s_fpos[idx];
+            const int64_t row = (int64_t)(vv[k] >> cb);
+            if(!staged){
+                const uint32_t len = (vv[k] & len_mask) + 1u;
+                for(uint32_t t = 0; t < len; t++) flat[fpos + t] = row + (int64_t)t;
+            }
+            if((bal[k] >> lane) & 1ull){
+                const int64_t g = gpre + (int64_t)s_h[k * 4 + w] + __popcll(bal[k] & lt);
+                keys[g] = row; start[g] = fpos; kval[g] = kv[k];
+                // the next head of the tile, if any: later in this word, else the first later word
+                int nx = -1;
+                const uint64_t above = lane == 63 ? 0ull : (bal[k] & (~0ull << (lane + 1)));
+                if(above) nx = k * 256 + w * 64 + __builtin_ctzll(above);
+                else for(int q = k * 4 + w + 1; q < GRF_ITEMS * 4; q++) if(s_hb[q]){ nx = (q >> 2) * 256 + (q & 3) * 64 + __builtin_ctzll(s_hb[q]); break; }
+                if(nx >= 0) cnt[g] = (int64_t)s_fpos[nx] - (int64_t)s_fpos[idx];
+            }
         }
     }
 }
-static int32_t group_i32_runs(hpa_ctx *c, V z, const int32_t *x, L n, bool *done){
-    *done = false;
+// count of each tile's last group (its end lies in a later tile): start[g + 1] - start[g], with
+// start[G] = n written here
+__global__ void k_grf_fix(const int64_t *goff, L gt, int64_t *start, int64_t *cnt, L n){
+    const L G = goff[gt];
+    for(L t = (L)blockIdx.x * blockDim.x + threadIdx.x; t < gt; t += (L)gridDim.x * blockDim.x){
+        if(goff[t + 1] > goff[t]){
+            const L g = goff[t + 1] - 1;
+            cnt[g] = (g + 1 < G ? start[g + 1] : n) - start[g];
+        }
+    }
+    if(blockIdx.x == 0 && threadIdx.x == 0) start[G] = n;
+}
+static int32_t group_i32_runs(hpa_ctx *c, V z, const int32_t *x, L n, int cb, const GiScan &sc){
     int32_t e;
-    const int sbits = n > 1 ? 64 - __builtin_clzll((unsigned long long)(n - 1)) : 1;
-    const int cb = 32 - sbits < 6 ? 32 - sbits : 6;
-    if(cb < 2) R 0;                                          // runs of at most 2 rows: not worth it
     const unsigned cap_mask = (1u << cb) - 1;
-    const L mark = c->dcur;
-    const L nt = ceil_div(n, RUN_TILE);
-    uint32_t *tcount = (uint32_t*)dalloc(c, 4 * nt);
-    int64_t *toff = (int64_t*)dalloc(c, 8 * (nt + 1));
-    if(!tcount || !toff) R HPA_E_MEMORY;
+    const L nt = ceil_div(n, RUN_TILE), nr = sc.nr;
     ProfScope ps("group_runs", c->stream);
-    hipLaunchKernelGGL(k_gr_head_count, dim3((unsigned)nt), dim3(256), 0, c->stream, x, n, cap_mask, tcount);
-    if((e = scan_u32_to_i64(c, tcount, toff, nt, toff + nt))) R e;
-    int64_t nr = 0; if((e = d2h(c, &nr, toff + nt, 8))) R e;
-    if((double)nr > GR_RUNS_MAX_SHARE * (double)n){ arena_set(c, mark); R 0; }
     uint32_t *rkey = (uint32_t*)dalloc(c, 4 * nr), *rval = (uint32_t*)dalloc(c, 4 * nr);
     if(!rkey || !rval) R HPA_E_MEMORY;
-    hipLaunchKernelGGL(k_gr_head_write, dim3((unsigned)nt), dim3(256), 0, c->stream, x, n, cap_mask, cb, (const int64_t*)toff, rkey, rval);
+    hipLaunchKernelGGL(k_gr_head_write, dim3((unsigned)nt), dim3(256), 0, c->stream, x, n, cap_mask, cb, (const int64_t*)sc.toff,
+                       (int)(RUN_TILE / GI_CHUNK), rkey, rval);
     if((e = check_launch("group_runs"))) R e;
+    // the runs' keys are some of the rows' keys: the rows' varying bits cover theirs, and rows out
+    // of order (a key below the one before is always a new run) leave the runs out of order
+    const RsHint h{sc.s.orv, sc.s.andv, true};
     uint32_t *sk, *sv;
-    if((e = radix_sort_pairs_u32(c, rkey, rval, nr, 0, 32, &sk, &sv))) R e;
-    uint32_t *len = (uint32_t*)dalloc(c, 4 * nr);
-    int64_t *fpos = (int64_t*)dalloc(c, 8 * (nr + 1));
-    int64_t *flat = (int64_t*)dalloc(c, 8 * n);
-    if(!len || !fpos || !flat) R HPA_E_MEMORY;
-    hipLaunchKernelGGL(k_gr_len, dim3(grid_for(nr, 256, 4)), dim3(256), 0, c->stream, (const uint32_t*)sv, (L)nr, cap_mask, len);
-    if((e = scan_u32_to_i64(c, len, fpos, nr, fpos + nr))) R e;
-    hipLaunchKernelGGL(k_gr_expand, dim3(grid_for(nr, 256, 1)), dim3(256), 0, c->stream, (const uint32_t*)sv, cb,
-                       (const int64_t*)fpos, (L)nr, n, flat);
-    // groups: heads of the sorted run keys
-    const L gt = ceil_div(nr, RUN_TILE);
-    uint32_t *gcount = (uint32_t*)dalloc(c, 4 * gt);
-    int64_t *goff = (int64_t*)dalloc(c, 8 * (gt + 1));
-    if(!gcount || !goff) R HPA_E_MEMORY;
-    hipLaunchKernelGGL(k_run_tilecount<uint32_t>, dim3((unsigned)gt), dim3(256), 0, c->stream, (const uint32_t*)sk, (const uint32_t*)sv,
-                       (const uint64_t*)nullptr, (L)nr, gcount);
-    if((e = scan_u32_to_i64(c, gcount, goff, gt, goff + gt))) R e;
-    int64_t G = 0; if((e = d2h(c, &G, goff + gt, 8))) R e;
-    uint32_t *kval = (uint32_t*)dalloc(c, 4 * G);           // before keys: dies with them
-    int64_t *keys = (int64_t*)dalloc(c, 8 * G), *start = (int64_t*)dalloc(c, 8 * (G + 1)), *cnt = (int64_t*)dalloc(c, 8 * G);
-    if(!kval || !keys || !start || !cnt) R HPA_E_MEMORY;
-    hipLaunchKernelGGL(k_gr_group_write, dim3((unsigned)gt), dim3(256), 0, c->stream, (const uint32_t*)sk, (const uint32_t*)sv, cb,
-                       (const int64_t*)fpos, (L)nr, (const int64_t*)goff, keys, start, kval);
-    hipLaunchKernelGGL(k_run_counts, dim3(grid_for(G, 256, 1)), dim3(256), 0, c->stream, start, (L)G, n, cnt);
+    if((e = radix_sort_t<uint32_t>(c, rkey, rval, nr, 0, 32, &sk, &sv, nullptr, false, false, &h))) R e;
+    const L gt = ceil_div(nr, GRF_TILE);
+    uint32_t *gcount = (uint32_t*)dalloc(c, 4 * gt), *rcount = (uint32_t*)dalloc(c, 4 * gt);
+    int64_t *goff = (int64_t*)dalloc(c, 8 * (gt + 1)), *roff = (int64_t*)dalloc(c, 8 * (gt + 1));
+    if(!gcount || !rcount || !goff || !roff) R HPA_E_MEMORY;
+    hipLaunchKernelGGL(k_grf_count, dim3((unsigned)gt), dim3(256), 0, c->stream, (const uint32_t*)sk, (const uint32_t*)sv, (L)nr, cap_mask,
+                       gcount, rcount);
     if((e = check_launch("group_runs"))) R e;
+    if((e = scan_u32_to_i64(c, gcount, goff, gt, goff + gt))) R e;
+    if((e = scan_u32_to_i64(c, rcount, roff, gt, roff + gt))) R e;
+    // outputs for as many groups as runs
+    uint32_t *kval = (uint32_t*)dalloc(c, 4 * nr);           // before keys: dies with them
+    int64_t *keys = (int64_t*)dalloc(c, 8 * nr), *start = (int64_t*)dalloc(c, 8 * (nr + 1)), *cnt = (int64_t*)dalloc(c, 8 * nr);
+    int64_t *flat = (int64_t*)dalloc(c, 8 * n);
+    if(!kval || !keys || !start || !cnt || !flat) R HPA_E_MEMORY;
+    hipLaunchKernelGGL(k_grf_write, dim3((unsigned)gt), dim3(256), 0, c->stream, (const uint32_t*)sk, (const uint32_t*)sv, (L)nr, cb, cap_mask,
+                       (const int64_t*)goff, (const int64_t*)roff, flat, keys, start, cnt, kval);
+    hipLaunchKernelGGL(k_grf_fix, dim3(grid_for(gt, 256, 1)), dim3(256), 0, c->stream, (const int64_t*)goff, gt, start, cnt, n);
+    if((e = check_launch("group_runs"))) R e;
+    int64_t G = 0; if((e = d2h(c, &G, goff + gt, 8))) R e;
     ps.stop(); ps.bytes = 12.0 * n + 8.0 * G;        // int group: keys read, row list and keys written
-    *done = true;
     if(G > 1) note_group_vals(c, keys, x, n, kval);
     note_group_offsets(c, cnt, G, start);
     R make_group_dict(c, z, keys, cnt, flat, G, n);
@@ -1528,19 +1619,36 @@ static int32_t group_int(hpa_ctx *c, V z, V x){      // lib_group_by_normal_int,
     L n = x->len; int32_t e;
     const void *p = nullptr;
     if(n > 0 && (e = dev_view(c, x, &p))) R e;
-    double change = 1.0;
-    if(n > 0){
-        bool done; if((e = small_group_i32(c, z, (const int32_t*)p, n, &done, &change)) || done) R e;
+    if(n <= 0){
+        int64_t *flat = (int64_t*)dalloc(c, 8); if(!flat) R HPA_E_MEMORY;
+        uint32_t *sk = nullptr;
+        if((e = radix_sort_flat_u32(c, (const uint32_t*)p, n, 0, 32, &sk, flat))) R e;
+        R runs_to_group_flat_u32(c, z, sk, flat, n, p);
     }
-    if(n >= GR_RUNS_MIN && change <= GR_RUNS_MAX_CHANGE){
-        bool done; if((e = group_i32_runs(c, z, (const int32_t*)p, n, &done)) || done) R e;
+    const int sbits = n > 1 ? 64 - __builtin_clzll((unsigned long long)(n - 1)) : 1;
+    const int cb = 32 - sbits < 6 ? 32 - sbits : 6;
+    GiScan sc;
+    if((e = gi_scan(c, (const int32_t*)p, n, cb >= 2 ? (1u << cb) - 1 : 63u, &sc))) R e;
+    // keys on fewer than SG_BINS consecutive unsigned values: counting sort
+    if(sc.s.hi - sc.s.lo < (unsigned)SG_BINS){
+        SgSrc s; memset(&s, 0, sizeof(s));
+        s.x = (const int32_t*)p; s.base = sc.s.lo; s.vec = (uintptr_t)p % 16 == 0 ? 1 : 0;
+        bool missed;
+        if((e = small_group_run(c, z, s, n, (int)(sc.s.hi - sc.s.lo) + 1, &missed))) R e;
+        if(!missed) R 0;
+        set_error_detail("group: a key outside the scanned range"); R HPA_E_DEVICE;
     }
+    const double change = n > 1 ? (double)sc.s.ch / (double)(n - 1) : 1.0;
+    if(sc.s.desc && n >= GR_RUNS_MIN && cb >= 2 && change <= GR_RUNS_MAX_CHANGE && (double)sc.nr <= GR_RUNS_MAX_SHARE * (double)n)
+        R group_i32_runs(c, z, (const int32_t*)p, n, cb, sc);
     // the unsigned radix key of an i32 is its bit pattern: the column is sorted in place of keys;
-    // the sort's last pass writes the flat list
-    int64_t *flat = (int64_t*)dalloc(c, 8 * (n > 0 ? n : 1)); if(!flat) R HPA_E_MEMORY;
+    // the sort's last pass writes the flat list; at most one group per key value of the range
+    const RsHint h{sc.s.orv, sc.s.andv, sc.s.desc != 0};
+    int64_t *flat = (int64_t*)dalloc(c, 8 * n); if(!flat) R HPA_E_MEMORY;
     uint32_t *sk = nullptr;
-    if((e = radix_sort_flat_u32(c, (const uint32_t*)p, n, 0, 32, &sk, flat))) R e;
-    R runs_to_group_flat_u32(c, z, sk, flat, n, p);
+    if((e = radix_sort_flat_u32(c, (const uint32_t*)p, n, 0, 32, &sk, flat, &h))) R e;
+    const L span = (L)(sc.s.hi - sc.s.lo) + 1;
+    R runs_to_group_flat_u32(c, z, sk, flat, n, p, span < n ? span : n);
 }
 
 static int32_t any_high_i64(hpa_ctx *c, const int64_t *p, L n, bool *hi){
