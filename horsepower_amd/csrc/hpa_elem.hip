@@ -388,10 +388,33 @@ int32_t gather_dev(hpa_ctx *c, void *dst, const void *src, const int64_t *idx, L
     R gather_checked(c, dst, src, idx, n, INT64_MAX, w, nullptr);
 }
 
+// A few host-held indices (a limit's range, say) travel in the launch arguments: checked on the
+// host, no copy to the device and no flag to read back
+#define GATHER_ARG_MAX 64
+struct GatherArgs { int64_t idx[GATHER_ARG_MAX]; };
+__global__ void k_gather_args(const char *src, GatherArgs a, int n, int w, char *dst){
+    const int t = threadIdx.x;
+    if(t >= n) return;
+    const char *s = src + a.idx[t] * w; char *d = dst + (L)t * w;
+    for(int b = 0; b < w; b++) d[b] = s[b];
+}
+
 extern "C" int32_t pfnIndex(hpa_V z, hpa_V x, hpa_V y){
     hpa_ctx *c = ctx_cur();
     if(!is_int(y->typ)) R HPA_E_DOMAIN;
     L nz = y->len, nx = x->len;
+    if(is_basic(x->typ) && y->typ == HPA_L && nz > 1 && nz <= GATHER_ARG_MAX && nx > 0 && !is_device(y->g) && is_device(x->g)){
+        GatherArgs ga;
+        memcpy(ga.idx, y->g, (size_t)(8 * nz));
+        for(L i = 0; i < nz; i++) if(ga.idx[i] < 0 || ga.idx[i] >= nx) R HPA_E_INDEX;
+        const L w = type_width(x->typ);
+        void *dst = dalloc(c, type_bytes(x->typ, nz)); if(!dst) R HPA_E_MEMORY;
+        hipLaunchKernelGGL(k_gather_args, dim3(1), dim3(GATHER_ARG_MAX), 0, c->stream, (const char*)x->g, ga, (int)nz, (int)w, (char*)dst);
+        int32_t e; if((e = check_launch("gather"))) R e;
+        z->typ = x->typ; z->len = nz; z->g = (char*)dst;
+        if(x->typ == HPA_L){ auto it = c->bounds.find(x->g); if(it != c->bounds.end()) note_bound(c, dst, it->second); }
+        R 0;
+    }
     if(is_basic(x->typ)){
         if(x->typ == HPA_S || x->typ == HPA_X) { /* host pointers / complex: width-8 copy still valid */ }
         L w = type_width(x->typ);

@@ -1886,13 +1886,20 @@ __device__ __forceinline__ uint64_t br_key(const SmallCols &s, int q, L r){
 }
 // k0 = first column's key; or_and[0] |= every key, or_and[1] &= every key
 __global__ void __launch_bounds__(256) k_br_key(SmallCols s, L n, uint64_t *k0, unsigned long long *or_and){
+    __shared__ unsigned long long so[4], sa[4];
     unsigned long long o = 0, a = ~0ull;
     for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x){
         const uint64_t v = br_key(s, 0, i);
         k0[i] = v; o |= v; a &= v;
     }
     for(int d = 32; d > 0; d >>= 1){ o |= __shfl_down(o, d, 64); a &= __shfl_down(a, d, 64); }
-    if((threadIdx.x & 63) == 0){ atomicOr(or_and, o); atomicAnd(or_and + 1, a); }
+    if((threadIdx.x & 63) == 0){ so[threadIdx.x >> 6] = o; sa[threadIdx.x >> 6] = a; }
+    __syncthreads();
+    // one pair of atomics per workgroup: the waves' atomics on two words serialised the kernel
+    if(threadIdx.x == 0){
+        for(int q = 1; q < 4; q++){ o |= so[q]; a &= sa[q]; }
+        atomicOr(or_and, o); atomicAnd(or_and + 1, a);
+    }
 }
 __global__ void k_br_init(uint32_t *p, L n, unsigned long long *or_and){
     for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x) p[i] = 0;
@@ -2391,8 +2398,8 @@ extern "C" int32_t pfnOrderBy(hpa_V z, hpa_V x, hpa_V y){
         for(L q = 0; q < nc; q++) ref_cols |= qsort_ref_type(a[q].typ) || a[q].typ == HPA_C;
         if((e = order_perm(c, x, fl.data(), &perm, &ties, &known))) R e;
         if(ref_cols){
-            bool t2 = true;
-            if(nc <= SMALL_COLS && (e = full_ties(c, (int)nc, a, perm, n, &t2))) R e;
+            bool t2 = known ? ties : true;
+            if(!known && nc <= SMALL_COLS && (e = full_ties(c, (int)nc, a, perm, n, &t2))) R e;
             if(!t2) R order_emit(c, z, perm, n, false);          // the lexicographic order is unique
             std::vector<int32_t> ty((size_t)nc); std::vector<const void*> cp((size_t)nc);
             for(L q = 0; q < nc; q++){ ty[q] = a[q].typ; if((e = dev_view(c, a + q, &cp[q]))) R e; }

@@ -131,7 +131,7 @@ class FirstFit:
 
 class Server:
     def __init__(self, B, tables, slots=4, pool_bytes=64 << 30, merge=True, max_skip_s=0.05,
-                 heavy=("q3",), heavy_slots=1):
+                 heavy=("q3",), heavy_slots=1, heavy_share=None):
         self.B, self.lib, self.T = B, B.lib, tables
         self.slots, self.merge, self.max_skip_s = slots, merge, max_skip_s
         self.base = self.lib.dll.hpaDeviceAlloc(pool_bytes)
@@ -142,6 +142,9 @@ class Server:
         # arrival order: two of them sharing the device would each take about twice as long,
         # while short requests keep filling the other slots
         self.heavy, self.heavy_slots = tuple(heavy), heavy_slots
+        # while a heavy request runs, at most heavy_share short requests run beside it (None: any
+        # number): fewer neighbours take less of the memory bandwidth the heavy request needs
+        self.heavy_share = heavy_share
         self.ctx = [self.lib.dll.hpaCtxCreate(-1) for _ in range(slots)]
         if not all(self.ctx):
             raise HorseError(E_MEMORY, "hpaCtxCreate", self.lib.detail())
@@ -201,7 +204,7 @@ class Server:
         cv = threading.Condition()
         free_slots = list(range(self.slots))
         pending = collections.deque()
-        state = dict(done=0, error=None, heavy=0)
+        state = dict(done=0, error=None, heavy=0, short=0)
         N = len(reqs)
         nxt = 0
         t0 = time.perf_counter()
@@ -224,6 +227,8 @@ class Server:
                 free_slots.append(slot)
                 if job[0].kind in self.heavy:
                     state["heavy"] -= 1
+                else:
+                    state["short"] -= 1
                 if err is None:
                     state["done"] += len(job)
                 elif err.code == E_MEMORY:
@@ -250,6 +255,9 @@ class Server:
                     for i, r in enumerate(pending):
                         if r.kind in self.heavy and state["heavy"] >= self.heavy_slots:
                             continue
+                        if (r.kind not in self.heavy and self.heavy_share is not None and state["heavy"] > 0
+                                and state["short"] >= self.heavy_share):
+                            continue
                         if head is not None and now - head.arrival > self.max_skip_s:
                             break                            # it waits for memory: no overtaking
                         got = pool.alloc(r.est)
@@ -275,6 +283,8 @@ class Server:
                         q.admit = now
                     if r.kind in self.heavy:
                         state["heavy"] += 1
+                    else:
+                        state["short"] += 1
                     self.peak_in_use = max(self.peak_in_use, pool.in_use())
                     slot = free_slots.pop()
                     th = threading.Thread(target=worker, args=(slot, job, off, size), daemon=True)
