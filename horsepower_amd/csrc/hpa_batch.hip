@@ -59,6 +59,8 @@ struct BQuery {
     int64_t *poff;                                            // exclusive scan of pcnt [nparts * ntiles + 1]
     uint16_t *skey; double *sval;                             // partition-contiguous rows (capacity n)
     int64_t pitem0;
+    unsigned long long *pcsum;                                // per PS_CHUNK-entry chunk of pcnt: its sum
+    int64_t pch0;                                             // first scan chunk of the query
 };
 
 struct BPlan {
@@ -69,6 +71,7 @@ struct BPlan {
     int32_t *gkey; double *gsum; int64_t *gcnt;               // groups of every query, query after query
     int tile_rows, lds_keys, part_keys, part_shift;
     const int *pq; int npq;                                   // partitioned queries
+    int64_t total_pchunks;                                    // scan chunks of all partitioned queries
 };
 
 enum { SEL_TILE, SEL_EMIT, SEL_PART };
@@ -245,31 +248,52 @@ __global__ void k_batch_agg(BPlan p){
 // 16384 entries (sixteen consecutive per thread: a wave scan of the thread sums, then the 16
 // wave totals), carrying the running total between rounds.
 #define PS_PER 16
+// The (partition, tile) count matrices of the partitioned queries are scanned in chunks of
+// PS_CHUNK entries, one workgroup a chunk over every query at once: k_batch_psum sums each
+// chunk, k_batch_pscan scans its chunk from the sum of the query's earlier chunks (a few values),
+// so the largest query's matrix no longer runs through one workgroup.
+#define PS_CHUNK (1024 * PS_PER)
+// the partitioned query (index into p.q) owning scan chunk t
+__device__ __forceinline__ int find_pchunk(const BPlan &p, int64_t t){
+    int lo = 0, hi = p.npq - 1;
+    while(lo < hi){
+        const int mid = (lo + hi + 1) >> 1;
+        if(p.q[p.pq[mid]].pch0 <= t) lo = mid; else hi = mid - 1;
+    }
+    return p.pq[lo];
+}
+__global__ void __launch_bounds__(1024) k_batch_psum(BPlan p){
+    __shared__ unsigned long long s_w[16];
+    const BQuery &q = p.q[find_pchunk(p, blockIdx.x)];
+    const int64_t ch = (int64_t)blockIdx.x - q.pch0, N = (int64_t)q.nparts * q.ntiles, r0 = ch * PS_CHUNK;
+    const int t = threadIdx.x;
+    unsigned long long s = 0;
+    #pragma unroll
+    for(int k = 0; k < PS_PER; k++){ const int64_t i = r0 + (int64_t)k * 1024 + t; s += i < N ? q.pcnt[i] : 0u; }
+    for(int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if((t & 63) == 0) s_w[t >> 6] = s;
+    __syncthreads();
+    if(t == 0){ unsigned long long a = 0; for(int k = 0; k < 16; k++) a += s_w[k]; q.pcsum[ch] = a; }
+}
 __global__ void __launch_bounds__(1024) k_batch_pscan(BPlan p){
     __shared__ unsigned long long s_w[16];
     __shared__ unsigned long long s_carry;
-    const BQuery &q = p.q[p.pq[blockIdx.x]];
+    const BQuery &q = p.q[find_pchunk(p, blockIdx.x)];
+    const int64_t ch = (int64_t)blockIdx.x - q.pch0, N = (int64_t)q.nparts * q.ntiles, r0 = ch * PS_CHUNK;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int64_t N = (int64_t)q.nparts * q.ntiles;
-    if(t == 0) s_carry = 0;
+    if(t == 0){ unsigned long long a = 0; for(int64_t k = 0; k < ch; k++) a += q.pcsum[k]; s_carry = a; }
+    unsigned v[PS_PER]; unsigned long long s = 0;
+    #pragma unroll
+    for(int k = 0; k < PS_PER; k++){ const int64_t i = r0 + (int64_t)t * PS_PER + k; v[k] = i < N ? q.pcnt[i] : 0u; s += v[k]; }
+    unsigned long long inc = s;
+    for(int o = 1; o < 64; o <<= 1){ const unsigned long long y = __shfl_up(inc, o, 64); if(lane >= o) inc += y; }
+    if(lane == 63) s_w[w] = inc;
     __syncthreads();
-    for(int64_t r0 = 0; r0 < N; r0 += 1024 * PS_PER){
-        unsigned v[PS_PER]; unsigned long long s = 0;
-        #pragma unroll
-        for(int k = 0; k < PS_PER; k++){ const int64_t i = r0 + (int64_t)t * PS_PER + k; v[k] = i < N ? q.pcnt[i] : 0u; s += v[k]; }
-        unsigned long long inc = s;
-        for(int o = 1; o < 64; o <<= 1){ const unsigned long long y = __shfl_up(inc, o, 64); if(lane >= o) inc += y; }
-        if(lane == 63) s_w[w] = inc;
-        __syncthreads();
-        unsigned long long pre = s_carry + inc - s, tot = 0;
-        for(int k = 0; k < 16; k++){ if(k < w) pre += s_w[k]; tot += s_w[k]; }
-        #pragma unroll
-        for(int k = 0; k < PS_PER; k++){ const int64_t i = r0 + (int64_t)t * PS_PER + k; if(i < N) q.poff[i] = (int64_t)pre; pre += v[k]; }
-        __syncthreads();
-        if(t == 0) s_carry += tot;
-        __syncthreads();
-    }
-    if(t == 0) q.poff[N] = (int64_t)s_carry;
+    unsigned long long pre = s_carry + inc - s, tot = 0;
+    for(int k = 0; k < 16; k++){ if(k < w) pre += s_w[k]; tot += s_w[k]; }
+    #pragma unroll
+    for(int k = 0; k < PS_PER; k++){ const int64_t i = r0 + (int64_t)t * PS_PER + k; if(i < N) q.poff[i] = (int64_t)pre; pre += v[k]; }
+    if(t == 0 && r0 + PS_CHUNK >= N) q.poff[N] = (int64_t)(s_carry + tot);
 }
 
 // Scatter of a partitioned query's row tile, staged through LDS so that the writes go out as
@@ -467,7 +491,7 @@ struct hpa_batch {
     int32_t *gkey = nullptr; double *gsum = nullptr; int64_t *gcnt = nullptr;
     int64_t slots = 0;             // sum of the key ranges: capacity of the output
     int *dpq = nullptr; int npq = 0;
-    int64_t total_tiles = 0, total_etiles = 0, total_pitems = 0;
+    int64_t total_tiles = 0, total_etiles = 0, total_pitems = 0, total_pchunks = 0;
     int block = 256, tile_rows = 8192, lds_keys = 2730, part_keys = 2048, grid = 0, max_parts = 0;
     hipGraph_t graph = nullptr; hipGraphExec_t exec = nullptr; bool captured = false;
     std::vector<int64_t> ng; bool ng_valid = false;
@@ -491,7 +515,7 @@ static BPlan plan_of(hpa_batch *b){
     p.estatus = b->estatus; p.gkey = b->gkey; p.gsum = b->gsum; p.gcnt = b->gcnt;
     p.part_keys = b->part_keys; p.part_shift = 0;
     while(b->part_keys > 0 && (1 << p.part_shift) < b->part_keys) p.part_shift++;
-    p.pq = b->dpq; p.npq = b->npq;
+    p.pq = b->dpq; p.npq = b->npq; p.total_pchunks = b->total_pchunks;
     return p;
 }
 
@@ -513,7 +537,7 @@ static int32_t batch_layout(hpa_batch *b, int block, int tile_rows, int lds_keys
         set_error_detail("batch tile %d: partitioning needs tiles of at least %d rows", tile_rows, PART_MIN_TILE); R HPA_E_DOMAIN;
     }
     b->block = block; b->tile_rows = tile_rows; b->lds_keys = lds_keys; b->part_keys = part_keys;
-    int64_t t = 0, pi = 0;
+    int64_t t = 0, pi = 0, pc = 0;
     std::vector<int> pq;
     b->max_parts = 0;
     for(int qi = 0; qi < b->nq; qi++){
@@ -527,9 +551,13 @@ static int32_t batch_layout(hpa_batch *b, int block, int tile_rows, int lds_keys
         int64_t sp = q.part ? ceil_div(ceil_div(q.n, P), 65536) : 1;
         q.psplit = (int32_t)(sp < 1 ? 1 : sp > 64 ? 64 : sp);
         q.pitem0 = pi; pi += (int64_t)q.nparts * q.psplit;
-        if(q.part){ pq.push_back(qi); b->max_parts = q.nparts > b->max_parts ? q.nparts : b->max_parts; }
+        q.pch0 = pc;
+        if(q.part){
+            pq.push_back(qi); b->max_parts = q.nparts > b->max_parts ? q.nparts : b->max_parts;
+            pc += ceil_div((int64_t)q.nparts * q.ntiles, PS_CHUNK);
+        }
     }
-    b->total_tiles = t; b->total_pitems = pi;
+    b->total_tiles = t; b->total_pitems = pi; b->total_pchunks = pc;
     b->npq = (int)pq.size();
     b->grid = num_cus() * (2048 / block);
     drop_graph(b);
@@ -548,7 +576,8 @@ static int32_t batch_enqueue(hpa_batch *b, hipStream_t s){
     if((size_t)b->max_parts * 4 > agg_lds) agg_lds = (size_t)b->max_parts * 4;
     hipLaunchKernelGGL(k_batch_agg, dim3(b->grid), dim3(b->block), agg_lds, s, p);
     if(b->npq){
-        hipLaunchKernelGGL(k_batch_pscan, dim3((unsigned)b->npq), dim3(1024), 0, s, p);
+        hipLaunchKernelGGL(k_batch_psum, dim3((unsigned)b->total_pchunks), dim3(1024), 0, s, p);
+        hipLaunchKernelGGL(k_batch_pscan, dim3((unsigned)b->total_pchunks), dim3(1024), 0, s, p);
         hipLaunchKernelGGL(k_batch_scatter, dim3(num_cus()), dim3(SC_BLOCK), 0, s, p);
         hipLaunchKernelGGL(k_batch_pagg, dim3(num_cus() * 4), dim3(256), lds_bytes(b->part_keys), s, p);
     }
@@ -625,7 +654,8 @@ extern "C" hpa_batch *hpaBatchCreate(int32_t nq, hpa_V *keys, hpa_V *vals, hpa_V
             const int64_t M = ceil_div(range, PART_MIN_KEYS) * ceil_div(q.n, PART_MIN_TILE);
             q.pcnt = (unsigned*)dalloc(c, 4 * M); q.poff = (int64_t*)dalloc(c, 8 * (M + 1));
             q.skey = (uint16_t*)dalloc(c, 2 * q.n); q.sval = (double*)dalloc(c, 8 * q.n);
-            if(!q.pcnt || !q.poff || !q.skey || !q.sval) R fail("arena full", qi);
+            q.pcsum = (unsigned long long*)dalloc(c, 8 * ceil_div(M, PS_CHUNK));
+            if(!q.pcnt || !q.poff || !q.skey || !q.sval || !q.pcsum) R fail("arena full", qi);
         }
     }
     b->total_etiles = et;
