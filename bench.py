@@ -75,6 +75,7 @@ def parse():
     p.add_argument("--slots", type=int, default=4)
     p.add_argument("--heavy-share", type=int, default=-1)
     p.add_argument("--heavy-slots", type=int, default=1)
+    p.add_argument("--heavy-fifo", type=int, default=0)
     return p.parse_args()
 
 
@@ -550,7 +551,8 @@ def bench_serve(a, B=None):
     nf = fact["fk"].contents.len
     print(f"[bench] serve tables: {n_dim} orders, {nf} lineitems in HBM ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     srv = S.Server(B, T, slots=a.slots, pool_bytes=128 << 30, merge=True,
-                   heavy_share=None if a.heavy_share < 0 else a.heavy_share, heavy_slots=a.heavy_slots)
+                   heavy_share=None if a.heavy_share < 0 else a.heavy_share, heavy_slots=a.heavy_slots,
+                   heavy_fifo=bool(a.heavy_fifo))
     est = srv.calibrate()
     print(f"[bench] calibrated arena estimates (MiB): { {k: v >> 20 for k, v in est.items()} }", file=sys.stderr, flush=True)
     # service time of the per-query fused path sets the offered load

@@ -166,6 +166,7 @@ class GpuBackend(Backend):
         super().__init__(lib or load())
         self.dev = self.lib
         self._aggs = {}
+        self._fga_args = {}
 
     def node(self):
         return self.lib.node()
@@ -303,21 +304,29 @@ class GpuBackend(Backend):
         groups are read from the result nodes instead."""
         from .vnode import HorseError, aggs
         nk, na = len(keys), len(spec)
-        zk = [self.node() for _ in range(nk)]
-        za = [self.node() for _ in range(na)]
-        hit = self._aggs.get(id(spec))
-        if hit is None or hit[0] is not spec:
-            hit = (spec, aggs(spec))
-            self._aggs[id(spec)] = hit
-        buf = np.empty(cap * (nk + 8 * na) + 16, dtype=np.uint8)
-        base = buf.__array_interface__["data"][0]
-        offs = [k * cap for k in range(nk)] + [((nk * cap + 7) & ~7) + 8 * cap * a for a in range(na)]
-        ng = C.c_int64(0)
-        rc = self.lib.dll.hpaFusedGroupAggHost(nk, (V * nk)(*keys), len(xs), (V * len(xs))(*xs),
-                                               C.byref(cond) if cond is not None else None, na,
-                                               C.cast(hit[1], C.c_void_p), None, (V * nk)(*zk), (V * na)(*za),
-                                               C.c_int64(cap), (C.c_void_p * nk)(*[base + o for o in offs[:nk]]), None,
-                                               (C.c_void_p * na)(*[base + o for o in offs[nk:]]), C.byref(ng))
+        # the call's argument arrays are built once per (columns, spec, cap) and reused: the
+        # per-call foreign-function work is one node allocation and the call itself
+        ck = (tuple(id(k) for k in keys), tuple(id(x) for x in xs), id(spec), id(cond), cap)
+        hit = self._fga_args.get(ck)
+        if hit is None or hit[0] is not spec or hit[1] is not cond:
+            buf_bytes = cap * (nk + 8 * na) + 16
+            offs = [k * cap for k in range(nk)] + [((nk * cap + 7) & ~7) + 8 * cap * a for a in range(na)]
+            hit = (spec, cond, list(keys), list(xs), aggs(spec), (V * nk)(*keys), (V * len(xs))(*xs),
+                   (V * (nk + na))(), buf_bytes, offs, C.c_int64(0), C.c_int64(cap),
+                   C.byref(cond) if cond is not None else None)
+            if len(self._fga_args) >= 256:
+                self._fga_args.clear()
+            self._fga_args[ck] = hit
+        (_, _, _, _, agg_arr, karr, xarr, zarr, buf_bytes, offs, ng, capv, condp) = hit
+        self.lib.rc("hpaAllocNodes", nk + na, zarr)
+        buf = np.empty(buf_bytes, dtype=np.uint8)
+        base = buf.ctypes.data
+        dst = (C.c_void_p * (nk + na))(*[base + o for o in offs])
+        rc = self.lib.dll.hpaFusedGroupAggHost(nk, karr, len(xs), xarr, condp, na, C.cast(agg_arr, C.c_void_p), None,
+                                               zarr, C.cast(C.addressof(zarr) + nk * C.sizeof(V), C.POINTER(V)),
+                                               capv, dst, None, C.cast(C.addressof(dst) + nk * C.sizeof(C.c_void_p), C.POINTER(C.c_void_p)),
+                                               C.byref(ng))
+        zk, za = list(zarr[:nk]), list(zarr[nk:])
         if rc == 99:
             return None
         if rc != 0:

@@ -919,7 +919,9 @@ struct FgaHost { L cap; void **keys; int64_t *first; void **aggs; int64_t *ng; }
 // hpaFusedGroupAgg: see the public header.  Returns HPA_E_NOT_IMPL (nothing written) when a
 // block meets more distinct key codes than it has slots; the caller then runs the unfused plan.
 // prepared inputs of one distinct fused group-aggregate request (see hpaFusedGroupAgg)
-struct FgaPrep { CExpr ce; Konst kc; std::vector<Acc> acc; std::vector<int> used; FgaOut o; };
+// fn[i]: the fga, fga_merge and fga_merge2 kernels with 4 << i slots, once compiled
+#define FGA_NSV 3
+struct FgaPrep { CExpr ce; Konst kc; std::vector<Acc> acc; std::vector<int> used; FgaOut o; hipFunction_t fn[FGA_NSV][3] = {}; };
 static std::map<std::string, FgaPrep> g_fga_prep;
 static int32_t fga_run(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_expr *cond,
                        int32_t na, const hpa_agg *aggs, hpa_V zfirst, hpa_V *zkeys, hpa_V *zaggs, const FgaHost *hd){
@@ -949,11 +951,12 @@ static int32_t fga_run(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_
     raw.append((const char*)&na, 4);
     for(int a = 0; a < na; a++){ raw.append((const char*)&aggs[a].kind, 4); add_expr(aggs[a].kind == 4 ? nullptr : aggs[a].body); }
     FgaPrep prep;
+    FgaPrep *stored = nullptr;                 // the map entry (std::map entries do not move)
     bool have = false;
     {
         std::lock_guard<std::mutex> lk(g_jit_mu);
         auto it = g_fga_prep.find(raw);
-        if(it != g_fga_prep.end()){ prep = it->second; have = true; }
+        if(it != g_fga_prep.end()){ prep = it->second; stored = &it->second; have = true; }
     }
     if(!have){
         std::vector<int> uc(nx, 0), ub(nx, 0);
@@ -980,7 +983,7 @@ static int32_t fga_run(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_
         prep.used.assign(nx, 0);
         for(int k = 0; k < nx; k++) prep.used[k] = uc[k] | ub[k];
         std::lock_guard<std::mutex> lk(g_jit_mu);
-        g_fga_prep[raw] = prep;
+        stored = &(g_fga_prep[raw] = prep);
     }
     const CExpr &ce = prep.ce;
     const Konst &kc = prep.kc;
@@ -1036,14 +1039,19 @@ static int32_t fga_run(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_
     // slot counts tried in turn: fewer slots keep the accumulators in fewer registers; a block
     // that meets more distinct codes than slots reports overflow and the next size reruns
     int hv[2] = {0, 0};
-    for(int ns = 4; ns <= FGA_NS; ns *= 2){
-        std::string key = "fga|" + std::to_string(nk) + "|" + std::to_string(ns) + "|" + std::to_string(nx) + "|" + types_key(nx, typ) + "|" + ce.s;
-        for(int k = 0; k < nx; k++) key += used[k] ? '1' : '0';
-        for(auto &ac : acc) key += "|" + ac.body + "#" + std::to_string(ac.cls) + (ac.f ? "f" : "i");
-        auto gen = [&](){ return fga_source(nk, ns, nx, typ, ce, used, acc); };
-        hipFunction_t f1, f2, f3;
-        if((e = jit_get_keyed(key, "fga", gen, &f1)) || (e = jit_get_keyed(key, "fga_merge", gen, &f2)) ||
-           (e = jit_get_keyed(key, "fga_merge2", gen, &f3))) R e;
+    static_assert((4 << (FGA_NSV - 1)) >= FGA_NS, "a kernel cache entry for every slot count");
+    for(int ns = 4, nsi = 0; ns <= FGA_NS; ns *= 2, nsi++){
+        hipFunction_t f1 = prep.fn[nsi][0], f2 = prep.fn[nsi][1], f3 = prep.fn[nsi][2];
+        if(!f1 || !f2 || !f3){
+            std::string key = "fga|" + std::to_string(nk) + "|" + std::to_string(ns) + "|" + std::to_string(nx) + "|" + types_key(nx, typ) + "|" + ce.s;
+            for(int k = 0; k < nx; k++) key += used[k] ? '1' : '0';
+            for(auto &ac : acc) key += "|" + ac.body + "#" + std::to_string(ac.cls) + (ac.f ? "f" : "i");
+            auto gen = [&](){ return fga_source(nk, ns, nx, typ, ce, used, acc); };
+            if((e = jit_get_keyed(key, "fga", gen, &f1)) || (e = jit_get_keyed(key, "fga_merge", gen, &f2)) ||
+               (e = jit_get_keyed(key, "fga_merge2", gen, &f3))) R e;
+            std::lock_guard<std::mutex> lk(g_jit_mu);
+            stored->fn[nsi][0] = f1; stored->fn[nsi][1] = f2; stored->fn[nsi][2] = f3;
+        }
         {
             ProfScope ps("fused_group_agg", c->stream, (double)n * cb);
             if((e = jit_launch(c, f1, (unsigned)G, 256, ga))) R e;

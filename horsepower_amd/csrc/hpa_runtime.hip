@@ -218,6 +218,11 @@ void *halloc(hpa_ctx *c, L bytes){
 V node_alloc(hpa_ctx *c){ R (V)halloc(c, sizeof(V0)); }
 
 extern "C" hpa_V hpaAllocNode(void){ R node_alloc(ctx_cur()); }
+extern "C" int32_t hpaAllocNodes(int32_t n, hpa_V *out){
+    hpa_ctx *c = ctx_cur();
+    for(int32_t i = 0; i < n; i++){ out[i] = node_alloc(c); if(!out[i]) R HPA_E_MEMORY; }
+    R 0;
+}
 
 extern "C" int64_t hpaGetHeapOffset(void){
     hpa_ctx *c = ctx_cur();

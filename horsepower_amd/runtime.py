@@ -47,6 +47,7 @@ class Device(PrimitiveLib):
         _bind(dll, "hpaErrorString", C.c_char_p, [i32])
         _bind(dll, "hpaLastErrorDetail", C.c_char_p, [])
         _bind(dll, "hpaAllocNode", V, [])
+        _bind(dll, "hpaAllocNodes", i32, [i32, C.POINTER(V)])
         _bind(dll, "hpaGetHeapOffset", i64, [])
         _bind(dll, "hpaSetHeapOffset", i64, [i64])
         _bind(dll, "hpaInitV", i32, [V, i32, i64])

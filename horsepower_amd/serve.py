@@ -131,7 +131,7 @@ class FirstFit:
 
 class Server:
     def __init__(self, B, tables, slots=4, pool_bytes=64 << 30, merge=True, max_skip_s=0.05,
-                 heavy=("q3",), heavy_slots=1, heavy_share=None):
+                 heavy=("q3",), heavy_slots=1, heavy_share=None, heavy_fifo=False):
         self.B, self.lib, self.T = B, B.lib, tables
         self.slots, self.merge, self.max_skip_s = slots, merge, max_skip_s
         self.base = self.lib.dll.hpaDeviceAlloc(pool_bytes)
@@ -145,6 +145,9 @@ class Server:
         # while a heavy request runs, at most heavy_share short requests run beside it (None: any
         # number): fewer neighbours take less of the memory bandwidth the heavy request needs
         self.heavy_share = heavy_share
+        # heavy_fifo: a short request does not run beside a heavy one that arrived before it (as
+        # in arrival order, it waits for that one to finish); shorts that arrived earlier still do
+        self.heavy_fifo = heavy_fifo
         self.ctx = [self.lib.dll.hpaCtxCreate(-1) for _ in range(slots)]
         if not all(self.ctx):
             raise HorseError(E_MEMORY, "hpaCtxCreate", self.lib.detail())
@@ -205,6 +208,7 @@ class Server:
         free_slots = list(range(self.slots))
         pending = collections.deque()
         state = dict(done=0, error=None, heavy=0, short=0)
+        heavy_running = []                                   # arrival times of running heavy requests
         N = len(reqs)
         nxt = 0
         t0 = time.perf_counter()
@@ -227,6 +231,7 @@ class Server:
                 free_slots.append(slot)
                 if job[0].kind in self.heavy:
                     state["heavy"] -= 1
+                    heavy_running.remove(job[0].arrival)
                 else:
                     state["short"] -= 1
                 if err is None:
@@ -258,6 +263,8 @@ class Server:
                         if (r.kind not in self.heavy and self.heavy_share is not None and state["heavy"] > 0
                                 and state["short"] >= self.heavy_share):
                             continue
+                        if r.kind not in self.heavy and self.heavy_fifo and heavy_running and min(heavy_running) < r.arrival:
+                            continue
                         if head is not None and now - head.arrival > self.max_skip_s:
                             break                            # it waits for memory: no overtaking
                         got = pool.alloc(r.est)
@@ -276,6 +283,8 @@ class Server:
                         for q in list(pending):
                             if len(job) == 8:
                                 break
+                            if self.heavy_fifo and heavy_running and min(heavy_running) < q.arrival:
+                                continue
                             if q.kind == "q6":
                                 pending.remove(q)
                                 job.append(q)
@@ -283,6 +292,7 @@ class Server:
                         q.admit = now
                     if r.kind in self.heavy:
                         state["heavy"] += 1
+                        heavy_running.append(r.arrival)
                     else:
                         state["short"] += 1
                     self.peak_in_use = max(self.peak_in_use, pool.in_use())

@@ -117,6 +117,9 @@ const char *hpaLastErrorDetail(void);
 
 /* allocNode (backend/h_memory.cc:155-157): a zeroed host V0 from the current context. */
 hpa_V    hpaAllocNode(void);
+/* n allocNode calls in one: out[0..n) receive zeroed host V0s (HPA_E_MEMORY when the host arena
+ * is full).  No counterpart in the reference; it saves a foreign-function caller n crossings. */
+int32_t  hpaAllocNodes(int32_t n, hpa_V *out);
 /* getHeapOffset / setHeapOffset (backend/h_memory.cc:147-153): arena watermark of the current
  * context; setting it releases every payload and node allocated after the mark. */
 int64_t  hpaGetHeapOffset(void);
