@@ -53,6 +53,11 @@ struct hpa_ctx {
     // (allocated before the counts, so a rewind that frees them frees the counts too)
     struct GroupOffs { L g; const int64_t *offs; };
     std::map<const void*, GroupOffs> goffs;
+    // group counts vectors made by the run-compressed i32 group, with its sorted runs ((first row
+    // << cb) | (length - 1)) and each group's first run (g + 1 entries): grouped reductions over
+    // it read each run's rows in place of the flat list (allocated before the counts)
+    struct GroupRuns { L g, n, nr; const uint32_t *sv; const uint32_t *grun; int cb; };
+    std::map<const void*, GroupRuns> gruns;
 };
 // set the arena top of c (rewind or truncate), dropping side-table entries above it
 void arena_set(hpa_ctx *c, L cur);
@@ -61,6 +66,8 @@ bool within_bound(hpa_ctx *c, const void *p, L limit);
 void note_group_vals(hpa_ctx *c, const void *keys, const void *src, L n, const void *vals);
 const void *group_vals(hpa_ctx *c, const void *keys, const void *src, L n);
 void note_group_offsets(hpa_ctx *c, const void *counts, L g, const int64_t *offs);
+void note_group_runs(hpa_ctx *c, const void *counts, const hpa_ctx::GroupRuns &r);
+const hpa_ctx::GroupRuns *group_runs(hpa_ctx *c, const void *counts, L g);
 int32_t resident_alloc(V x, int32_t typ, L len);   // long-lived HBM payload (hpaFreeResident)
 
 hpa_ctx *ctx_cur();

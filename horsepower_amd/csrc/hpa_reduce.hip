@@ -401,6 +401,49 @@ __global__ void __launch_bounds__(256) k_seg_thread(const T *x, const int64_t *i
     }
 }
 
+// The same fold over a group made of sorted runs (the run-compressed i32 group): thread s folds
+// group s's runs in order, each run's rows read in place (first row, length), so the flat list is
+// not read; the rows and their order are the flat list's, so the results equal k_seg_thread's.
+template<typename T, bool FLT>
+__global__ void __launch_bounds__(256) k_seg_runs(const T *x, const uint32_t *sv, const uint32_t *grun, int cb, L g, int kind, void *out){
+    const unsigned lmask = (1u << cb) - 1u;
+    for(L s = (L)blockIdx.x * blockDim.x + threadIdx.x; s < g; s += (L)gridDim.x * blockDim.x){
+        const uint32_t r0 = grun[s], r1 = grun[s + 1];
+        KS facc = {0, 0};
+        unsigned long long iacc = 0;
+        T r = T();
+        bool first = true;
+        L cnt = 0;
+        for(uint32_t q = r0; q < r1; q++){
+            const uint32_t v = sv[q];
+            const L row = (L)(v >> cb);
+            const uint32_t len = (v & lmask) + 1u;
+            cnt += len;
+            for(uint32_t t = 0; t < len; t += 4){
+                T w[4];
+                #pragma unroll
+                for(int u = 0; u < 4; u++) w[u] = t + u < len ? x[row + t + u] : T();
+                #pragma unroll
+                for(int u = 0; u < 4; u++){
+                    if(t + u < len){
+                        if(kind == GK_SUM || kind == GK_AVG){
+                            if(FLT) ks_add(facc, (double)w[u]); else iacc += (unsigned long long)(int64_t)w[u];
+                        } else {
+                            if(first){ r = w[u]; first = false; }
+                            else r = kind == GK_MAX ? (r > w[u] ? r : w[u]) : (r < w[u] ? r : w[u]);
+                        }
+                    }
+                }
+            }
+        }
+        if(kind == GK_SUM || kind == GK_AVG){
+            if(FLT){ double q = facc.s + facc.c; ((double*)out)[s] = kind == GK_AVG ? q / (double)cnt : q; }
+            else if(kind == GK_AVG) ((double*)out)[s] = (int64_t)iacc * 1.0 / (double)cnt;
+            else ((int64_t*)out)[s] = (int64_t)iacc;
+        } else ((T*)out)[s] = r;
+    }
+}
+
 template<typename T, bool FLT>
 __global__ void __launch_bounds__(256) k_seg_wave(const T *x, const int64_t *idx, const int64_t *off, L g,
                                                   int kind, void *out, L nx, int *bad){
@@ -506,10 +549,20 @@ __global__ void k_seg_maxlen(const int64_t *off, L g, unsigned long long *mx){
 }
 
 template<typename T, bool FLT>
-static int32_t seg_reduce_t(hpa_ctx *c, const T *x, const int64_t *idx, const int64_t *off, L g, L n, int kind, void *out, L nx, int *bad){
+static int32_t seg_reduce_t(hpa_ctx *c, const T *x, const int64_t *idx, const int64_t *off, L g, L n, int kind, void *out, L nx, int *bad,
+                            const hpa_ctx::GroupRuns *runs = nullptr){
     if(g <= 0) R 0;
     double bytes = (double)n * ((idx ? 8 : 0) + sizeof(T)) + 8.0 * g + 8.0 * (g + 1);
     double avg = (double)n / (double)g;
+    if(runs && avg <= 16.0){
+        ProfScope ps("group_reduce", c->stream, (double)n * sizeof(T) + 4.0 * runs->nr + 4.0 * (g + 1) + 8.0 * g);
+        // one group a thread, no grid-stride loop: each thread's loads depend on its run's first
+        // row, so many threads in flight, not a long per-thread chain, cover the latency
+        const L blocks = ceil_div(g, 256);
+        hipLaunchKernelGGL((k_seg_runs<T,FLT>), dim3((unsigned)(blocks < (1LL << 31) ? blocks : (1LL << 31) - 1)), dim3(256), 0, c->stream,
+                           x, runs->sv, runs->grun, runs->cb, g, kind, out);
+        R check_launch("group_reduce");
+    }
     if(avg <= 16.0){
         ProfScope ps("group_reduce", c->stream, bytes);
         hipLaunchKernelGGL((k_seg_thread<T,FLT>), dim3(grid_for(g, 256, 1)), dim3(256), 0, c->stream, x, idx, off, g, kind, out, nx, bad);
@@ -550,7 +603,7 @@ static int32_t seg_out_type(int32_t t, int kind){
 
 // Grouped reduction into a fresh device vector of g results.
 static int32_t seg_reduce(hpa_ctx *c, V x, const void *px, const int64_t *idx, const int64_t *off, L g, L n, int kind,
-                          void **out, int32_t *tout){
+                          void **out, int32_t *tout, const hpa_ctx::GroupRuns *runs = nullptr){
     int32_t t = x->typ;
     *tout = seg_out_type(t, kind);
     if(kind == GK_LEN) R HPA_E_DOMAIN;
@@ -565,13 +618,13 @@ static int32_t seg_reduce(hpa_ctx *c, V x, const void *px, const int64_t *idx, c
     if(idx && !within_bound(c, idx, x->len)){ bad = (int*)dalloc(c, 16); if(!bad) R HPA_E_MEMORY; hipMemsetAsync(bad, 0, 4, c->stream); }
     L nx = x->len;
     switch(t){
-        case HPA_B: e = seg_reduce_t<uint8_t,false>(c, (const uint8_t*)px, idx, off, g, n, kind, dst, nx, bad); break;
-        case HPA_J: e = seg_reduce_t<int8_t,false>(c, (const int8_t*)px, idx, off, g, n, kind, dst, nx, bad); break;
-        case HPA_H: e = seg_reduce_t<int16_t,false>(c, (const int16_t*)px, idx, off, g, n, kind, dst, nx, bad); break;
-        case HPA_I: e = seg_reduce_t<int32_t,false>(c, (const int32_t*)px, idx, off, g, n, kind, dst, nx, bad); break;
-        case HPA_L: e = seg_reduce_t<int64_t,false>(c, (const int64_t*)px, idx, off, g, n, kind, dst, nx, bad); break;
-        case HPA_F: e = seg_reduce_t<float,true>(c, (const float*)px, idx, off, g, n, kind, dst, nx, bad); break;
-        case HPA_E: e = seg_reduce_t<double,true>(c, (const double*)px, idx, off, g, n, kind, dst, nx, bad); break;
+        case HPA_B: e = seg_reduce_t<uint8_t,false>(c, (const uint8_t*)px, idx, off, g, n, kind, dst, nx, bad, runs); break;
+        case HPA_J: e = seg_reduce_t<int8_t,false>(c, (const int8_t*)px, idx, off, g, n, kind, dst, nx, bad, runs); break;
+        case HPA_H: e = seg_reduce_t<int16_t,false>(c, (const int16_t*)px, idx, off, g, n, kind, dst, nx, bad, runs); break;
+        case HPA_I: e = seg_reduce_t<int32_t,false>(c, (const int32_t*)px, idx, off, g, n, kind, dst, nx, bad, runs); break;
+        case HPA_L: e = seg_reduce_t<int64_t,false>(c, (const int64_t*)px, idx, off, g, n, kind, dst, nx, bad, runs); break;
+        case HPA_F: e = seg_reduce_t<float,true>(c, (const float*)px, idx, off, g, n, kind, dst, nx, bad, runs); break;
+        case HPA_E: e = seg_reduce_t<double,true>(c, (const double*)px, idx, off, g, n, kind, dst, nx, bad, runs); break;
         default: R HPA_E_NOT_IMPL;
     }
     if(e) R e;
@@ -639,8 +692,11 @@ extern "C" int32_t hpaGroupReduce(hpa_V z, int32_t kind, hpa_V x, hpa_V y){
     }
     const void *px = nullptr;
     if(x->len > 0 && (e = dev_view(c, x, &px))) R e;
+    // a group built from sorted runs of x's own length: fold each run's rows in place
+    const hpa_ctx::GroupRuns *runs = is_flat(y) ? group_runs(c, y->g, g) : nullptr;
+    if(runs && (runs->n != n || x->len != n)) runs = nullptr;
     void *out; int32_t tout;
-    if((e = seg_reduce(c, x, px, idx, off, g, n, kind, &out, &tout))) R e;
+    if((e = seg_reduce(c, x, px, idx, off, g, n, kind, &out, &tout, runs))) R e;
     set_vec(z, tout, g, out, c);
     R 0;
 }

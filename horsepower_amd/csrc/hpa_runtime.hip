@@ -176,7 +176,7 @@ extern "C" int32_t hpaCtxSetArena(hpa_ctx *c, void *base, int64_t bytes){
     if(base && !(is_device(base) && is_device((char*)base + bytes - 1))){ set_error_detail("hpaCtxSetArena: not a device pool range"); R HPA_E_DOMAIN; }
     if(hipStreamSynchronize(c->stream) != hipSuccess) R HPA_E_DEVICE;
     c->dbase = (char*)base; c->dtop = base ? bytes : 0; c->dcur = 0; c->dpeak = 0;
-    c->hcur = 0; c->marks.clear(); c->bounds.clear(); c->gvals.clear(); c->goffs.clear();
+    c->hcur = 0; c->marks.clear(); c->bounds.clear(); c->gvals.clear(); c->goffs.clear(); c->gruns.clear();
     for(auto &hb : c->host_bufs) free(hb.p);
     c->host_bufs.clear();
     R 0;
@@ -257,6 +257,7 @@ void arena_set(hpa_ctx *c, L cur){
     c->bounds.erase(c->bounds.lower_bound(top), c->bounds.end());
     c->gvals.erase(c->gvals.lower_bound(top), c->gvals.end());
     c->goffs.erase(c->goffs.lower_bound(top), c->goffs.end());
+    c->gruns.erase(c->gruns.lower_bound(top), c->gruns.end());
     str_prune(c, cur);
     c->dcur = cur;
 }
@@ -275,6 +276,14 @@ void note_group_vals(hpa_ctx *c, const void *keys, const void *src, L n, const v
 const void *group_vals(hpa_ctx *c, const void *keys, const void *src, L n){
     auto it = c->gvals.find(keys);
     R it != c->gvals.end() && it->second.src == src && it->second.n == n ? it->second.vals : nullptr;
+}
+void note_group_runs(hpa_ctx *c, const void *counts, const hpa_ctx::GroupRuns &r){
+    if(r.g > 0 && in_arena(c, counts) && in_arena(c, r.sv) && in_arena(c, r.grun) && (const void*)r.sv < counts &&
+       (const void*)r.grun < counts) c->gruns[counts] = r;
+}
+const hpa_ctx::GroupRuns *group_runs(hpa_ctx *c, const void *counts, L g){
+    auto it = c->gruns.find(counts);
+    R it != c->gruns.end() && it->second.g == g ? &it->second : nullptr;
 }
 void note_group_offsets(hpa_ctx *c, const void *counts, L g, const int64_t *offs){
     if(g > 0 && in_arena(c, counts) && in_arena(c, offs) && (const void*)offs < counts) c->goffs[counts] = {g, offs};

@@ -1469,7 +1469,7 @@ __global__ void __launch_bounds__(256) k_grf_count(const uint32_t *sk, const uin
 #define GRF_STAGE 16384
 __global__ void __launch_bounds__(256) k_grf_write(const uint32_t *sk, const uint32_t *sv, L nr, int cb, unsigned len_mask,
                                                    const int64_t *goff, const int64_t *roff, int64_t *flat, int64_t *keys,
-                                                   int64_t *start, int64_t *cnt, uint32_t *kval){
+                                                   int64_t *start, int64_t *cnt, uint32_t *kval, uint32_t *grun){
     __shared__ uint32_t s_h[GRF_ITEMS * 4], s_r[GRF_ITEMS * 4];
     __shared__ uint64_t s_hb[GRF_ITEMS * 4];              // head flags of the tile's runs, 64 a word
     __shared__ uint32_t s_fpos[GRF_TILE], s_row[GRF_TILE];  // each run's tile-relative flat position, first row
@@ -1552,7 +1552,7 @@ __global__ void __launch_bounds__(256) k_grf_write(const uint32_t *sk, const uin
             }
             if((bal[k] >> lane) & 1ull){
                 const int64_t g = gpre + (int64_t)s_h[k * 4 + w] + __popcll(bal[k] & lt);
-                keys[g] = row; start[g] = fpos; kval[g] = kv[k];
+                keys[g] = row; start[g] = fpos; kval[g] = kv[k]; grun[g] = (uint32_t)i;
                 // the next head of the tile, if any: later in this word, else the first later word
                 int nx = -1;
                 const uint64_t above = lane == 63 ? 0ull : (bal[k] & (~0ull << (lane + 1)));
@@ -1565,7 +1565,7 @@ __global__ void __launch_bounds__(256) k_grf_write(const uint32_t *sk, const uin
 }
 // count of each tile's last group (its end lies in a later tile): start[g + 1] - start[g], with
 // start[G] = n written here
-__global__ void k_grf_fix(const int64_t *goff, L gt, int64_t *start, int64_t *cnt, L n){
+__global__ void k_grf_fix(const int64_t *goff, L gt, int64_t *start, int64_t *cnt, L n, uint32_t *grun, L nr){
     const L G = goff[gt];
     for(L t = (L)blockIdx.x * blockDim.x + threadIdx.x; t < gt; t += (L)gridDim.x * blockDim.x){
         if(goff[t + 1] > goff[t]){
@@ -1573,7 +1573,7 @@ __global__ void k_grf_fix(const int64_t *goff, L gt, int64_t *start, int64_t *cn
             cnt[g] = (g + 1 < G ? start[g + 1] : n) - start[g];
         }
     }
-    if(blockIdx.x == 0 && threadIdx.x == 0) start[G] = n;
+    if(blockIdx.x == 0 && threadIdx.x == 0){ start[G] = n; grun[G] = (uint32_t)nr; }
 }
 static int32_t group_i32_runs(hpa_ctx *c, V z, const int32_t *x, L n, int cb, const GiScan &sc){
     int32_t e;
@@ -1599,19 +1599,23 @@ static int32_t group_i32_runs(hpa_ctx *c, V z, const int32_t *x, L n, int cb, co
     if((e = check_launch("group_runs"))) R e;
     if((e = scan_u32_to_i64(c, gcount, goff, gt, goff + gt))) R e;
     if((e = scan_u32_to_i64(c, rcount, roff, gt, roff + gt))) R e;
-    // outputs for as many groups as runs
+    // outputs for as many groups as runs (the first run of each group before the counts: the
+    // run side table dies with them)
+    uint32_t *grun = (uint32_t*)dalloc(c, 4 * (nr + 1));
     uint32_t *kval = (uint32_t*)dalloc(c, 4 * nr);           // before keys: dies with them
     int64_t *keys = (int64_t*)dalloc(c, 8 * nr), *start = (int64_t*)dalloc(c, 8 * (nr + 1)), *cnt = (int64_t*)dalloc(c, 8 * nr);
     int64_t *flat = (int64_t*)dalloc(c, 8 * n);
-    if(!kval || !keys || !start || !cnt || !flat) R HPA_E_MEMORY;
+    if(!grun || !kval || !keys || !start || !cnt || !flat) R HPA_E_MEMORY;
     hipLaunchKernelGGL(k_grf_write, dim3((unsigned)gt), dim3(256), 0, c->stream, (const uint32_t*)sk, (const uint32_t*)sv, (L)nr, cb, cap_mask,
-                       (const int64_t*)goff, (const int64_t*)roff, flat, keys, start, cnt, kval);
-    hipLaunchKernelGGL(k_grf_fix, dim3(grid_for(gt, 256, 1)), dim3(256), 0, c->stream, (const int64_t*)goff, gt, start, cnt, n);
+                       (const int64_t*)goff, (const int64_t*)roff, flat, keys, start, cnt, kval, grun);
+    hipLaunchKernelGGL(k_grf_fix, dim3(grid_for(gt, 256, 1)), dim3(256), 0, c->stream, (const int64_t*)goff, gt, start, cnt, n,
+                       grun, (L)nr);
     if((e = check_launch("group_runs"))) R e;
     int64_t G = 0; if((e = d2h(c, &G, goff + gt, 8))) R e;
     ps.stop(); ps.bytes = 12.0 * n + 8.0 * G;        // int group: keys read, row list and keys written
     if(G > 1) note_group_vals(c, keys, x, n, kval);
     note_group_offsets(c, cnt, G, start);
+    note_group_runs(c, cnt, hpa_ctx::GroupRuns{G, n, nr, sv, grun, cb});
     R make_group_dict(c, z, keys, cnt, flat, G, n);
 }
 
