@@ -196,7 +196,7 @@ static int32_t pk_build(hpa_ctx *c, const int32_t *x, L n, unsigned long long **
 // that miss touch nothing else.  Taken when the arena has room for the table's address range.
 #define DENSE_PER_KEY 8
 #define GUARD_PER_KEY 512
-#define GUARD_MAX_RANGE (1ll << 28)
+#define GUARD_MAX_RANGE (1ll << 30)
 // smallest and largest key: per-workgroup partials part[2 b], part[2 b + 1] (16-byte loads
 // when aligned), folded by one workgroup into out[0] min, out[1] max
 __device__ __forceinline__ void i32_minmax_block(int lo, int hi, int *part){

@@ -82,3 +82,23 @@ def test_join_single_pair_after_swap(gpu, orc):
     for B in (gpu, orc):
         z = B.read_list(B.pfnJoinIndex(B.vec(x), B.vec(y), B.sym("eq")))
         assert z[0].tolist() == [17] and z[1].tolist() == [0]
+
+
+def test_join_wide_sparse_range(gpu, orc):
+    """Unique build keys spread over a range of about 5e8 values (over 2^28, at most 512 slots a
+    key: the direct table behind a presence bitmap), probes half hitting: pairs equal to the
+    oracle's, and index_of and member equal too."""
+    r = np.random.default_rng(77)
+    nb = 1_200_000
+    build = np.sort(r.choice(500_000_000, nb, replace=False)).astype(np.int32)
+    r.shuffle(build)
+    probe = np.where(r.random(4_000_000) < 0.5, build[r.integers(0, nb, 4_000_000)],
+                     r.integers(0, 500_000_000, 4_000_000)).astype(np.int32)
+    eq = gpu.sym("eq")
+    g = gpu.read_list(gpu.pfnJoinIndex(gpu.vec(build), gpu.vec(probe), eq))
+    o = orc.read_list(orc.pfnJoinIndex(orc.vec(build), orc.vec(probe), orc.sym("eq")))
+    assert np.array_equal(g[0], o[0]) and np.array_equal(g[1], o[1])
+    gi = gpu.read(gpu.pfnIndexOf(gpu.vec(build), gpu.vec(probe)))
+    assert np.array_equal(gi, orc.read(orc.pfnIndexOf(orc.vec(build), orc.vec(probe))))
+    gm = gpu.read(gpu.pfnMember(gpu.vec(probe), gpu.vec(build)))
+    assert np.array_equal(gm, np.isin(probe, build).astype(np.uint8))
