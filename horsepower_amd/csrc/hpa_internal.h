@@ -19,6 +19,8 @@ typedef int64_t L;
 // mapped completion buffer of a context: a 64-byte head holding the sequence number, then the data
 #define PUB_HEAD 64
 #define PUB_BYTES (64 << 10)
+#define PUB_ASYNC_SLOTS 64          // the buffer's last 64 x 64 bytes: reads begun and ended later
+#define PUB_ASYNC_BYTES 64
 struct hpa_ctx {
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -36,6 +38,7 @@ struct hpa_ctx {
     // coherent host memory mapped into the device: small reads are published there by a kernel
     // followed by a sequence number the host spins on (no blocking stream synchronisation)
     char *mapped = nullptr; char *mapped_dev = nullptr; uint64_t pub_seq = 0;
+    char pub_fallback[PUB_ASYNC_SLOTS][PUB_ASYNC_BYTES];      // begun reads when nothing is mapped
     // host buffers handed out by hpaToHost, each with the creation sequence number and the
     // device watermark at creation; a rewind releases the ones made after its mark
     struct HostBuf { void *p; uint64_t seq; L dcur; };
@@ -101,6 +104,10 @@ int32_t dev_view(hpa_ctx *c, V x, const void **p);
 int32_t read_scalar(hpa_ctx *c, V x, void *dst, L width);
 // Copy small device data to host (synchronous on the context stream).
 int32_t d2h(hpa_ctx *c, void *dst, const void *src, L bytes);
+// A small read begun now, in stream order, and collected later (at most PUB_ASYNC_BYTES bytes;
+// at most PUB_ASYNC_SLOTS begun and not yet ended): the host can queue more work before it waits.
+int32_t d2h_begin(hpa_ctx *c, const void *src, L bytes, uint64_t *ticket);
+int32_t d2h_end(hpa_ctx *c, uint64_t ticket, void *dst, L bytes);
 int32_t check_launch(const char *what);
 
 // ---------------------------------------------------------------- timing

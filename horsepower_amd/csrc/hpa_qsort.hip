@@ -75,10 +75,14 @@ __global__ void k_qs_init(const uint32_t *lo, const uint32_t *hi, L nseg, L n, Q
 }
 
 // ---------------------------------------------------------------- big segments, one level
-// tiles per segment: the positions lo+1 .. hi-1 in tiles of QS_TILE
-__global__ void k_qs_ntiles(const unsigned *blo, const unsigned *bhi, L nb, int64_t *nt){
-    for(L s = (L)blockIdx.x * blockDim.x + threadIdx.x; s < nb; s += (L)gridDim.x * blockDim.x)
-        nt[s] = ((L)(bhi[s] - blo[s] - 1) + QS_TILE - 1) / QS_TILE;
+// The level's count of big segments is read on the device (*nbp): the host queues a level
+// before it knows the count, and grids are sized for the most segments a level can hold.
+// tiles per segment: the positions lo+1 .. hi-1 in tiles of QS_TILE; 0 past the level's segments
+// (up to nbmax), so a scan of nbmax entries puts the tile total at tstart[nb]
+__global__ void k_qs_ntiles(const unsigned *blo, const unsigned *bhi, const unsigned *nbp, L nbmax, int64_t *nt){
+    const L nb = *nbp;
+    for(L s = (L)blockIdx.x * blockDim.x + threadIdx.x; s < nbmax; s += (L)gridDim.x * blockDim.x)
+        nt[s] = s < nb ? ((L)(bhi[s] - blo[s] - 1) + QS_TILE - 1) / QS_TILE : 0;
 }
 // segment of tile t: the last s with tstart[s] <= t (every big segment has at least one tile)
 __device__ __forceinline__ L qs_tile_seg(const int64_t *tstart, L nb, L t){
@@ -87,10 +91,10 @@ __device__ __forceinline__ L qs_tile_seg(const int64_t *tstart, L nb, L t){
     return a;
 }
 struct QsTile { L s; unsigned lo, hi, a, b; L r; };
-__device__ __forceinline__ bool qs_tile(const int64_t *tstart, L nb, const unsigned *blo, const unsigned *bhi, QsTile &T){
+__device__ __forceinline__ bool qs_tile(const int64_t *tstart, const unsigned *nbp, const unsigned *blo, const unsigned *bhi, QsTile &T){
     __shared__ L s_seg;
-    const L t = blockIdx.x;
-    if(t >= tstart[nb]) return false;
+    const L t = blockIdx.x, nb = *nbp;
+    if(nb == 0 || t >= tstart[nb]) return false;
     if(threadIdx.x == 0) s_seg = qs_tile_seg(tstart, nb, t);
     __syncthreads();
     T.s = s_seg; T.lo = blo[T.s]; T.hi = bhi[T.s]; T.r = t - tstart[T.s];
@@ -100,9 +104,9 @@ __device__ __forceinline__ bool qs_tile(const int64_t *tstart, L nb, const unsig
 }
 // per tile: number of G and of keys unequal to the pivot
 template<typename KT>
-__global__ void __launch_bounds__(QS_BLOCK) k_qs_count(const KT *K, const int64_t *tstart, L nb, const unsigned *blo,
+__global__ void __launch_bounds__(QS_BLOCK) k_qs_count(const KT *K, const int64_t *tstart, const unsigned *nbp, const unsigned *blo,
                                                        const unsigned *bhi, bool up, unsigned *tg, unsigned *tne){
-    QsTile T; if(!qs_tile(tstart, nb, blo, bhi, T)) return;
+    QsTile T; if(!qs_tile(tstart, nbp, blo, bhi, T)) return;
     const KT pv = K[T.lo];
     unsigned g = 0, ne = 0;
     for(int i = 0; i < QS_ROWS; i++){
@@ -134,11 +138,11 @@ __device__ __forceinline__ unsigned qs_block_incl(unsigned x, unsigned *sw, unsi
 }
 // per segment: exclusive scan of its tiles' G counts (gpre), the segment's G and unequal
 // totals, and S = 0 (the tile holding position j overwrites it)
-__global__ void __launch_bounds__(1024) k_qs_segscan(const int64_t *tstart, L nb, const unsigned *tg, const unsigned *tne,
+__global__ void __launch_bounds__(1024) k_qs_segscan(const int64_t *tstart, const unsigned *nbp, const unsigned *tg, const unsigned *tne,
                                                      unsigned *gpre, unsigned *segG, unsigned *segNE, unsigned *segS){
     __shared__ unsigned sw[16];
     const L s = blockIdx.x;
-    if(s >= nb) return;
+    if(s >= (L)*nbp) return;
     const L t0 = tstart[s], t1 = tstart[s + 1];
     unsigned run = 0, ne = 0;
     for(L base = t0; base < t1; base += 1024){
@@ -158,12 +162,12 @@ __global__ void __launch_bounds__(1024) k_qs_segscan(const int64_t *tstart, L nb
 // at its rank from the right (PL); the thread holding j writes S, the number of pairs.  A tile
 // is staged in LDS and each thread takes 16 consecutive positions, so ranks follow position.
 template<typename KT>
-__global__ void __launch_bounds__(QS_BLOCK) k_qs_place(const KT *K, const int64_t *tstart, L nb, const unsigned *blo,
+__global__ void __launch_bounds__(QS_BLOCK) k_qs_place(const KT *K, const int64_t *tstart, const unsigned *nbp, const unsigned *blo,
                                                        const unsigned *bhi, bool up, const unsigned *gpre,
                                                        const unsigned *segG, unsigned *segS, unsigned *PG, unsigned *PL){
     __shared__ KT sk[QS_TILE];
     __shared__ unsigned sw[QS_BLOCK / 64];
-    QsTile T; if(!qs_tile(tstart, nb, blo, bhi, T)) return;
+    QsTile T; if(!qs_tile(tstart, nbp, blo, bhi, T)) return;
     const KT pv = K[T.lo];
     const unsigned cnt = T.b - T.a;
     for(int i = 0; i < QS_ROWS; i++){
@@ -195,9 +199,9 @@ __global__ void __launch_bounds__(QS_BLOCK) k_qs_place(const KT *K, const int64_
 }
 // the S pairs of each segment swap keys and rows
 template<typename KT>
-__global__ void __launch_bounds__(QS_BLOCK) k_qs_swap(KT *K, uint32_t *P, const int64_t *tstart, L nb, const unsigned *blo,
+__global__ void __launch_bounds__(QS_BLOCK) k_qs_swap(KT *K, uint32_t *P, const int64_t *tstart, const unsigned *nbp, const unsigned *blo,
                                                       const unsigned *bhi, const unsigned *segS, const unsigned *PG, const unsigned *PL){
-    QsTile T; if(!qs_tile(tstart, nb, blo, bhi, T)) return;
+    QsTile T; if(!qs_tile(tstart, nbp, blo, bhi, T)) return;
     const unsigned S = segS[T.s];
     const L k0 = T.r * QS_TILE;
     for(int i = 0; i < QS_ROWS; i++){
@@ -210,8 +214,9 @@ __global__ void __launch_bounds__(QS_BLOCK) k_qs_swap(KT *K, uint32_t *P, const 
 }
 // pivot swap and the children of every segment that is not all equal to its pivot
 template<typename KT>
-__global__ void k_qs_finish(KT *K, uint32_t *P, L nb, const unsigned *blo, const unsigned *bhi, const unsigned *segG,
+__global__ void k_qs_finish(KT *K, uint32_t *P, const unsigned *nbp, const unsigned *blo, const unsigned *bhi, const unsigned *segG,
                             const unsigned *segNE, QsState *st, int nxt, unsigned *nblo, unsigned *nbhi, unsigned *slo, unsigned *shi){
+    const L nb = *nbp;
     for(L s = (L)blockIdx.x * blockDim.x + threadIdx.x; s < nb; s += (L)gridDim.x * blockDim.x){
         if(segNE[s] == 0) continue;                       // leftSame: every key equals the pivot
         const unsigned lo = blo[s], hi = bhi[s], j = hi - 1 - segG[s];
@@ -369,6 +374,11 @@ __global__ void __launch_bounds__(QS_BLOCK) k_qs_small(KT *K, uint32_t *P, const
 }
 
 // ---------------------------------------------------------------- driver
+// Levels are queued one ahead of what the host knows: after level L the count of level L + 1's
+// big segments is published into mapped host memory (a begun read) and level L + 1 is queued at
+// once; the host then waits for that count while level L + 1 runs, and stops when it is zero
+// (that last queued level finds no segment and does nothing).  So the device never idles for a
+// read between levels.
 template<typename KT>
 static int32_t qsort_levels(hpa_ctx *c, KT *K, uint32_t *P, L n, QsState *st, unsigned nbig, unsigned *bl[2][2],
                             unsigned *slo, unsigned *shi, bool up){
@@ -379,33 +389,42 @@ static int32_t qsort_levels(hpa_ctx *c, KT *K, uint32_t *P, L n, QsState *st, un
     unsigned *segG = (unsigned*)dalloc(c, 4 * nbmax), *segNE = (unsigned*)dalloc(c, 4 * nbmax), *segS = (unsigned*)dalloc(c, 4 * nbmax);
     unsigned *PG = (unsigned*)dalloc(c, 4 * n), *PL = (unsigned*)dalloc(c, 4 * n);
     if(!tstart || !tg || !tne || !gpre || !segG || !segNE || !segS || !PG || !PL) R HPA_E_MEMORY;
+    if(nbig == 0) R 0;
     int cur = 0;
-    for(int level = 0; nbig > 0; level++){
+    auto queue_level = [&](int cu) -> int32_t {
+        unsigned *blo = bl[cu][0], *bhi = bl[cu][1];
+        const unsigned *nbp = &st->nbig[cu];
+        hipMemsetAsync(&st->nbig[cu ^ 1], 0, 4, c->stream);
+        hipLaunchKernelGGL(k_qs_ntiles, dim3(grid_for(nbmax, 256, 1)), dim3(256), 0, c->stream, blo, bhi, nbp, nbmax, tstart);
+        int32_t e2;
+        if((e2 = check_launch("order_qsort"))) R e2;
+        if((e2 = scan_i64(c, tstart, tstart, nbmax, tstart + nbmax))) R e2;
+        hipLaunchKernelGGL(k_qs_count<KT>, dim3((unsigned)tmax), dim3(QS_BLOCK), 0, c->stream, (const KT*)K, (const int64_t*)tstart, nbp,
+                           (const unsigned*)blo, (const unsigned*)bhi, up, tg, tne);
+        hipLaunchKernelGGL(k_qs_segscan, dim3((unsigned)nbmax), dim3(1024), 0, c->stream, (const int64_t*)tstart, nbp, (const unsigned*)tg,
+                           (const unsigned*)tne, gpre, segG, segNE, segS);
+        hipLaunchKernelGGL(k_qs_place<KT>, dim3((unsigned)tmax), dim3(QS_BLOCK), 0, c->stream, (const KT*)K, (const int64_t*)tstart, nbp,
+                           (const unsigned*)blo, (const unsigned*)bhi, up, (const unsigned*)gpre, (const unsigned*)segG, segS, PG, PL);
+        hipLaunchKernelGGL(k_qs_swap<KT>, dim3((unsigned)tmax), dim3(QS_BLOCK), 0, c->stream, K, P, (const int64_t*)tstart, nbp,
+                           (const unsigned*)blo, (const unsigned*)bhi, (const unsigned*)segS, (const unsigned*)PG, (const unsigned*)PL);
+        hipLaunchKernelGGL(k_qs_finish<KT>, dim3(grid_for(nbmax, 256, 1)), dim3(256), 0, c->stream, K, P, nbp, (const unsigned*)blo,
+                           (const unsigned*)bhi, (const unsigned*)segG, (const unsigned*)segNE, st, cu ^ 1,
+                           bl[cu ^ 1][0], bl[cu ^ 1][1], slo, shi);
+        R check_launch("order_qsort");
+    };
+    // level 0 (nbig > 0 known), then: count of the next level begun, next level queued, count awaited
+    if((e = queue_level(cur))) R e;
+    for(int level = 1; ; level++){
         if(level >= QS_MAX_LEVELS){
             set_error_detail("order: the reference's quicksort recursion is deeper than %d levels of segments over %d rows on this input", QS_MAX_LEVELS, QS_SMALL);
             R HPA_E_NOT_IMPL;
         }
-        const L nb = nbig;
-        unsigned *blo = bl[cur][0], *bhi = bl[cur][1];
-        const L ntmax = n / QS_TILE + nb + 1;
-        hipMemsetAsync(&st->nbig[cur ^ 1], 0, 4, c->stream);
-        hipLaunchKernelGGL(k_qs_ntiles, dim3(grid_for(nb, 256, 1)), dim3(256), 0, c->stream, blo, bhi, nb, tstart);
-        if((e = check_launch("order_qsort"))) R e;
-        if((e = scan_i64(c, tstart, tstart, nb, tstart + nb))) R e;
-        hipLaunchKernelGGL(k_qs_count<KT>, dim3((unsigned)ntmax), dim3(QS_BLOCK), 0, c->stream, (const KT*)K, (const int64_t*)tstart, nb,
-                           (const unsigned*)blo, (const unsigned*)bhi, up, tg, tne);
-        hipLaunchKernelGGL(k_qs_segscan, dim3((unsigned)nb), dim3(1024), 0, c->stream, (const int64_t*)tstart, nb, (const unsigned*)tg,
-                           (const unsigned*)tne, gpre, segG, segNE, segS);
-        hipLaunchKernelGGL(k_qs_place<KT>, dim3((unsigned)ntmax), dim3(QS_BLOCK), 0, c->stream, (const KT*)K, (const int64_t*)tstart, nb,
-                           (const unsigned*)blo, (const unsigned*)bhi, up, (const unsigned*)gpre, (const unsigned*)segG, segS, PG, PL);
-        hipLaunchKernelGGL(k_qs_swap<KT>, dim3((unsigned)ntmax), dim3(QS_BLOCK), 0, c->stream, K, P, (const int64_t*)tstart, nb,
-                           (const unsigned*)blo, (const unsigned*)bhi, (const unsigned*)segS, (const unsigned*)PG, (const unsigned*)PL);
-        hipLaunchKernelGGL(k_qs_finish<KT>, dim3(grid_for(nb, 256, 1)), dim3(256), 0, c->stream, K, P, nb, (const unsigned*)blo,
-                           (const unsigned*)bhi, (const unsigned*)segG, (const unsigned*)segNE, st, cur ^ 1,
-                           bl[cur ^ 1][0], bl[cur ^ 1][1], slo, shi);
-        if((e = check_launch("order_qsort"))) R e;
+        uint64_t ticket;
+        if((e = d2h_begin(c, &st->nbig[cur ^ 1], 4, &ticket))) R e;
         cur ^= 1;
-        if((e = d2h(c, &nbig, &st->nbig[cur], 4))) R e;
+        if((e = queue_level(cur))) R e;
+        if((e = d2h_end(c, ticket, &nbig, 4))) R e;
+        if(nbig == 0) break;
     }
     R 0;
 }

@@ -337,11 +337,11 @@ __global__ void __launch_bounds__(256) k_publish(PubList r, char *dst, unsigned 
 static int32_t pub_wait(hpa_ctx *c, uint64_t seq){
     volatile uint64_t *f = (volatile uint64_t*)c->mapped;
     const auto t0 = std::chrono::steady_clock::now();
-    for(uint64_t it = 1; *f != seq; it++){
+    for(uint64_t it = 1; *f < seq; it++){
         __builtin_ia32_pause();
         if((it & 255) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(PUB_SPIN_US)){
             if(hipStreamSynchronize(c->stream) != hipSuccess){ set_error_detail("stream failed before a device read"); R HPA_E_DEVICE; }
-            if(*f != seq){ set_error_detail("device read: completion number %llu not published", (unsigned long long)seq); R HPA_E_DEVICE; }
+            if(*f < seq){ set_error_detail("device read: completion number %llu not published", (unsigned long long)seq); R HPA_E_DEVICE; }
             break;
         }
     }
@@ -353,7 +353,7 @@ static int32_t pub_wait(hpa_ctx *c, uint64_t seq){
 static bool pub_fits(int n, const int64_t *bytes){
     if(n > PUB_MAX) R false;
     L tot = 0; for(int k = 0; k < n; k++) tot += (bytes[k] + 15) & ~(L)15;
-    R tot <= PUB_BYTES;
+    R tot <= PUB_BYTES - PUB_ASYNC_SLOTS * PUB_ASYNC_BYTES;
 }
 static int32_t pub_read(hpa_ctx *c, int n, const void *const *src, void *const *dst, const int64_t *bytes){
     PubList r; r.n = 0; L off = 0;
@@ -374,9 +374,29 @@ static int32_t pub_read(hpa_ctx *c, int n, const void *const *src, void *const *
     R 0;
 }
 
+// begun reads land in slot (ticket % PUB_ASYNC_SLOTS) at the end of the data area; the
+// sequence number is shared with the synchronous reads, so a later number implies an earlier one
+static L pub_async_off(uint64_t ticket){ R PUB_HEAD + PUB_BYTES - (L)PUB_ASYNC_SLOTS * PUB_ASYNC_BYTES + (L)(ticket % PUB_ASYNC_SLOTS) * PUB_ASYNC_BYTES; }
+int32_t d2h_begin(hpa_ctx *c, const void *src, L bytes, uint64_t *ticket){
+    if(bytes <= 0 || bytes > PUB_ASYNC_BYTES){ set_error_detail("d2h_begin of %lld bytes", (long long)bytes); R HPA_E_DOMAIN; }
+    const uint64_t seq = ++c->pub_seq;
+    *ticket = seq;
+    if(!c->mapped) R d2h(c, c->pub_fallback[seq % PUB_ASYNC_SLOTS], src, bytes);
+    PubList r; r.n = 1; r.src[0] = (const char*)src; r.off[0] = 0; r.bytes[0] = bytes;
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(256), 0, c->stream, r, c->mapped_dev + pub_async_off(seq),
+                       (unsigned long long*)c->mapped_dev, (unsigned long long)seq);
+    R check_launch("device read");
+}
+int32_t d2h_end(hpa_ctx *c, uint64_t ticket, void *dst, L bytes){
+    if(!c->mapped){ memcpy(dst, c->pub_fallback[ticket % PUB_ASYNC_SLOTS], (size_t)bytes); R 0; }
+    int32_t e = pub_wait(c, ticket); if(e) R e;
+    memcpy(dst, c->mapped + pub_async_off(ticket), (size_t)bytes);
+    R 0;
+}
+
 int32_t d2h(hpa_ctx *c, void *dst, const void *src, L bytes){
     if(bytes <= 0) R 0;
-    if(c->mapped && bytes <= PUB_BYTES && is_device(src)){
+    if(c->mapped && bytes <= PUB_BYTES - PUB_ASYNC_SLOTS * PUB_ASYNC_BYTES && is_device(src)){
         const void *s1[1] = {src}; void *d1[1] = {dst}; const int64_t b1[1] = {bytes};
         R pub_read(c, 1, s1, d1, b1);
     }
