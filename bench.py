@@ -74,8 +74,9 @@ def parse():
     p.add_argument("--rate", type=float, default=0.0)
     p.add_argument("--slots", type=int, default=4)
     p.add_argument("--heavy-share", type=int, default=-1)
-    p.add_argument("--heavy-slots", type=int, default=1)
+    p.add_argument("--heavy-slots", type=int, default=2)
     p.add_argument("--heavy-fifo", type=int, default=0)
+    p.add_argument("--late-share", type=int, default=0)
     return p.parse_args()
 
 
@@ -552,7 +553,7 @@ def bench_serve(a, B=None):
     print(f"[bench] serve tables: {n_dim} orders, {nf} lineitems in HBM ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     srv = S.Server(B, T, slots=a.slots, pool_bytes=128 << 30, merge=True,
                    heavy_share=None if a.heavy_share < 0 else a.heavy_share, heavy_slots=a.heavy_slots,
-                   heavy_fifo=bool(a.heavy_fifo))
+                   heavy_fifo=bool(a.heavy_fifo), late_share=a.late_share)
     est = srv.calibrate()
     print(f"[bench] calibrated arena estimates (MiB): { {k: v >> 20 for k, v in est.items()} }", file=sys.stderr, flush=True)
     # service time of the per-query fused path sets the offered load
@@ -565,6 +566,13 @@ def bench_serve(a, B=None):
     runs = {}
     runs["server"] = S.summarize(srv.serve(S.make_requests(N, rate, seed=2024)))
     print(f"[bench] server: {runs['server']}", file=sys.stderr, flush=True)
+    # the tail-first policy on the same stream: no short request that arrived after a running
+    # heavy one runs beside it
+    fifo, late = srv.heavy_fifo, srv.late_share
+    srv.heavy_fifo, srv.late_share = True, 0
+    runs["server_tail_first"] = S.summarize(srv.serve(S.make_requests(N, rate, seed=2024)))
+    srv.heavy_fifo, srv.late_share = fifo, late
+    print(f"[bench] server, tail first: {runs['server_tail_first']}", file=sys.stderr, flush=True)
     ser = S.serve_serial(B, T, S.make_requests(N, rate, seed=2024), plans="fused")
     runs["serial_fused"] = S.summarize(ser)
     print(f"[bench] serial fused: {runs['serial_fused']}", file=sys.stderr, flush=True)
@@ -586,7 +594,10 @@ def bench_serve(a, B=None):
            "config": {"workload": f"serving: Poisson mix of q6/q1/q3 requests (45/35/20%) with random constants, "
                                   f"{nf} lineitem + {n_dim} order rows resident", "rate_per_s": rate,
                       "slots": a.slots, "requests": N},
-           "latency_ms": {"p50": runs["server"]["p50_ms"], "p99": runs["server"]["p99_ms"]},
+           "latency_ms": {"p50": runs["server"]["p50_ms"], "p99": runs["server"]["p99_ms"],
+                          "tail_first_p50": runs["server_tail_first"]["p50_ms"], "tail_first_p99": runs["server_tail_first"]["p99_ms"],
+                          "one_at_a_time_p50": runs["serial_fused"]["p50_ms"], "one_at_a_time_p99": runs["serial_fused"]["p99_ms"]},
+           "closed_loop_vs_one_at_a_time": round(runs["server_closed"]["queries_per_s"] / runs["serial_fused_closed"]["queries_per_s"], 3),
            "runs": runs, "equal_results": equal, "arena_estimates_mib": {k: v >> 20 for k, v in est.items()},
            "peak_pool_in_use_mib": srv.peak_in_use >> 20, "cpu_baseline": None,
            "cpu_baseline_note": "the reference has no serving loop; its per-request primitives are timed in the query workloads"}

@@ -341,6 +341,37 @@ class GpuBackend(Backend):
             out.append(buf[o:o + g * dt.itemsize].view(dt))
         return out[:nk], out[nk:]
 
+    def fused_group_agg_buckets(self, keys, xs, cond, bucket, bbits, spec, cap=64):
+        """hpaFusedGroupAggBuckets: groups (bucket, keys) with the bucket an expression over xs
+        below 2^bbits; returns ([key arrays], bucket array, [aggregate arrays]) in (bucket, keys)
+        order, read back with one synchronisation (at most cap groups)."""
+        from .vnode import HorseError, aggs
+        nk, na = len(keys), len(spec)
+        zarr = (V * (nk + na))()
+        self.lib.rc("hpaAllocNodes", nk + na, zarr)
+        buf = np.empty(cap * (nk + 1 + 8 * na) + 16, dtype=np.uint8)
+        base = buf.ctypes.data
+        offs = [k * cap for k in range(nk + 1)] + [(((nk + 1) * cap + 7) & ~7) + 8 * cap * a for a in range(na)]
+        ng = C.c_int64(0)
+        agg_arr = aggs(spec)
+        rc = self.lib.dll.hpaFusedGroupAggBuckets(
+            nk, (V * nk)(*keys), len(xs), (V * len(xs))(*xs), C.byref(cond) if cond is not None else None, C.byref(bucket), bbits,
+            na, C.cast(agg_arr, C.c_void_p), zarr, C.cast(C.addressof(zarr) + nk * C.sizeof(V), C.POINTER(V)), C.c_int64(cap),
+            (C.c_void_p * nk)(*[base + o for o in offs[:nk]]), C.c_void_p(base + offs[nk]),
+            (C.c_void_p * na)(*[base + o for o in offs[nk + 1:]]), C.byref(ng))
+        if rc == 99:
+            return None
+        if rc != 0:
+            raise HorseError(rc, "hpaFusedGroupAggBuckets", self.lib.detail())
+        g = ng.value
+        if g > cap:
+            raise HorseError(99, "hpaFusedGroupAggBuckets", f"{g} groups above the {cap} read back")
+        zk, za = list(zarr[:nk]), list(zarr[nk:])
+        ks = [buf[o:o + g].view(_dtype_of(z.contents.typ)) for z, o in zip(zk, offs[:nk])]
+        bk = buf[offs[nk]:offs[nk] + g].copy()
+        ag = [buf[o:o + g * 8].view(_dtype_of(z.contents.typ)) for z, o in zip(za, offs[nk + 1:])]
+        return ks, bk, ag
+
     def group_reduce(self, kind, x, groups):
         z = self.node()
         self.lib.rc("hpaGroupReduce", z, kind, x, groups)

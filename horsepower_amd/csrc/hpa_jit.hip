@@ -541,8 +541,10 @@ int32_t jit_compress_map(hpa_ctx *c, int otyp, const DevCols &cols, const hpa_ex
 // Accumulator classes: sums (and avgs, which share the sum of the same body), mins, maxes.
 struct Acc { std::string body; bool f; int cls; };   // cls 0 sum, 2 min, 3 max
 
+// bk (may be null): a bucket number per row, kept in code bits 16 .. 16 + bbits (bbits <= 2), so
+// groups are (bucket, key bytes); the merge's code bitmap grows to match
 static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const CExpr &ce,
-                              const std::vector<int> &used, const std::vector<Acc> &acc){
+                              const std::vector<int> &used, const std::vector<Acc> &acc, const CExpr *bk = nullptr, int bbits = 0){
     const int NA = (int)acc.size() > 0 ? (int)acc.size() : 1;
     const int U = FGA_ROWS;
     std::ostringstream s;
@@ -619,8 +621,10 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
     std::ostringstream row;
     row << "            if(" << ce.s << "){\n";
     // signed key types have their sign bit flipped, so ascending codes are ascending keys
-    if(nk == 2) row << "                const unsigned code = (((unsigned)kb0 ^ a.kx0) << 8) | ((unsigned)kb1 ^ a.kx1);\n";
-    else row << "                const unsigned code = (unsigned)kb0 ^ a.kx0;\n";
+    // (the bucket is masked to bbits: a value past them cannot index past the merge's code bitmap)
+    const std::string bpart = bk ? "((((unsigned)(" + bk->s + ")) & " + std::to_string((1 << bbits) - 1) + "u) << 16) | " : "";
+    if(nk == 2) row << "                const unsigned code = " << bpart << "(((unsigned)kb0 ^ a.kx0) << 8) | ((unsigned)kb1 ^ a.kx1);\n";
+    else row << "                const unsigned code = " << bpart << "((unsigned)kb0 ^ a.kx0);\n";
     row << "                const unsigned sl = slot_of(ht, s_code, &s_used, code);\n"
            "                if(sl >= NS){ __hip_atomic_store(&s_ovf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); goto fga_done; }\n";
     for(int k = 0; k < (int)acc.size(); k++) row << "                const " << (acc[k].f ? "double" : "i64") << " b" << k << " = " << acc[k].body << ";\n";
@@ -732,11 +736,12 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
             else o << ind << "m.q[(i64)r * NA + " << k << "] = (i64)t" << k << ";\n";
         }
     };
-    s << "#define MS " << FGA_MS << "\n#define MCAP " << FGA_MCAP << "\n"
+    const int cwt = 2 << bbits;                              // code-bitmap words per merge thread
+    s << "#define MS " << FGA_MS << "\n#define MCAP " << FGA_MCAP << "\n#define CWT " << cwt << "\n"
          "struct M { const void *part; const unsigned *used; int G; unsigned *codes; int *g; u64 *cnt; i64 *first; double *f; i64 *q;\n"
          "           u64 *rc; i64 *rfs; double *rf; i64 *rq; int *ovf; };\n"
          "extern \"C\" __global__ void __launch_bounds__(1024) fga_merge(M m){\n"
-         "    __shared__ unsigned bits[2048];\n"
+         "    __shared__ unsigned bits[1024 * CWT];\n"
          "    __shared__ unsigned wsum[16];\n"
          "    __shared__ int sovf;\n"
          "    __shared__ unsigned scodes[MCAP];\n"
@@ -744,7 +749,7 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
          "    const P *part = (const P*)m.part;\n"
          "    const unsigned *pcode = (const unsigned*)(part + (i64)m.G * NS);\n"
          "    const int SL = m.G * NS;\n"
-         "    for(int k = threadIdx.x; k < 2048; k += 1024) bits[k] = 0;\n"
+         "    for(int k = threadIdx.x; k < 1024 * CWT; k += 1024) bits[k] = 0;\n"
          "    if(threadIdx.x == 0) sovf = 0;\n"
          "    __syncthreads();\n"
          "    // one thread per slot: the used count and the code load together; bit 31 of a used\n"
@@ -758,7 +763,9 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
          "    __syncthreads();\n"
          "    // list position of this thread's codes: a scan inside each wave, then the wave totals\n"
          "    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;\n"
-         "    const unsigned cc = __popc(bits[2 * threadIdx.x]) + __popc(bits[2 * threadIdx.x + 1]);\n"
+         "    unsigned cc = 0;\n"
+         "    #pragma unroll\n"
+         "    for(int h = 0; h < CWT; h++) cc += __popc(bits[CWT * threadIdx.x + h]);\n"
          "    unsigned inc = cc;\n"
          "    for(int o = 1; o < 64; o <<= 1){ const unsigned t = __shfl_up(inc, o, 64); if(lane >= o) inc += t; }\n"
          "    if(lane == 63) wsum[wv] = inc;\n"
@@ -766,8 +773,8 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
          "    unsigned pos = inc - cc, gt = 0;\n"
          "    for(int q = 0; q < 16; q++){ const unsigned t = wsum[q]; if(q < wv) pos += t; gt += t; }\n"
          "    const bool b0 = blockIdx.x == 0;\n"
-         "    for(int h = 0; h < 2; h++){ unsigned wd = bits[2 * threadIdx.x + h]; while(wd){ int bt = __ffs(wd) - 1; wd &= wd - 1;\n"
-         "        const unsigned cv = (unsigned)((2 * threadIdx.x + h) * 32 + bt); if(b0) m.codes[pos] = cv; if(pos < MCAP) scodes[pos] = cv; pos++; } }\n"
+         "    for(int h = 0; h < CWT; h++){ unsigned wd = bits[CWT * threadIdx.x + h]; while(wd){ int bt = __ffs(wd) - 1; wd &= wd - 1;\n"
+         "        const unsigned cv = (unsigned)((CWT * threadIdx.x + h) * 32 + bt); if(b0) m.codes[pos] = cv; if(pos < MCAP) scodes[pos] = cv; pos++; } }\n"
          "    const int g = (int)gt;\n"
          "    const bool split = g <= MCAP && gridDim.x > 1;\n"
          "    if(b0 && threadIdx.x == 0){ *m.g = g; *m.ovf = sovf; }\n"
@@ -849,12 +856,13 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
     // order (one thread per group); then every group's typed outputs are written (the layout of
     // FgaOut: key bytes, first row, and per output a count, an average, or the accumulator)
     s << "struct EO { int na, stride, nk; unsigned kx0, kx1; int kind[16]; int accid[16]; int accf[16]; void *out[16];\n"
-         "            unsigned char *key0, *key1; i64 *first; };\n"
+         "            unsigned char *key0, *key1; i64 *first; unsigned char *bucket; };\n"
          "struct M2 { M m; EO o; };\n"
          "__device__ __forceinline__ void fga_emit(const M &m, const EO &o, int r){\n"
          "    const unsigned cd = m.codes[r];\n"
-         "    if(o.nk == 2){ o.key0[r] = (unsigned char)((cd >> 8) ^ o.kx0); o.key1[r] = (unsigned char)((cd & 255) ^ o.kx1); }\n"
-         "    else o.key0[r] = (unsigned char)(cd ^ o.kx0);\n"
+         "    if(o.bucket) o.bucket[r] = (unsigned char)(cd >> 16);\n"
+         "    if(o.nk == 2){ o.key0[r] = (unsigned char)(((cd >> 8) & 255) ^ o.kx0); o.key1[r] = (unsigned char)((cd & 255) ^ o.kx1); }\n"
+         "    else o.key0[r] = (unsigned char)((cd & 255) ^ o.kx0);\n"
          "    o.first[r] = m.first[r];\n"
          "    for(int a = 0; a < o.na; a++){\n"
          "        const i64 at = (i64)r * o.stride + o.accid[a];\n"
@@ -903,7 +911,8 @@ static std::string fga_source(int nk, int ns, int nx, const int32_t *typ, const 
 
 // outputs of the merge are laid out [group][accumulator]; fga_merge2 makes typed vectors of
 // them through this layout (mirrored by EO in the generated source)
-struct FgaOut { int na, stride, nk; unsigned kx0, kx1; int kind[16]; int accid[16]; int accf[16]; void *out[16]; uint8_t *key0, *key1; int64_t *first; };
+struct FgaOut { int na, stride, nk; unsigned kx0, kx1; int kind[16]; int accid[16]; int accf[16]; void *out[16]; uint8_t *key0, *key1; int64_t *first;
+                uint8_t *bucket; };
 // a one-element result is inline: its value comes from `host` when given, else from the device
 static void set_dev_vec(hpa_ctx *c, V z, int32_t typ, L len, void *p, const void *host = nullptr){
     z->typ = typ; z->len = len;
@@ -914,17 +923,20 @@ static void set_dev_vec(hpa_ctx *c, V z, int32_t typ, L len, void *p, const void
     else z->g = (char*)p;
 }
 // host destinations of hpaFusedGroupAggHost
-struct FgaHost { L cap; void **keys; int64_t *first; void **aggs; int64_t *ng; };
+struct FgaHost { L cap; void **keys; int64_t *first; void **aggs; int64_t *ng; uint8_t *bucket = nullptr; };
 
 // hpaFusedGroupAgg: see the public header.  Returns HPA_E_NOT_IMPL (nothing written) when a
 // block meets more distinct key codes than it has slots; the caller then runs the unfused plan.
 // prepared inputs of one distinct fused group-aggregate request (see hpaFusedGroupAgg)
 // fn[i]: the fga, fga_merge and fga_merge2 kernels with 4 << i slots, once compiled
 #define FGA_NSV 3
-struct FgaPrep { CExpr ce; Konst kc; std::vector<Acc> acc; std::vector<int> used; FgaOut o; hipFunction_t fn[FGA_NSV][3] = {}; };
+struct FgaPrep { CExpr ce; Konst kc; std::vector<Acc> acc; std::vector<int> used; FgaOut o; hipFunction_t fn[FGA_NSV][3] = {};
+                 CExpr bk; bool has_bk = false; };
 static std::map<std::string, FgaPrep> g_fga_prep;
 static int32_t fga_run(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_expr *cond,
-                       int32_t na, const hpa_agg *aggs, hpa_V zfirst, hpa_V *zkeys, hpa_V *zaggs, const FgaHost *hd){
+                       int32_t na, const hpa_agg *aggs, hpa_V zfirst, hpa_V *zkeys, hpa_V *zaggs, const FgaHost *hd,
+                       const hpa_expr *bucket = nullptr, int bbits = 0){
+    if(bbits < 0 || bbits > 2 || (bucket && bbits == 0) || (!bucket && bbits)) R HPA_E_DOMAIN;
     hpa_ctx *c = ctx_cur();
     if(nk < 1 || nk > 2 || na < 1 || na > 16 || nx < 0 || nx > EXPR_MAX_COLS) R HPA_E_DOMAIN;
     if(!jit_enabled()) R HPA_E_NOT_IMPL;
@@ -948,6 +960,7 @@ static int32_t fga_run(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_
         if(ex && m > 0) raw.append((const char*)ex->code, sizeof(hpa_ins) * (size_t)m);
     };
     add_expr(cond);
+    add_expr(bucket); raw.append((const char*)&bbits, 4);
     raw.append((const char*)&na, 4);
     for(int a = 0; a < na; a++){ raw.append((const char*)&aggs[a].kind, 4); add_expr(aggs[a].kind == 4 ? nullptr : aggs[a].body); }
     FgaPrep prep;
@@ -962,6 +975,7 @@ static int32_t fga_run(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_
         std::vector<int> uc(nx, 0), ub(nx, 0);
         prep.ce = CExpr{COND_TRUE, false};
         if(cond && (e = expr_to_c(cond, nx, typ, &prep.ce, uc, &prep.kc))) R e;
+        if(bucket){ if((e = expr_to_c(bucket, nx, typ, &prep.bk, uc, &prep.kc))) R e; prep.has_bk = true; }
         std::vector<std::string> acc_key;                    // bodies with literal constants
         memset(&prep.o, 0, sizeof(prep.o));
         prep.o.na = na; prep.o.nk = nk;
@@ -1040,13 +1054,18 @@ static int32_t fga_run(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_
     // that meets more distinct codes than slots reports overflow and the next size reruns
     int hv[2] = {0, 0};
     static_assert((4 << (FGA_NSV - 1)) >= FGA_NS, "a kernel cache entry for every slot count");
-    for(int ns = 4, nsi = 0; ns <= FGA_NS; ns *= 2, nsi++){
+    const CExpr *bkp = prep.has_bk ? &prep.bk : nullptr;
+    // a bucketed request meets up to 2^bbits times the key codes: start at the largest slot count
+    // rather than rerun the pass after an overflow
+    const int ns0 = bkp ? FGA_NS : 4, nsi0 = bkp ? FGA_NSV - 1 : 0;
+    for(int ns = ns0, nsi = nsi0; ns <= FGA_NS; ns *= 2, nsi++){
         hipFunction_t f1 = prep.fn[nsi][0], f2 = prep.fn[nsi][1], f3 = prep.fn[nsi][2];
         if(!f1 || !f2 || !f3){
             std::string key = "fga|" + std::to_string(nk) + "|" + std::to_string(ns) + "|" + std::to_string(nx) + "|" + types_key(nx, typ) + "|" + ce.s;
             for(int k = 0; k < nx; k++) key += used[k] ? '1' : '0';
             for(auto &ac : acc) key += "|" + ac.body + "#" + std::to_string(ac.cls) + (ac.f ? "f" : "i");
-            auto gen = [&](){ return fga_source(nk, ns, nx, typ, ce, used, acc); };
+            if(bkp) key += "|bk" + std::to_string(bbits) + ":" + bkp->s;
+            auto gen = [&](){ return fga_source(nk, ns, nx, typ, ce, used, acc, bkp, bbits); };
             if((e = jit_get_keyed(key, "fga", gen, &f1)) || (e = jit_get_keyed(key, "fga_merge", gen, &f2)) ||
                (e = jit_get_keyed(key, "fga_merge2", gen, &f3))) R e;
             std::lock_guard<std::mutex> lk(g_jit_mu);
@@ -1066,7 +1085,8 @@ static int32_t fga_run(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_
         if(!o.key0){
             o.key0 = (uint8_t*)dalloc(c, GM); o.key1 = (uint8_t*)dalloc(c, GM);
             o.first = (int64_t*)dalloc(c, 8 * GM);
-            if(!o.key0 || !o.key1 || !o.first){ arena_set(c, mark); R HPA_E_MEMORY; }
+            o.bucket = bkp ? (uint8_t*)dalloc(c, GM) : nullptr;
+            if(!o.key0 || !o.key1 || !o.first || (bkp && !o.bucket)){ arena_set(c, mark); R HPA_E_MEMORY; }
             for(int a = 0; a < na; a++){ o.out[a] = dalloc(c, 8 * GM); if(!o.out[a]){ arena_set(c, mark); R HPA_E_MEMORY; } }
         }
         struct { decltype(ma) m; FgaOut o; } m2;
@@ -1075,11 +1095,12 @@ static int32_t fga_run(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_
         if(hd){
             // the flag, the group count and the first cap groups of every output in one read
             const L cap = hd->cap < GM ? hd->cap : GM;
-            const void *src[3 + 2 * 16]; void *dst[3 + 2 * 16]; int64_t nb[3 + 2 * 16]; int m = 0;
+            const void *src[4 + 2 * 16]; void *dst[4 + 2 * 16]; int64_t nb[4 + 2 * 16]; int m = 0;
             src[m] = ovf; dst[m] = hv; nb[m++] = 8;
             src[m] = o.key0; dst[m] = hd->keys[0]; nb[m++] = cap;
             if(nk == 2){ src[m] = o.key1; dst[m] = hd->keys[1]; nb[m++] = cap; }
             if(hd->first){ src[m] = o.first; dst[m] = hd->first; nb[m++] = 8 * cap; }
+            if(hd->bucket && o.bucket){ src[m] = o.bucket; dst[m] = hd->bucket; nb[m++] = cap; }
             for(int a = 0; a < na; a++){ src[m] = o.out[a]; dst[m] = hd->aggs[a]; nb[m++] = 8 * cap; }
             if((e = read_staged(c, m, src, dst, nb))) R e;
         } else if((e = d2h(c, hv, ovf, 8))) R e;
@@ -1111,6 +1132,15 @@ extern "C" int32_t hpaFusedGroupAggHost(int32_t nk, hpa_V *keys, int32_t nx, hpa
     if(cap < 0 || !hkeys || !haggs || !ng || (nk == 2 && !hkeys[1])) R HPA_E_DOMAIN;
     FgaHost hd{cap, hkeys, zfirst ? hfirst : nullptr, haggs, ng};
     R fga_run(nk, keys, nx, x, cond, na, aggs, zfirst, zkeys, zaggs, &hd);
+}
+
+extern "C" int32_t hpaFusedGroupAggBuckets(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_expr *cond,
+                                           const hpa_expr *bucket, int32_t bbits, int32_t na, const hpa_agg *aggs,
+                                           hpa_V *zkeys, hpa_V *zaggs, int64_t cap, void **hkeys, uint8_t *hbucket,
+                                           void **haggs, int64_t *ng){
+    if(!bucket || bbits < 1 || bbits > 2 || cap < 0 || !hkeys || !haggs || !ng || !hbucket || (nk == 2 && !hkeys[1])) R HPA_E_DOMAIN;
+    FgaHost hd{cap, hkeys, nullptr, haggs, ng, hbucket};
+    R fga_run(nk, keys, nx, x, cond, na, aggs, nullptr, zkeys, zaggs, &hd, bucket, bbits);
 }
 
 // Diagnostic for hosts without a GPU: emit the generated sources of one fused group-aggregate,

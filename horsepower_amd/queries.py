@@ -162,6 +162,69 @@ def q1_chf(B, T, ship_max=D.Q1_SHIP_MAX):
     return dict(zip(Q1_NAMES, keys + aggs))
 
 
+@functools.lru_cache(maxsize=64)
+def q1_many_spec(cuts):
+    """Condition, bucket and sums of a shared Q1 pass over ascending distinct cutoffs `cuts`
+    (x = ship_day, quantity, price, discount, tax): rows with ship_day <= the largest cutoff, in
+    bucket = the number of cutoffs below their ship_day; the sums every Q1 output derives from."""
+    cond = expr([("COL", 0), ("I", cuts[-1]), "LEQ"])
+    prog = [("I", 0)]
+    for c in cuts[:-1]:
+        prog += [("COL", 0), ("I", c), "GT", "ADD"]
+    bucket = expr(prog)
+    qty, price, disc = expr([("COL", 1)]), expr([("COL", 2)]), expr([("COL", 3)])
+    dprice = expr([("COL", 2), ("I", 1), ("COL", 3), "SUB", "MUL"])
+    charge = expr([("COL", 2), ("I", 1), ("COL", 3), "SUB", "MUL", ("I", 1), ("COL", 4), "ADD", "MUL"])
+    spec = [("sum", qty), ("sum", price), ("sum", dprice), ("sum", charge), ("sum", disc), ("len", None)]
+    return cond, bucket, spec
+
+
+Q1_MANY_MAX = 4
+
+
+def q1_many(B, T, params):
+    """Several Q1 requests (one dict with ship_max each; at most Q1_MANY_MAX distinct cutoffs) in
+    one pass over the table: the pass groups on (bucket, flag, status), the bucket being how many
+    of the cutoffs lie below a row's ship_day, and each request adds up the buckets at or below
+    its own cutoff -- integer sums and counts exactly, FP64 sums with math.fsum of the bucket
+    sums -- and divides for the averages as q1_chf's kernel does.  Groups come out in (flag,
+    status) order, the order q1_chf returns."""
+    import math
+    cuts = tuple(sorted(set(int(p.get("ship_max", D.Q1_SHIP_MAX)) for p in params)))
+    if len(cuts) > Q1_MANY_MAX:
+        raise ValueError(f"q1_many: {len(cuts)} distinct cutoffs (at most {Q1_MANY_MAX})")
+    if len(cuts) == 1:
+        r = q1_chf(B, T, cuts[0])
+        return [dict(r) for _ in params]
+    cond, bucket, spec = q1_many_spec(cuts)
+    xs = [T["ship_day"], T["quantity"], T["price"], T["discount"], T["tax"]]
+    bbits = 1 if len(cuts) <= 2 else 2
+    (kf, ks), bk, (sq, sp, sdp, sc, sd, cnt) = B.fused_group_agg_buckets([T["return_flag"], T["line_status"]], xs, cond,
+                                                                        bucket, bbits, spec)
+    out = []
+    for p in params:
+        lim = cuts.index(int(p.get("ship_max", D.Q1_SHIP_MAX)))
+        acc = {}
+        for g in range(bk.size):
+            if bk[g] > lim:
+                continue
+            a = acc.setdefault((int(kf[g]), int(ks[g])), [0, [], [], [], [], 0])
+            a[0] += int(sq[g]); a[1].append(float(sp[g])); a[2].append(float(sdp[g])); a[3].append(float(sc[g]))
+            a[4].append(float(sd[g])); a[5] += int(cnt[g])
+        # (flag, status) in the fused kernel's key order: each char compared as a signed byte
+        keys = sorted(acc, key=lambda k: ((k[0] ^ 0x80) << 8) | (k[1] ^ 0x80))
+        res = {n: [] for n in Q1_NAMES}
+        for k in keys:
+            a = acc[k]
+            n = a[5]
+            s_qty, s_p, s_dp, s_c, s_d = a[0], math.fsum(a[1]), math.fsum(a[2]), math.fsum(a[3]), math.fsum(a[4])
+            for name, v in zip(Q1_NAMES, (k[0], k[1], s_qty, s_p, s_dp, s_c, float(s_qty) / n, s_p / n, s_d / n, n)):
+                res[name].append(v)
+        dts = {"return_flag": kf.dtype, "line_status": ks.dtype, "sum_qty": np.int64, "count_order": np.int64}
+        out.append({n: np.array(v, dtype=dts.get(n, np.float64)) for n, v in res.items()})
+    return out
+
+
 # ------------------------------------------------------------------------------------- Q3
 Q3_NAMES = ("orderkey", "revenue", "order_day")
 

@@ -96,6 +96,9 @@ class Device(PrimitiveLib):
         _bind(dll, "hpaFusedGroupAggHost", i32, [i32, C.POINTER(V), i32, C.POINTER(V), C.POINTER(Expr), i32,
                                                  vp, V, C.POINTER(V), C.POINTER(V), i64, C.POINTER(vp),
                                                  C.POINTER(i64), C.POINTER(vp), C.POINTER(i64)])
+        _bind(dll, "hpaFusedGroupAggBuckets", i32, [i32, C.POINTER(V), i32, C.POINTER(V), C.POINTER(Expr), C.POINTER(Expr), i32,
+                                                    i32, vp, C.POINTER(V), C.POINTER(V), i64, C.POINTER(vp), vp,
+                                                    C.POINTER(vp), C.POINTER(i64)])
         _bind(dll, "hpaJitCompileCheck", i32, [i32, C.POINTER(i32), C.POINTER(Expr), i32, vp])
         _bind(dll, "hpaBatchCreate", vp, [i32, C.POINTER(V), C.POINTER(V), C.POINTER(V),
                                           C.POINTER(i32), C.POINTER(i32)])

@@ -11,7 +11,9 @@ carved from one device pool.
   fit yet, but not once the head of the queue has waited longer than `max_skip_s`.
 * Continuous batching.  Requests that arrive while others run are admitted as slots and memory
   free up.  Waiting Q6 requests are merged (up to 8) into one shared scan
-  (hpaFusedReduceMany): one pass over the fact columns answers all of them.
+  (hpaFusedReduceMany): one pass over the fact columns answers all of them.  Waiting Q1
+  requests with up to four distinct ship-date cutoffs are merged into one bucketed group-by
+  pass (hpaFusedGroupAggBuckets, queries.q1_many).
 * Overlap.  Each slot's worker thread issues its request's primitives on its own stream;
   ctypes releases the GIL inside every library call, so several requests are on the device
   at once.
@@ -131,7 +133,7 @@ class FirstFit:
 
 class Server:
     def __init__(self, B, tables, slots=4, pool_bytes=64 << 30, merge=True, max_skip_s=0.05,
-                 heavy=("q3",), heavy_slots=1, heavy_share=None, heavy_fifo=False):
+                 heavy=("q3",), heavy_slots=1, heavy_share=None, heavy_fifo=False, late_share=0):
         self.B, self.lib, self.T = B, B.lib, tables
         self.slots, self.merge, self.max_skip_s = slots, merge, max_skip_s
         self.base = self.lib.dll.hpaDeviceAlloc(pool_bytes)
@@ -148,6 +150,8 @@ class Server:
         # heavy_fifo: a short request does not run beside a heavy one that arrived before it (as
         # in arrival order, it waits for that one to finish); shorts that arrived earlier still do
         self.heavy_fifo = heavy_fifo
+        # with heavy_fifo: how many such later short jobs may still run beside the heavy ones
+        self.late_share = late_share
         self.ctx = [self.lib.dll.hpaCtxCreate(-1) for _ in range(slots)]
         if not all(self.ctx):
             raise HorseError(E_MEMORY, "hpaCtxCreate", self.lib.detail())
@@ -170,6 +174,8 @@ class Server:
 
     def _execute(self, job):
         B, T = self.B, self.T
+        if job[0].kind == "q1" and len(job) > 1:
+            return Q.q1_many(B, T["fact"], [r.params for r in job])
         if len(job) > 1 or (job[0].kind == "q6" and self.merge):
             return Q.q6_many(B, T["fact"], [r.params for r in job])
         r = job[0]
@@ -190,6 +196,12 @@ class Server:
                 self.lib.rc("hpaCtxSetArena", ctx, C.c_void_p(self.base), self.pool_bytes)
                 Q.q6_many(self.B, self.T["fact"], [cases[0][1]] * k)
                 self.est["q6"] = max(self.est["q6"], self.lib.dll.hpaCtxArenaPeak(ctx))
+            if self.merge:
+                # the shared Q1 passes for two to four cutoffs (their kernels compiled here too)
+                for k in range(2, Q.Q1_MANY_MAX + 1):
+                    self.lib.rc("hpaCtxSetArena", ctx, C.c_void_p(self.base), self.pool_bytes)
+                    Q.q1_many(self.B, self.T["fact"], [dict(ship_max=2300 + 50 * j) for j in range(k)])
+                    self.est["q1"] = max(self.est["q1"], self.lib.dll.hpaCtxArenaPeak(ctx))
         finally:
             self.lib.dll.hpaSync()
             self.lib.dll.hpaCtxSetCurrent(None)
@@ -207,13 +219,13 @@ class Server:
         cv = threading.Condition()
         free_slots = list(range(self.slots))
         pending = collections.deque()
-        state = dict(done=0, error=None, heavy=0, short=0)
+        state = dict(done=0, error=None, heavy=0, short=0, late=0)
         heavy_running = []                                   # arrival times of running heavy requests
         N = len(reqs)
         nxt = 0
         t0 = time.perf_counter()
 
-        def worker(slot, job, off, size):
+        def worker(slot, job, off, size, late):
             err = None
             try:
                 self._bind(slot, off, size)
@@ -234,6 +246,7 @@ class Server:
                     heavy_running.remove(job[0].arrival)
                 else:
                     state["short"] -= 1
+                    state["late"] -= late
                 if err is None:
                     state["done"] += len(job)
                 elif err.code == E_MEMORY:
@@ -263,7 +276,8 @@ class Server:
                         if (r.kind not in self.heavy and self.heavy_share is not None and state["heavy"] > 0
                                 and state["short"] >= self.heavy_share):
                             continue
-                        if r.kind not in self.heavy and self.heavy_fifo and heavy_running and min(heavy_running) < r.arrival:
+                        if (r.kind not in self.heavy and self.heavy_fifo and heavy_running and min(heavy_running) < r.arrival
+                                and state["late"] >= self.late_share):
                             continue
                         if head is not None and now - head.arrival > self.max_skip_s:
                             break                            # it waits for memory: no overtaking
@@ -279,25 +293,38 @@ class Server:
                     r = pending[i]
                     del pending[i]
                     job = [r]
-                    if self.merge and r.kind == "q6":
+                    if self.merge and r.kind in ("q6", "q1"):
+                        # Q6 requests share one scan (up to 8); Q1 requests one bucketed pass
+                        # (up to Q1_MANY_MAX distinct cutoffs)
+                        cap = 8 if r.kind == "q6" else Q.Q1_MANY_MAX
+                        cuts = {r.params.get("ship_max")}
                         for q in list(pending):
-                            if len(job) == 8:
+                            if len(job) == cap:
                                 break
-                            if self.heavy_fifo and heavy_running and min(heavy_running) < q.arrival:
+                            if (self.heavy_fifo and heavy_running and min(heavy_running) < q.arrival
+                                    and not (min(heavy_running) < r.arrival)):
+                                continue                     # an on-time job takes no late request along
+                            if q.kind != r.kind:
                                 continue
-                            if q.kind == "q6":
-                                pending.remove(q)
-                                job.append(q)
+                            if r.kind == "q1":
+                                if len(cuts | {q.params.get("ship_max")}) > Q.Q1_MANY_MAX:
+                                    continue
+                                cuts.add(q.params.get("ship_max"))
+                            pending.remove(q)
+                            job.append(q)
                     for q in job:
                         q.admit = now
+                    late = int(r.kind not in self.heavy and self.heavy_fifo and bool(heavy_running)
+                               and min(heavy_running) < r.arrival)
                     if r.kind in self.heavy:
                         state["heavy"] += 1
                         heavy_running.append(r.arrival)
                     else:
                         state["short"] += 1
+                        state["late"] += late
                     self.peak_in_use = max(self.peak_in_use, pool.in_use())
                     slot = free_slots.pop()
-                    th = threading.Thread(target=worker, args=(slot, job, off, size), daemon=True)
+                    th = threading.Thread(target=worker, args=(slot, job, off, size, late), daemon=True)
                     threads.append(th)
                     th.start()
                 if state["done"] >= N:

@@ -317,6 +317,17 @@ int32_t hpaFusedGroupAgg(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hp
 int32_t hpaFusedGroupAggHost(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_expr *cond,
                              int32_t na, const hpa_agg *aggs, hpa_V zfirst, hpa_V *zkeys, hpa_V *zaggs,
                              int64_t cap, void **hkeys, int64_t *hfirst, void **haggs, int64_t *ng);
+/* hpaFusedGroupAggHost over groups (bucket, keys): `bucket` is an expression over x giving each
+ * row a bucket number below 2^bbits (bbits 1 or 2; the value is taken modulo 2^bbits), and
+ * hbucket (cap bytes) receives each group's bucket beside its key bytes; groups come out in
+ * (bucket, keys) order.  Requests that differ only in one range bound of a column share one
+ * pass this way: with bucket = the number of bounds below the row's value, a request's groups
+ * are the sum of the buckets up to its bound.  No counterpart in the reference: it is the
+ * serving loop's shared scan. */
+int32_t hpaFusedGroupAggBuckets(int32_t nk, hpa_V *keys, int32_t nx, hpa_V *x, const hpa_expr *cond,
+                                const hpa_expr *bucket, int32_t bbits, int32_t na, const hpa_agg *aggs,
+                                hpa_V *zkeys, hpa_V *zaggs, int64_t cap, void **hkeys, uint8_t *hbucket,
+                                void **haggs, int64_t *ng);
 /* Diagnostic (no device needed): generate the fused group-aggregate kernel for these column
  * types and expressions and compile it with hipRTC; 0 when the generated code compiles. */
 int32_t hpaJitCompileCheck(int32_t nx, const int32_t *coltyp, const hpa_expr *cond,

@@ -200,3 +200,27 @@ def test_compress_map(gpu, orc, n, sel):
     assert np.array_equal(gpu.read(zk), ok)
     assert np.array_equal(gpu.read(zr), orv)
     assert np.array_equal(gpu.read(zb), (q[m != 0] < 24).astype(np.uint8))
+
+
+@pytest.mark.parametrize("cuts", [(2300, 2434), (2434, 2300, 2300), (1000, 1263, 1264, 2500), (2600, -1), (2434,)])
+def test_q1_many(gpu, orc, cuts):
+    """Several Q1 requests in one bucketed pass (the serving loop's shared Q1 scan) equal each
+    request run alone (q1_chf) and the oracle's interpreter sequence: keys and counts exactly,
+    FP64 within 1e-12.  Cutoffs repeat, straddle the status boundary (1263), lie past every row
+    and below every row (no group)."""
+    t = D.gen_q1(700_001, seed=1009)
+    Tg, To = Q.load_table(gpu, t), Q.load_table(orc, t)
+    m = gpu.mark()
+    many = Q.q1_many(gpu, Tg, [dict(ship_max=c) for c in cuts])
+    for c, r in zip(cuts, many):
+        alone = Q.q1_chf(gpu, Tg, c)
+        for k in Q.Q1_NAMES:
+            close(r[k], alone[k], (c, k))
+        if c < t["ship_day"].min():
+            # no row selected: no group (the interpreter's raze of an empty group list is an error)
+            assert all(np.asarray(r[k]).size == 0 for k in Q.Q1_NAMES)
+            continue
+        o = Q.q1_plain(orc, To, c)
+        for k in Q.Q1_NAMES:
+            close(r[k], o[k], (c, k))
+    gpu.release(m)
