@@ -61,7 +61,7 @@ def measured_traffic(workload, rows, kernel):
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="all", choices=["all", "q1", "q6", "q3", "batch", "prims", "serve", "copy", "ingest"])
     p.add_argument("--rows", type=int, default=0)
@@ -97,9 +97,18 @@ def workload(name, rows):
 
 
 def run_steps(B, plan, tables, steps, mark):
-    for _ in range(steps):
-        plan(B, *tables)
-        B.release(mark)
+    # the collector stays off while steps run (as timeit does): a collection inside a short timed
+    # region shows up as host time of the step it lands in
+    import gc
+    was = gc.isenabled()
+    gc.disable()
+    try:
+        for _ in range(steps):
+            plan(B, *tables)
+            B.release(mark)
+    finally:
+        if was:
+            gc.enable()
 
 
 def free_tables(B, tables):
