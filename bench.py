@@ -155,7 +155,10 @@ def bench_query(a, B, name, rows=0, tables=None, desc=None, cpu=True, plain=True
     ms = dt * 1000.0 / a.steps
     print(f"[bench] {name} {nrows} rows: {a.steps} steps, {ms:.3f} ms/step", file=sys.stderr, flush=True)
     kern = profile_kernels(lib, lambda: run_steps(B, plan, tables, 1, mark), a.profile_steps)
-    dom = max(kern, key=lambda k: kern[k]["ms"]) if kern else None
+    # the dominant kernel scope among those with algorithmic bytes (a scope without them, such as
+    # the reference-order quicksort, has no roofline)
+    withb = {k: v for k, v in kern.items() if v["bytes"] > 0}
+    dom = max(withb, key=lambda k: withb[k]["ms"]) if withb else (max(kern, key=lambda k: kern[k]["ms"]) if kern else None)
     roof = None
     if dom:
         roof = roofline_of(kern, dom)
