@@ -45,6 +45,7 @@
 #define PART_MAX 4096          // partitions per query (LDS counters of the count and scatter kernels)
 #define PART_MIN_KEYS 1024     // smallest partition width the tunables may choose
 #define PART_MIN_TILE 8192     // smallest row tile the tunables may choose (sizes the count matrix)
+#define BATCH_CTRS 4           // queue counters of a run
 
 struct BQuery {
     const int32_t *key; const double *val; const int32_t *flt;
@@ -59,13 +60,14 @@ struct BQuery {
     int64_t *poff;                                            // exclusive scan of pcnt [nparts * ntiles + 1]
     uint16_t *skey; double *sval;                             // partition-contiguous rows (capacity n)
     int64_t pitem0;
+    int64_t stile0;                                           // first scatter item (tiles of partitioned queries)
     unsigned long long *pcsum;                                // per PS_CHUNK-entry chunk of pcnt: its sum
     int64_t pch0;                                             // first scan chunk of the query
 };
 
 struct BPlan {
     BQuery *q; int nq;
-    int64_t total_tiles, total_etiles, total_pitems;
+    int64_t total_tiles, total_etiles, total_pitems, total_stiles;
     unsigned *ctr;                                            // [0] agg [1] emit [2] scatter [3] pagg
     uint64_t *estatus;                                        // emit look-back, per slot tile of all queries
     int32_t *gkey; double *gsum; int64_t *gcnt;               // groups of every query, query after query
@@ -74,9 +76,9 @@ struct BPlan {
     int64_t total_pchunks;                                    // scan chunks of all partitioned queries
 };
 
-enum { SEL_TILE, SEL_EMIT, SEL_PART };
+enum { SEL_TILE, SEL_EMIT, SEL_PART, SEL_STILE };
 __device__ __forceinline__ int64_t item0_of(const BQuery &q, int sel){
-    return sel == SEL_TILE ? q.tile0 : sel == SEL_EMIT ? q.etile0 : q.pitem0;
+    return sel == SEL_TILE ? q.tile0 : sel == SEL_EMIT ? q.etile0 : sel == SEL_PART ? q.pitem0 : q.stile0;
 }
 // the query owning queue item t: the last query whose first item is <= t
 __device__ __forceinline__ int find_query(const BQuery *q, int nq, int64_t t, int sel){
@@ -84,6 +86,31 @@ __device__ __forceinline__ int find_query(const BQuery *q, int nq, int64_t t, in
     while(lo < hi){
         int mid = (lo + hi + 1) >> 1;
         if(item0_of(q[mid], sel) <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// A query's columns and slot arrays are reached through its descriptor in device memory, so the
+// compiler cannot tell that they are global memory and would issue flat accesses, each waiting
+// for the one before; these views say that they are.  Row loads go to a clamped index (the last
+// row of the range) instead of under a branch, so a thread's loads are all in flight together.
+template<typename T> using gptr = T __attribute__((address_space(1))) *;
+template<typename T> __device__ __forceinline__ gptr<T> G(T *p){ return (gptr<T>)p; }
+
+// The persistent kernels look an item's query up in an LDS copy of every query's first item
+// (up to QTAB queries; more are searched in the descriptors in global memory), filled before the
+// barrier of the first next_item.  A query with no items of a queue shares its first item with
+// the next query and never owns one: the search takes the last query whose first item is <= t.
+#define QTAB 256
+__device__ __forceinline__ void load_qtab(const BPlan &p, int sel, int64_t *t0){
+    if(p.nq <= QTAB) for(int k = threadIdx.x; k < p.nq; k += blockDim.x) t0[k] = item0_of(p.q[k], sel);
+}
+__device__ __forceinline__ int find_query_tab(const BPlan &p, const int64_t *t0, int64_t t, int sel){
+    if(p.nq > QTAB) return find_query(p.q, p.nq, t, sel);
+    int lo = 0, hi = p.nq - 1;
+    while(lo < hi){
+        const int mid = (lo + hi + 1) >> 1;
+        if(t0[mid] <= t) lo = mid; else hi = mid - 1;
     }
     return lo;
 }
@@ -112,6 +139,7 @@ __device__ __forceinline__ int64_t next_item(unsigned *ctr, int64_t *s_item){
     __syncthreads();
     return item;
 }
+
 
 __global__ void k_batch_minmax(BQuery *qs, int nq, int32_t *mm){
     __shared__ int32_t smin[4], smax[4];
@@ -144,18 +172,20 @@ __global__ void k_batch_clear(BPlan p){
         if(blockIdx.x == 0 && threadIdx.x < 2) q.ng[threadIdx.x] = 0;
     }
     if(blockIdx.y == 0) for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < p.total_etiles; i += stride) p.estatus[i] = 0;
-    if(blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 4) p.ctr[threadIdx.x] = 0;
+    if(blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < BATCH_CTRS) p.ctr[threadIdx.x] = 0;
 }
 
 extern __shared__ double g_batch_lds[];
 
 __global__ void k_batch_agg(BPlan p){
     __shared__ int64_t s_item;
+    __shared__ int64_t s_t0[QTAB];
     const int nw = blockDim.x >> 6, w = threadIdx.x >> 6;
+    load_qtab(p, SEL_TILE, s_t0);
     while(true){
         const int64_t item = next_item(&p.ctr[0], &s_item);
         if(item >= p.total_tiles) break;
-        const BQuery &q = p.q[find_query(p.q, p.nq, item, SEL_TILE)];
+        const BQuery &q = p.q[find_query_tab(p, s_t0, item, SEL_TILE)];
         const int64_t r0 = (item - q.tile0) * p.tile_rows;
         const int64_t r1 = r0 + p.tile_rows < q.n ? r0 + p.tile_rows : q.n;
         const int32_t kmin = q.kmin, op = q.op, thr = q.thr;
@@ -168,10 +198,17 @@ __global__ void k_batch_agg(BPlan p){
             __syncthreads();
             // four rows per thread per step, their loads issued together
             const int64_t step = (int64_t)blockDim.x;
+            const gptr<const int32_t> flt = G(q.flt), key = G(q.key);
+            // the next step's rows are loaded before this step's counts (clamped to the last row)
+            int32_t nf[4], nk[4];
+            #pragma unroll
+            for(int u = 0; u < 4; u++){ const int64_t i = r0 + threadIdx.x + u * step, ic = i < r1 ? i : r1 - 1; nf[u] = flt[ic]; nk[u] = key[ic]; }
             for(int64_t i0 = r0 + threadIdx.x; i0 < r1; i0 += 4 * step){
                 int32_t f[4], k[4];
                 #pragma unroll
-                for(int u = 0; u < 4; u++){ const int64_t i = i0 + u * step; f[u] = i < r1 ? q.flt[i] : 0; k[u] = i < r1 ? q.key[i] : 0; }
+                for(int u = 0; u < 4; u++){ f[u] = nf[u]; k[u] = nk[u]; }
+                #pragma unroll
+                for(int u = 0; u < 4; u++){ const int64_t i = i0 + (4 + u) * step, ic = i < r1 ? i : r1 - 1; nf[u] = flt[ic]; nk[u] = key[ic]; }
                 #pragma unroll
                 for(int u = 0; u < 4; u++){
                     if(i0 + u * step < r1 && batch_pass(op, f[u], thr)){
@@ -182,7 +219,8 @@ __global__ void k_batch_agg(BPlan p){
             }
             __syncthreads();
             const int64_t tile = item - q.tile0;
-            for(int k = threadIdx.x; k < q.nparts; k += blockDim.x) q.pcnt[(int64_t)k * q.ntiles + tile] = lc[k];
+            const gptr<unsigned> pcnt = G(q.pcnt);
+            for(int k = threadIdx.x; k < q.nparts; k += blockDim.x) pcnt[(int64_t)k * q.ntiles + tile] = lc[k];
             __syncthreads();
         } else if(q.range <= p.lds_keys){
             int reps = p.lds_keys / q.range; reps = reps > nw ? nw : reps;
@@ -193,24 +231,39 @@ __global__ void k_batch_agg(BPlan p){
             __syncthreads();
             const int base = (w % reps) * q.range;
             const int64_t step = (int64_t)blockDim.x;
+            const gptr<const int32_t> flt = G(q.flt), key = G(q.key);
+            const gptr<const double> val = G(q.val);
+            // the next step's rows are loaded before this step's adds (clamped to the last row)
+            int32_t nf[4], nk[4]; double nv[4];
+            #pragma unroll
+            for(int u = 0; u < 4; u++){
+                const int64_t i = r0 + threadIdx.x + u * step, ic = i < r1 ? i : r1 - 1;
+                nf[u] = flt[ic]; nk[u] = key[ic]; nv[u] = val[ic];
+            }
             for(int64_t i0 = r0 + threadIdx.x; i0 < r1; i0 += 4 * step){
                 int32_t f[4], k[4]; double v[4];
                 #pragma unroll
+                for(int u = 0; u < 4; u++){ f[u] = nf[u]; k[u] = nk[u]; v[u] = nv[u]; }
+                #pragma unroll
                 for(int u = 0; u < 4; u++){
-                    const int64_t i = i0 + u * step;
-                    const bool in = i < r1;
-                    f[u] = in ? q.flt[i] : 0; k[u] = in ? q.key[i] : 0; v[u] = in ? q.val[i] : 0.0;
+                    const int64_t i = i0 + (4 + u) * step, ic = i < r1 ? i : r1 - 1;
+                    nf[u] = flt[ic]; nk[u] = key[ic]; nv[u] = val[ic];
+                }
+                // the step's returning adds first, then their rounding errors and counts
+                bool ok[4]; uint32_t sl[4]; double old[4];
+                #pragma unroll
+                for(int u = 0; u < 4; u++){
+                    sl[u] = (uint32_t)k[u] - (uint32_t)kmin;
+                    const bool pass = i0 + u * step < r1 && batch_pass(op, f[u], thr);
+                    ok[u] = pass && sl[u] < range;
+                    if(pass && !ok[u]) bad = true;
+                    if(ok[u]) old[u] = atomicAdd(&lhi[base + sl[u]], v[u]);
                 }
                 #pragma unroll
                 for(int u = 0; u < 4; u++){
-                    if(i0 + u * step < r1 && batch_pass(op, f[u], thr)){
-                        const uint32_t s = (uint32_t)k[u] - (uint32_t)kmin;
-                        if(s < range){
-                            double old = atomicAdd(&lhi[base + s], v[u]);
-                            atomicAdd(&llo[base + s], two_sum_err(old, v[u], old + v[u]));
-                            atomicAdd(&lcnt[base + s], 1u);
-                        } else bad = true;
-                    }
+                    if(!ok[u]) continue;
+                    atomicAdd(&llo[base + sl[u]], two_sum_err(old[u], v[u], old[u] + v[u]));
+                    atomicAdd(&lcnt[base + sl[u]], 1u);
                 }
             }
             __syncthreads();
@@ -228,11 +281,13 @@ __global__ void k_batch_agg(BPlan p){
             }
             __syncthreads();
         } else {
+            const gptr<const int32_t> flt = G(q.flt), key = G(q.key);
+            const gptr<const double> val = G(q.val);
             for(int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x){
-                if(batch_pass(op, q.flt[i], thr)){
-                    uint32_t s = (uint32_t)q.key[i] - (uint32_t)kmin;
+                if(batch_pass(op, flt[i], thr)){
+                    uint32_t s = (uint32_t)key[i] - (uint32_t)kmin;
                     if(s < range){
-                        double v = q.val[i];
+                        double v = val[i];
                         double old = unsafeAtomicAdd(&q.hi[s], v);
                         unsafeAtomicAdd(&q.lo[s], two_sum_err(old, v, old + v));
                         atomicAdd(&q.cnt[s], 1u);
@@ -267,9 +322,13 @@ __global__ void __launch_bounds__(1024) k_batch_psum(BPlan p){
     const BQuery &q = p.q[find_pchunk(p, blockIdx.x)];
     const int64_t ch = (int64_t)blockIdx.x - q.pch0, N = (int64_t)q.nparts * q.ntiles, r0 = ch * PS_CHUNK;
     const int t = threadIdx.x;
+    const gptr<const unsigned> pcnt = G((const unsigned*)q.pcnt);
     unsigned long long s = 0;
+    unsigned v[PS_PER];
     #pragma unroll
-    for(int k = 0; k < PS_PER; k++){ const int64_t i = r0 + (int64_t)k * 1024 + t; s += i < N ? q.pcnt[i] : 0u; }
+    for(int k = 0; k < PS_PER; k++){ const int64_t i = r0 + (int64_t)k * 1024 + t; v[k] = pcnt[i < N ? i : N - 1]; }
+    #pragma unroll
+    for(int k = 0; k < PS_PER; k++) s += r0 + (int64_t)k * 1024 + t < N ? v[k] : 0u;
     for(int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
     if((t & 63) == 0) s_w[t >> 6] = s;
     __syncthreads();
@@ -282,9 +341,13 @@ __global__ void __launch_bounds__(1024) k_batch_pscan(BPlan p){
     const int64_t ch = (int64_t)blockIdx.x - q.pch0, N = (int64_t)q.nparts * q.ntiles, r0 = ch * PS_CHUNK;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     if(t == 0){ unsigned long long a = 0; for(int64_t k = 0; k < ch; k++) a += q.pcsum[k]; s_carry = a; }
+    const gptr<const unsigned> pcnt = G((const unsigned*)q.pcnt);
+    const gptr<int64_t> poff = G(q.poff);
     unsigned v[PS_PER]; unsigned long long s = 0;
     #pragma unroll
-    for(int k = 0; k < PS_PER; k++){ const int64_t i = r0 + (int64_t)t * PS_PER + k; v[k] = i < N ? q.pcnt[i] : 0u; s += v[k]; }
+    for(int k = 0; k < PS_PER; k++){ const int64_t i = r0 + (int64_t)t * PS_PER + k; v[k] = pcnt[i < N ? i : N - 1]; }
+    #pragma unroll
+    for(int k = 0; k < PS_PER; k++){ if(r0 + (int64_t)t * PS_PER + k >= N) v[k] = 0u; s += v[k]; }
     unsigned long long inc = s;
     for(int o = 1; o < 64; o <<= 1){ const unsigned long long y = __shfl_up(inc, o, 64); if(lane >= o) inc += y; }
     if(lane == 63) s_w[w] = inc;
@@ -292,8 +355,8 @@ __global__ void __launch_bounds__(1024) k_batch_pscan(BPlan p){
     unsigned long long pre = s_carry + inc - s, tot = 0;
     for(int k = 0; k < 16; k++){ if(k < w) pre += s_w[k]; tot += s_w[k]; }
     #pragma unroll
-    for(int k = 0; k < PS_PER; k++){ const int64_t i = r0 + (int64_t)t * PS_PER + k; if(i < N) q.poff[i] = (int64_t)pre; pre += v[k]; }
-    if(t == 0 && r0 + PS_CHUNK >= N) q.poff[N] = (int64_t)(s_carry + tot);
+    for(int k = 0; k < PS_PER; k++){ const int64_t i = r0 + (int64_t)t * PS_PER + k; if(i < N) poff[i] = (int64_t)pre; pre += v[k]; }
+    if(t == 0 && r0 + PS_CHUNK >= N) poff[N] = (int64_t)(s_carry + tot);
 }
 
 // Scatter of a partitioned query's row tile, staged through LDS so that the writes go out as
@@ -304,44 +367,65 @@ __global__ void __launch_bounds__(1024) k_batch_pscan(BPlan p){
 // partition's run -- at each partition's running output position (its tile base from the
 // (partition, tile) scan plus the rows of earlier chunks).  The next chunk continues the same
 // runs, so a partition's bytes reach L2 in order from one workgroup instead of one 2-byte key
-// and one 8-byte value per lane to as many lines.
+// and one 8-byte value per lane to as many lines.  The queue holds the partitioned queries'
+// tiles only.  One workgroup per CU: the next chunk's filter and key columns are loaded while
+// the current chunk is ranked and written, and its values with its filter.
 #define SC_BLOCK 1024
 #define SC_ROWS 8
 #define SC_CHUNK (SC_ROWS * SC_BLOCK)
 __global__ void __launch_bounds__(SC_BLOCK) k_batch_scatter(BPlan p){
-    // 144 KB of LDS: one workgroup per CU
+    // 146 KB of LDS: one workgroup per CU
     __shared__ int64_t s_item;
+    __shared__ int64_t s_t0[QTAB];
     __shared__ uint32_t s_wsum[SC_BLOCK / 64];
     __shared__ int64_t pos0[PART_MAX];                                   // next output place per partition
     __shared__ uint32_t coff[PART_MAX];                                  // chunk counts, then offsets
     __shared__ double st_val[SC_CHUNK];
     __shared__ uint16_t st_key[SC_CHUNK], st_part[SC_CHUNK];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    load_qtab(p, SEL_STILE, s_t0);
     while(true){
         const int64_t item = next_item(&p.ctr[2], &s_item);
-        if(item >= p.total_tiles) break;
-        const BQuery &q = p.q[find_query(p.q, p.nq, item, SEL_TILE)];
+        if(item >= p.total_stiles) break;
+        const BQuery &q = p.q[find_query_tab(p, s_t0, item, SEL_STILE)];
         if(!q.part) continue;                                            // uniform over the workgroup
-        const int64_t tile = item - q.tile0;
+        const int64_t tile = item - q.stile0;
         const int64_t r0 = tile * p.tile_rows;
         const int64_t r1 = r0 + p.tile_rows < q.n ? r0 + p.tile_rows : q.n;
         const uint32_t range = (uint32_t)q.range, mask = (uint32_t)p.part_keys - 1u;
+        const gptr<const int32_t> flt = G(q.flt), key = G(q.key);
+        const gptr<const double> val = G(q.val);
+        const gptr<uint16_t> skey = G(q.skey);
+        const gptr<double> sval = G(q.sval);
+        const gptr<const int64_t> poff = G((const int64_t*)q.poff);
+        const int32_t op = q.op, thr = q.thr, kmin = q.kmin;
         const int P = q.nparts;
-        for(int k = threadIdx.x; k < P; k += SC_BLOCK) pos0[k] = q.poff[(int64_t)k * q.ntiles + tile];
+        for(int k = threadIdx.x; k < P; k += SC_BLOCK) pos0[k] = poff[(int64_t)k * q.ntiles + tile];
+        int32_t nf[SC_ROWS], nk[SC_ROWS];
+        #pragma unroll
+        for(int u = 0; u < SC_ROWS; u++){
+            const int64_t i = r0 + (int64_t)u * SC_BLOCK + threadIdx.x, ic = i < r1 ? i : r1 - 1;
+            nf[u] = flt[ic]; nk[u] = key[ic];
+        }
         for(int64_t c0 = r0; c0 < r1; c0 += SC_CHUNK){
             for(int k = threadIdx.x; k < P; k += SC_BLOCK) coff[k] = 0u;
-            __syncthreads();
             uint32_t s[SC_ROWS], rk[SC_ROWS]; double v[SC_ROWS]; bool ok[SC_ROWS];
             #pragma unroll
             for(int u = 0; u < SC_ROWS; u++){
                 const int64_t i = c0 + (int64_t)u * SC_BLOCK + threadIdx.x;
-                ok[u] = false; s[u] = 0; v[u] = 0.0;
-                if(i < r1 && batch_pass(q.op, q.flt[i], q.thr)){
-                    s[u] = (uint32_t)q.key[i] - (uint32_t)q.kmin;
-                    ok[u] = s[u] < range;
-                    if(ok[u]) v[u] = q.val[i];
-                }
+                s[u] = (uint32_t)nk[u] - (uint32_t)kmin;
+                ok[u] = i < r1 && batch_pass(op, nf[u], thr) && s[u] < range;
+                v[u] = val[i < r1 ? i : r1 - 1];
             }
+            // the next chunk's filter and keys, in flight during this chunk's LDS work (the
+            // last chunk loads its own last row again)
+            const int64_t c1 = c0 + SC_CHUNK;
+            #pragma unroll
+            for(int u = 0; u < SC_ROWS; u++){
+                const int64_t i = c1 + (int64_t)u * SC_BLOCK + threadIdx.x, ic = i < r1 ? i : r1 - 1;
+                nf[u] = flt[ic]; nk[u] = key[ic];
+            }
+            __syncthreads();
             #pragma unroll
             for(int u = 0; u < SC_ROWS; u++) rk[u] = ok[u] ? atomicAdd(&coff[s[u] >> p.part_shift], 1u) : 0u;
             __syncthreads();
@@ -368,7 +452,7 @@ __global__ void __launch_bounds__(SC_BLOCK) k_batch_scatter(BPlan p){
             for(uint32_t j = threadIdx.x; j < total; j += SC_BLOCK){
                 const uint32_t k = st_part[j];
                 const int64_t pos = pos0[k] + (int64_t)(j - coff[k]);
-                q.skey[pos] = st_key[j]; q.sval[pos] = st_val[j];
+                skey[pos] = st_key[j]; sval[pos] = st_val[j];
             }
             __syncthreads();
             for(int k = threadIdx.x; k < P; k += SC_BLOCK) pos0[k] += (int64_t)((k + 1 < P ? coff[k + 1] : total) - coff[k]);
@@ -384,21 +468,45 @@ __global__ void k_batch_pagg(BPlan p){
     __shared__ int64_t s_item;
     double *lhi = g_batch_lds, *llo = lhi + p.part_keys;
     unsigned *lcnt = (unsigned*)(llo + p.part_keys);
+    __shared__ int64_t s_t0[QTAB];
+    load_qtab(p, SEL_PART, s_t0);
     while(true){
         const int64_t item = next_item(&p.ctr[3], &s_item);
         if(item >= p.total_pitems) break;
-        const BQuery &q = p.q[find_query(p.q, p.nq, item, SEL_PART)];
+        const BQuery &q = p.q[find_query_tab(p, s_t0, item, SEL_PART)];
         const int64_t pt = (item - q.pitem0) / q.psplit, sl = (item - q.pitem0) % q.psplit;
         for(int k = threadIdx.x; k < p.part_keys; k += blockDim.x){ lhi[k] = 0.0; llo[k] = 0.0; lcnt[k] = 0u; }
         __syncthreads();
-        const int64_t pa = q.poff[pt * q.ntiles], pb = q.poff[(pt + 1) * q.ntiles];
+        const gptr<const int64_t> poff = G((const int64_t*)q.poff);
+        const gptr<const uint16_t> skey = G((const uint16_t*)q.skey);
+        const gptr<const double> sval = G((const double*)q.sval);
+        const int64_t pa = poff[pt * q.ntiles], pb = poff[(pt + 1) * q.ntiles];
         const int64_t a = pa + (pb - pa) * sl / q.psplit, b = pa + (pb - pa) * (sl + 1) / q.psplit;
-        for(int64_t i = a + threadIdx.x; i < b; i += blockDim.x){
-            const uint32_t s = q.skey[i];
-            const double v = q.sval[i];
-            double old = atomicAdd(&lhi[s], v);
-            atomicAdd(&llo[s], two_sum_err(old, v, old + v));
-            atomicAdd(&lcnt[s], 1u);
+        const int64_t step = (int64_t)blockDim.x;
+        if(a < b){
+            // eight rows a thread per step; the next step's rows are loaded before this step's adds
+            // (clamped to the slice's last row)
+            constexpr int PU = 8;
+            uint32_t ns[PU]; double nv[PU];
+            #pragma unroll
+            for(int u = 0; u < PU; u++){ const int64_t i = a + threadIdx.x + u * step, ic = i < b ? i : b - 1; ns[u] = skey[ic]; nv[u] = sval[ic]; }
+            for(int64_t i0 = a + threadIdx.x; i0 < b; i0 += PU * step){
+                uint32_t s[PU]; double v[PU];
+                #pragma unroll
+                for(int u = 0; u < PU; u++){ s[u] = ns[u]; v[u] = nv[u]; }
+                #pragma unroll
+                for(int u = 0; u < PU; u++){ const int64_t i = i0 + (PU + u) * step, ic = i < b ? i : b - 1; ns[u] = skey[ic]; nv[u] = sval[ic]; }
+                // the step's returning adds first, then their rounding errors and counts
+                double old[PU];
+                #pragma unroll
+                for(int u = 0; u < PU; u++) if(i0 + u * step < b) old[u] = atomicAdd(&lhi[s[u]], v[u]);
+                #pragma unroll
+                for(int u = 0; u < PU; u++){
+                    if(i0 + u * step >= b) continue;
+                    atomicAdd(&llo[s[u]], two_sum_err(old[u], v[u], old[u] + v[u]));
+                    atomicAdd(&lcnt[s[u]], 1u);
+                }
+            }
         }
         __syncthreads();
         const int64_t base = pt * (int64_t)p.part_keys;
@@ -423,18 +531,27 @@ __global__ void __launch_bounds__(256) k_batch_emit(BPlan p){
     __shared__ int64_t s_pre;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    __shared__ int64_t s_t0[QTAB];
+    load_qtab(p, SEL_EMIT, s_t0);
     while(true){
         const int64_t item = next_item(&p.ctr[1], &s_item);
         if(item >= p.total_etiles) break;
-        const BQuery &q = p.q[find_query(p.q, p.nq, item, SEL_EMIT)];
+        const BQuery &q = p.q[find_query_tab(p, s_t0, item, SEL_EMIT)];
         const int64_t tile = item - q.etile0;
         const int64_t base = tile * EMIT_TILE;
         uint64_t bal[EMIT_ITEMS];
+        const gptr<const unsigned> cnt = G((const unsigned*)q.cnt);
+        unsigned cv[EMIT_ITEMS];
+        #pragma unroll
+        for(int k = 0; k < EMIT_ITEMS; k++){
+            const int64_t e = base + (int64_t)k * 256 + threadIdx.x, ec = e < q.range ? e : q.range - 1;
+            int64_t s = ec + q.rot; if(s >= q.range) s -= q.range;
+            cv[k] = cnt[s];
+        }
         #pragma unroll
         for(int k = 0; k < EMIT_ITEMS; k++){
             int64_t e = base + (int64_t)k * 256 + threadIdx.x;
-            int64_t s = e + q.rot; if(s >= q.range) s -= q.range;
-            bool m = e < q.range && q.cnt[s] != 0u;
+            bool m = e < q.range && cv[k] != 0u;
             bal[k] = __ballot(m);
             if(lane == 0) s_cnt[k * 4 + w] = (uint32_t)__popcll(bal[k]);
         }
@@ -472,7 +589,7 @@ __global__ void __launch_bounds__(256) k_batch_emit(BPlan p){
                 int64_t e = base + (int64_t)k * 256 + threadIdx.x;
                 int64_t s = e + q.rot; if(s >= q.range) s -= q.range;
                 int64_t pos = pre + s_cnt[k * 4 + w] + __popcll(bal[k] & lt);
-                p.gkey[pos] = (int32_t)(s + q.kmin); p.gsum[pos] = q.hi[s] + q.lo[s]; p.gcnt[pos] = (int64_t)q.cnt[s];
+                p.gkey[pos] = (int32_t)(s + q.kmin); p.gsum[pos] = G((const double*)q.hi)[s] + G((const double*)q.lo)[s]; p.gcnt[pos] = (int64_t)cv[k];
             }
         }
         __syncthreads();
@@ -491,7 +608,7 @@ struct hpa_batch {
     int32_t *gkey = nullptr; double *gsum = nullptr; int64_t *gcnt = nullptr;
     int64_t slots = 0;             // sum of the key ranges: capacity of the output
     int *dpq = nullptr; int npq = 0;
-    int64_t total_tiles = 0, total_etiles = 0, total_pitems = 0, total_pchunks = 0;
+    int64_t total_tiles = 0, total_etiles = 0, total_pitems = 0, total_pchunks = 0, total_stiles = 0;
     int block = 256, tile_rows = 8192, lds_keys = 2730, part_keys = 2048, grid = 0, max_parts = 0;
     hipGraph_t graph = nullptr; hipGraphExec_t exec = nullptr; bool captured = false;
     std::vector<int64_t> ng; bool ng_valid = false;
@@ -510,7 +627,7 @@ struct hpa_batch {
 static BPlan plan_of(hpa_batch *b){
     BPlan p;
     p.q = b->dq; p.nq = b->nq; p.total_tiles = b->total_tiles; p.total_etiles = b->total_etiles;
-    p.total_pitems = b->total_pitems;
+    p.total_pitems = b->total_pitems; p.total_stiles = b->total_stiles;
     p.ctr = b->ctr; p.tile_rows = b->tile_rows; p.lds_keys = b->lds_keys;
     p.estatus = b->estatus; p.gkey = b->gkey; p.gsum = b->gsum; p.gcnt = b->gcnt;
     p.part_keys = b->part_keys; p.part_shift = 0;
@@ -537,7 +654,7 @@ static int32_t batch_layout(hpa_batch *b, int block, int tile_rows, int lds_keys
         set_error_detail("batch tile %d: partitioning needs tiles of at least %d rows", tile_rows, PART_MIN_TILE); R HPA_E_DOMAIN;
     }
     b->block = block; b->tile_rows = tile_rows; b->lds_keys = lds_keys; b->part_keys = part_keys;
-    int64_t t = 0, pi = 0, pc = 0;
+    int64_t t = 0, pi = 0, pc = 0, st = 0;
     std::vector<int> pq;
     b->max_parts = 0;
     for(int qi = 0; qi < b->nq; qi++){
@@ -552,12 +669,14 @@ static int32_t batch_layout(hpa_batch *b, int block, int tile_rows, int lds_keys
         q.psplit = (int32_t)(sp < 1 ? 1 : sp > 64 ? 64 : sp);
         q.pitem0 = pi; pi += (int64_t)q.nparts * q.psplit;
         q.pch0 = pc;
+        q.stile0 = st;
         if(q.part){
+            st += q.ntiles;
             pq.push_back(qi); b->max_parts = q.nparts > b->max_parts ? q.nparts : b->max_parts;
             pc += ceil_div((int64_t)q.nparts * q.ntiles, PS_CHUNK);
         }
     }
-    b->total_tiles = t; b->total_pitems = pi; b->total_pchunks = pc;
+    b->total_tiles = t; b->total_pitems = pi; b->total_pchunks = pc; b->total_stiles = st;
     b->npq = (int)pq.size();
     b->grid = num_cus() * (2048 / block);
     drop_graph(b);
@@ -664,7 +783,7 @@ extern "C" hpa_batch *hpaBatchCreate(int32_t nq, hpa_V *keys, hpa_V *vals, hpa_V
     b->gcnt = (int64_t*)dalloc(c, 8 * b->slots);
     if(!b->estatus || !b->gkey || !b->gsum || !b->gcnt) R fail("arena full", 0);
     b->ngdev = (int64_t*)dalloc(c, 16 * (L)nq);
-    b->ctr = (unsigned*)dalloc(c, 16);
+    b->ctr = (unsigned*)dalloc(c, 4 * BATCH_CTRS);
     if(!b->ngdev || !b->ctr) R fail("arena full", 0);
     for(int qi = 0; qi < nq; qi++) b->hq[qi].ng = b->ngdev + 2 * qi;
     b->ng.assign(2 * (size_t)nq, 0);
