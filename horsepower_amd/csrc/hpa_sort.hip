@@ -489,24 +489,60 @@ __device__ __forceinline__ K shfl_up1(K v){
     }
     return (K)__shfl_up((uint32_t)v, 1, 64);
 }
-// run head of row i where the lanes of the wave hold consecutive rows: the previous key comes
-// from the lane below (lane 0 loads it); every lane of the wave must call it
+// run head of row i = iw + lane where the lanes of the wave hold consecutive rows from iw (iw
+// wave-uniform, n > 0): the previous key comes from the lane below, lane 0's from one
+// wave-uniform load.  Keys are loaded at indices clamped to the last row instead of under a
+// branch, so the loads of a thread's rows are in flight together.  Every lane of the wave must
+// call it; *cur is row i's key (the last row's beyond n).
 template<typename K>
-__device__ __forceinline__ bool run_head_w(const K *sk, const uint32_t *sidx, const uint64_t *eq, L i, L n){
-    if(eq) return i < n && run_head(sk, sidx, eq, i);
-    const K cur = i < n ? sk[i] : (K)0;
-    K prev = shfl_up1(cur);
-    if((threadIdx.x & 63) == 0 && i > 0 && i <= n) prev = sk[i - 1];
-    return i < n && (i == 0 || cur != prev);
+__device__ __forceinline__ bool run_head_w(const K *sk, const uint32_t *sidx, const uint64_t *eq, L i, L iw, L n, K *cur){
+    if(eq){ *cur = (K)0; return i < n && run_head(sk, sidx, eq, i); }
+    const K k = sk[i < n ? i : n - 1];
+    const K pw = iw > 0 ? sk[iw - 1 < n ? iw - 1 : n - 1] : (K)0;
+    K prev = shfl_up1(k);
+    if((threadIdx.x & 63) == 0) prev = pw;
+    *cur = k;
+    return i < n && (i == 0 || k != prev);
 }
-template<typename K>
+// EQ: runs of equal eq[sidx[i]] (the key words behind the sorted row ids) instead of equal sk[i]
+// The run heads of ITEMS steps of 256 consecutive rows (row base + 256 k + threadIdx.x): every
+// key load is issued before the first comparison.  bal[k]: the wave's heads of step k; kv[k]:
+// the thread's key of step k.
+template<typename K, int ITEMS>
+__device__ __forceinline__ void run_heads(const K *sk, L base, L n, uint64_t *bal, K *kv){
+    const L wbase = base + (L)__builtin_amdgcn_readfirstlane(threadIdx.x & ~63u);
+    K pw[ITEMS];
+    #pragma unroll
+    for(int k = 0; k < ITEMS; k++){
+        const L i = base + (L)k * 256 + threadIdx.x, iw = wbase + (L)k * 256;
+        kv[k] = sk[i < n ? i : n - 1];
+        pw[k] = iw > 0 ? sk[iw - 1 < n ? iw - 1 : n - 1] : (K)0;
+    }
+    #pragma unroll
+    for(int k = 0; k < ITEMS; k++){
+        const L i = base + (L)k * 256 + threadIdx.x;
+        K prev = shfl_up1(kv[k]);
+        if((threadIdx.x & 63) == 0) prev = pw[k];
+        bal[k] = __ballot(i < n && (i == 0 || kv[k] != prev));
+    }
+}
+template<typename K, bool EQ>
 __global__ void __launch_bounds__(256) k_run_tilecount(const K *sk, const uint32_t *sidx, const uint64_t *eq, L n, uint32_t *tcount){
     const L base = (L)blockIdx.x * RUN_TILE;
+    const L wbase = base + (L)__builtin_amdgcn_readfirstlane(threadIdx.x & ~63u);
     uint32_t c = 0;
-    #pragma unroll
-    for(int k = 0; k < RUN_ITEMS; k++){
-        L i = base + (L)k * 256 + threadIdx.x;
-        c += run_head_w(sk, sidx, eq, i, n) ? 1u : 0u;
+    if(EQ){
+        #pragma unroll
+        for(int k = 0; k < RUN_ITEMS; k++){
+            const L i = base + (L)k * 256 + threadIdx.x;
+            K cur;
+            c += run_head_w(sk, sidx, eq, i, wbase + (L)k * 256, n, &cur) ? 1u : 0u;
+        }
+    } else {
+        uint64_t bal[RUN_ITEMS]; K kv[RUN_ITEMS];
+        run_heads<K, RUN_ITEMS>(sk, base, n, bal, kv);
+        #pragma unroll
+        for(int k = 0; k < RUN_ITEMS; k++) c += (uint32_t)((bal[k] >> (threadIdx.x & 63)) & 1ull);
     }
     __shared__ uint32_t sh[4];
     for(int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
@@ -516,22 +552,44 @@ __global__ void __launch_bounds__(256) k_run_tilecount(const K *sk, const uint32
 }
 // (with flat_in: the flat list is already written -- the sort's last pass wrote it -- and a
 // head's key is read from it)
-template<typename K>
+template<typename K, bool EQ>
 __global__ void __launch_bounds__(256) k_run_write(const K *sk, const uint32_t *sidx, const uint64_t *eq, L n,
                                                    const int64_t *toff, int64_t *keys, int64_t *start, int64_t *flat,
                                                    uint32_t *kval, const int64_t *flat_in){
     __shared__ uint32_t s_cnt[RUN_ITEMS * 4];
     const L base = (L)blockIdx.x * RUN_TILE;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const L wbase = base + (L)__builtin_amdgcn_readfirstlane(threadIdx.x & ~63u);
     uint64_t bal[RUN_ITEMS];
+    K kv[RUN_ITEMS];
+    int64_t hv[RUN_ITEMS];                      // each head's key: its first row's original index
+    // every load of the step first, then the stores (which may alias the inputs for the compiler)
+    if(!flat_in){
+        #pragma unroll
+        for(int k = 0; k < RUN_ITEMS; k++){ const L i = base + (L)k * 256 + threadIdx.x; hv[k] = (int64_t)sidx[i < n ? i : n - 1]; }
+    }
+    if(EQ){
+        #pragma unroll
+        for(int k = 0; k < RUN_ITEMS; k++){
+            const L i = base + (L)k * 256 + threadIdx.x;
+            bal[k] = __ballot(run_head_w(sk, sidx, eq, i, wbase + (L)k * 256, n, &kv[k]));
+        }
+    } else run_heads<K, RUN_ITEMS>(sk, base, n, bal, kv);
     #pragma unroll
-    for(int k = 0; k < RUN_ITEMS; k++){
-        L i = base + (L)k * 256 + threadIdx.x;
-        bool h = false;
-        if(i < n && !flat_in) flat[i] = (int64_t)sidx[i];
-        h = run_head_w(sk, sidx, eq, i, n);
-        bal[k] = __ballot(h);
-        if(lane == 0) s_cnt[k * 4 + w] = (uint32_t)__popcll(bal[k]);
+    for(int k = 0; k < RUN_ITEMS; k++) if(lane == 0) s_cnt[k * 4 + w] = (uint32_t)__popcll(bal[k]);
+    if(!flat_in){
+        #pragma unroll
+        for(int k = 0; k < RUN_ITEMS; k++){ const L i = base + (L)k * 256 + threadIdx.x; if(i < n) flat[i] = hv[k]; }
+    }
+    // with flat_in, the heads' entries are read together: lanes that are no head read the wave's
+    // first row of the step (a line the heads' loads share or one more), so nothing waits under a branch
+    if(flat_in){
+        #pragma unroll
+        for(int k = 0; k < RUN_ITEMS; k++){
+            const L i = base + (L)k * 256 + threadIdx.x, iw = wbase + (L)k * 256;
+            const L j = ((bal[k] >> lane) & 1ull) ? i : (iw < n ? iw : n - 1);
+            hv[k] = flat_in[j];
+        }
     }
     __syncthreads();
     if(w == 0){
@@ -548,8 +606,8 @@ __global__ void __launch_bounds__(256) k_run_write(const K *sk, const uint32_t *
         if((bal[k] >> lane) & 1ull){
             L i = base + (L)k * 256 + threadIdx.x;
             int64_t r = pre + s_cnt[k * 4 + w] + __popcll(bal[k] & lt);
-            keys[r] = flat_in ? flat_in[i] : (int64_t)sidx[i]; start[r] = i;
-            if(kval) kval[r] = (uint32_t)sk[i];
+            keys[r] = hv[k]; start[r] = i;
+            if(kval) kval[r] = (uint32_t)kv[k];
         }
     }
 }
@@ -604,10 +662,13 @@ static int32_t runs_to_group_t(hpa_ctx *c, V z, const K *sk, const uint32_t *sid
         int64_t *cnt = (int64_t*)dalloc(c, 8 * gmax);
         int64_t *flat = flat_in ? flat_in : (int64_t*)dalloc(c, 8 * n);
         if(!keys || !start || !cnt || !flat || (src && !kval)) R HPA_E_MEMORY;
-        hipLaunchKernelGGL(k_run_tilecount<K>, dim3((unsigned)nt), dim3(256), 0, c->stream, sk, sidx, eq, n, tcount);
+        if(eq) hipLaunchKernelGGL((k_run_tilecount<K, true>), dim3((unsigned)nt), dim3(256), 0, c->stream, sk, sidx, eq, n, tcount);
+        else hipLaunchKernelGGL((k_run_tilecount<K, false>), dim3((unsigned)nt), dim3(256), 0, c->stream, sk, sidx, eq, n, tcount);
         if((e = scan_u32_to_i64(c, tcount, toff, nt, toff + nt))) R e;
-        hipLaunchKernelGGL(k_run_write<K>, dim3((unsigned)nt), dim3(256), 0, c->stream, sk, sidx, eq, n, (const int64_t*)toff, keys, start,
-                           flat, kval, (const int64_t*)flat_in);
+        if(eq) hipLaunchKernelGGL((k_run_write<K, true>), dim3((unsigned)nt), dim3(256), 0, c->stream, sk, sidx, eq, n, (const int64_t*)toff,
+                                  keys, start, flat, kval, (const int64_t*)flat_in);
+        else hipLaunchKernelGGL((k_run_write<K, false>), dim3((unsigned)nt), dim3(256), 0, c->stream, sk, sidx, eq, n, (const int64_t*)toff,
+                                keys, start, flat, kval, (const int64_t*)flat_in);
         hipLaunchKernelGGL(k_run_counts_dev, dim3(grid_for(gmax, 256, 1)), dim3(256), 0, c->stream, start, (const int64_t*)(toff + nt), n, cnt);
         if((e = check_launch("group_runs"))) R e;
         int64_t g = 0; if((e = d2h(c, &g, toff + nt, 8))) R e;
@@ -618,7 +679,8 @@ static int32_t runs_to_group_t(hpa_ctx *c, V z, const K *sk, const uint32_t *sid
     }
     int64_t g = 0;
     if(n > 0){
-        hipLaunchKernelGGL(k_run_tilecount<K>, dim3((unsigned)nt), dim3(256), 0, c->stream, sk, sidx, eq, n, tcount);
+        if(eq) hipLaunchKernelGGL((k_run_tilecount<K, true>), dim3((unsigned)nt), dim3(256), 0, c->stream, sk, sidx, eq, n, tcount);
+        else hipLaunchKernelGGL((k_run_tilecount<K, false>), dim3((unsigned)nt), dim3(256), 0, c->stream, sk, sidx, eq, n, tcount);
         if((e = scan_u32_to_i64(c, tcount, toff, nt, toff + nt))) R e;
         if((e = d2h(c, &g, toff + nt, 8))) R e;
     }
@@ -629,8 +691,10 @@ static int32_t runs_to_group_t(hpa_ctx *c, V z, const K *sk, const uint32_t *sid
     int64_t *flat = flat_in ? flat_in : (int64_t*)dalloc(c, 8 * (n > 0 ? n : 1));
     if(!keys || !start || !cnt || !flat) R HPA_E_MEMORY;
     if(n > 0){
-        hipLaunchKernelGGL(k_run_write<K>, dim3((unsigned)nt), dim3(256), 0, c->stream, sk, sidx, eq, n, toff, keys, start, flat, kval,
-                           (const int64_t*)flat_in);
+        if(eq) hipLaunchKernelGGL((k_run_write<K, true>), dim3((unsigned)nt), dim3(256), 0, c->stream, sk, sidx, eq, n, toff, keys, start,
+                                  flat, kval, (const int64_t*)flat_in);
+        else hipLaunchKernelGGL((k_run_write<K, false>), dim3((unsigned)nt), dim3(256), 0, c->stream, sk, sidx, eq, n, toff, keys, start,
+                                flat, kval, (const int64_t*)flat_in);
         hipLaunchKernelGGL(k_run_counts, dim3(grid_for(g, 256, 1)), dim3(256), 0, c->stream, start, g, n, cnt);
     }
     if((e = check_launch("group_runs"))) R e;
@@ -1407,7 +1471,7 @@ __global__ void __launch_bounds__(256) k_gr_head_write(const int32_t *x, L n, un
     uint64_t bal[RUN_ITEMS];
     int32_t xv[RUN_ITEMS];
     #pragma unroll
-    for(int k = 0; k < RUN_ITEMS; k++){ const L i = base + (L)k * 256 + threadIdx.x; xv[k] = i < n ? x[i] : 0; }
+    for(int k = 0; k < RUN_ITEMS; k++){ const L i = base + (L)k * 256 + threadIdx.x; xv[k] = x[i < n ? i : n - 1]; }
     #pragma unroll
     for(int k = 0; k < RUN_ITEMS; k++){
         bal[k] = __ballot(gr_head(xv[k], base + (L)k * 256 + threadIdx.x, n, cap_mask));
@@ -1449,11 +1513,16 @@ __global__ void __launch_bounds__(256) k_grf_count(const uint32_t *sk, const uin
     __shared__ uint32_t sh[4], sr[4];
     const L base = (L)blockIdx.x * GRF_TILE;
     uint32_t h = 0, rows = 0;
+    uint32_t vv[GRF_ITEMS];
+    #pragma unroll
+    for(int k = 0; k < GRF_ITEMS; k++){ const L i = base + (L)k * 256 + threadIdx.x; vv[k] = sv[i < nr ? i : nr - 1]; }
+    uint64_t bal[GRF_ITEMS]; uint32_t kv[GRF_ITEMS];
+    run_heads<uint32_t, GRF_ITEMS>(sk, base, nr, bal, kv);
     #pragma unroll
     for(int k = 0; k < GRF_ITEMS; k++){
         const L i = base + (L)k * 256 + threadIdx.x;
-        h += run_head_w<uint32_t>(sk, nullptr, nullptr, i, nr) ? 1u : 0u;
-        if(i < nr) rows += (sv[i] & len_mask) + 1u;
+        h += (uint32_t)((bal[k] >> (threadIdx.x & 63)) & 1ull);
+        if(i < nr) rows += (vv[k] & len_mask) + 1u;
     }
     for(int o = 32; o > 0; o >>= 1){ h += __shfl_down(h, o, 64); rows += __shfl_down(rows, o, 64); }
     if((threadIdx.x & 63) == 0){ sh[threadIdx.x >> 6] = h; sr[threadIdx.x >> 6] = rows; }
@@ -1485,11 +1554,13 @@ __global__ void __launch_bounds__(256) k_grf_write(const uint32_t *sk, const uin
     uint32_t kv[GRF_ITEMS], vv[GRF_ITEMS], ex[GRF_ITEMS];
     uint64_t bal[GRF_ITEMS];
     #pragma unroll
+    for(int k = 0; k < GRF_ITEMS; k++){ const L i = base + (L)k * 256 + threadIdx.x; vv[k] = sv[i < nr ? i : nr - 1]; }
+    run_heads<uint32_t, GRF_ITEMS>(sk, base, nr, bal, kv);
+    #pragma unroll
     for(int k = 0; k < GRF_ITEMS; k++){
         const L i = base + (L)k * 256 + threadIdx.x;
         const bool valid = i < nr;
-        kv[k] = valid ? sk[i] : 0u; vv[k] = valid ? sv[i] : 0u;
-        bal[k] = __ballot(run_head_w<uint32_t>(sk, nullptr, nullptr, i, nr));
+        if(!valid){ kv[k] = 0u; vv[k] = 0u; }
         const uint32_t len = valid ? (vv[k] & len_mask) + 1u : 0u;
         uint32_t inc = len;
         for(int o = 1; o < 64; o <<= 1){ const uint32_t t = __shfl_up(inc, o, 64); if(lane >= o) inc += t; }
