@@ -458,14 +458,40 @@ __global__ void k_dense_probe_join(const int32_t *y, L ny, DenseTab t, uint32_t 
 // cheaper probed once (1.5M keys x 6M probes: 241 us once, 291 us twice).
 #define DJ_ITEMS 16
 #define DJ_TILE (256 * DJ_ITEMS)
+// dense_row of the DJ_ITEMS rows base + 256 k + threadIdx.x in three rounds of loads -- keys,
+// bitmap words, table entries -- each round's loads at clamped indices and in flight together
+// (rows past n and keys out of range read entry 0 and come out -1).  With `need_row` false on
+// a guarded table the table is not read: rr[k] is 0 for a hit.
+__device__ __forceinline__ void dense_rows(const DenseTab &t, const int32_t *y, L base, L n, bool need_row, int32_t *rr){
+    int64_t d[DJ_ITEMS]; bool in[DJ_ITEMS];
+    #pragma unroll
+    for(int k = 0; k < DJ_ITEMS; k++){ const L i = base + (L)k * 256 + threadIdx.x; d[k] = (int64_t)y[i < n ? i : n - 1] - t.lo; }
+    #pragma unroll
+    for(int k = 0; k < DJ_ITEMS; k++){ const L i = base + (L)k * 256 + threadIdx.x; in[k] = i < n && d[k] >= 0 && d[k] < t.m; }
+    if(t.pres){
+        uint32_t wv[DJ_ITEMS];
+        #pragma unroll
+        for(int k = 0; k < DJ_ITEMS; k++) wv[k] = t.pres[in[k] ? (d[k] >> 5) : 0];
+        #pragma unroll
+        for(int k = 0; k < DJ_ITEMS; k++) in[k] = in[k] && ((wv[k] >> (d[k] & 31)) & 1u);
+        if(!need_row){
+            #pragma unroll
+            for(int k = 0; k < DJ_ITEMS; k++) rr[k] = in[k] ? 0 : -1;
+            return;
+        }
+    }
+    #pragma unroll
+    for(int k = 0; k < DJ_ITEMS; k++) rr[k] = t.tab[in[k] ? d[k] : 0];
+    #pragma unroll
+    for(int k = 0; k < DJ_ITEMS; k++) rr[k] = in[k] ? rr[k] : -1;
+}
 __global__ void __launch_bounds__(256) k_dj_count(const int32_t *y, L n, DenseTab t, uint32_t *tcount){
     const L base = (L)blockIdx.x * DJ_TILE;
     uint32_t c = 0;
+    int32_t rr[DJ_ITEMS];
+    dense_rows(t, y, base, n, false, rr);
     #pragma unroll
-    for(int k = 0; k < DJ_ITEMS; k++){
-        const L i = base + (L)k * 256 + threadIdx.x;
-        if(i < n && dense_row(t, y[i]) >= 0) c++;
-    }
+    for(int k = 0; k < DJ_ITEMS; k++) c += rr[k] >= 0 ? 1u : 0u;
     __shared__ uint32_t sh[4];
     for(int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
     if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
@@ -479,10 +505,9 @@ __global__ void __launch_bounds__(256) k_dj_write(const int32_t *y, L n, DenseTa
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint64_t bal[DJ_ITEMS];
     int32_t rr[DJ_ITEMS];
+    dense_rows(t, y, base, n, true, rr);
     #pragma unroll
     for(int k = 0; k < DJ_ITEMS; k++){
-        const L i = base + (L)k * 256 + threadIdx.x;
-        rr[k] = i < n ? dense_row(t, y[i]) : -1;
         bal[k] = __ballot(rr[k] >= 0);
         if(lane == 0) s_cnt[k * 4 + w] = (uint32_t)__popcll(bal[k]);
     }

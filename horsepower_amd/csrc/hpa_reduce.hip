@@ -362,18 +362,24 @@ __global__ void __launch_bounds__(256) k_seg_thread(const T *x, const int64_t *i
         bool first = true;
         for(L c0 = A; c0 < E; c0 += SEG_CAP){
             const L c1 = c0 + SEG_CAP < E ? c0 + SEG_CAP : E;
+            // index and value loads at clamped positions, so each set is in flight together (an
+            // index out of [0, nx) reads x[0] and is flagged; nx is above 0 whenever a flat list
+            // has entries, the indices being rows of x)
             int64_t k[SEG_CAP / 64];
             #pragma unroll
-            for(int t = 0; t < SEG_CAP / 64; t++){ const L j = c0 + t * 64 + lane; k[t] = j < c1 ? (idx ? idx[j] : j) : 0; }
+            for(int t = 0; t < SEG_CAP / 64; t++){ const L j = c0 + t * 64 + lane, jc = j < c1 ? j : c1 - 1; k[t] = idx ? idx[jc] : jc; }
+            T v[SEG_CAP / 64];
+            #pragma unroll
+            for(int t = 0; t < SEG_CAP / 64; t++){
+                const bool inb = !idx || (uint64_t)k[t] < (uint64_t)nx;
+                v[t] = x[inb ? k[t] : 0];
+            }
             #pragma unroll
             for(int t = 0; t < SEG_CAP / 64; t++){
                 const L j = c0 + t * 64 + lane;
-                T v = T();
-                if(j < c1){
-                    if(idx && (uint64_t)k[t] >= (uint64_t)nx){ if(bad) *bad = 1; }
-                    else v = x[k[t]];
-                }
-                buf[w][t * 64 + lane] = v;
+                const bool inb = !idx || (uint64_t)k[t] < (uint64_t)nx;
+                if(j < c1 && !inb && bad) *bad = 1;
+                buf[w][t * 64 + lane] = j < c1 && inb ? v[t] : T();
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -420,9 +426,10 @@ __global__ void __launch_bounds__(256) k_seg_runs(const T *x, const uint32_t *sv
             const uint32_t len = (v & lmask) + 1u;
             cnt += len;
             for(uint32_t t = 0; t < len; t += 4){
+                // four rows of the run loaded together (past its end, its last row again)
                 T w[4];
                 #pragma unroll
-                for(int u = 0; u < 4; u++) w[u] = t + u < len ? x[row + t + u] : T();
+                for(int u = 0; u < 4; u++) w[u] = x[row + (t + u < len ? t + u : len - 1)];
                 #pragma unroll
                 for(int u = 0; u < 4; u++){
                     if(t + u < len){
@@ -617,6 +624,8 @@ static int32_t seg_reduce(hpa_ctx *c, V x, const void *px, const int64_t *idx, c
     // row lists made here (a group's flat list) carry a bound on their rows: no check to read back
     if(idx && !within_bound(c, idx, x->len)){ bad = (int*)dalloc(c, 16); if(!bad) R HPA_E_MEMORY; hipMemsetAsync(bad, 0, 4, c->stream); }
     L nx = x->len;
+    // rows of an empty vector: every entry is out of range (the kernels read x[0] for them)
+    if(n > 0 && nx <= 0) R HPA_E_INDEX;
     switch(t){
         case HPA_B: e = seg_reduce_t<uint8_t,false>(c, (const uint8_t*)px, idx, off, g, n, kind, dst, nx, bad, runs); break;
         case HPA_J: e = seg_reduce_t<int8_t,false>(c, (const int8_t*)px, idx, off, g, n, kind, dst, nx, bad, runs); break;

@@ -988,12 +988,18 @@ __device__ __forceinline__ void sg8_bins(const SgSrc &s, const uint16_t *ra, con
         if(STORE){ if((j & 3) == 0) pk[g][j >> 2] = v & 255u; else pk[g][j >> 2] |= (v & 255u) << (8 * (j & 3)); }
     };
     const L rt = t0 + (L)threadIdx.x * 16;
+    // the vector loads of all groups go out together: a group past the end loads the last whole
+    // aligned 16 rows instead (and is then read row by row), so no load waits under a branch
+    const bool vok = s.vec && n >= 16;
+    const L rlast = vok ? ((n - 16) & ~(L)15) : 0;
     if(s.a){
         uint4 qa[SG8_G], qb[SG8_G];
-        #pragma unroll
-        for(int g = 0; g < SG8_G; g++){
-            const L r = rt + g * 4096;
-            if(s.vec && r + 16 <= n){ qa[g] = *(const uint4*)(s.a + r); qb[g] = *(const uint4*)(s.b + r); }
+        if(vok){
+            #pragma unroll
+            for(int g = 0; g < SG8_G; g++){
+                const L r = rt + g * 4096, rc = r + 16 <= n ? r : rlast;
+                qa[g] = *(const uint4*)(s.a + rc); qb[g] = *(const uint4*)(s.b + rc);
+            }
         }
         #pragma unroll
         for(int g = 0; g < SG8_G; g++){
@@ -1008,12 +1014,12 @@ __device__ __forceinline__ void sg8_bins(const SgSrc &s, const uint16_t *ra, con
         }
     } else {
         uint4 qx[SG8_G][4];
-        #pragma unroll
-        for(int g = 0; g < SG8_G; g++){
-            const L r = rt + g * 4096;
-            if(s.vec && r + 16 <= n){
+        if(vok){
+            #pragma unroll
+            for(int g = 0; g < SG8_G; g++){
+                const L r = rt + g * 4096, rc = r + 16 <= n ? r : rlast;
                 #pragma unroll
-                for(int k = 0; k < 4; k++) qx[g][k] = *(const uint4*)(s.x + r + 4 * k);
+                for(int k = 0; k < 4; k++) qx[g][k] = *(const uint4*)(s.x + rc + 4 * k);
             }
         }
         #pragma unroll
