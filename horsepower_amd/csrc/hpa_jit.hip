@@ -494,7 +494,13 @@ int32_t jit_compress_map(hpa_ctx *c, int otyp, const DevCols &cols, const hpa_ex
          "    const int t = threadIdx.x, lane = t & 63, w = t >> 6;\n"
          "    const i64 i0 = (i64)blockIdx.x * 4096 + (i64)t * 16;\n"
          "    const unsigned m = a.mb[(i64)blockIdx.x * 256 + t];\n"
-         "    const unsigned cnt = __popc(m);\n"
+         "    const bool full = i0 + 16 <= a.n;\n";
+    // the thread's column loads go out as soon as its selection bits are known, before the scan
+    // and the barrier (none for 16 unselected rows)
+    for(int k = 0; k < nx; k++) if(used[k])
+        s << "    R16<" << ctype(cols.typ[k]) << "> r" << k << ";\n"
+          << "    if(m && full) r" << k << " = *(const R16<" << ctype(cols.typ[k]) << ">*)((const " << ctype(cols.typ[k]) << "*)a.c[" << k << "] + i0);\n";
+    s << "    const unsigned cnt = __popc(m);\n"
          "    unsigned inc = cnt;\n"
          "    for(int o = 1; o < 64; o <<= 1){ unsigned y = __shfl_up(inc, o, 64); if(lane >= o) inc += y; }\n"
          "    if(lane == 63) sw[w] = inc;\n"
@@ -503,10 +509,7 @@ int32_t jit_compress_map(hpa_ctx *c, int otyp, const DevCols &cols, const hpa_ex
          "    for(int q = 0; q < 4; q++){ if(q < w) wbase += sw[q]; total += sw[q]; }\n"
          "    if(total == 0) return;\n"
          "    unsigned off = wbase + inc - cnt;\n"
-         "    if(m && i0 + 16 <= a.n){\n";
-    for(int k = 0; k < nx; k++) if(used[k])
-        s << "        const R16<" << ctype(cols.typ[k]) << "> r" << k << " = *(const R16<" << ctype(cols.typ[k]) << ">*)((const "
-          << ctype(cols.typ[k]) << "*)a.c[" << k << "] + i0);\n";
+         "    if(m && full){\n";
     s << "        #pragma unroll\n"
          "        for(int k = 0; k < 16; k++) if((m >> k) & 1u){\n";
     for(int k = 0; k < nx; k++) if(used[k]) s << "            const " << ctype(cols.typ[k]) << " v" << k << " = r" << k << ".v[k];\n";
