@@ -1723,13 +1723,14 @@ static int32_t group_int(hpa_ctx *c, V z, V x){      // lib_group_by_normal_int,
     if(sc.s.desc && n >= GR_RUNS_MIN && cb >= 2 && change <= GR_RUNS_MAX_CHANGE && (double)sc.nr <= GR_RUNS_MAX_SHARE * (double)n)
         R group_i32_runs(c, z, (const int32_t*)p, n, cb, sc);
     // the unsigned radix key of an i32 is its bit pattern: the column is sorted in place of keys;
-    // the sort's last pass writes the flat list; at most one group per key value of the range
+    // the sort's last pass scatters 4-byte row ids, which the run pass widens into the flat list
+    // with sequential writes (an i64 scatter half-fills its lines) and takes each run's key from;
+    // at most one group per key value of the range
     const RsHint h{sc.s.orv, sc.s.andv, sc.s.desc != 0};
-    int64_t *flat = (int64_t*)dalloc(c, 8 * n); if(!flat) R HPA_E_MEMORY;
-    uint32_t *sk = nullptr;
-    if((e = radix_sort_flat_u32(c, (const uint32_t*)p, n, 0, 32, &sk, flat, &h))) R e;
+    uint32_t *sk = nullptr, *sv = nullptr;
+    if((e = radix_sort_t<uint32_t>(c, (const uint32_t*)p, nullptr, n, 0, 32, &sk, &sv, nullptr, false, false, &h))) R e;
     const L span = (L)(sc.s.hi - sc.s.lo) + 1;
-    R runs_to_group_flat_u32(c, z, sk, flat, n, p, span < n ? span : n);
+    R runs_to_group_t<uint32_t>(c, z, sk, sv, n, nullptr, p, nullptr, span < n ? span : n);
 }
 
 static int32_t any_high_i64(hpa_ctx *c, const int64_t *p, L n, bool *hi){
