@@ -149,6 +149,23 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned g){
     return x * per + (x < rem ? x : rem) + q;
 }
 
+// Lanes of the wave whose `nb`-bit digit d equals this lane's, among `valid` lanes: one ballot
+// per digit bit, then per 32-bit half of the mask p &= ~(ballot ^ m), m all ones where this
+// lane's bit is set, as one three-input bitwise op (v_bitop3 truth table f(S0, S1, S2) read at
+// S0 = 0xF0, S1 = 0xCC, S2 = 0xAA: p & ~(b ^ m) = 0x90).
+__device__ __forceinline__ uint64_t match_digit(unsigned d, bool valid, int nb){
+    const uint64_t v = __ballot(valid);
+    uint32_t plo = (uint32_t)v, phi = (uint32_t)(v >> 32);
+    #pragma unroll
+    for(int b = 0; b < nb; b++){
+        const uint32_t m = (uint32_t)(((int32_t)(d << (31 - b))) >> 31);
+        const uint64_t bb = __ballot(m != 0u);
+        plo = __builtin_amdgcn_bitop3_b32(plo, (uint32_t)bb, m, 0x90);
+        phi = __builtin_amdgcn_bitop3_b32(phi, (uint32_t)(bb >> 32), m, 0x90);
+    }
+    return ((uint64_t)phi << 32) | plo;
+}
+
 // ---------------------------------------------------------------- shared device algorithms
 // exclusive scan of n int64 values (in place allowed); total returned through *total_dev
 // (device pointer, may be null).

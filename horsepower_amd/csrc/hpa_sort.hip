@@ -137,12 +137,7 @@ __global__ void __launch_bounds__(RS_BLOCK) k_rs_scatter(const K *kin, const uin
         L i = tbase + (L)r * 64 + lane;
         bool valid = i < n;
         unsigned d = (unsigned)((kr[r] >> shift) & (K)dmask);
-        uint64_t peers = __ballot(valid);
-        #pragma unroll
-        for(int b = 0; b < DB; b++){
-            uint64_t bb = __ballot((d >> b) & 1);
-            peers &= ((d >> b) & 1) ? bb : ~bb;
-        }
+        const uint64_t peers = match_digit(d, valid, DB);
         uint32_t before = (uint32_t)__popcll(peers & lt);
         uint32_t cur = wcnt[w][d];
         if(valid && before == 0) wcnt[w][d] = (uint16_t)(cur + (uint32_t)__popcll(peers));
@@ -865,14 +860,7 @@ __device__ __forceinline__ void sg_bins(const SgSrc &s, const uint16_t *ra, cons
     *(uint4*)(sd + threadIdx.x * 16) = make_uint4(pk[0], pk[1], pk[2], pk[3]);
 }
 // lanes of this wave holding the same bin (kb ballots), among valid lanes
-__device__ __forceinline__ uint64_t sg_peers(unsigned d, bool valid, int kb){
-    uint64_t peers = __ballot(valid);
-    for(int b = 0; b < kb; b++){
-        uint64_t bb = __ballot((d >> b) & 1);
-        peers &= ((d >> b) & 1) ? bb : ~bb;
-    }
-    return peers;
-}
+__device__ __forceinline__ uint64_t sg_peers(unsigned d, bool valid, int kb){ R match_digit(d, valid, kb); }
 // 9..256 bins: one 4096-row tile per workgroup; a row's stable rank inside its wave's quarter of
 // the tile comes from kb ballots (lanes sharing the bin)
 __global__ void __launch_bounds__(256) k_sg_hist(SgSrc s, L n, int nbins, int kb, uint32_t *cnt, L ntiles, int *miss){
