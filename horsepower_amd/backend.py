@@ -239,6 +239,18 @@ class GpuBackend(Backend):
                     (V * max(nm, 1))(*zm), typs, exprs, len(xs), (V * max(len(xs), 1))(*xs))
         return zs, zm
 
+    def fused_compress_map(self, cond, ys, maps, xs):
+        """hpaFusedCompressMap: compress of plain columns ys and of FE maps [(typ, expr)] over
+        columns xs by the condition cond (an expression over xs), one count and one write pass."""
+        n, nm = len(ys), len(maps)
+        zs = [self.node() for _ in range(n)]
+        zm = [self.node() for _ in range(nm)]
+        typs = (C.c_int32 * max(nm, 1))(*[t for t, _ in maps])
+        exprs = (Expr * max(nm, 1))(*[e for _, e in maps])
+        self.lib.rc("hpaFusedCompressMap", n, (V * max(n, 1))(*zs), (V * max(n, 1))(*ys), nm, (V * max(nm, 1))(*zm),
+                    typs, exprs, len(xs), (V * max(len(xs), 1))(*xs), C.byref(cond))
+        return zs, zm
+
     def index_many(self, xs, y):
         """Peephole index: [index(x, y) for x in xs] in one launch."""
         n = len(xs)

@@ -83,6 +83,10 @@ int32_t cols_view(hpa_ctx *c, int nx, V *x, DevCols *cols, L *n);
 int32_t jit_fused_reduce(hpa_ctx *c, int kind, int nx, V *x, const hpa_expr *cond, const hpa_expr *body,
                          const DevCols &cols, L n, void *part, int G);
 int32_t jit_fused_map(hpa_ctx *c, int otyp, int nx, V *x, const hpa_expr *ex, const DevCols &cols, L n, void *dz);
+// whether run-time generated kernels are in use (hipRTC present, not switched off)
+bool jit_enabled();
+// count pass of a compress by a condition over cols: selection bits per 16 rows into mb, tile counts
+int32_t jit_compress_count(hpa_ctx *c, const DevCols &cols, const hpa_expr *cond, L n, uint16_t *mb, uint32_t *tcount);
 int32_t jit_compress_map(hpa_ctx *c, int otyp, const DevCols &cols, const hpa_expr *ex, const uint16_t *mb,
                          const int64_t *toff, L n, void *dz);
 int32_t jit_fused_reduce_many(hpa_ctx *c, int nq, int nx, V *x, const hpa_expr *conds, const hpa_expr *body,

@@ -39,7 +39,7 @@ static std::map<std::string, JitFn> g_jit;
 static int g_jit_disabled = -1;
 static double g_jit_compile_ms = 0;
 
-static bool jit_enabled(){
+bool jit_enabled(){
     if(g_jit_disabled < 0){ const char *e = getenv("HPA_NO_JIT"); g_jit_disabled = (e && *e && *e != '0') ? 1 : 0; }
     R g_jit_disabled == 0;
 }
@@ -112,6 +112,17 @@ static std::string types_key(int nx, const int32_t *typ){
     R k;
 }
 
+// A prepared request of one generated-kernel kind: its function and the constants its launch
+// passes, kept by the request's raw bytes (kind, column types, expression code), so a repeated
+// request skips building the C text and its key.
+static std::string raw_key(const char *tag, int nx, const int32_t *typ, int extra, const hpa_expr *e, const hpa_expr *e2 = nullptr){
+    std::string k(tag);
+    k.append((const char*)&nx, sizeof(nx)); k.append((const char*)typ, sizeof(int32_t) * (size_t)nx);
+    k.append((const char*)&extra, sizeof(extra));
+    for(const hpa_expr *x : {e, e2}) if(x){ k.append((const char*)&x->n, sizeof(x->n)); k.append((const char*)x->code, sizeof(hpa_ins) * (size_t)x->n); }
+    R k;
+}
+
 template<typename A>
 static int32_t jit_launch(hpa_ctx *c, hipFunction_t fn, unsigned grid, unsigned block, A &args){
     void *params[] = {&args};
@@ -141,6 +152,21 @@ struct CExpr { std::string s; bool f; };
 #define JIT_MAX_CONST 32
 #define KONST_FIELDS "i64 ki[32]; double kf[32];"
 struct Konst { int ni = 0, nf = 0; long long i[JIT_MAX_CONST]; double f[JIT_MAX_CONST]; };
+// prepared requests by raw key (raw_key): the function and the launch's constants
+struct JitPrep { hipFunction_t fn = nullptr; Konst kc; };
+static std::map<std::string, JitPrep> g_jit_prep;
+static bool prep_find(const std::string &k, JitPrep *p){
+    std::lock_guard<std::mutex> lk(g_jit_mu);
+    auto it = g_jit_prep.find(k);
+    if(it == g_jit_prep.end()) R false;
+    *p = it->second; R true;
+}
+static void prep_store(const std::string &k, const JitPrep &p){
+    if(g_compile_only) R;
+    std::lock_guard<std::mutex> lk(g_jit_mu);
+    if(g_jit_prep.size() >= 4096) g_jit_prep.clear();
+    g_jit_prep[k] = p;
+}
 
 static int32_t expr_to_c(const hpa_expr *e, int nx, const int32_t *typ, CExpr *out, std::vector<int> &used,
                          Konst *kp = nullptr){
@@ -437,9 +463,13 @@ int32_t jit_fused_reduce_many(hpa_ctx *c, int nq, int nx, V *x, const hpa_expr *
 int32_t jit_fused_map(hpa_ctx *c, int otyp, int nx, V *x, const hpa_expr *ex, const DevCols &cols, L n, void *dz){
     if(!jit_enabled()) R HPA_E_NOT_IMPL;
     int32_t typ[EXPR_MAX_COLS]; for(int k = 0; k < nx; k++) typ[k] = x[k]->typ;
+    const std::string rk = raw_key("fe", nx, typ, otyp, ex);
+    JitPrep prep; int32_t e;
     std::vector<int> used(nx, 0);
-    CExpr be; int32_t e;
-    Konst kc;
+    const bool have = prep_find(rk, &prep);
+    if(!have){
+    CExpr be;
+    Konst &kc = prep.kc;
     if((e = expr_to_c(ex, nx, typ, &be, used, &kc))) R e;
     const char *ot = otyp == HPA_B ? "unsigned char" : otyp == HPA_I ? "int" : otyp == HPA_L ? "long long" : "double";
     auto gen = [&](){
@@ -454,16 +484,20 @@ int32_t jit_fused_map(hpa_ctx *c, int otyp, int nx, V *x, const hpa_expr *ex, co
     s << "    }\n}\n";
     return s.str();
     };
-    hipFunction_t fn;
     const std::string key = "fe|" + std::to_string(otyp) + "|" + std::to_string(nx) + "|" + types_key(nx, typ) + "|" + be.s;
-    if((e = jit_get_keyed(key, "fe_map", gen, &fn))) R e;
+    if((e = jit_get_keyed(key, "fe_map", gen, &prep.fn))) R e;
+    prep_store(rk, prep);
+    } else {
+        // the columns the expression reads, for the scope's bytes
+        for(int i = 0; i < ex->n; i++) if(ex->code[i].op == HPA_OP_COL && ex->code[i].arg >= 0 && ex->code[i].arg < nx) used[ex->code[i].arg] = 1;
+    }
     JArgs a; memset(&a, 0, sizeof(a));
     for(int k = 0; k < nx; k++) a.c[k] = cols.p[k];
-    memcpy(a.ki, kc.i, sizeof(a.ki)); memcpy(a.kf, kc.f, sizeof(a.kf));
+    memcpy(a.ki, prep.kc.i, sizeof(a.ki)); memcpy(a.kf, prep.kc.f, sizeof(a.kf));
     a.n = n; a.out = dz;
     double cb = 0; for(int k = 0; k < nx; k++) if(used[k]) cb += (double)type_width(typ[k]);
     ProfScope ps("fused_map", c->stream, (double)n * (cb + type_width(otyp)));
-    R jit_launch(c, fn, (unsigned)grid_for(n, 256, 4), 256, a);
+    R jit_launch(c, prep.fn, (unsigned)grid_for(n, 256, 4), 256, a);
 }
 
 // ---------------------------------------------------------------- FE map of a compress (cp_map)
@@ -477,9 +511,12 @@ int32_t jit_compress_map(hpa_ctx *c, int otyp, const DevCols &cols, const hpa_ex
     if(!jit_enabled()) R HPA_E_NOT_IMPL;
     const int nx = cols.n;
     for(int k = 0; k < nx; k++) if((uintptr_t)cols.p[k] & 15) R HPA_E_NOT_IMPL;
+    const std::string rk = raw_key("cpm", nx, cols.typ, otyp, ex);
+    JitPrep prep; int32_t e;
+    if(!prep_find(rk, &prep)){
     std::vector<int> used(nx, 0);
-    CExpr be; int32_t e;
-    Konst kc;
+    CExpr be;
+    Konst &kc = prep.kc;
     if((e = expr_to_c(ex, nx, cols.typ, &be, used, &kc))) R e;
     const char *ot = otyp == HPA_B ? "unsigned char" : otyp == HPA_I ? "int" : otyp == HPA_L ? "long long" : "double";
     const std::string val = otyp == HPA_B ? "(unsigned char)((" + be.s + ") != 0)" : "(" + std::string(ot) + ")(" + be.s + ")";
@@ -528,16 +565,81 @@ int32_t jit_compress_map(hpa_ctx *c, int otyp, const DevCols &cols, const hpa_ex
          "}\n";
     return s.str();
     };
-    hipFunction_t fn;
     const std::string key = "cpm|" + std::to_string(otyp) + "|" + std::to_string(nx) + "|" + types_key(nx, cols.typ) + "|" + be.s;
-    if((e = jit_get_keyed(key, "cp_map", gen, &fn))) R e;
+    if((e = jit_get_keyed(key, "cp_map", gen, &prep.fn))) R e;
+    prep_store(rk, prep);
+    }
     struct CArgs { const void *c[EXPR_MAX_COLS]; long long n; void *out; const uint16_t *mb; const int64_t *toff;
                    long long ki[JIT_MAX_CONST]; double kf[JIT_MAX_CONST]; } a;
     memset(&a, 0, sizeof(a));
     for(int k = 0; k < nx; k++) a.c[k] = cols.p[k];
-    memcpy(a.ki, kc.i, sizeof(a.ki)); memcpy(a.kf, kc.f, sizeof(a.kf));
+    memcpy(a.ki, prep.kc.i, sizeof(a.ki)); memcpy(a.kf, prep.kc.f, sizeof(a.kf));
     a.n = n; a.out = dz; a.mb = mb; a.toff = toff;
-    R jit_launch(c, fn, (unsigned)ceil_div(n, 4096), 256, a);
+    R jit_launch(c, prep.fn, (unsigned)ceil_div(n, 4096), 256, a);
+}
+
+// ---------------------------------------------------------------- condition count of a compress (cp_cnt)
+// The count pass of the FA compress scan with its condition evaluated in place of a byte mask
+// (4096-row tiles, thread t owns the 16 consecutive rows [16t, 16t + 16)): the used columns are
+// read with 16-row vector loads, the condition's 16 selection bits go to mb[tile * 256 + t] for
+// the write pass and their count to tcount[tile].  Every column must start on a 16-byte boundary.
+int32_t jit_compress_count(hpa_ctx *c, const DevCols &cols, const hpa_expr *cond, L n, uint16_t *mb, uint32_t *tcount){
+    if(!jit_enabled()) R HPA_E_NOT_IMPL;
+    const int nx = cols.n;
+    for(int k = 0; k < nx; k++) if((uintptr_t)cols.p[k] & 15) R HPA_E_NOT_IMPL;
+    const std::string rk = raw_key("cpc", nx, cols.typ, 0, cond);
+    JitPrep prep; int32_t e;
+    if(!prep_find(rk, &prep)){
+    std::vector<int> used(nx, 0);
+    CExpr be;
+    Konst &kc = prep.kc;
+    if((e = expr_to_c(cond, nx, cols.typ, &be, used, &kc))) R e;
+    auto gen = [&](){
+    std::ostringstream s;
+    s << PREAMBLE
+      << "template<typename T> struct __attribute__((aligned(16))) R16 { T v[16]; };\n"
+         "struct A { const void *c[" << EXPR_MAX_COLS << "]; i64 n; unsigned short *mb; unsigned *tcount; " KONST_FIELDS " };\n"
+         "extern \"C\" __global__ void __launch_bounds__(256) cp_cnt(A a){\n"
+         "    __shared__ unsigned sw[4];\n"
+         "    const int t = threadIdx.x;\n"
+         "    const i64 i0 = (i64)blockIdx.x * 4096 + (i64)t * 16;\n"
+         "    unsigned m = 0;\n"
+         "    if(i0 + 16 <= a.n){\n";
+    for(int k = 0; k < nx; k++) if(used[k])
+        s << "        const R16<" << ctype(cols.typ[k]) << "> r" << k << " = *(const R16<" << ctype(cols.typ[k]) << ">*)((const "
+          << ctype(cols.typ[k]) << "*)a.c[" << k << "] + i0);\n";
+    s << "        #pragma unroll\n"
+         "        for(int k = 0; k < 16; k++){\n";
+    for(int k = 0; k < nx; k++) if(used[k]) s << "            const " << ctype(cols.typ[k]) << " v" << k << " = r" << k << ".v[k];\n";
+    s << "            if(" << be.s << ") m |= 1u << k;\n"
+         "        }\n"
+         "    } else {\n"
+         "        for(int k = 0; k < 16 && i0 + k < a.n; k++){\n"
+         "            const i64 i = i0 + k;\n"
+      << loads_code(nx, cols.typ, used)
+      << "            if(" << be.s << ") m |= 1u << k;\n"
+         "        }\n"
+         "    }\n"
+         "    a.mb[(i64)blockIdx.x * 256 + t] = (unsigned short)m;\n"
+         "    unsigned c = __popc(m);\n"
+         "    for(int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);\n"
+         "    if((t & 63) == 0) sw[t >> 6] = c;\n"
+         "    __syncthreads();\n"
+         "    if(t == 0) a.tcount[blockIdx.x] = sw[0] + sw[1] + sw[2] + sw[3];\n"
+         "}\n";
+    return s.str();
+    };
+    const std::string key = "cpc|" + std::to_string(nx) + "|" + types_key(nx, cols.typ) + "|" + be.s;
+    if((e = jit_get_keyed(key, "cp_cnt", gen, &prep.fn))) R e;
+    prep_store(rk, prep);
+    }
+    struct CArgs { const void *c[EXPR_MAX_COLS]; long long n; uint16_t *mb; uint32_t *tcount;
+                   long long ki[JIT_MAX_CONST]; double kf[JIT_MAX_CONST]; } a;
+    memset(&a, 0, sizeof(a));
+    for(int k = 0; k < nx; k++) a.c[k] = cols.p[k];
+    memcpy(a.ki, prep.kc.i, sizeof(a.ki)); memcpy(a.kf, prep.kc.f, sizeof(a.kf));
+    a.n = n; a.mb = mb; a.tcount = tcount;
+    R jit_launch(c, prep.fn, (unsigned)ceil_div(n, 4096), 256, a);
 }
 
 // ---------------------------------------------------------------- fused group + aggregates (fga)

@@ -38,6 +38,10 @@ struct ExprPred {
     DevExpr e; DevCols x;
     __device__ __forceinline__ bool operator()(L i) const { return expr_true(e, eval_expr(e, x, i)); }
 };
+// a condition's byte mask, interpreted (when no generated code is available)
+__global__ void k_cond_mask(ExprPred p, L n, uint8_t *m){
+    for(L i = (L)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (L)gridDim.x * blockDim.x) m[i] = p(i) ? 1 : 0;
+}
 
 __device__ __forceinline__ void copy_elem(char *dst, const char *src, int w, int64_t pos, L i){
     switch(w){
@@ -276,9 +280,14 @@ static int32_t compress_write(hpa_ctx *c, MaskPred pred, L n, CpCols cols, int64
 }
 
 // pfnCompress (h_primitive.cc:1479-1521)
-// nm > 0: also zm[j] = compress(x, maps->e[j](maps->x)) (byte-mask mode only; no aliasing)
+// nm > 0: also zm[j] = compress(x, maps->e[j](maps->x)) (no aliasing)
+// cond (expression mode, the source of ep): the count pass is generated code that evaluates it on
+// 16-row vector loads and keeps the selection bits, and the write pass runs from those bits as
+// for a byte mask -- no mask is written; needs 16-element aligned columns.  Without that path the
+// call returns HPA_E_NOT_IMPL with nothing done when `gen_only`, else evaluates cond per row.
 static int32_t compress_many(hpa_ctx *c, int ny, V *z, V x, V *y, bool expr_mode, const ExprPred *ep, L n_expr,
-                             int nm = 0, V *zm = nullptr, CpMaps *maps = nullptr){
+                             int nm = 0, V *zm = nullptr, CpMaps *maps = nullptr, const hpa_expr *cond = nullptr,
+                             bool gen_only = false){
     L n = expr_mode ? n_expr : x->len;
     const bool alias_ok = nm == 0;
     for(int k = 0; k < ny; k++){
@@ -314,9 +323,31 @@ static int32_t compress_many(hpa_ctx *c, int ny, V *z, V x, V *y, bool expr_mode
     for(int j = 0; j < nm; j++) maxb += (type_bytes(maps->typ[j], n > 0 ? n : 1) + 255) & ~(L)255;
     const bool spec = n > 0 && c->dcur + maxb + 4096 + (16 + 2 * CP_BLOCK) * ceil_div(n, CP_TILE) <= c->dtop;
     uint16_t *mb = nullptr;
-    e = expr_mode ? compress_count(c, *ep, nullptr, n, &toff, spec ? nullptr : &kk)
-                  : compress_count(c, MaskPred{m}, m, n, &toff, spec ? nullptr : &kk, &mb);
-    if(e){ arena_set(c, mark); R e; }
+    // expression mode through the generated count: every column 16-element aligned (the write
+    // pass from selection bits reads 16 rows of a column per thread)
+    bool gen = expr_mode && cond && n > 0 && jit_enabled();
+    for(int k = 0; k < ny && gen; k++) gen = ((uintptr_t)cols.src[k] % (uintptr_t)(16 * cols.w[k])) == 0 && cols.w[k] <= 8;
+    for(int k = 0; k < ep->x.n && gen; k++) gen = ((uintptr_t)ep->x.p[k] & 15) == 0;
+    for(int k = 0; k < (nm > 0 ? maps->x.n : 0) && gen; k++) gen = ((uintptr_t)maps->x.p[k] & 15) == 0;
+    if(expr_mode && gen_only && !gen){ arena_set(c, mark); R HPA_E_NOT_IMPL; }
+    if(gen){
+        const L ntiles = ceil_div(n, CP_TILE);
+        uint32_t *tcount = (uint32_t*)dalloc(c, 4 * ntiles);
+        toff = (int64_t*)dalloc(c, 8 * (ntiles + 1));
+        mb = (uint16_t*)dalloc(c, 2 * CP_BLOCK * ntiles);
+        if(!tcount || !toff || !mb){ arena_set(c, mark); R HPA_E_MEMORY; }
+        e = jit_compress_count(c, ep->x, cond, n, mb, tcount);
+        if(e == HPA_E_NOT_IMPL && !gen_only){ gen = false; mb = nullptr; toff = nullptr; }
+        else if(e){ arena_set(c, mark); R e; }
+        else if((e = scan_u32_to_i64(c, tcount, toff, ntiles, toff + ntiles)) || (!spec && (e = d2h(c, &kk, toff + ntiles, 8)))){
+            arena_set(c, mark); R e;
+        }
+    }
+    if(!gen){
+        e = expr_mode ? compress_count(c, *ep, nullptr, n, &toff, spec ? nullptr : &kk)
+                      : compress_count(c, MaskPred{m}, m, n, &toff, spec ? nullptr : &kk, &mb);
+        if(e){ arena_set(c, mark); R e; }
+    }
     if(!spec && kk == n && n > 0 && !expr_mode && alias_ok){   // copyV alias (h_primitive.cc:1513-1515)
         ps.stop(); ps.bytes = (double)n;
         arena_set(c, mark);
@@ -331,8 +362,8 @@ static int32_t compress_many(hpa_ctx *c, int ny, V *z, V x, V *y, bool expr_mode
         maps->dst[j] = (char*)dalloc(c, type_bytes(maps->typ[j], spec ? n : (kk > 0 ? kk : 1)));
         if(!maps->dst[j]){ arena_set(c, mark); R HPA_E_MEMORY; }
     }
-    e = expr_mode ? compress_write(c, *ep, n, cols, nullptr, toff)
-                  : compress_write(c, MaskPred{m}, n, cols, nullptr, toff, mb, nm > 0 ? maps : nullptr);
+    e = expr_mode && !gen ? compress_write(c, *ep, n, cols, nullptr, toff)
+                          : compress_write(c, MaskPred{m}, n, cols, nullptr, toff, mb, nm > 0 ? maps : nullptr);
     if(e) R e;
     if(spec){
         if((e = d2h(c, &kk, toff + ceil_div(n, CP_TILE), 8))) R e;
@@ -401,6 +432,47 @@ extern "C" int32_t hpaCompressMap(int32_t ny, hpa_V *z, hpa_V mask, hpa_V *y, in
     }
     if(ny == 0 && nm == 0) R 0;
     R compress_many(c, ny, z, mask, y, false, nullptr, 0, nm, zm, &mp);
+}
+
+// hpaCompressMap with the mask's condition as an expression over x in place of the mask: the
+// FA compress scan of one condition, with its plain columns and FE maps (optimizer/autofusion.cc:
+// 814-817, 886-899).  The condition is evaluated by the count pass, which keeps each row's
+// selection for the write pass, so no mask is materialised.
+extern "C" int32_t hpaFusedCompressMap(int32_t ny, hpa_V *z, hpa_V *y, int32_t nm, hpa_V *zm, const int32_t *mtyp,
+                                       const hpa_expr *maps, int32_t nx, hpa_V *x, const hpa_expr *cond){
+    hpa_ctx *c = ctx_cur();
+    if(nm < 0 || nm > CP_MAXM || ny < 0 || !cond) R HPA_E_NOT_IMPL;
+    CpMaps mp; memset(&mp, 0, sizeof(mp)); mp.n = nm;
+    L n; int32_t e;
+    if((e = cols_view(c, nx, x, &mp.x, &n))) R e;
+    for(int k = 0; k < ny; k++) if(y[k]->len != n) R HPA_E_LENGTH;
+    int32_t ct[EXPR_MAX_COLS]; for(int k = 0; k < nx; k++) ct[k] = x[k]->typ;
+    for(int j = 0; j < nm; j++){
+        if(!(mtyp[j] == HPA_B || mtyp[j] == HPA_I || mtyp[j] == HPA_L || mtyp[j] == HPA_E)) R HPA_E_TYPE;
+        mp.typ[j] = mtyp[j];
+        mp.hx[j] = maps + j;
+        if((e = expr_compile(maps + j, nx, ct, &mp.e[j]))) R e;
+    }
+    ExprPred ep; ep.x = mp.x;
+    if((e = expr_compile(cond, nx, ct, &ep.e))) R e;
+    if(ny == 0 && nm == 0) R 0;
+    if(n > 1){
+        e = compress_many(c, ny, z, nullptr, y, true, &ep, n, nm, zm, &mp, cond, true);
+        if(e != HPA_E_NOT_IMPL) R e;
+    }
+    // otherwise the condition into a byte mask, then hpaCompressMap's path
+    L mark = c->dcur;
+    uint8_t *mask = (uint8_t*)dalloc(c, n > 0 ? n : 1);
+    if(!mask) R HPA_E_MEMORY;
+    V0 m; memset(&m, 0, sizeof(m));
+    m.typ = HPA_B; m.len = n; m.g = (char*)mask;
+    if(n > 0 && (e = jit_fused_map(c, HPA_B, nx, x, cond, mp.x, n, mask))){
+        if(e != HPA_E_NOT_IMPL){ arena_set(c, mark); R e; }
+        hipLaunchKernelGGL(k_cond_mask, dim3(grid_for(n, 256, 4)), dim3(256), 0, c->stream, ep, n, mask);
+        if((e = check_launch("fused_compress_map"))) R e;
+    }
+    if(n == 1){ uint8_t b = 0; if((e = d2h(c, &b, mask, 1))) R e; m.g = (char*)&m.b; m.i64 = 0; m.b = b; }
+    R compress_many(c, ny, z, &m, y, false, nullptr, 0, nm, zm, &mp);
 }
 
 extern "C" int32_t hpaCompressMany(int32_t ny, hpa_V *z, hpa_V mask, hpa_V *y){
@@ -502,6 +574,12 @@ extern "C" int32_t hpaFusedCompress(int32_t ny, hpa_V *z, hpa_V *y, int32_t nx, 
     if((e = cols_view(c, nx, x, &ep.x, &n))) R e;
     int32_t ct[EXPR_MAX_COLS]; for(int k = 0; k < nx; k++) ct[k] = x[k]->typ;
     if((e = expr_compile(cond, nx, ct, &ep.e))) R e;
+    for(int k = 0; k < ny; k++) if(y[k]->len != n) R HPA_E_LENGTH;
+    if(n > 1){
+        // the condition evaluated by the generated count pass (no mask written)
+        e = compress_many(c, ny, z, nullptr, y, true, &ep, n, 0, nullptr, nullptr, cond, true);
+        if(e != HPA_E_NOT_IMPL) R e;
+    }
     if(n > 1){
         // the condition through generated code into a byte mask, then the mask compress
         L mark = c->dcur;

@@ -259,18 +259,31 @@ def q3_plain(B, Dm, F, day=D.Q3_DAY, limit=10, segment=1):
 Q3_REVENUE = expr([("I", 1), ("COL", 0), "SUB", ("COL", 1), "MUL"])
 
 
+@functools.lru_cache(maxsize=64)
+def q3_dim_cond(segment, day):
+    """Q3's order filter over (segment, order_day): segment == s and order_day < day."""
+    return expr([("COL", 0), ("I", segment), "EQ", ("COL", 1), ("I", day), "LT", "AND"])
+
+
+@functools.lru_cache(maxsize=64)
+def q3_fact_cond(day):
+    """Q3's lineitem filter over (discount, price, ship_day): ship_day > day."""
+    return expr([("COL", 2), ("I", day), "GT"])
+
+
 def q3_fused(B, Dm, F, day=D.Q3_DAY, limit=10, segment=1):
     """Q3 as the optimising compiler emits it (src/horseir/v1/compile-static/cur/q3.h:1-56):
     one mask per table and the multi-column peephole compress of its filtered columns with the FE
     revenue map fused into the fact table's compress, group, the grouped sum as one FP loop over the group lists (q3_peephole_1,
     cur/q3.h:30-50: each group's revenue summed through its row indices, nothing gathered
     first), join and the multi-column gathers."""
-    d_mask = B.pfnAnd(B.pfnEq(Dm["segment"], B.scalar(segment, H_I)), B.pfnLt(Dm["order_day"], B.scalar(day, H_I)))
-    lk, lday = B.compress_many(d_mask, [Dm["key"], Dm["order_day"]])
-    f_mask = B.pfnGt(F["ship_day"], B.scalar(day, H_I))
+    # each table's filter evaluated by its compress's count pass (the FA compress scan of one
+    # condition): no mask is written
+    lk, lday = B.fused_compress([Dm["key"], Dm["order_day"]], [Dm["segment"], Dm["order_day"]], q3_dim_cond(segment, day))
     # the fact compress with the FE revenue map fused: rk and (1 - discount) * price written in
     # the one pass, price and discount never materialised compressed
-    (rk,), (rev,) = B.compress_map(f_mask, [F["fk"]], [(H_E, Q3_REVENUE)], [F["discount"], F["price"]])
+    (rk,), (rev,) = B.fused_compress_map(q3_fact_cond(day), [F["fk"]], [(H_E, Q3_REVENUE)],
+                                         [F["discount"], F["price"], F["ship_day"]])
     grp = B.pfnGroup(rk)
     gkeys, gvals = B.pfnKeys(grp), B.pfnValues(grp)
     gk = B.pfnIndex(rk, gkeys)

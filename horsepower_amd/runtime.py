@@ -89,6 +89,8 @@ class Device(PrimitiveLib):
         _bind(dll, "hpaCompressMany", i32, [i32, C.POINTER(V), V, C.POINTER(V)])
         _bind(dll, "hpaCompressMap", i32, [i32, C.POINTER(V), V, C.POINTER(V), i32, C.POINTER(V),
                                            C.POINTER(C.c_int32), C.POINTER(Expr), i32, C.POINTER(V)])
+        _bind(dll, "hpaFusedCompressMap", i32, [i32, C.POINTER(V), C.POINTER(V), i32, C.POINTER(V), C.POINTER(i32),
+                                              C.POINTER(Expr), i32, C.POINTER(V), C.POINTER(Expr)])
         _bind(dll, "hpaIndexMany", i32, [i32, C.POINTER(V), C.POINTER(V), V])
         _bind(dll, "hpaGroupReduce", i32, [V, i32, V, V])
         _bind(dll, "hpaFusedGroupAgg", i32, [i32, C.POINTER(V), i32, C.POINTER(V), C.POINTER(Expr), i32,

@@ -343,6 +343,13 @@ int32_t hpaCompressMany(int32_t ny, hpa_V *z, hpa_V mask, hpa_V *y);
  * Outputs never alias the inputs (an all-true mask copies). */
 int32_t hpaCompressMap(int32_t ny, hpa_V *z, hpa_V mask, hpa_V *y, int32_t nm, hpa_V *zm,
                        const int32_t *mtyp, const hpa_expr *maps, int32_t nx, hpa_V *x);
+/* hpaCompressMap with the mask given as its condition: cond, an expression over x (the columns
+ * the maps read), selects the rows; z[k] = y[k][cond] and zm[j] = maps[j](x)[cond] as type
+ * mtyp[j].  The FA compress scan of one condition (optimizer/autofusion.cc:886-899): the count
+ * pass evaluates cond and keeps each row's selection for the write pass, so no byte mask is
+ * written.  Results equal hpaCompressMap's with the mask cond(x). */
+int32_t hpaFusedCompressMap(int32_t ny, hpa_V *z, hpa_V *y, int32_t nm, hpa_V *zm, const int32_t *mtyp,
+                            const hpa_expr *maps, int32_t nx, hpa_V *x, const hpa_expr *cond);
 /* FP1/FP2/FP5/FP6/FP7 grouped reductions (optimizer/pattern.cc:180-219): for each group list
  * t of y (flat or normal list of i64 row indices), z[g] = reduce_{j in t}(x[j]);
  * kind 0 sum, 1 avg, 2 min, 3 max, 4 len.  z is a vector over groups. */
