@@ -153,7 +153,7 @@ struct CExpr { std::string s; bool f; };
 #define KONST_FIELDS "i64 ki[32]; double kf[32];"
 struct Konst { int ni = 0, nf = 0; long long i[JIT_MAX_CONST]; double f[JIT_MAX_CONST]; };
 // prepared requests by raw key (raw_key): the function and the launch's constants
-struct JitPrep { hipFunction_t fn = nullptr; Konst kc; };
+struct JitPrep { hipFunction_t fn = nullptr; Konst kc; std::vector<int> used; };
 static std::map<std::string, JitPrep> g_jit_prep;
 static bool prep_find(const std::string &k, JitPrep *p){
     std::lock_guard<std::mutex> lk(g_jit_mu);
@@ -348,18 +348,20 @@ int32_t jit_fused_reduce(hpa_ctx *c, int kind, int nx, V *x, const hpa_expr *con
                          const DevCols &cols, L n, void *part, int G){
     if(!jit_enabled()) R HPA_E_NOT_IMPL;
     int32_t typ[EXPR_MAX_COLS]; for(int k = 0; k < nx; k++) typ[k] = x[k]->typ;
+    const std::string rk = raw_key("fa", nx, typ, cond ? 1 : 0, body, cond);
+    JitPrep prep;
+    int32_t e;
+    if(prep_find(rk, &prep)){ if(!cols_aligned(nx, typ, prep.used, cols)) R HPA_E_NOT_IMPL; }
+    else {
     std::vector<int> uc(nx, 0), ub(nx, 0), late;
     CExpr ce{COND_TRUE, false}, be;
-    Konst kc;
-    int32_t e;
+    Konst &kc = prep.kc;
     if(cond && (e = expr_to_c(cond, nx, typ, &ce, uc, &kc))) R e;
     if((e = expr_to_c(body, nx, typ, &be, ub, &kc))) R e;
     split_used(nx, uc, ub, late);
-    {
-        std::vector<int> used(nx, 0);
-        for(int k = 0; k < nx; k++) used[k] = uc[k] | ub[k];
-        if(!cols_aligned(nx, typ, used, cols)) R HPA_E_NOT_IMPL;
-    }
+    prep.used.assign(nx, 0);
+    for(int k = 0; k < nx; k++) prep.used[k] = uc[k] | ub[k];
+    if(!cols_aligned(nx, typ, prep.used, cols)) R HPA_E_NOT_IMPL;
     auto gen = [&](){
     std::ostringstream s;
     s << PREAMBLE;
@@ -390,16 +392,17 @@ int32_t jit_fused_reduce(hpa_ctx *c, int kind, int nx, V *x, const hpa_expr *con
          "}\n";
     return s.str();
     };
-    hipFunction_t fn;
     const std::string key = "fa|" + std::to_string(nx) + "|" + types_key(nx, typ) + "|" + ce.s + "|" + be.s;
-    if((e = jit_get_keyed(key, "fa_reduce", gen, &fn))) R e;
+    if((e = jit_get_keyed(key, "fa_reduce", gen, &prep.fn))) R e;
+    prep_store(rk, prep);
+    }
     JArgs a; memset(&a, 0, sizeof(a));
     for(int k = 0; k < nx; k++) a.c[k] = cols.p[k];
-    memcpy(a.ki, kc.i, sizeof(a.ki)); memcpy(a.kf, kc.f, sizeof(a.kf));
+    memcpy(a.ki, prep.kc.i, sizeof(a.ki)); memcpy(a.kf, prep.kc.f, sizeof(a.kf));
     a.n = n; a.out = part;
     double cb = 0; for(int k = 0; k < nx; k++) cb += (double)type_width(typ[k]);
     ProfScope ps("fused_reduce", c->stream, (double)n * cb);
-    R jit_launch(c, fn, (unsigned)G, 256, a);
+    R jit_launch(c, prep.fn, (unsigned)G, 256, a);
 }
 
 // ---------------------------------------------------------------- shared-scan FA reduce (fa_many)
