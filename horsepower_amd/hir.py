@@ -11,7 +11,9 @@ Fusion, over the straight-line statement list:
   * FA reduction: sum / avg / min / max of compress(mask, body) (the compress used once), or of an
     element-wise tree, becomes one hpaFusedReduce with the mask's and the body's trees;
   * FA compress: the compresses of one mask run as one hpaCompressMany; a compressed
-    element-wise tree (used once) is evaluated inside the compress (hpaCompressMap);
+    element-wise tree (used once) is evaluated inside the compress (hpaCompressMap); a mask that
+    is an element-wise statement used only by those compresses becomes their condition, counted
+    without building the mask (hpaFusedCompress / hpaFusedCompressMap);
   * FE: an element-wise tree of two or more operations that is not absorbed by a sink becomes
     one hpaFusedMap;
   * FP: raze(each(f, each_right(index, v, groups))) with f in sum / avg / min / max / len becomes
@@ -300,6 +302,10 @@ def plan(stmts, ret, fuse=True):
     for s in stmts:
         if s.fn == "compress" and len(s.args) == 2 and isinstance(s.args[0], Var) and s.target not in to_reduce:
             mask_groups.setdefault(s.args[0].name, []).append(s)
+    # masks used by nothing but their group's compresses: the compress evaluates the mask's tree
+    # as its condition in its count pass (FA compress scan of one condition), no mask is built
+    cond_masks = {m for m, g in mask_groups.items() if m in ew and uses.get(m, 0) == len(g)
+                  and all(isinstance(a, Var) and a.name == m for a in (x.args[0] for x in g))}
     for s in stmts:
         if s.target in done:
             continue
@@ -329,10 +335,17 @@ def plan(stmts, ret, fuse=True):
                 ops.append(("reduce", s.target, REDUCE[s.fn], None, tree(s.args[0]), s))
                 done.add(s.target)
                 continue
+        # a mask that becomes its compresses' condition is not computed here
+        if s.target in cond_masks:
+            absorbed.add(s.target)
+            done.add(s.target)
+            continue
         # FA compress of one mask
         if s.fn == "compress" and isinstance(s.args[0], Var) and s.args[0].name in mask_groups:
-            group = [g for g in mask_groups.pop(s.args[0].name) if g.target not in done]
-            if len(group) >= 2 or (len(group) == 1 and isinstance(group[0].args[1], Var) and inlinable(group[0].args[1].name)):
+            mname = s.args[0].name
+            group = [g for g in mask_groups.pop(mname) if g.target not in done]
+            if len(group) >= 2 or mname in cond_masks or \
+                    (len(group) == 1 and isinstance(group[0].args[1], Var) and inlinable(group[0].args[1].name)):
                 plain_ys, maps = [], []
                 for g in group:
                     y = g.args[1]
@@ -341,7 +354,11 @@ def plan(stmts, ret, fuse=True):
                     else:
                         plain_ys.append((g.target, y))
                     done.add(g.target)
-                ops.append(("compress", s.args[0].name, plain_ys, maps))
+                cond = None
+                if mname in cond_masks:
+                    d = defs[mname]
+                    cond = ("op", d.fn, tree(d.args[0]), tree(d.args[1]))
+                ops.append(("compress", mname, plain_ys, maps, cond))
                 continue
         if s.target in ew and not inlinable(s.target):
             t = ("op", s.fn, tree(s.args[0]), tree(s.args[1]))
@@ -538,16 +555,44 @@ def run(program, B, env=None, fuse=True):
                     val = B.pfnCompress(_run_tree_plain(B, env, cond), val)
                 env[target] = getattr(B, PFN[s.fn])(val)
         elif kind == "compress":
-            _, mask, plain_ys, maps = op
-            m = env[mask]
+            _, mask, plain_ys, maps, cond = op
+            if cond is not None:
+                # the rows: the length of the condition's leaves (it has a column leaf, being a
+                # vector mask's tree); the condition runs fused when its columns allow it
+                lens = _leaf_lens(cond, env, set())
+                n = next(iter(lens)) if len(lens) == 1 else None
+                if n is None or not _fusable(cond, env):
+                    env[mask] = _run_tree_plain(B, env, cond)
+                    cond = None
+            if cond is None:
+                n = env[mask].contents.len
             ok_maps = [(tg, t) for tg, t in maps
-                       if t[0] == "op" and _fusable(t, env) and _leaf_lens(t, env, set()) <= {m.contents.len}]
+                       if t[0] == "op" and _fusable(t, env) and _leaf_lens(t, env, set()) <= {n}]
             bad = [(tg, t) for tg, t in maps if (tg, t) not in ok_maps]
             for tg, t in bad:
                 plain_ys.append((tg, ("tree", t)))
             ys = []
             for tg, y in plain_ys:
                 ys.append(_run_tree_plain(B, env, y[1]) if isinstance(y, tuple) else _arg_node(B, env, y))
+            if cond is not None:
+                code = _Code(env)
+                _emit(cond, code)
+                ce = expr(code.code)
+                exprs = []
+                for tg, t in ok_maps:
+                    code.code = []
+                    _emit(t, code)
+                    exprs.append((_tree_type(t, env), expr(code.code)))
+                if ok_maps:
+                    zs, zm = B.fused_compress_map(ce, ys, exprs, code.cols)
+                    for (tg, _), z in zip(ok_maps, zm):
+                        env[tg] = z
+                else:
+                    zs = B.fused_compress(ys, code.cols, ce)
+                for (tg, _), z in zip(plain_ys, zs):
+                    env[tg] = z
+                continue
+            m = env[mask]
             if ok_maps:
                 code = _Code(env)
                 exprs = []

@@ -48,11 +48,12 @@ def test_parse_forms():
 def test_plan_q6_shape():
     p = H.plan(*H.parse(Q6_HIR))
     kinds = p.kinds()
-    # four column reads, the mask as one FE map, one two-column compress, one fused sum
-    assert kinds == ["stmt"] * 4 + ["map", "compress", "reduce"], kinds
-    m = p.ops[4]
-    assert m[1] == "keep" and m[2][0] == "op"
-    assert p.ops[5][2] == [("p", p.ops[5][2][0][1]), ("d", p.ops[5][2][1][1])]
+    # four column reads, one two-column compress with the mask's tree as its condition (no
+    # mask built: the mask has no other use), one fused sum
+    assert kinds == ["stmt"] * 4 + ["compress", "reduce"], kinds
+    c = p.ops[4]
+    assert c[1] == "keep" and c[4][0] == "op" and c[4][1] == "and"
+    assert c[2] == [("p", c[2][0][1]), ("d", c[2][1][1])]
     assert H.plan(*H.parse(Q6_HIR), fuse=False).kinds() == ["stmt"] * 17
 
 
