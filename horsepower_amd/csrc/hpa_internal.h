@@ -159,7 +159,7 @@ __device__ __forceinline__ uint64_t match_digit(unsigned d, bool valid, int nb){
     #pragma unroll
     for(int b = 0; b < nb; b++){
         const uint32_t m = (uint32_t)(((int32_t)(d << (31 - b))) >> 31);
-        const uint64_t bb = __ballot(m != 0u);
+        const uint64_t bb = __builtin_amdgcn_uicmp(m, 0u, 33);      // wave compare m != 0 (33: not equal)
         plo = __builtin_amdgcn_bitop3_b32(plo, (uint32_t)bb, m, 0x90);
         phi = __builtin_amdgcn_bitop3_b32(phi, (uint32_t)(bb >> 32), m, 0x90);
     }
